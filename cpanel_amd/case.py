@@ -1,0 +1,161 @@
+"""Host-side mirror of CPanel's geometry / cpCase interface over the C ABI.
+
+    ctx  = Context(device=0)
+    geom = ctx.geometry(mesh)          # geometry(inputParams*) + readTri
+    geom.assemble(CP_FP64)             # geometry::setInfCoeff
+    res  = geom.solve([(V, alpha, beta), ...], prec)   # cpCase::run per condition
+
+Results carry mu, Cp, CL, CDi, Fbody, Fwind, CM as cpCase reports them
+(cpCase.h:129-133).  Errors raise CPanelError with the library's message.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import abi
+from .abi import CP_FP32_REFINE, CP_FP64  # noqa: F401
+
+
+class CPanelError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__("%s: %s" % (abi.lib().cp_status_string(status).decode(), msg))
+        self.status = status
+
+
+def _p(a, t=C.c_double):
+    return None if a is None else a.ctypes.data_as(C.POINTER(t))
+
+
+class Context:
+    def __init__(self, device=0, hbm_budget_bytes=0):
+        L = abi.lib()
+        h = C.c_void_p()
+        st = L.cp_create(device, hbm_budget_bytes, C.byref(h))
+        if st != abi.CP_OK:
+            raise CPanelError(st, "cp_create(device=%d)" % device)
+        self.h = h
+
+    def check(self, st, what=""):
+        if st not in (abi.CP_OK,):
+            raise CPanelError(st, "%s %s" % (what, abi.lib().cp_last_error(self.h).decode()))
+        return st
+
+    def close(self):
+        if self.h:
+            abi.lib().cp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def geometry(self, mesh, key=0):
+        return Geometry(self, mesh, key)
+
+    def bytes_in_use(self):
+        return abi.lib().cp_bytes_in_use(self.h)
+
+    def timing(self, name):
+        ms = C.c_double()
+        n = C.c_int64()
+        self.check(abi.lib().cp_timing_get(self.h, name.encode(), C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def timing_enable(self, on=True):
+        abi.lib().cp_timing_enable(self.h, 1 if on else 0)
+
+    def timing_reset(self):
+        self.check(abi.lib().cp_timing_reset(self.h))
+
+
+class Geometry:
+    def __init__(self, ctx, mesh, key=0):
+        self.ctx = ctx
+        L = abi.lib()
+        h = C.c_void_p()
+        if isinstance(mesh, str):
+            st = L.cp_geometry_from_tri(ctx.h, mesh.encode(), key, C.byref(h))
+        else:
+            self._xyz = np.ascontiguousarray(mesh["xyz"], dtype=np.float64)
+            self._tri = np.ascontiguousarray(mesh["tri"], dtype=np.int32)
+            self._ids = np.ascontiguousarray(mesh["ids"], dtype=np.int32)
+            m = abi.Mesh(len(self._xyz), _p(self._xyz), len(self._tri), _p(self._tri, C.c_int32),
+                         _p(self._ids, C.c_int32))
+            st = L.cp_geometry_upload(ctx.h, C.byref(m), key, C.byref(h))
+        ctx.check(st, "geometry upload")
+        self.h = h
+        info = abi.GeomInfo()
+        ctx.check(L.cp_geometry_info(h, C.byref(info)))
+        self.nb, self.nw, self.lda = info.n_body, info.n_wake, info.lda
+        self.n_wakelines = info.n_wakelines
+
+    def release(self):
+        if self.h:
+            abi.lib().cp_geometry_release(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+    def body(self):
+        n = self.nb
+        ti = np.zeros(n, np.int32)
+        c = np.zeros((n, 3))
+        nr = np.zeros((n, 3))
+        a = np.zeros(n)
+        self.ctx.check(abi.lib().cp_geometry_body(self.h, _p(ti, C.c_int32), _p(c), _p(nr), _p(a)))
+        return dict(tri_index=ti, center=c, normal=nr, area=a)
+
+    def wake(self):
+        n = self.nw
+        ti = np.zeros(max(n, 1), np.int32)
+        idx = np.zeros((max(n, 1), 4), np.int32)
+        cf = np.zeros(max(n, 1))
+        self.ctx.check(abi.lib().cp_geometry_wake(self.h, _p(ti, C.c_int32), _p(idx, C.c_int32), _p(cf)))
+        return dict(tri_index=ti[:n], idx4=idx[:n], coef=cf[:n])
+
+    def clusters(self):
+        n = self.nb
+        cl = np.zeros((n, 16), np.int32)
+        ts = np.zeros(n, np.int32)
+        tip = np.zeros(n, np.int32)
+        self.ctx.check(abi.lib().cp_geometry_clusters(self.h, _p(cl, C.c_int32), _p(ts, C.c_int32),
+                                                      _p(tip, C.c_int32)))
+        return cl, ts, tip
+
+    def assemble(self, prec=CP_FP64):
+        self.ctx.check(abi.lib().cp_assemble(self.h, prec), "assemble")
+
+    def matrices(self, prec=CP_FP64):
+        n = self.nb
+        A = np.zeros((n, n), order="F")
+        B = np.zeros((n, n), order="F")
+        self.ctx.check(abi.lib().cp_read_matrices(self.h, prec, _p(A), _p(B), n), "read matrices")
+        return A, B
+
+    def solve(self, conditions, prec=CP_FP64, mach=0.1, Sref=1.0, bref=1.0, cref=1.0, cg=(0, 0, 0), tol=1e-10,
+              max_iter=0, max_refine=0, inner_tol=0.0, want_cp=True, allow_unconverged=False):
+        k = len(conditions)
+        fs = (abi.Freestream * k)(*[abi.Freestream(*map(float, c)) for c in conditions])
+        o = abi.SolveOpts(mach, Sref, bref, cref, (C.c_double * 3)(*cg), tol, max_iter, max_refine, inner_tol)
+        mu = np.zeros((k, self.nb))
+        cp = np.zeros((k, self.nb)) if want_cp else None
+        info = (abi.SolveInfo * k)()
+        st = abi.lib().cp_solve(self.h, prec, k, fs, C.byref(o), _p(mu), _p(cp), info)
+        if st == abi.CP_NOT_CONVERGED and allow_unconverged:
+            pass
+        else:
+            self.ctx.check(st, "solve")
+        out = []
+        for q in range(k):
+            r = info[q]
+            out.append(dict(mu=mu[q], cp=None if cp is None else cp[q], CL=r.CL, CDi=r.CDi,
+                            Fbody=np.array(r.Fbody[:]), Fwind=np.array(r.Fwind[:]), CM=np.array(r.CM[:]),
+                            rel_residual=r.rel_residual, iters=r.iters, refinements=r.refinements,
+                            converged=bool(r.converged)))
+        return out

@@ -1,0 +1,70 @@
+// K3: batched multi-RHS solve in FP32 with FP64 iterative refinement.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "krylov.h"
+
+namespace cpgpu {
+
+constexpr int kRhsTile = 64;  // right-hand sides per batched launch
+
+struct BatchedOpts {
+    double tol = 1e-10;       // outer: ||b - A x|| / ||b|| in FP64
+    int max_iter = 0;         // inner FP32 iterations per sweep
+    int max_refine = 10;      // FP64 refinement sweeps
+    double inner_tol = 1e-5;  // inner residual reduction per sweep
+};
+
+// GEMM tile shape chosen by the autotuner (kslices = split of the K range)
+struct GemmShape {
+    int kslices = 8;
+};
+
+class BatchedSolver {
+public:
+    BatchedSolver(int n, int64_t lda, int capacity);
+    ~BatchedSolver();
+    BatchedSolver(const BatchedSolver&) = delete;
+    BatchedSolver& operator=(const BatchedSolver&) = delete;
+    int capacity() const { return cap_; }
+    // Solves A32 x_c = -B32 sigma_c (sigma_c = -Vinf_c . n) for k columns.
+    // normals: device SoA (3*lda); vinf4: device (Vx,Vy,Vz,|V|) per column;
+    // x: device FP64 output, column c at x + c*ldx.
+    void solve(const float* A32, const float* B32, const double* normals, const double* vinf4, int k,
+               const BatchedOpts& o, double* x, int64_t ldx, SolveStats* stats, hipStream_t s);
+    void set_shape(const GemmShape& g) { shape_ = g; }
+    const GemmShape& shape() const { return shape_; }
+    // Y = A32 * P for one 64-column tile (exposed for tests / tuning)
+    void matvec32(const float* A32, const float* P, float* Y, hipStream_t s);
+    // R = sign * A32 * X64 (+ B if given), FP64 accumulation
+    void matvec64acc(const float* A32, const double* X, double* Y, const double* Badd, double sign, hipStream_t s);
+
+private:
+    void tile_solve(const float* A32, const float* B32, const double* normals, const double* vinf4, int c0,
+                    int kc, const BatchedOpts& o, double* x, int64_t ldx, SolveStats* stats, hipStream_t s);
+    int n_, cap_;
+    int64_t lda_;
+    int nrb_;      // row blocks of the reduction kernels
+    GemmShape shape_;
+    void* ws_ = nullptr;
+    // FP64 multivectors (lda x 64 row-major)
+    double *X_, *Bv_, *R_, *S_;
+    // FP32 multivectors
+    float *r_, *r0_, *p_, *v_, *sv_, *t_, *y_, *z_, *d_;
+    float* invd_;
+    float* part32_;     // kslices x lda x 64
+    double* part64_;    // kslices x lda x 64
+    double* dpart_;     // nrb x 64 x 4 partial column dots
+    double* sc_;        // 64 x 16 per-column scalars
+    int* act_;          // 64 active flags
+    double* host_sc_ = nullptr;
+    int* host_act_ = nullptr;
+};
+
+void launch_sigma(int n, int64_t lda, const double* normals, const double* vinf4, double* sigma, hipStream_t s);
+
+}  // namespace cpgpu

@@ -1,0 +1,598 @@
+// K3: multi-right-hand-side solve, FP32 Krylov + FP64 iterative refinement.
+//
+// Up to 64 freestream conditions of one geometry share every pass over the
+// matrix: the BiCGSTAB matrix-vector product becomes Y = A32 * P with P an
+// N x 64 multivector, a dense contraction that runs on the matrix cores
+// (v_mfma_f32_32x32x2f32, FP32 in, FP32 accumulate).  Each column keeps its
+// own BiCGSTAB scalars (rho, alpha, omega), so every condition follows
+// exactly the recurrence it would follow alone; the columns only share the
+// reads of A.
+//
+// Refinement: x (FP64) is corrected by FP32 inner solves of A32 d = r/|r|;
+// the residual r = b - A32 x is formed with FP64 accumulation on the FP64
+// matrix cores (v_mfma_f64_16x16x4f64, A32 entries widened exactly).  The
+// right-hand side b = -B32 sigma uses the same FP64 kernel.
+//
+// Layout: multivectors are row-major lda x 64 (the 64 values of one row are
+// contiguous), A32/B32 column-major.  GEMM work is split over K into
+// kslices; workgroup id -> (kslice = id % kslices, row block = id / kslices)
+// so with kslices a multiple of 8 every XCD works on one K slice and keeps
+// that slice of the multivector in its own L2.
+#include "batched.h"
+#include "device.h"
+#include "timer.h"
+
+namespace cpgpu {
+
+namespace {
+
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+
+constexpr int KP = kRhsTile;   // 64 columns
+constexpr int kT = 256;
+constexpr int kRedRows = 64;   // rows per reduction block
+
+enum {
+    Q_RHO = 0, Q_ALPHA, Q_OMEGA, Q_RR, Q_R0R, Q_R0V, Q_TT, Q_TS, Q_RR0, Q_BB, Q_RR64, Q_SCALE, Q_BETA, Q_NQ = 16
+};
+
+// ---------------------------------------------------------------- GEMMs
+__global__ __launch_bounds__(256) void gemm_f32_kernel(const float* __restrict__ A, int64_t lda, int n,
+                                                       const float* __restrict__ P, float* __restrict__ part,
+                                                       int kchunk, int KS) {
+    const int bid = blockIdx.x;
+    const int ks = bid % KS, rb = bid / KS;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t row0 = (int64_t)rb * 256 + wave * 64;
+    const int k0 = ks * kchunk, k1 = min(n, k0 + kchunk);
+    const int lr = lane & 31, lk = lane >> 5;
+    floatx16 acc00 = {}, acc01 = {}, acc10 = {}, acc11 = {};
+    const float* Ab = A + row0 + lr;
+    const float* Pb = P + lr;
+    int k = k0;
+    for (; k + 16 <= k1; k += 16) {
+        float a0[8], a1[8], b0[8], b1[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int64_t kk = k + 2 * u + lk;
+            a0[u] = Ab[kk * lda];
+            a1[u] = Ab[kk * lda + 32];
+            b0[u] = Pb[kk * KP];
+            b1[u] = Pb[kk * KP + 32];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            acc00 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0[u], b0[u], acc00, 0, 0, 0);
+            acc01 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0[u], b1[u], acc01, 0, 0, 0);
+            acc10 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1[u], b0[u], acc10, 0, 0, 0);
+            acc11 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1[u], b1[u], acc11, 0, 0, 0);
+        }
+    }
+    for (; k < k1; k += 2) {
+        const int64_t kk = k + lk;
+        const bool ok = kk < k1;
+        float a0 = ok ? Ab[kk * lda] : 0.f, a1 = ok ? Ab[kk * lda + 32] : 0.f;
+        float b0 = ok ? Pb[kk * KP] : 0.f, b1 = ok ? Pb[kk * KP + 32] : 0.f;
+        acc00 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b0, acc00, 0, 0, 0);
+        acc01 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b1, acc01, 0, 0, 0);
+        acc10 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b0, acc10, 0, 0, 0);
+        acc11 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b1, acc11, 0, 0, 0);
+    }
+    float* out = part + (int64_t)ks * lda * KP;
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+        const int64_t r = row0 + 8 * (j >> 2) + 4 * lk + (j & 3);
+        out[r * KP + lr] = acc00[j];
+        out[r * KP + 32 + lr] = acc01[j];
+        out[(r + 32) * KP + lr] = acc10[j];
+        out[(r + 32) * KP + 32 + lr] = acc11[j];
+    }
+}
+
+__global__ __launch_bounds__(256) void gemm_f64acc_kernel(const float* __restrict__ A, int64_t lda, int n,
+                                                          const double* __restrict__ X, double* __restrict__ part,
+                                                          int kchunk, int KS) {
+    const int bid = blockIdx.x;
+    const int ks = bid % KS, rb = bid / KS;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t row0 = (int64_t)rb * 256 + wave * 64;
+    const int k0 = ks * kchunk, k1 = min(n, k0 + kchunk);
+    const int lr = lane & 15, lk = lane >> 4;
+    doublex4 acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) acc[a][b] = doublex4{0, 0, 0, 0};
+    const float* Ab = A + row0 + lr;
+    const double* Xb = X + lr;
+    for (int k = k0; k < k1; k += 4) {
+        const int64_t kk = k + lk;
+        const bool ok = kk < k1;
+        double a[4], b[4];
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            a[t] = ok ? (double)Ab[kk * lda + 16 * t] : 0.0;
+            b[t] = ok ? Xb[kk * KP + 16 * t] : 0.0;
+        }
+#pragma unroll
+        for (int rt = 0; rt < 4; rt++)
+#pragma unroll
+            for (int ct = 0; ct < 4; ct++)
+                acc[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rt], b[ct], acc[rt][ct], 0, 0, 0);
+    }
+    double* out = part + (int64_t)ks * lda * KP;
+#pragma unroll
+    for (int rt = 0; rt < 4; rt++)
+#pragma unroll
+        for (int ct = 0; ct < 4; ct++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int64_t r = row0 + 16 * rt + 4 * lk + j;
+                out[r * KP + 16 * ct + lr] = acc[rt][ct][j];
+            }
+}
+
+// ---------------------------------------------------------------- helpers
+// Per-column block partial sums: thread t -> column t%64, row group t/64.
+__device__ void col_partial(double v, double* sh, double* dst) {
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    if (threadIdx.x < KP) {
+        double s = sh[threadIdx.x] + sh[threadIdx.x + 64];
+        s += sh[threadIdx.x + 128];
+        s += sh[threadIdx.x + 192];
+        dst[threadIdx.x] = s;
+    }
+    __syncthreads();
+}
+
+__global__ void sigma_rows_kernel(int n, int64_t lda, int kc, const double* __restrict__ nrm,
+                                  const double* __restrict__ vinf4, double* __restrict__ S) {
+    int64_t idx = (int64_t)blockIdx.x * kT + threadIdx.x;
+    if (idx >= lda * KP) return;
+    int64_t i = idx / KP;
+    int c = (int)(idx % KP);
+    double v = 0;
+    if (i < n && c < kc)
+        v = -(vinf4[4 * c] * nrm[i] + vinf4[4 * c + 1] * nrm[lda + i] + vinf4[4 * c + 2] * nrm[2 * lda + i]);
+    S[idx] = v;
+}
+
+// Y = Badd + sign * sum_ks part ; partial ||Y_c||^2
+__global__ __launch_bounds__(256) void reduce64_kernel(const double* __restrict__ part, int KS, int64_t lda, int n,
+                                                       const double* __restrict__ Badd, double sign,
+                                                       double* __restrict__ Y, double* __restrict__ dpart) {
+    __shared__ double sh[kT];
+    const int c = threadIdx.x & 63, g = threadIdx.x >> 6;
+    double acc = 0;
+    for (int rr = g; rr < kRedRows; rr += 4) {
+        int64_t i = (int64_t)blockIdx.x * kRedRows + rr;
+        int64_t o = i * KP + c;
+        double s = 0;
+        for (int q = 0; q < KS; q++) s += part[(int64_t)q * lda * KP + o];
+        double y = (Badd ? Badd[o] : 0.0) + sign * s;
+        if (i >= n) y = 0;
+        Y[o] = y;
+        acc += y * y;
+    }
+    col_partial(acc, sh, dpart + (int64_t)blockIdx.x * KP);
+}
+
+// sum of per-block column partials (fixed order) into sc[c][q]
+__global__ void colsum_kernel(const double* __restrict__ dpart, int nblk, int nq, int q0, int q1,
+                              double* __restrict__ sc) {
+    int c = threadIdx.x;
+    if (c >= KP) return;
+    for (int q = 0; q < nq; q++) {
+        double s = 0;
+        const double* d = dpart + (int64_t)q * nblk * KP;
+        for (int b = 0; b < nblk; b++) s += d[(int64_t)b * KP + c];
+        sc[c * Q_NQ + (q == 0 ? q0 : q1)] = s;
+    }
+}
+
+// inner init: r = R/|R| (FP32), r0 = r, p = v = d = 0, partial r.r
+__global__ __launch_bounds__(256) void inner_init_kernel(int n, const double* __restrict__ R,
+                                                         const double* __restrict__ sc, const int* __restrict__ act,
+                                                         float* __restrict__ r, float* __restrict__ r0,
+                                                         float* __restrict__ p, float* __restrict__ v,
+                                                         float* __restrict__ d, double* __restrict__ dpart) {
+    __shared__ double sh[kT];
+    const int c = threadIdx.x & 63, g = threadIdx.x >> 6;
+    double scale = sc[c * Q_NQ + Q_SCALE];
+    double inv = scale > 0 ? 1.0 / scale : 0.0;
+    double acc = 0;
+    for (int rr = g; rr < kRedRows; rr += 4) {
+        int64_t i = (int64_t)blockIdx.x * kRedRows + rr;
+        int64_t o = i * KP + c;
+        float ri = (act[c] && i < n) ? (float)(R[o] * inv) : 0.f;
+        r[o] = ri;
+        r0[o] = ri;
+        p[o] = 0.f;
+        v[o] = 0.f;
+        d[o] = 0.f;
+        acc += (double)ri * ri;
+    }
+    col_partial(acc, sh, dpart + (int64_t)blockIdx.x * KP);
+}
+
+// per-column scalar steps (one block of 64 threads)
+__global__ void scal_init_kernel(double* __restrict__ sc) {
+    int c = threadIdx.x;
+    if (c >= KP) return;
+    double* s = sc + c * Q_NQ;
+    s[Q_RR0] = s[Q_RR];
+    s[Q_R0R] = s[Q_RR];
+    s[Q_RHO] = 1;
+    s[Q_ALPHA] = 1;
+    s[Q_OMEGA] = 1;
+}
+
+// rho/beta step at the top of an inner iteration; deactivates converged or
+// broken-down columns (breakdown is left to the next refinement sweep)
+__global__ void scal_beta_kernel(double* __restrict__ sc, int* __restrict__ act, double tol2) {
+    int c = threadIdx.x;
+    if (c >= KP) return;
+    double* s = sc + c * Q_NQ;
+    if (!act[c]) return;
+    if (s[Q_RR] <= tol2 * s[Q_RR0]) {
+        act[c] = 0;
+        return;
+    }
+    double rho_old = s[Q_RHO];
+    double rho = s[Q_R0R];
+    const double eps = 1.1920928955078125e-07;
+    if (!(fabs(rho) > eps * eps * s[Q_RR0]) || !(s[Q_OMEGA] != 0)) {
+        act[c] = 0;
+        return;
+    }
+    s[Q_BETA] = (rho / rho_old) * (s[Q_ALPHA] / s[Q_OMEGA]);
+    s[Q_RHO] = rho;
+}
+
+__global__ void scal_alpha_kernel(double* __restrict__ sc, const int* __restrict__ act) {
+    int c = threadIdx.x;
+    if (c >= KP || !act[c]) return;
+    double* s = sc + c * Q_NQ;
+    s[Q_ALPHA] = s[Q_RHO] / s[Q_R0V];
+}
+
+__global__ void scal_omega_kernel(double* __restrict__ sc, const int* __restrict__ act) {
+    int c = threadIdx.x;
+    if (c >= KP || !act[c]) return;
+    double* s = sc + c * Q_NQ;
+    s[Q_OMEGA] = s[Q_TT] > 0 ? s[Q_TS] / s[Q_TT] : 0.0;
+}
+
+// p = r + beta (p - omega v); y = invd p
+__global__ __launch_bounds__(256) void inner_p_kernel(int64_t rows, const double* __restrict__ sc,
+                                                      const int* __restrict__ act, const float* __restrict__ r,
+                                                      float* __restrict__ p, const float* __restrict__ v,
+                                                      const float* __restrict__ invd, float* __restrict__ y) {
+    int64_t idx = (int64_t)blockIdx.x * kT + threadIdx.x;
+    if (idx >= rows * KP) return;
+    int c = (int)(idx & 63);
+    if (!act[c]) return;
+    float beta = (float)sc[c * Q_NQ + Q_BETA], omega = (float)sc[c * Q_NQ + Q_OMEGA];
+    float pi = r[idx] + beta * (p[idx] - omega * v[idx]);
+    p[idx] = pi;
+    y[idx] = invd[idx >> 6] * pi;
+}
+
+// v = sum parts; partial r0.v
+__global__ __launch_bounds__(256) void inner_v_kernel(const float* __restrict__ part, int KS, int64_t lda, int n,
+                                                      const int* __restrict__ act, float* __restrict__ v,
+                                                      const float* __restrict__ r0, double* __restrict__ dpart) {
+    __shared__ double sh[kT];
+    const int c = threadIdx.x & 63, g = threadIdx.x >> 6;
+    double acc = 0;
+    if (act[c]) {
+        for (int rr = g; rr < kRedRows; rr += 4) {
+            int64_t i = (int64_t)blockIdx.x * kRedRows + rr;
+            int64_t o = i * KP + c;
+            float s = 0;
+            for (int q = 0; q < KS; q++) s += part[(int64_t)q * lda * KP + o];
+            if (i >= n) s = 0;
+            v[o] = s;
+            acc += (double)r0[o] * s;
+        }
+    }
+    col_partial(acc, sh, dpart + (int64_t)blockIdx.x * KP);
+}
+
+// s = r - alpha v; z = invd s
+__global__ __launch_bounds__(256) void inner_s_kernel(int64_t rows, const double* __restrict__ sc,
+                                                      const int* __restrict__ act, const float* __restrict__ r,
+                                                      const float* __restrict__ v, const float* __restrict__ invd,
+                                                      float* __restrict__ s, float* __restrict__ z) {
+    int64_t idx = (int64_t)blockIdx.x * kT + threadIdx.x;
+    if (idx >= rows * KP) return;
+    int c = (int)(idx & 63);
+    if (!act[c]) return;
+    float alpha = (float)sc[c * Q_NQ + Q_ALPHA];
+    float si = r[idx] - alpha * v[idx];
+    s[idx] = si;
+    z[idx] = invd[idx >> 6] * si;
+}
+
+// t = sum parts; partial t.t and t.s
+__global__ __launch_bounds__(256) void inner_t_kernel(const float* __restrict__ part, int KS, int64_t lda, int n,
+                                                      const int* __restrict__ act, float* __restrict__ t,
+                                                      const float* __restrict__ s, double* __restrict__ dtt,
+                                                      double* __restrict__ dts) {
+    __shared__ double sh[kT];
+    const int c = threadIdx.x & 63, g = threadIdx.x >> 6;
+    double a = 0, b = 0;
+    if (act[c]) {
+        for (int rr = g; rr < kRedRows; rr += 4) {
+            int64_t i = (int64_t)blockIdx.x * kRedRows + rr;
+            int64_t o = i * KP + c;
+            float x = 0;
+            for (int q = 0; q < KS; q++) x += part[(int64_t)q * lda * KP + o];
+            if (i >= n) x = 0;
+            t[o] = x;
+            a += (double)x * x;
+            b += (double)x * s[o];
+        }
+    }
+    col_partial(a, sh, dtt + (int64_t)blockIdx.x * KP);
+    col_partial(b, sh, dts + (int64_t)blockIdx.x * KP);
+}
+
+// d += alpha y + omega z; r = s - omega t; partial r.r and r0.r
+__global__ __launch_bounds__(256) void inner_x_kernel(int n, const double* __restrict__ sc,
+                                                      const int* __restrict__ act, float* __restrict__ d,
+                                                      const float* __restrict__ y, const float* __restrict__ z,
+                                                      float* __restrict__ r, const float* __restrict__ s,
+                                                      const float* __restrict__ t, const float* __restrict__ r0,
+                                                      double* __restrict__ drr, double* __restrict__ dr0r) {
+    __shared__ double sh[kT];
+    const int c = threadIdx.x & 63, g = threadIdx.x >> 6;
+    double a = 0, b = 0;
+    if (act[c]) {
+        float alpha = (float)sc[c * Q_NQ + Q_ALPHA], omega = (float)sc[c * Q_NQ + Q_OMEGA];
+        for (int rr = g; rr < kRedRows; rr += 4) {
+            int64_t i = (int64_t)blockIdx.x * kRedRows + rr;
+            int64_t o = i * KP + c;
+            d[o] += alpha * y[o] + omega * z[o];
+            float ri = s[o] - omega * t[o];
+            r[o] = ri;
+            a += (double)ri * ri;
+            b += (double)r0[o] * ri;
+        }
+    }
+    col_partial(a, sh, drr + (int64_t)blockIdx.x * KP);
+    col_partial(b, sh, dr0r + (int64_t)blockIdx.x * KP);
+}
+
+// X += scale_c * d  (FP64 accumulation of the FP32 correction)
+__global__ __launch_bounds__(256) void correct_kernel(int64_t rows, const double* __restrict__ sc,
+                                                      const int* __restrict__ sweep, double* __restrict__ X,
+                                                      const float* __restrict__ d) {
+    int64_t idx = (int64_t)blockIdx.x * kT + threadIdx.x;
+    if (idx >= rows * KP) return;
+    int c = (int)(idx & 63);
+    if (!sweep[c]) return;
+    X[idx] += sc[c * Q_NQ + Q_SCALE] * (double)d[idx];
+}
+
+__global__ void invd32_kernel(int n, const float* __restrict__ A, int64_t lda, float* __restrict__ invd) {
+    int64_t j = (int64_t)blockIdx.x * kT + threadIdx.x;
+    if (j >= lda) return;
+    float d = j < n ? A[j + j * lda] : 0.f;
+    invd[j] = d != 0 ? 1.f / d : 1.f;
+}
+
+__global__ void scatter_x_kernel(int n, int kc, const double* __restrict__ X, double* __restrict__ x, int64_t ldx) {
+    int64_t idx = (int64_t)blockIdx.x * kT + threadIdx.x;
+    int64_t i = idx / KP;
+    int c = (int)(idx % KP);
+    if (i >= n || c >= kc) return;
+    x[(int64_t)c * ldx + i] = X[idx];
+}
+
+__global__ void sigma_kernel(int n, const double* __restrict__ nrm, int64_t lda, const double* __restrict__ v4,
+                             double* __restrict__ sig) {
+    int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+    if (i >= lda) return;
+    sig[i] = i < n ? -(v4[0] * nrm[i] + v4[1] * nrm[lda + i] + v4[2] * nrm[2 * lda + i]) : 0.0;
+}
+
+inline unsigned nblocks(int64_t work) { return (unsigned)((work + kT - 1) / kT); }
+
+}  // namespace
+
+void launch_sigma(int n, int64_t lda, const double* normals, const double* vinf4, double* sigma, hipStream_t s) {
+    hipLaunchKernelGGL(sigma_kernel, dim3(nblocks(lda)), dim3(kT), 0, s, n, normals, lda, vinf4, sigma);
+    CP_LAUNCH_CHECK();
+}
+
+BatchedSolver::BatchedSolver(int n, int64_t lda, int capacity) : n_(n), cap_(capacity), lda_(lda) {
+    nrb_ = (int)(lda / kRedRows);
+    const size_t mv64 = (size_t)lda * KP * sizeof(double);
+    const size_t mv32 = (size_t)lda * KP * sizeof(float);
+    const int KSmax = 32;
+    size_t bytes = 4 * mv64 + 9 * mv32 + (size_t)lda * sizeof(float) + KSmax * mv32 + KSmax * mv64 +
+                   (size_t)4 * nrb_ * KP * sizeof(double) + KP * Q_NQ * sizeof(double) + KP * sizeof(int) + 1024;
+    CP_HIP(hipMalloc(&ws_, bytes));
+    char* p = (char*)ws_;
+    auto take = [&](size_t b) {
+        char* q = p;
+        p += (b + 255) / 256 * 256;
+        return q;
+    };
+    X_ = (double*)take(mv64);
+    Bv_ = (double*)take(mv64);
+    R_ = (double*)take(mv64);
+    S_ = (double*)take(mv64);
+    float** f32[9] = {&r_, &r0_, &p_, &v_, &sv_, &t_, &y_, &z_, &d_};
+    for (auto q : f32) *q = (float*)take(mv32);
+    invd_ = (float*)take((size_t)lda * sizeof(float));
+    part32_ = (float*)take(KSmax * mv32);
+    part64_ = (double*)take(KSmax * mv64);
+    dpart_ = (double*)take((size_t)4 * nrb_ * KP * sizeof(double));
+    sc_ = (double*)take(KP * Q_NQ * sizeof(double));
+    act_ = (int*)take(KP * sizeof(int));
+    CP_HIP(hipHostMalloc(&host_sc_, KP * Q_NQ * sizeof(double), hipHostMallocDefault));
+    CP_HIP(hipHostMalloc(&host_act_, 2 * KP * sizeof(int), hipHostMallocDefault));
+    // default shape: 8 K slices (one per XCD), more for small matrices
+    int rb = (int)(lda / 256);
+    shape_.kslices = rb >= 64 ? 8 : (rb >= 16 ? 16 : 32);
+}
+
+BatchedSolver::~BatchedSolver() {
+    if (ws_) (void)hipFree(ws_);
+    if (host_sc_) (void)hipHostFree(host_sc_);
+    if (host_act_) (void)hipHostFree(host_act_);
+}
+
+void BatchedSolver::matvec32(const float* A32, const float* P, float* Y, hipStream_t s) {
+    const int KS = shape_.kslices;
+    int kchunk = (int)round_up((n_ + KS - 1) / KS, 16);
+    dim3 grid((unsigned)((lda_ / 256) * KS));
+    {
+        TimedScope ts("gemm_f32", s);
+        hipLaunchKernelGGL(gemm_f32_kernel, grid, dim3(256), 0, s, A32, lda_, n_, P, part32_, kchunk, KS);
+    }
+    CP_LAUNCH_CHECK();
+    if (Y) {
+        // plain reduction (used by tests and tuning)
+        hipLaunchKernelGGL(inner_v_kernel, dim3((unsigned)nrb_), dim3(kT), 0, s, (const float*)part32_, KS, lda_, n_,
+                           (const int*)act_, Y, (const float*)Y, dpart_);
+        CP_LAUNCH_CHECK();
+    }
+}
+
+void BatchedSolver::matvec64acc(const float* A32, const double* X, double* Y, const double* Badd, double sign,
+                                hipStream_t s) {
+    const int KS = shape_.kslices;
+    int kchunk = (int)round_up((n_ + KS - 1) / KS, 16);
+    dim3 grid((unsigned)((lda_ / 256) * KS));
+    {
+        TimedScope ts("resid_f64", s);
+        hipLaunchKernelGGL(gemm_f64acc_kernel, grid, dim3(256), 0, s, A32, lda_, n_, X, part64_, kchunk, KS);
+    }
+    CP_LAUNCH_CHECK();
+    hipLaunchKernelGGL(reduce64_kernel, dim3((unsigned)nrb_), dim3(kT), 0, s, (const double*)part64_, KS, lda_, n_,
+                       Badd, sign, Y, dpart_);
+    CP_LAUNCH_CHECK();
+}
+
+void BatchedSolver::solve(const float* A32, const float* B32, const double* normals, const double* vinf4, int k,
+                          const BatchedOpts& o, double* x, int64_t ldx, SolveStats* stats, hipStream_t s) {
+    for (int c0 = 0; c0 < k; c0 += KP) {
+        int kc = std::min(KP, k - c0);
+        tile_solve(A32, B32, normals, vinf4 + 4 * c0, c0, kc, o, x + (int64_t)c0 * ldx, ldx, stats + c0, s);
+    }
+}
+
+void BatchedSolver::tile_solve(const float* A32, const float* B32, const double* normals, const double* vinf4,
+                               int c0, int kc, const BatchedOpts& o, double* x, int64_t ldx, SolveStats* stats,
+                               hipStream_t s) {
+    (void)c0;
+    const int64_t rows = lda_;
+    const unsigned nel = nblocks(rows * KP);
+    const int KS = shape_.kslices;
+    int* host_sweep = host_act_ + KP;
+
+    // sigma multivector and RHS b = -B32 sigma (FP64 accumulation)
+    hipLaunchKernelGGL(sigma_rows_kernel, dim3(nel), dim3(kT), 0, s, n_, lda_, kc, normals, vinf4, S_);
+    matvec64acc(B32, S_, Bv_, nullptr, -1.0, s);
+    auto colsum = [&](const double* dp, int nq, int q0, int q1) {
+        hipLaunchKernelGGL(colsum_kernel, dim3(1), dim3(KP), 0, s, dp, nrb_, nq, q0, q1, sc_);
+        CP_LAUNCH_CHECK();
+    };
+    colsum(dpart_, 1, Q_BB, 0);
+    hipLaunchKernelGGL(invd32_kernel, dim3(nblocks(lda_)), dim3(kT), 0, s, n_, A32, lda_, invd_);
+    CP_HIP(hipMemsetAsync(X_, 0, (size_t)lda_ * KP * sizeof(double), s));
+    std::vector<double> bb(KP, 0.0);
+    CP_HIP(hipMemcpyAsync(host_sc_, sc_, KP * Q_NQ * sizeof(double), hipMemcpyDeviceToHost, s));
+    CP_HIP(hipStreamSynchronize(s));
+    for (int c = 0; c < kc; c++) bb[c] = host_sc_[c * Q_NQ + Q_BB];
+    std::vector<int> done(KP, 1);
+    for (int c = 0; c < kc; c++) {
+        done[c] = bb[c] == 0;
+        stats[c] = SolveStats();
+    }
+    const double tol2 = o.tol * o.tol;
+    const double itol2 = o.inner_tol * o.inner_tol;
+    int sweep = 0;
+    std::vector<double> rel(KP, 0.0);
+    while (true) {
+        // residual R = b - A32 X in FP64
+        matvec64acc(A32, X_, R_, Bv_, -1.0, s);
+        colsum(dpart_, 1, Q_RR64, 0);
+        CP_HIP(hipMemcpyAsync(host_sc_, sc_, KP * Q_NQ * sizeof(double), hipMemcpyDeviceToHost, s));
+        CP_HIP(hipStreamSynchronize(s));
+        bool any = false;
+        for (int c = 0; c < KP; c++) {
+            host_sweep[c] = 0;
+            if (c >= kc || bb[c] == 0) continue;
+            double rr = host_sc_[c * Q_NQ + Q_RR64];
+            rel[c] = std::sqrt(rr / bb[c]);
+            stats[c].rel_residual = rel[c];
+            if (rr <= tol2 * bb[c]) done[c] = 1;
+            if (!done[c]) {
+                host_sweep[c] = 1;
+                any = true;
+                host_sc_[c * Q_NQ + Q_SCALE] = std::sqrt(rr);
+            } else {
+                host_sc_[c * Q_NQ + Q_SCALE] = 0;
+            }
+        }
+        if (!any || sweep >= o.max_refine) break;
+        ++sweep;
+        // inner FP32 solve of A32 d = R / |R| on the sweeping columns
+        CP_HIP(hipMemcpyAsync(sc_, host_sc_, KP * Q_NQ * sizeof(double), hipMemcpyHostToDevice, s));
+        CP_HIP(hipMemcpyAsync(act_, host_sweep, KP * sizeof(int), hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(inner_init_kernel, dim3((unsigned)nrb_), dim3(kT), 0, s, n_, (const double*)R_,
+                           (const double*)sc_, (const int*)act_, r_, r0_, p_, v_, d_, dpart_);
+        colsum(dpart_, 1, Q_RR, 0);
+        hipLaunchKernelGGL(scal_init_kernel, dim3(1), dim3(KP), 0, s, sc_);
+        int it = 0;
+        const int maxit = o.max_iter > 0 ? o.max_iter : 2 * n_;
+        while (it < maxit) {
+            hipLaunchKernelGGL(scal_beta_kernel, dim3(1), dim3(KP), 0, s, sc_, act_, itol2);
+            // check for all columns finished (one small read per iteration)
+            CP_HIP(hipMemcpyAsync(host_act_, act_, KP * sizeof(int), hipMemcpyDeviceToHost, s));
+            CP_HIP(hipStreamSynchronize(s));
+            int nact = 0;
+            for (int c = 0; c < KP; c++) nact += host_act_[c] != 0;
+            if (nact == 0) break;
+            hipLaunchKernelGGL(inner_p_kernel, dim3(nel), dim3(kT), 0, s, rows, (const double*)sc_, (const int*)act_,
+                               (const float*)r_, p_, (const float*)v_, (const float*)invd_, y_);
+            matvec32(A32, y_, nullptr, s);
+            hipLaunchKernelGGL(inner_v_kernel, dim3((unsigned)nrb_), dim3(kT), 0, s, (const float*)part32_, KS, lda_,
+                               n_, (const int*)act_, v_, (const float*)r0_, dpart_);
+            colsum(dpart_, 1, Q_R0V, 0);
+            hipLaunchKernelGGL(scal_alpha_kernel, dim3(1), dim3(KP), 0, s, sc_, (const int*)act_);
+            hipLaunchKernelGGL(inner_s_kernel, dim3(nel), dim3(kT), 0, s, rows, (const double*)sc_, (const int*)act_,
+                               (const float*)r_, (const float*)v_, (const float*)invd_, sv_, z_);
+            matvec32(A32, z_, nullptr, s);
+            hipLaunchKernelGGL(inner_t_kernel, dim3((unsigned)nrb_), dim3(kT), 0, s, (const float*)part32_, KS, lda_,
+                               n_, (const int*)act_, t_, (const float*)sv_, dpart_, dpart_ + (size_t)nrb_ * KP);
+            colsum(dpart_, 2, Q_TT, Q_TS);
+            hipLaunchKernelGGL(scal_omega_kernel, dim3(1), dim3(KP), 0, s, sc_, (const int*)act_);
+            hipLaunchKernelGGL(inner_x_kernel, dim3((unsigned)nrb_), dim3(kT), 0, s, n_, (const double*)sc_,
+                               (const int*)act_, d_, (const float*)y_, (const float*)z_, r_, (const float*)sv_,
+                               (const float*)t_, (const float*)r0_, dpart_, dpart_ + (size_t)nrb_ * KP);
+            colsum(dpart_, 2, Q_RR, Q_R0R);
+            CP_LAUNCH_CHECK();
+            ++it;
+            for (int c = 0; c < kc; c++)
+                if (host_act_[c]) stats[c].iters++;
+        }
+        // X += |R| d on the columns of this sweep
+        CP_HIP(hipMemcpyAsync(act_, host_sweep, KP * sizeof(int), hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(correct_kernel, dim3(nel), dim3(kT), 0, s, rows, (const double*)sc_, (const int*)act_, X_,
+                           (const float*)d_);
+        CP_LAUNCH_CHECK();
+        for (int c = 0; c < kc; c++)
+            if (host_sweep[c]) stats[c].refinements++;
+    }
+    hipLaunchKernelGGL(scatter_x_kernel, dim3(nel), dim3(kT), 0, s, n_, kc, (const double*)X_, x, ldx);
+    CP_LAUNCH_CHECK();
+}
+
+}  // namespace cpgpu

@@ -1,0 +1,607 @@
+// C ABI of the MI355X CPanel solve path (include/cpgpu.h).
+//
+// Objects:
+//   cp_ctx  - one GPU: device id, stream, HBM budget accounting, timers.
+//   cp_geom - one mesh: host topology, device panel tables, resident A/B in
+//             FP64 and/or FP32, post-processing tables, solver workspaces.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/cpgpu.h"
+#include "batched.h"
+#include "device.h"
+#include "geometry.h"
+#include "krylov.h"
+#include "panelrec.h"
+#include "post.h"
+#include "timer.h"
+
+using namespace cpgpu;
+
+struct cp_ctx {
+    int device = 0;
+    uint64_t budget = 0;
+    uint64_t in_use = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    int hip_err = 0;
+    KernelTimer timer;
+    std::mutex mu;
+};
+
+namespace {
+
+struct DevBuf {
+    cp_ctx* ctx = nullptr;
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) {
+            (void)hipFree(p);
+            ctx->in_use -= bytes;
+            p = nullptr;
+            bytes = 0;
+        }
+    }
+    template <class T>
+    T* as() const {
+        return reinterpret_cast<T*>(p);
+    }
+};
+
+struct BudgetError {
+    uint64_t need;
+};
+
+void dalloc(cp_ctx* c, DevBuf& b, size_t bytes) {
+    b.release();
+    if (c->budget && c->in_use + bytes > c->budget) throw BudgetError{bytes};
+    b.ctx = c;
+    CP_HIP(hipMalloc(&b.p, bytes));
+    b.bytes = bytes;
+    c->in_use += bytes;
+}
+
+template <class T>
+void upload(cp_ctx* c, DevBuf& b, const std::vector<T>& v) {
+    dalloc(c, b, std::max<size_t>(v.size(), 1) * sizeof(T));
+    if (!v.empty()) CP_HIP(hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+}
+
+}  // namespace
+
+struct cp_geom {
+    cp_ctx* ctx = nullptr;
+    uint64_t key = 0;
+    Geometry host;
+    DevGeom dev;
+    DevBuf body, wake, cxyz, fold_ptr, fold_w, fold_c, normals;
+    DevBuf W;          // wake influence N x Nw (FP64)
+    DevBuf A64, B64, A32, B32;
+    bool have64 = false, have32 = false;
+    // post tables
+    bool post_ready = false;
+    PostHost post_host;
+    PostDev post;
+    DevBuf p_nbr, p_w1, p_w2, p_mode, p_te, p_rphi, p_rv, p_rw, p_simp, p_ctr, p_bez, p_area;
+    std::unique_ptr<Bicgstab64> bicg;
+    std::unique_ptr<BatchedSolver> batched;
+};
+
+namespace {
+
+cp_status fail(cp_ctx* c, cp_status s, const std::string& msg) {
+    if (c) c->err = msg;
+    return s;
+}
+
+template <class F>
+cp_status guarded(cp_ctx* c, F&& f) {
+    if (!c) return CP_BAD_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    set_current_timer(&c->timer);
+    try {
+        if (c->device >= 0) CP_HIP(hipSetDevice(c->device));
+        cp_status s = f();
+        set_current_timer(nullptr);
+        return s;
+    } catch (const HipError& e) {
+        set_current_timer(nullptr);
+        c->hip_err = (int)e.code;
+        c->err = std::string("HIP error ") + hipGetErrorString(e.code) + " in " + e.where;
+        return CP_HIP_ERROR;
+    } catch (const BudgetError& e) {
+        set_current_timer(nullptr);
+        c->err = "HBM budget exceeded: need " + std::to_string(e.need) + " more bytes, " +
+                 std::to_string(c->in_use) + " of " + std::to_string(c->budget) + " in use";
+        return CP_OUT_OF_BUDGET;
+    } catch (const std::bad_alloc&) {
+        set_current_timer(nullptr);
+        c->err = "host allocation failed";
+        return CP_OUT_OF_BUDGET;
+    }
+}
+
+// cpCase::windToBody (cpCase.cpp:142-166): body-axis freestream and the
+// wind<-body transform T (row major)
+void wind_to_body(const cp_freestream& f, double T[9], double Vb[3]) {
+    double a = f.alpha_deg * kPI / 180, b = f.beta_deg * kPI / 180;
+    T[0] = std::cos(a) * std::cos(b);
+    T[1] = std::cos(a) * std::sin(b);
+    T[2] = -std::sin(a);
+    T[3] = -std::sin(b);
+    T[4] = std::cos(b);
+    T[5] = 0;
+    T[6] = std::sin(a) * std::cos(b);
+    T[7] = std::sin(a) * std::sin(b);
+    T[8] = std::cos(a);
+    Vb[0] = T[0] * f.V;
+    Vb[1] = T[3] * f.V;
+    Vb[2] = T[6] * f.V;
+}
+
+cp_status build_geom(cp_ctx* c, const cp_mesh* m, uint64_t key, cp_geom** out) {
+    std::unique_ptr<cp_geom> g(new cp_geom());
+    g->ctx = c;
+    g->key = key;
+    std::string err;
+    if (!build_geometry(m->n_nodes, m->xyz, m->n_tris, m->tri, m->ids, g->host, err))
+        return fail(c, CP_BAD_MESH, err);
+    const Geometry& G = g->host;
+    const int n = (int)G.body.size(), nw = (int)G.wake.size();
+    DevGeom& d = g->dev;
+    d.n = n;
+    d.nw = nw;
+    d.lda = round_up(n, 512);
+    if (c->device < 0) {  // host-only context: topology tables only
+        *out = g.release();
+        return CP_OK;
+    }
+    std::vector<PanelRec> br(n), wr(std::max(nw, 1));
+    for (int i = 0; i < n; i++) br[i] = make_panel_rec(G.body[i]);
+    for (int j = 0; j < nw; j++) wr[j] = make_panel_rec(G.wake[j]);
+    upload(c, g->body, br);
+    upload(c, g->wake, wr);
+    // collocation points padded with a far point (padding rows are zeroed)
+    std::vector<double> cxyz(3 * (size_t)d.lda, 1.0e6), nrm(3 * (size_t)d.lda, 0.0);
+    for (int i = 0; i < n; i++) {
+        cxyz[i] = G.body[i].center.x;
+        cxyz[d.lda + i] = G.body[i].center.y;
+        cxyz[2 * d.lda + i] = G.body[i].center.z;
+        nrm[i] = G.body[i].normal.x;
+        nrm[d.lda + i] = G.body[i].normal.y;
+        nrm[2 * d.lda + i] = G.body[i].normal.z;
+    }
+    upload(c, g->cxyz, cxyz);
+    upload(c, g->normals, nrm);
+    // fold lists per body column in (wake panel, slot) order
+    std::vector<std::vector<std::pair<int, double>>> cols(n);
+    for (int j = 0; j < nw; j++) {
+        double t = G.fold_coef[j];
+        double cf[4] = {1 - t, t - 1, t, -t};
+        for (int s = 0; s < 4; s++) cols[G.fold_idx[4 * j + s]].push_back({j, cf[s]});
+    }
+    std::vector<int> fp(n + 1, 0), fw;
+    std::vector<double> fc;
+    for (int i = 0; i < n; i++) {
+        fp[i] = (int)fw.size();
+        for (auto& e : cols[i]) {
+            fw.push_back(e.first);
+            fc.push_back(e.second);
+        }
+    }
+    fp[n] = (int)fw.size();
+    upload(c, g->fold_ptr, fp);
+    upload(c, g->fold_w, fw);
+    upload(c, g->fold_c, fc);
+    d.body = g->body.as<PanelRec>();
+    d.wake = g->wake.as<PanelRec>();
+    d.cx = g->cxyz.as<double>();
+    d.cy = d.cx + d.lda;
+    d.cz = d.cx + 2 * d.lda;
+    d.fold_ptr = g->fold_ptr.as<int>();
+    d.fold_w = g->fold_w.as<int>();
+    d.fold_c = g->fold_c.as<double>();
+    *out = g.release();
+    return CP_OK;
+}
+
+void ensure_wake_influence(cp_geom* g) {
+    if (g->W.p || g->dev.nw == 0) return;
+    cp_ctx* c = g->ctx;
+    dalloc(c, g->W, (size_t)g->dev.lda * g->dev.nw * sizeof(double));
+    launch_wake_influence(g->dev, g->W.as<double>(), g->dev.lda, c->stream);
+}
+
+void ensure_post(cp_geom* g) {
+    if (g->post_ready) return;
+    cp_ctx* c = g->ctx;
+    build_post_tables(g->host, g->post_host);
+    const PostHost& H = g->post_host;
+    const int n = H.n;
+    upload(c, g->p_nbr, H.nbr);
+    upload(c, g->p_w1, H.w1);
+    upload(c, g->p_w2, H.w2);
+    upload(c, g->p_mode, H.mode);
+    upload(c, g->p_te, H.te_idx);
+    upload(c, g->p_rphi, H.rphi);
+    upload(c, g->p_rv, H.rv);
+    upload(c, g->p_rw, H.rw);
+    upload(c, g->p_simp, H.simpson);
+    std::vector<double> ctr(3 * (size_t)n), bez(3 * (size_t)n), area(n);
+    for (int i = 0; i < n; i++) {
+        const PanelGeom& p = g->host.body[i];
+        ctr[i] = p.center.x;
+        ctr[n + i] = p.center.y;
+        ctr[2 * (size_t)n + i] = p.center.z;
+        bez[i] = p.bez.x;
+        bez[n + i] = p.bez.y;
+        bez[2 * (size_t)n + i] = p.bez.z;
+        area[i] = p.area;
+    }
+    upload(c, g->p_ctr, ctr);
+    upload(c, g->p_bez, bez);
+    upload(c, g->p_area, area);
+    PostDev& P = g->post;
+    P.n = n;
+    P.n_te = H.n_te;
+    P.n_st = H.n_st;
+    P.n_wakes = H.n_wakes;
+    P.nbr = g->p_nbr.as<int32_t>();
+    P.w1 = g->p_w1.as<double>();
+    P.w2 = g->p_w2.as<double>();
+    P.mode = g->p_mode.as<int32_t>();
+    P.te_idx = g->p_te.as<int32_t>();
+    P.rphi = g->p_rphi.as<double>();
+    P.rv = g->p_rv.as<double>();
+    P.rw = g->p_rw.as<double>();
+    P.simpson = g->p_simp.as<double>();
+    P.ctr = g->p_ctr.as<double>();
+    P.bez = g->p_bez.as<double>();
+    P.area = g->p_area.as<double>();
+    g->post_ready = true;
+}
+
+void fill_info(const double* r, const double T[9], cp_solve_info& inf) {
+    inf.CL = r[0];
+    inf.CDi = r[1];
+    for (int a = 0; a < 3; a++) inf.Fbody[a] = r[2 + a];
+    for (int a = 0; a < 3; a++) inf.CM[a] = r[8 + a];
+    // bodyToWind: T^T F (cpCase.cpp:168-172)
+    for (int a = 0; a < 3; a++) inf.Fwind[a] = T[0 * 3 + a] * r[2] + T[1 * 3 + a] * r[3] + T[2 * 3 + a] * r[4];
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* cp_status_string(cp_status s) {
+    switch (s) {
+        case CP_OK: return "ok";
+        case CP_NOT_CONVERGED: return "not converged";
+        case CP_OUT_OF_BUDGET: return "out of HBM budget";
+        case CP_BAD_MESH: return "bad mesh";
+        case CP_HIP_ERROR: return "HIP error";
+        case CP_BAD_ARG: return "bad argument";
+        case CP_NO_DEVICE: return "no device";
+        case CP_NOT_ASSEMBLED: return "matrices not assembled";
+    }
+    return "unknown";
+}
+
+cp_status cp_create(int device, uint64_t budget, cp_ctx** out) {
+    if (!out) return CP_BAD_ARG;
+    *out = nullptr;
+    if (device == -1) {  // host-only: geometry/topology queries, no device work
+        cp_ctx* c = new cp_ctx();
+        c->device = -1;
+        c->budget = budget;
+        *out = c;
+        return CP_OK;
+    }
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= device || device < 0) return CP_NO_DEVICE;
+    cp_ctx* c = new cp_ctx();
+    c->device = device;
+    c->budget = budget;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return CP_HIP_ERROR;
+    }
+    *out = c;
+    return CP_OK;
+}
+
+void cp_destroy(cp_ctx* c) {
+    if (!c) return;
+    if (c->device < 0) {
+        delete c;
+        return;
+    }
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* cp_last_error(const cp_ctx* c) { return c ? c->err.c_str() : "no context"; }
+int cp_last_hip_error(const cp_ctx* c) { return c ? c->hip_err : 0; }
+uint64_t cp_bytes_in_use(const cp_ctx* c) { return c ? c->in_use : 0; }
+
+cp_status cp_geometry_upload(cp_ctx* c, const cp_mesh* m, uint64_t key, cp_geom** out) {
+    if (!m || !out || !m->xyz || !m->tri || !m->ids || m->n_nodes <= 0 || m->n_tris <= 0) return CP_BAD_ARG;
+    return guarded(c, [&] { return build_geom(c, m, key, out); });
+}
+
+cp_status cp_geometry_from_tri(cp_ctx* c, const char* path, uint64_t key, cp_geom** out) {
+    if (!path || !out) return CP_BAD_ARG;
+    std::vector<double> xyz;
+    std::vector<int32_t> tri, ids;
+    std::string err;
+    if (!read_tri_file(path, xyz, tri, ids, err)) return fail(c, CP_BAD_MESH, err);
+    cp_mesh m{(int32_t)(xyz.size() / 3), xyz.data(), (int32_t)ids.size(), tri.data(), ids.data()};
+    return cp_geometry_upload(c, &m, key, out);
+}
+
+void cp_geometry_release(cp_geom* g) {
+    if (!g) return;
+    cp_ctx* c = g->ctx;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (c->device >= 0) {
+        (void)hipSetDevice(c->device);
+        (void)hipStreamSynchronize(c->stream);
+    }
+    delete g;
+}
+
+cp_status cp_geometry_info(const cp_geom* g, cp_geom_info* info) {
+    if (!g || !info) return CP_BAD_ARG;
+    info->n_body = g->dev.n;
+    info->n_wake = g->dev.nw;
+    int nl = 0;
+    for (auto& w : g->host.wakes) nl += (int)w.lines.size();
+    info->n_wakelines = nl;
+    info->n_nodes = g->host.n_nodes;
+    info->lda = g->dev.lda;
+    return CP_OK;
+}
+
+cp_status cp_geometry_body(const cp_geom* g, int32_t* tri_index, double* center, double* normal, double* area) {
+    if (!g) return CP_BAD_ARG;
+    const Geometry& G = g->host;
+    for (size_t i = 0; i < G.body.size(); i++) {
+        if (tri_index) tri_index[i] = G.body_tri[i];
+        if (center) {
+            center[3 * i] = G.body[i].center.x;
+            center[3 * i + 1] = G.body[i].center.y;
+            center[3 * i + 2] = G.body[i].center.z;
+        }
+        if (normal) {
+            normal[3 * i] = G.body[i].normal.x;
+            normal[3 * i + 1] = G.body[i].normal.y;
+            normal[3 * i + 2] = G.body[i].normal.z;
+        }
+        if (area) area[i] = G.body[i].area;
+    }
+    return CP_OK;
+}
+
+cp_status cp_geometry_wake(const cp_geom* g, int32_t* tri_index, int32_t* idx4, double* coef) {
+    if (!g) return CP_BAD_ARG;
+    const Geometry& G = g->host;
+    for (size_t j = 0; j < G.wake.size(); j++) {
+        if (tri_index) tri_index[j] = G.wake_tri[j];
+        if (idx4)
+            for (int s = 0; s < 4; s++) idx4[4 * j + s] = G.fold_idx[4 * j + s];
+        if (coef) coef[j] = G.fold_coef[j];
+    }
+    return CP_OK;
+}
+
+cp_status cp_geometry_clusters(const cp_geom* g, int32_t* cl16, int32_t* ts_order, int32_t* tip) {
+    if (!g) return CP_BAD_ARG;
+    const Geometry& G = g->host;
+    for (size_t i = 0; i < G.body.size(); i++) {
+        if (cl16)
+            for (int k = 0; k < 16; k++) cl16[16 * i + k] = G.cluster[16 * i + k];
+        if (ts_order) ts_order[i] = G.ts_order[i];
+        if (tip) tip[i] = G.tip[i];
+    }
+    return CP_OK;
+}
+
+cp_status cp_assemble(cp_geom* g, cp_prec prec) {
+    if (!g || (prec != CP_FP64 && prec != CP_FP32_REFINE)) return CP_BAD_ARG;
+    cp_ctx* c = g->ctx;
+    if (c->device < 0) return fail(c, CP_NO_DEVICE, "host-only context");
+    return guarded(c, [&] {
+        ensure_wake_influence(g);
+        const int64_t lda = g->dev.lda;
+        const size_t elems = (size_t)lda * g->dev.n;
+        const double* W = g->W.as<double>();
+        if (prec == CP_FP64) {
+            if (!g->have64) {
+                dalloc(c, g->A64, elems * sizeof(double));
+                dalloc(c, g->B64, elems * sizeof(double));
+            }
+            {
+                TimedScope ts("assemble_f64", c->stream);
+                launch_assemble_f64(g->dev, W, lda, g->A64.as<double>(), g->B64.as<double>(), c->stream);
+            }
+            g->have64 = true;
+        } else {
+            if (!g->have32) {
+                dalloc(c, g->A32, elems * sizeof(float));
+                dalloc(c, g->B32, elems * sizeof(float));
+            }
+            {
+                TimedScope ts("assemble_f32", c->stream);
+                launch_assemble_f32(g->dev, W, lda, g->A32.as<float>(), g->B32.as<float>(), c->stream);
+            }
+            g->have32 = true;
+        }
+        CP_HIP(hipStreamSynchronize(c->stream));
+        return CP_OK;
+    });
+}
+
+cp_status cp_read_matrices(cp_geom* g, cp_prec prec, double* A, double* B, int64_t ld) {
+    if (!g || ld < g->dev.n) return CP_BAD_ARG;
+    cp_ctx* c = g->ctx;
+    if (c->device < 0) return fail(c, CP_NO_DEVICE, "host-only context");
+    return guarded(c, [&] {
+        const int n = g->dev.n;
+        const int64_t lda = g->dev.lda;
+        if (prec == CP_FP64) {
+            if (!g->have64) return fail(c, CP_NOT_ASSEMBLED, "FP64 matrices not assembled");
+            if (A)
+                CP_HIP(hipMemcpy2D(A, ld * sizeof(double), g->A64.p, lda * sizeof(double), n * sizeof(double), n,
+                                   hipMemcpyDeviceToHost));
+            if (B)
+                CP_HIP(hipMemcpy2D(B, ld * sizeof(double), g->B64.p, lda * sizeof(double), n * sizeof(double), n,
+                                   hipMemcpyDeviceToHost));
+        } else {
+            if (!g->have32) return fail(c, CP_NOT_ASSEMBLED, "FP32 matrices not assembled");
+            std::vector<float> tmp((size_t)n * n);
+            const void* src[2] = {g->A32.p, g->B32.p};
+            double* dst[2] = {A, B};
+            for (int q = 0; q < 2; q++) {
+                if (!dst[q]) continue;
+                CP_HIP(hipMemcpy2D(tmp.data(), n * sizeof(float), src[q], lda * sizeof(float), n * sizeof(float), n,
+                                   hipMemcpyDeviceToHost));
+                for (int j = 0; j < n; j++)
+                    for (int i = 0; i < n; i++) dst[q][i + (int64_t)j * ld] = tmp[i + (size_t)j * n];
+            }
+        }
+        return CP_OK;
+    });
+}
+
+cp_status cp_solve(cp_geom* g, cp_prec prec, int32_t k, const cp_freestream* fs, const cp_solve_opts* opts,
+                   double* mu, double* cpout, cp_solve_info* info) {
+    if (!g || !fs || !opts || k <= 0) return CP_BAD_ARG;
+    if (prec != CP_FP64 && prec != CP_FP32_REFINE) return CP_BAD_ARG;
+    cp_ctx* c = g->ctx;
+    if (c->device < 0) return fail(c, CP_NO_DEVICE, "host-only context");
+    return guarded(c, [&]() -> cp_status {
+        if (prec == CP_FP64 && !g->have64) return fail(c, CP_NOT_ASSEMBLED, "call cp_assemble(CP_FP64) first");
+        if (prec == CP_FP32_REFINE && !g->have32)
+            return fail(c, CP_NOT_ASSEMBLED, "call cp_assemble(CP_FP32_REFINE) first");
+        ensure_post(g);
+        const int n = g->dev.n;
+        const int64_t lda = g->dev.lda;
+        hipStream_t s = c->stream;
+        const double tol = opts->tol > 0 ? opts->tol : 1e-10;
+        std::vector<double> T(9 * (size_t)k), vinf4(4 * (size_t)k);
+        for (int q = 0; q < k; q++) {
+            double Vb[3];
+            wind_to_body(fs[q], &T[9 * (size_t)q], Vb);
+            vinf4[4 * q] = Vb[0];
+            vinf4[4 * q + 1] = Vb[1];
+            vinf4[4 * q + 2] = Vb[2];
+            vinf4[4 * q + 3] = fs[q].V;
+        }
+        DevBuf dv4, dmu, dcp, dpart, dres;
+        upload(c, dv4, vinf4);
+        dalloc(c, dmu, (size_t)lda * k * sizeof(double));
+        std::vector<SolveStats> stats(k);
+        if (prec == CP_FP64) {
+            if (!g->bicg) g->bicg.reset(new Bicgstab64(n, lda));
+            DevBuf dsig, drhs;
+            dalloc(c, dsig, (size_t)lda * sizeof(double));
+            dalloc(c, drhs, (size_t)lda * sizeof(double));
+            for (int q = 0; q < k; q++) {
+                // sigma_j = -Vinf . n_j  (bodyPanel::setSigma, bodyPanel.cpp:97-100)
+                launch_sigma(n, lda, g->normals.as<double>(), dv4.as<double>() + 4 * q, dsig.as<double>(), s);
+                // RHS = -B sigma
+                g->bicg->gemv().apply(g->B64.as<double>(), dsig.as<double>(), drhs.as<double>(), -1.0,
+                                      g->bicg->gemv_scratch(), s);
+                stats[q] = g->bicg->solve(g->A64.as<double>(), drhs.as<double>(), dmu.as<double>() + lda * q, tol,
+                                          opts->max_iter, s);
+            }
+        } else {
+            if (!g->batched || g->batched->capacity() < k) g->batched.reset(new BatchedSolver(n, lda, std::max(k, 1)));
+            BatchedOpts bo;
+            bo.tol = tol;
+            bo.max_iter = opts->max_iter > 0 ? opts->max_iter : 2 * n;
+            bo.max_refine = opts->max_refine > 0 ? opts->max_refine : 10;
+            bo.inner_tol = opts->inner_tol > 0 ? opts->inner_tol : 1e-5;
+            g->batched->solve(g->A32.as<float>(), g->B32.as<float>(), g->normals.as<double>(), dv4.as<double>(), k,
+                              bo, dmu.as<double>(), lda, stats.data(), s);
+        }
+        int nblk = post_partial_blocks(n);
+        dalloc(c, dpart, (size_t)nblk * k * 6 * sizeof(double));
+        dalloc(c, dres, (size_t)16 * k * sizeof(double));
+        if (cpout) dalloc(c, dcp, (size_t)lda * k * sizeof(double));
+        PostParams pp;
+        pp.PG = std::sqrt(1 - opts->mach * opts->mach);
+        pp.Sref = opts->Sref;
+        pp.bref = opts->bref;
+        pp.cref = opts->cref;
+        for (int a = 0; a < 3; a++) pp.cg[a] = opts->cg[a];
+        launch_post(g->post, pp, k, dmu.as<double>(), lda, dv4.as<double>(), cpout ? dcp.as<double>() : nullptr, lda,
+                    dpart.as<double>(), nblk, dres.as<double>(), s);
+        std::vector<double> res(16 * (size_t)k);
+        CP_HIP(hipMemcpyAsync(res.data(), dres.p, res.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+        if (mu)
+            CP_HIP(hipMemcpy2DAsync(mu, n * sizeof(double), dmu.p, lda * sizeof(double), n * sizeof(double), k,
+                                    hipMemcpyDeviceToHost, s));
+        if (cpout)
+            CP_HIP(hipMemcpy2DAsync(cpout, n * sizeof(double), dcp.p, lda * sizeof(double), n * sizeof(double), k,
+                                    hipMemcpyDeviceToHost, s));
+        CP_HIP(hipStreamSynchronize(s));
+        bool all_conv = true;
+        for (int q = 0; q < k; q++) {
+            if (info) {
+                std::memset(&info[q], 0, sizeof(cp_solve_info));
+                fill_info(&res[16 * (size_t)q], &T[9 * (size_t)q], info[q]);
+                info[q].rel_residual = stats[q].rel_residual;
+                info[q].iters = stats[q].iters;
+                info[q].refinements = stats[q].refinements;
+                info[q].converged = stats[q].rel_residual <= 1e-10;
+            }
+            all_conv = all_conv && stats[q].rel_residual <= 1e-10;
+        }
+        return all_conv ? CP_OK : CP_NOT_CONVERGED;
+    });
+}
+
+cp_status cp_timing_reset(cp_ctx* c) {
+    if (!c) return CP_BAD_ARG;
+    return guarded(c, [&] {
+        c->timer.reset();
+        return CP_OK;
+    });
+}
+
+cp_status cp_timing_enable(cp_ctx* c, int on) {
+    if (!c) return CP_BAD_ARG;
+    c->timer.enabled = on != 0;
+    return CP_OK;
+}
+
+cp_status cp_timing_get(cp_ctx* c, const char* kernel, double* total_ms, int64_t* launches) {
+    if (!c || !kernel) return CP_BAD_ARG;
+    return guarded(c, [&] {
+        double ms = 0;
+        long long nl = 0;
+        c->timer.collect();
+        c->timer.get(kernel, ms, nl);
+        if (total_ms) *total_ms = ms;
+        if (launches) *launches = nl;
+        return CP_OK;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,51 @@
+// Common device-side declarations: error checking, launch helpers and the
+// per-geometry device tables.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "influence.h"
+
+namespace cpgpu {
+
+// Error carried out of a HIP call; converted to cp_status at the C ABI.
+struct HipError {
+    hipError_t code;
+    std::string where;
+};
+
+#define CP_HIP(call)                                                             \
+    do {                                                                         \
+        hipError_t e_ = (call);                                                  \
+        if (e_ != hipSuccess) throw ::cpgpu::HipError{e_, std::string(#call)};   \
+    } while (0)
+
+#define CP_LAUNCH_CHECK() CP_HIP(hipGetLastError())
+
+inline int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
+
+// Device tables of one geometry (all owned by the geometry object).
+struct DevGeom {
+    int n = 0;        // body panels
+    int nw = 0;       // wake panels (first row)
+    int64_t lda = 0;  // leading dimension of A, B (rows padded to 256)
+    PanelRec* body = nullptr;     // n records
+    PanelRec* wake = nullptr;     // nw records
+    double* cx = nullptr;         // collocation points (= body centres), lda each
+    double* cy = nullptr;
+    double* cz = nullptr;
+    // Kutta folding by body column: for column c, entries [ptr[c], ptr[c+1])
+    int* fold_ptr = nullptr;
+    int* fold_w = nullptr;
+    double* fold_c = nullptr;
+};
+
+// ---- kernel launchers (assemble.hip) ----
+void launch_wake_influence(const DevGeom& g, double* W, int64_t ldw, hipStream_t s);
+void launch_assemble_f64(const DevGeom& g, const double* W, int64_t ldw, double* A, double* B, hipStream_t s);
+void launch_assemble_f32(const DevGeom& g, const double* W, int64_t ldw, float* A, float* B, hipStream_t s);
+
+}  // namespace cpgpu

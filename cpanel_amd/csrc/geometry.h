@@ -1,0 +1,75 @@
+// Host-side panel geometry and topology for the MI355X solve path.
+//
+// Replaces the topology half of geometry::readTri (geometry.cpp:132-484) with
+// flat arrays: the device never sees pointers, only the resolved SoA tables
+// built here (panel order, Kutta folding table, CHTLS clusters, wake lines).
+#pragma once
+
+#include <array>
+#include <cstdint>
+#include <functional>
+#include <string>
+#include <vector>
+
+namespace cpgpu {
+
+struct Vec3 {
+    double x, y, z;
+};
+
+// One triangular or quadrilateral panel after setGeom (panel.cpp:33-104).
+struct PanelGeom {
+    int nv = 3;
+    int node[4] = {-1, -1, -1, -1};
+    Vec3 v[4];
+    Vec3 center, normal, bez;
+    double area = 0, long_side = 0;
+};
+
+struct WakeLineInfo {
+    int upper, lower;  // body panel indices (reference order)
+    Vec3 p1, p2;       // trailing-edge end points (wakeLine::setDimensions)
+    double y;          // pMid.y, the sort key (wake::addWakeLine)
+};
+
+struct WakeInfo {
+    std::vector<int> panels;        // indices into wake panel order
+    std::vector<WakeLineInfo> lines;
+    double yMin = 0, yMax = 0, x0 = 0, xf = 0, z0 = 0, zf = 0;
+};
+
+// Fully resolved geometry of one .tri mesh.
+struct Geometry {
+    int n_nodes = 0;
+    std::vector<Vec3> nodes;
+    std::vector<PanelGeom> body;   // reference body order (surfaces prepended)
+    std::vector<PanelGeom> wake;   // reference wake order
+    std::vector<int> body_tri, wake_tri;  // file triangle index
+    // Kutta folding table per wake panel (wakePanel::interpPanels)
+    std::vector<int> fold_idx;     // 4 per wake panel: U1, L1, U2, L2
+    std::vector<double> fold_coef; // interpCoeff t
+    // CHTLS clusters (bodyPanel::setCluster), 16 entries, -1 padded
+    std::vector<int> cluster;
+    std::vector<int> ts_order;
+    std::vector<char> tip, upper, lower;
+    // trailing-edge flags of the upper TE panel edges needed by pntInWake
+    std::vector<WakeInfo> wakes;
+    // per wake panel: TE flag and upper body panel (for wake::pntInWake)
+    std::vector<char> wake_te;
+    std::vector<int> wake_upper;
+    // per body panel: edges flagged TE (node pairs), up to 3
+    std::vector<int> body_te_edges;  // 6 per body panel (n1,n2)*3, -1 if not TE
+};
+
+// Builds the geometry from 0-based connectivity.  Returns false with a
+// message for meshes the reference would reject or mis-handle.
+bool build_geometry(int n_nodes, const double* xyz, int n_tris, const int32_t* tri, const int32_t* ids,
+                    Geometry& out, std::string& err);
+
+bool read_tri_file(const std::string& path, std::vector<double>& xyz, std::vector<int32_t>& tri,
+                   std::vector<int32_t>& ids, std::string& err);
+
+// Simple parallel loop over [0, n) on the host using std::thread.
+void parallel_for(int n, int nthreads, const std::function<void(int, int)>& body);
+
+}  // namespace cpgpu

@@ -1,0 +1,162 @@
+// Panel influence kernels shared by the gfx950 kernels and the host
+// precompute (Trefftz functionals, CHTLS).  FP64 near-field functions keep the
+// reference's operation order with FMA contraction disabled so that GPU and
+// CPU agree to the last bits where the formula is well conditioned.
+//
+// Reference formulas: panel::vortexPhi (panel.cpp:713-735),
+// bodyPanel::srcSidePhi (bodyPanel.cpp:361-373), bodyPanel::panelPhiInf
+// (bodyPanel.cpp:132-190), panel::dubPhiInf (panel.cpp:208-252),
+// panel::vortexV (panel.cpp:707-711), panel::pntDubV (panel.cpp:773-776),
+// bodyPanel::panelVInf / srcSideV (bodyPanel.cpp:305-348, 375-390).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+#define CP_HD __host__ __device__ __forceinline__
+
+namespace cpgpu {
+
+constexpr double kPI = 3.14159265358979323846;
+
+// One panel as the kernels read it: a 256-byte record.  Within a wavefront
+// every lane reads the same record, so the loads go through the scalar cache.
+struct alignas(256) PanelRec {
+    double v[12];        // vertices x,y,z (nv of them)
+    double c[3];         // centre
+    double n[3];         // unit normal
+    double l[3];         // local x: unit(vertex0 - centre) (panel::getLocalSys)
+    double m[3];         // local y: n x l
+    double area;
+    double long_side;
+    double far2_lo;      // 25*L^2*(1-1e-13): below -> near field for sure
+    double far2_hi;      // 25*L^2*(1+1e-13): above -> far field for sure
+    double area_4pi;     // area / (4 pi)
+    int32_t nv;
+    int32_t pad0;
+};
+static_assert(sizeof(PanelRec) == 256, "PanelRec must be 256 bytes");
+
+// far-field test of the reference: |pjk| / longSide > 5, decided exactly
+// from r^2 except inside a relative band of 1e-13 around the threshold.
+CP_HD bool is_far(const PanelRec& P, double r2) {
+#pragma clang fp contract(off)
+    if (r2 > P.far2_hi) return true;
+    if (r2 < P.far2_lo) return false;
+    return sqrt(r2) / P.long_side > 5;
+}
+
+CP_HD double sq_norm3(double x, double y, double z) {
+#pragma clang fp contract(off)
+    return x * x + y * y + z * z;
+}
+
+// panel::vortexPhi, reference order of operations
+CP_HD double vortex_phi(double PN, double Al, const double* a, const double* s, const double* l, const double* m,
+                        double na, double nb) {
+#pragma clang fp contract(off)
+    double axs0 = a[1] * s[2] - a[2] * s[1];
+    double axs1 = a[2] * s[0] - a[0] * s[2];
+    double axs2 = a[0] * s[1] - a[1] * s[0];
+    double q0 = l[1] * axs2 - l[2] * axs1;
+    double q1 = l[2] * axs0 - l[0] * axs2;
+    double q2 = l[0] * axs1 - l[1] * axs0;
+    double PA = a[0] * q0 + a[1] * q1 + a[2] * q2;
+    double sm = s[0] * m[0] + s[1] * m[1] + s[2] * m[2];
+    double PB = PA - Al * sm;
+    double num = sm * PN * (nb * PA - na * PB);
+    double den = PA * PB + PN * PN * na * nb * (sm * sm);
+    if (den == 0 && fabs(PN) < 1e-15) return PN >= 0 ? 0.5 * kPI : -0.5 * kPI;
+    return atan2(num, den);
+}
+
+// Near-field (edge formula) source and doublet potential of a panel at a
+// point; the caller has already excluded the far field.  itself: the point
+// is the panel centre (|pjk| < 1e-10), doublet -> -0.5.
+CP_HD void near_phi_f64(const PanelRec& P, double px, double py, double pz, double& src, double& dub) {
+#pragma clang fp contract(off)
+    double pjk0 = px - P.c[0], pjk1 = py - P.c[1], pjk2 = pz - P.c[2];
+    double r = sqrt(sq_norm3(pjk0, pjk1, pjk2));
+    bool itself = r < 1e-10;
+    double PN = pjk0 * P.n[0] + pjk1 * P.n[1] + pjk2 * P.n[2];
+    const int nv = P.nv;
+    double d[4][3], nd[4];
+    for (int k = 0; k < nv; k++) {
+        d[k][0] = px - P.v[3 * k];
+        d[k][1] = py - P.v[3 * k + 1];
+        d[k][2] = pz - P.v[3 * k + 2];
+        nd[k] = sqrt(sq_norm3(d[k][0], d[k][1], d[k][2]));
+    }
+    double s_src = 0, s_dub = 0;
+    for (int k = 0; k < nv; k++) {
+        int k2 = (k + 1 == nv) ? 0 : k + 1;
+        const double* a = d[k];
+        double s[3] = {P.v[3 * k2] - P.v[3 * k], P.v[3 * k2 + 1] - P.v[3 * k + 1], P.v[3 * k2 + 2] - P.v[3 * k + 2]};
+        double sxa0 = s[1] * a[2] - s[2] * a[1];
+        double sxa1 = s[2] * a[0] - s[0] * a[2];
+        double sxa2 = s[0] * a[1] - s[1] * a[0];
+        double Al = P.n[0] * sxa0 + P.n[1] * sxa1 + P.n[2] * sxa2;
+        double phiV = 0;
+        if (!itself) {
+            phiV = vortex_phi(PN, Al, a, s, P.l, P.m, nd[k], nd[k2]);
+            s_dub += phiV;
+        }
+        double A = nd[k], B = nd[k2], S = sqrt(sq_norm3(s[0], s[1], s[2]));
+        double GL = 0;
+        if (fabs(A + B - S) > 0 && S > 0) GL = 1 / S * log(fabs((A + B + S) / (A + B - S)));
+        s_src += Al * GL - PN * phiV;
+    }
+    src = s_src / (4 * kPI);
+    dub = itself ? -0.5 : s_dub / (4 * kPI);
+}
+
+// bodyPanel::panelPhiInf in FP64 (far field via the point formulas).
+CP_HD void panel_phi_inf_f64(const PanelRec& P, double px, double py, double pz, double& src, double& dub) {
+#pragma clang fp contract(off)
+    double d0 = px - P.c[0], d1 = py - P.c[1], d2 = pz - P.c[2];
+    double r2 = sq_norm3(d0, d1, d2);
+    if (is_far(P, r2)) {
+        double r = sqrt(r2);
+        double PN = d0 * P.n[0] + d1 * P.n[1] + d2 * P.n[2];
+        src = P.area / (4 * kPI * r);
+        dub = PN * P.area / (4 * kPI * (r * r * r));
+        return;
+    }
+    near_phi_f64(P, px, py, pz, src, dub);
+}
+
+// panel::dubPhiInf in FP64: self (|pjk| < 1e-7) -> -0.5; far -> point
+// doublet; else the edge sum of vortexPhi.
+CP_HD double dub_phi_inf_f64(const PanelRec& P, double px, double py, double pz) {
+#pragma clang fp contract(off)
+    double d0 = px - P.c[0], d1 = py - P.c[1], d2 = pz - P.c[2];
+    double r2 = sq_norm3(d0, d1, d2);
+    double r = sqrt(r2);
+    double PN = d0 * P.n[0] + d1 * P.n[1] + d2 * P.n[2];
+    if (r < 0.0000001) return -0.5;
+    if (is_far(P, r2)) return PN * P.area / (4 * kPI * (r * r * r));
+    const int nv = P.nv;
+    double d[4][3], nd[4];
+    for (int k = 0; k < nv; k++) {
+        d[k][0] = px - P.v[3 * k];
+        d[k][1] = py - P.v[3 * k + 1];
+        d[k][2] = pz - P.v[3 * k + 2];
+        nd[k] = sqrt(sq_norm3(d[k][0], d[k][1], d[k][2]));
+    }
+    double phi = 0;
+    for (int k = 0; k < nv; k++) {
+        int k2 = (k + 1 == nv) ? 0 : k + 1;
+        double s[3] = {P.v[3 * k2] - P.v[3 * k], P.v[3 * k2 + 1] - P.v[3 * k + 1], P.v[3 * k2 + 2] - P.v[3 * k + 2]};
+        const double* a = d[k];
+        double sxa0 = s[1] * a[2] - s[2] * a[1];
+        double sxa1 = s[2] * a[0] - s[0] * a[2];
+        double sxa2 = s[0] * a[1] - s[1] * a[0];
+        double Al = P.n[0] * sxa0 + P.n[1] * sxa1 + P.n[2] * sxa2;
+        phi += vortex_phi(PN, Al, a, s, P.l, P.m, nd[k], nd[k2]);
+    }
+    return phi / (4 * kPI);
+}
+
+}  // namespace cpgpu

@@ -1,0 +1,170 @@
+// Post-solve outputs on the device for k conditions at once:
+//   phi_i = Vinf . c_i - mu_i               (panel::setPotential, panel.cpp:106-111)
+//   V_i   = CHTLS weighted sum over the cluster, NaN fallbacks, V_x /= PG
+//                                            (bodyPanel::pntVelocity, bodyPanel.cpp:490-524)
+//   Cp_i  = 1 - (|V_i| / V)^2               (bodyPanel::computeCp, :591-594)
+//   F, CM sums                               (cpCase::compVelocity, cpCase.cpp:342-365)
+//   Trefftz CL, CDi                          (wake::trefftzPlane, wake.cpp:146-188)
+// Sums use a fixed reduction tree, so a condition's outputs do not depend on
+// which other conditions share the launch.
+#include "device.h"
+#include "post.h"
+
+namespace cpgpu {
+
+namespace {
+
+constexpr int kT = 256;
+constexpr int kNP = 6;  // partial sums per block: F[3], M[3]
+
+__device__ double block_sum_post(double v, double* sh) {
+    int t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (int o = kT / 2; o > 0; o >>= 1) {
+        if (t < o) sh[t] += sh[t + o];
+        __syncthreads();
+    }
+    double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+__global__ __launch_bounds__(256) void post_panel_kernel(
+    int n, const int32_t* __restrict__ nbr, const double* __restrict__ w1, const double* __restrict__ w2,
+    const int32_t* __restrict__ mode, const double* __restrict__ ctr, const double* __restrict__ bez,
+    const double* __restrict__ area, PostParams pp, const double* __restrict__ mu, int64_t ldmu,
+    const double* __restrict__ vinf4, double* __restrict__ cp, int64_t ldcp, double* __restrict__ partials) {
+    __shared__ double sh[kT];
+    const int c = blockIdx.y;
+    const int i = blockIdx.x * kT + threadIdx.x;
+    const double Vx = vinf4[4 * c], Vy = vinf4[4 * c + 1], Vz = vinf4[4 * c + 2], Vm = vinf4[4 * c + 3];
+    const double* m = mu + (int64_t)c * ldmu;
+    const double* cx = ctr;
+    const double* cy = ctr + n;
+    const double* cz = ctr + 2 * (int64_t)n;
+    double f[kNP] = {0, 0, 0, 0, 0, 0};
+    if (i < n) {
+        double phi_i = Vx * cx[i] + Vy * cy[i] + Vz * cz[i] - m[i];
+        const int32_t* nb = nbr + 16 * (int64_t)i;
+        int cnt = nb[0];
+        double df[kClusterMax];
+        for (int k = 0; k < cnt; k++) {
+            int j = nb[1 + k];
+            df[k] = (Vx * cx[j] + Vy * cy[j] + Vz * cz[j] - m[j]) - phi_i;
+        }
+        double v0 = 0, v1 = 0, v2 = 0;
+        const double* w = w1 + 45 * (int64_t)i;
+        for (int k = 0; k < cnt; k++) {
+            v0 += w[3 * k] * df[k];
+            v1 += w[3 * k + 1] * df[k];
+            v2 += w[3 * k + 2] * df[k];
+        }
+        bool bad = (v0 != v0) || (v1 != v1) || (v2 != v2);
+        if (bad && mode[i] == 1) {
+            const double* ww = w2 + 45 * (int64_t)i;
+            v0 = v1 = v2 = 0;
+            for (int k = 0; k < cnt; k++) {
+                v0 += ww[3 * k] * df[k];
+                v1 += ww[3 * k + 1] * df[k];
+                v2 += ww[3 * k + 2] * df[k];
+            }
+            bad = (v0 != v0) || (v1 != v1) || (v2 != v2);
+        }
+        const double bx = bez[i], by = bez[n + i], bz = bez[2 * (int64_t)n + i];
+        if (bad) {
+            // (bez x Vinf) x bez   (bodyPanel.cpp:516-520)
+            double q0 = by * Vz - bz * Vy, q1 = bz * Vx - bx * Vz, q2 = bx * Vy - by * Vx;
+            v0 = q1 * bz - q2 * by;
+            v1 = q2 * bx - q0 * bz;
+            v2 = q0 * by - q1 * bx;
+        }
+        v0 /= pp.PG;
+        double vn = sqrt(v0 * v0 + v1 * v1 + v2 * v2) / Vm;
+        double Cp = 1 - vn * vn;
+        if (cp) cp[(int64_t)c * ldcp + i] = Cp;
+        double a = area[i];
+        double F0 = -Cp * a * bx, F1 = -Cp * a * by, F2 = -Cp * a * bz;
+        f[0] = F0 / pp.Sref;
+        f[1] = F1 / pp.Sref;
+        f[2] = F2 / pp.Sref;
+        double r0 = cx[i] - pp.cg[0], r1 = cy[i] - pp.cg[1], r2 = cz[i] - pp.cg[2];
+        f[3] = (r1 * F2 - r2 * F1) / (pp.Sref * pp.bref);
+        f[4] = (r2 * F0 - r0 * F2) / (pp.Sref * pp.cref);
+        f[5] = (r0 * F1 - r1 * F0) / (pp.Sref * pp.bref);
+    }
+    for (int q = 0; q < kNP; q++) {
+        double s = block_sum_post(f[q], sh);
+        if (threadIdx.x == 0) partials[((int64_t)c * gridDim.x + blockIdx.x) * kNP + q] = s;
+    }
+}
+
+// one block per condition: force sums and the Trefftz plane
+__global__ __launch_bounds__(256) void post_final_kernel(
+    int nblk, const double* __restrict__ partials, int n_wakes, int n_st, int n_te,
+    const int32_t* __restrict__ te_idx, const double* __restrict__ rphi, const double* __restrict__ rv,
+    const double* __restrict__ rw, const double* __restrict__ simpson, PostParams pp,
+    const double* __restrict__ mu, int64_t ldmu, const double* __restrict__ vinf4, double* __restrict__ res) {
+    __shared__ double sh[kT];
+    const int c = blockIdx.x;
+    const double* m = mu + (int64_t)c * ldmu;
+    const double Vm = vinf4[4 * c + 3];
+    double out[kNP];
+    for (int q = 0; q < kNP; q++) {
+        double s = 0;
+        for (int b = threadIdx.x; b < nblk; b += kT) s += partials[((int64_t)c * nblk + b) * kNP + q];
+        out[q] = block_sum_post(s, sh);
+    }
+    double CL = 0, CD = 0;
+    for (int w = 0; w < n_wakes; w++) {
+        double cl = 0, cd = 0;
+        for (int st = threadIdx.x; st < n_st; st += kT) {
+            const int64_t row = ((int64_t)w * n_st + st) * n_te;
+            double dphi = 0, v = 0, ww = 0;
+            for (int k = 0; k < n_te; k++) {
+                double mk = m[te_idx[k]];
+                dphi += rphi[row + k] * mk;
+                v += rv[row + k] * mk;
+                ww += rw[row + k] * mk;
+            }
+            double sw = simpson[(int64_t)w * n_st + st];
+            double Cl = 2 * dphi / (Vm * pp.Sref);
+            double Cd = dphi * sqrt(v * v + ww * ww) / (Vm * Vm * pp.Sref);
+            cl += sw * Cl;
+            cd += sw * Cd;
+        }
+        cl = block_sum_post(cl, sh);
+        cd = block_sum_post(cd, sh);
+        CL += cl / pp.PG;
+        CD += cd / (pp.PG * pp.PG);
+    }
+    if (threadIdx.x == 0) {
+        double* r = res + 16 * (int64_t)c;
+        r[0] = CL;
+        r[1] = CD;
+        r[2] = out[0];
+        r[3] = out[1];
+        r[4] = out[2];
+        r[8] = out[3];
+        r[9] = out[4];
+        r[10] = out[5];
+    }
+}
+
+}  // namespace
+
+int post_partial_blocks(int n) { return (n + kT - 1) / kT; }
+
+void launch_post(const PostDev& P, const PostParams& pp, int k, const double* mu, int64_t ldmu,
+                 const double* vinf4, double* cp, int64_t ldcp, double* partials, int nblk, double* res,
+                 hipStream_t s) {
+    dim3 g1((unsigned)nblk, (unsigned)k);
+    hipLaunchKernelGGL(post_panel_kernel, g1, dim3(kT), 0, s, P.n, P.nbr, P.w1, P.w2, P.mode, P.ctr, P.bez, P.area,
+                       pp, mu, ldmu, vinf4, cp, ldcp, partials);
+    CP_LAUNCH_CHECK();
+    hipLaunchKernelGGL(post_final_kernel, dim3((unsigned)k), dim3(kT), 0, s, nblk, partials, P.n_wakes, P.n_st,
+                       P.n_te, P.te_idx, P.rphi, P.rv, P.rw, P.simpson, pp, mu, ldmu, vinf4, res);
+    CP_LAUNCH_CHECK();
+}
+
+}  // namespace cpgpu

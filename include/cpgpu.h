@@ -1,0 +1,139 @@
+/*
+ * cpgpu — C ABI of the MI355X (gfx950) CPanel solve path.
+ *
+ * The reference (jakeddavis/CPanel) has no plugin API; the seams it exposes
+ * are C++ class methods.  Each entry point below replaces one of them:
+ *
+ *   cp_geometry_upload / cp_geometry_from_tri
+ *       geometry::geometry -> readTri topology     (CPanel/geometry.cpp:132-484)
+ *   cp_assemble
+ *       geometry::setInfCoeff, getA/getB           (CPanel/geometry.cpp:809-891,
+ *                                                   CPanel/geometry.h:310-312)
+ *   cp_solve
+ *       cpCase::run = setSourceStrengths + solveMatrixEq + compVelocity +
+ *       trefftzPlaneAnalysis                       (CPanel/cpCase.cpp:31-139,
+ *                                                   174-214, 342-365, 541-552)
+ *   cp_bh_* / cp_panel_to_points
+ *       particleFMM::build / barnesHutVel / barnesHutStretch,
+ *       bodyPanel::panelV / partStretching         (CPanel/particleFMM.h:52-56,
+ *                                                   CPanel/bodyPanel.cpp:122-130,697-729)
+ *
+ * Conventions: every call returns cp_status; nothing exits or prompts; host
+ * buffers are caller owned, device buffers context owned; matrices are
+ * column-major.  One context per GPU; a context is not reentrant (serialise
+ * calls on one context; use one context per host thread).
+ */
+#ifndef CPGPU_H
+#define CPGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    CP_OK = 0,
+    CP_NOT_CONVERGED = 1,   /* solve ran but ||r||/||b|| > 1e-10 (cpCase.cpp:199-202) */
+    CP_OUT_OF_BUDGET = 2,   /* request exceeds the context's HBM budget */
+    CP_BAD_MESH = 3,        /* topology the reference cannot handle */
+    CP_HIP_ERROR = 4,       /* a HIP runtime call failed; see cp_last_hip_error */
+    CP_BAD_ARG = 5,
+    CP_NO_DEVICE = 6,
+    CP_NOT_ASSEMBLED = 7
+} cp_status;
+
+typedef enum {
+    CP_FP64 = 0,            /* plain FP64 assembly and FP64 BiCGSTAB */
+    CP_FP32_REFINE = 1      /* FP32 A,B (FP64 near field), FP32 Krylov, FP64 refinement */
+} cp_prec;
+
+typedef struct cp_ctx cp_ctx;
+typedef struct cp_geom cp_geom;
+
+/* ---- context ------------------------------------------------------------ */
+/* device = -1 creates a host-only context: geometry/topology queries work,
+ * device work returns CP_NO_DEVICE.  hbm_budget_bytes = 0: no budget. */
+cp_status cp_create(int device, uint64_t hbm_budget_bytes, cp_ctx** out);
+void cp_destroy(cp_ctx* ctx);
+const char* cp_last_error(const cp_ctx* ctx);
+int cp_last_hip_error(const cp_ctx* ctx);
+const char* cp_status_string(cp_status s);
+/* bytes of device memory currently held by the context */
+uint64_t cp_bytes_in_use(const cp_ctx* ctx);
+
+/* ---- geometry (replaces geometry::readTri, geometry.cpp:132-484) --------- */
+typedef struct {
+    int32_t n_nodes;
+    const double* xyz;      /* 3*n_nodes */
+    int32_t n_tris;
+    const int32_t* tri;     /* 3*n_tris, 0-based node indices */
+    const int32_t* ids;     /* surface ID per triangle; >1000 = wake */
+} cp_mesh;
+
+typedef struct {
+    int32_t n_body;         /* body panels (rows/cols of A, B) */
+    int32_t n_wake;         /* prescribed wake panels */
+    int32_t n_wakelines;
+    int32_t n_nodes;
+    int64_t lda;            /* device leading dimension of A, B */
+} cp_geom_info;
+
+cp_status cp_geometry_upload(cp_ctx* ctx, const cp_mesh* mesh, uint64_t key, cp_geom** out);
+cp_status cp_geometry_from_tri(cp_ctx* ctx, const char* path, uint64_t key, cp_geom** out);
+void cp_geometry_release(cp_geom* g);
+cp_status cp_geometry_info(const cp_geom* g, cp_geom_info* info);
+/* body panels in solve order: file triangle index, centre[3], normal[3], area */
+cp_status cp_geometry_body(const cp_geom* g, int32_t* tri_index, double* center, double* normal, double* area);
+/* Kutta folding table (wakePanel::interpPanels, wakePanel.cpp:48-81) */
+cp_status cp_geometry_wake(const cp_geom* g, int32_t* tri_index, int32_t* idx4, double* coef);
+/* CHTLS clusters (bodyPanel::setCluster, bodyPanel.cpp:402-487), 16 per panel */
+cp_status cp_geometry_clusters(const cp_geom* g, int32_t* cluster16, int32_t* ts_order, int32_t* tip);
+
+/* ---- assembly (geometry::setInfCoeff, geometry.cpp:809-891) -------------- */
+cp_status cp_assemble(cp_geom* g, cp_prec prec);
+/* copy A and B (as FP64) to host arrays with leading dimension ld */
+cp_status cp_read_matrices(cp_geom* g, cp_prec prec, double* A, double* B, int64_t ld);
+
+/* ---- steady solve (cpCase::run, cpCase.cpp:31-139) ----------------------- */
+typedef struct {
+    double V, alpha_deg, beta_deg;     /* cpCase::windToBody (cpCase.cpp:142-166) */
+} cp_freestream;
+
+typedef struct {
+    double mach;            /* Prandtl-Glauert factor sqrt(1-M^2) (cpCase.h:116) */
+    double Sref, bref, cref;
+    double cg[3];
+    double tol;             /* target ||r||/||b|| (<=0: 1e-10) */
+    int32_t max_iter;       /* Krylov iterations per solve (<=0: 2N) */
+    int32_t max_refine;     /* FP64 refinement sweeps, CP_FP32_REFINE (<=0: 10) */
+    double inner_tol;       /* FP32 inner solve reduction per sweep (<=0: 1e-5) */
+} cp_solve_opts;
+
+typedef struct {
+    double CL, CDi;         /* Trefftz plane (cpCase.cpp:541-552) */
+    double Fbody[3], Fwind[3], CM[3];   /* surface integration (cpCase.cpp:342-365) */
+    double rel_residual;    /* final ||b - A mu|| / ||b|| in FP64 */
+    int32_t iters;          /* Krylov iterations */
+    int32_t refinements;    /* FP64 refinement sweeps */
+    int32_t converged;      /* rel_residual <= 1e-10 */
+    int32_t pad;
+} cp_solve_info;
+
+/* Solves k freestream conditions on one assembled geometry.  mu, cp: host
+ * arrays n_body*k (column c = condition c) or NULL; info: k entries. */
+cp_status cp_solve(cp_geom* g, cp_prec prec, int32_t k, const cp_freestream* fs, const cp_solve_opts* opts,
+                   double* mu, double* cp, cp_solve_info* info);
+
+/* ---- timing: HIP-event totals of the named kernel on the context stream -- */
+cp_status cp_timing_reset(cp_ctx* ctx);
+/* kernel: "assemble_f64", "assemble_f32", "gemv_f64", "gemm_f32", "resid_f64", ...
+ * returns total milliseconds and launch count since the last reset */
+cp_status cp_timing_get(cp_ctx* ctx, const char* kernel, double* total_ms, int64_t* launches);
+cp_status cp_timing_enable(cp_ctx* ctx, int on);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CPGPU_H */

@@ -1,0 +1,141 @@
+"""GPU parity of the steady path through the C ABI against the FP64 oracle.
+
+Tolerances (stated per quantity):
+  * FP64 assembly: |A - A_oracle| <= 1e-12 * max|A| entry-wise (same formulas,
+    FMA contraction off in the near field; differences come from atan2/log/pow
+    last-bit differences only);
+  * FP32 assembly: near-field entries are the FP64 value rounded once, far
+    field within 4e-7 relative;
+  * FP64 solve: ||mu - mu_oracle|| / ||mu_oracle|| <= 1e-8;
+  * mixed FP32 + FP64 refinement: |dCL| <= 1e-5, |dCp| <= 1e-4 against FP64.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from cpanel_amd import CP_FP32_REFINE, CP_FP64, meshgen
+from cpanel_amd.case import Context
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+P4412 = dict(mach=0.1, Sref=6.0, bref=6.0, cref=1.0, cg=(0.25, 0.0, 0.0))
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    say("[gpu] creating context on device 0")
+    return Context(0)
+
+
+def build_pair(ctx, mesh):
+    og = O.Geom.from_arrays(mesh["xyz"], mesh["tri"], mesh["ids"])
+    pg = ctx.geometry(mesh)
+    return og, pg
+
+
+@pytest.fixture(scope="module")
+def c1s(ctx):
+    m = meshgen.wing("4412", 1.0, 6.0, 40, 12, "uniform", 20.0)
+    og, pg = build_pair(ctx, m)
+    A, B = og.assemble()
+    return og, pg, A, B
+
+
+def test_assembly_fp64_matches_oracle(c1s):
+    og, pg, A, B = c1s
+    say("[gpu] assemble fp64 n=%d" % pg.nb)
+    pg.assemble(CP_FP64)
+    Ag, Bg = pg.matrices(CP_FP64)
+    sa, sb = np.abs(A).max(), np.abs(B).max()
+    assert np.abs(Ag - A).max() <= 1e-12 * sa
+    assert np.abs(Bg - B).max() <= 1e-12 * sb
+
+
+def test_assembly_fp32(c1s):
+    og, pg, A, B = c1s
+    say("[gpu] assemble fp32 n=%d" % pg.nb)
+    pg.assemble(CP_FP32_REFINE)
+    Ag, Bg = pg.matrices(CP_FP32_REFINE)
+    assert np.abs(Ag - A).max() <= 4e-7 * np.abs(A).max()
+    assert np.abs(Bg - B).max() <= 4e-7 * np.abs(B).max()
+    # entry-wise relative error of the far field (FP32 arithmetic)
+    rel = np.abs(Bg - B) / np.maximum(np.abs(B), 1e-30)
+    assert np.median(rel) < 1e-7
+
+
+def test_solve_fp64_matches_oracle(c1s):
+    og, pg, A, B = c1s
+    pg.assemble(CP_FP64)
+    conds = [(1.0, 5.0, 0.0), (1.0, -2.0, 3.0), (2.0, 8.0, -4.0)]
+    say("[gpu] solve fp64 k=%d" % len(conds))
+    res = pg.solve(conds, CP_FP64, tol=1e-12, **P4412)
+    for (V, a, b), r in zip(conds, res):
+        ref = og.run_case(A, B, V=V, alpha=a, beta=b, tol=1e-13, **P4412)
+        rel = np.linalg.norm(r["mu"] - ref["mu"]) / np.linalg.norm(ref["mu"])
+        assert rel <= 1e-8, rel
+        assert r["rel_residual"] <= 1e-12
+        # outputs on the same mu: post-processing parity
+        post = og.post(r["mu"], V=V, alpha=a, beta=b, **P4412)
+        assert abs(r["CL"] - post["CL"]) <= 1e-10
+        assert abs(r["CDi"] - post["CDi"]) <= 1e-10
+        assert np.abs(r["Fbody"] - post["Fbody"]).max() <= 1e-10
+        assert np.abs(r["Fwind"] - post["Fwind"]).max() <= 1e-10
+        assert np.abs(r["CM"] - post["CM"]).max() <= 1e-10
+        assert np.abs(r["cp"] - post["cp"]).max() <= 1e-9
+        # and end to end
+        assert abs(r["CL"] - ref["CL"]) <= 1e-8
+        assert abs(r["CDi"] - ref["CDi"]) <= 1e-8
+
+
+def test_4412_fixture_on_gpu(ctx, golden, fixtures_json):
+    say("[gpu] 4412 fixture")
+    g = ctx.geometry(os.path.join(golden, "4412.tri"))
+    g.assemble(CP_FP64)
+    fx = fixtures_json["4412"]
+    r = g.solve([(1.0, 5.0, 0.0)], CP_FP64, tol=1e-12, **P4412)[0]
+    tol = 0.5e-5 + 1e-12
+    assert abs(r["CL"] - fx["CL"]) <= tol
+    assert abs(r["CDi"] - fx["CDi"]) <= tol
+    assert abs(r["Fwind"][2] - fx["CL_surface"]) <= tol
+    assert abs(r["Fwind"][0] - fx["CD_surface"]) <= tol
+    assert abs(r["CM"][1] - fx["Cm"]) <= tol
+    s = np.load(os.path.join(golden, "4412_surface.npz"))
+    assert np.all(np.abs(r["mu"] - s["mu"]) <= 5e-6 * np.maximum(np.abs(s["mu"]), 0.1))
+    assert np.all(np.abs(r["cp"] - s["cp"]) <= 5e-6 * np.maximum(np.abs(s["cp"]), 0.1))
+
+
+def test_mixed_precision_against_fp64(c1s):
+    og, pg, A, B = c1s
+    pg.assemble(CP_FP64)
+    pg.assemble(CP_FP32_REFINE)
+    conds = [(1.0, a, b) for a in (-4.0, 2.0, 10.0) for b in (-5.0, 0.0, 5.0)]
+    say("[gpu] mixed k=%d" % len(conds))
+    r64 = pg.solve(conds, CP_FP64, tol=1e-12, **P4412)
+    r32 = pg.solve(conds, CP_FP32_REFINE, tol=1e-10, **P4412)
+    for a, b in zip(r64, r32):
+        assert b["rel_residual"] <= 1e-10
+        assert abs(a["CL"] - b["CL"]) <= 1e-5
+        assert abs(a["CDi"] - b["CDi"]) <= 1e-5
+        assert np.abs(a["cp"] - b["cp"]).max() <= 1e-4
+        assert np.linalg.norm(a["mu"] - b["mu"]) / np.linalg.norm(a["mu"]) <= 1e-5
+
+
+def test_batched_column_independent_of_batch(c1s):
+    og, pg, A, B = c1s
+    pg.assemble(CP_FP32_REFINE)
+    target = (1.0, 3.0, 1.5)
+    say("[gpu] batch independence")
+    alone = pg.solve([target], CP_FP32_REFINE, **P4412)[0]
+    mix = pg.solve([(1.0, -3.0, 2.0), target, (1.5, 9.0, -4.0), (0.7, 0.0, 0.0)], CP_FP32_REFINE, **P4412)[1]
+    assert np.array_equal(alone["mu"], mix["mu"])
+    assert alone["CL"] == mix["CL"]
+    again = pg.solve([target], CP_FP32_REFINE, **P4412)[0]
+    assert np.array_equal(alone["mu"], again["mu"])
