@@ -60,7 +60,7 @@ __global__ __launch_bounds__(256) void post_panel_kernel(
             v1 += w[3 * k + 1] * df[k];
             v2 += w[3 * k + 2] * df[k];
         }
-        bool bad = (v0 != v0) || (v1 != v1) || (v2 != v2);
+        bool bad = cnt == 0 || (v0 != v0) || (v1 != v1) || (v2 != v2);
         if (bad && mode[i] == 1) {
             const double* ww = w2 + 45 * (int64_t)i;
             v0 = v1 = v2 = 0;
@@ -69,7 +69,7 @@ __global__ __launch_bounds__(256) void post_panel_kernel(
                 v1 += ww[3 * k + 1] * df[k];
                 v2 += ww[3 * k + 2] * df[k];
             }
-            bad = (v0 != v0) || (v1 != v1) || (v2 != v2);
+            bad = cnt == 0 || (v0 != v0) || (v1 != v1) || (v2 != v2);
         }
         const double bx = bez[i], by = bez[n + i], bz = bez[2 * (int64_t)n + i];
         if (bad) {

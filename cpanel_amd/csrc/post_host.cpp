@@ -174,8 +174,18 @@ bool finite_all(const double* p, int n) {
     return true;
 }
 
+// A cluster without neighbours or with a zero-order fit has no derivative
+// weights (the reference indexes an empty matrix there); NaN weights route the
+// panel to the last-resort fallback velocity.
+bool degenerate_fit(const Geometry& G, int i, int nf, double* w) {
+    if (nf > 0 && G.ts_order[i] >= 1) return false;
+    for (int k = 0; k < 3 * std::max(nf, 1); k++) w[k] = std::numeric_limits<double>::quiet_NaN();
+    return true;
+}
+
 // 2D weights in the panel's local frame mapped back to global (velocity2D)
 void weights_2d(const Geometry& G, int i, const int* cl, int nf, double* w) {
+    if (degenerate_fit(G, i, nf, w)) return;
     const PanelGeom& p = G.body[i];
     PanelRec r = make_panel_rec(p);
     std::vector<double> Xf(2 * nf);
@@ -196,6 +206,7 @@ void weights_2d(const Geometry& G, int i, const int* cl, int nf, double* w) {
 }
 
 void weights_3d(const Geometry& G, int i, const int* cl, int nf, double* w) {
+    if (degenerate_fit(G, i, nf, w)) return;
     const PanelGeom& p = G.body[i];
     std::vector<double> Xf(3 * nf), Vb(3 * nf);
     for (int k = 0; k < nf; k++) {
@@ -258,7 +269,6 @@ void build_post_tables(const Geometry& G, PostHost& P) {
             for (int k = 0; k < cnt; k++) P.nbr[16 * (size_t)i + 1 + k] = idx[k];
             double* w1 = &P.w1[45 * (size_t)i];
             double* w2 = &P.w2[45 * (size_t)i];
-            if (cnt == 0) continue;
             if (G.tip[i]) {
                 weights_2d(G, i, idx, cnt, w1);
             } else {

@@ -4,7 +4,7 @@ A .tri file (read by geometry::readTri, CPanel/geometry.cpp:155-239) holds
 node coordinates, 1-based triangle connectivity and one surface ID per
 triangle; IDs above 1000 are wake surfaces.  Meshes built here have one body
 ID (1) and one wake ID (1001), outward normals, a sharp closed trailing edge
-shared by the upper and lower panels, tips closed by fan triangulation, and
+shared by the upper and lower panels, flat tip caps of npt-2 triangles, and
 a flat prescribed wake whose first node row coincides with the trailing edge
 (welded by the reader, geometry.cpp:631-667).
 """
@@ -54,9 +54,17 @@ def span_stations(span, n_strips, spacing="uniform"):
 
 
 def wing(code="4412", chord=1.0, span=6.0, n_chord=80, n_span=30, spacing="uniform", wake_len=20.0,
-         taper=1.0, sweep_deg=0.0, with_wake=True, wake_rows=1):
+         taper=1.0, sweep_deg=0.0, with_wake=True, wake_rows=1, noise=1e-8, seed=1234):
     """Returns dict(xyz (nn,3), tri (nt,3) 0-based, ids (nt,)).
-    n_chord = panels around the section (n_chord/2 per surface)."""
+    n_chord = panels around the section (n_chord/2 per surface).
+
+    noise: relative amplitude (times the chord) of a seeded perturbation of
+    the x and z coordinates of interior-station nodes, the level of the
+    digits CAD exporters write.  Without it every panel of an untwisted wing
+    has exactly coplanar neighbours and bodyPanel::setTipFlag
+    (bodyPanel.cpp:58-95, angle < 1e-8) marks the whole wing as tip patches.
+    The trailing-edge and leading-edge nodes, and the end stations (flat tip
+    caps), are left exact."""
     assert n_chord % 2 == 0
     lx, lz = naca4(code, n_chord // 2)
     npt = len(lx)
@@ -71,6 +79,15 @@ def wing(code="4412", chord=1.0, span=6.0, n_chord=80, n_span=30, spacing="unifo
         xyz[j * npt:(j + 1) * npt, 0] = xle + c * lx
         xyz[j * npt:(j + 1) * npt, 1] = y
         xyz[j * npt:(j + 1) * npt, 2] = c * lz
+    if noise > 0:
+        rng = np.random.default_rng(seed)
+        for j in range(1, nst - 1):
+            sl = slice(j * npt, (j + 1) * npt)
+            d = rng.uniform(-1.0, 1.0, size=(npt, 2)) * noise * chord
+            d[0] = 0.0            # trailing edge
+            d[npt // 2] = 0.0     # leading edge
+            xyz[sl, 0] += d[:, 0]
+            xyz[sl, 2] += d[:, 1]
     tris = []
     for j in range(nst - 1):
         for k in range(npt):
@@ -80,11 +97,22 @@ def wing(code="4412", chord=1.0, span=6.0, n_chord=80, n_span=30, spacing="unifo
             d = (j + 1) * npt + k
             tris.append((a, b, c_))
             tris.append((a, c_, d))
-    # tips: fan from the trailing-edge node, outward normals (-y at the left tip)
+    # tips: the section polygon (npt points) closed by npt-2 triangles laid
+    # strip-wise between the lower and upper surface points (a fan from one
+    # vertex would make the tip panels' centroids nearly collinear, which
+    # leaves the 2-D CHTLS fits of tip panels singular); outward normals
     for j, sgn in ((0, -1.0), (nst - 1, 1.0)):
         base = j * npt
-        for k in range(1, npt - 1):
-            t = (base, base + k, base + k + 1)
+        h = npt // 2
+        poly_tris = [(0, 1, npt - 1)]
+        for q in range(1, h - 1):
+            lo0, lo1 = q, q + 1
+            up0, up1 = npt - q, npt - q - 1
+            poly_tris.append((lo0, lo1, up0))
+            poly_tris.append((lo1, up1, up0))
+        poly_tris.append((h - 1, h, h + 1))
+        for a, b, c_ in poly_tris:
+            t = (base + a, base + b, base + c_)
             p = xyz[list(t)]
             nrm = np.cross(p[1] - p[0], p[2] - p[0])
             if nrm[1] * sgn < 0:

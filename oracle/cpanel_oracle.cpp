@@ -1067,6 +1067,11 @@ V3 velocity2d(const or_geom& G, const Panel& b, V3 pnt, double pntPot) {
         Xf[2 * i + 1] = c.y;
         df[i] = G.P[cl[i]].phi - pntPot;
     }
+    const double nan = std::numeric_limits<double>::quiet_NaN();
+    // no neighbours or a zero-order fit: the reference reads past the end of
+    // an empty Eigen matrix here (undefined); the oracle returns NaN so the
+    // caller's last-resort fallback applies
+    if (nf == 0 || b.TSorder < 1) return V3{nan, nan, nan};
     std::vector<double> F;
     int t;
     chtls(2, X0, nf, Xf.data(), b.TSorder, 0, nullptr, nullptr, F, t);
@@ -1095,6 +1100,10 @@ V3 velocity3d(const or_geom& G, const Panel& b, V3 pnt, double pntPot) {
         df[i] = c.phi - pntPot;
     }
     double X0[3] = {pnt.x, pnt.y, pnt.z};
+    if (nf == 0 || b.TSorder < 1) {  // see velocity2d
+        const double nan = std::numeric_limits<double>::quiet_NaN();
+        return V3{nan, nan, nan};
+    }
     std::vector<double> F;
     int t;
     chtls(3, X0, nf, Xf.data(), b.TSorder, nf, Xf.data(), Vb.data(), F, t);
