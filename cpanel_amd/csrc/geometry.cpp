@@ -21,6 +21,11 @@
 
 #include "geometry.h"
 
+// Host precompute follows the reference operation order without FMA
+// contraction: CHTLS fits on near-flat clusters are ill conditioned and
+// amplify last-bit differences.
+#pragma clang fp contract(off)
+
 namespace cpgpu {
 
 namespace {

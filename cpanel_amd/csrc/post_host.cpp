@@ -7,6 +7,11 @@
 #include "panelrec.h"
 #include "post.h"
 
+// Host precompute follows the reference operation order without FMA
+// contraction: CHTLS fits on near-flat clusters are ill conditioned and
+// amplify last-bit differences.
+#pragma clang fp contract(off)
+
 namespace cpgpu {
 
 namespace {
@@ -136,9 +141,12 @@ void chtls_weights(int N, const double* X0, int nf, const double* Xf, int order,
             tau[k] = (beta - c0) / beta;
         }
         M[k + (size_t)k * t] = beta;
+        // Eigen's applyHouseholderOnTheLeft: tmp = essential^T * bottom, then
+        // tmp += row(0) (this order matters for ill-conditioned fits)
         auto reflect = [&](double* col) {
-            double s = col[k];
+            double s = 0;
             for (int i = k + 1; i < t; i++) s += M[i + (size_t)k * t] * col[i];
+            s += col[k];
             col[k] -= tau[k] * s;
             for (int i = k + 1; i < t; i++) col[i] -= tau[k] * M[i + (size_t)k * t] * s;
         };
@@ -150,8 +158,9 @@ void chtls_weights(int N, const double* X0, int nf, const double* Xf, int order,
         double* col = &R[(size_t)j * t];
         for (int k = 0; k < t; k++) {
             if (tau[k] == 0) continue;
-            double s = col[k];
+            double s = 0;
             for (int i = k + 1; i < t; i++) s += M[i + (size_t)k * t] * col[i];
+            s += col[k];
             col[k] -= tau[k] * s;
             for (int i = k + 1; i < t; i++) col[i] -= tau[k] * M[i + (size_t)k * t] * s;
         }
