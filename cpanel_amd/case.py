@@ -138,6 +138,13 @@ class Geometry:
         self.ctx.check(abi.lib().cp_read_matrices(self.h, prec, _p(A), _p(B), n), "read matrices")
         return A, B
 
+    def matvec(self, X, kernel=0):
+        """Y = A X through one device kernel (0: FP64 GEMV, 1: FP32 MFMA, 2: FP64-acc MFMA)."""
+        X = np.asfortranarray(np.atleast_2d(np.asarray(X, dtype=np.float64).T).T)
+        Y = np.zeros_like(X, order="F")
+        self.ctx.check(abi.lib().cp_matvec(self.h, kernel, X.shape[1], _p(X), _p(Y)), "matvec")
+        return Y
+
     def solve(self, conditions, prec=CP_FP64, mach=0.1, Sref=1.0, bref=1.0, cref=1.0, cg=(0, 0, 0), tol=1e-10,
               max_iter=0, max_refine=0, inner_tol=0.0, want_cp=True, allow_unconverged=False):
         k = len(conditions)

@@ -128,7 +128,9 @@ __global__ __launch_bounds__(256) void gemm_f64acc_kernel(const float* __restric
         for (int ct = 0; ct < 4; ct++)
 #pragma unroll
             for (int j = 0; j < 4; j++) {
-                const int64_t r = row0 + 16 * rt + 4 * lk + j;
+                // v_mfma_f64_16x16x4f64 result: register j of lane l holds
+                // row 4*j + l/16, column l%16 of the 16x16 tile
+                const int64_t r = row0 + 16 * rt + 4 * j + lk;
                 out[r * KP + 16 * ct + lr] = acc[rt][ct][j];
             }
 }
@@ -177,6 +179,16 @@ __global__ __launch_bounds__(256) void reduce64_kernel(const double* __restrict_
         acc += y * y;
     }
     col_partial(acc, sh, dpart + (int64_t)blockIdx.x * KP);
+}
+
+// Y = sum_ks part (FP32, fixed order); rows >= n zeroed
+__global__ __launch_bounds__(256) void reduce32_kernel(const float* __restrict__ part, int KS, int64_t lda, int n,
+                                                       float* __restrict__ Y) {
+    int64_t idx = (int64_t)blockIdx.x * kT + threadIdx.x;
+    if (idx >= lda * KP) return;
+    float s = 0;
+    for (int q = 0; q < KS; q++) s += part[(int64_t)q * lda * KP + idx];
+    Y[idx] = (idx / KP) < n ? s : 0.f;
 }
 
 // sum of per-block column partials (fixed order) into sc[c][q]
@@ -457,9 +469,8 @@ void BatchedSolver::matvec32(const float* A32, const float* P, float* Y, hipStre
     }
     CP_LAUNCH_CHECK();
     if (Y) {
-        // plain reduction (used by tests and tuning)
-        hipLaunchKernelGGL(inner_v_kernel, dim3((unsigned)nrb_), dim3(kT), 0, s, (const float*)part32_, KS, lda_, n_,
-                           (const int*)act_, Y, (const float*)Y, dpart_);
+        hipLaunchKernelGGL(reduce32_kernel, dim3(nblocks(lda_ * KP)), dim3(kT), 0, s, (const float*)part32_, KS, lda_,
+                           n_, Y);
         CP_LAUNCH_CHECK();
     }
 }

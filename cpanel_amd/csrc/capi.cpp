@@ -577,6 +577,64 @@ cp_status cp_solve(cp_geom* g, cp_prec prec, int32_t k, const cp_freestream* fs,
     });
 }
 
+cp_status cp_matvec(cp_geom* g, int kernel, int32_t k, const double* X, double* Y) {
+    if (!g || !X || !Y || k <= 0 || kernel < 0 || kernel > 2) return CP_BAD_ARG;
+    if (kernel > 0 && k > kRhsTile) return CP_BAD_ARG;
+    cp_ctx* c = g->ctx;
+    if (c->device < 0) return fail(c, CP_NO_DEVICE, "host-only context");
+    return guarded(c, [&]() -> cp_status {
+        const int n = g->dev.n;
+        const int64_t lda = g->dev.lda;
+        hipStream_t s = c->stream;
+        if (kernel == 0) {
+            if (!g->have64) return fail(c, CP_NOT_ASSEMBLED, "FP64 matrices not assembled");
+            if (!g->bicg) g->bicg.reset(new Bicgstab64(n, lda));
+            DevBuf dx, dy;
+            dalloc(c, dx, (size_t)lda * sizeof(double));
+            dalloc(c, dy, (size_t)lda * sizeof(double));
+            CP_HIP(hipMemsetAsync(dx.p, 0, (size_t)lda * sizeof(double), s));
+            for (int q = 0; q < k; q++) {
+                CP_HIP(hipMemcpyAsync(dx.p, X + (size_t)q * n, n * sizeof(double), hipMemcpyHostToDevice, s));
+                g->bicg->gemv().apply(g->A64.as<double>(), dx.as<double>(), dy.as<double>(), 1.0,
+                                      g->bicg->gemv_scratch(), s);
+                CP_HIP(hipMemcpyAsync(Y + (size_t)q * n, dy.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
+            }
+            CP_HIP(hipStreamSynchronize(s));
+            return CP_OK;
+        }
+        if (!g->have32) return fail(c, CP_NOT_ASSEMBLED, "FP32 matrices not assembled");
+        if (!g->batched) g->batched.reset(new BatchedSolver(n, lda, kRhsTile));
+        // row-major lda x 64 multivectors
+        const size_t elems = (size_t)lda * kRhsTile;
+        if (kernel == 1) {
+            std::vector<float> xr(elems, 0.f), yr(elems);
+            for (int q = 0; q < k; q++)
+                for (int i = 0; i < n; i++) xr[(size_t)i * kRhsTile + q] = (float)X[(size_t)q * n + i];
+            DevBuf dx, dy;
+            upload(c, dx, xr);
+            dalloc(c, dy, elems * sizeof(float));
+            g->batched->matvec32(g->A32.as<float>(), dx.as<float>(), dy.as<float>(), s);
+            CP_HIP(hipMemcpyAsync(yr.data(), dy.p, elems * sizeof(float), hipMemcpyDeviceToHost, s));
+            CP_HIP(hipStreamSynchronize(s));
+            for (int q = 0; q < k; q++)
+                for (int i = 0; i < n; i++) Y[(size_t)q * n + i] = yr[(size_t)i * kRhsTile + q];
+        } else {
+            std::vector<double> xr(elems, 0.0), yr(elems);
+            for (int q = 0; q < k; q++)
+                for (int i = 0; i < n; i++) xr[(size_t)i * kRhsTile + q] = X[(size_t)q * n + i];
+            DevBuf dx, dy;
+            upload(c, dx, xr);
+            dalloc(c, dy, elems * sizeof(double));
+            g->batched->matvec64acc(g->A32.as<float>(), dx.as<double>(), dy.as<double>(), nullptr, 1.0, s);
+            CP_HIP(hipMemcpyAsync(yr.data(), dy.p, elems * sizeof(double), hipMemcpyDeviceToHost, s));
+            CP_HIP(hipStreamSynchronize(s));
+            for (int q = 0; q < k; q++)
+                for (int i = 0; i < n; i++) Y[(size_t)q * n + i] = yr[(size_t)i * kRhsTile + q];
+        }
+        return CP_OK;
+    });
+}
+
 cp_status cp_timing_reset(cp_ctx* c) {
     if (!c) return CP_BAD_ARG;
     return guarded(c, [&] {

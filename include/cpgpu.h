@@ -126,6 +126,14 @@ typedef struct {
 cp_status cp_solve(cp_geom* g, cp_prec prec, int32_t k, const cp_freestream* fs, const cp_solve_opts* opts,
                    double* mu, double* cp, cp_solve_info* info);
 
+/* Y = A X through exactly one device kernel (tests, kernel timing); the
+ * product A*x that cpCase::solveMatrixEq hands to Eigen (cpCase.cpp:191-198).
+ *   kernel 0: FP64 GEMV on the FP64 A (k launches)
+ *   kernel 1: FP32 MFMA GEMM on the FP32 A, X rounded to FP32 (k <= 64)
+ *   kernel 2: FP64-accumulated MFMA GEMM on the FP32 A (k <= 64)
+ * X, Y: host, n_body x k column-major. */
+cp_status cp_matvec(cp_geom* g, int kernel, int32_t k, const double* X, double* Y);
+
 /* ---- timing: HIP-event totals of the named kernel on the context stream -- */
 cp_status cp_timing_reset(cp_ctx* ctx);
 /* kernel: "assemble_f64", "assemble_f32", "gemv_f64", "gemm_f32", "resid_f64", ...
