@@ -139,7 +139,8 @@ class Geometry:
         return A, B
 
     def matvec(self, X, kernel=0):
-        """Y = A X through one device kernel (0: FP64 GEMV, 1: FP32 MFMA, 2: FP64-acc MFMA)."""
+        """Y = A X through one device kernel (0: FP64 GEMV, 1: FP32 MFMA, 2: FP64-acc MFMA,
+        3: FP64-acc MFMA + FP64 near-field correction)."""
         X = np.asfortranarray(np.atleast_2d(np.asarray(X, dtype=np.float64).T).T)
         Y = np.zeros_like(X, order="F")
         self.ctx.check(abi.lib().cp_matvec(self.h, kernel, X.shape[1], _p(X), _p(Y)), "matvec")

@@ -114,7 +114,40 @@ __global__ __launch_bounds__(256) void assemble_f32_kernel(
     }
 }
 
+// FP64 value of the listed entries (body influence + Kutta folding, the
+// same arithmetic as assemble_f64_kernel) minus the stored FP32 value.
+__global__ __launch_bounds__(256) void near_correction_kernel(
+    const PanelRec* __restrict__ pan, const double* __restrict__ cx, const double* __restrict__ cy,
+    const double* __restrict__ cz, int64_t lda, const float* __restrict__ A32, const float* __restrict__ B32,
+    const int* __restrict__ fold_ptr, const int* __restrict__ fold_w, const double* __restrict__ fold_c,
+    const double* __restrict__ W, int64_t ldw, int nnz, const int* __restrict__ rows, const int* __restrict__ cols,
+    double* __restrict__ dA, double* __restrict__ dB) {
+    int e = blockIdx.x * kRows + threadIdx.x;
+    if (e >= nnz) return;
+    const int i = rows[e], j = cols[e];
+    const PanelRec& P = pan[j];
+    double src, dub;
+    panel_phi_inf_f64(P, cx[i], cy[i], cz[i], src, dub);
+    for (int q = fold_ptr[j]; q < fold_ptr[j + 1]; q++) {
+#pragma clang fp contract(off)
+        double t = W[i + (int64_t)fold_w[q] * ldw] * fold_c[q];
+        dub = dub + t;
+    }
+    const int64_t o = i + (int64_t)j * lda;
+    dA[e] = dub - (double)A32[o];
+    dB[e] = src - (double)B32[o];
+}
+
 }  // namespace
+
+void launch_near_correction(const DevGeom& g, const double* W, int64_t ldw, const float* A32, const float* B32,
+                            int nnz, const int* rows, const int* cols, double* dA, double* dB, hipStream_t s) {
+    if (nnz == 0) return;
+    hipLaunchKernelGGL(near_correction_kernel, dim3((unsigned)((nnz + kRows - 1) / kRows)), dim3(kRows), 0, s, g.body,
+                       g.cx, g.cy, g.cz, g.lda, A32, B32, g.fold_ptr, g.fold_w, g.fold_c, W, ldw, nnz, rows, cols, dA,
+                       dB);
+    CP_LAUNCH_CHECK();
+}
 
 void launch_wake_influence(const DevGeom& g, double* W, int64_t ldw, hipStream_t s) {
     if (g.nw == 0) return;

@@ -40,8 +40,17 @@ public:
     const GemmShape& shape() const { return shape_; }
     // Y = A32 * P for one 64-column tile (exposed for tests / tuning)
     void matvec32(const float* A32, const float* P, float* Y, hipStream_t s);
-    // R = sign * A32 * X64 (+ B if given), FP64 accumulation
-    void matvec64acc(const float* A32, const double* X, double* Y, const double* Badd, double sign, hipStream_t s);
+    // Y = Badd + sign * (A32 + D) X64 with FP64 accumulation; D = the sparse
+    // near-field correction given by corr (values on the set_near pattern)
+    void matvec64acc(const float* A32, const double* X, double* Y, const double* Badd, double sign, hipStream_t s,
+                     const double* corr = nullptr);
+    // FP64 near-field corrections of A32 and B32 (device CSR by row)
+    void set_near(const int* rowptr, const int* cols, const double* dA, const double* dB) {
+        near_rowptr_ = rowptr;
+        near_cols_ = cols;
+        near_dA_ = dA;
+        near_dB_ = dB;
+    }
 
 private:
     void tile_solve(const float* A32, const float* B32, const double* normals, const double* vinf4, int c0,
@@ -61,6 +70,10 @@ private:
     double* dpart_;     // nrb x 64 x 4 partial column dots
     double* sc_;        // 64 x 16 per-column scalars
     int* act_;          // 64 active flags
+    const int* near_rowptr_ = nullptr;
+    const int* near_cols_ = nullptr;
+    const double* near_dA_ = nullptr;
+    const double* near_dB_ = nullptr;
     double* host_sc_ = nullptr;
     int* host_act_ = nullptr;
 };

@@ -161,10 +161,14 @@ __global__ void sigma_rows_kernel(int n, int64_t lda, int kc, const double* __re
     S[idx] = v;
 }
 
-// Y = Badd + sign * sum_ks part ; partial ||Y_c||^2
+// Y = Badd + sign * (sum_ks part + D X) ; partial ||Y_c||^2.  D is the
+// optional sparse FP64 near-field correction (CSR by row).
 __global__ __launch_bounds__(256) void reduce64_kernel(const double* __restrict__ part, int KS, int64_t lda, int n,
                                                        const double* __restrict__ Badd, double sign,
-                                                       double* __restrict__ Y, double* __restrict__ dpart) {
+                                                       double* __restrict__ Y, double* __restrict__ dpart,
+                                                       const int* __restrict__ rowptr, const int* __restrict__ cols,
+                                                       const double* __restrict__ vals,
+                                                       const double* __restrict__ X) {
     __shared__ double sh[kT];
     const int c = threadIdx.x & 63, g = threadIdx.x >> 6;
     double acc = 0;
@@ -173,6 +177,8 @@ __global__ __launch_bounds__(256) void reduce64_kernel(const double* __restrict_
         int64_t o = i * KP + c;
         double s = 0;
         for (int q = 0; q < KS; q++) s += part[(int64_t)q * lda * KP + o];
+        if (vals && i < n)
+            for (int e = rowptr[i]; e < rowptr[i + 1]; e++) s += vals[e] * X[(int64_t)cols[e] * KP + c];
         double y = (Badd ? Badd[o] : 0.0) + sign * s;
         if (i >= n) y = 0;
         Y[o] = y;
@@ -476,7 +482,7 @@ void BatchedSolver::matvec32(const float* A32, const float* P, float* Y, hipStre
 }
 
 void BatchedSolver::matvec64acc(const float* A32, const double* X, double* Y, const double* Badd, double sign,
-                                hipStream_t s) {
+                                hipStream_t s, const double* corr) {
     const int KS = shape_.kslices;
     int kchunk = (int)round_up((n_ + KS - 1) / KS, 16);
     dim3 grid((unsigned)((lda_ / 256) * KS));
@@ -485,8 +491,9 @@ void BatchedSolver::matvec64acc(const float* A32, const double* X, double* Y, co
         hipLaunchKernelGGL(gemm_f64acc_kernel, grid, dim3(256), 0, s, A32, lda_, n_, X, part64_, kchunk, KS);
     }
     CP_LAUNCH_CHECK();
+    const double* vals = (corr && near_rowptr_) ? corr : nullptr;
     hipLaunchKernelGGL(reduce64_kernel, dim3((unsigned)nrb_), dim3(kT), 0, s, (const double*)part64_, KS, lda_, n_,
-                       Badd, sign, Y, dpart_);
+                       Badd, sign, Y, dpart_, near_rowptr_, near_cols_, vals, X);
     CP_LAUNCH_CHECK();
 }
 
@@ -509,7 +516,7 @@ void BatchedSolver::tile_solve(const float* A32, const float* B32, const double*
 
     // sigma multivector and RHS b = -B32 sigma (FP64 accumulation)
     hipLaunchKernelGGL(sigma_rows_kernel, dim3(nel), dim3(kT), 0, s, n_, lda_, kc, normals, vinf4, S_);
-    matvec64acc(B32, S_, Bv_, nullptr, -1.0, s);
+    matvec64acc(B32, S_, Bv_, nullptr, -1.0, s, near_dB_);
     auto colsum = [&](const double* dp, int nq, int q0, int q1) {
         hipLaunchKernelGGL(colsum_kernel, dim3(1), dim3(KP), 0, s, dp, nrb_, nq, q0, q1, sc_);
         CP_LAUNCH_CHECK();
@@ -532,7 +539,7 @@ void BatchedSolver::tile_solve(const float* A32, const float* B32, const double*
     std::vector<double> rel(KP, 0.0);
     while (true) {
         // residual R = b - A32 X in FP64
-        matvec64acc(A32, X_, R_, Bv_, -1.0, s);
+        matvec64acc(A32, X_, R_, Bv_, -1.0, s, near_dA_);
         colsum(dpart_, 1, Q_RR64, 0);
         CP_HIP(hipMemcpyAsync(host_sc_, sc_, KP * Q_NQ * sizeof(double), hipMemcpyDeviceToHost, s));
         CP_HIP(hipStreamSynchronize(s));

@@ -90,6 +90,10 @@ struct cp_geom {
     DevBuf W;          // wake influence N x Nw (FP64)
     DevBuf A64, B64, A32, B32;
     bool have64 = false, have32 = false;
+    // FP64 near-field corrections of A32/B32 (entries within one panel
+    // diameter: |c_i - c_j| <= longSide_j)
+    DevBuf near_rowptr, near_rows, near_cols, near_dA, near_dB;
+    int near_nnz = 0;
     // post tables
     bool post_ready = false;
     PostHost post_host;
@@ -449,6 +453,25 @@ cp_status cp_assemble(cp_geom* g, cp_prec prec) {
                 TimedScope ts("assemble_f32", c->stream);
                 launch_assemble_f32(g->dev, W, lda, g->A32.as<float>(), g->B32.as<float>(), c->stream);
             }
+            if (!g->near_rowptr.p) {
+                std::vector<int> rp, cl;
+                near_pairs(g->host, 1.0, rp, cl);
+                std::vector<int> rows(cl.size());
+                for (int i = 0; i < g->dev.n; i++)
+                    for (int e = rp[i]; e < rp[i + 1]; e++) rows[e] = i;
+                g->near_nnz = (int)cl.size();
+                upload(c, g->near_rowptr, rp);
+                upload(c, g->near_cols, cl);
+                upload(c, g->near_rows, rows);
+                dalloc(c, g->near_dA, std::max<size_t>(cl.size(), 1) * sizeof(double));
+                dalloc(c, g->near_dB, std::max<size_t>(cl.size(), 1) * sizeof(double));
+            }
+            {
+                TimedScope ts("near_f64", c->stream);
+                launch_near_correction(g->dev, W, lda, g->A32.as<float>(), g->B32.as<float>(), g->near_nnz,
+                                       g->near_rows.as<int>(), g->near_cols.as<int>(), g->near_dA.as<double>(),
+                                       g->near_dB.as<double>(), c->stream);
+            }
             g->have32 = true;
         }
         CP_HIP(hipStreamSynchronize(c->stream));
@@ -532,6 +555,8 @@ cp_status cp_solve(cp_geom* g, cp_prec prec, int32_t k, const cp_freestream* fs,
             }
         } else {
             if (!g->batched || g->batched->capacity() < k) g->batched.reset(new BatchedSolver(n, lda, std::max(k, 1)));
+            g->batched->set_near(g->near_rowptr.as<int>(), g->near_cols.as<int>(), g->near_dA.as<double>(),
+                                 g->near_dB.as<double>());
             BatchedOpts bo;
             bo.tol = tol;
             bo.max_iter = opts->max_iter > 0 ? opts->max_iter : 2 * n;
@@ -578,7 +603,7 @@ cp_status cp_solve(cp_geom* g, cp_prec prec, int32_t k, const cp_freestream* fs,
 }
 
 cp_status cp_matvec(cp_geom* g, int kernel, int32_t k, const double* X, double* Y) {
-    if (!g || !X || !Y || k <= 0 || kernel < 0 || kernel > 2) return CP_BAD_ARG;
+    if (!g || !X || !Y || k <= 0 || kernel < 0 || kernel > 3) return CP_BAD_ARG;
     if (kernel > 0 && k > kRhsTile) return CP_BAD_ARG;
     cp_ctx* c = g->ctx;
     if (c->device < 0) return fail(c, CP_NO_DEVICE, "host-only context");
@@ -604,6 +629,8 @@ cp_status cp_matvec(cp_geom* g, int kernel, int32_t k, const double* X, double* 
         }
         if (!g->have32) return fail(c, CP_NOT_ASSEMBLED, "FP32 matrices not assembled");
         if (!g->batched) g->batched.reset(new BatchedSolver(n, lda, kRhsTile));
+        g->batched->set_near(g->near_rowptr.as<int>(), g->near_cols.as<int>(), g->near_dA.as<double>(),
+                             g->near_dB.as<double>());
         // row-major lda x 64 multivectors
         const size_t elems = (size_t)lda * kRhsTile;
         if (kernel == 1) {
@@ -625,7 +652,8 @@ cp_status cp_matvec(cp_geom* g, int kernel, int32_t k, const double* X, double* 
             DevBuf dx, dy;
             upload(c, dx, xr);
             dalloc(c, dy, elems * sizeof(double));
-            g->batched->matvec64acc(g->A32.as<float>(), dx.as<double>(), dy.as<double>(), nullptr, 1.0, s);
+            g->batched->matvec64acc(g->A32.as<float>(), dx.as<double>(), dy.as<double>(), nullptr, 1.0, s,
+                                    kernel == 3 ? g->near_dA.as<double>() : nullptr);
             CP_HIP(hipMemcpyAsync(yr.data(), dy.p, elems * sizeof(double), hipMemcpyDeviceToHost, s));
             CP_HIP(hipStreamSynchronize(s));
             for (int q = 0; q < k; q++)

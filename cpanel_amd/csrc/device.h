@@ -47,5 +47,9 @@ struct DevGeom {
 void launch_wake_influence(const DevGeom& g, double* W, int64_t ldw, hipStream_t s);
 void launch_assemble_f64(const DevGeom& g, const double* W, int64_t ldw, double* A, double* B, hipStream_t s);
 void launch_assemble_f32(const DevGeom& g, const double* W, int64_t ldw, float* A, float* B, hipStream_t s);
+// FP64 near-field corrections dA = A64 - A32, dB = B64 - B32 on the listed
+// (row, column) entries of the FP32 matrices.
+void launch_near_correction(const DevGeom& g, const double* W, int64_t ldw, const float* A32, const float* B32,
+                            int nnz, const int* rows, const int* cols, double* dA, double* dB, hipStream_t s);
 
 }  // namespace cpgpu

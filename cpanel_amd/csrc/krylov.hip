@@ -15,6 +15,7 @@
 // restart rule).
 #include "device.h"
 #include "krylov.h"
+#include "timer.h"
 
 namespace cpgpu {
 
@@ -249,6 +250,7 @@ Gemv64::Gemv64(int n, int64_t lda) : n_(n), lda_(lda) {
 }
 
 void Gemv64::partial(const double* A, const double* x, double* part, hipStream_t s) const {
+    TimedScope ts("gemv_f64", s);
     dim3 grid((unsigned)(lda_ / kRowsPerBlock), (unsigned)S_);
     hipLaunchKernelGGL(gemv_f64_partial_kernel, grid, dim3(kThreads), 0, s, A, lda_, n_, x, part, chunk_);
     CP_LAUNCH_CHECK();

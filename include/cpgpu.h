@@ -131,6 +131,8 @@ cp_status cp_solve(cp_geom* g, cp_prec prec, int32_t k, const cp_freestream* fs,
  *   kernel 0: FP64 GEMV on the FP64 A (k launches)
  *   kernel 1: FP32 MFMA GEMM on the FP32 A, X rounded to FP32 (k <= 64)
  *   kernel 2: FP64-accumulated MFMA GEMM on the FP32 A (k <= 64)
+ *   kernel 3: kernel 2 plus the FP64 near-field correction (entries with
+ *             |c_i - c_j| <= longSide_j carry their FP64 value)
  * X, Y: host, n_body x k column-major. */
 cp_status cp_matvec(cp_geom* g, int kernel, int32_t k, const double* X, double* Y);
 

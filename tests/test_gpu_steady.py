@@ -89,6 +89,12 @@ def test_matvec_kernels(c1s):
         Y2 = pg.matvec(X, 2)
         ref2 = A32 @ X
         assert np.abs(Y2 - ref2).max() <= 1e-13 * np.abs(ref2).max()
+        # near-field correction: entries within one panel diameter carry
+        # their FP64 value, so the product moves toward A64 x
+        Y3 = pg.matvec(X, 3)
+        e2 = np.abs(Y2 - ref).max()
+        e3 = np.abs(Y3 - ref).max()
+        assert e3 <= 0.3 * e2, (e2, e3)
         Y1 = pg.matvec(X, 1)
         ref1 = A32 @ X.astype(np.float32).astype(np.float64)
         scale = np.abs(A32) @ np.abs(X)
@@ -164,7 +170,7 @@ def test_mixed_precision_against_fp64(c1s):
         # with |Cp| >> 1 are thin tip-cap triangles whose CHTLS fits are ill
         # conditioned and amplify any change of mu)
         assert np.all(d <= 1e-4 * np.maximum(1.0, np.abs(a["cp"])))
-        assert np.linalg.norm(a["mu"] - b["mu"]) / np.linalg.norm(a["mu"]) <= 1e-5
+        assert np.linalg.norm(a["mu"] - b["mu"]) / np.linalg.norm(a["mu"]) <= 1e-7
 
 
 def test_batched_column_independent_of_batch(c1s):
