@@ -16,12 +16,19 @@ struct BatchedOpts {
     double tol = 1e-10;       // outer: ||b - A x|| / ||b|| in FP64
     int max_iter = 0;         // inner FP32 iterations per sweep
     int max_refine = 10;      // FP64 refinement sweeps
-    double inner_tol = 1e-5;  // inner residual reduction per sweep
+    double inner_tol = 1e-3;  // inner residual reduction per sweep
 };
 
-// GEMM tile shape chosen by the autotuner (kslices = split of the K range)
+// GEMM tile shape chosen by the autotuner: rt = 32-row tiles per wavefront,
+// kslices = split of the K range over workgroups
 struct GemmShape {
+    int rt = 2;
     int kslices = 8;
+};
+
+struct TuneRecord {
+    GemmShape shape;
+    float ms;
 };
 
 class BatchedSolver {
@@ -36,7 +43,15 @@ public:
     // x: device FP64 output, column c at x + c*ldx.
     void solve(const float* A32, const float* B32, const double* normals, const double* vinf4, int k,
                const BatchedOpts& o, double* x, int64_t ldx, SolveStats* stats, hipStream_t s);
-    void set_shape(const GemmShape& g) { shape_ = g; }
+    void set_shape(const GemmShape& g) {
+        shape_ = g;
+        tuned_ = true;
+    }
+    void set_autotune(bool on) { autotune_ = on; }
+    // times every candidate shape on A32 and keeps the fastest (once per
+    // leading dimension and process)
+    void tune(const float* A32, hipStream_t s);
+    const std::vector<TuneRecord>& tune_log() const { return tune_log_; }
     const GemmShape& shape() const { return shape_; }
     // Y = A32 * P for one 64-column tile (exposed for tests / tuning)
     void matvec32(const float* A32, const float* P, float* Y, hipStream_t s);
@@ -55,6 +70,9 @@ public:
 private:
     void tile_solve(const float* A32, const float* B32, const double* normals, const double* vinf4, int c0,
                     int kc, const BatchedOpts& o, double* x, int64_t ldx, SolveStats* stats, hipStream_t s);
+    void launch_gemm32(const GemmShape& sh, const float* A32, const float* P, hipStream_t s);
+    bool tuned_ = false, autotune_ = true;
+    std::vector<TuneRecord> tune_log_;
     int n_, cap_;
     int64_t lda_;
     int nrb_;      // row blocks of the reduction kernels

@@ -18,6 +18,9 @@
 // kslices; workgroup id -> (kslice = id % kslices, row block = id / kslices)
 // so with kslices a multiple of 8 every XCD works on one K slice and keeps
 // that slice of the multivector in its own L2.
+#include <map>
+#include <mutex>
+
 #include "batched.h"
 #include "device.h"
 #include "timer.h"
@@ -34,60 +37,83 @@ constexpr int kT = 256;
 constexpr int kRedRows = 64;   // rows per reduction block
 
 enum {
-    Q_RHO = 0, Q_ALPHA, Q_OMEGA, Q_RR, Q_R0R, Q_R0V, Q_TT, Q_TS, Q_RR0, Q_BB, Q_RR64, Q_SCALE, Q_BETA, Q_NQ = 16
+    Q_RHO = 0, Q_ALPHA, Q_OMEGA, Q_RR, Q_R0R, Q_R0V, Q_TT, Q_TS, Q_RR0, Q_BB, Q_RR64, Q_SCALE, Q_BETA, Q_IT, Q_CHK,
+    Q_CHKIT, Q_NQ = 16
 };
 
+// stagnation window of the inner FP32 solve: a column stops when its
+// residual fell by less than 2x over the last kStagWin iterations (the FP32
+// solve has reached its attainable accuracy; refinement takes over)
+constexpr int kStagWin = 25;
+
 // ---------------------------------------------------------------- GEMMs
+// Y_part[ks] = A32[:, K slice ks] * P[K slice ks, :] for 64 columns.
+// RT row tiles of 32 rows per wavefront (4 wavefronts per workgroup), the
+// whole 64-column width per wavefront (2 column tiles), K consumed 16 at a
+// time.  The next K chunk's operands are loaded into registers before the
+// current chunk's MFMAs are issued, so HBM latency overlaps the matrix cores.
+// Operand layout of v_mfma_f32_32x32x2f32: lane l supplies A[row l%32][k l/32]
+// and B[k l/32][col l%32]; result register j of lane l is row
+// 8*(j/4) + 4*(l/32) + j%4, column l%32.
+template <int RT>
 __global__ __launch_bounds__(256) void gemm_f32_kernel(const float* __restrict__ A, int64_t lda, int n,
                                                        const float* __restrict__ P, float* __restrict__ part,
                                                        int kchunk, int KS) {
+    constexpr int U = 8;  // K steps of 2 per chunk (16 K values)
     const int bid = blockIdx.x;
     const int ks = bid % KS, rb = bid / KS;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int64_t row0 = (int64_t)rb * 256 + wave * 64;
+    const int64_t row0 = ((int64_t)rb * 4 + wave) * (32 * RT);
     const int k0 = ks * kchunk, k1 = min(n, k0 + kchunk);
     const int lr = lane & 31, lk = lane >> 5;
-    floatx16 acc00 = {}, acc01 = {}, acc10 = {}, acc11 = {};
+    floatx16 acc[RT][2];
+#pragma unroll
+    for (int t = 0; t < RT; t++) acc[t][0] = acc[t][1] = floatx16{};
     const float* Ab = A + row0 + lr;
     const float* Pb = P + lr;
-    int k = k0;
-    for (; k + 16 <= k1; k += 16) {
-        float a0[8], a1[8], b0[8], b1[8];
+    float a[RT][U], b[2][U], an[RT][U], bn[2][U];
+    auto load = [&](int k, float (&aa)[RT][U], float (&bb)[2][U]) {
 #pragma unroll
-        for (int u = 0; u < 8; u++) {
+        for (int u = 0; u < U; u++) {
             const int64_t kk = k + 2 * u + lk;
-            a0[u] = Ab[kk * lda];
-            a1[u] = Ab[kk * lda + 32];
-            b0[u] = Pb[kk * KP];
-            b1[u] = Pb[kk * KP + 32];
-        }
+            const bool ok = kk < k1;
+            const int64_t kc = ok ? kk : 0;
 #pragma unroll
-        for (int u = 0; u < 8; u++) {
-            acc00 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0[u], b0[u], acc00, 0, 0, 0);
-            acc01 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0[u], b1[u], acc01, 0, 0, 0);
-            acc10 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1[u], b0[u], acc10, 0, 0, 0);
-            acc11 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1[u], b1[u], acc11, 0, 0, 0);
+            for (int t = 0; t < RT; t++) aa[t][u] = ok ? Ab[kc * lda + 32 * t] : 0.f;
+            bb[0][u] = ok ? Pb[kc * KP] : 0.f;
+            bb[1][u] = ok ? Pb[kc * KP + 32] : 0.f;
         }
-    }
-    for (; k < k1; k += 2) {
-        const int64_t kk = k + lk;
-        const bool ok = kk < k1;
-        float a0 = ok ? Ab[kk * lda] : 0.f, a1 = ok ? Ab[kk * lda + 32] : 0.f;
-        float b0 = ok ? Pb[kk * KP] : 0.f, b1 = ok ? Pb[kk * KP + 32] : 0.f;
-        acc00 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b0, acc00, 0, 0, 0);
-        acc01 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b1, acc01, 0, 0, 0);
-        acc10 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b0, acc10, 0, 0, 0);
-        acc11 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b1, acc11, 0, 0, 0);
+    };
+    if (k0 < k1) load(k0, a, b);
+    for (int k = k0; k < k1; k += 2 * U) {
+        const bool more = k + 2 * U < k1;
+        if (more) load(k + 2 * U, an, bn);
+#pragma unroll
+        for (int u = 0; u < U; u++)
+#pragma unroll
+            for (int t = 0; t < RT; t++) {
+                acc[t][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[t][u], b[0][u], acc[t][0], 0, 0, 0);
+                acc[t][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[t][u], b[1][u], acc[t][1], 0, 0, 0);
+            }
+        if (more) {
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+#pragma unroll
+                for (int t = 0; t < RT; t++) a[t][u] = an[t][u];
+                b[0][u] = bn[0][u];
+                b[1][u] = bn[1][u];
+            }
+        }
     }
     float* out = part + (int64_t)ks * lda * KP;
 #pragma unroll
-    for (int j = 0; j < 16; j++) {
-        const int64_t r = row0 + 8 * (j >> 2) + 4 * lk + (j & 3);
-        out[r * KP + lr] = acc00[j];
-        out[r * KP + 32 + lr] = acc01[j];
-        out[(r + 32) * KP + lr] = acc10[j];
-        out[(r + 32) * KP + 32 + lr] = acc11[j];
-    }
+    for (int t = 0; t < RT; t++)
+#pragma unroll
+        for (int j = 0; j < 16; j++) {
+            const int64_t r = row0 + 32 * t + 8 * (j >> 2) + 4 * lk + (j & 3);
+            out[r * KP + lr] = acc[t][0][j];
+            out[r * KP + 32 + lr] = acc[t][1][j];
+        }
 }
 
 __global__ __launch_bounds__(256) void gemm_f64acc_kernel(const float* __restrict__ A, int64_t lda, int n,
@@ -198,15 +224,24 @@ __global__ __launch_bounds__(256) void reduce32_kernel(const float* __restrict__
 }
 
 // sum of per-block column partials (fixed order) into sc[c][q]
-__global__ void colsum_kernel(const double* __restrict__ dpart, int nblk, int nq, int q0, int q1,
-                              double* __restrict__ sc) {
-    int c = threadIdx.x;
-    if (c >= KP) return;
+// 1024 threads: column c = t % 64, group g = t / 64 sums blocks b = g mod 16;
+// the 16 group sums are then added in a fixed order
+__global__ __launch_bounds__(1024) void colsum_kernel(const double* __restrict__ dpart, int nblk, int nq, int q0,
+                                                      int q1, double* __restrict__ sc) {
+    __shared__ double sh[1024];
+    const int c = threadIdx.x & 63, g = threadIdx.x >> 6;
     for (int q = 0; q < nq; q++) {
         double s = 0;
         const double* d = dpart + (int64_t)q * nblk * KP;
-        for (int b = 0; b < nblk; b++) s += d[(int64_t)b * KP + c];
-        sc[c * Q_NQ + (q == 0 ? q0 : q1)] = s;
+        for (int b = g; b < nblk; b += 16) s += d[(int64_t)b * KP + c];
+        sh[threadIdx.x] = s;
+        __syncthreads();
+        if (g == 0) {
+            double t = 0;
+            for (int k = 0; k < 16; k++) t += sh[k * 64 + c];
+            sc[c * Q_NQ + (q == 0 ? q0 : q1)] = t;
+        }
+        __syncthreads();
     }
 }
 
@@ -245,6 +280,9 @@ __global__ void scal_init_kernel(double* __restrict__ sc) {
     s[Q_RHO] = 1;
     s[Q_ALPHA] = 1;
     s[Q_OMEGA] = 1;
+    s[Q_IT] = 0;
+    s[Q_CHK] = s[Q_RR];
+    s[Q_CHKIT] = 0;
 }
 
 // rho/beta step at the top of an inner iteration; deactivates converged or
@@ -258,6 +296,16 @@ __global__ void scal_beta_kernel(double* __restrict__ sc, int* __restrict__ act,
         act[c] = 0;
         return;
     }
+    const double it = s[Q_IT];
+    if (it - s[Q_CHKIT] >= kStagWin) {
+        if (s[Q_RR] > 0.25 * s[Q_CHK]) {  // |r| fell by less than 2x over the window
+            act[c] = 0;
+            return;
+        }
+        s[Q_CHK] = s[Q_RR];
+        s[Q_CHKIT] = it;
+    }
+    s[Q_IT] = it + 1;
     double rho_old = s[Q_RHO];
     double rho = s[Q_R0R];
     const double eps = 1.1920928955078125e-07;
@@ -454,8 +502,10 @@ BatchedSolver::BatchedSolver(int n, int64_t lda, int capacity) : n_(n), cap_(cap
     act_ = (int*)take(KP * sizeof(int));
     CP_HIP(hipHostMalloc(&host_sc_, KP * Q_NQ * sizeof(double), hipHostMallocDefault));
     CP_HIP(hipHostMalloc(&host_act_, 2 * KP * sizeof(int), hipHostMallocDefault));
-    // default shape: 8 K slices (one per XCD), more for small matrices
+    // default shape before tuning: 8 K slices (one per XCD), more for small
+    // matrices
     int rb = (int)(lda / 256);
+    shape_.rt = 2;
     shape_.kslices = rb >= 64 ? 8 : (rb >= 16 ? 16 : 32);
 }
 
@@ -465,15 +515,76 @@ BatchedSolver::~BatchedSolver() {
     if (host_act_) (void)hipHostFree(host_act_);
 }
 
-void BatchedSolver::matvec32(const float* A32, const float* P, float* Y, hipStream_t s) {
-    const int KS = shape_.kslices;
+void BatchedSolver::launch_gemm32(const GemmShape& sh, const float* A32, const float* P, hipStream_t s) {
+    const int KS = sh.kslices;
     int kchunk = (int)round_up((n_ + KS - 1) / KS, 16);
-    dim3 grid((unsigned)((lda_ / 256) * KS));
-    {
-        TimedScope ts("gemm_f32", s);
-        hipLaunchKernelGGL(gemm_f32_kernel, grid, dim3(256), 0, s, A32, lda_, n_, P, part32_, kchunk, KS);
+    const int rows_per_block = 4 * 32 * sh.rt;
+    dim3 grid((unsigned)((lda_ / rows_per_block) * KS));
+    TimedScope ts("gemm_f32", s);
+    switch (sh.rt) {
+        case 1: hipLaunchKernelGGL(gemm_f32_kernel<1>, grid, dim3(256), 0, s, A32, lda_, n_, P, part32_, kchunk, KS); break;
+        case 2: hipLaunchKernelGGL(gemm_f32_kernel<2>, grid, dim3(256), 0, s, A32, lda_, n_, P, part32_, kchunk, KS); break;
+        default: hipLaunchKernelGGL(gemm_f32_kernel<4>, grid, dim3(256), 0, s, A32, lda_, n_, P, part32_, kchunk, KS); break;
     }
     CP_LAUNCH_CHECK();
+}
+
+void BatchedSolver::tune(const float* A32, hipStream_t s) {
+    if (tuned_) return;
+    tuned_ = true;
+    static std::mutex mu;
+    static std::map<int64_t, GemmShape> cache;  // per leading dimension, process wide
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        auto it = cache.find(lda_);
+        if (it != cache.end()) {
+            shape_ = it->second;
+            return;
+        }
+    }
+    // candidates: row tiles per wavefront x K slices; the row-block count must
+    // divide lda (a multiple of 512)
+    std::vector<GemmShape> cand;
+    for (int rt : {1, 2, 4})
+        for (int ks : {4, 8, 16, 32}) {
+            if (lda_ % (128 * rt) != 0) continue;
+            GemmShape g;
+            g.rt = rt;
+            g.kslices = ks;
+            cand.push_back(g);
+        }
+    hipEvent_t e0, e1;
+    CP_HIP(hipEventCreate(&e0));
+    CP_HIP(hipEventCreate(&e1));
+    KernelTimer* saved = current_timer();
+    set_current_timer(nullptr);
+    float best = 1e30f;
+    GemmShape bestg = shape_;
+    for (const GemmShape& g : cand) {
+        launch_gemm32(g, A32, p_, s);  // warm
+        CP_HIP(hipEventRecord(e0, s));
+        for (int r = 0; r < 3; r++) launch_gemm32(g, A32, p_, s);
+        CP_HIP(hipEventRecord(e1, s));
+        CP_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        CP_HIP(hipEventElapsedTime(&ms, e0, e1));
+        tune_log_.push_back({g, ms / 3});
+        if (ms < best) {
+            best = ms;
+            bestg = g;
+        }
+    }
+    set_current_timer(saved);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    shape_ = bestg;
+    std::lock_guard<std::mutex> lk(mu);
+    cache[lda_] = bestg;
+}
+
+void BatchedSolver::matvec32(const float* A32, const float* P, float* Y, hipStream_t s) {
+    const int KS = shape_.kslices;
+    launch_gemm32(shape_, A32, P, s);
     if (Y) {
         hipLaunchKernelGGL(reduce32_kernel, dim3(nblocks(lda_ * KP)), dim3(kT), 0, s, (const float*)part32_, KS, lda_,
                            n_, Y);
@@ -511,18 +622,19 @@ void BatchedSolver::tile_solve(const float* A32, const float* B32, const double*
     (void)c0;
     const int64_t rows = lda_;
     const unsigned nel = nblocks(rows * KP);
-    const int KS = shape_.kslices;
     int* host_sweep = host_act_ + KP;
 
     // sigma multivector and RHS b = -B32 sigma (FP64 accumulation)
     hipLaunchKernelGGL(sigma_rows_kernel, dim3(nel), dim3(kT), 0, s, n_, lda_, kc, normals, vinf4, S_);
     matvec64acc(B32, S_, Bv_, nullptr, -1.0, s, near_dB_);
     auto colsum = [&](const double* dp, int nq, int q0, int q1) {
-        hipLaunchKernelGGL(colsum_kernel, dim3(1), dim3(KP), 0, s, dp, nrb_, nq, q0, q1, sc_);
+        hipLaunchKernelGGL(colsum_kernel, dim3(1), dim3(1024), 0, s, dp, nrb_, nq, q0, q1, sc_);
         CP_LAUNCH_CHECK();
     };
     colsum(dpart_, 1, Q_BB, 0);
     hipLaunchKernelGGL(invd32_kernel, dim3(nblocks(lda_)), dim3(kT), 0, s, n_, A32, lda_, invd_);
+    if (autotune_) tune(A32, s);
+    const int KS = shape_.kslices;
     CP_HIP(hipMemsetAsync(X_, 0, (size_t)lda_ * KP * sizeof(double), s));
     std::vector<double> bb(KP, 0.0);
     CP_HIP(hipMemcpyAsync(host_sc_, sc_, KP * Q_NQ * sizeof(double), hipMemcpyDeviceToHost, s));
