@@ -15,7 +15,7 @@ constexpr int kRhsTile = 64;  // right-hand sides per batched launch
 struct BatchedOpts {
     double tol = 1e-10;       // outer: ||b - A x|| / ||b|| in FP64
     int max_iter = 0;         // inner FP32 iterations per sweep
-    int max_refine = 10;      // FP64 refinement sweeps
+    int max_refine = 20;      // FP64 refinement sweeps
     double inner_tol = 1e-3;  // inner residual reduction per sweep
 };
 

@@ -38,13 +38,14 @@ constexpr int kRedRows = 64;   // rows per reduction block
 
 enum {
     Q_RHO = 0, Q_ALPHA, Q_OMEGA, Q_RR, Q_R0R, Q_R0V, Q_TT, Q_TS, Q_RR0, Q_BB, Q_RR64, Q_SCALE, Q_BETA, Q_IT, Q_CHK,
-    Q_CHKIT, Q_NQ = 16
+    Q_CHKIT, Q_MIN, Q_NQ = 20
 };
 
-// stagnation window of the inner FP32 solve: a column stops when its
-// residual fell by less than 2x over the last kStagWin iterations (the FP32
-// solve has reached its attainable accuracy; refinement takes over)
-constexpr int kStagWin = 25;
+// stagnation window of the inner FP32 solve: a column stops when the best
+// residual seen so far fell by less than 2x over the last kStagWin
+// iterations (BiCGSTAB plateaus are shorter; a longer stall means the FP32
+// solve has reached its attainable accuracy and refinement takes over)
+constexpr int kStagWin = 60;
 
 // ---------------------------------------------------------------- GEMMs
 // Y_part[ks] = A32[:, K slice ks] * P[K slice ks, :] for 64 columns.
@@ -283,6 +284,7 @@ __global__ void scal_init_kernel(double* __restrict__ sc) {
     s[Q_IT] = 0;
     s[Q_CHK] = s[Q_RR];
     s[Q_CHKIT] = 0;
+    s[Q_MIN] = s[Q_RR];
 }
 
 // rho/beta step at the top of an inner iteration; deactivates converged or
@@ -297,12 +299,13 @@ __global__ void scal_beta_kernel(double* __restrict__ sc, int* __restrict__ act,
         return;
     }
     const double it = s[Q_IT];
+    s[Q_MIN] = fmin(s[Q_MIN], s[Q_RR]);
     if (it - s[Q_CHKIT] >= kStagWin) {
-        if (s[Q_RR] > 0.25 * s[Q_CHK]) {  // |r| fell by less than 2x over the window
+        if (s[Q_MIN] > 0.25 * s[Q_CHK]) {  // best |r| fell by less than 2x over the window
             act[c] = 0;
             return;
         }
-        s[Q_CHK] = s[Q_RR];
+        s[Q_CHK] = s[Q_MIN];
         s[Q_CHKIT] = it;
     }
     s[Q_IT] = it + 1;

@@ -560,7 +560,7 @@ cp_status cp_solve(cp_geom* g, cp_prec prec, int32_t k, const cp_freestream* fs,
             BatchedOpts bo;
             bo.tol = tol;
             bo.max_iter = opts->max_iter > 0 ? opts->max_iter : 2 * n;
-            bo.max_refine = opts->max_refine > 0 ? opts->max_refine : 10;
+            bo.max_refine = opts->max_refine > 0 ? opts->max_refine : 20;
             bo.inner_tol = opts->inner_tol > 0 ? opts->inner_tol : 1e-3;
             g->batched->solve(g->A32.as<float>(), g->B32.as<float>(), g->normals.as<double>(), dv4.as<double>(), k,
                               bo, dmu.as<double>(), lda, stats.data(), s);

@@ -107,7 +107,7 @@ typedef struct {
     double cg[3];
     double tol;             /* target ||r||/||b|| (<=0: 1e-10) */
     int32_t max_iter;       /* Krylov iterations per solve (<=0: 2N) */
-    int32_t max_refine;     /* FP64 refinement sweeps, CP_FP32_REFINE (<=0: 10) */
+    int32_t max_refine;     /* FP64 refinement sweeps, CP_FP32_REFINE (<=0: 20) */
     double inner_tol;       /* FP32 inner solve reduction per sweep (<=0: 1e-3) */
 } cp_solve_opts;
 
