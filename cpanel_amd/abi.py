@@ -35,7 +35,8 @@ class Freestream(C.Structure):
 class SolveOpts(C.Structure):
     _fields_ = [("mach", C.c_double), ("Sref", C.c_double), ("bref", C.c_double), ("cref", C.c_double),
                 ("cg", C.c_double * 3), ("tol", C.c_double), ("max_iter", C.c_int32),
-                ("max_refine", C.c_int32), ("inner_tol", C.c_double)]
+                ("max_refine", C.c_int32), ("inner_tol", C.c_double), ("precond", C.c_int32),
+                ("pad", C.c_int32)]
 
 
 class SolveInfo(C.Structure):

@@ -147,10 +147,11 @@ class Geometry:
         return Y
 
     def solve(self, conditions, prec=CP_FP64, mach=0.1, Sref=1.0, bref=1.0, cref=1.0, cg=(0, 0, 0), tol=1e-10,
-              max_iter=0, max_refine=0, inner_tol=0.0, want_cp=True, allow_unconverged=False):
+              max_iter=0, max_refine=0, inner_tol=0.0, want_cp=True, allow_unconverged=False, precond=0):
         k = len(conditions)
         fs = (abi.Freestream * k)(*[abi.Freestream(*map(float, c)) for c in conditions])
-        o = abi.SolveOpts(mach, Sref, bref, cref, (C.c_double * 3)(*cg), tol, max_iter, max_refine, inner_tol)
+        o = abi.SolveOpts(mach, Sref, bref, cref, (C.c_double * 3)(*cg), tol, max_iter, max_refine, inner_tol,
+                          precond, 0)
         mu = np.zeros((k, self.nb))
         cp = np.zeros((k, self.nb)) if want_cp else None
         info = (abi.SolveInfo * k)()

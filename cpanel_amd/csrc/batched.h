@@ -59,6 +59,18 @@ public:
     // near-field correction given by corr (values on the set_near pattern)
     void matvec64acc(const float* A32, const double* X, double* Y, const double* Badd, double sign, hipStream_t s,
                      const double* corr = nullptr);
+    // strip block-Jacobi preconditioner for the inner solve (device arrays;
+    // null perm = Jacobi).  inv: FP32 inverses, column-major per block.
+    void set_block_jacobi(const int* perm, const int* ptr, const int64_t* off, const float* inv, int nblocks,
+                          int max_b) {
+        bj_ = perm != nullptr;
+        bj_perm_ = perm;
+        bj_ptr_ = ptr;
+        bj_off_ = off;
+        bj_inv_ = inv;
+        bj_nblocks_ = nblocks;
+        bj_maxb_ = max_b;
+    }
     // FP64 near-field corrections of A32 and B32 (device CSR by row)
     void set_near(const int* rowptr, const int* cols, const double* dA, const double* dB) {
         near_rowptr_ = rowptr;
@@ -71,6 +83,13 @@ private:
     void tile_solve(const float* A32, const float* B32, const double* normals, const double* vinf4, int c0,
                     int kc, const BatchedOpts& o, double* x, int64_t ldx, SolveStats* stats, hipStream_t s);
     void launch_gemm32(const GemmShape& sh, const float* A32, const float* P, hipStream_t s);
+    void apply_block_jacobi(const float* P, float* Y, hipStream_t s);
+    bool bj_ = false;
+    const int* bj_perm_ = nullptr;
+    const int* bj_ptr_ = nullptr;
+    const int64_t* bj_off_ = nullptr;
+    const float* bj_inv_ = nullptr;
+    int bj_nblocks_ = 0, bj_maxb_ = 0;
     bool tuned_ = false, autotune_ = true;
     std::vector<TuneRecord> tune_log_;
     int n_, cap_;
@@ -97,5 +116,8 @@ private:
 };
 
 void launch_sigma(int n, int64_t lda, const double* normals, const double* vinf4, double* sigma, hipStream_t s);
+// copies the diagonal blocks of A32 into packed column-major storage
+void launch_bj_gather(const float* A32, int64_t lda, const int* perm, const int* ptr, const int64_t* off, int nblocks,
+                      int max_b, float* out, hipStream_t s);
 
 }  // namespace cpgpu

@@ -346,7 +346,7 @@ __global__ __launch_bounds__(256) void inner_p_kernel(int64_t rows, const double
     float beta = (float)sc[c * Q_NQ + Q_BETA], omega = (float)sc[c * Q_NQ + Q_OMEGA];
     float pi = r[idx] + beta * (p[idx] - omega * v[idx]);
     p[idx] = pi;
-    y[idx] = invd[idx >> 6] * pi;
+    if (invd) y[idx] = invd[idx >> 6] * pi;  // Jacobi; else block Jacobi follows
 }
 
 // v = sum parts; partial r0.v
@@ -382,7 +382,7 @@ __global__ __launch_bounds__(256) void inner_s_kernel(int64_t rows, const double
     float alpha = (float)sc[c * Q_NQ + Q_ALPHA];
     float si = r[idx] - alpha * v[idx];
     s[idx] = si;
-    z[idx] = invd[idx >> 6] * si;
+    if (invd) z[idx] = invd[idx >> 6] * si;
 }
 
 // t = sum parts; partial t.t and t.s
@@ -468,6 +468,52 @@ __global__ void sigma_kernel(int n, const double* __restrict__ nrm, int64_t lda,
     sig[i] = i < n ? -(v4[0] * nrm[i] + v4[1] * nrm[lda + i] + v4[2] * nrm[2 * lda + i]) : 0.0;
 }
 
+// ---------------------------------------------------------------- block Jacobi
+// gather the diagonal blocks of A32 (column-major per block, FP32)
+__global__ __launch_bounds__(256) void bj_gather_kernel(const float* __restrict__ A, int64_t lda,
+                                                        const int* __restrict__ perm, const int* __restrict__ ptr,
+                                                        const int64_t* __restrict__ off, float* __restrict__ out) {
+    const int q = blockIdx.y;
+    const int b0 = ptr[q], b = ptr[q + 1] - b0;
+    const int64_t e = (int64_t)blockIdx.x * kT + threadIdx.x;
+    if (e >= (int64_t)b * b) return;
+    const int r = (int)(e % b), c = (int)(e / b);
+    out[off[q] + e] = A[perm[b0 + r] + (int64_t)perm[b0 + c] * lda];
+}
+
+constexpr int kBjCols = 8;  // multivector columns per workgroup
+
+// Y[rows of block q] = inv_q * P[rows of block q]; the block's gathered rows of
+// P (kBjCols columns) are staged in LDS and read as broadcasts, the inverse is
+// column-major so consecutive lanes (rows) read consecutive addresses.
+__global__ __launch_bounds__(256) void bj_apply_kernel(const int* __restrict__ perm, const int* __restrict__ ptr,
+                                                       const int64_t* __restrict__ off, const float* __restrict__ inv,
+                                                       const float* __restrict__ P, float* __restrict__ Y) {
+    extern __shared__ float Pl[];  // b x kBjCols
+    const int q = blockIdx.x, c0 = blockIdx.y * kBjCols;
+    const int b0 = ptr[q], b = ptr[q + 1] - b0;
+    const float* M = inv + off[q];
+    for (int s = threadIdx.x; s < b; s += kT) {
+        const float* src = P + (int64_t)perm[b0 + s] * KP + c0;
+#pragma unroll
+        for (int c = 0; c < kBjCols; c++) Pl[s * kBjCols + c] = src[c];
+    }
+    __syncthreads();
+    for (int r = threadIdx.x; r < b; r += kT) {
+        float acc[kBjCols];
+#pragma unroll
+        for (int c = 0; c < kBjCols; c++) acc[c] = 0.f;
+        for (int s2 = 0; s2 < b; s2++) {
+            const float m = M[(int64_t)s2 * b + r];
+#pragma unroll
+            for (int c = 0; c < kBjCols; c++) acc[c] += m * Pl[s2 * kBjCols + c];
+        }
+        float* dst = Y + (int64_t)perm[b0 + r] * KP + c0;
+#pragma unroll
+        for (int c = 0; c < kBjCols; c++) dst[c] = acc[c];
+    }
+}
+
 inline unsigned nblocks(int64_t work) { return (unsigned)((work + kT - 1) / kT); }
 
 }  // namespace
@@ -503,6 +549,8 @@ BatchedSolver::BatchedSolver(int n, int64_t lda, int capacity) : n_(n), cap_(cap
     dpart_ = (double*)take((size_t)4 * nrb_ * KP * sizeof(double));
     sc_ = (double*)take(KP * Q_NQ * sizeof(double));
     act_ = (int*)take(KP * sizeof(int));
+    CP_HIP(hipMemset(y_, 0, mv32));
+    CP_HIP(hipMemset(z_, 0, mv32));
     CP_HIP(hipHostMalloc(&host_sc_, KP * Q_NQ * sizeof(double), hipHostMallocDefault));
     CP_HIP(hipHostMalloc(&host_act_, 2 * KP * sizeof(int), hipHostMallocDefault));
     // default shape before tuning: 8 K slices (one per XCD), more for small
@@ -611,6 +659,21 @@ void BatchedSolver::matvec64acc(const float* A32, const double* X, double* Y, co
     CP_LAUNCH_CHECK();
 }
 
+void BatchedSolver::apply_block_jacobi(const float* P, float* Y, hipStream_t s) {
+    TimedScope ts("bj_apply", s);
+    dim3 grid((unsigned)bj_nblocks_, (unsigned)(KP / kBjCols));
+    hipLaunchKernelGGL(bj_apply_kernel, grid, dim3(kT), (size_t)bj_maxb_ * kBjCols * sizeof(float), s, bj_perm_,
+                       bj_ptr_, bj_off_, bj_inv_, P, Y);
+    CP_LAUNCH_CHECK();
+}
+
+void launch_bj_gather(const float* A32, int64_t lda, const int* perm, const int* ptr, const int64_t* off, int nblocks,
+                      int max_b, float* out, hipStream_t s) {
+    dim3 grid((unsigned)(((int64_t)max_b * max_b + kT - 1) / kT), (unsigned)nblocks);
+    hipLaunchKernelGGL(bj_gather_kernel, grid, dim3(kT), 0, s, A32, lda, perm, ptr, off, out);
+    CP_LAUNCH_CHECK();
+}
+
 void BatchedSolver::solve(const float* A32, const float* B32, const double* normals, const double* vinf4, int k,
                           const BatchedOpts& o, double* x, int64_t ldx, SolveStats* stats, hipStream_t s) {
     for (int c0 = 0; c0 < k; c0 += KP) {
@@ -694,14 +757,18 @@ void BatchedSolver::tile_solve(const float* A32, const float* B32, const double*
             for (int c = 0; c < KP; c++) nact += host_act_[c] != 0;
             if (nact == 0) break;
             hipLaunchKernelGGL(inner_p_kernel, dim3(nel), dim3(kT), 0, s, rows, (const double*)sc_, (const int*)act_,
-                               (const float*)r_, p_, (const float*)v_, (const float*)invd_, y_);
+                               (const float*)r_, p_, (const float*)v_, bj_ ? (const float*)nullptr : (const float*)invd_,
+                               y_);
+            if (bj_) apply_block_jacobi(p_, y_, s);
             matvec32(A32, y_, nullptr, s);
             hipLaunchKernelGGL(inner_v_kernel, dim3((unsigned)nrb_), dim3(kT), 0, s, (const float*)part32_, KS, lda_,
                                n_, (const int*)act_, v_, (const float*)r0_, dpart_);
             colsum(dpart_, 1, Q_R0V, 0);
             hipLaunchKernelGGL(scal_alpha_kernel, dim3(1), dim3(KP), 0, s, sc_, (const int*)act_);
             hipLaunchKernelGGL(inner_s_kernel, dim3(nel), dim3(kT), 0, s, rows, (const double*)sc_, (const int*)act_,
-                               (const float*)r_, (const float*)v_, (const float*)invd_, sv_, z_);
+                               (const float*)r_, (const float*)v_, bj_ ? (const float*)nullptr : (const float*)invd_,
+                               sv_, z_);
+            if (bj_) apply_block_jacobi(sv_, z_, s);
             matvec32(A32, z_, nullptr, s);
             hipLaunchKernelGGL(inner_t_kernel, dim3((unsigned)nrb_), dim3(kT), 0, s, (const float*)part32_, KS, lda_,
                                n_, (const int*)act_, t_, (const float*)sv_, dpart_, dpart_ + (size_t)nrb_ * KP);

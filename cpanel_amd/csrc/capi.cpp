@@ -21,6 +21,7 @@
 #include "krylov.h"
 #include "panelrec.h"
 #include "post.h"
+#include "precond.h"
 #include "timer.h"
 
 using namespace cpgpu;
@@ -94,6 +95,10 @@ struct cp_geom {
     // diameter: |c_i - c_j| <= longSide_j)
     DevBuf near_rowptr, near_rows, near_cols, near_dA, near_dB;
     int near_nnz = 0;
+    // strip block-Jacobi preconditioner of the batched FP32 inner solve
+    BlockPlan bj_plan;
+    DevBuf bj_perm, bj_ptr, bj_off, bj_inv;
+    bool bj_ready = false;
     // post tables
     bool post_ready = false;
     PostHost post_host;
@@ -275,6 +280,34 @@ void ensure_post(cp_geom* g) {
     P.bez = g->p_bez.as<double>();
     P.area = g->p_area.as<double>();
     g->post_ready = true;
+}
+
+// Strip block-Jacobi inverses of the FP32 matrix: plan on the host, gather the
+// diagonal blocks on the device, invert in FP64 on host threads, upload FP32.
+void ensure_block_jacobi(cp_geom* g) {
+    if (g->bj_ready) return;
+    cp_ctx* c = g->ctx;
+    BlockPlan& P = g->bj_plan;
+    if (P.perm.empty()) {
+        build_strip_blocks(g->host, 768, 1024, P);
+        upload(c, g->bj_perm, P.perm);
+        upload(c, g->bj_ptr, P.ptr);
+        upload(c, g->bj_off, P.off);
+        dalloc(c, g->bj_inv, std::max<int64_t>(P.off.back(), 1) * sizeof(float));
+    }
+    const int64_t total = P.off.back();
+    launch_bj_gather(g->A32.as<float>(), g->dev.lda, g->bj_perm.as<int>(), g->bj_ptr.as<int>(), g->bj_off.as<int64_t>(),
+                     P.nblocks(), P.max_b, g->bj_inv.as<float>(), c->stream);
+    std::vector<float> f(total);
+    CP_HIP(hipMemcpyAsync(f.data(), g->bj_inv.p, total * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    CP_HIP(hipStreamSynchronize(c->stream));
+    std::vector<double> d(f.begin(), f.end());
+    bool ok = invert_blocks(P, d);
+    for (int64_t e = 0; e < total; e++) f[e] = (float)d[e];
+    if (ok) CP_HIP(hipMemcpyAsync(g->bj_inv.p, f.data(), total * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    CP_HIP(hipStreamSynchronize(c->stream));
+    g->bj_ready = true;
+    if (!ok) g->bj_plan.perm.clear();  // singular block: fall back to Jacobi
 }
 
 void fill_info(const double* r, const double T[9], cp_solve_info& inf) {
@@ -473,6 +506,7 @@ cp_status cp_assemble(cp_geom* g, cp_prec prec) {
                                        g->near_dB.as<double>(), c->stream);
             }
             g->have32 = true;
+            g->bj_ready = false;  // inverses follow the new A32
         }
         CP_HIP(hipStreamSynchronize(c->stream));
         return CP_OK;
@@ -557,6 +591,14 @@ cp_status cp_solve(cp_geom* g, cp_prec prec, int32_t k, const cp_freestream* fs,
             if (!g->batched || g->batched->capacity() < k) g->batched.reset(new BatchedSolver(n, lda, std::max(k, 1)));
             g->batched->set_near(g->near_rowptr.as<int>(), g->near_cols.as<int>(), g->near_dA.as<double>(),
                                  g->near_dB.as<double>());
+            if (opts->precond != 1) {
+                ensure_block_jacobi(g);
+            }
+            if (opts->precond != 1 && !g->bj_plan.perm.empty())
+                g->batched->set_block_jacobi(g->bj_perm.as<int>(), g->bj_ptr.as<int>(), g->bj_off.as<int64_t>(),
+                                             g->bj_inv.as<float>(), g->bj_plan.nblocks(), g->bj_plan.max_b);
+            else
+                g->batched->set_block_jacobi(nullptr, nullptr, nullptr, nullptr, 0, 0);
             BatchedOpts bo;
             bo.tol = tol;
             bo.max_iter = opts->max_iter > 0 ? opts->max_iter : 2 * n;

@@ -184,3 +184,21 @@ def test_batched_column_independent_of_batch(c1s):
     assert alone["CL"] == mix["CL"]
     again = pg.solve([target], CP_FP32_REFINE, **P4412)[0]
     assert np.array_equal(alone["mu"], again["mu"])
+
+
+def test_block_jacobi_same_solution_fewer_iterations(c1s):
+    """Strip block-Jacobi changes only the inner iteration count, not the
+    refined solution (both satisfy ||b - A x|| / ||b|| <= 1e-10 on the same
+    corrected FP32 system)."""
+    og, pg, A, B = c1s
+    pg.assemble(CP_FP32_REFINE)
+    conds = [(1.0, a, b) for a in (-4.0, 4.0, 10.0) for b in (-5.0, 5.0)]
+    say("[gpu] block jacobi vs jacobi k=%d" % len(conds))
+    rj = pg.solve(conds, CP_FP32_REFINE, precond=1, **P4412)
+    rb = pg.solve(conds, CP_FP32_REFINE, precond=0, **P4412)
+    say("[gpu] inner iterations jacobi %s block-jacobi %s" % ([r["iters"] for r in rj], [r["iters"] for r in rb]))
+    for a, b in zip(rj, rb):
+        assert b["rel_residual"] <= 1e-10
+        assert np.linalg.norm(a["mu"] - b["mu"]) / np.linalg.norm(a["mu"]) <= 1e-9
+        assert abs(a["CL"] - b["CL"]) <= 1e-9
+    assert sum(r["iters"] for r in rb) < sum(r["iters"] for r in rj)
