@@ -199,6 +199,7 @@ def test_block_jacobi_same_solution_fewer_iterations(c1s):
     say("[gpu] inner iterations jacobi %s block-jacobi %s" % ([r["iters"] for r in rj], [r["iters"] for r in rb]))
     for a, b in zip(rj, rb):
         assert b["rel_residual"] <= 1e-10
-        assert np.linalg.norm(a["mu"] - b["mu"]) / np.linalg.norm(a["mu"]) <= 1e-9
-        assert abs(a["CL"] - b["CL"]) <= 1e-9
+        # both stop at ||r||/||b|| <= 1e-10: solutions agree to cond(A) * 1e-10
+        assert np.linalg.norm(a["mu"] - b["mu"]) / np.linalg.norm(a["mu"]) <= 1e-7
+        assert abs(a["CL"] - b["CL"]) <= 1e-8
     assert sum(r["iters"] for r in rb) < sum(r["iters"] for r in rj)
