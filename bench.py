@@ -129,6 +129,8 @@ def main():
     log("mixed batched: %.1f ms/step (%d conditions); gemm_f32 avg %.3f ms x %d (%.1f TFLOP/s), resid_f64 avg %.3f"
         " ms x %d" % (1e3 * t_step, k, gemm_avg, mm_n, achieved, rs_ms / max(rs_n, 1), rs_n))
 
+    rep = g.solver_report()
+    log("solver: %s" % json.dumps(rep))
     dcl = None
     valid = k
     if fp64 is not None:
@@ -176,6 +178,8 @@ def main():
             "max_abs_dCL": float(dcl.max()) if dcl is not None else None,
             "resid_f64_avg_ms": rs_ms / max(rs_n, 1),
             "gemm_f32_avg_ms": gemm_avg,
+            "gemm_shape": rep.get("gemm_shape"),
+            "block_jacobi_blocks": rep.get("block_jacobi_blocks"),
         },
     }
     print(json.dumps(out), flush=True)

@@ -65,6 +65,7 @@ SIGNATURES = {
     "cp_solve": (C.c_int, [C.c_void_p, C.c_int, C.c_int32, C.POINTER(Freestream), C.POINTER(SolveOpts), _d, _d,
                            C.POINTER(SolveInfo)]),
     "cp_matvec": (C.c_int, [C.c_void_p, C.c_int, C.c_int32, _d, _d]),
+    "cp_solver_report": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int32]),
     "cp_timing_reset": (C.c_int, [C.c_void_p]),
     "cp_timing_enable": (C.c_int, [C.c_void_p, C.c_int]),
     "cp_timing_get": (C.c_int, [C.c_void_p, C.c_char_p, _d, _i64]),

@@ -146,6 +146,12 @@ class Geometry:
         self.ctx.check(abi.lib().cp_matvec(self.h, kernel, X.shape[1], _p(X), _p(Y)), "matvec")
         return Y
 
+    def solver_report(self):
+        import json
+        buf = C.create_string_buffer(1 << 16)
+        self.ctx.check(abi.lib().cp_solver_report(self.h, buf, len(buf)), "solver report")
+        return json.loads(buf.value.decode())
+
     def solve(self, conditions, prec=CP_FP64, mach=0.1, Sref=1.0, bref=1.0, cref=1.0, cg=(0, 0, 0), tol=1e-10,
               max_iter=0, max_refine=0, inner_tol=0.0, want_cp=True, allow_unconverged=False, precond=0):
         k = len(conditions)

@@ -22,6 +22,8 @@ struct BatchedOpts {
 // GEMM tile shape chosen by the autotuner: rt = 32-row tiles per wavefront,
 // kslices = split of the K range over workgroups
 struct GemmShape {
+    int variant = 0;  // 0: 4-byte A loads, rt tiles per wave; 1/2: 16-byte A loads, 4 interleaved
+                      // tiles, 16/8 K values in flight
     int rt = 2;
     int kslices = 8;
 };

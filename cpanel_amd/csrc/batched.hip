@@ -18,6 +18,8 @@
 // kslices; workgroup id -> (kslice = id % kslices, row block = id / kslices)
 // so with kslices a multiple of 8 every XCD works on one K slice and keeps
 // that slice of the multivector in its own L2.
+#include <cstdio>
+#include <cstdlib>
 #include <map>
 #include <mutex>
 
@@ -114,6 +116,74 @@ __global__ __launch_bounds__(256) void gemm_f32_kernel(const float* __restrict__
             const int64_t r = row0 + 32 * t + 8 * (j >> 2) + 4 * lk + (j & 3);
             out[r * KP + lr] = acc[t][0][j];
             out[r * KP + 32 + lr] = acc[t][1][j];
+        }
+}
+
+// Variant with 16-byte A loads: a wavefront owns 128 rows as 4 interleaved
+// 32-row tiles (tile t = rows 4m+t, m = 0..31).  Lane l loads the float4 of
+// rows 4(l%32)..4(l%32)+3 of column k + l/32; component t is that lane's A
+// operand for tile t.  One load instruction feeds 8 MFMAs (4 row tiles x 2
+// column tiles); the output rows are un-interleaved at the store.
+template <int U>
+__global__ __launch_bounds__(256) void gemm_f32_x4_kernel(const float* __restrict__ A, int64_t lda, int n,
+                                                          const float* __restrict__ P, float* __restrict__ part,
+                                                          int kchunk, int KS) {
+    const int bid = blockIdx.x;
+    const int ks = bid % KS, rb = bid / KS;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t row0 = ((int64_t)rb * 4 + wave) * 128;
+    const int k0 = ks * kchunk, k1 = min(n, k0 + kchunk);
+    const int m = lane & 31, kh = lane >> 5;
+    floatx16 acc[4][2];
+#pragma unroll
+    for (int t = 0; t < 4; t++) acc[t][0] = acc[t][1] = floatx16{};
+    const float* Ab = A + row0 + 4 * m;
+    const float* Pb = P + m;
+    float4 a[U], an[U];
+    float b[2][U], bn[2][U];
+    auto load = [&](int k, float4 (&aa)[U], float (&bb)[2][U]) {
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int64_t kk = k + 2 * u + kh;
+            const bool ok = kk < k1;
+            const int64_t kc = ok ? kk : 0;
+            float4 v = *reinterpret_cast<const float4*>(Ab + kc * lda);
+            aa[u] = ok ? v : make_float4(0.f, 0.f, 0.f, 0.f);
+            bb[0][u] = ok ? Pb[kc * KP] : 0.f;
+            bb[1][u] = ok ? Pb[kc * KP + 32] : 0.f;
+        }
+    };
+    if (k0 < k1) load(k0, a, b);
+    for (int k = k0; k < k1; k += 2 * U) {
+        const bool more = k + 2 * U < k1;
+        if (more) load(k + 2 * U, an, bn);
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const float av[4] = {a[u].x, a[u].y, a[u].z, a[u].w};
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                acc[t][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[t], b[0][u], acc[t][0], 0, 0, 0);
+                acc[t][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[t], b[1][u], acc[t][1], 0, 0, 0);
+            }
+        }
+        if (more) {
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                a[u] = an[u];
+                b[0][u] = bn[0][u];
+                b[1][u] = bn[1][u];
+            }
+        }
+    }
+    float* out = part + (int64_t)ks * lda * KP;
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+#pragma unroll
+        for (int j = 0; j < 16; j++) {
+            const int rho = 8 * (j >> 2) + 4 * kh + (j & 3);
+            const int64_t r = row0 + 4 * rho + t;
+            out[r * KP + m] = acc[t][0][j];
+            out[r * KP + 32 + m] = acc[t][1][j];
         }
 }
 
@@ -558,6 +628,17 @@ BatchedSolver::BatchedSolver(int n, int64_t lda, int capacity) : n_(n), cap_(cap
     int rb = (int)(lda / 256);
     shape_.rt = 2;
     shape_.kslices = rb >= 64 ? 8 : (rb >= 16 ? 16 : 32);
+    // CPGPU_GEMM_SHAPE="variant,rt,kslices" pins the shape (tests, tuning runs)
+    if (const char* e = std::getenv("CPGPU_GEMM_SHAPE")) {
+        int v = 0, r = 2, k = 8;
+        if (std::sscanf(e, "%d,%d,%d", &v, &r, &k) == 3 && v >= 0 && v <= 2 && (r == 1 || r == 2 || r == 4) &&
+            (k == 4 || k == 8 || k == 16 || k == 32)) {
+            shape_.variant = v;
+            shape_.rt = r;
+            shape_.kslices = k;
+            tuned_ = true;
+        }
+    }
 }
 
 BatchedSolver::~BatchedSolver() {
@@ -569,9 +650,18 @@ BatchedSolver::~BatchedSolver() {
 void BatchedSolver::launch_gemm32(const GemmShape& sh, const float* A32, const float* P, hipStream_t s) {
     const int KS = sh.kslices;
     int kchunk = (int)round_up((n_ + KS - 1) / KS, 16);
+    TimedScope ts("gemm_f32", s);
+    if (sh.variant >= 1) {  // 16-byte A loads, 512 rows per workgroup
+        dim3 grid((unsigned)((lda_ / 512) * KS));
+        if (sh.variant == 1)
+            hipLaunchKernelGGL(gemm_f32_x4_kernel<8>, grid, dim3(256), 0, s, A32, lda_, n_, P, part32_, kchunk, KS);
+        else
+            hipLaunchKernelGGL(gemm_f32_x4_kernel<4>, grid, dim3(256), 0, s, A32, lda_, n_, P, part32_, kchunk, KS);
+        CP_LAUNCH_CHECK();
+        return;
+    }
     const int rows_per_block = 4 * 32 * sh.rt;
     dim3 grid((unsigned)((lda_ / rows_per_block) * KS));
-    TimedScope ts("gemm_f32", s);
     switch (sh.rt) {
         case 1: hipLaunchKernelGGL(gemm_f32_kernel<1>, grid, dim3(256), 0, s, A32, lda_, n_, P, part32_, kchunk, KS); break;
         case 2: hipLaunchKernelGGL(gemm_f32_kernel<2>, grid, dim3(256), 0, s, A32, lda_, n_, P, part32_, kchunk, KS); break;
@@ -601,6 +691,14 @@ void BatchedSolver::tune(const float* A32, hipStream_t s) {
             if (lda_ % (128 * rt) != 0) continue;
             GemmShape g;
             g.rt = rt;
+            g.kslices = ks;
+            cand.push_back(g);
+        }
+    for (int v : {1, 2})
+        for (int ks : {4, 8, 16, 32}) {
+            GemmShape g;
+            g.variant = v;
+            g.rt = 4;
             g.kslices = ks;
             cand.push_back(g);
         }

@@ -705,6 +705,29 @@ cp_status cp_matvec(cp_geom* g, int kernel, int32_t k, const double* X, double* 
     });
 }
 
+cp_status cp_solver_report(cp_geom* g, char* buf, int32_t len) {
+    if (!g || !buf || len <= 0) return CP_BAD_ARG;
+    std::string r = "{";
+    if (g->batched) {
+        const GemmShape& sh = g->batched->shape();
+        r += "\"gemm_shape\": {\"variant\": " + std::to_string(sh.variant) + ", \"rt\": " + std::to_string(sh.rt) +
+             ", \"kslices\": " + std::to_string(sh.kslices) + "}, \"tune\": [";
+        bool first = true;
+        for (const TuneRecord& t : g->batched->tune_log()) {
+            if (!first) r += ", ";
+            first = false;
+            r += "[" + std::to_string(t.shape.variant) + ", " + std::to_string(t.shape.rt) + ", " +
+                 std::to_string(t.shape.kslices) + ", " + std::to_string(t.ms) + "]";
+        }
+        r += "], ";
+    }
+    r += "\"block_jacobi_blocks\": " + std::to_string(g->bj_plan.perm.empty() ? 0 : g->bj_plan.nblocks()) +
+         ", \"block_jacobi_max\": " + std::to_string(g->bj_plan.max_b) + ", \"near_nnz\": " +
+         std::to_string(g->near_nnz) + "}";
+    std::snprintf(buf, (size_t)len, "%s", r.c_str());
+    return (int32_t)r.size() < len ? CP_OK : CP_BAD_ARG;
+}
+
 cp_status cp_timing_reset(cp_ctx* c) {
     if (!c) return CP_BAD_ARG;
     return guarded(c, [&] {

@@ -140,6 +140,10 @@ cp_status cp_solve(cp_geom* g, cp_prec prec, int32_t k, const cp_freestream* fs,
  * X, Y: host, n_body x k column-major. */
 cp_status cp_matvec(cp_geom* g, int kernel, int32_t k, const double* X, double* Y);
 
+/* JSON text describing the batched solver of a geometry: tuned GEMM shape
+ * and the autotuner's timings, block-Jacobi blocks, near-field entries. */
+cp_status cp_solver_report(cp_geom* g, char* buf, int32_t len);
+
 /* ---- timing: HIP-event totals of the named kernel on the context stream -- */
 cp_status cp_timing_reset(cp_ctx* ctx);
 /* kernel: "assemble_f64", "assemble_f32", "gemv_f64", "gemm_f32", "resid_f64", ...

@@ -203,3 +203,22 @@ def test_block_jacobi_same_solution_fewer_iterations(c1s):
         assert np.linalg.norm(a["mu"] - b["mu"]) / np.linalg.norm(a["mu"]) <= 1e-7
         assert abs(a["CL"] - b["CL"]) <= 1e-8
     assert sum(r["iters"] for r in rb) < sum(r["iters"] for r in rj)
+
+
+@pytest.mark.parametrize("shape", ["0,1,8", "0,2,16", "0,4,4", "1,4,8", "2,4,32"])
+def test_gemm_shapes(ctx, shape, monkeypatch):
+    """Every GEMM variant/shape the autotuner can pick gives the FP32 product
+    (FP32 accumulation bound as in test_matvec_kernels)."""
+    monkeypatch.setenv("CPGPU_GEMM_SHAPE", shape)
+    m = meshgen.wing("4412", 1.0, 6.0, 40, 12, "uniform", 20.0)
+    g = ctx.geometry(m)
+    g.assemble(CP_FP32_REFINE)
+    A32, _ = g.matrices(CP_FP32_REFINE)
+    say("[gpu] gemm shape %s" % shape)
+    X = np.random.default_rng(9).normal(size=(g.nb, 64))
+    Y = g.matvec(X, 1)
+    ref = A32 @ X.astype(np.float32).astype(np.float64)
+    scale = np.abs(A32) @ np.abs(X)
+    assert np.all(np.abs(Y - ref) <= 4e-6 * scale + 1e-30)
+    rep = g.solver_report()
+    assert rep["gemm_shape"] == dict(zip(("variant", "rt", "kslices"), map(int, shape.split(","))))
