@@ -62,6 +62,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 comparison path")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="run one elementwise copy of 256 MiB first (HBM counter calibration under a profiler)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only (this build has no multi-GPU path); got --gpus %d" % args.gpus,
@@ -76,6 +78,15 @@ def main():
     kw = dict(mach=0.1, Sref=span * 1.0, bref=span, cref=1.0, cg=(0.25, 0.0, 0.0))
     conds = conditions()
     k = len(conds)
+
+    if args.copy_calib:
+        # reads 256 MiB and writes 256 MiB: the FETCH_SIZE / WRITE_SIZE scale of a plain stream
+        import torch
+        x = torch.ones(64 << 20, dtype=torch.float32, device="cuda:0")
+        y = torch.empty_like(x)
+        torch.neg(x, out=y)
+        torch.cuda.synchronize()
+        del x, y
 
     ctx = Context(0)
     t0 = time.perf_counter()

@@ -87,6 +87,7 @@ private:
     void launch_gemm32(const GemmShape& sh, const float* A32, const float* P, hipStream_t s);
     void apply_block_jacobi(const float* P, float* Y, hipStream_t s);
     bool bj_ = false;
+    bool bj_mfma_ = true;  // matrix-core block product (else vector ALU)
     const int* bj_perm_ = nullptr;
     const int* bj_ptr_ = nullptr;
     const int64_t* bj_off_ = nullptr;

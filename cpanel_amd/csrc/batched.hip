@@ -273,6 +273,7 @@ __global__ __launch_bounds__(256) void reduce64_kernel(const double* __restrict_
         int64_t i = (int64_t)blockIdx.x * kRedRows + rr;
         int64_t o = i * KP + c;
         double s = 0;
+#pragma unroll 8
         for (int q = 0; q < KS; q++) s += part[(int64_t)q * lda * KP + o];
         if (vals && i < n)
             for (int e = rowptr[i]; e < rowptr[i + 1]; e++) s += vals[e] * X[(int64_t)cols[e] * KP + c];
@@ -290,6 +291,7 @@ __global__ __launch_bounds__(256) void reduce32_kernel(const float* __restrict__
     int64_t idx = (int64_t)blockIdx.x * kT + threadIdx.x;
     if (idx >= lda * KP) return;
     float s = 0;
+#pragma unroll 8
     for (int q = 0; q < KS; q++) s += part[(int64_t)q * lda * KP + idx];
     Y[idx] = (idx / KP) < n ? s : 0.f;
 }
@@ -431,6 +433,7 @@ __global__ __launch_bounds__(256) void inner_v_kernel(const float* __restrict__ 
             int64_t i = (int64_t)blockIdx.x * kRedRows + rr;
             int64_t o = i * KP + c;
             float s = 0;
+#pragma unroll 8
             for (int q = 0; q < KS; q++) s += part[(int64_t)q * lda * KP + o];
             if (i >= n) s = 0;
             v[o] = s;
@@ -468,6 +471,7 @@ __global__ __launch_bounds__(256) void inner_t_kernel(const float* __restrict__ 
             int64_t i = (int64_t)blockIdx.x * kRedRows + rr;
             int64_t o = i * KP + c;
             float x = 0;
+#pragma unroll 8
             for (int q = 0; q < KS; q++) x += part[(int64_t)q * lda * KP + o];
             if (i >= n) x = 0;
             t[o] = x;
@@ -584,6 +588,47 @@ __global__ __launch_bounds__(256) void bj_apply_kernel(const int* __restrict__ p
     }
 }
 
+// Matrix-core form: Y_q = inv_q * P_q as a 32x32x2 FP32 MFMA product per
+// wavefront (32 block rows x 32 multivector columns), the K range = the block.
+// Workgroup (x, q): rows [64x, 64x + 64) of block q, all 64 columns; wave w
+// takes row tile w/2, column tile w%2.  Rows and K values past the block end
+// load zeros and are not stored.
+__global__ __launch_bounds__(256) void bj_apply_mfma_kernel(const int* __restrict__ perm, const int* __restrict__ ptr,
+                                                            const int64_t* __restrict__ off,
+                                                            const float* __restrict__ inv, const float* __restrict__ P,
+                                                            float* __restrict__ Y) {
+    constexpr int U = 8;  // K steps of 2 per chunk
+    const int q = blockIdx.y;
+    const int b0 = ptr[q], b = ptr[q + 1] - b0;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int r0 = blockIdx.x * 64 + 32 * (wave >> 1);
+    if (r0 >= b) return;  // no LDS, no barrier: idle waves leave
+    const int c = 32 * (wave & 1) + (lane & 31);
+    const int kh = lane >> 5;
+    const int r = r0 + (lane & 31);
+    const bool rok = r < b;
+    const float* M = inv + off[q];
+    const int* pq = perm + b0;
+    floatx16 acc = floatx16{};
+    for (int k = 0; k < b; k += 2 * U) {
+        float a[U], x[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int kk = k + 2 * u + kh;
+            const bool ok = kk < b;
+            a[u] = (ok && rok) ? M[(int64_t)kk * b + r] : 0.f;
+            x[u] = ok ? P[(int64_t)pq[kk] * KP + c] : 0.f;
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a[u], x[u], acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+        const int rr = r0 + 8 * (j >> 2) + 4 * kh + (j & 3);
+        if (rr < b) Y[(int64_t)pq[rr] * KP + c] = acc[j];
+    }
+}
+
 inline unsigned nblocks(int64_t work) { return (unsigned)((work + kT - 1) / kT); }
 
 }  // namespace
@@ -628,6 +673,8 @@ BatchedSolver::BatchedSolver(int n, int64_t lda, int capacity) : n_(n), cap_(cap
     int rb = (int)(lda / 256);
     shape_.rt = 2;
     shape_.kslices = rb >= 64 ? 8 : (rb >= 16 ? 16 : 32);
+    // CPGPU_BJ_VALU=1 selects the vector-ALU block-Jacobi product (tests)
+    if (const char* e = std::getenv("CPGPU_BJ_VALU")) bj_mfma_ = e[0] != '1';
     // CPGPU_GEMM_SHAPE="variant,rt,kslices" pins the shape (tests, tuning runs)
     if (const char* e = std::getenv("CPGPU_GEMM_SHAPE")) {
         int v = 0, r = 2, k = 8;
@@ -709,10 +756,17 @@ void BatchedSolver::tune(const float* A32, hipStream_t s) {
     set_current_timer(nullptr);
     float best = 1e30f;
     GemmShape bestg = shape_;
+    // each candidate is timed with the split-K reduction its K slices imply
+    auto run = [&](const GemmShape& g) {
+        launch_gemm32(g, A32, p_, s);
+        hipLaunchKernelGGL(reduce32_kernel, dim3(nblocks(lda_ * KP)), dim3(kT), 0, s, (const float*)part32_,
+                           g.kslices, lda_, n_, v_);
+        CP_LAUNCH_CHECK();
+    };
     for (const GemmShape& g : cand) {
-        launch_gemm32(g, A32, p_, s);  // warm
+        run(g);  // warm
         CP_HIP(hipEventRecord(e0, s));
-        for (int r = 0; r < 3; r++) launch_gemm32(g, A32, p_, s);
+        for (int r = 0; r < 3; r++) run(g);
         CP_HIP(hipEventRecord(e1, s));
         CP_HIP(hipEventSynchronize(e1));
         float ms = 0;
@@ -743,9 +797,13 @@ void BatchedSolver::matvec32(const float* A32, const float* P, float* Y, hipStre
 
 void BatchedSolver::matvec64acc(const float* A32, const double* X, double* Y, const double* Badd, double sign,
                                 hipStream_t s, const double* corr) {
-    const int KS = shape_.kslices;
+    // K slices: enough workgroups for two per CU, no more (each slice adds an
+    // lda x 64 FP64 partial to write and reduce)
+    const int rb = (int)(lda_ / 256);
+    int KS = 4;
+    while (KS < 32 && rb * KS < 512) KS *= 2;
     int kchunk = (int)round_up((n_ + KS - 1) / KS, 16);
-    dim3 grid((unsigned)((lda_ / 256) * KS));
+    dim3 grid((unsigned)(rb * KS));
     {
         TimedScope ts("resid_f64", s);
         hipLaunchKernelGGL(gemm_f64acc_kernel, grid, dim3(256), 0, s, A32, lda_, n_, X, part64_, kchunk, KS);
@@ -759,9 +817,14 @@ void BatchedSolver::matvec64acc(const float* A32, const double* X, double* Y, co
 
 void BatchedSolver::apply_block_jacobi(const float* P, float* Y, hipStream_t s) {
     TimedScope ts("bj_apply", s);
-    dim3 grid((unsigned)bj_nblocks_, (unsigned)(KP / kBjCols));
-    hipLaunchKernelGGL(bj_apply_kernel, grid, dim3(kT), (size_t)bj_maxb_ * kBjCols * sizeof(float), s, bj_perm_,
-                       bj_ptr_, bj_off_, bj_inv_, P, Y);
+    if (bj_mfma_) {
+        dim3 grid((unsigned)((bj_maxb_ + 63) / 64), (unsigned)bj_nblocks_);
+        hipLaunchKernelGGL(bj_apply_mfma_kernel, grid, dim3(kT), 0, s, bj_perm_, bj_ptr_, bj_off_, bj_inv_, P, Y);
+    } else {
+        dim3 grid((unsigned)bj_nblocks_, (unsigned)(KP / kBjCols));
+        hipLaunchKernelGGL(bj_apply_kernel, grid, dim3(kT), (size_t)bj_maxb_ * kBjCols * sizeof(float), s, bj_perm_,
+                           bj_ptr_, bj_off_, bj_inv_, P, Y);
+    }
     CP_LAUNCH_CHECK();
 }
 
@@ -859,8 +922,11 @@ void BatchedSolver::tile_solve(const float* A32, const float* B32, const double*
                                y_);
             if (bj_) apply_block_jacobi(p_, y_, s);
             matvec32(A32, y_, nullptr, s);
-            hipLaunchKernelGGL(inner_v_kernel, dim3((unsigned)nrb_), dim3(kT), 0, s, (const float*)part32_, KS, lda_,
-                               n_, (const int*)act_, v_, (const float*)r0_, dpart_);
+            {
+                TimedScope tr("reduce_f32", s);
+                hipLaunchKernelGGL(inner_v_kernel, dim3((unsigned)nrb_), dim3(kT), 0, s, (const float*)part32_, KS,
+                                   lda_, n_, (const int*)act_, v_, (const float*)r0_, dpart_);
+            }
             colsum(dpart_, 1, Q_R0V, 0);
             hipLaunchKernelGGL(scal_alpha_kernel, dim3(1), dim3(KP), 0, s, sc_, (const int*)act_);
             hipLaunchKernelGGL(inner_s_kernel, dim3(nel), dim3(kT), 0, s, rows, (const double*)sc_, (const int*)act_,
@@ -868,8 +934,12 @@ void BatchedSolver::tile_solve(const float* A32, const float* B32, const double*
                                sv_, z_);
             if (bj_) apply_block_jacobi(sv_, z_, s);
             matvec32(A32, z_, nullptr, s);
-            hipLaunchKernelGGL(inner_t_kernel, dim3((unsigned)nrb_), dim3(kT), 0, s, (const float*)part32_, KS, lda_,
-                               n_, (const int*)act_, t_, (const float*)sv_, dpart_, dpart_ + (size_t)nrb_ * KP);
+            {
+                TimedScope tr("reduce_f32", s);
+                hipLaunchKernelGGL(inner_t_kernel, dim3((unsigned)nrb_), dim3(kT), 0, s, (const float*)part32_, KS,
+                                   lda_, n_, (const int*)act_, t_, (const float*)sv_, dpart_,
+                                   dpart_ + (size_t)nrb_ * KP);
+            }
             colsum(dpart_, 2, Q_TT, Q_TS);
             hipLaunchKernelGGL(scal_omega_kernel, dim3(1), dim3(KP), 0, s, sc_, (const int*)act_);
             hipLaunchKernelGGL(inner_x_kernel, dim3((unsigned)nrb_), dim3(kT), 0, s, n_, (const double*)sc_,

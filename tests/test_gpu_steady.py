@@ -222,3 +222,23 @@ def test_gemm_shapes(ctx, shape, monkeypatch):
     assert np.all(np.abs(Y - ref) <= 4e-6 * scale + 1e-30)
     rep = g.solver_report()
     assert rep["gemm_shape"] == dict(zip(("variant", "rt", "kslices"), map(int, shape.split(","))))
+
+
+def test_block_jacobi_mfma_matches_valu(ctx, monkeypatch):
+    """The matrix-core block-Jacobi product and the vector-ALU one give the
+    same refined solutions (both FP32 products of the same inverses)."""
+    m = meshgen.wing("4412", 1.0, 6.0, 40, 12, "uniform", 20.0)
+    conds = [(1.0, a, b) for a in (-4.0, 6.0) for b in (-5.0, 2.5)]
+    kw = dict(mach=0.1, Sref=6.0, bref=6.0, cref=1.0, cg=(0.25, 0.0, 0.0), tol=1e-12)
+    say("[gpu] block jacobi mfma vs valu")
+    g1 = ctx.geometry(m)
+    g1.assemble(CP_FP32_REFINE)
+    r1 = g1.solve(conds, CP_FP32_REFINE, **kw)
+    monkeypatch.setenv("CPGPU_BJ_VALU", "1")
+    g2 = ctx.geometry(m)
+    g2.assemble(CP_FP32_REFINE)
+    r2 = g2.solve(conds, CP_FP32_REFINE, **kw)
+    for a, b in zip(r1, r2):
+        assert a["rel_residual"] <= 1e-12 and b["rel_residual"] <= 1e-12
+        assert np.linalg.norm(a["mu"] - b["mu"]) / np.linalg.norm(a["mu"]) <= 1e-9
+        assert abs(a["CL"] - b["CL"]) <= 1e-10
