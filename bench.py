@@ -35,6 +35,29 @@ def conditions():
     return [(1.0, float(a), float(b)) for b in (-5.0, 0.0, 2.5, 5.0) for a in np.linspace(-4, 10, 16)]
 
 
+def gemm_kernel_name(shape):
+    """Kernel symbol of the autotuned FP32 GEMM shape (as the profiler names it)."""
+    if shape is None:
+        return None
+    if shape["variant"] == 1:
+        return "gemm_f32_x4_kernel<8>"
+    if shape["variant"] == 2:
+        return "gemm_f32_x4_kernel<4>"
+    return "gemm_f32_kernel<%d>" % shape["rt"]
+
+
+def profiled_traffic(workload, kernel):
+    """HBM bytes per dispatch of `kernel` from the committed rocprofv3 --pmc
+    summary of this workload (FETCH_SIZE + WRITE_SIZE, copy-calibrated)."""
+    path = os.path.join(ROOT, "profiles", workload + ".traffic.json")
+    try:
+        with open(path) as f:
+            k = json.load(f)["kernels"].get(kernel)
+    except (OSError, ValueError):
+        return None
+    return None if k is None else k["read_bytes"] + k["write_bytes"]
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -142,6 +165,7 @@ def main():
 
     rep = g.solver_report()
     log("solver: %s" % json.dumps(rep))
+    kname = gemm_kernel_name(rep.get("gemm_shape"))
     dcl = None
     valid = k
     if fp64 is not None:
@@ -178,8 +202,10 @@ def main():
                    "n_body": n, "conditions": k, "precision": "FP32 A,B + FP32 BiCGSTAB + FP64 refinement",
                    "tol": 1e-10},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": MI355X_FP32_MFMA_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / MI355X_FP32_MFMA_TFLOPS, "traffic": None,
-                     "kernel": "gemm_f32 (A32 x 64-column multivector)"},
+                     "frac": achieved / MI355X_FP32_MFMA_TFLOPS,
+                     "traffic": (profiled_traffic("mixed_solve_naca0012_ar10_64cond", kname)
+                                 if not args.small else None),
+                     "kernel": "%s (A32 x 64-column multivector, %.0f MB of A per launch)" % (kname, 4.0 * n * n / 1e6)},
         "cpu_baseline": cpu,
         "extra": {
             "assembly_ginfl_per_s_fp64": pairs / t_a64 / 1e6, "assembly_ms_fp64": t_a64,
