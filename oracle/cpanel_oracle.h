@@ -108,6 +108,28 @@ void or_case_post(or_geom* g, const or_case_params* p, const double* mu_in,
  * (geometry.cpp:1229-1237), with the wake mu left by the last case run. */
 double or_wake_potential(const or_geom* g, const double* pnt);
 
+/* ---- vortex-particle kernels (vp_oracle.cpp) ---------------------------- */
+/* particle octree (octree.h:39-195, node.h:30-214), max members per leaf and
+ * Barnes-Hut opening angle theta (octree.h:37); pseudo-particles as
+ * particleFMM::computeMultExp (particleFMM.cpp:29-58) */
+void* or_bh_build(int n, const double* xyz, const double* alpha, const double* radius,
+                  int max_members, double theta);
+void or_bh_destroy(void* tree);
+void or_bh_info(void* tree, int* n_nodes, int* levels, double* origin, double* half);
+/* mode 0: velocity at points (algebraic kernel), 1: velocity on particles
+ * (Gaussian), 2: stretching, 3: diffusion (particleFMM.cpp:65-116) */
+void or_bh_eval(void* tree, int mode, int m, const double* pts, double* out);
+/* direct sums with the same kernels; mode 4: Gaussian velocity at points */
+void or_vp_direct(int mode, int n, const double* xyz, const double* alpha, const double* radius,
+                  int m, const double* pts, double* out);
+void or_vp_pair(int mode, const double* xs, const double* as, double rs, const double* xt,
+                const double* at, double rt, double* out);
+/* panel -> point velocity of body (sigma, mu) and wake (mu_w) panels
+ * (bodyPanel::panelV bodyPanel.cpp:122-130, wakePanel::panelV wakePanel.cpp:89-92) */
+void or_panels_to_points(int nb, const int* nvb, const double* vb, const double* sigma,
+                         const double* mu, int nw, const int* nvw, const double* vw,
+                         const double* mu_w, int m, const double* pts, double* out);
+
 #ifdef __cplusplus
 }
 #endif
