@@ -45,6 +45,12 @@ class SolveInfo(C.Structure):
                 ("refinements", C.c_int32), ("converged", C.c_int32), ("pad", C.c_int32)]
 
 
+class BhOpts(C.Structure):
+    _fields_ = [("max_members", C.c_int32), ("pad", C.c_int32), ("theta", C.c_double)]
+
+
+CP_BH_VEL_POINTS, CP_BH_VEL, CP_BH_STRETCH, CP_BH_DIFF = 0, 1, 2, 3
+
 # Every symbol declared in include/cpgpu.h with (restype, argtypes).
 SIGNATURES = {
     "cp_create": (C.c_int, [C.c_int, C.c_uint64, C.POINTER(C.c_void_p)]),
@@ -66,6 +72,11 @@ SIGNATURES = {
                            C.POINTER(SolveInfo)]),
     "cp_matvec": (C.c_int, [C.c_void_p, C.c_int, C.c_int32, _d, _d]),
     "cp_solver_report": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int32]),
+    "cp_bh_build": (C.c_int, [C.c_void_p, C.c_int32, _d, _d, _d, C.POINTER(BhOpts), C.POINTER(C.c_void_p)]),
+    "cp_bh_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), _d, _d]),
+    "cp_bh_eval": (C.c_int, [C.c_void_p, C.c_int, C.c_int32, _d, _d]),
+    "cp_bh_release": (None, [C.c_void_p]),
+    "cp_panel_to_points": (C.c_int, [C.c_void_p, _d, _d, _d, C.c_int32, _d, _d]),
     "cp_timing_reset": (C.c_int, [C.c_void_p]),
     "cp_timing_enable": (C.c_int, [C.c_void_p, C.c_int]),
     "cp_timing_get": (C.c_int, [C.c_void_p, C.c_char_p, _d, _i64]),

@@ -54,6 +54,9 @@ class Context:
     def geometry(self, mesh, key=0):
         return Geometry(self, mesh, key)
 
+    def bh_tree(self, xyz, alpha, radius, max_members=10, theta=0.5):
+        return BHTree(self, xyz, alpha, radius, max_members, theta)
+
     def bytes_in_use(self):
         return abi.lib().cp_bytes_in_use(self.h)
 
@@ -146,6 +149,22 @@ class Geometry:
         self.ctx.check(abi.lib().cp_matvec(self.h, kernel, X.shape[1], _p(X), _p(Y)), "matvec")
         return Y
 
+    def panel_to_points(self, sigma, mu, pts, mu_wake=None):
+        """Velocity induced at pts (m x 3) by the body panels (source sigma,
+        doublet mu, solve order) and the wake panels (doublet mu_wake)."""
+        sg = np.ascontiguousarray(sigma, dtype=np.float64)
+        m_ = np.ascontiguousarray(mu, dtype=np.float64)
+        if sg.shape != (self.nb,) or m_.shape != (self.nb,):
+            raise ValueError("sigma and mu need one value per body panel")
+        mw = None if mu_wake is None else np.ascontiguousarray(mu_wake, dtype=np.float64)
+        if mw is not None and mw.shape != (self.nw,):
+            raise ValueError("mu_wake needs one value per wake panel")
+        q = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 3)
+        out = np.zeros_like(q)
+        self.ctx.check(abi.lib().cp_panel_to_points(self.h, _p(sg), _p(m_), _p(mw), len(q), _p(q), _p(out)),
+                       "cp_panel_to_points")
+        return out
+
     def solver_report(self):
         import json
         buf = C.create_string_buffer(1 << 16)
@@ -174,3 +193,51 @@ class Geometry:
                             rel_residual=r.rel_residual, iters=r.iters, refinements=r.refinements,
                             converged=bool(r.converged)))
         return out
+
+
+class BHTree:
+    """Particle octree with pseudo-particles on the device
+    (particleFMM::build, particleFMM.cpp:22-27); eval() runs the Barnes-Hut
+    sums of particleFMM::barnesHutVel/Stretch/Diff (particleFMM.cpp:65-116)."""
+
+    def __init__(self, ctx, xyz, alpha, radius, max_members=10, theta=0.5):
+        self.ctx = ctx
+        self.xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+        self.alpha = np.ascontiguousarray(alpha, dtype=np.float64).reshape(-1, 3)
+        self.radius = np.ascontiguousarray(radius, dtype=np.float64).reshape(-1)
+        self.n = len(self.radius)
+        if self.xyz.shape[0] != self.n or self.alpha.shape[0] != self.n:
+            raise ValueError("xyz, alpha and radius must describe the same particles")
+        h = C.c_void_p()
+        o = abi.BhOpts(max_members, 0, theta)
+        ctx.check(abi.lib().cp_bh_build(ctx.h, self.n, _p(self.xyz), _p(self.alpha), _p(self.radius), C.byref(o),
+                                        C.byref(h)), "cp_bh_build")
+        self.h = h
+
+    def info(self):
+        nn, lv = C.c_int32(), C.c_int32()
+        o, h = np.zeros(3), C.c_double()
+        abi.lib().cp_bh_info(self.h, C.byref(nn), C.byref(lv), _p(o), C.byref(h))
+        return dict(n_nodes=nn.value, levels=lv.value, origin=o, half=np.full(3, h.value))
+
+    def eval(self, mode, pts=None):
+        if mode == abi.CP_BH_VEL_POINTS:
+            q = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 3)
+            out = np.zeros_like(q)
+            self.ctx.check(abi.lib().cp_bh_eval(self.h, mode, len(q), _p(q), _p(out)), "cp_bh_eval")
+        else:
+            out = np.zeros((self.n, 3))
+            self.ctx.check(abi.lib().cp_bh_eval(self.h, mode, 0, None, _p(out)), "cp_bh_eval")
+        return out
+
+    def close(self):
+        if getattr(self, "h", None):
+            abi.lib().cp_bh_release(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+

@@ -23,6 +23,8 @@
 #include "post.h"
 #include "precond.h"
 #include "timer.h"
+#include "vp.h"
+#include "vp_tree.h"
 
 using namespace cpgpu;
 
@@ -756,3 +758,126 @@ cp_status cp_timing_get(cp_ctx* c, const char* kernel, double* total_ms, int64_t
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------- VP kernels
+struct cp_tree {
+    cp_ctx* ctx = nullptr;
+    BhTreeHost host;
+    DevBhTree dev;
+    DevBuf nodes, mpos, mstr, mrad, order;
+    double theta = 0.5;
+};
+
+cp_status cp_bh_build(cp_ctx* c, int32_t n, const double* xyz, const double* alpha, const double* radius,
+                      const cp_bh_opts* opts, cp_tree** out) {
+    if (!c || !out || n < 0 || (n > 0 && (!xyz || !alpha || !radius))) return CP_BAD_ARG;
+    *out = nullptr;
+    int maxm = (opts && opts->max_members > 0) ? opts->max_members : 10;
+    double theta = (opts && opts->theta >= 0) ? opts->theta : 0.5;
+    std::unique_ptr<cp_tree> t(new cp_tree);
+    t->ctx = c;
+    t->theta = theta;
+    cp_status st = guarded(c, [&]() -> cp_status {
+        build_bh_tree(n, xyz, alpha, radius, maxm, t->host);
+        t->dev.n = n;
+        t->dev.n_nodes = (int)t->host.nodes.size();
+        if (c->device < 0) return CP_OK;  // host-only context: tree queries only
+        upload(c, t->nodes, t->host.nodes);
+        upload(c, t->mpos, t->host.mpos);
+        upload(c, t->mstr, t->host.mstr);
+        upload(c, t->mrad, t->host.mrad);
+        upload(c, t->order, t->host.order);
+        t->dev.n = n;
+        t->dev.n_nodes = (int)t->host.nodes.size();
+        t->dev.nodes = t->nodes.as<BhNode>();
+        t->dev.mpos = t->mpos.as<double>();
+        t->dev.mstr = t->mstr.as<double>();
+        t->dev.mrad = t->mrad.as<double>();
+        t->dev.order = t->order.as<int>();
+        return CP_OK;
+    });
+    if (st == CP_OK) *out = t.release();
+    return st;
+}
+
+cp_status cp_bh_info(const cp_tree* t, int32_t* n_nodes, int32_t* levels, double* origin, double* half) {
+    if (!t) return CP_BAD_ARG;
+    if (n_nodes) *n_nodes = (int32_t)t->host.nodes.size();
+    if (levels) *levels = t->host.levels;
+    if (origin)
+        for (int d = 0; d < 3; d++) origin[d] = t->host.origin[d];
+    if (half) *half = t->host.half;
+    return CP_OK;
+}
+
+cp_status cp_bh_eval(cp_tree* t, cp_bh_mode mode, int32_t m, const double* pts, double* out) {
+    if (!t || !out || mode < CP_BH_VEL_POINTS || mode > CP_BH_DIFF) return CP_BAD_ARG;
+    if (mode == CP_BH_VEL_POINTS && (m < 0 || (m > 0 && !pts))) return CP_BAD_ARG;
+    cp_ctx* c = t->ctx;
+    if (c->device < 0) return fail(c, CP_NO_DEVICE, "host-only context");
+    return guarded(c, [&]() -> cp_status {
+        hipStream_t s = c->stream;
+        const int nt = mode == CP_BH_VEL_POINTS ? m : t->dev.n;
+        if (nt == 0) return CP_OK;
+        DevBuf dout, dpts;
+        dalloc(c, dout, (size_t)3 * nt * sizeof(double));
+        if (t->dev.n_nodes <= 1) {
+            // root alone (a leaf, or no particles): the sums start at the
+            // root's children, so every result is zero
+            std::memset(out, 0, (size_t)3 * nt * sizeof(double));
+            return CP_OK;
+        }
+        if (mode == CP_BH_VEL_POINTS) {
+            dalloc(c, dpts, (size_t)3 * m * sizeof(double));
+            CP_HIP(hipMemcpyAsync(dpts.p, pts, (size_t)3 * m * sizeof(double), hipMemcpyHostToDevice, s));
+            TimedScope ts("bh_eval", s);
+            launch_bh_points(t->dev, t->theta, m, dpts.as<double>(), dout.as<double>(), s);
+        } else {
+            TimedScope ts("bh_eval", s);
+            launch_bh_particles(t->dev, (int)mode, t->theta, dout.as<double>(), s);
+        }
+        CP_HIP(hipMemcpyAsync(out, dout.p, (size_t)3 * nt * sizeof(double), hipMemcpyDeviceToHost, s));
+        CP_HIP(hipStreamSynchronize(s));
+        return CP_OK;
+    });
+}
+
+void cp_bh_release(cp_tree* t) {
+    if (!t) return;
+    cp_ctx* c = t->ctx;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (c->device >= 0) (void)hipSetDevice(c->device);
+    delete t;
+}
+
+cp_status cp_panel_to_points(cp_geom* g, const double* sigma, const double* mu, const double* mu_wake, int32_t m,
+                             const double* pts, double* vel) {
+    if (!g || !sigma || !mu || m < 0 || (m > 0 && (!pts || !vel))) return CP_BAD_ARG;
+    cp_ctx* c = g->ctx;
+    if (c->device < 0) return fail(c, CP_NO_DEVICE, "host-only context");
+    if (m == 0) return CP_OK;
+    return guarded(c, [&]() -> cp_status {
+        hipStream_t s = c->stream;
+        const int nb = g->dev.n, nw = g->dev.nw;
+        std::vector<double> str((size_t)nb * 2 + std::max(nw, 1), 0.0);
+        std::memcpy(str.data(), sigma, nb * sizeof(double));
+        std::memcpy(str.data() + nb, mu, nb * sizeof(double));
+        if (mu_wake && nw > 0) std::memcpy(str.data() + 2 * nb, mu_wake, nw * sizeof(double));
+        const int KS = p2p_slices(m, nb + nw);
+        DevBuf dstr, dpts, dpart, dout;
+        upload(c, dstr, str);
+        dalloc(c, dpts, (size_t)3 * m * sizeof(double));
+        dalloc(c, dpart, (size_t)3 * m * KS * sizeof(double));
+        dalloc(c, dout, (size_t)3 * m * sizeof(double));
+        CP_HIP(hipMemcpyAsync(dpts.p, pts, (size_t)3 * m * sizeof(double), hipMemcpyHostToDevice, s));
+        {
+            TimedScope ts("panel_to_points", s);
+            launch_panel_to_points(g->dev, dstr.as<double>(), dstr.as<double>() + nb, dstr.as<double>() + 2 * nb, m,
+                                   dpts.as<double>(), KS, dpart.as<double>(), dout.as<double>(), s);
+        }
+        CP_HIP(hipMemcpyAsync(vel, dout.p, (size_t)3 * m * sizeof(double), hipMemcpyDeviceToHost, s));
+        CP_HIP(hipStreamSynchronize(s));
+        return CP_OK;
+    });
+}
+

@@ -159,4 +159,101 @@ CP_HD double dub_phi_inf_f64(const PanelRec& P, double px, double py, double pz)
     return phi / (4 * kPI);
 }
 
+// ---- velocities (panel -> point) ------------------------------------------
+constexpr double kCore = 0.05;  // panel::core (panel.h:67)
+
+// panel::vortexV (panel.cpp:707-711): (a x b)(|a|+|b|) / (|a||b|(|a||b| + a.b) + (core |s|)^2)
+CP_HD void vortex_v(const double* a, const double* b, const double* s, double* out) {
+#pragma clang fp contract(off)
+    double na = sqrt(sq_norm3(a[0], a[1], a[2]));
+    double nb = sqrt(sq_norm3(b[0], b[1], b[2]));
+    double cs = kCore * sqrt(sq_norm3(s[0], s[1], s[2]));
+    double den = na * nb * (na * nb + (a[0] * b[0] + a[1] * b[1] + a[2] * b[2])) + cs * cs;
+    double f = na + nb;
+    out[0] = (a[1] * b[2] - a[2] * b[1]) * f / den;
+    out[1] = (a[2] * b[0] - a[0] * b[2]) * f / den;
+    out[2] = (a[0] * b[1] - a[1] * b[0]) * f / den;
+}
+
+// panel::pntDubV (panel.cpp:773-776): area (3 (pjk.n) pjk - r^2 n) / (4 pi r^5)
+CP_HD void pnt_dub_v(const PanelRec& P, const double* pjk, double r, double* out) {
+#pragma clang fp contract(off)
+    double pn3 = 3 * (pjk[0] * P.n[0] + pjk[1] * P.n[1] + pjk[2] * P.n[2]);
+    double r2 = r * r;
+    double den = 4 * kPI * pow(r, 5);
+    for (int k = 0; k < 3; k++) out[k] = P.area * (pn3 * pjk[k] - r2 * P.n[k]) / den;
+}
+
+// bodyPanel::panelVInf (bodyPanel.cpp:305-348) in FP64: source and doublet
+// velocity of a unit-strength panel at p.  Far field: point doublet and the
+// point source with the reference's sign (negated, bodyPanel.cpp:314).
+CP_HD void panel_v_inf_f64(const PanelRec& P, const double* p, double* vs, double* vd) {
+#pragma clang fp contract(off)
+    double pjk[3] = {p[0] - P.c[0], p[1] - P.c[1], p[2] - P.c[2]};
+    double r2 = sq_norm3(pjk[0], pjk[1], pjk[2]);
+    if (is_far(P, r2)) {
+        double r = sqrt(r2);
+        double den = 4 * kPI * pow(r, 3);
+        for (int k = 0; k < 3; k++) vs[k] = -1.0 * (P.area * pjk[k] / den);
+        pnt_dub_v(P, pjk, r, vd);
+        return;
+    }
+    double PN = pjk[0] * P.n[0] + pjk[1] * P.n[1] + pjk[2] * P.n[2];
+    const int nv = P.nv;
+    double S[3] = {0, 0, 0}, D[3] = {0, 0, 0};
+    for (int k = 0; k < nv; k++) {
+        int k2 = (k + 1 == nv) ? 0 : k + 1;
+        double a[3] = {p[0] - P.v[3 * k], p[1] - P.v[3 * k + 1], p[2] - P.v[3 * k + 2]};
+        double b[3] = {p[0] - P.v[3 * k2], p[1] - P.v[3 * k2 + 1], p[2] - P.v[3 * k2 + 2]};
+        double s[3] = {a[0] - b[0], a[1] - b[1], a[2] - b[2]};
+        double sxa0 = s[1] * a[2] - s[2] * a[1];
+        double sxa1 = s[2] * a[0] - s[0] * a[2];
+        double sxa2 = s[0] * a[1] - s[1] * a[0];
+        double Al = P.n[0] * sxa0 + P.n[1] * sxa1 + P.n[2] * sxa2;
+        double w[3];
+        vortex_v(a, b, s, w);
+        D[0] += w[0];
+        D[1] += w[1];
+        D[2] += w[2];
+        // bodyPanel::srcSideV (bodyPanel.cpp:375-390)
+        double A = sqrt(sq_norm3(a[0], a[1], a[2])), B = sqrt(sq_norm3(b[0], b[1], b[2]));
+        double Sl = sqrt(sq_norm3(s[0], s[1], s[2]));
+        double GL = 0;
+        if (fabs(A + B - Sl) > 0 && Sl > 0) GL = 1 / Sl * log(fabs((A + B + Sl) / (A + B - Sl)));
+        double CJK = vortex_phi(PN, Al, a, s, P.l, P.m, A, B);
+        double sm = s[0] * P.m[0] + s[1] * P.m[1] + s[2] * P.m[2];
+        double sl = s[0] * P.l[0] + s[1] * P.l[1] + s[2] * P.l[2];
+        for (int q = 0; q < 3; q++) S[q] += GL * (sm * P.l[q] - sl * P.m[q]) + CJK * P.n[q];
+    }
+    for (int q = 0; q < 3; q++) {
+        vd[q] = D[q] / (4 * kPI);
+        vs[q] = S[q] / (4 * kPI);
+    }
+}
+
+// panel::dubVInf (panel.cpp:666-705) in FP64 (wake panels)
+CP_HD void dub_v_inf_f64(const PanelRec& P, const double* p, double* vd) {
+#pragma clang fp contract(off)
+    double pjk[3] = {p[0] - P.c[0], p[1] - P.c[1], p[2] - P.c[2]};
+    double r2 = sq_norm3(pjk[0], pjk[1], pjk[2]);
+    if (is_far(P, r2)) {
+        pnt_dub_v(P, pjk, sqrt(r2), vd);
+        return;
+    }
+    double D[3] = {0, 0, 0};
+    const int nv = P.nv;
+    for (int k = 0; k < nv; k++) {
+        int k2 = (k + 1 == nv) ? 0 : k + 1;
+        double a[3] = {p[0] - P.v[3 * k], p[1] - P.v[3 * k + 1], p[2] - P.v[3 * k + 2]};
+        double b[3] = {p[0] - P.v[3 * k2], p[1] - P.v[3 * k2 + 1], p[2] - P.v[3 * k2 + 2]};
+        double s[3] = {P.v[3 * k2] - P.v[3 * k], P.v[3 * k2 + 1] - P.v[3 * k + 1], P.v[3 * k2 + 2] - P.v[3 * k + 2]};
+        double w[3];
+        vortex_v(a, b, s, w);
+        D[0] += w[0];
+        D[1] += w[1];
+        D[2] += w[2];
+    }
+    for (int q = 0; q < 3; q++) vd[q] = D[q] / (4 * kPI);
+}
+
 }  // namespace cpgpu

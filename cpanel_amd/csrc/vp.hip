@@ -1,0 +1,285 @@
+// K4 / K5: vortex-particle evaluations on gfx950.
+//
+// K4, Barnes-Hut: one lane per target walks the flattened octree (pre-order
+// with skip links, so the walk is a forward scan without a stack).  Particle
+// targets are taken in leaf order, so the lanes of a wavefront are spatial
+// neighbours and mostly walk the same nodes.  Per node: a leaf sums its
+// members directly (self excluded by slot), an internal node is replaced by
+// its pseudo-particle when s/d < theta, else the walk descends.  This is the
+// reference's recursion (node.h:236-421) in the same visiting order.  FP64,
+// FMA contraction off.
+//
+// K5, panel -> point velocity: one lane per point, the panels of a K slice
+// read as wave-uniform 256-byte records, FP64 (bodyPanel::panelV,
+// wakePanel::panelV); slices are summed in a fixed order.
+#include <hip/hip_runtime.h>
+
+#include "device.h"
+#include "vp.h"
+
+namespace cpgpu {
+
+namespace {
+
+#pragma clang fp contract(off)
+
+constexpr double kCoreOverlap = 1.5;  // particle::coreOverlap (particle.h:33)
+constexpr double kNu = 1.983e-5;      // particle.cpp:126
+constexpr int kT = 256;
+
+struct Src {
+    double x, y, z, ax, ay, az, r;
+};
+
+__device__ inline double nrm3(double x, double y, double z) { return sqrt(x * x + y * y + z * z); }
+
+// particle::velInflAlgSmooth (particle.cpp:31-38)
+__device__ inline void k_alg(const Src& s, const double* p, double* acc) {
+    double sigma = kCoreOverlap * s.r;
+    double dx = p[0] - s.x, dy = p[1] - s.y, dz = p[2] - s.z;
+    double r = nrm3(dx, dy, dz);
+    double r2 = r * r, s2 = sigma * sigma;
+    double f = -1 / (4 * kPI) * (r2 + 2.5 * s2) / pow(r2 + s2, 2.5);
+    acc[0] += f * (dy * s.az - dz * s.ay);
+    acc[1] += f * (dz * s.ax - dx * s.az);
+    acc[2] += f * (dx * s.ay - dy * s.ax);
+}
+
+__device__ inline double sym_sigma(double rs, double rt) {
+    double a = kCoreOverlap * rs, b = kCoreOverlap * rt;
+    return sqrt(a * a + b * b) / sqrt(2.0);
+}
+
+// particle::velInfl(particle*) (particle.cpp:42-58): source s on target t
+__device__ inline void k_gauss(const Src& s, const Src& t, double* acc) {
+    double sigma = sym_sigma(s.r, t.r);
+    double dx = t.x - s.x, dy = t.y - s.y, dz = t.z - s.z;
+    double rho = nrm3(dx, dy, dz) / sigma;
+    double K = (1 / (4 * kPI * rho) * erf(rho / sqrt(2.0)) - 1 / pow(2 * kPI, 1.5) * exp(-0.5 * rho * rho)) /
+               (rho * rho);
+    double f = -1 / (sigma * sigma * sigma) * K;
+    acc[0] += f * (dy * s.az - dz * s.ay);
+    acc[1] += f * (dz * s.ax - dx * s.az);
+    acc[2] += f * (dx * s.ay - dy * s.ax);
+}
+
+// source->vortexStretching(target) (particle.cpp:80-118):
+// (1/sigma^3) alpha_t x ((K I - F d d^T / sigma^2) alpha_s), d = x_s - x_t,
+// zero beyond 5 source radii
+__device__ inline void k_stretch(const Src& s, const Src& t, double* acc) {
+    double dx = s.x - t.x, dy = s.y - t.y, dz = s.z - t.z;
+    double dist = nrm3(dx, dy, dz);
+    if (dist > 5 * s.r) return;
+    double sig = sym_sigma(s.r, t.r);
+    double rho = dist / sig;
+    double xi = 1 / pow(2 * kPI, 1.5) * exp(-rho * rho / 2);
+    double G = 1 / (4 * kPI * rho) * erf(rho / sqrt(2.0));
+    double K = (G - xi) / (rho * rho);
+    double F = (3 * K - xi) / (rho * rho);
+    double c = 1 / (sig * sig) * F;
+    // M alpha_s
+    double da = dx * s.ax + dy * s.ay + dz * s.az;
+    double mx = K * s.ax - c * dx * da;
+    double my = K * s.ay - c * dy * da;
+    double mz = K * s.az - c * dz * da;
+    double s3 = 1 / (sig * sig * sig);
+    acc[0] += s3 * (t.ay * mz - t.az * my);
+    acc[1] += s3 * (t.az * mx - t.ax * mz);
+    acc[2] += s3 * (t.ax * my - t.ay * mx);
+}
+
+// source->viscousDiffusion(target) (particle.cpp:122-140)
+__device__ inline void k_diff(const Src& s, const Src& t, double* acc) {
+    double dx = s.x - t.x, dy = s.y - t.y, dz = s.z - t.z;
+    double sig = sym_sigma(s.r, t.r);
+    double rho = nrm3(dx, dy, dz) / sig;
+    double Vs = 4 * kPI / 3 * (s.r * s.r * s.r);
+    double Vt = 4 * kPI / 3 * (t.r * t.r * t.r);
+    double xi = 1 / (pow(2 * kPI, 1.5) * (rho * rho * rho)) * exp(-rho * rho / 2);
+    double f = 2 * kNu / (sig * sig) * xi;
+    acc[0] += f * (Vs * t.ax - Vt * s.ax);
+    acc[1] += f * (Vs * t.ay - Vt * s.ay);
+    acc[2] += f * (Vs * t.az - Vt * s.az);
+}
+
+__device__ inline Src member(const double* mpos, const double* mstr, const double* mrad, int i) {
+    return Src{mpos[3 * i], mpos[3 * i + 1], mpos[3 * i + 2], mstr[3 * i], mstr[3 * i + 1], mstr[3 * i + 2], mrad[i]};
+}
+
+__device__ inline Src pseudo(const BhNode& nd) { return Src{nd.ex, nd.ey, nd.ez, nd.sx, nd.sy, nd.sz, nd.radius}; }
+
+// node::isFarField (node.h:395-411): s/d < theta
+__device__ inline bool far_cell(const BhNode& nd, double px, double py, double pz, double theta) {
+    double d = nrm3(nd.ex - px, nd.ey - py, nd.ez - pz);
+    return nd.width / d < theta;
+}
+
+// velocity at points, algebraic kernel (node::calcVel(POI))
+__global__ __launch_bounds__(kT) void bh_points_kernel(const BhNode* __restrict__ nodes, int n_nodes,
+                                                       const double* __restrict__ mpos,
+                                                       const double* __restrict__ mstr,
+                                                       const double* __restrict__ mrad, double theta, int m,
+                                                       const double* __restrict__ pts, double* __restrict__ out) {
+    int t = blockIdx.x * kT + threadIdx.x;
+    if (t >= m) return;
+    double p[3] = {pts[3 * t], pts[3 * t + 1], pts[3 * t + 2]};
+    double acc[3] = {0, 0, 0};
+    int i = 1;  // root excluded: its children start at node 1
+    while (i < n_nodes) {
+        const BhNode nd = nodes[i];
+        if (nd.count >= 0) {
+            for (int k = 0; k < nd.count; k++) k_alg(member(mpos, mstr, mrad, nd.first + k), p, acc);
+            i = nd.skip;
+        } else if (far_cell(nd, p[0], p[1], p[2], theta)) {
+            k_alg(pseudo(nd), p, acc);
+            i = nd.skip;
+        } else {
+            i = i + 1;
+        }
+    }
+    out[3 * t] = acc[0];
+    out[3 * t + 1] = acc[1];
+    out[3 * t + 2] = acc[2];
+}
+
+// velocity / stretching / diffusion on the tree's own particles
+template <int MODE>
+__global__ __launch_bounds__(kT) void bh_particles_kernel(const BhNode* __restrict__ nodes, int n_nodes,
+                                                          const double* __restrict__ mpos,
+                                                          const double* __restrict__ mstr,
+                                                          const double* __restrict__ mrad, int n, double theta,
+                                                          const int* __restrict__ order, double* __restrict__ out) {
+    int st = blockIdx.x * kT + threadIdx.x;  // target slot (leaf order)
+    if (st >= n) return;
+    const Src tg = member(mpos, mstr, mrad, st);
+    double acc[3] = {0, 0, 0};
+    int i = 1;
+    while (i < n_nodes) {
+        const BhNode nd = nodes[i];
+        if (nd.count >= 0) {
+            for (int k = 0; k < nd.count; k++) {
+                int ms = nd.first + k;
+                if (ms == st) continue;  // pointer self-exclusion of node.h:272
+                const Src s = member(mpos, mstr, mrad, ms);
+                if (MODE == CP_BH_VEL)
+                    k_gauss(s, tg, acc);
+                else if (MODE == CP_BH_STRETCH)
+                    k_stretch(s, tg, acc);
+                else
+                    k_diff(s, tg, acc);
+            }
+            i = nd.skip;
+        } else if (far_cell(nd, tg.x, tg.y, tg.z, theta)) {
+            const Src s = pseudo(nd);
+            if (MODE == CP_BH_VEL)
+                k_gauss(s, tg, acc);
+            else if (MODE == CP_BH_STRETCH)
+                k_stretch(s, tg, acc);
+            else
+                k_diff(s, tg, acc);
+            i = nd.skip;
+        } else {
+            i = i + 1;
+        }
+    }
+    const int o = order[st];
+    out[3 * o] = acc[0];
+    out[3 * o + 1] = acc[1];
+    out[3 * o + 2] = acc[2];
+}
+
+// panel -> point velocity over the panels [p0, p1) of the combined list
+// (body panels, then wake panels); partial sums per K slice
+__global__ __launch_bounds__(kT) void p2p_kernel(const PanelRec* __restrict__ body, int nb,
+                                                 const PanelRec* __restrict__ wake, int nw,
+                                                 const double* __restrict__ sigma, const double* __restrict__ mu,
+                                                 const double* __restrict__ muw, int m,
+                                                 const double* __restrict__ pts, int chunk,
+                                                 double* __restrict__ part) {
+    const int t = blockIdx.x * kT + threadIdx.x;
+    const int ks = blockIdx.y;
+    const int p0 = ks * chunk, p1 = min(nb + nw, p0 + chunk);
+    double acc[3] = {0, 0, 0};
+    if (t < m) {
+        double p[3] = {pts[3 * t], pts[3 * t + 1], pts[3 * t + 2]};
+        for (int j = p0; j < p1; j++) {
+            if (j < nb) {
+                double vs[3], vd[3];
+                panel_v_inf_f64(body[j], p, vs, vd);
+                const double sg = sigma[j], mj = mu[j];
+                for (int q = 0; q < 3; q++) acc[q] += sg * vs[q] + mj * vd[q];
+            } else {
+                double vd[3];
+                dub_v_inf_f64(wake[j - nb], p, vd);
+                const double mw = muw[j - nb];
+                for (int q = 0; q < 3; q++) acc[q] += mw * vd[q];
+            }
+        }
+    }
+    if (t < m) {
+        double* o = part + ((int64_t)ks * m + t) * 3;
+        o[0] = acc[0];
+        o[1] = acc[1];
+        o[2] = acc[2];
+    }
+}
+
+__global__ void p2p_reduce_kernel(const double* __restrict__ part, int KS, int m, double* __restrict__ out) {
+    int e = blockIdx.x * kT + threadIdx.x;
+    if (e >= 3 * m) return;
+    double s = 0;
+    for (int q = 0; q < KS; q++) s += part[(int64_t)q * 3 * m + e];
+    out[e] = s;
+}
+
+}  // namespace
+
+void launch_bh_points(const DevBhTree& T, double theta, int m, const double* pts, double* out, hipStream_t s) {
+    if (m <= 0) return;
+    hipLaunchKernelGGL(bh_points_kernel, dim3((unsigned)((m + kT - 1) / kT)), dim3(kT), 0, s, T.nodes, T.n_nodes,
+                       T.mpos, T.mstr, T.mrad, theta, m, pts, out);
+    CP_LAUNCH_CHECK();
+}
+
+void launch_bh_particles(const DevBhTree& T, int mode, double theta, double* out, hipStream_t s) {
+    if (T.n <= 0) return;
+    dim3 grid((unsigned)((T.n + kT - 1) / kT));
+    switch (mode) {
+        case CP_BH_VEL:
+            hipLaunchKernelGGL(bh_particles_kernel<CP_BH_VEL>, grid, dim3(kT), 0, s, T.nodes, T.n_nodes, T.mpos,
+                               T.mstr, T.mrad, T.n, theta, T.order, out);
+            break;
+        case CP_BH_STRETCH:
+            hipLaunchKernelGGL(bh_particles_kernel<CP_BH_STRETCH>, grid, dim3(kT), 0, s, T.nodes, T.n_nodes, T.mpos,
+                               T.mstr, T.mrad, T.n, theta, T.order, out);
+            break;
+        default:
+            hipLaunchKernelGGL(bh_particles_kernel<CP_BH_DIFF>, grid, dim3(kT), 0, s, T.nodes, T.n_nodes, T.mpos,
+                               T.mstr, T.mrad, T.n, theta, T.order, out);
+            break;
+    }
+    CP_LAUNCH_CHECK();
+}
+
+int p2p_slices(int m, int npanels) {
+    int blocks = (m + kT - 1) / kT;
+    int ks = 1;
+    while (ks < 64 && blocks * ks < 1024 && npanels / (2 * ks) >= 32) ks *= 2;
+    return ks;
+}
+
+void launch_panel_to_points(const DevGeom& g, const double* sigma, const double* mu, const double* muw, int m,
+                            const double* pts, int KS, double* part, double* out, hipStream_t s) {
+    if (m <= 0) return;
+    const int np = g.n + g.nw;
+    const int chunk = (np + KS - 1) / KS;
+    dim3 grid((unsigned)((m + kT - 1) / kT), (unsigned)KS);
+    hipLaunchKernelGGL(p2p_kernel, grid, dim3(kT), 0, s, g.body, g.n, g.wake, g.nw, sigma, mu, muw, m, pts, chunk,
+                       part);
+    CP_LAUNCH_CHECK();
+    hipLaunchKernelGGL(p2p_reduce_kernel, dim3((unsigned)((3 * m + kT - 1) / kT)), dim3(kT), 0, s,
+                       (const double*)part, KS, m, out);
+    CP_LAUNCH_CHECK();
+}
+
+}  // namespace cpgpu

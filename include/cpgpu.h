@@ -140,6 +140,47 @@ cp_status cp_solve(cp_geom* g, cp_prec prec, int32_t k, const cp_freestream* fs,
  * X, Y: host, n_body x k column-major. */
 cp_status cp_matvec(cp_geom* g, int kernel, int32_t k, const double* X, double* Y);
 
+/* ---- vortex particles: Barnes-Hut (particleFMM::build / barnesHutVel /
+ *      barnesHutStretch / barnesHutDiff, CPanel/particleFMM.h:52-56) ------- */
+typedef struct cp_tree cp_tree;
+
+typedef enum {
+    CP_BH_VEL_POINTS = 0,   /* velocity at points, algebraic kernel
+                               (particleFMM::barnesHutVel(Vector3d), node.h:236-262) */
+    CP_BH_VEL = 1,          /* velocity on the particles, Gaussian kernel
+                               (barnesHutVel(particle*), node.h:264-290) */
+    CP_BH_STRETCH = 2,      /* vortex stretching (barnesHutStretch, node.h:292-318) */
+    CP_BH_DIFF = 3          /* viscous diffusion (barnesHutDiff, node.h:333-359) */
+} cp_bh_mode;
+
+typedef struct {
+    int32_t max_members;    /* leaf capacity (<=0: 10, cpCaseVP.cpp:79) */
+    int32_t pad;
+    double theta;           /* opening angle s/d (<0: 0.5, octree.h:37) */
+} cp_bh_opts;
+
+/* Builds the particle octree and its pseudo-particles (octree::addData,
+ * octree.h:151-186; particleFMM::computeMultExp, particleFMM.cpp:29-58) and
+ * uploads it.  xyz, alpha: 3 per particle (interleaved), radius: 1 per particle.
+ * On a host-only context the tree is built for cp_bh_info only. */
+cp_status cp_bh_build(cp_ctx* ctx, int32_t n, const double* xyz, const double* alpha, const double* radius,
+                      const cp_bh_opts* opts, cp_tree** out);
+/* node count, deepest level (root = 1), root origin[3] and half width */
+cp_status cp_bh_info(const cp_tree* t, int32_t* n_nodes, int32_t* levels, double* origin, double* half);
+/* Barnes-Hut sums from the root's children (root excluded,
+ * particleFMM.cpp:65-116).  CP_BH_VEL_POINTS: m points pts[3m] -> out[3m];
+ * the other modes: the tree's own n particles -> out[3n] (pts ignored). */
+cp_status cp_bh_eval(cp_tree* t, cp_bh_mode mode, int32_t m, const double* pts, double* out);
+void cp_bh_release(cp_tree* t);
+
+/* ---- panel -> point velocity (bodyPanel::panelV, bodyPanel.cpp:122-130;
+ *      wakePanel::panelV, wakePanel.cpp:89-92) ------------------------------
+ * vel[3m] = sum over body panels of sigma_j vSrc_j + mu_j vDub_j plus wake
+ * panels mu_w vDub_w at the points pts[3m]; sigma, mu: n_body (solve order),
+ * mu_wake: n_wake or NULL (zero). */
+cp_status cp_panel_to_points(cp_geom* g, const double* sigma, const double* mu, const double* mu_wake, int32_t m,
+                             const double* pts, double* vel);
+
 /* JSON text describing the batched solver of a geometry: tuned GEMM shape
  * and the autotuner's timings, block-Jacobi blocks, near-field entries. */
 cp_status cp_solver_report(cp_geom* g, char* buf, int32_t len);

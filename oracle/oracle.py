@@ -89,6 +89,14 @@ def dub_phi_inf(verts, poi):
     return lib().or_dub_phi_inf(len(v) // 3, _p(v), _p(q))
 
 
+def dub_v_inf(verts, poi):
+    v = np.ascontiguousarray(verts, dtype=np.float64).reshape(-1)
+    q = np.ascontiguousarray(poi, dtype=np.float64)
+    d = np.zeros(3)
+    lib().or_dub_v_inf(len(v) // 3, _p(v), _p(q), _p(d))
+    return d
+
+
 def panel_v_inf(verts, poi):
     v = np.ascontiguousarray(verts, dtype=np.float64).reshape(-1)
     q = np.ascontiguousarray(poi, dtype=np.float64)
