@@ -45,6 +45,19 @@ class SolveInfo(C.Structure):
                 ("refinements", C.c_int32), ("converged", C.c_int32), ("pad", C.c_int32)]
 
 
+class RequestStats(C.Structure):
+    _fields_ = [("queue_s", C.c_double), ("prepare_s", C.c_double), ("solve_s", C.c_double),
+                ("latency_s", C.c_double), ("cache_hit", C.c_int32), ("batch_conditions", C.c_int32),
+                ("n_body", C.c_int32), ("pad", C.c_int32)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("conditions", C.c_int64),
+                ("batches", C.c_int64), ("cache_hits", C.c_int64), ("cache_misses", C.c_int64),
+                ("evictions", C.c_int64), ("resident_geometries", C.c_int64),
+                ("resident_bytes_est", C.c_uint64), ("device_bytes", C.c_uint64)]
+
+
 class BhOpts(C.Structure):
     _fields_ = [("max_members", C.c_int32), ("pad", C.c_int32), ("theta", C.c_double)]
 
@@ -77,6 +90,12 @@ SIGNATURES = {
     "cp_bh_eval": (C.c_int, [C.c_void_p, C.c_int, C.c_int32, _d, _d]),
     "cp_bh_release": (None, [C.c_void_p]),
     "cp_panel_to_points": (C.c_int, [C.c_void_p, _d, _d, _d, C.c_int32, _d, _d]),
+    "cp_server_create": (C.c_int, [C.c_void_p, C.c_int32, C.c_uint64, C.POINTER(C.c_void_p)]),
+    "cp_server_submit": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(Mesh), C.c_int32, C.POINTER(Freestream),
+                                   C.POINTER(SolveOpts), C.c_int32, C.POINTER(C.c_uint64)]),
+    "cp_server_wait": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(SolveInfo), _d, C.POINTER(RequestStats)]),
+    "cp_server_get_stats": (C.c_int, [C.c_void_p, C.POINTER(ServerStats)]),
+    "cp_server_destroy": (None, [C.c_void_p]),
     "cp_timing_reset": (C.c_int, [C.c_void_p]),
     "cp_timing_enable": (C.c_int, [C.c_void_p, C.c_int]),
     "cp_timing_get": (C.c_int, [C.c_void_p, C.c_char_p, _d, _i64]),

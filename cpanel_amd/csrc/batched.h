@@ -36,6 +36,8 @@ struct TuneRecord {
 class BatchedSolver {
 public:
     BatchedSolver(int n, int64_t lda, int capacity);
+    // device bytes of the workspace of a solver with leading dimension lda
+    static size_t workspace_bytes(int64_t lda);
     ~BatchedSolver();
     BatchedSolver(const BatchedSolver&) = delete;
     BatchedSolver& operator=(const BatchedSolver&) = delete;

@@ -638,14 +638,22 @@ void launch_sigma(int n, int64_t lda, const double* normals, const double* vinf4
     CP_LAUNCH_CHECK();
 }
 
+size_t BatchedSolver::workspace_bytes(int64_t lda) {
+    const size_t mv64 = (size_t)lda * KP * sizeof(double);
+    const size_t mv32 = (size_t)lda * KP * sizeof(float);
+    const int KSmax = 32;
+    const int nrb = (int)(lda / kRedRows);
+    // 19 sub-buffers, each rounded up to 256 bytes
+    return 4 * mv64 + 9 * mv32 + (size_t)lda * sizeof(float) + KSmax * mv32 + KSmax * mv64 +
+           (size_t)4 * nrb * KP * sizeof(double) + KP * Q_NQ * sizeof(double) + KP * sizeof(int) + 19 * 256;
+}
+
 BatchedSolver::BatchedSolver(int n, int64_t lda, int capacity) : n_(n), cap_(capacity), lda_(lda) {
     nrb_ = (int)(lda / kRedRows);
     const size_t mv64 = (size_t)lda * KP * sizeof(double);
     const size_t mv32 = (size_t)lda * KP * sizeof(float);
     const int KSmax = 32;
-    size_t bytes = 4 * mv64 + 9 * mv32 + (size_t)lda * sizeof(float) + KSmax * mv32 + KSmax * mv64 +
-                   (size_t)4 * nrb_ * KP * sizeof(double) + KP * Q_NQ * sizeof(double) + KP * sizeof(int) + 1024;
-    CP_HIP(hipMalloc(&ws_, bytes));
+    CP_HIP(hipMalloc(&ws_, workspace_bytes(lda)));
     char* p = (char*)ws_;
     auto take = [&](size_t b) {
         char* q = p;

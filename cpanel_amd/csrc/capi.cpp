@@ -8,6 +8,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <new>
@@ -31,12 +32,18 @@ using namespace cpgpu;
 struct cp_ctx {
     int device = 0;
     uint64_t budget = 0;
-    uint64_t in_use = 0;
-    hipStream_t stream = nullptr;
-    std::string err;
+    uint64_t in_use = 0;          // device bytes held (live + pooled), under alloc_mu
+    hipStream_t stream = nullptr; // context-level calls (Barnes-Hut trees)
+    std::string err;              // last error, under err_mu
     int hip_err = 0;
     KernelTimer timer;
-    std::mutex mu;
+    std::mutex mu;                // serialises context-level calls
+    std::mutex alloc_mu;
+    std::mutex err_mu;
+    // freed device blocks kept for reuse: a hipFree waits for the whole
+    // device, which would stall the other geometries' streams
+    std::multimap<size_t, void*> pool;
+    uint64_t pooled = 0;
 };
 
 namespace {
@@ -49,10 +56,14 @@ struct DevBuf {
     DevBuf(const DevBuf&) = delete;
     DevBuf& operator=(const DevBuf&) = delete;
     ~DevBuf() { release(); }
+    // returns the block to the context's pool; callers release only buffers
+    // whose last use on the device has completed (every API call
+    // synchronises its stream before its temporaries go)
     void release() {
         if (p) {
-            (void)hipFree(p);
-            ctx->in_use -= bytes;
+            std::lock_guard<std::mutex> lk(ctx->alloc_mu);
+            ctx->pool.emplace(bytes, p);
+            ctx->pooled += bytes;
             p = nullptr;
             bytes = 0;
         }
@@ -67,13 +78,59 @@ struct BudgetError {
     uint64_t need;
 };
 
+// frees pooled blocks (largest first) until `need` more bytes fit the budget
+void trim_pool(cp_ctx* c, uint64_t need) {
+    while (!c->pool.empty() && c->budget && c->in_use + need > c->budget) {
+        auto it = std::prev(c->pool.end());
+        (void)hipFree(it->second);
+        c->in_use -= it->first;
+        c->pooled -= it->first;
+        c->pool.erase(it);
+    }
+}
+
 void dalloc(cp_ctx* c, DevBuf& b, size_t bytes) {
     b.release();
-    if (c->budget && c->in_use + bytes > c->budget) throw BudgetError{bytes};
+    bytes = (bytes + 255) / 256 * 256;
+    std::lock_guard<std::mutex> lk(c->alloc_mu);
     b.ctx = c;
-    CP_HIP(hipMalloc(&b.p, bytes));
+    // reuse a pooled block of at most 1.25x the request
+    auto it = c->pool.lower_bound(bytes);
+    if (it != c->pool.end() && it->first <= bytes + bytes / 4) {
+        b.p = it->second;
+        b.bytes = it->first;
+        c->pooled -= it->first;
+        c->pool.erase(it);
+        return;
+    }
+    trim_pool(c, bytes);
+    if (c->budget && c->in_use + bytes > c->budget) throw BudgetError{bytes};
+    hipError_t e = hipMalloc(&b.p, bytes);
+    if (e != hipSuccess) {  // device full: give the pool back and retry once
+        (void)hipGetLastError();
+        while (!c->pool.empty()) {
+            auto q = std::prev(c->pool.end());
+            (void)hipFree(q->second);
+            c->in_use -= q->first;
+            c->pooled -= q->first;
+            c->pool.erase(q);
+        }
+        CP_HIP(hipMalloc(&b.p, bytes));
+    }
     b.bytes = bytes;
     c->in_use += bytes;
+}
+
+// device bytes allocated outside DevBuf (solver workspaces)
+void account(cp_ctx* c, int64_t delta) {
+    std::lock_guard<std::mutex> lk(c->alloc_mu);
+    c->in_use = (uint64_t)((int64_t)c->in_use + delta);
+}
+
+// would `bytes` more fit the budget (counting pooled blocks as free)?
+bool fits(cp_ctx* c, uint64_t bytes) {
+    std::lock_guard<std::mutex> lk(c->alloc_mu);
+    return !c->budget || c->in_use - c->pooled + bytes <= c->budget;
 }
 
 template <class T>
@@ -87,6 +144,9 @@ void upload(cp_ctx* c, DevBuf& b, const std::vector<T>& v) {
 struct cp_geom {
     cp_ctx* ctx = nullptr;
     uint64_t key = 0;
+    hipStream_t stream = nullptr;  // every device call on this geometry
+    std::mutex mu;                 // serialises calls on this geometry
+    int64_t solver_bytes = 0;      // workspaces counted in ctx->in_use
     Geometry host;
     DevGeom dev;
     DevBuf body, wake, cxyz, fold_ptr, fold_w, fold_c, normals;
@@ -113,14 +173,25 @@ struct cp_geom {
 namespace {
 
 cp_status fail(cp_ctx* c, cp_status s, const std::string& msg) {
-    if (c) c->err = msg;
+    if (c) {
+        std::lock_guard<std::mutex> lk(c->err_mu);
+        c->err = msg;
+    }
     return s;
 }
 
+void set_err(cp_ctx* c, const std::string& msg) {
+    std::lock_guard<std::mutex> lk(c->err_mu);
+    c->err = msg;
+}
+
+// runs f with HIP and budget errors turned into cp_status; `lock` (if any)
+// is held for the call
 template <class F>
-cp_status guarded(cp_ctx* c, F&& f) {
+cp_status guarded_impl(cp_ctx* c, std::mutex* lock, F&& f) {
     if (!c) return CP_BAD_ARG;
-    std::lock_guard<std::mutex> lk(c->mu);
+    std::unique_lock<std::mutex> lk;
+    if (lock) lk = std::unique_lock<std::mutex>(*lock);
     set_current_timer(&c->timer);
     try {
         if (c->device >= 0) CP_HIP(hipSetDevice(c->device));
@@ -130,18 +201,28 @@ cp_status guarded(cp_ctx* c, F&& f) {
     } catch (const HipError& e) {
         set_current_timer(nullptr);
         c->hip_err = (int)e.code;
-        c->err = std::string("HIP error ") + hipGetErrorString(e.code) + " in " + e.where;
+        set_err(c, std::string("HIP error ") + hipGetErrorString(e.code) + " in " + e.where);
         return CP_HIP_ERROR;
     } catch (const BudgetError& e) {
         set_current_timer(nullptr);
-        c->err = "HBM budget exceeded: need " + std::to_string(e.need) + " more bytes, " +
-                 std::to_string(c->in_use) + " of " + std::to_string(c->budget) + " in use";
+        set_err(c, "HBM budget exceeded: need " + std::to_string(e.need) + " more bytes, " +
+                       std::to_string(c->in_use) + " of " + std::to_string(c->budget) + " in use");
         return CP_OUT_OF_BUDGET;
     } catch (const std::bad_alloc&) {
         set_current_timer(nullptr);
-        c->err = "host allocation failed";
+        set_err(c, "host allocation failed");
         return CP_OUT_OF_BUDGET;
     }
+}
+
+template <class F>
+cp_status guarded(cp_ctx* c, F&& f) {
+    return guarded_impl(c, c ? &c->mu : nullptr, std::forward<F>(f));
+}
+
+template <class F>
+cp_status guarded_g(cp_geom* g, F&& f) {
+    return guarded_impl(g->ctx, &g->mu, std::forward<F>(f));
 }
 
 // cpCase::windToBody (cpCase.cpp:142-166): body-axis freestream and the
@@ -179,6 +260,7 @@ cp_status build_geom(cp_ctx* c, const cp_mesh* m, uint64_t key, cp_geom** out) {
         *out = g.release();
         return CP_OK;
     }
+    CP_HIP(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
     std::vector<PanelRec> br(n), wr(std::max(nw, 1));
     for (int i = 0; i < n; i++) br[i] = make_panel_rec(G.body[i]);
     for (int j = 0; j < nw; j++) wr[j] = make_panel_rec(G.wake[j]);
@@ -232,7 +314,7 @@ void ensure_wake_influence(cp_geom* g) {
     if (g->W.p || g->dev.nw == 0) return;
     cp_ctx* c = g->ctx;
     dalloc(c, g->W, (size_t)g->dev.lda * g->dev.nw * sizeof(double));
-    launch_wake_influence(g->dev, g->W.as<double>(), g->dev.lda, c->stream);
+    launch_wake_influence(g->dev, g->W.as<double>(), g->dev.lda, g->stream);
 }
 
 void ensure_post(cp_geom* g) {
@@ -299,15 +381,15 @@ void ensure_block_jacobi(cp_geom* g) {
     }
     const int64_t total = P.off.back();
     launch_bj_gather(g->A32.as<float>(), g->dev.lda, g->bj_perm.as<int>(), g->bj_ptr.as<int>(), g->bj_off.as<int64_t>(),
-                     P.nblocks(), P.max_b, g->bj_inv.as<float>(), c->stream);
+                     P.nblocks(), P.max_b, g->bj_inv.as<float>(), g->stream);
     std::vector<float> f(total);
-    CP_HIP(hipMemcpyAsync(f.data(), g->bj_inv.p, total * sizeof(float), hipMemcpyDeviceToHost, c->stream));
-    CP_HIP(hipStreamSynchronize(c->stream));
+    CP_HIP(hipMemcpyAsync(f.data(), g->bj_inv.p, total * sizeof(float), hipMemcpyDeviceToHost, g->stream));
+    CP_HIP(hipStreamSynchronize(g->stream));
     std::vector<double> d(f.begin(), f.end());
     bool ok = invert_blocks(P, d);
     for (int64_t e = 0; e < total; e++) f[e] = (float)d[e];
-    if (ok) CP_HIP(hipMemcpyAsync(g->bj_inv.p, f.data(), total * sizeof(float), hipMemcpyHostToDevice, c->stream));
-    CP_HIP(hipStreamSynchronize(c->stream));
+    if (ok) CP_HIP(hipMemcpyAsync(g->bj_inv.p, f.data(), total * sizeof(float), hipMemcpyHostToDevice, g->stream));
+    CP_HIP(hipStreamSynchronize(g->stream));
     g->bj_ready = true;
     if (!ok) g->bj_plan.perm.clear();  // singular block: fall back to Jacobi
 }
@@ -371,16 +453,26 @@ void cp_destroy(cp_ctx* c) {
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     (void)hipStreamDestroy(c->stream);
+    for (auto& e : c->pool) (void)hipFree(e.second);
     delete c;
 }
 
-const char* cp_last_error(const cp_ctx* c) { return c ? c->err.c_str() : "no context"; }
+const char* cp_last_error(const cp_ctx* c) {
+    if (!c) return "no context";
+    // a thread-local copy: the context's string may be replaced by another
+    // thread's error while the caller reads it
+    thread_local std::string copy;
+    std::lock_guard<std::mutex> lk(const_cast<cp_ctx*>(c)->err_mu);
+    copy = c->err;
+    return copy.c_str();
+}
 int cp_last_hip_error(const cp_ctx* c) { return c ? c->hip_err : 0; }
 uint64_t cp_bytes_in_use(const cp_ctx* c) { return c ? c->in_use : 0; }
 
 cp_status cp_geometry_upload(cp_ctx* c, const cp_mesh* m, uint64_t key, cp_geom** out) {
     if (!m || !out || !m->xyz || !m->tri || !m->ids || m->n_nodes <= 0 || m->n_tris <= 0) return CP_BAD_ARG;
-    return guarded(c, [&] { return build_geom(c, m, key, out); });
+    if (!c) return CP_BAD_ARG;
+    return guarded_impl(c, nullptr, [&] { return build_geom(c, m, key, out); });
 }
 
 cp_status cp_geometry_from_tri(cp_ctx* c, const char* path, uint64_t key, cp_geom** out) {
@@ -396,12 +488,18 @@ cp_status cp_geometry_from_tri(cp_ctx* c, const char* path, uint64_t key, cp_geo
 void cp_geometry_release(cp_geom* g) {
     if (!g) return;
     cp_ctx* c = g->ctx;
-    std::lock_guard<std::mutex> lk(c->mu);
-    if (c->device >= 0) {
-        (void)hipSetDevice(c->device);
-        (void)hipStreamSynchronize(c->stream);
+    hipStream_t st = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g->mu);
+        if (c->device >= 0) {
+            (void)hipSetDevice(c->device);
+            if (g->stream) (void)hipStreamSynchronize(g->stream);
+        }
+        if (g->solver_bytes) account(c, -g->solver_bytes);
+        st = g->stream;
     }
     delete g;
+    if (st) (void)hipStreamDestroy(st);
 }
 
 cp_status cp_geometry_info(const cp_geom* g, cp_geom_info* info) {
@@ -464,7 +562,7 @@ cp_status cp_assemble(cp_geom* g, cp_prec prec) {
     if (!g || (prec != CP_FP64 && prec != CP_FP32_REFINE)) return CP_BAD_ARG;
     cp_ctx* c = g->ctx;
     if (c->device < 0) return fail(c, CP_NO_DEVICE, "host-only context");
-    return guarded(c, [&] {
+    return guarded_g(g, [&] {
         ensure_wake_influence(g);
         const int64_t lda = g->dev.lda;
         const size_t elems = (size_t)lda * g->dev.n;
@@ -475,8 +573,8 @@ cp_status cp_assemble(cp_geom* g, cp_prec prec) {
                 dalloc(c, g->B64, elems * sizeof(double));
             }
             {
-                TimedScope ts("assemble_f64", c->stream);
-                launch_assemble_f64(g->dev, W, lda, g->A64.as<double>(), g->B64.as<double>(), c->stream);
+                TimedScope ts("assemble_f64", g->stream);
+                launch_assemble_f64(g->dev, W, lda, g->A64.as<double>(), g->B64.as<double>(), g->stream);
             }
             g->have64 = true;
         } else {
@@ -485,8 +583,8 @@ cp_status cp_assemble(cp_geom* g, cp_prec prec) {
                 dalloc(c, g->B32, elems * sizeof(float));
             }
             {
-                TimedScope ts("assemble_f32", c->stream);
-                launch_assemble_f32(g->dev, W, lda, g->A32.as<float>(), g->B32.as<float>(), c->stream);
+                TimedScope ts("assemble_f32", g->stream);
+                launch_assemble_f32(g->dev, W, lda, g->A32.as<float>(), g->B32.as<float>(), g->stream);
             }
             if (!g->near_rowptr.p) {
                 std::vector<int> rp, cl;
@@ -502,15 +600,15 @@ cp_status cp_assemble(cp_geom* g, cp_prec prec) {
                 dalloc(c, g->near_dB, std::max<size_t>(cl.size(), 1) * sizeof(double));
             }
             {
-                TimedScope ts("near_f64", c->stream);
+                TimedScope ts("near_f64", g->stream);
                 launch_near_correction(g->dev, W, lda, g->A32.as<float>(), g->B32.as<float>(), g->near_nnz,
                                        g->near_rows.as<int>(), g->near_cols.as<int>(), g->near_dA.as<double>(),
-                                       g->near_dB.as<double>(), c->stream);
+                                       g->near_dB.as<double>(), g->stream);
             }
             g->have32 = true;
             g->bj_ready = false;  // inverses follow the new A32
         }
-        CP_HIP(hipStreamSynchronize(c->stream));
+        CP_HIP(hipStreamSynchronize(g->stream));
         return CP_OK;
     });
 }
@@ -519,17 +617,18 @@ cp_status cp_read_matrices(cp_geom* g, cp_prec prec, double* A, double* B, int64
     if (!g || ld < g->dev.n) return CP_BAD_ARG;
     cp_ctx* c = g->ctx;
     if (c->device < 0) return fail(c, CP_NO_DEVICE, "host-only context");
-    return guarded(c, [&] {
+    return guarded_g(g, [&] {
         const int n = g->dev.n;
         const int64_t lda = g->dev.lda;
         if (prec == CP_FP64) {
             if (!g->have64) return fail(c, CP_NOT_ASSEMBLED, "FP64 matrices not assembled");
             if (A)
-                CP_HIP(hipMemcpy2D(A, ld * sizeof(double), g->A64.p, lda * sizeof(double), n * sizeof(double), n,
-                                   hipMemcpyDeviceToHost));
+                CP_HIP(hipMemcpy2DAsync(A, ld * sizeof(double), g->A64.p, lda * sizeof(double), n * sizeof(double), n,
+                                        hipMemcpyDeviceToHost, g->stream));
             if (B)
-                CP_HIP(hipMemcpy2D(B, ld * sizeof(double), g->B64.p, lda * sizeof(double), n * sizeof(double), n,
-                                   hipMemcpyDeviceToHost));
+                CP_HIP(hipMemcpy2DAsync(B, ld * sizeof(double), g->B64.p, lda * sizeof(double), n * sizeof(double), n,
+                                        hipMemcpyDeviceToHost, g->stream));
+            CP_HIP(hipStreamSynchronize(g->stream));
         } else {
             if (!g->have32) return fail(c, CP_NOT_ASSEMBLED, "FP32 matrices not assembled");
             std::vector<float> tmp((size_t)n * n);
@@ -537,8 +636,9 @@ cp_status cp_read_matrices(cp_geom* g, cp_prec prec, double* A, double* B, int64
             double* dst[2] = {A, B};
             for (int q = 0; q < 2; q++) {
                 if (!dst[q]) continue;
-                CP_HIP(hipMemcpy2D(tmp.data(), n * sizeof(float), src[q], lda * sizeof(float), n * sizeof(float), n,
-                                   hipMemcpyDeviceToHost));
+                CP_HIP(hipMemcpy2DAsync(tmp.data(), n * sizeof(float), src[q], lda * sizeof(float), n * sizeof(float),
+                                        n, hipMemcpyDeviceToHost, g->stream));
+                CP_HIP(hipStreamSynchronize(g->stream));
                 for (int j = 0; j < n; j++)
                     for (int i = 0; i < n; i++) dst[q][i + (int64_t)j * ld] = tmp[i + (size_t)j * n];
             }
@@ -553,14 +653,14 @@ cp_status cp_solve(cp_geom* g, cp_prec prec, int32_t k, const cp_freestream* fs,
     if (prec != CP_FP64 && prec != CP_FP32_REFINE) return CP_BAD_ARG;
     cp_ctx* c = g->ctx;
     if (c->device < 0) return fail(c, CP_NO_DEVICE, "host-only context");
-    return guarded(c, [&]() -> cp_status {
+    return guarded_g(g, [&]() -> cp_status {
         if (prec == CP_FP64 && !g->have64) return fail(c, CP_NOT_ASSEMBLED, "call cp_assemble(CP_FP64) first");
         if (prec == CP_FP32_REFINE && !g->have32)
             return fail(c, CP_NOT_ASSEMBLED, "call cp_assemble(CP_FP32_REFINE) first");
         ensure_post(g);
         const int n = g->dev.n;
         const int64_t lda = g->dev.lda;
-        hipStream_t s = c->stream;
+        hipStream_t s = g->stream;
         const double tol = opts->tol > 0 ? opts->tol : 1e-10;
         std::vector<double> T(9 * (size_t)k), vinf4(4 * (size_t)k);
         for (int q = 0; q < k; q++) {
@@ -590,7 +690,13 @@ cp_status cp_solve(cp_geom* g, cp_prec prec, int32_t k, const cp_freestream* fs,
                                           opts->max_iter, s);
             }
         } else {
-            if (!g->batched || g->batched->capacity() < k) g->batched.reset(new BatchedSolver(n, lda, std::max(k, 1)));
+            if (!g->batched) {
+                const int64_t need = (int64_t)BatchedSolver::workspace_bytes(lda);
+                if (!fits(c, (uint64_t)need)) throw BudgetError{(uint64_t)need};
+                g->batched.reset(new BatchedSolver(n, lda, kRhsTile));
+                g->solver_bytes += need;
+                account(c, need);
+            }
             g->batched->set_near(g->near_rowptr.as<int>(), g->near_cols.as<int>(), g->near_dA.as<double>(),
                                  g->near_dB.as<double>());
             if (opts->precond != 1) {
@@ -651,10 +757,10 @@ cp_status cp_matvec(cp_geom* g, int kernel, int32_t k, const double* X, double* 
     if (kernel > 0 && k > kRhsTile) return CP_BAD_ARG;
     cp_ctx* c = g->ctx;
     if (c->device < 0) return fail(c, CP_NO_DEVICE, "host-only context");
-    return guarded(c, [&]() -> cp_status {
+    return guarded_g(g, [&]() -> cp_status {
         const int n = g->dev.n;
         const int64_t lda = g->dev.lda;
-        hipStream_t s = c->stream;
+        hipStream_t s = g->stream;
         if (kernel == 0) {
             if (!g->have64) return fail(c, CP_NOT_ASSEMBLED, "FP64 matrices not assembled");
             if (!g->bicg) g->bicg.reset(new Bicgstab64(n, lda));
@@ -672,7 +778,12 @@ cp_status cp_matvec(cp_geom* g, int kernel, int32_t k, const double* X, double* 
             return CP_OK;
         }
         if (!g->have32) return fail(c, CP_NOT_ASSEMBLED, "FP32 matrices not assembled");
-        if (!g->batched) g->batched.reset(new BatchedSolver(n, lda, kRhsTile));
+        if (!g->batched) {
+            const int64_t need = (int64_t)BatchedSolver::workspace_bytes(lda);
+            g->batched.reset(new BatchedSolver(n, lda, kRhsTile));
+            g->solver_bytes += need;
+            account(c, need);
+        }
         g->batched->set_near(g->near_rowptr.as<int>(), g->near_cols.as<int>(), g->near_dA.as<double>(),
                              g->near_dB.as<double>());
         // row-major lda x 64 multivectors
@@ -709,6 +820,7 @@ cp_status cp_matvec(cp_geom* g, int kernel, int32_t k, const double* X, double* 
 
 cp_status cp_solver_report(cp_geom* g, char* buf, int32_t len) {
     if (!g || !buf || len <= 0) return CP_BAD_ARG;
+    std::lock_guard<std::mutex> lk(g->mu);
     std::string r = "{";
     if (g->batched) {
         const GemmShape& sh = g->batched->shape();
@@ -856,8 +968,8 @@ cp_status cp_panel_to_points(cp_geom* g, const double* sigma, const double* mu, 
     cp_ctx* c = g->ctx;
     if (c->device < 0) return fail(c, CP_NO_DEVICE, "host-only context");
     if (m == 0) return CP_OK;
-    return guarded(c, [&]() -> cp_status {
-        hipStream_t s = c->stream;
+    return guarded_g(g, [&]() -> cp_status {
+        hipStream_t s = g->stream;
         const int nb = g->dev.n, nw = g->dev.nw;
         std::vector<double> str((size_t)nb * 2 + std::max(nw, 1), 0.0);
         std::memcpy(str.data(), sigma, nb * sizeof(double));

@@ -15,36 +15,29 @@ KernelTimer::~KernelTimer() {
         (void)hipEventDestroy(p.b);
     }
     for (auto e : pool_) (void)hipEventDestroy(e);
-    if (open_a_) (void)hipEventDestroy(open_a_);
 }
 
 hipEvent_t KernelTimer::take() {
-    if (!pool_.empty()) {
-        hipEvent_t e = pool_.back();
-        pool_.pop_back();
-        return e;
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        if (!pool_.empty()) {
+            hipEvent_t e = pool_.back();
+            pool_.pop_back();
+            return e;
+        }
     }
     hipEvent_t e = nullptr;
     (void)hipEventCreate(&e);
     return e;
 }
 
-void KernelTimer::begin(const char* name, hipStream_t s) {
-    open_name_ = name;
-    open_a_ = take();
-    (void)hipEventRecord(open_a_, s);
+void KernelTimer::push(const char* name, hipEvent_t a, hipEvent_t b) {
+    std::lock_guard<std::mutex> lk(mu_);
+    pending_.push_back({name, a, b});
+    if (pending_.size() > 4096) collect_locked();
 }
 
-void KernelTimer::end(hipStream_t s) {
-    if (!open_a_) return;
-    hipEvent_t b = take();
-    (void)hipEventRecord(b, s);
-    pending_.push_back({open_name_, open_a_, b});
-    open_a_ = nullptr;
-    if (pending_.size() > 4096) collect();
-}
-
-void KernelTimer::collect() {
+void KernelTimer::collect_locked() {
     for (auto& p : pending_) {
         (void)hipEventSynchronize(p.b);
         float ms = 0;
@@ -58,12 +51,19 @@ void KernelTimer::collect() {
     pending_.clear();
 }
 
+void KernelTimer::collect() {
+    std::lock_guard<std::mutex> lk(mu_);
+    collect_locked();
+}
+
 void KernelTimer::reset() {
-    collect();
+    std::lock_guard<std::mutex> lk(mu_);
+    collect_locked();
     totals_.clear();
 }
 
 bool KernelTimer::get(const std::string& name, double& ms, long long& n) {
+    std::lock_guard<std::mutex> lk(mu_);
     auto it = totals_.find(name);
     if (it == totals_.end()) {
         ms = 0;

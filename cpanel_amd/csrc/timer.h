@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -13,8 +14,10 @@ class KernelTimer {
 public:
     bool enabled = false;
     ~KernelTimer();
-    void begin(const char* name, hipStream_t s);
-    void end(hipStream_t s);
+    // an event from the pool (thread safe)
+    hipEvent_t take();
+    // records a finished interval [a, b] under `name` (thread safe)
+    void push(const char* name, hipEvent_t a, hipEvent_t b);
     // waits for pending events and accumulates them
     void collect();
     void reset();
@@ -25,26 +28,35 @@ private:
         std::string name;
         hipEvent_t a, b;
     };
-    hipEvent_t take();
+    void collect_locked();
+    std::mutex mu_;
     std::vector<Pending> pending_;
     std::vector<hipEvent_t> pool_;
     std::map<std::string, std::pair<double, long long>> totals_;
-    std::string open_name_;
-    hipEvent_t open_a_ = nullptr;
 };
 
 // Timer of the context whose API call is running on this thread (or null).
 KernelTimer* current_timer();
 void set_current_timer(KernelTimer* t);
 
+// Times the work enqueued on stream s during its lifetime under `name`.
 struct TimedScope {
     KernelTimer* t;
     hipStream_t s;
-    TimedScope(const char* name, hipStream_t st) : t(current_timer()), s(st) {
-        if (t && t->enabled) t->begin(name, s);
+    const char* name;
+    hipEvent_t a = nullptr;
+    TimedScope(const char* nm, hipStream_t st) : t(current_timer()), s(st), name(nm) {
+        if (t && t->enabled) {
+            a = t->take();
+            (void)hipEventRecord(a, s);
+        }
     }
     ~TimedScope() {
-        if (t && t->enabled) t->end(s);
+        if (a) {
+            hipEvent_t b = t->take();
+            (void)hipEventRecord(b, s);
+            t->push(name, a, b);
+        }
     }
 };
 

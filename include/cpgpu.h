@@ -181,6 +181,47 @@ void cp_bh_release(cp_tree* t);
 cp_status cp_panel_to_points(cp_geom* g, const double* sigma, const double* mu, const double* mu_wake, int32_t m,
                              const double* pts, double* vel);
 
+/* ---- serving: continuous batching of (geometry, conditions) requests -----
+ * Replaces the reference's sequential case loop (CPanelMgr.cpp:49-88).
+ * Worker threads (slots) take requests in arrival order; each geometry is
+ * uploaded and assembled once (CP_FP32_REFINE) and kept on the device under
+ * the caller's key, with its own stream, so requests on different
+ * geometries overlap on the GPU; queued requests for the same geometry and
+ * options are merged into one batched solve of up to 64 conditions.  New
+ * geometries are admitted against hbm_budget (0: no limit), evicting idle
+ * geometries least recently used first. */
+typedef struct cp_server cp_server;
+
+typedef struct {
+    double queue_s;         /* submit -> a worker took the request */
+    double prepare_s;       /* geometry upload + assembly (0 on a cache hit) */
+    double solve_s;         /* batched solve + post-processing */
+    double latency_s;       /* submit -> result */
+    int32_t cache_hit;      /* geometry was resident */
+    int32_t batch_conditions;  /* columns of the solve it ran in */
+    int32_t n_body;
+    int32_t pad;
+} cp_request_stats;
+
+typedef struct {
+    int64_t submitted, completed, conditions, batches;
+    int64_t cache_hits, cache_misses, evictions;
+    int64_t resident_geometries;
+    uint64_t resident_bytes_est, device_bytes;
+} cp_server_stats;
+
+cp_status cp_server_create(cp_ctx* ctx, int32_t slots, uint64_t hbm_budget, cp_server** out);
+/* k <= 64 conditions; the mesh is copied (used if the key is not resident);
+ * want_mu: keep mu (n_body x k) for cp_server_wait */
+cp_status cp_server_submit(cp_server* s, uint64_t key, const cp_mesh* mesh, int32_t k, const cp_freestream* fs,
+                           const cp_solve_opts* opts, int32_t want_mu, uint64_t* id);
+/* blocks until request id is done; info: k entries, mu: n_body*k or NULL.
+ * Returns the request's cp_solve status. */
+cp_status cp_server_wait(cp_server* s, uint64_t id, cp_solve_info* info, double* mu, cp_request_stats* st);
+cp_status cp_server_get_stats(cp_server* s, cp_server_stats* out);
+/* finishes running requests, releases every resident geometry */
+void cp_server_destroy(cp_server* s);
+
 /* JSON text describing the batched solver of a geometry: tuned GEMM shape
  * and the autotuner's timings, block-Jacobi blocks, near-field entries. */
 cp_status cp_solver_report(cp_geom* g, char* buf, int32_t len);
