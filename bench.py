@@ -58,6 +58,60 @@ def profiled_traffic(workload, kernel):
     return None if k is None else k["read_bytes"] + k["write_bytes"]
 
 
+def vp_bench(ctx, log):
+    """Vortex-particle kernels on the swept tapered 4412 case: 80,000
+    particles (4 per trailing-edge edge per step over 400 steps) and 6,116
+    body panels.  Barnes-Hut (theta 0.5, 10 per leaf) in each mode, and the
+    panel -> particle velocity of every particle against every panel."""
+    from cpanel_amd import abi, meshgen
+    mesh = meshgen.config_c4()
+    X, A, R = meshgen.c4_particles(mesh)
+    out = {"particles": int(len(R))}
+    t0 = time.perf_counter()
+    tree = ctx.bh_tree(X, A, R)
+    out["bh_build_ms"] = 1e3 * (time.perf_counter() - t0)
+    out["bh_nodes"] = tree.info()["n_nodes"]
+    names = {abi.CP_BH_VEL: "vel", abi.CP_BH_STRETCH: "stretch", abi.CP_BH_DIFF: "diff"}
+    for mode, nm in names.items():
+        tree.eval(mode)  # warm
+        ctx.timing_reset()
+        for _ in range(3):
+            tree.eval(mode)
+        ms, n = ctx.timing("bh_eval")
+        out["bh_%s_ms" % nm] = ms / max(n, 1)
+    g = ctx.geometry(mesh)
+    rng = np.random.default_rng(1)
+    sig, mu = rng.normal(size=g.nb), rng.normal(size=g.nb)
+    g.panel_to_points(sig, mu, X[:1000])
+    ctx.timing_reset()
+    g.panel_to_points(sig, mu, X, mu_wake=rng.normal(size=g.nw))
+    ms, n = ctx.timing("panel_to_points")
+    out["panel_to_points_ms"] = ms
+    out["panel_to_points_gpairs_per_s"] = len(R) * (g.nb + g.nw) / (ms * 1e-3) / 1e9
+    log("vp: %s" % json.dumps(out))
+    return out
+
+
+def serve_bench(ctx, n_req, qps, slots, seed, log):
+    """Mixed-size request stream through the continuous-batching server:
+    Poisson arrivals, NACA wings of 500..20,000 body panels (log-uniform),
+    1..32 conditions each, 30 % geometry reuse, 200 GB HBM budget."""
+    from cpanel_amd import serve
+    stream, geoms = serve.make_stream(n_req, qps, seed=seed, n_lo=500, n_hi=20000, reuse=0.3, max_cond=32)
+    S = serve.Server(ctx, slots=slots, hbm_budget=200 << 30)
+    res, wall = serve.run_stream(S, stream, geoms)
+    st = S.stats()
+    S.close()
+    summ = serve.summarize(res, wall)
+    summ.update(dict(qps=qps, slots=slots, seed=seed, geometries=len(geoms), evictions=st["evictions"],
+                     batches=st["batches"],
+                     mean_prepare_ms=1e3 * float(np.mean([r["stats"]["prepare_s"] for r in res
+                                                          if not r["stats"]["cache_hit"]] or [0])),
+                     mean_solve_ms=1e3 * float(np.mean([r["stats"]["solve_s"] for r in res]))))
+    log("serving: %s" % json.dumps(summ))
+    return summ
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -85,6 +139,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 comparison path")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--serve-requests", type=int, default=200, help="requests in the serving stream (0: skip)")
+    ap.add_argument("--serve-qps", type=float, default=10.0)
+    ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-seed", type=int, default=0)
+    ap.add_argument("--no-vp", action="store_true", help="skip the vortex-particle kernels")
     ap.add_argument("--copy-calib", action="store_true",
                     help="run one elementwise copy of 256 MiB first (HBM counter calibration under a profiler)")
     args = ap.parse_args()
@@ -175,6 +234,10 @@ def main():
             (dcl.max(), valid, k, t_fp64 / t_step))
     value = valid / t_step
 
+    vp = None if args.no_vp else vp_bench(ctx, log)
+    srv = serve_bench(ctx, args.serve_requests, args.serve_qps, args.serve_slots, args.serve_seed, log) \
+        if args.serve_requests > 0 else None
+
     cpu = None
     if not args.no_cpu:
         nth = args.cpu_threads or min(os.cpu_count() or 1, 16)
@@ -207,6 +270,8 @@ def main():
                                  if not args.small else None),
                      "kernel": "%s (A32 x 64-column multivector, %.0f MB of A per launch)" % (kname, 4.0 * n * n / 1e6)},
         "cpu_baseline": cpu,
+        "latency_p50_ms": srv["p50_ms"] if srv else None,
+        "latency_p99_ms": srv["p99_ms"] if srv else None,
         "extra": {
             "assembly_ginfl_per_s_fp64": pairs / t_a64 / 1e6, "assembly_ms_fp64": t_a64,
             "assembly_ginfl_per_s_fp32": pairs / t_a32 / 1e6, "assembly_ms_fp32": t_a32,
@@ -215,6 +280,8 @@ def main():
             "max_abs_dCL": float(dcl.max()) if dcl is not None else None,
             "resid_f64_avg_ms": rs_ms / max(rs_n, 1),
             "gemm_f32_avg_ms": gemm_avg,
+            "vortex_particles": vp,
+            "serving": srv,
             "gemm_shape": rep.get("gemm_shape"),
             "block_jacobi_blocks": rep.get("block_jacobi_blocks"),
         },

@@ -175,5 +175,48 @@ def config_c2():
     return wing("0012", 1.0, 10.0, 160, 80, "cosine", 30.0)
 
 
+def config_c4():
+    """Swept (20 deg at the quarter chord), tapered (0.5) NACA 4412, span 8,
+    60 panels around x 50 strips (6,116 body tris); one buffer wake row of
+    one chord behind the trailing edge."""
+    return wing("4412", 1.0, 8.0, 60, 50, "uniform", 1.0, taper=0.5, sweep_deg=20.0)
+
+
+def c4_particles(mesh, steps=400, per_edge=4, dt=0.02, seed=0):
+    """Synthetic vortex-particle wake of the C4 case: per_edge particles per
+    trailing-edge edge per step, convected downstream at |V| = 1 for `steps`
+    steps (dt chord/|V|), with seeded jitter; strengths of a trailing vortex
+    sheet; radius 0.1 x mean panel size.  Returns xyz, alpha, radius."""
+    rng = np.random.default_rng(seed)
+    xyz = np.asarray(mesh["xyz"])
+    tri = np.asarray(mesh["tri"])
+    ids = np.asarray(mesh["ids"])
+    w = tri[ids > 1000]
+    # wake panels (TE_j, far_j, far_j+1) and (TE_j, far_j+1, TE_j+1): TE edge = (a, c) of the second kind
+    te = [(t[0], t[2]) for t in w[1::2]]
+    p0 = np.array([xyz[a] for a, b in te])
+    p1 = np.array([xyz[b] for a, b in te])
+    body = tri[ids < 1000]
+    e = xyz[body[:, 1]] - xyz[body[:, 0]]
+    f = xyz[body[:, 2]] - xyz[body[:, 0]]
+    mean_size = float(np.mean(np.sqrt(0.5 * np.linalg.norm(np.cross(e, f), axis=1))))
+    n_edges = len(te)
+    out_x, out_a = [], []
+    for s in range(steps):
+        for q in range(per_edge):
+            u = (q + 0.5) / per_edge
+            x = p0 + u * (p1 - p0)
+            x = x + np.array([1.0, 0.0, 0.0]) * (dt * (steps - s - 0.5))
+            x = x + rng.normal(scale=0.01, size=x.shape)
+            seg = p1 - p0
+            g = np.sin(np.linspace(0, np.pi, n_edges))[:, None]
+            a = seg * g * 0.05 + rng.normal(scale=1e-3, size=x.shape)
+            out_x.append(x)
+            out_a.append(a)
+    X = np.concatenate(out_x)
+    A = np.concatenate(out_a)
+    return X, A, np.full(len(X), 0.1 * mean_size)
+
+
 def small_wing(code="4412", n_chord=20, n_span=6, span=4.0, spacing="uniform", taper=1.0, sweep_deg=0.0):
     return wing(code, 1.0, span, n_chord, n_span, spacing, 10.0, taper, sweep_deg)
