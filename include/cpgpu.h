@@ -20,8 +20,10 @@
  *
  * Conventions: every call returns cp_status; nothing exits or prompts; host
  * buffers are caller owned, device buffers context owned; matrices are
- * column-major.  One context per GPU; a context is not reentrant (serialise
- * calls on one context; use one context per host thread).
+ * column-major.  One context per GPU.  Each geometry has its own stream and
+ * lock: calls on one geometry serialise, calls on different geometries may
+ * run concurrently from different host threads.  Barnes-Hut trees use the
+ * context's stream and lock.
  */
 #ifndef CPGPU_H
 #define CPGPU_H
