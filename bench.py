@@ -43,6 +43,10 @@ def gemm_kernel_name(shape):
         return "gemm_f32_x4_kernel<8>"
     if shape["variant"] == 2:
         return "gemm_f32_x4_kernel<4>"
+    if shape["variant"] == 3:
+        return "gemm_f32_x4s3_kernel<4>"
+    if shape["variant"] == 4:
+        return "gemm_f32_x4s3_kernel<2>"
     return "gemm_f32_kernel<%d>" % shape["rt"]
 
 

@@ -23,7 +23,8 @@ struct BatchedOpts {
 // kslices = split of the K range over workgroups
 struct GemmShape {
     int variant = 0;  // 0: 4-byte A loads, rt tiles per wave; 1/2: 16-byte A loads, 4 interleaved
-                      // tiles, 16/8 K values in flight
+                      // tiles, 16/8 K values per chunk, two register stages; 3/4: the same with
+                      // 8/4 K values per chunk and three stages
     int rt = 2;
     int kslices = 8;
 };

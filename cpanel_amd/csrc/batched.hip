@@ -187,6 +187,76 @@ __global__ __launch_bounds__(256) void gemm_f32_x4_kernel(const float* __restric
         }
 }
 
+// Same tiling as gemm_f32_x4_kernel with three register stages: the loads of
+// chunk i+2 are in flight while chunk i is multiplied, so HBM latency up to
+// two chunks' MFMA time is hidden by one wavefront.  Chunks past the K range
+// load zeros (their MFMAs add nothing).
+template <int U>
+__global__ __launch_bounds__(256) void gemm_f32_x4s3_kernel(const float* __restrict__ A, int64_t lda, int n,
+                                                            const float* __restrict__ P, float* __restrict__ part,
+                                                            int kchunk, int KS) {
+    const int bid = blockIdx.x;
+    const int ks = bid % KS, rb = bid / KS;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t row0 = ((int64_t)rb * 4 + wave) * 128;
+    const int k0 = ks * kchunk, k1 = min(n, k0 + kchunk);
+    const int m = lane & 31, kh = lane >> 5;
+    floatx16 acc[4][2];
+#pragma unroll
+    for (int t = 0; t < 4; t++) acc[t][0] = acc[t][1] = floatx16{};
+    const float* Ab = A + row0 + 4 * m;
+    const float* Pb = P + m;
+    constexpr int D = 2 * U;  // K values per chunk
+    float4 a0[U], a1[U], a2[U];
+    float b0[2][U], b1[2][U], b2[2][U];
+    auto load = [&](int k, float4 (&aa)[U], float (&bb)[2][U]) {
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int64_t kk = k + 2 * u + kh;
+            const bool ok = kk < k1;
+            const int64_t kc = ok ? kk : 0;
+            float4 v = *reinterpret_cast<const float4*>(Ab + kc * lda);
+            aa[u] = ok ? v : make_float4(0.f, 0.f, 0.f, 0.f);
+            bb[0][u] = ok ? Pb[kc * KP] : 0.f;
+            bb[1][u] = ok ? Pb[kc * KP + 32] : 0.f;
+        }
+    };
+    auto mma = [&](const float4 (&aa)[U], const float (&bb)[2][U]) {
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const float av[4] = {aa[u].x, aa[u].y, aa[u].z, aa[u].w};
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                acc[t][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[t], bb[0][u], acc[t][0], 0, 0, 0);
+                acc[t][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[t], bb[1][u], acc[t][1], 0, 0, 0);
+            }
+        }
+    };
+    // chunks rounded up to a multiple of 3 (at most two all-zero chunks)
+    const int nch = (k1 - k0 + D - 1) / D;
+    const int kend = k0 + ((nch + 2) / 3) * 3 * D;
+    load(k0, a0, b0);
+    load(k0 + D, a1, b1);
+    for (int k = k0; k < kend; k += 3 * D) {
+        load(k + 2 * D, a2, b2);
+        mma(a0, b0);
+        load(k + 3 * D, a0, b0);
+        mma(a1, b1);
+        load(k + 4 * D, a1, b1);
+        mma(a2, b2);
+    }
+    float* out = part + (int64_t)ks * lda * KP;
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+#pragma unroll
+        for (int j = 0; j < 16; j++) {
+            const int rho = 8 * (j >> 2) + 4 * kh + (j & 3);
+            const int64_t r = row0 + 4 * rho + t;
+            out[r * KP + m] = acc[t][0][j];
+            out[r * KP + 32 + m] = acc[t][1][j];
+        }
+}
+
 __global__ __launch_bounds__(256) void gemm_f64acc_kernel(const float* __restrict__ A, int64_t lda, int n,
                                                           const double* __restrict__ X, double* __restrict__ part,
                                                           int kchunk, int KS) {
@@ -686,7 +756,7 @@ BatchedSolver::BatchedSolver(int n, int64_t lda, int capacity) : n_(n), cap_(cap
     // CPGPU_GEMM_SHAPE="variant,rt,kslices" pins the shape (tests, tuning runs)
     if (const char* e = std::getenv("CPGPU_GEMM_SHAPE")) {
         int v = 0, r = 2, k = 8;
-        if (std::sscanf(e, "%d,%d,%d", &v, &r, &k) == 3 && v >= 0 && v <= 2 && (r == 1 || r == 2 || r == 4) &&
+        if (std::sscanf(e, "%d,%d,%d", &v, &r, &k) == 3 && v >= 0 && v <= 4 && (r == 1 || r == 2 || r == 4) &&
             (k == 4 || k == 8 || k == 16 || k == 32)) {
             shape_.variant = v;
             shape_.rt = r;
@@ -710,8 +780,14 @@ void BatchedSolver::launch_gemm32(const GemmShape& sh, const float* A32, const f
         dim3 grid((unsigned)((lda_ / 512) * KS));
         if (sh.variant == 1)
             hipLaunchKernelGGL(gemm_f32_x4_kernel<8>, grid, dim3(256), 0, s, A32, lda_, n_, P, part32_, kchunk, KS);
-        else
+        else if (sh.variant == 2)
             hipLaunchKernelGGL(gemm_f32_x4_kernel<4>, grid, dim3(256), 0, s, A32, lda_, n_, P, part32_, kchunk, KS);
+        else if (sh.variant == 3)
+            hipLaunchKernelGGL(gemm_f32_x4s3_kernel<4>, grid, dim3(256), 0, s, A32, lda_, n_, P, part32_, kchunk,
+                               KS);
+        else
+            hipLaunchKernelGGL(gemm_f32_x4s3_kernel<2>, grid, dim3(256), 0, s, A32, lda_, n_, P, part32_, kchunk,
+                               KS);
         CP_LAUNCH_CHECK();
         return;
     }
@@ -749,7 +825,7 @@ void BatchedSolver::tune(const float* A32, hipStream_t s) {
             g.kslices = ks;
             cand.push_back(g);
         }
-    for (int v : {1, 2})
+    for (int v : {1, 2, 3, 4})
         for (int ks : {4, 8, 16, 32}) {
             GemmShape g;
             g.variant = v;

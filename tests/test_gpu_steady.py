@@ -205,7 +205,7 @@ def test_block_jacobi_same_solution_fewer_iterations(c1s):
     assert sum(r["iters"] for r in rb) < sum(r["iters"] for r in rj)
 
 
-@pytest.mark.parametrize("shape", ["0,1,8", "0,2,16", "0,4,4", "1,4,8", "2,4,32"])
+@pytest.mark.parametrize("shape", ["0,1,8", "0,2,16", "0,4,4", "1,4,8", "2,4,32", "3,4,16", "4,4,32", "3,4,4"])
 def test_gemm_shapes(ctx, shape, monkeypatch):
     """Every GEMM variant/shape the autotuner can pick gives the FP32 product
     (FP32 accumulation bound as in test_matvec_kernels)."""
