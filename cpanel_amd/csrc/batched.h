@@ -24,7 +24,7 @@ struct BatchedOpts {
 struct GemmShape {
     int variant = 0;  // 0: 4-byte A loads, rt tiles per wave; 1/2: 16-byte A loads, 4 interleaved
                       // tiles, 16/8 K values per chunk, two register stages; 3/4: the same with
-                      // 8/4 K values per chunk and three stages
+                      // 8/4 K values per chunk and three stages; 5: bf16x3 on packed tiles
     int rt = 2;
     int kslices = 8;
 };
@@ -76,6 +76,11 @@ public:
         bj_nblocks_ = nblocks;
         bj_maxb_ = max_b;
     }
+    // bf16 hi/lo tiles of A32 (pack_a_bf16x2); enables GEMM variant 5
+    void set_packed(const void* apk, int nkb) {
+        apk_ = apk;
+        nkb_ = nkb;
+    }
     // FP64 near-field corrections of A32 and B32 (device CSR by row)
     void set_near(const int* rowptr, const int* cols, const double* dA, const double* dB) {
         near_rowptr_ = rowptr;
@@ -89,8 +94,12 @@ private:
                     int kc, const BatchedOpts& o, double* x, int64_t ldx, SolveStats* stats, hipStream_t s);
     void launch_gemm32(const GemmShape& sh, const float* A32, const float* P, hipStream_t s);
     void apply_block_jacobi(const float* P, float* Y, hipStream_t s);
+    const void* apk_ = nullptr;
+    int nkb_ = 0;
+    void* ppk_ = nullptr;  // bf16 hi/lo tiles of the multivector being multiplied
     bool bj_ = false;
     bool bj_mfma_ = true;  // matrix-core block product (else vector ALU)
+    bool bf16x3_ = true;   // bf16x3 GEMM is a tuner candidate
     const int* bj_perm_ = nullptr;
     const int* bj_ptr_ = nullptr;
     const int64_t* bj_off_ = nullptr;
@@ -122,6 +131,10 @@ private:
 };
 
 void launch_sigma(int n, int64_t lda, const double* normals, const double* vinf4, double* sigma, hipStream_t s);
+// bf16 hi/lo tiles of A32 for the bf16x3 GEMM: K blocks of 16 and bytes
+int packed_kblocks(int n);
+size_t packed_a_bytes(int64_t lda, int n);
+void launch_pack_a(const float* A32, int64_t lda, int n, void* out, hipStream_t s);
 // copies the diagonal blocks of A32 into packed column-major storage
 void launch_bj_gather(const float* A32, int64_t lda, const int* perm, const int* ptr, const int64_t* off, int nblocks,
                       int max_b, float* out, hipStream_t s);

@@ -257,6 +257,157 @@ __global__ __launch_bounds__(256) void gemm_f32_x4s3_kernel(const float* __restr
         }
 }
 
+// ---------------------------------------------------------------- bf16x3
+// FP32 product from bf16 matrix cores: every FP32 value x is split as
+// x = hi + lo with hi = bf16(x) and lo = bf16(x - hi) (|x - hi - lo| <=
+// 2^-17 |x|), and a*p ~ hi_a hi_p + hi_a lo_p + lo_a hi_p (the dropped lo*lo
+// term is <= 2^-16 |a p|), accumulated in FP32 by v_mfma_f32_32x32x16_bf16.
+// The inner Krylov solve only needs a 1e-3 reduction per refinement sweep,
+// and the FP64 refinement residual uses the exact FP32 matrix, so the
+// refined solution is unchanged; the product costs a third of the FP32
+// matrix-core time and becomes bound by reading the matrix from HBM.
+//
+// Packed tiles (one per 32 rows x 16 K values): 64 lanes x 32 bytes, lane l
+// holding row l%32, K values 8(l/32)..8(l/32)+7 as 8 bf16 hi then 8 bf16 lo:
+// the operand layout of the MFMA, so each lane loads its operands with two
+// 16-byte loads and a wavefront reads one contiguous 2 KiB tile.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+__device__ inline uint32_t bf16_bits_rne(float x) {
+    uint32_t u = __float_as_uint(x);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return u >> 16;
+}
+
+__device__ inline void split8(const float (&v)[8], uint4& hi, uint4& lo) {
+    uint32_t h[8], l[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        h[i] = bf16_bits_rne(v[i]);
+        const float r = v[i] - __uint_as_float(h[i] << 16);
+        l[i] = bf16_bits_rne(r);
+    }
+    hi = make_uint4(h[0] | (h[1] << 16), h[2] | (h[3] << 16), h[4] | (h[5] << 16), h[6] | (h[7] << 16));
+    lo = make_uint4(l[0] | (l[1] << 16), l[2] | (l[3] << 16), l[4] | (l[5] << 16), l[6] | (l[7] << 16));
+}
+
+// A (column-major, lda) -> tiles [row tile][K block]; K values >= n are 0
+__global__ __launch_bounds__(256) void pack_a_bf16x2_kernel(const float* __restrict__ A, int64_t lda, int n,
+                                                            int nkb, int64_t ntiles, uint4* __restrict__ out) {
+    const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t tile = g >> 6;
+    if (tile >= ntiles) return;
+    const int lane = (int)(g & 63);
+    const int64_t rt = tile / nkb;
+    const int kb = (int)(tile % nkb);
+    const int64_t row = rt * 32 + (lane & 31);
+    const int kbase = kb * 16 + 8 * (lane >> 5);
+    float v[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        const int k = kbase + i;
+        v[i] = k < n ? A[row + (int64_t)k * lda] : 0.f;
+    }
+    uint4 hi, lo;
+    split8(v, hi, lo);
+    out[2 * g] = hi;
+    out[2 * g + 1] = lo;
+}
+
+// P (row-major lda x 64) -> tiles [K block][column tile 0/1]; rows >= n are 0
+__global__ __launch_bounds__(256) void pack_p_bf16x2_kernel(const float* __restrict__ P, int n, int nkb,
+                                                            uint4* __restrict__ out) {
+    const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t tile = g >> 6;
+    if (tile >= 2 * (int64_t)nkb) return;
+    const int lane = (int)(g & 63);
+    const int kb = (int)(tile >> 1), ct = (int)(tile & 1);
+    const int col = 32 * ct + (lane & 31);
+    const int kbase = kb * 16 + 8 * (lane >> 5);
+    float v[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        const int k = kbase + i;
+        v[i] = k < n ? P[(int64_t)k * KP + col] : 0.f;
+    }
+    uint4 hi, lo;
+    split8(v, hi, lo);
+    out[2 * g] = hi;
+    out[2 * g + 1] = lo;
+}
+
+// Y_part[ks] = A~ P~ over the K blocks of slice ks.  Wavefront: 4 row tiles
+// (128 rows) x 2 column tiles; workgroup: 512 rows.  Two register stages.
+__global__ __launch_bounds__(256) void gemm_bf16x3_kernel(const uint4* __restrict__ Apk, int nkb,
+                                                          const uint4* __restrict__ Ppk, float* __restrict__ part,
+                                                          int64_t lda, int kbchunk, int KS) {
+    const int bid = blockIdx.x;
+    const int ks = bid % KS, rb = bid / KS;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t rt0 = ((int64_t)rb * 4 + wave) * 4;
+    const int kb0 = ks * kbchunk, kb1 = min(nkb, kb0 + kbchunk);
+    floatx16 acc[4][2];
+#pragma unroll
+    for (int t = 0; t < 4; t++) acc[t][0] = acc[t][1] = floatx16{};
+    uint4 ah[4], al[4], ph[2], pl[2], nah[4], nal[4], nph[2], npl[2];
+    auto load = [&](int kb, uint4 (&h)[4], uint4 (&l)[4], uint4 (&qh)[2], uint4 (&ql)[2]) {
+        const bool ok = kb < kb1;
+        const int kc = ok ? kb : kb0;
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            const uint4* src = Apk + (((rt0 + t) * nkb + kc) * 64 + lane) * 2;
+            h[t] = src[0];
+            l[t] = src[1];
+        }
+#pragma unroll
+        for (int c = 0; c < 2; c++) {
+            const uint4* src = Ppk + (((int64_t)kc * 2 + c) * 64 + lane) * 2;
+            qh[c] = src[0];
+            ql[c] = src[1];
+        }
+    };
+    if (kb0 < kb1) load(kb0, ah, al, ph, pl);
+    for (int kb = kb0; kb < kb1; kb++) {
+        const bool more = kb + 1 < kb1;
+        if (more) load(kb + 1, nah, nal, nph, npl);
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            const bf16x8 xh = __builtin_bit_cast(bf16x8, ah[t]);
+            const bf16x8 xl = __builtin_bit_cast(bf16x8, al[t]);
+#pragma unroll
+            for (int c = 0; c < 2; c++) {
+                const bf16x8 yh = __builtin_bit_cast(bf16x8, ph[c]);
+                const bf16x8 yl = __builtin_bit_cast(bf16x8, pl[c]);
+                acc[t][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xl, yh, acc[t][c], 0, 0, 0);
+                acc[t][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xh, yl, acc[t][c], 0, 0, 0);
+                acc[t][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xh, yh, acc[t][c], 0, 0, 0);
+            }
+        }
+        if (more) {
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                ah[t] = nah[t];
+                al[t] = nal[t];
+            }
+#pragma unroll
+            for (int c = 0; c < 2; c++) {
+                ph[c] = nph[c];
+                pl[c] = npl[c];
+            }
+        }
+    }
+    float* out = part + (int64_t)ks * lda * KP;
+    const int m = lane & 31, kh = lane >> 5;
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+#pragma unroll
+        for (int j = 0; j < 16; j++) {
+            const int64_t r = (rt0 + t) * 32 + 8 * (j >> 2) + 4 * kh + (j & 3);
+            out[r * KP + m] = acc[t][0][j];
+            out[r * KP + 32 + m] = acc[t][1][j];
+        }
+}
+
 __global__ __launch_bounds__(256) void gemm_f64acc_kernel(const float* __restrict__ A, int64_t lda, int n,
                                                           const double* __restrict__ X, double* __restrict__ part,
                                                           int kchunk, int KS) {
@@ -703,6 +854,18 @@ inline unsigned nblocks(int64_t work) { return (unsigned)((work + kT - 1) / kT);
 
 }  // namespace
 
+int packed_kblocks(int n) { return (n + 15) / 16; }
+
+size_t packed_a_bytes(int64_t lda, int n) { return (size_t)(lda / 32) * packed_kblocks(n) * 2048; }
+
+void launch_pack_a(const float* A32, int64_t lda, int n, void* out, hipStream_t s) {
+    const int nkb = packed_kblocks(n);
+    const int64_t ntiles = (lda / 32) * (int64_t)nkb;
+    hipLaunchKernelGGL(pack_a_bf16x2_kernel, dim3((unsigned)((ntiles * 64 + 255) / 256)), dim3(256), 0, s, A32, lda,
+                       n, nkb, ntiles, (uint4*)out);
+    CP_LAUNCH_CHECK();
+}
+
 void launch_sigma(int n, int64_t lda, const double* normals, const double* vinf4, double* sigma, hipStream_t s) {
     hipLaunchKernelGGL(sigma_kernel, dim3(nblocks(lda)), dim3(kT), 0, s, n, normals, lda, vinf4, sigma);
     CP_LAUNCH_CHECK();
@@ -714,8 +877,8 @@ size_t BatchedSolver::workspace_bytes(int64_t lda) {
     const int KSmax = 32;
     const int nrb = (int)(lda / kRedRows);
     // 19 sub-buffers, each rounded up to 256 bytes
-    return 4 * mv64 + 9 * mv32 + (size_t)lda * sizeof(float) + KSmax * mv32 + KSmax * mv64 +
-           (size_t)4 * nrb * KP * sizeof(double) + KP * Q_NQ * sizeof(double) + KP * sizeof(int) + 19 * 256;
+    return 4 * mv64 + 10 * mv32 + (size_t)lda * sizeof(float) + KSmax * mv32 + KSmax * mv64 +
+           (size_t)4 * nrb * KP * sizeof(double) + KP * Q_NQ * sizeof(double) + KP * sizeof(int) + 20 * 256;
 }
 
 BatchedSolver::BatchedSolver(int n, int64_t lda, int capacity) : n_(n), cap_(capacity), lda_(lda) {
@@ -737,6 +900,7 @@ BatchedSolver::BatchedSolver(int n, int64_t lda, int capacity) : n_(n), cap_(cap
     float** f32[9] = {&r_, &r0_, &p_, &v_, &sv_, &t_, &y_, &z_, &d_};
     for (auto q : f32) *q = (float*)take(mv32);
     invd_ = (float*)take((size_t)lda * sizeof(float));
+    ppk_ = (void*)take(mv32);  // packed_kblocks(n) * 4 KiB <= lda / 16 * 4 KiB
     part32_ = (float*)take(KSmax * mv32);
     part64_ = (double*)take(KSmax * mv64);
     dpart_ = (double*)take((size_t)4 * nrb_ * KP * sizeof(double));
@@ -751,12 +915,14 @@ BatchedSolver::BatchedSolver(int n, int64_t lda, int capacity) : n_(n), cap_(cap
     int rb = (int)(lda / 256);
     shape_.rt = 2;
     shape_.kslices = rb >= 64 ? 8 : (rb >= 16 ? 16 : 32);
+    // CPGPU_NO_BF16X3=1 keeps the inner product in FP32 matrix-core arithmetic
+    if (const char* e = std::getenv("CPGPU_NO_BF16X3")) bf16x3_ = e[0] != '1';
     // CPGPU_BJ_VALU=1 selects the vector-ALU block-Jacobi product (tests)
     if (const char* e = std::getenv("CPGPU_BJ_VALU")) bj_mfma_ = e[0] != '1';
     // CPGPU_GEMM_SHAPE="variant,rt,kslices" pins the shape (tests, tuning runs)
     if (const char* e = std::getenv("CPGPU_GEMM_SHAPE")) {
         int v = 0, r = 2, k = 8;
-        if (std::sscanf(e, "%d,%d,%d", &v, &r, &k) == 3 && v >= 0 && v <= 4 && (r == 1 || r == 2 || r == 4) &&
+        if (std::sscanf(e, "%d,%d,%d", &v, &r, &k) == 3 && v >= 0 && v <= 5 && (r == 1 || r == 2 || r == 4) &&
             (k == 4 || k == 8 || k == 16 || k == 32)) {
             shape_.variant = v;
             shape_.rt = r;
@@ -776,6 +942,19 @@ void BatchedSolver::launch_gemm32(const GemmShape& sh, const float* A32, const f
     const int KS = sh.kslices;
     int kchunk = (int)round_up((n_ + KS - 1) / KS, 16);
     TimedScope ts("gemm_f32", s);
+    if (sh.variant == 5) {  // bf16x3 on packed tiles
+        if (!apk_) throw HipError{hipErrorInvalidValue, "bf16x3 GEMM without packed A"};
+        const int nkb = nkb_;
+        hipLaunchKernelGGL(pack_p_bf16x2_kernel, dim3((unsigned)((2 * (int64_t)nkb * 64 + 255) / 256)), dim3(256), 0,
+                           s, P, n_, nkb, (uint4*)ppk_);
+        CP_LAUNCH_CHECK();
+        const int kbchunk = (nkb + KS - 1) / KS;
+        dim3 grid((unsigned)((lda_ / 512) * KS));
+        hipLaunchKernelGGL(gemm_bf16x3_kernel, grid, dim3(256), 0, s, (const uint4*)apk_, nkb, (const uint4*)ppk_,
+                           part32_, lda_, kbchunk, KS);
+        CP_LAUNCH_CHECK();
+        return;
+    }
     if (sh.variant >= 1) {  // 16-byte A loads, 512 rows per workgroup
         dim3 grid((unsigned)((lda_ / 512) * KS));
         if (sh.variant == 1)
@@ -805,10 +984,11 @@ void BatchedSolver::tune(const float* A32, hipStream_t s) {
     if (tuned_) return;
     tuned_ = true;
     static std::mutex mu;
-    static std::map<int64_t, GemmShape> cache;  // per leading dimension, process wide
+    static std::map<int64_t, GemmShape> cache;  // per leading dimension (and bf16x3 use), process wide
+    const int64_t ckey = 2 * lda_ + ((apk_ && bf16x3_) ? 1 : 0);
     {
         std::lock_guard<std::mutex> lk(mu);
-        auto it = cache.find(lda_);
+        auto it = cache.find(ckey);
         if (it != cache.end()) {
             shape_ = it->second;
             return;
@@ -825,8 +1005,9 @@ void BatchedSolver::tune(const float* A32, hipStream_t s) {
             g.kslices = ks;
             cand.push_back(g);
         }
-    for (int v : {1, 2, 3, 4})
+    for (int v : {1, 2, 3, 4, 5})
         for (int ks : {4, 8, 16, 32}) {
+            if (v == 5 && (!apk_ || !bf16x3_)) continue;
             GemmShape g;
             g.variant = v;
             g.rt = 4;
@@ -866,7 +1047,7 @@ void BatchedSolver::tune(const float* A32, hipStream_t s) {
     (void)hipEventDestroy(e1);
     shape_ = bestg;
     std::lock_guard<std::mutex> lk(mu);
-    cache[lda_] = bestg;
+    cache[ckey] = bestg;
 }
 
 void BatchedSolver::matvec32(const float* A32, const float* P, float* Y, hipStream_t s) {

@@ -152,6 +152,7 @@ struct cp_geom {
     DevBuf body, wake, cxyz, fold_ptr, fold_w, fold_c, normals;
     DevBuf W;          // wake influence N x Nw (FP64)
     DevBuf A64, B64, A32, B32;
+    DevBuf A16;        // bf16 hi/lo tiles of A32 (bf16x3 inner products)
     bool have64 = false, have32 = false;
     // FP64 near-field corrections of A32/B32 (entries within one panel
     // diameter: |c_i - c_j| <= longSide_j)
@@ -605,6 +606,8 @@ cp_status cp_assemble(cp_geom* g, cp_prec prec) {
                                        g->near_rows.as<int>(), g->near_cols.as<int>(), g->near_dA.as<double>(),
                                        g->near_dB.as<double>(), g->stream);
             }
+            if (!g->A16.p) dalloc(c, g->A16, packed_a_bytes(lda, g->dev.n));
+            launch_pack_a(g->A32.as<float>(), lda, g->dev.n, g->A16.p, g->stream);
             g->have32 = true;
             g->bj_ready = false;  // inverses follow the new A32
         }
@@ -699,6 +702,7 @@ cp_status cp_solve(cp_geom* g, cp_prec prec, int32_t k, const cp_freestream* fs,
             }
             g->batched->set_near(g->near_rowptr.as<int>(), g->near_cols.as<int>(), g->near_dA.as<double>(),
                                  g->near_dB.as<double>());
+            g->batched->set_packed(g->A16.p, packed_kblocks(n));
             if (opts->precond != 1) {
                 ensure_block_jacobi(g);
             }
@@ -786,6 +790,7 @@ cp_status cp_matvec(cp_geom* g, int kernel, int32_t k, const double* X, double* 
         }
         g->batched->set_near(g->near_rowptr.as<int>(), g->near_cols.as<int>(), g->near_dA.as<double>(),
                              g->near_dB.as<double>());
+        g->batched->set_packed(g->A16.p, packed_kblocks(n));
         // row-major lda x 64 multivectors
         const size_t elems = (size_t)lda * kRhsTile;
         if (kernel == 1) {

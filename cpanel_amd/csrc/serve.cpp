@@ -78,7 +78,7 @@ uint64_t estimate_bytes(const Req& r) {
     for (int32_t id : r.ids) (id > 1000 ? nw : nb)++;
     (void)nt;
     const int64_t lda = round_up(nb, 512);
-    uint64_t b = 2ull * lda * nb * 4;                      // A32, B32
+    uint64_t b = 3ull * lda * (nb + 15) * 4;               // A32, B32, bf16 tiles of A32
     b += (uint64_t)lda * std::max<int64_t>(nw, 1) * 8;     // wake block W
     b += (uint64_t)nb * 120 * 28;                          // near-field CSR (~120 entries per row)
     b += (uint64_t)nb * 768 * 4;                           // block-Jacobi inverses
