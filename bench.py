@@ -47,6 +47,8 @@ def gemm_kernel_name(shape):
         return "gemm_f32_x4s3_kernel<4>"
     if shape["variant"] == 4:
         return "gemm_f32_x4s3_kernel<2>"
+    if shape["variant"] == 5:
+        return "gemm_bf16x3_kernel"
     return "gemm_f32_kernel<%d>" % shape["rt"]
 
 
@@ -229,6 +231,21 @@ def main():
     rep = g.solver_report()
     log("solver: %s" % json.dumps(rep))
     kname = gemm_kernel_name(rep.get("gemm_shape"))
+    traffic = profiled_traffic("mixed_solve_naca0012_ar10_64cond", kname) if not args.small else None
+    if rep.get("gemm_shape", {}).get("variant") == 5:
+        # bf16x3: the product needs a third of the FP32 matrix-core time and is
+        # bound by streaming the packed matrix (4 bytes per entry: bf16 hi + lo)
+        # plus the packed multivector and the output
+        alg_bytes = 4.0 * n * n + 2 * 4.0 * n * 64
+        gbs = alg_bytes / (gemm_avg * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": gbs, "peak": MI355X_HBM_GBS, "unit": "GB/s", "frac": gbs / MI355X_HBM_GBS,
+                "traffic": traffic,
+                "kernel": "%s (A32 as bf16 hi/lo tiles x 64-column multivector, %.0f MB per launch; %.1f "
+                          "FP32-equivalent TFLOP/s)" % (kname, alg_bytes / 1e6, achieved)}
+    else:
+        roof = {"bound": "mfma", "achieved": achieved, "peak": MI355X_FP32_MFMA_TFLOPS, "unit": "TFLOP/s",
+                "frac": achieved / MI355X_FP32_MFMA_TFLOPS, "traffic": traffic,
+                "kernel": "%s (A32 x 64-column multivector, %.0f MB of A per launch)" % (kname, 4.0 * n * n / 1e6)}
     dcl = None
     valid = k
     if fp64 is not None:
@@ -263,16 +280,14 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32+fp64-refine",
+        "dtype": ("fp32(bf16x3)+fp64-refine" if rep.get("gemm_shape", {}).get("variant") == 5
+                  else "fp32+fp64-refine"),
         "data": "synthetic (generated NACA wing mesh)",
         "config": {"workload": ("naca4412_ar6_4956tri_64cond" if args.small else "naca0012_ar10_25916tri_64cond"),
-                   "n_body": n, "conditions": k, "precision": "FP32 A,B + FP32 BiCGSTAB + FP64 refinement",
+                   "n_body": n, "conditions": k, "precision": "FP32 A,B (FP64 near field) + FP32 BiCGSTAB%s + FP64 refinement" % (
+                       " (bf16x3 matrix products)" if rep.get("gemm_shape", {}).get("variant") == 5 else ""),
                    "tol": 1e-10},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": MI355X_FP32_MFMA_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / MI355X_FP32_MFMA_TFLOPS,
-                     "traffic": (profiled_traffic("mixed_solve_naca0012_ar10_64cond", kname)
-                                 if not args.small else None),
-                     "kernel": "%s (A32 x 64-column multivector, %.0f MB of A per launch)" % (kname, 4.0 * n * n / 1e6)},
+        "roofline": roof,
         "cpu_baseline": cpu,
         "latency_p50_ms": srv["p50_ms"] if srv else None,
         "latency_p99_ms": srv["p99_ms"] if srv else None,

@@ -941,20 +941,24 @@ BatchedSolver::~BatchedSolver() {
 void BatchedSolver::launch_gemm32(const GemmShape& sh, const float* A32, const float* P, hipStream_t s) {
     const int KS = sh.kslices;
     int kchunk = (int)round_up((n_ + KS - 1) / KS, 16);
-    TimedScope ts("gemm_f32", s);
     if (sh.variant == 5) {  // bf16x3 on packed tiles
         if (!apk_) throw HipError{hipErrorInvalidValue, "bf16x3 GEMM without packed A"};
         const int nkb = nkb_;
-        hipLaunchKernelGGL(pack_p_bf16x2_kernel, dim3((unsigned)((2 * (int64_t)nkb * 64 + 255) / 256)), dim3(256), 0,
-                           s, P, n_, nkb, (uint4*)ppk_);
+        {
+            TimedScope tp("pack_p", s);
+            hipLaunchKernelGGL(pack_p_bf16x2_kernel, dim3((unsigned)((2 * (int64_t)nkb * 64 + 255) / 256)),
+                               dim3(256), 0, s, P, n_, nkb, (uint4*)ppk_);
+        }
         CP_LAUNCH_CHECK();
         const int kbchunk = (nkb + KS - 1) / KS;
         dim3 grid((unsigned)((lda_ / 512) * KS));
+        TimedScope ts("gemm_f32", s);
         hipLaunchKernelGGL(gemm_bf16x3_kernel, grid, dim3(256), 0, s, (const uint4*)apk_, nkb, (const uint4*)ppk_,
                            part32_, lda_, kbchunk, KS);
         CP_LAUNCH_CHECK();
         return;
     }
+    TimedScope ts("gemm_f32", s);
     if (sh.variant >= 1) {  // 16-byte A loads, 512 rows per workgroup
         dim3 grid((unsigned)((lda_ / 512) * KS));
         if (sh.variant == 1)
