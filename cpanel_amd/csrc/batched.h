@@ -67,8 +67,9 @@ public:
     // strip block-Jacobi preconditioner for the inner solve (device arrays;
     // null perm = Jacobi).  inv: FP32 inverses, column-major per block.
     void set_block_jacobi(const int* perm, const int* ptr, const int64_t* off, const float* inv, int nblocks,
-                          int max_b) {
+                          int max_b, const int* iperm = nullptr) {
         bj_ = perm != nullptr;
+        bj_iperm_ = iperm;
         bj_perm_ = perm;
         bj_ptr_ = ptr;
         bj_off_ = off;
@@ -100,7 +101,10 @@ private:
     bool bj_ = false;
     bool bj_mfma_ = true;  // matrix-core block product (else vector ALU)
     bool bf16x3_ = true;   // bf16x3 GEMM is a tuner candidate
+    bool f64_pipelined_ = true;  // pipelined FP64-accumulated product
     const int* bj_perm_ = nullptr;
+    const int* bj_iperm_ = nullptr;  // body index -> position in block order
+    float* pp_ = nullptr;            // multivector in block order
     const int* bj_ptr_ = nullptr;
     const int64_t* bj_off_ = nullptr;
     const float* bj_inv_ = nullptr;

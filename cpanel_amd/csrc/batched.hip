@@ -453,6 +453,76 @@ __global__ __launch_bounds__(256) void gemm_f64acc_kernel(const float* __restric
             }
 }
 
+// Pipelined form: a wavefront owns 64 rows as 4 interleaved 16-row tiles
+// (tile t = rows 4m + t, m = 0..15); lane l loads the float4 of rows
+// 4(l%16)..+3 at column k + l/16 and its components feed the 4 row tiles.
+// U K steps of 4 per chunk; the next chunk's loads are issued before the
+// current chunk's MFMAs.
+template <int U>
+__global__ __launch_bounds__(256) void gemm_f64acc_p_kernel(const float* __restrict__ A, int64_t lda, int n,
+                                                            const double* __restrict__ X, double* __restrict__ part,
+                                                            int kchunk, int KS) {
+    const int bid = blockIdx.x;
+    const int ks = bid % KS, rb = bid / KS;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t row0 = (int64_t)rb * 256 + wave * 64;
+    const int k0 = ks * kchunk, k1 = min(n, k0 + kchunk);
+    const int m = lane & 15, lk = lane >> 4;
+    doublex4 acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) acc[a][b] = doublex4{0, 0, 0, 0};
+    const float* Ab = A + row0 + 4 * m;
+    const double* Xb = X + m;
+    float4 a[U], an[U];
+    double b[U][4], bn[U][4];
+    auto load = [&](int k, float4 (&aa)[U], double (&bb)[U][4]) {
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int64_t kk = k + 4 * u + lk;
+            const bool ok = kk < k1;
+            const int64_t kc = ok ? kk : 0;
+            float4 v = *reinterpret_cast<const float4*>(Ab + kc * lda);
+            aa[u] = ok ? v : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+            for (int c = 0; c < 4; c++) bb[u][c] = ok ? Xb[kc * KP + 16 * c] : 0.0;
+        }
+    };
+    if (k0 < k1) load(k0, a, b);
+    for (int k = k0; k < k1; k += 4 * U) {
+        const bool more = k + 4 * U < k1;
+        if (more) load(k + 4 * U, an, bn);
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const double av[4] = {(double)a[u].x, (double)a[u].y, (double)a[u].z, (double)a[u].w};
+#pragma unroll
+            for (int rt = 0; rt < 4; rt++)
+#pragma unroll
+                for (int ct = 0; ct < 4; ct++)
+                    acc[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[rt], b[u][ct], acc[rt][ct], 0, 0, 0);
+        }
+        if (more) {
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                a[u] = an[u];
+#pragma unroll
+                for (int c = 0; c < 4; c++) b[u][c] = bn[u][c];
+            }
+        }
+    }
+    double* out = part + (int64_t)ks * lda * KP;
+#pragma unroll
+    for (int rt = 0; rt < 4; rt++)
+#pragma unroll
+        for (int ct = 0; ct < 4; ct++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int64_t r = row0 + 4 * (4 * j + lk) + rt;
+                out[r * KP + 16 * ct + m] = acc[rt][ct][j];
+            }
+}
+
 // ---------------------------------------------------------------- helpers
 // Per-column block partial sums: thread t -> column t%64, row group t/64.
 __device__ void col_partial(double v, double* sh, double* dst) {
@@ -627,11 +697,13 @@ __global__ void scal_omega_kernel(double* __restrict__ sc, const int* __restrict
     s[Q_OMEGA] = s[Q_TT] > 0 ? s[Q_TS] / s[Q_TT] : 0.0;
 }
 
-// p = r + beta (p - omega v); y = invd p
-__global__ __launch_bounds__(256) void inner_p_kernel(int64_t rows, const double* __restrict__ sc,
+// p = r + beta (p - omega v); y = invd p (Jacobi), or p also written in
+// block order to pp (row i -> iperm[i]) for the block-Jacobi product
+__global__ __launch_bounds__(256) void inner_p_kernel(int64_t rows, int n, const double* __restrict__ sc,
                                                       const int* __restrict__ act, const float* __restrict__ r,
                                                       float* __restrict__ p, const float* __restrict__ v,
-                                                      const float* __restrict__ invd, float* __restrict__ y) {
+                                                      const float* __restrict__ invd, float* __restrict__ y,
+                                                      const int* __restrict__ iperm, float* __restrict__ pp) {
     int64_t idx = (int64_t)blockIdx.x * kT + threadIdx.x;
     if (idx >= rows * KP) return;
     int c = (int)(idx & 63);
@@ -639,7 +711,9 @@ __global__ __launch_bounds__(256) void inner_p_kernel(int64_t rows, const double
     float beta = (float)sc[c * Q_NQ + Q_BETA], omega = (float)sc[c * Q_NQ + Q_OMEGA];
     float pi = r[idx] + beta * (p[idx] - omega * v[idx]);
     p[idx] = pi;
-    if (invd) y[idx] = invd[idx >> 6] * pi;  // Jacobi; else block Jacobi follows
+    if (invd) y[idx] = invd[idx >> 6] * pi;
+    const int64_t i = idx >> 6;
+    if (iperm && i < n) pp[(int64_t)iperm[i] * KP + c] = pi;
 }
 
 // v = sum parts; partial r0.v
@@ -664,11 +738,12 @@ __global__ __launch_bounds__(256) void inner_v_kernel(const float* __restrict__ 
     col_partial(acc, sh, dpart + (int64_t)blockIdx.x * KP);
 }
 
-// s = r - alpha v; z = invd s
-__global__ __launch_bounds__(256) void inner_s_kernel(int64_t rows, const double* __restrict__ sc,
+// s = r - alpha v; z = invd s (Jacobi), or s also written in block order
+__global__ __launch_bounds__(256) void inner_s_kernel(int64_t rows, int n, const double* __restrict__ sc,
                                                       const int* __restrict__ act, const float* __restrict__ r,
                                                       const float* __restrict__ v, const float* __restrict__ invd,
-                                                      float* __restrict__ s, float* __restrict__ z) {
+                                                      float* __restrict__ s, float* __restrict__ z,
+                                                      const int* __restrict__ iperm, float* __restrict__ pp) {
     int64_t idx = (int64_t)blockIdx.x * kT + threadIdx.x;
     if (idx >= rows * KP) return;
     int c = (int)(idx & 63);
@@ -677,6 +752,8 @@ __global__ __launch_bounds__(256) void inner_s_kernel(int64_t rows, const double
     float si = r[idx] - alpha * v[idx];
     s[idx] = si;
     if (invd) z[idx] = invd[idx >> 6] * si;
+    const int64_t i = idx >> 6;
+    if (iperm && i < n) pp[(int64_t)iperm[i] * KP + c] = si;
 }
 
 // t = sum parts; partial t.t and t.s
@@ -850,6 +927,58 @@ __global__ __launch_bounds__(256) void bj_apply_mfma_kernel(const int* __restric
     }
 }
 
+// Block product from the block-ordered multivector PP (row b0 + s of PP is
+// row perm[b0 + s] of P): same tiling as bj_apply_mfma_kernel, contiguous
+// operand loads, next chunk loaded before the current chunk's MFMAs.
+template <int U>
+__global__ __launch_bounds__(256) void bj_apply_pp_kernel(const int* __restrict__ perm, const int* __restrict__ ptr,
+                                                          const int64_t* __restrict__ off,
+                                                          const float* __restrict__ inv,
+                                                          const float* __restrict__ PP, float* __restrict__ Y) {
+    const int q = blockIdx.y;
+    const int b0 = ptr[q], b = ptr[q + 1] - b0;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int r0 = blockIdx.x * 64 + 32 * (wave >> 1);
+    if (r0 >= b) return;  // no LDS, no barrier: idle waves leave
+    const int c = 32 * (wave & 1) + (lane & 31);
+    const int kh = lane >> 5;
+    const int r = r0 + (lane & 31);
+    const bool rok = r < b;
+    const float* M = inv + off[q];
+    const float* Pq = PP + (int64_t)b0 * KP;
+    floatx16 acc = floatx16{};
+    float a[U], x[U], an[U], xn[U];
+    auto load = [&](int k, float (&aa)[U], float (&xx)[U]) {
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int kk = k + 2 * u + kh;
+            const bool ok = kk < b;
+            aa[u] = (ok && rok) ? M[(int64_t)kk * b + r] : 0.f;
+            xx[u] = ok ? Pq[(int64_t)kk * KP + c] : 0.f;
+        }
+    };
+    load(0, a, x);
+    for (int k = 0; k < b; k += 2 * U) {
+        const bool more = k + 2 * U < b;
+        if (more) load(k + 2 * U, an, xn);
+#pragma unroll
+        for (int u = 0; u < U; u++) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a[u], x[u], acc, 0, 0, 0);
+        if (more) {
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                a[u] = an[u];
+                x[u] = xn[u];
+            }
+        }
+    }
+    const int* pq = perm + b0;
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+        const int rr = r0 + 8 * (j >> 2) + 4 * kh + (j & 3);
+        if (rr < b) Y[(int64_t)pq[rr] * KP + c] = acc[j];
+    }
+}
+
 inline unsigned nblocks(int64_t work) { return (unsigned)((work + kT - 1) / kT); }
 
 }  // namespace
@@ -877,8 +1006,8 @@ size_t BatchedSolver::workspace_bytes(int64_t lda) {
     const int KSmax = 32;
     const int nrb = (int)(lda / kRedRows);
     // 19 sub-buffers, each rounded up to 256 bytes
-    return 4 * mv64 + 10 * mv32 + (size_t)lda * sizeof(float) + KSmax * mv32 + KSmax * mv64 +
-           (size_t)4 * nrb * KP * sizeof(double) + KP * Q_NQ * sizeof(double) + KP * sizeof(int) + 20 * 256;
+    return 4 * mv64 + 11 * mv32 + (size_t)lda * sizeof(float) + KSmax * mv32 + KSmax * mv64 +
+           (size_t)4 * nrb * KP * sizeof(double) + KP * Q_NQ * sizeof(double) + KP * sizeof(int) + 21 * 256;
 }
 
 BatchedSolver::BatchedSolver(int n, int64_t lda, int capacity) : n_(n), cap_(capacity), lda_(lda) {
@@ -901,6 +1030,7 @@ BatchedSolver::BatchedSolver(int n, int64_t lda, int capacity) : n_(n), cap_(cap
     for (auto q : f32) *q = (float*)take(mv32);
     invd_ = (float*)take((size_t)lda * sizeof(float));
     ppk_ = (void*)take(mv32);  // packed_kblocks(n) * 4 KiB <= lda / 16 * 4 KiB
+    pp_ = (float*)take(mv32);
     part32_ = (float*)take(KSmax * mv32);
     part64_ = (double*)take(KSmax * mv64);
     dpart_ = (double*)take((size_t)4 * nrb_ * KP * sizeof(double));
@@ -908,6 +1038,7 @@ BatchedSolver::BatchedSolver(int n, int64_t lda, int capacity) : n_(n), cap_(cap
     act_ = (int*)take(KP * sizeof(int));
     CP_HIP(hipMemset(y_, 0, mv32));
     CP_HIP(hipMemset(z_, 0, mv32));
+    CP_HIP(hipMemset(pp_, 0, mv32));
     CP_HIP(hipHostMalloc(&host_sc_, KP * Q_NQ * sizeof(double), hipHostMallocDefault));
     CP_HIP(hipHostMalloc(&host_act_, 2 * KP * sizeof(int), hipHostMallocDefault));
     // default shape before tuning: 8 K slices (one per XCD), more for small
@@ -915,6 +1046,8 @@ BatchedSolver::BatchedSolver(int n, int64_t lda, int capacity) : n_(n), cap_(cap
     int rb = (int)(lda / 256);
     shape_.rt = 2;
     shape_.kslices = rb >= 64 ? 8 : (rb >= 16 ? 16 : 32);
+    // CPGPU_F64_SIMPLE=1 selects the unpipelined FP64-accumulated product (tests)
+    if (const char* e = std::getenv("CPGPU_F64_SIMPLE")) f64_pipelined_ = e[0] != '1';
     // CPGPU_NO_BF16X3=1 keeps the inner product in FP32 matrix-core arithmetic
     if (const char* e = std::getenv("CPGPU_NO_BF16X3")) bf16x3_ = e[0] != '1';
     // CPGPU_BJ_VALU=1 selects the vector-ALU block-Jacobi product (tests)
@@ -1075,7 +1208,10 @@ void BatchedSolver::matvec64acc(const float* A32, const double* X, double* Y, co
     dim3 grid((unsigned)(rb * KS));
     {
         TimedScope ts("resid_f64", s);
-        hipLaunchKernelGGL(gemm_f64acc_kernel, grid, dim3(256), 0, s, A32, lda_, n_, X, part64_, kchunk, KS);
+        if (f64_pipelined_)
+            hipLaunchKernelGGL(gemm_f64acc_p_kernel<2>, grid, dim3(256), 0, s, A32, lda_, n_, X, part64_, kchunk, KS);
+        else
+            hipLaunchKernelGGL(gemm_f64acc_kernel, grid, dim3(256), 0, s, A32, lda_, n_, X, part64_, kchunk, KS);
     }
     CP_LAUNCH_CHECK();
     const double* vals = (corr && near_rowptr_) ? corr : nullptr;
@@ -1086,7 +1222,11 @@ void BatchedSolver::matvec64acc(const float* A32, const double* X, double* Y, co
 
 void BatchedSolver::apply_block_jacobi(const float* P, float* Y, hipStream_t s) {
     TimedScope ts("bj_apply", s);
-    if (bj_mfma_) {
+    if (bj_mfma_ && bj_iperm_) {  // P was also written in block order to pp_
+        dim3 grid((unsigned)((bj_maxb_ + 63) / 64), (unsigned)bj_nblocks_);
+        hipLaunchKernelGGL(bj_apply_pp_kernel<8>, grid, dim3(kT), 0, s, bj_perm_, bj_ptr_, bj_off_, bj_inv_,
+                           (const float*)pp_, Y);
+    } else if (bj_mfma_) {
         dim3 grid((unsigned)((bj_maxb_ + 63) / 64), (unsigned)bj_nblocks_);
         hipLaunchKernelGGL(bj_apply_mfma_kernel, grid, dim3(kT), 0, s, bj_perm_, bj_ptr_, bj_off_, bj_inv_, P, Y);
     } else {
@@ -1186,9 +1326,10 @@ void BatchedSolver::tile_solve(const float* A32, const float* B32, const double*
             int nact = 0;
             for (int c = 0; c < KP; c++) nact += host_act_[c] != 0;
             if (nact == 0) break;
-            hipLaunchKernelGGL(inner_p_kernel, dim3(nel), dim3(kT), 0, s, rows, (const double*)sc_, (const int*)act_,
-                               (const float*)r_, p_, (const float*)v_, bj_ ? (const float*)nullptr : (const float*)invd_,
-                               y_);
+            const int* ip = (bj_ && bj_mfma_) ? bj_iperm_ : nullptr;
+            hipLaunchKernelGGL(inner_p_kernel, dim3(nel), dim3(kT), 0, s, rows, n_, (const double*)sc_,
+                               (const int*)act_, (const float*)r_, p_, (const float*)v_,
+                               bj_ ? (const float*)nullptr : (const float*)invd_, y_, ip, pp_);
             if (bj_) apply_block_jacobi(p_, y_, s);
             matvec32(A32, y_, nullptr, s);
             {
@@ -1198,9 +1339,9 @@ void BatchedSolver::tile_solve(const float* A32, const float* B32, const double*
             }
             colsum(dpart_, 1, Q_R0V, 0);
             hipLaunchKernelGGL(scal_alpha_kernel, dim3(1), dim3(KP), 0, s, sc_, (const int*)act_);
-            hipLaunchKernelGGL(inner_s_kernel, dim3(nel), dim3(kT), 0, s, rows, (const double*)sc_, (const int*)act_,
-                               (const float*)r_, (const float*)v_, bj_ ? (const float*)nullptr : (const float*)invd_,
-                               sv_, z_);
+            hipLaunchKernelGGL(inner_s_kernel, dim3(nel), dim3(kT), 0, s, rows, n_, (const double*)sc_,
+                               (const int*)act_, (const float*)r_, (const float*)v_,
+                               bj_ ? (const float*)nullptr : (const float*)invd_, sv_, z_, ip, pp_);
             if (bj_) apply_block_jacobi(sv_, z_, s);
             matvec32(A32, z_, nullptr, s);
             {

@@ -160,7 +160,7 @@ struct cp_geom {
     int near_nnz = 0;
     // strip block-Jacobi preconditioner of the batched FP32 inner solve
     BlockPlan bj_plan;
-    DevBuf bj_perm, bj_ptr, bj_off, bj_inv;
+    DevBuf bj_perm, bj_ptr, bj_off, bj_inv, bj_iperm;
     bool bj_ready = false;
     // post tables
     bool post_ready = false;
@@ -376,6 +376,9 @@ void ensure_block_jacobi(cp_geom* g) {
     if (P.perm.empty()) {
         build_strip_blocks(g->host, 768, 1024, P);
         upload(c, g->bj_perm, P.perm);
+        std::vector<int> ip(g->dev.n, 0);
+        for (size_t q = 0; q < P.perm.size(); q++) ip[P.perm[q]] = (int)q;
+        upload(c, g->bj_iperm, ip);
         upload(c, g->bj_ptr, P.ptr);
         upload(c, g->bj_off, P.off);
         dalloc(c, g->bj_inv, std::max<int64_t>(P.off.back(), 1) * sizeof(float));
@@ -708,7 +711,8 @@ cp_status cp_solve(cp_geom* g, cp_prec prec, int32_t k, const cp_freestream* fs,
             }
             if (opts->precond != 1 && !g->bj_plan.perm.empty())
                 g->batched->set_block_jacobi(g->bj_perm.as<int>(), g->bj_ptr.as<int>(), g->bj_off.as<int64_t>(),
-                                             g->bj_inv.as<float>(), g->bj_plan.nblocks(), g->bj_plan.max_b);
+                                             g->bj_inv.as<float>(), g->bj_plan.nblocks(), g->bj_plan.max_b,
+                                             g->bj_iperm.as<int>());
             else
                 g->batched->set_block_jacobi(nullptr, nullptr, nullptr, nullptr, 0, 0);
             BatchedOpts bo;
