@@ -145,6 +145,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 comparison path")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--inner-tol", type=float, default=0.0, help="inner FP32 solve reduction per sweep (0: default)")
     ap.add_argument("--serve-requests", type=int, default=200, help="requests in the serving stream (0: skip)")
     ap.add_argument("--serve-qps", type=float, default=10.0)
     ap.add_argument("--serve-slots", type=int, default=4)
@@ -211,13 +212,13 @@ def main():
 
     # mixed-precision batched path
     for _ in range(args.warmup):
-        g.solve(conds, CP_FP32_REFINE, tol=1e-10, **kw)
+        g.solve(conds, CP_FP32_REFINE, tol=1e-10, inner_tol=args.inner_tol, **kw)
     ctx.timing_reset()
     times = []
     res = None
     for _ in range(args.steps):
         t0 = time.perf_counter()
-        res = g.solve(conds, CP_FP32_REFINE, tol=1e-10, **kw)
+        res = g.solve(conds, CP_FP32_REFINE, tol=1e-10, inner_tol=args.inner_tol, **kw)
         times.append(time.perf_counter() - t0)
     t_step = float(np.mean(times))
     mm_ms, mm_n = ctx.timing("gemm_f32")
