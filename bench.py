@@ -49,6 +49,8 @@ def gemm_kernel_name(shape):
         return "gemm_f32_x4s3_kernel<2>"
     if shape["variant"] == 5:
         return "gemm_bf16x3_kernel"
+    if shape["variant"] in (6, 7, 8, 9):
+        return "gemm_bf16x3_s_kernel<%d, %d>" % {6: (2, 3), 7: (2, 4), 8: (4, 3), 9: (4, 2)}[shape["variant"]]
     return "gemm_f32_kernel<%d>" % shape["rt"]
 
 
@@ -233,7 +235,7 @@ def main():
     log("solver: %s" % json.dumps(rep))
     kname = gemm_kernel_name(rep.get("gemm_shape"))
     traffic = profiled_traffic("mixed_solve_naca0012_ar10_64cond", kname) if not args.small else None
-    if rep.get("gemm_shape", {}).get("variant") == 5:
+    if rep.get("gemm_shape", {}).get("variant", 0) >= 5:
         # bf16x3: the product needs a third of the FP32 matrix-core time and is
         # bound by streaming the packed matrix (4 bytes per entry: bf16 hi + lo)
         # plus the packed multivector and the output
@@ -281,12 +283,12 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": ("fp32(bf16x3)+fp64-refine" if rep.get("gemm_shape", {}).get("variant") == 5
+        "dtype": ("fp32(bf16x3)+fp64-refine" if rep.get("gemm_shape", {}).get("variant", 0) >= 5
                   else "fp32+fp64-refine"),
         "data": "synthetic (generated NACA wing mesh)",
         "config": {"workload": ("naca4412_ar6_4956tri_64cond" if args.small else "naca0012_ar10_25916tri_64cond"),
                    "n_body": n, "conditions": k, "precision": "FP32 A,B (FP64 near field) + FP32 BiCGSTAB%s + FP64 refinement" % (
-                       " (bf16x3 matrix products)" if rep.get("gemm_shape", {}).get("variant") == 5 else ""),
+                       " (bf16x3 matrix products)" if rep.get("gemm_shape", {}).get("variant", 0) >= 5 else ""),
                    "tol": 1e-10},
         "roofline": roof,
         "cpu_baseline": cpu,

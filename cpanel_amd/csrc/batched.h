@@ -25,6 +25,8 @@ struct GemmShape {
     int variant = 0;  // 0: 4-byte A loads, rt tiles per wave; 1/2: 16-byte A loads, 4 interleaved
                       // tiles, 16/8 K values per chunk, two register stages; 3/4: the same with
                       // 8/4 K values per chunk and three stages; 5: bf16x3 on packed tiles
+                      // (4 row tiles, 2 stages); 6/7: bf16x3, 2 row tiles, 3/4 stages;
+                      // 8/9: bf16x3, 4 row tiles, 3/2 stages (generic form)
     int rt = 2;
     int kslices = 8;
 };

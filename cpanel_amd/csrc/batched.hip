@@ -408,6 +408,83 @@ __global__ __launch_bounds__(256) void gemm_bf16x3_kernel(const uint4* __restric
         }
 }
 
+// Generalised bf16x3 product: RT row tiles per wavefront (workgroup = 4 x RT
+// x 32 rows), S register stages (the loads for K block kb + S - 1 are issued
+// before block kb is multiplied).  K blocks past the slice load zeros.
+template <int RT, int S>
+__global__ __launch_bounds__(256) void gemm_bf16x3_s_kernel(const uint4* __restrict__ Apk, int nkb,
+                                                            const uint4* __restrict__ Ppk, float* __restrict__ part,
+                                                            int64_t lda, int kbchunk, int KS) {
+    const int bid = blockIdx.x;
+    const int ks = bid % KS, rb = bid / KS;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t rt0 = ((int64_t)rb * 4 + wave) * RT;
+    const int kb0 = ks * kbchunk, kb1 = min(nkb, kb0 + kbchunk);
+    floatx16 acc[RT][2];
+#pragma unroll
+    for (int t = 0; t < RT; t++) acc[t][0] = acc[t][1] = floatx16{};
+    uint4 ah[S][RT], al[S][RT], ph[S][2], pl[S][2];
+    const uint4 z = make_uint4(0u, 0u, 0u, 0u);
+    auto load = [&](int kb, uint4 (&h)[RT], uint4 (&l)[RT], uint4 (&qh)[2], uint4 (&ql)[2]) {
+        if (kb < kb1) {
+#pragma unroll
+            for (int t = 0; t < RT; t++) {
+                const uint4* src = Apk + (((rt0 + t) * nkb + kb) * 64 + lane) * 2;
+                h[t] = src[0];
+                l[t] = src[1];
+            }
+#pragma unroll
+            for (int c = 0; c < 2; c++) {
+                const uint4* src = Ppk + (((int64_t)kb * 2 + c) * 64 + lane) * 2;
+                qh[c] = src[0];
+                ql[c] = src[1];
+            }
+        } else {
+#pragma unroll
+            for (int t = 0; t < RT; t++) h[t] = l[t] = z;
+#pragma unroll
+            for (int c = 0; c < 2; c++) qh[c] = ql[c] = z;
+        }
+    };
+    auto mma = [&](const uint4 (&h)[RT], const uint4 (&l)[RT], const uint4 (&qh)[2], const uint4 (&ql)[2]) {
+#pragma unroll
+        for (int t = 0; t < RT; t++) {
+            const bf16x8 xh = __builtin_bit_cast(bf16x8, h[t]);
+            const bf16x8 xl = __builtin_bit_cast(bf16x8, l[t]);
+#pragma unroll
+            for (int c = 0; c < 2; c++) {
+                const bf16x8 yh = __builtin_bit_cast(bf16x8, qh[c]);
+                const bf16x8 yl = __builtin_bit_cast(bf16x8, ql[c]);
+                acc[t][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xl, yh, acc[t][c], 0, 0, 0);
+                acc[t][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xh, yl, acc[t][c], 0, 0, 0);
+                acc[t][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xh, yh, acc[t][c], 0, 0, 0);
+            }
+        }
+    };
+    const int nsteps = kb1 - kb0;
+    const int kbend = kb0 + (nsteps > 0 ? (nsteps + S - 1) / S * S : 0);
+#pragma unroll
+    for (int st = 0; st < S - 1; st++) load(kb0 + st, ah[st], al[st], ph[st], pl[st]);
+    for (int kb = kb0; kb < kbend; kb += S) {
+#pragma unroll
+        for (int st = 0; st < S; st++) {
+            const int nx = (st + S - 1) % S;
+            load(kb + st + S - 1, ah[nx], al[nx], ph[nx], pl[nx]);
+            mma(ah[st], al[st], ph[st], pl[st]);
+        }
+    }
+    float* out = part + (int64_t)ks * lda * KP;
+    const int m = lane & 31, kh = lane >> 5;
+#pragma unroll
+    for (int t = 0; t < RT; t++)
+#pragma unroll
+        for (int j = 0; j < 16; j++) {
+            const int64_t r = (rt0 + t) * 32 + 8 * (j >> 2) + 4 * kh + (j & 3);
+            out[r * KP + m] = acc[t][0][j];
+            out[r * KP + 32 + m] = acc[t][1][j];
+        }
+}
+
 __global__ __launch_bounds__(256) void gemm_f64acc_kernel(const float* __restrict__ A, int64_t lda, int n,
                                                           const double* __restrict__ X, double* __restrict__ part,
                                                           int kchunk, int KS) {
@@ -1055,7 +1132,7 @@ BatchedSolver::BatchedSolver(int n, int64_t lda, int capacity) : n_(n), cap_(cap
     // CPGPU_GEMM_SHAPE="variant,rt,kslices" pins the shape (tests, tuning runs)
     if (const char* e = std::getenv("CPGPU_GEMM_SHAPE")) {
         int v = 0, r = 2, k = 8;
-        if (std::sscanf(e, "%d,%d,%d", &v, &r, &k) == 3 && v >= 0 && v <= 5 && (r == 1 || r == 2 || r == 4) &&
+        if (std::sscanf(e, "%d,%d,%d", &v, &r, &k) == 3 && v >= 0 && v <= 9 && (r == 1 || r == 2 || r == 4) &&
             (k == 4 || k == 8 || k == 16 || k == 32)) {
             shape_.variant = v;
             shape_.rt = r;
@@ -1088,6 +1165,42 @@ void BatchedSolver::launch_gemm32(const GemmShape& sh, const float* A32, const f
         TimedScope ts("gemm_f32", s);
         hipLaunchKernelGGL(gemm_bf16x3_kernel, grid, dim3(256), 0, s, (const uint4*)apk_, nkb, (const uint4*)ppk_,
                            part32_, lda_, kbchunk, KS);
+        CP_LAUNCH_CHECK();
+        return;
+    }
+    if (sh.variant >= 6 && sh.variant <= 9) {  // bf16x3, RT row tiles, S stages
+        if (!apk_) throw HipError{hipErrorInvalidValue, "bf16x3 GEMM without packed A"};
+        const int nkb = nkb_;
+        {
+            TimedScope tp("pack_p", s);
+            hipLaunchKernelGGL(pack_p_bf16x2_kernel, dim3((unsigned)((2 * (int64_t)nkb * 64 + 255) / 256)),
+                               dim3(256), 0, s, P, n_, nkb, (uint4*)ppk_);
+        }
+        CP_LAUNCH_CHECK();
+        const int kbchunk = (nkb + KS - 1) / KS;
+        const int rows_wg = (sh.variant == 6 || sh.variant == 7) ? 256 : 512;
+        dim3 grid((unsigned)((lda_ / rows_wg) * KS));
+        TimedScope ts("gemm_f32", s);
+        const uint4* Ap = (const uint4*)apk_;
+        const uint4* Pp = (const uint4*)ppk_;
+        switch (sh.variant) {
+            case 6:
+                hipLaunchKernelGGL((gemm_bf16x3_s_kernel<2, 3>), grid, dim3(256), 0, s, Ap, nkb, Pp, part32_, lda_,
+                                   kbchunk, KS);
+                break;
+            case 7:
+                hipLaunchKernelGGL((gemm_bf16x3_s_kernel<2, 4>), grid, dim3(256), 0, s, Ap, nkb, Pp, part32_, lda_,
+                                   kbchunk, KS);
+                break;
+            case 8:
+                hipLaunchKernelGGL((gemm_bf16x3_s_kernel<4, 3>), grid, dim3(256), 0, s, Ap, nkb, Pp, part32_, lda_,
+                                   kbchunk, KS);
+                break;
+            default:
+                hipLaunchKernelGGL((gemm_bf16x3_s_kernel<4, 2>), grid, dim3(256), 0, s, Ap, nkb, Pp, part32_, lda_,
+                                   kbchunk, KS);
+                break;
+        }
         CP_LAUNCH_CHECK();
         return;
     }
@@ -1142,9 +1255,9 @@ void BatchedSolver::tune(const float* A32, hipStream_t s) {
             g.kslices = ks;
             cand.push_back(g);
         }
-    for (int v : {1, 2, 3, 4, 5})
+    for (int v : {1, 2, 3, 4, 5, 6, 7, 8, 9})
         for (int ks : {4, 8, 16, 32}) {
-            if (v == 5 && (!apk_ || !bf16x3_)) continue;
+            if (v >= 5 && (!apk_ || !bf16x3_)) continue;
             GemmShape g;
             g.variant = v;
             g.rt = 4;
