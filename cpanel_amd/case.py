@@ -165,6 +165,38 @@ class Geometry:
                        "cp_panel_to_points")
         return out
 
+    def set_buffer_wake(self, verts):
+        """verts: (m, nv, 3) panels of the second buffer-wake row (nv = 3 or 4)."""
+        V = np.asarray(verts, dtype=np.float64)
+        m = V.shape[0]
+        nv = np.full(m, V.shape[1] if m else 3, dtype=np.int32)
+        buf = np.zeros((max(m, 1), 12))
+        if m:
+            buf[:m, :3 * V.shape[1]] = V.reshape(m, -1)
+        self.ctx.check(abi.lib().cp_set_buffer_wake(self.h, m, _p(nv, C.c_int32), _p(buf)), "cp_set_buffer_wake")
+        self.nw2 = m
+
+    def solve_rhs(self, sigma, prec=CP_FP64, mu_w2=None, x0=None, tol=1e-10, max_iter=0, max_refine=0,
+                  inner_tol=0.0, precond=0, allow_unconverged=False):
+        """Solves A mu = -B sigma - C mu_w2 (cpCaseVP::solveVPmatrixEq) for the
+        columns of sigma (n_body x k or n_body); returns (mu, infos)."""
+        sg = np.asarray(sigma, dtype=np.float64)
+        one = sg.ndim == 1
+        sg = np.ascontiguousarray(sg.reshape(self.nb, -1).T)  # k x n (column c contiguous)
+        k = sg.shape[0]
+        mw = None if mu_w2 is None else np.ascontiguousarray(np.asarray(mu_w2, np.float64).reshape(-1, k).T)
+        x = None if x0 is None else np.ascontiguousarray(np.asarray(x0, np.float64).reshape(self.nb, -1).T)
+        o = abi.SolveOpts(0.1, 1.0, 1.0, 1.0, (C.c_double * 3)(0, 0, 0), tol, max_iter, max_refine, inner_tol,
+                          precond, 0)
+        mu = np.zeros((k, self.nb))
+        info = (abi.SolveInfo * k)()
+        st = abi.lib().cp_solve_rhs(self.h, prec, k, _p(sg), _p(mw), _p(x), C.byref(o), _p(mu), info)
+        if st != abi.CP_OK and not (st == abi.CP_NOT_CONVERGED and allow_unconverged):
+            self.ctx.check(st, "cp_solve_rhs")
+        infos = [dict(rel_residual=i.rel_residual, iters=i.iters, refinements=i.refinements,
+                      converged=bool(i.converged)) for i in info]
+        return (mu[0] if one else mu.T), infos
+
     def solver_report(self):
         import json
         buf = C.create_string_buffer(1 << 16)

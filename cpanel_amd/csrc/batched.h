@@ -50,6 +50,11 @@ public:
     // x: device FP64 output, column c at x + c*ldx.
     void solve(const float* A32, const float* B32, const double* normals, const double* vinf4, int k,
                const BatchedOpts& o, double* x, int64_t ldx, SolveStats* stats, hipStream_t s);
+    // Same with caller-given columns (device, column-major, leading dimension
+    // ldx): sigma (replaces -Vinf . n), rhs_add (added to -B sigma, or null)
+    // and x0 (initial guess, or null = zero)
+    void solve_rhs(const float* A32, const float* B32, const double* sigma, const double* rhs_add, const double* x0,
+                   int k, const BatchedOpts& o, double* x, int64_t ldx, SolveStats* stats, hipStream_t s);
     void set_shape(const GemmShape& g) {
         shape_ = g;
         tuned_ = true;
@@ -94,7 +99,8 @@ public:
 
 private:
     void tile_solve(const float* A32, const float* B32, const double* normals, const double* vinf4, int c0,
-                    int kc, const BatchedOpts& o, double* x, int64_t ldx, SolveStats* stats, hipStream_t s);
+                    int kc, const BatchedOpts& o, double* x, int64_t ldx, SolveStats* stats, hipStream_t s,
+                    const double* sig_cm = nullptr, const double* add_cm = nullptr, const double* x0_cm = nullptr);
     void launch_gemm32(const GemmShape& sh, const float* A32, const float* P, hipStream_t s);
     void apply_block_jacobi(const float* P, float* Y, hipStream_t s);
     const void* apk_ = nullptr;

@@ -902,6 +902,16 @@ __global__ void invd32_kernel(int n, const float* __restrict__ A, int64_t lda, f
     invd[j] = d != 0 ? 1.f / d : 1.f;
 }
 
+// row-major lda x 64 multivector from kc column-major columns (rest zero)
+__global__ void gather_cols_kernel(int n, int64_t lda, int kc, const double* __restrict__ src, int64_t ld,
+                                   double* __restrict__ dst) {
+    int64_t idx = (int64_t)blockIdx.x * kT + threadIdx.x;
+    if (idx >= lda * KP) return;
+    int64_t i = idx / KP;
+    int c = (int)(idx % KP);
+    dst[idx] = (i < n && c < kc) ? src[(int64_t)c * ld + i] : 0.0;
+}
+
 __global__ void scatter_x_kernel(int n, int kc, const double* __restrict__ X, double* __restrict__ x, int64_t ldx) {
     int64_t idx = (int64_t)blockIdx.x * kT + threadIdx.x;
     int64_t i = idx / KP;
@@ -1365,17 +1375,32 @@ void BatchedSolver::solve(const float* A32, const float* B32, const double* norm
     }
 }
 
+void BatchedSolver::solve_rhs(const float* A32, const float* B32, const double* sigma, const double* rhs_add,
+                              const double* x0, int k, const BatchedOpts& o, double* x, int64_t ldx,
+                              SolveStats* stats, hipStream_t s) {
+    for (int c0 = 0; c0 < k; c0 += KP) {
+        int kc = std::min(KP, k - c0);
+        const int64_t off = (int64_t)c0 * ldx;
+        tile_solve(A32, B32, nullptr, nullptr, c0, kc, o, x + off, ldx, stats + c0, s, sigma + off,
+                   rhs_add ? rhs_add + off : nullptr, x0 ? x0 + off : nullptr);
+    }
+}
+
 void BatchedSolver::tile_solve(const float* A32, const float* B32, const double* normals, const double* vinf4,
                                int c0, int kc, const BatchedOpts& o, double* x, int64_t ldx, SolveStats* stats,
-                               hipStream_t s) {
+                               hipStream_t s, const double* sig_cm, const double* add_cm, const double* x0_cm) {
     (void)c0;
     const int64_t rows = lda_;
     const unsigned nel = nblocks(rows * KP);
     int* host_sweep = host_act_ + KP;
 
-    // sigma multivector and RHS b = -B32 sigma (FP64 accumulation)
-    hipLaunchKernelGGL(sigma_rows_kernel, dim3(nel), dim3(kT), 0, s, n_, lda_, kc, normals, vinf4, S_);
-    matvec64acc(B32, S_, Bv_, nullptr, -1.0, s, near_dB_);
+    // sigma multivector and RHS b = add - (B32 + dB) sigma (FP64 accumulation)
+    if (sig_cm)
+        hipLaunchKernelGGL(gather_cols_kernel, dim3(nel), dim3(kT), 0, s, n_, lda_, kc, sig_cm, ldx, S_);
+    else
+        hipLaunchKernelGGL(sigma_rows_kernel, dim3(nel), dim3(kT), 0, s, n_, lda_, kc, normals, vinf4, S_);
+    if (add_cm) hipLaunchKernelGGL(gather_cols_kernel, dim3(nel), dim3(kT), 0, s, n_, lda_, kc, add_cm, ldx, R_);
+    matvec64acc(B32, S_, Bv_, add_cm ? R_ : nullptr, -1.0, s, near_dB_);
     auto colsum = [&](const double* dp, int nq, int q0, int q1) {
         hipLaunchKernelGGL(colsum_kernel, dim3(1), dim3(1024), 0, s, dp, nrb_, nq, q0, q1, sc_);
         CP_LAUNCH_CHECK();
@@ -1384,7 +1409,10 @@ void BatchedSolver::tile_solve(const float* A32, const float* B32, const double*
     hipLaunchKernelGGL(invd32_kernel, dim3(nblocks(lda_)), dim3(kT), 0, s, n_, A32, lda_, invd_);
     if (autotune_) tune(A32, s);
     const int KS = shape_.kslices;
-    CP_HIP(hipMemsetAsync(X_, 0, (size_t)lda_ * KP * sizeof(double), s));
+    if (x0_cm)
+        hipLaunchKernelGGL(gather_cols_kernel, dim3(nel), dim3(kT), 0, s, n_, lda_, kc, x0_cm, ldx, X_);
+    else
+        CP_HIP(hipMemsetAsync(X_, 0, (size_t)lda_ * KP * sizeof(double), s));
     std::vector<double> bb(KP, 0.0);
     CP_HIP(hipMemcpyAsync(host_sc_, sc_, KP * Q_NQ * sizeof(double), hipMemcpyDeviceToHost, s));
     CP_HIP(hipStreamSynchronize(s));

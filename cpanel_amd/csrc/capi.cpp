@@ -153,6 +153,9 @@ struct cp_geom {
     DevBuf W;          // wake influence N x Nw (FP64)
     DevBuf A64, B64, A32, B32;
     DevBuf A16;        // bf16 hi/lo tiles of A32 (bf16x3 inner products)
+    // second buffer-wake row of the vortex-particle case: panels and C
+    DevBuf w2, Cw;
+    int nw2 = 0;
     bool have64 = false, have32 = false;
     // FP64 near-field corrections of A32/B32 (entries within one panel
     // diameter: |c_i - c_j| <= longSide_j)
@@ -999,6 +1002,133 @@ cp_status cp_panel_to_points(cp_geom* g, const double* sigma, const double* mu, 
         CP_HIP(hipMemcpyAsync(vel, dout.p, (size_t)3 * m * sizeof(double), hipMemcpyDeviceToHost, s));
         CP_HIP(hipStreamSynchronize(s));
         return CP_OK;
+    });
+}
+
+// ---------------------------------------------------------------- VP solve
+cp_status cp_set_buffer_wake(cp_geom* g, int32_t m, const int32_t* nv, const double* verts) {
+    if (!g || m < 0 || (m > 0 && (!nv || !verts))) return CP_BAD_ARG;
+    cp_ctx* c = g->ctx;
+    if (c->device < 0) return fail(c, CP_NO_DEVICE, "host-only context");
+    for (int j = 0; j < m; j++)
+        if (nv[j] != 3 && nv[j] != 4) return fail(c, CP_BAD_ARG, "buffer wake panels need 3 or 4 vertices");
+    return guarded_g(g, [&]() -> cp_status {
+        g->nw2 = m;
+        if (m == 0) {
+            g->Cw.release();
+            g->w2.release();
+            return CP_OK;
+        }
+        std::vector<PanelRec> rec(m);
+        for (int j = 0; j < m; j++) {
+            PanelGeom P;
+            P.nv = nv[j];
+            for (int q = 0; q < nv[j]; q++)
+                P.v[q] = Vec3{verts[12 * j + 3 * q], verts[12 * j + 3 * q + 1], verts[12 * j + 3 * q + 2]};
+            panel_setup(P);
+            rec[j] = make_panel_rec(P);
+        }
+        upload(c, g->w2, rec);
+        dalloc(c, g->Cw, (size_t)g->dev.lda * m * sizeof(double));
+        // C(i, j) = dubPhiInf of buffer panel j at collocation point i
+        // (geometry.cpp:876-882): the wake-influence kernel on these panels
+        DevGeom d = g->dev;
+        d.wake = g->w2.as<PanelRec>();
+        d.nw = m;
+        launch_wake_influence(d, g->Cw.as<double>(), g->dev.lda, g->stream);
+        CP_HIP(hipStreamSynchronize(g->stream));
+        return CP_OK;
+    });
+}
+
+cp_status cp_solve_rhs(cp_geom* g, cp_prec prec, int32_t k, const double* sigma, const double* mu_w2,
+                       const double* x0, const cp_solve_opts* opts, double* mu, cp_solve_info* info) {
+    if (!g || !sigma || !opts || k <= 0 || (prec != CP_FP64 && prec != CP_FP32_REFINE)) return CP_BAD_ARG;
+    cp_ctx* c = g->ctx;
+    if (c->device < 0) return fail(c, CP_NO_DEVICE, "host-only context");
+    if (mu_w2 && g->nw2 == 0) return fail(c, CP_BAD_ARG, "mu_w2 given but no buffer wake set");
+    return guarded_g(g, [&]() -> cp_status {
+        if (prec == CP_FP64 && !g->have64) return fail(c, CP_NOT_ASSEMBLED, "call cp_assemble(CP_FP64) first");
+        if (prec == CP_FP32_REFINE && !g->have32)
+            return fail(c, CP_NOT_ASSEMBLED, "call cp_assemble(CP_FP32_REFINE) first");
+        const int n = g->dev.n;
+        const int64_t lda = g->dev.lda;
+        hipStream_t s = g->stream;
+        const double tol = opts->tol > 0 ? opts->tol : 1e-10;
+        // caller columns, padded to lda
+        std::vector<double> hs((size_t)lda * k, 0.0), hx;
+        for (int q = 0; q < k; q++) std::memcpy(&hs[(size_t)q * lda], sigma + (size_t)q * n, n * sizeof(double));
+        DevBuf dsig, dadd, dx0, dmu, dmw;
+        upload(c, dsig, hs);
+        if (x0) {
+            hx.assign((size_t)lda * k, 0.0);
+            for (int q = 0; q < k; q++) std::memcpy(&hx[(size_t)q * lda], x0 + (size_t)q * n, n * sizeof(double));
+            upload(c, dx0, hx);
+        }
+        if (mu_w2) {
+            std::vector<double> mw(mu_w2, mu_w2 + (size_t)g->nw2 * k);
+            upload(c, dmw, mw);
+            dalloc(c, dadd, (size_t)lda * k * sizeof(double));
+            CP_HIP(hipMemsetAsync(dadd.p, 0, (size_t)lda * k * sizeof(double), s));
+            launch_buffer_rhs(n, lda, g->nw2, g->Cw.as<double>(), dmw.as<double>(), k, dadd.as<double>(), lda, s);
+        }
+        dalloc(c, dmu, (size_t)lda * k * sizeof(double));
+        std::vector<SolveStats> stats(k);
+        if (prec == CP_FP64) {
+            if (!g->bicg) g->bicg.reset(new Bicgstab64(n, lda));
+            DevBuf drhs;
+            dalloc(c, drhs, (size_t)lda * sizeof(double));
+            for (int q = 0; q < k; q++) {
+                // RHS = -B sigma - C mu_w2 (cpCaseVP.cpp:272)
+                g->bicg->gemv().apply(g->B64.as<double>(), dsig.as<double>() + lda * q, drhs.as<double>(), -1.0,
+                                      g->bicg->gemv_scratch(), s);
+                if (mu_w2) launch_add_vec(n, dadd.as<double>() + lda * q, drhs.as<double>(), s);
+                stats[q] = g->bicg->solve(g->A64.as<double>(), drhs.as<double>(), dmu.as<double>() + lda * q, tol,
+                                          opts->max_iter, s, nullptr, x0 ? dx0.as<double>() + lda * q : nullptr);
+            }
+        } else {
+            if (!g->batched) {
+                const int64_t need = (int64_t)BatchedSolver::workspace_bytes(lda);
+                if (!fits(c, (uint64_t)need)) throw BudgetError{(uint64_t)need};
+                g->batched.reset(new BatchedSolver(n, lda, kRhsTile));
+                g->solver_bytes += need;
+                account(c, need);
+            }
+            g->batched->set_near(g->near_rowptr.as<int>(), g->near_cols.as<int>(), g->near_dA.as<double>(),
+                                 g->near_dB.as<double>());
+            g->batched->set_packed(g->A16.p, packed_kblocks(n));
+            if (opts->precond != 1) ensure_block_jacobi(g);
+            if (opts->precond != 1 && !g->bj_plan.perm.empty())
+                g->batched->set_block_jacobi(g->bj_perm.as<int>(), g->bj_ptr.as<int>(), g->bj_off.as<int64_t>(),
+                                             g->bj_inv.as<float>(), g->bj_plan.nblocks(), g->bj_plan.max_b,
+                                             g->bj_iperm.as<int>());
+            else
+                g->batched->set_block_jacobi(nullptr, nullptr, nullptr, nullptr, 0, 0);
+            BatchedOpts bo;
+            bo.tol = tol;
+            bo.max_iter = opts->max_iter > 0 ? opts->max_iter : 2 * n;
+            bo.max_refine = opts->max_refine > 0 ? opts->max_refine : 20;
+            bo.inner_tol = opts->inner_tol > 0 ? opts->inner_tol : 1e-3;
+            g->batched->solve_rhs(g->A32.as<float>(), g->B32.as<float>(), dsig.as<double>(),
+                                  mu_w2 ? dadd.as<double>() : nullptr, x0 ? dx0.as<double>() : nullptr, k, bo,
+                                  dmu.as<double>(), lda, stats.data(), s);
+        }
+        if (mu)
+            CP_HIP(hipMemcpy2DAsync(mu, n * sizeof(double), dmu.p, lda * sizeof(double), n * sizeof(double), k,
+                                    hipMemcpyDeviceToHost, s));
+        CP_HIP(hipStreamSynchronize(s));
+        bool all_conv = true;
+        for (int q = 0; q < k; q++) {
+            if (info) {
+                std::memset(&info[q], 0, sizeof(cp_solve_info));
+                info[q].rel_residual = stats[q].rel_residual;
+                info[q].iters = stats[q].iters;
+                info[q].refinements = stats[q].refinements;
+                info[q].converged = stats[q].rel_residual <= 1e-10;
+            }
+            all_conv = all_conv && stats[q].rel_residual <= 1e-10;
+        }
+        return all_conv ? CP_OK : CP_NOT_CONVERGED;
     });
 }
 

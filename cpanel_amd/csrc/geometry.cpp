@@ -48,7 +48,7 @@ inline Vec3 unit(Vec3 a) {
 }
 
 // panel::setGeom (panel.cpp:33-104)
-void panel_setup(PanelGeom& p) {
+void setup_panel(PanelGeom& p) {
     p.long_side = 0;
     for (int i = 0; i < p.nv; i++) {
         double l = len(sub(p.v[(i + 1) % p.nv], p.v[i]));
@@ -357,7 +357,7 @@ bool build_geometry(int nn, const double* xyz, int nt, const int32_t* tri_in, co
             p.v[k] = B.nodes[p.node[k]];
         }
         for (int k = 0; k < 3; k++) B.pedge[i][k] = B.edge_of(p.node[k], p.node[(k + 1) % 3]);
-        panel_setup(p);
+        setup_panel(p);
         bool newGroup = (i == 0 || ids[i] != ids[i - 1]);
         if (ids[i] <= 1000) {
             if (newGroup) B.surf_lifting.push_back(0);
@@ -606,5 +606,7 @@ bool build_geometry(int nn, const double* xyz, int nt, const int32_t* tri_in, co
     }
     return true;
 }
+
+void panel_setup(PanelGeom& p) { setup_panel(p); }
 
 }  // namespace cpgpu

@@ -26,6 +26,9 @@ struct PanelGeom {
     double area = 0, long_side = 0;
 };
 
+// panel::setGeom (panel.cpp:33-104): centre, area, normal, longSide from v[]
+void panel_setup(PanelGeom& p);
+
 struct WakeLineInfo {
     int upper, lower;  // body panel indices (reference order)
     Vec3 p1, p2;       // trailing-edge end points (wakeLine::setDimensions)

@@ -42,8 +42,10 @@ public:
     Bicgstab64& operator=(const Bicgstab64&) = delete;
     // b, x: device vectors of length lda (x padded rows are zeroed).
     // invd: optional precomputed Jacobi diagonal (device), else from A.
+    // x0: optional initial guess (device, length lda); null = zero, Eigen's
+    // solve(); otherwise solveWithGuess (tolerance still relative to |b|)
     SolveStats solve(const double* A, const double* b, double* x, double tol, int maxit, hipStream_t s,
-                     const double* invd = nullptr);
+                     const double* invd = nullptr, const double* x0 = nullptr);
     double* gemv_scratch() { return part_; }
     const Gemv64& gemv() const { return gemv_; }
 

@@ -294,7 +294,7 @@ void Bicgstab64::fetch(hipStream_t s) {
 }
 
 SolveStats Bicgstab64::solve(const double* A, const double* b, double* x, double tol, int maxit, hipStream_t s,
-                             const double* invd_in) {
+                             const double* invd_in, const double* x0) {
     SolveStats st;
     const int n = n_;
     if (maxit <= 0) maxit = 2 * n;
@@ -320,7 +320,19 @@ SolveStats Bicgstab64::solve(const double* A, const double* b, double* x, double
     CP_HIP(hipMemcpyAsync(sc_ + SC_OMEGA, &init[1], sizeof(double), hipMemcpyHostToDevice, s));
     fetch(s);
     double rr = host_sc_[SC_RR];
-    double rhs_sq = rr, r0_sq = rr;
+    double rhs_sq = rr;
+    if (x0) {
+        // initial guess: r = b - A x0 (Eigen's solveWithGuess); |b| stays the scale
+        CP_HIP(hipMemcpyAsync(x, x0, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, s));
+        gemv_.partial(A, x, part_, s);
+        hipLaunchKernelGGL(residual_kernel, g, blk, 0, s, n, b, (const double*)part_, gemv_.slices(), lda_, r, r0,
+                           drr, dr0r);
+        hipLaunchKernelGGL(finish_dots_kernel, dim3(1), blk, 0, s, drr, dr0r, nb_, sc_);
+        CP_LAUNCH_CHECK();
+        fetch(s);
+        rr = host_sc_[SC_RR];
+    }
+    double r0_sq = rr;
     if (rhs_sq == 0) {
         st.rel_residual = 0;
         return st;

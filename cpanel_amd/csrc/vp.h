@@ -32,4 +32,10 @@ int p2p_slices(int m, int npanels);
 void launch_panel_to_points(const DevGeom& g, const double* sigma, const double* mu, const double* muw, int m,
                             const double* pts, int KS, double* part, double* out, hipStream_t s);
 
+// y[0..n) += a[0..n)
+void launch_add_vec(int n, const double* a, double* y, hipStream_t s);
+// out (column-major, ld) = -C mu for k columns: C is lda x m (FP64), mu m x k
+void launch_buffer_rhs(int n, int64_t lda, int m, const double* Cm, const double* mu, int k, double* out, int64_t ld,
+                       hipStream_t s);
+
 }  // namespace cpgpu

@@ -232,7 +232,39 @@ __global__ void p2p_reduce_kernel(const double* __restrict__ part, int KS, int m
     out[e] = s;
 }
 
+// out[i + c ld] = -sum_j C(i, j) mu[j + c m]: the second buffer-wake row's
+// contribution to the VP right-hand side (cpCaseVP.cpp:272), FP64
+__global__ __launch_bounds__(kT) void buffer_rhs_kernel(int n, int64_t lda, int m, const double* __restrict__ Cm,
+                                                        const double* __restrict__ mu, int k, double* __restrict__ out,
+                                                        int64_t ld) {
+    const int i = blockIdx.x * kT + threadIdx.x;
+    const int c = blockIdx.y;
+    if (i >= n || c >= k) return;
+    double acc = 0;
+    for (int j = 0; j < m; j++) acc += Cm[i + (int64_t)j * lda] * mu[(int64_t)c * m + j];
+    out[i + (int64_t)c * ld] = -acc;
+}
+
+__global__ void add_vec_kernel(int n, const double* __restrict__ a, double* __restrict__ y) {
+    int i = blockIdx.x * kT + threadIdx.x;
+    if (i < n) y[i] += a[i];
+}
+
 }  // namespace
+
+void launch_add_vec(int n, const double* a, double* y, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(add_vec_kernel, dim3((unsigned)((n + kT - 1) / kT)), dim3(kT), 0, s, n, a, y);
+    CP_LAUNCH_CHECK();
+}
+
+void launch_buffer_rhs(int n, int64_t lda, int m, const double* Cm, const double* mu, int k, double* out, int64_t ld,
+                       hipStream_t s) {
+    if (n <= 0 || k <= 0) return;
+    dim3 grid((unsigned)((n + kT - 1) / kT), (unsigned)k);
+    hipLaunchKernelGGL(buffer_rhs_kernel, grid, dim3(kT), 0, s, n, lda, m, Cm, mu, k, out, ld);
+    CP_LAUNCH_CHECK();
+}
 
 void launch_bh_points(const DevBhTree& T, double theta, int m, const double* pts, double* out, hipStream_t s) {
     if (m <= 0) return;

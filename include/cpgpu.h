@@ -183,6 +183,21 @@ void cp_bh_release(cp_tree* t);
 cp_status cp_panel_to_points(cp_geom* g, const double* sigma, const double* mu, const double* mu_wake, int32_t m,
                              const double* pts, double* vel);
 
+/* ---- vortex-particle solve (cpCaseVP::solveVPmatrixEq, cpCaseVP.cpp:265-301)
+ * The VP case's second buffer-wake row: m panels (nv[j] = 3 or 4 vertices,
+ * verts[12 j + 3 q + d]) whose doublet influence on the body collocation
+ * points forms C (n_body x m), as geometry::setInfCoeff builds it
+ * (geometry.cpp:876-882).  m = 0 clears it. */
+cp_status cp_set_buffer_wake(cp_geom* g, int32_t m, const int32_t* nv, const double* verts);
+/* Solves A mu = -B sigma - C mu_w2 for k columns with caller-given source
+ * strengths (sigma: n_body x k, column-major; the VP case's sigma includes the
+ * particle-induced velocity, cpCaseVP.cpp:173-200), optional buffer-wake
+ * strengths mu_w2 (m x k) and optional initial guess x0 (n_body x k; NULL =
+ * zero, the reference's choice; a previous step's mu = warm start).  mu:
+ * n_body x k.  info: iterations, refinements, residual (no post-processing). */
+cp_status cp_solve_rhs(cp_geom* g, cp_prec prec, int32_t k, const double* sigma, const double* mu_w2,
+                       const double* x0, const cp_solve_opts* opts, double* mu, cp_solve_info* info);
+
 /* ---- serving: continuous batching of (geometry, conditions) requests -----
  * Replaces the reference's sequential case loop (CPanelMgr.cpp:49-88).
  * Worker threads (slots) take requests in arrival order; each geometry is

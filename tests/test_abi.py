@@ -43,3 +43,15 @@ def test_bad_args():
     L = abi.lib()
     assert L.cp_create(0, 0, None) == abi.CP_BAD_ARG
     assert L.cp_geometry_info(None, None) == abi.CP_BAD_ARG
+
+
+def test_vp_solve_refused_without_device():
+    ctx = Context(device=-1)
+    g = ctx.geometry(meshgen.small_wing())
+    L = abi.lib()
+    sig = np.zeros(g.nb)
+    o = abi.SolveOpts()
+    st = L.cp_solve_rhs(g.h, abi.CP_FP64, 1, sig.ctypes.data_as(C.POINTER(C.c_double)), None, None, C.byref(o),
+                        None, None)
+    assert st == abi.CP_NO_DEVICE
+    assert L.cp_set_buffer_wake(g.h, 1, None, None) == abi.CP_BAD_ARG

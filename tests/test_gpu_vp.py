@@ -119,3 +119,60 @@ def test_panel_to_points_matches_oracle(ctx):
     got0 = g.panel_to_points(sigma, mu, pts)
     ref0 = V.panels_to_points(bv, sigma, mu, None, None, pts)
     assert close(got0, ref0, 1e-10)[0]
+
+
+@pytest.fixture(scope="module")
+def vp_case(ctx):
+    from cpanel_amd import CP_FP32_REFINE, CP_FP64
+    from oracle import oracle as O
+    m = meshgen.wing("4412", 1.0, 6.0, 40, 12, "uniform", 1.0)
+    g = ctx.geometry(m)
+    g.assemble(CP_FP64)
+    g.assemble(CP_FP32_REFINE)
+    og = O.Geom.from_arrays(m["xyz"], m["tri"], m["ids"])
+    A, B = og.assemble()
+    # second buffer-wake row: the first row moved one chord downstream
+    _, wv = panels_of(g, m)
+    w2 = wv + np.array([1.0, 0.0, 0.0])
+    g.set_buffer_wake(w2)
+    c = g.body()["center"]
+    C = np.array([[O.dub_phi_inf(w2[j], c[i]) for j in range(len(w2))] for i in range(g.nb)])
+    rng = np.random.default_rng(17)
+    nrm = g.body()["normal"]
+    vinf = np.array([np.cos(np.radians(6.0)), 0.0, np.sin(np.radians(6.0))])
+    sig = np.stack([-(nrm @ (vinf + 0.05 * rng.normal(size=3))) for _ in range(3)], axis=1)
+    muw2 = rng.normal(size=(len(w2), 3)) * 0.1
+    rhs = -B @ sig - C @ muw2
+    ref = np.linalg.solve(A, rhs)
+    return g, sig, muw2, ref
+
+
+@pytest.mark.parametrize("prec_name", ["fp64", "fp32_refine"])
+def test_vp_solve_matches_dense(vp_case, prec_name):
+    """cpCaseVP::solveVPmatrixEq: A mu = -B sigma - C mu_w2 against LAPACK on
+    the oracle's A, B and C (FP64: 1e-8 relative; mixed: 1e-7)."""
+    from cpanel_amd import CP_FP32_REFINE, CP_FP64
+    g, sig, muw2, ref = vp_case
+    prec = CP_FP64 if prec_name == "fp64" else CP_FP32_REFINE
+    say("[gpu-vp] vp solve %s" % prec_name)
+    mu, info = g.solve_rhs(sig, prec, mu_w2=muw2, tol=1e-12)
+    for c in range(sig.shape[1]):
+        rel = np.linalg.norm(mu[:, c] - ref[:, c]) / np.linalg.norm(ref[:, c])
+        assert rel <= (1e-8 if prec == CP_FP64 else 1e-7), (c, rel)
+        assert info[c]["rel_residual"] <= 1e-12
+
+
+@pytest.mark.parametrize("prec_name", ["fp64", "fp32_refine"])
+def test_vp_warm_start(vp_case, prec_name):
+    """A warm start (the previous step's mu) gives the same solution in fewer
+    iterations than the reference's zero start."""
+    from cpanel_amd import CP_FP32_REFINE, CP_FP64
+    g, sig, muw2, ref = vp_case
+    prec = CP_FP64 if prec_name == "fp64" else CP_FP32_REFINE
+    say("[gpu-vp] warm start %s" % prec_name)
+    cold, ic = g.solve_rhs(sig[:, 0], prec, mu_w2=muw2[:, 0], tol=1e-12)
+    x0 = ref[:, 0] * (1 + 1e-4 * np.random.default_rng(2).normal(size=g.nb))
+    warm, iw = g.solve_rhs(sig[:, 0], prec, mu_w2=muw2[:, 0], x0=x0, tol=1e-12)
+    say("[gpu-vp]   iterations cold %d warm %d" % (ic[0]["iters"], iw[0]["iters"]))
+    assert np.linalg.norm(warm - ref[:, 0]) / np.linalg.norm(ref[:, 0]) <= 1e-8
+    assert iw[0]["iters"] < ic[0]["iters"]
