@@ -174,5 +174,8 @@ def test_vp_warm_start(vp_case, prec_name):
     x0 = ref[:, 0] * (1 + 1e-4 * np.random.default_rng(2).normal(size=g.nb))
     warm, iw = g.solve_rhs(sig[:, 0], prec, mu_w2=muw2[:, 0], x0=x0, tol=1e-12)
     say("[gpu-vp]   iterations cold %d warm %d" % (ic[0]["iters"], iw[0]["iters"]))
-    assert np.linalg.norm(warm - ref[:, 0]) / np.linalg.norm(ref[:, 0]) <= 1e-8
+    # both solve the same system to 1e-12: they agree to cond(A) * 1e-12;
+    # against the FP64 LAPACK solution the mixed path carries its 3e-8
+    assert np.linalg.norm(warm - cold) / np.linalg.norm(cold) <= 1e-9
+    assert np.linalg.norm(warm - ref[:, 0]) / np.linalg.norm(ref[:, 0]) <= (1e-8 if prec == CP_FP64 else 1e-7)
     assert iw[0]["iters"] < ic[0]["iters"]
