@@ -165,6 +165,15 @@ class Geometry:
                        "cp_panel_to_points")
         return out
 
+    def solve_with_derivatives(self, condition, prec=CP_FP64, delta_deg=0.5, **kw):
+        """cpCase::run with stability derivatives (cpCase.cpp:635-665): the
+        condition and its alpha + delta, beta + delta neighbours solved as one
+        batch; returns (result, derivatives)."""
+        from .cpout import stability_derivatives
+        V, a, b = condition
+        r = self.solve([(V, a, b), (V, a + delta_deg, b), (V, a, b + delta_deg)], prec, **kw)
+        return r[0], stability_derivatives(r[0], r[1], r[2], delta_deg)
+
     def set_buffer_wake(self, verts):
         """verts: (m, nv, 3) panels of the second buffer-wake row (nv = 3 or 4)."""
         V = np.asarray(verts, dtype=np.float64)

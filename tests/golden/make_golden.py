@@ -8,6 +8,7 @@ writes small fixtures:
   4412_spanwise.npy                 - 2y/b, Cl, Cdi (spanwiseData_Wake1.csv)
   4412AR6_surface.npz               - per-panel arrays at alpha 5, beta 0.5 (see fixtures.json)
   fixtures.json                     - .CPout numbers and the kernel known-answer table
+  4412AR6_results.txt               - the RESULTS section of 4412AR6.CPout (output text format)
 
 Run: python tests/golden/make_golden.py [/root/reference]
 """
@@ -36,6 +37,9 @@ def vtu_arrays(path, names):
 def main():
     shutil.copy(os.path.join(CASES, "4412", "4412.tri"), os.path.join(OUT, "4412.tri"))
     shutil.copy(os.path.join(CASES, "4412AR6", "4412AR6.tri"), os.path.join(OUT, "4412AR6.tri"))
+    txt = open(os.path.join(CASES, "4412AR6", "4412AR6.CPout")).read()
+    with open(os.path.join(OUT, "4412AR6_results.txt"), "w") as f:
+        f.write(txt[txt.index("-----------RESULTS-----------"):])
     names = ["Doublet Strengths", "Source Strengths", "Velocity Potential", "Velocity", "Cp", "xPosition",
              "yPosition", "zPostition"]
     s = vtu_arrays(os.path.join(CASES, "4412", "V1_Mach0.1_alpha5_beta0", "surfaceData-0.vtu"), names)

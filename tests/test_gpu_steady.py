@@ -251,3 +251,35 @@ def test_block_jacobi_mfma_matches_valu(ctx, monkeypatch):
         assert a["rel_residual"] <= 1e-12 and b["rel_residual"] <= 1e-12
         assert np.linalg.norm(a["mu"] - b["mu"]) / np.linalg.norm(a["mu"]) <= 1e-9
         assert abs(a["CL"] - b["CL"]) <= 1e-10
+
+
+@pytest.mark.parametrize("prec", [CP_FP64, CP_FP32_REFINE])
+def test_4412ar6_stability_derivatives_on_gpu(ctx, golden, fixtures_json, prec):
+    """The 4412AR6 case with its stability derivatives (3 conditions in one
+    batch) against every number 4412AR6.CPout prints (5 decimals; CDi carries
+    the recorded 2.7e-7 drift past its print boundary)."""
+    from cpanel_amd import cpout
+    fx = fixtures_json["4412AR6"]
+    p = fx["params"]
+    say("[gpu] 4412AR6 stability derivatives prec %d" % prec)
+    g = ctx.geometry(os.path.join(golden, "4412AR6.tri"))
+    g.assemble(prec)
+    kw = dict(mach=p["mach"], Sref=p["Sref"], bref=p["bref"], cref=p["cref"], cg=tuple(p["cg"]), tol=1e-12)
+    r, d = g.solve_with_derivatives((p["V"], p["alpha"], p["beta"]), prec, **kw)
+    tol = 0.5e-5 + 1e-9
+    assert abs(r["CL"] - fx["CL"]) <= tol
+    assert abs(r["CDi"] - fx["CDi"]) <= 1e-5
+    assert abs(r["Fwind"][2] - fx["CL_surface"]) <= tol and abs(r["Fwind"][0] - fx["CD_surface"]) <= tol
+    assert abs(r["Fbody"][2] - fx["CN"]) <= tol and abs(r["Fbody"][0] - fx["CA"]) <= tol
+    assert abs(r["CM"][1] - fx["Cm"]) <= tol and abs(r["CM"][0] - fx["Cl"]) <= tol
+    assert abs(r["CM"][2] - fx["Cn"]) <= tol
+    # derivatives: differences of 5-decimal outputs over 0.5 deg amplify the
+    # print rounding by 1/0.0087: compare to 1e-4
+    got = dict(CL_alpha=d["dF_dAlpha"][2], CY_alpha=d["dF_dAlpha"][1], Cm_alpha=d["dM_dAlpha"][1],
+               Cl_alpha=d["dM_dAlpha"][0], Cn_alpha=d["dM_dAlpha"][2], CL_beta=d["dF_dBeta"][2],
+               CY_beta=d["dF_dBeta"][1], Cm_beta=d["dM_dBeta"][1], Cl_beta=d["dM_dBeta"][0],
+               Cn_beta=d["dM_dBeta"][2])
+    for k, v in got.items():
+        assert abs(v - fx[k]) <= 1e-4, (k, v, fx[k])
+    txt = cpout.case_block(1, p["V"], p["mach"], p["alpha"], p["beta"], r, d)
+    assert "CL = 0.65503" in txt
