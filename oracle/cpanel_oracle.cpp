@@ -1526,3 +1526,254 @@ double or_wake_potential(const or_geom* g, const double* pnt) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Panel -> particle stretching (bodyPanel::partStretching, bodyPanel.cpp:697-729;
+// wakePanel::partStretching, wakePanel.cpp:266-289).  M(i, j) = d u_j / d x_i
+// (the reference's "dudx dvdx dwdx / dudy ... " layout); the stretching is
+// M * alpha.  The formulas are restated as the reference writes them,
+// including its frames: the near-field source gradient is evaluated in the
+// panel's local frame and applied to the global strength, and the far-field
+// point source/doublet gradients use the panel centre's z where a relative z
+// would be expected.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct M3 {
+    double a[3][3];
+};
+
+inline M3 m3_zero() {
+    M3 m;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) m.a[i][j] = 0;
+    return m;
+}
+
+inline V3 to_local(const PanelGeo& g, V3 p) {
+    V3 l, m, n;
+    local_sys(g, l, m, n);
+    V3 d = p - g.center;
+    return {dot(l, d), dot(m, d), dot(n, d)};
+}
+
+// bodyPanel::velocityGradientPointSource (bodyPanel.cpp:733-763)
+M3 vg_point_source(const PanelGeo& g, double sigma, V3 poi) {
+    M3 M;
+    double x = g.center.x, y = g.center.y, z = g.center.z, x0 = poi.x, y0 = poi.y, z0 = poi.z;
+    double sdd = std::pow(std::pow(x - x0, 2) + std::pow(y - y0, 2) + z * z, 2.5);
+    double c = sigma * g.area / (4 * PI);
+    M.a[0][0] = (-2 * (x - x0) * (x - x0) + (y - y0) * (y - y0) + z * z) / sdd;
+    M.a[1][0] = -3 * (x - x0) * (y - y0) / sdd;
+    M.a[2][0] = -3 * (x - x0) * z / sdd;
+    M.a[0][1] = -3 * (x - x0) * (y - y0) / sdd;
+    M.a[1][1] = ((x - x0) * (x - x0) - (y - y0) * (y - y0) + z * z) / sdd;
+    M.a[2][1] = -3 * z * (y - y0) / sdd;
+    M.a[0][2] = -3 * (x - x0) * (z - z0) / sdd;
+    M.a[1][2] = -3 * (y - y0) * (z - z0) / sdd;
+    M.a[2][2] = (-3 * z * (z - z0) + ((x - x0) * (x - x0) + (y - y0) * (y - y0) + z * z)) / sdd;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) M.a[i][j] *= c;
+    return M;
+}
+
+// bodyPanel::velocityGradientTriSource / QuadSource (bodyPanel.cpp:766-1063):
+// the Katz source-panel velocity differentiated in local coordinates, one
+// term per edge i -> i+1
+M3 vg_panel_source(const PanelGeo& g, double sigma, V3 poi) {
+    M3 M = m3_zero();
+    const int nv = g.nv;
+    double X[4], Y[4];
+    for (int k = 0; k < nv; k++) {
+        V3 q = to_local(g, g.p[k]);
+        X[k] = q.x;
+        Y[k] = q.y;
+    }
+    V3 P = to_local(g, poi);
+    double x = P.x, y = P.y, z = P.z;
+    double r[4], drx[4], dry[4], drz[4], e[4], h[4];
+    for (int k = 0; k < nv; k++) {
+        double q = (x - X[k]) * (x - X[k]) + (y - Y[k]) * (y - Y[k]) + z * z;
+        r[k] = std::pow(q, 0.5);
+        drx[k] = (x - X[k]) / std::pow(q, 0.5);
+        dry[k] = (y - Y[k]) / std::pow(q, 0.5);
+        drz[k] = z / std::pow(q, 0.5);
+        e[k] = (x - X[k]) * (x - X[k]) + z * z;
+        h[k] = (x - X[k]) * (y - Y[k]);
+    }
+    for (int i = 0; i < nv; i++) {
+        int j = (i + 1) % nv;
+        double d = std::pow((X[j] - X[i]) * (X[j] - X[i]) + (Y[j] - Y[i]) * (Y[j] - Y[i]), 0.5);
+        double S = r[i] + r[j];
+        double G[3] = {drx[i] + drx[j], dry[i] + dry[j], drz[i] + drz[j]};
+        for (int k = 0; k < 3; k++) {
+            M.a[k][0] += (Y[j] - Y[i]) * (S + d) * G[k] * (2 * d) / d / (S - d) / std::pow(S + d, 2);
+            M.a[k][1] += (X[i] - X[j]) * (S + d) * G[k] * (2 * d) / d / (S - d) / std::pow(S + d, 2);
+        }
+        double mm = (Y[j] - Y[i]) / (X[j] - X[i]);
+        for (int t = 0; t < 2; t++) {
+            int q = t == 0 ? i : j;
+            double sg = t == 0 ? 1.0 : -1.0;
+            double me = mm * e[q] - h[q];
+            double zr = z * r[q];
+            double f = 1 / (1 + std::pow(me / zr, 2));
+            M.a[0][2] += sg * f * ((zr * (mm * 2 * (x - X[q]) - (y - Y[q])) - me * (z * drx[q])) / std::pow(zr, 2));
+            M.a[1][2] += sg * f * ((-z * r[q] * (x - X[q]) - me * (z * dry[q])) / std::pow(zr, 2));
+            M.a[2][2] += sg * f * ((zr * mm * (2 * z) - me * (r[q] + z * drz[q])) / std::pow(zr, 2));
+        }
+    }
+    double c = sigma / (4 * PI);
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) M.a[a][b] *= c;
+    return M;
+}
+
+// panel::velocityGradientPointDoublet (panel.cpp:813-840)
+M3 vg_point_doublet(const PanelGeo& g, double mu, V3 poi) {
+    M3 M;
+    double x = g.center.x, y = g.center.y, z = g.center.z, x0 = poi.x, y0 = poi.y;
+    double ddd = std::pow(std::pow(x - x0, 2) + std::pow(y - y0, 2) + z * z, 3.5);
+    double uv = 3 * mu * g.area / (4 * PI);
+    double w = -mu * g.area / (4 * PI);
+    double dx2 = std::pow(x - x0, 2), dy2 = std::pow(y - y0, 2);
+    M.a[0][0] = uv * z * (-4 * dx2 + dy2 + z * z) / ddd;
+    M.a[1][0] = uv * (-5) * z * (y - y0) * (x - x0) / ddd;
+    M.a[2][0] = uv * (x - x0) * (dx2 + dy2 - 4 * z * z) / ddd;
+    M.a[0][1] = uv * (-5) * (x - x0) * (y - y0) * z / ddd;
+    M.a[1][1] = uv * z * (dx2 - 4 * dy2 + z * z) / ddd;
+    M.a[2][1] = uv * (y - y0) * (dx2 + dy2 - 4 * z * z) / ddd;
+    M.a[0][2] = w * (-3) * (x - x0) * (dx2 + dy2 - 4 * z * z) / ddd;
+    M.a[1][2] = w * (-3) * (y - y0) * (dx2 + dy2 - 4 * z * z) / ddd;
+    M.a[2][2] = w * (-3) * z * (3 * dx2 + 3 * dy2 - 2 * z * z) / ddd;
+    return M;
+}
+
+// panel::gradDoub (panel.cpp:876-923): d/dx_i of the vortex-segment
+// velocity (a x b)(|a|+|b|) / (|a||b|(|a||b| + a.b) + (core |s|)^2)
+M3 grad_doub(V3 a, V3 b, V3 s) {
+    M3 M;
+    double na = norm(a), nb = norm(b);
+    V3 top = (na + nb) * cross(a, b);
+    double cs = CORE * norm(s);
+    double bot = na * nb * (na * nb + dot(a, b)) + cs * cs;
+    V3 E[3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    double dan[3] = {a.x / na, a.y / na, a.z / na}, dbn[3] = {b.x / nb, b.y / nb, b.z / nb};
+    for (int k = 0; k < 3; k++) {
+        V3 dtop = (na + nb) * (cross(E[k], b) + cross(a, E[k])) + (dan[k] + dbn[k]) * cross(a, b);
+        double dbot = dan[k] * nb * (na * nb + dot(a, b)) + na * dbn[k] * (na * nb + dot(a, b)) +
+                      na * nb * (dan[k] * nb + na * dbn[k] + dot(E[k], b) + dot(a, E[k]));
+        V3 row = (1.0 / (bot * bot)) * (bot * dtop - dbot * top);
+        M.a[k][0] = row.x;
+        M.a[k][1] = row.y;
+        M.a[k][2] = row.z;
+    }
+    return M;
+}
+
+// panel::nearFilamentCheck (panel.cpp:925-962)
+bool near_filament(V3 p1, V3 p2, V3 poi) {
+    double L = norm(p2 - p1), d1 = norm(p1 - poi), d2 = norm(p2 - poi);
+    if (d1 < CORE * L || d2 < CORE * L) return true;
+    bool outside = d1 > L || d2 > L;
+    double dist = norm(cross(p1 - poi, p2 - poi)) / L;
+    return dist < CORE * L && !outside;
+}
+
+// panel::onPanelCheck (panel.cpp:140-153): the point's projection lies
+// strictly inside the panel (the reference decides this with a convex hull
+// of the local vertices plus the point, boundary points kept, which is this
+// test for a convex panel) and its local z is below 5% of the longest side
+bool on_panel(const PanelGeo& g, V3 poi) {
+    V3 P = to_local(g, poi);
+    for (int k = 0; k < g.nv; k++) {
+        V3 a = to_local(g, g.p[k]), b = to_local(g, g.p[(k + 1) % g.nv]);
+        double cz = (b.x - a.x) * (P.y - a.y) - (b.y - a.y) * (P.x - a.x);
+        if (!(cz > 0)) return false;
+    }
+    return P.z < 0.05 * g.longSide;
+}
+
+// panel::velocityGradientDoublet (panel.cpp:842-874)
+M3 vg_panel_doublet(const PanelGeo& g, double mu, V3 poi) {
+    M3 M = m3_zero();
+    for (int i = 0; i < g.nv; i++) {
+        V3 p1 = g.p[i], p2 = g.p[(i + 1) % g.nv];
+        V3 a = poi - p1, b = poi - p2, s = p2 - p1;
+        bool onp = on_panel(g, poi);
+        bool nearf = near_filament(p1, p2, poi);
+        if (!nearf) {
+            M3 d = grad_doub(a, b, s);
+            for (int r = 0; r < 3; r++)
+                for (int c = onp ? 2 : 0; c < 3; c++) M.a[r][c] += d.a[r][c];
+        }
+    }
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) M.a[r][c] *= mu / (4 * PI);
+    return M;
+}
+
+M3 add3(M3 x, const M3& y) {
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) x.a[r][c] += y.a[r][c];
+    return x;
+}
+
+// bodyPanel / wakePanel ::partStretching velocity-gradient matrix
+M3 part_stretch_matrix(const PanelGeo& g, bool body, double sigma, double mu, V3 pos) {
+    bool far = norm(pos - g.center) / g.longSide > 5;
+    if (body) {
+        if (far) return add3(vg_point_source(g, sigma, pos), vg_point_doublet(g, mu, pos));
+        return add3(vg_panel_source(g, sigma, pos), vg_panel_doublet(g, mu, pos));
+    }
+    return far ? vg_point_doublet(g, mu, pos) : vg_panel_doublet(g, mu, pos);
+}
+
+}  // namespace
+
+extern "C" {
+
+/* velocity-gradient matrix M (row-major 3x3) of one panel at pos:
+ * kind 0: partStretching's (body: source sigma + doublet mu; wake: doublet
+ * mu), 1: point source, 2: panel source (local frame), 3: point doublet,
+ * 4: panel doublet, 5: gradDoub of edge 0 */
+void or_velgrad(int nv, const double* v, int body, int kind, double sigma, double mu, const double* pos,
+                double* out) {
+    PanelGeo g = make_panel(nv, v);
+    V3 p{pos[0], pos[1], pos[2]};
+    M3 M;
+    switch (kind) {
+        case 1: M = vg_point_source(g, sigma, p); break;
+        case 2: M = vg_panel_source(g, sigma, p); break;
+        case 3: M = vg_point_doublet(g, mu, p); break;
+        case 4: M = vg_panel_doublet(g, mu, p); break;
+        case 5: M = grad_doub(p - g.p[0], p - g.p[1], g.p[1] - g.p[0]); break;
+        default: M = part_stretch_matrix(g, body != 0, sigma, mu, p); break;
+    }
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) out[3 * r + c] = M.a[r][c];
+}
+
+/* sum over panels of partStretching on m particles: out[3t..] = sum_j M_j alpha_t */
+void or_panels_stretching(int nb, const int* nvb, const double* vb, const double* sigma, const double* mu, int nw,
+                          const int* nvw, const double* vw, const double* mu_w, int m, const double* pos,
+                          const double* alpha, double* out) {
+    std::vector<PanelGeo> B(nb), W(nw);
+    for (int j = 0; j < nb; j++) B[j] = make_panel(nvb[j], vb + 12 * j);
+    for (int j = 0; j < nw; j++) W[j] = make_panel(nvw[j], vw + 12 * j);
+#pragma omp parallel for schedule(static)
+    for (int t = 0; t < m; t++) {
+        V3 p{pos[3 * t], pos[3 * t + 1], pos[3 * t + 2]};
+        double al[3] = {alpha[3 * t], alpha[3 * t + 1], alpha[3 * t + 2]};
+        double acc[3] = {0, 0, 0};
+        auto apply = [&](const M3& M) {
+            for (int r = 0; r < 3; r++) acc[r] += M.a[r][0] * al[0] + M.a[r][1] * al[1] + M.a[r][2] * al[2];
+        };
+        for (int j = 0; j < nb; j++) apply(part_stretch_matrix(B[j], true, sigma[j], mu[j], p));
+        for (int j = 0; j < nw; j++) apply(part_stretch_matrix(W[j], false, 0.0, mu_w[j], p));
+        out[3 * t] = acc[0];
+        out[3 * t + 1] = acc[1];
+        out[3 * t + 2] = acc[2];
+    }
+}
+
+}  // extern "C"

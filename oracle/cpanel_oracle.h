@@ -130,6 +130,14 @@ void or_panels_to_points(int nb, const int* nvb, const double* vb, const double*
                          const double* mu, int nw, const int* nvw, const double* vw,
                          const double* mu_w, int m, const double* pts, double* out);
 
+/* ---- panel -> particle stretching (bodyPanel::partStretching, bodyPanel.cpp:697-1063;
+ * wakePanel::partStretching, wakePanel.cpp:266-289; panel.cpp:813-962) ------- */
+void or_velgrad(int nv, const double* v, int body, int kind, double sigma, double mu, const double* pos,
+                double* out);
+void or_panels_stretching(int nb, const int* nvb, const double* vb, const double* sigma, const double* mu, int nw,
+                          const int* nvw, const double* vw, const double* mu_w, int m, const double* pos,
+                          const double* alpha, double* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -25,6 +25,8 @@ def _lib():
         L.or_vp_direct.argtypes = [C.c_int, C.c_int, _d, _d, _d, C.c_int, _d, _d]
         L.or_vp_pair.argtypes = [C.c_int, _d, _d, C.c_double, _d, _d, C.c_double, _d]
         L.or_panels_to_points.argtypes = [C.c_int, _i, _d, _d, _d, C.c_int, _i, _d, _d, C.c_int, _d, _d]
+        L.or_velgrad.argtypes = [C.c_int, _d, C.c_int, C.c_int, C.c_double, C.c_double, _d, _d]
+        L.or_panels_stretching.argtypes = [C.c_int, _i, _d, _d, _d, C.c_int, _i, _d, _d, C.c_int, _d, _d, _d]
         L._vp_ready = True
     return L
 
@@ -101,4 +103,37 @@ def panels_to_points(body_verts, sigma, mu, wake_verts, mu_w, pts):
     mw = _a(mu_w) if nw else np.zeros(1)
     _lib().or_panels_to_points(nb, _p(nvb, _i), _p(vb), _p(_a(sigma)), _p(_a(mu)), nw, _p(nvw, _i), _p(vw),
                                _p(mw), len(q), _p(q), _p(out))
+    return out
+
+
+VG_PART, VG_POINT_SOURCE, VG_PANEL_SOURCE, VG_POINT_DOUBLET, VG_PANEL_DOUBLET, VG_EDGE = 0, 1, 2, 3, 4, 5
+
+
+def velgrad(verts, pos, kind=VG_PART, body=True, sigma=1.0, mu=1.0):
+    """3x3 M(i, j) = d u_j / d x_i of one panel at pos (see or_velgrad)."""
+    v = _a(verts).reshape(-1)
+    out = np.zeros(9)
+    _lib().or_velgrad(len(v) // 3, _p(v), 1 if body else 0, kind, float(sigma), float(mu), _p(_a(pos)), _p(out))
+    return out.reshape(3, 3)
+
+
+def panels_stretching(body_verts, sigma, mu, wake_verts, mu_w, pos, alpha):
+    def pack(V):
+        V = _a(V)
+        k = V.shape[0]
+        nv = np.full(k, V.shape[1], dtype=np.int32)
+        buf = np.zeros((k, 12))
+        buf[:, :3 * V.shape[1]] = V.reshape(k, -1)
+        return k, nv, buf
+    nb, nvb, vb = pack(body_verts)
+    if wake_verts is not None and len(wake_verts):
+        nw, nvw, vw = pack(wake_verts)
+        mw = _a(mu_w)
+    else:
+        nw, nvw, vw, mw = 0, np.zeros(1, np.int32), np.zeros((1, 12)), np.zeros(1)
+    q = _a(pos).reshape(-1, 3)
+    al = _a(alpha).reshape(-1, 3)
+    out = np.zeros_like(q)
+    _lib().or_panels_stretching(nb, _p(nvb, _i), _p(vb), _p(_a(sigma)), _p(_a(mu)), nw, _p(nvw, _i), _p(vw), _p(mw),
+                                len(q), _p(q), _p(al), _p(out))
     return out

@@ -163,3 +163,110 @@ def test_closed_doublet_surface_induces_no_velocity():
     # and a single panel does induce a velocity
     u1 = V.panels_to_points(verts[:1], np.zeros(1), np.ones(1), None, None, pts)
     assert np.abs(u1).max() > 1e-3
+
+
+# ---------------------------------------------------------------- stretching
+def _fd(f, p, h=1e-6):
+    J = np.zeros((3, 3))
+    for i in range(3):
+        e = np.zeros(3)
+        e[i] = h
+        J[i] = (f(p + e) - f(p - e)) / (2 * h)
+    return J  # J[i, j] = d f_j / d x_i, the layout of the gradient matrices
+
+
+def _generic_triangle(seed):
+    rng = np.random.default_rng(seed)
+    tri = rng.normal(size=(3, 3))
+    c = tri.mean(0)
+    n = np.cross(tri[1] - tri[0], tri[2] - tri[0])
+    n /= np.linalg.norm(n)
+    l = (tri[0] - c) / np.linalg.norm(tri[0] - c)
+    R = np.array([l, np.cross(n, l), n])
+    L = max(np.linalg.norm(tri[(k + 1) % 3] - tri[k]) for k in range(3))
+    return tri, c, n, R, L
+
+
+def test_panel_doublet_gradient_is_derivative_of_doublet_velocity():
+    """panel::velocityGradientDoublet (sum of gradDoub over the edges) is the
+    exact derivative of panel::dubVInf's near-field velocity."""
+    from oracle import oracle as O
+    for seed in (1, 2, 3):
+        tri, c, n, R, L = _generic_triangle(seed)
+        p = c + 0.8 * L * n + 0.4 * L * R[0]
+        M = V.velgrad(tri, p, V.VG_PANEL_DOUBLET, mu=1.0)
+        J = _fd(lambda q: O.dub_v_inf(tri, q), p)
+        assert np.abs(M - J).max() <= 1e-7 * np.abs(J).max()
+
+
+def test_panel_source_gradient_is_local_frame_derivative():
+    """bodyPanel::velocityGradientTriSource is minus the derivative of the
+    source velocity (bodyPanel::panelVInf) taken in the panel's local frame
+    (local coordinates in, local velocity components out)."""
+    from oracle import oracle as O
+    for seed in (3, 4, 5):
+        tri, c, n, R, L = _generic_triangle(seed)
+        for p in (c + 0.7 * L * n + 0.3 * L * R[0], c + np.array([1.0, 2.0, 0.5]) * L):
+            M = V.velgrad(tri, p, V.VG_PANEL_SOURCE, sigma=1.0)
+            J = R @ _fd(lambda q: O.panel_v_inf(tri, q)[0], p) @ R.T
+            assert np.abs(M + J).max() <= 1e-6 * np.abs(J).max()
+
+
+def test_point_gradients_far_field():
+    """Point doublet and point source gradients (panel.cpp:813-840,
+    bodyPanel.cpp:733-763) are minus the derivatives of the far-field
+    velocities for a panel normal to z and a point at z = 0 (where the
+    formulas' use of the centre's z is exact); the source's d v / d y entry
+    is the reference's ((x-x0)^2 - (y-y0)^2 + z^2) / r^5, which differs from
+    the derivative by (y-y0)^2 / r^5 and is kept as written."""
+    from oracle import oracle as O
+    tri = np.array([[1.0, 0, 0.3], [-1, 1, 0.3], [-1, -1, 0.3]])
+    c = tri.mean(0)
+    for p in (np.array([9.0, -7.0, 0.0]), np.array([-6.0, 10.0, 0.0])):
+        Md = V.velgrad(tri, p, V.VG_POINT_DOUBLET, mu=1.0)
+        Jd = _fd(lambda q: O.panel_v_inf(tri, q)[1], p)
+        assert np.abs(Md + Jd).max() <= 1e-7 * np.abs(Jd).max()
+        Ms = V.velgrad(tri, p, V.VG_POINT_SOURCE, sigma=1.0)
+        Js = _fd(lambda q: O.panel_v_inf(tri, q)[0], p)
+        mask = np.ones((3, 3), bool)
+        mask[1, 1] = False
+        assert np.abs((Ms + Js)[mask]).max() <= 1e-7 * np.abs(Js).max()
+        d = c - p
+        r5 = (d[0] ** 2 + d[1] ** 2 + c[2] ** 2) ** 2.5
+        area = 2.0
+        assert abs(Ms[1, 1] - (-Js[1, 1] + d[1] ** 2 / r5 * area / (4 * np.pi))) <= 1e-7 * np.abs(Js).max()
+
+
+def test_part_stretching_dispatch():
+    """partStretching: body panels add source + doublet gradients (point
+    forms beyond 5 longest sides), wake panels the doublet only; the
+    stretching is M alpha."""
+    tri, c, n, R, L = _generic_triangle(7)
+    al = np.array([0.3, -0.2, 0.5])
+    for p in (c + 0.6 * L * n, c + 7 * L * n):
+        far = np.linalg.norm(p - c) / L > 5
+        Mb = V.velgrad(tri, p, V.VG_PART, body=True, sigma=0.7, mu=-0.4)
+        ks, kd = (V.VG_POINT_SOURCE, V.VG_POINT_DOUBLET) if far else (V.VG_PANEL_SOURCE, V.VG_PANEL_DOUBLET)
+        assert np.allclose(Mb, V.velgrad(tri, p, ks, sigma=0.7) + V.velgrad(tri, p, kd, mu=-0.4), rtol=1e-14,
+                           atol=0)
+        Mw = V.velgrad(tri, p, V.VG_PART, body=False, mu=-0.4)
+        assert np.array_equal(Mw, V.velgrad(tri, p, kd, mu=-0.4))
+        s = V.panels_stretching(tri[None], [0.7], [-0.4], tri[None], [-0.4], p[None], al[None])[0]
+        assert np.allclose(s, Mb @ al + Mw @ al, rtol=1e-13, atol=0)
+
+
+def test_doublet_gradient_exclusions():
+    """Edges whose filament core (5 % of the edge) holds the point add
+    nothing; a point over the panel within 5 % of the longest side keeps
+    only the d w / d x_i column (panel.cpp:842-874)."""
+    tri, c, n, R, L = _generic_triangle(8)
+    on = c + 0.01 * L * n
+    M = V.velgrad(tri, on, V.VG_PANEL_DOUBLET, mu=1.0)
+    assert np.all(M[:, :2] == 0) and np.any(M[:, 2] != 0)
+    # a point just above vertex 0: edges 0 -> 1 and 2 -> 0 are skipped, only
+    # the edge 1 -> 2 term remains (gradDoub of that edge)
+    out = (tri[0] - c) / np.linalg.norm(tri[0] - c)  # just outside the panel's projection
+    near_vertex = tri[0] + 0.002 * L * out + 0.001 * L * n
+    Mv = V.velgrad(tri, near_vertex, V.VG_PANEL_DOUBLET, mu=4 * np.pi)
+    e12 = V.velgrad(np.roll(tri, -1, axis=0), near_vertex, V.VG_EDGE)
+    assert np.allclose(Mv, e12, rtol=1e-13, atol=0)
