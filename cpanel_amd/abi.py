@@ -90,6 +90,7 @@ SIGNATURES = {
     "cp_bh_eval": (C.c_int, [C.c_void_p, C.c_int, C.c_int32, _d, _d]),
     "cp_bh_release": (None, [C.c_void_p]),
     "cp_panel_to_points": (C.c_int, [C.c_void_p, _d, _d, _d, C.c_int32, _d, _d]),
+    "cp_panel_stretching": (C.c_int, [C.c_void_p, _d, _d, _d, C.c_int32, _d, _d, _d]),
     "cp_set_buffer_wake": (C.c_int, [C.c_void_p, C.c_int32, _i32, _d]),
     "cp_solve_rhs": (C.c_int, [C.c_void_p, C.c_int, C.c_int32, _d, _d, _d, C.POINTER(SolveOpts), _d,
                                C.POINTER(SolveInfo)]),

@@ -206,6 +206,23 @@ class Geometry:
                       converged=bool(i.converged)) for i in info]
         return (mu[0] if one else mu.T), infos
 
+    def panel_stretching(self, sigma, mu, pos, alpha, mu_wake=None):
+        """Stretching of particles (pos, alpha: m x 3) by the body panels
+        (sigma, mu) and wake panels (mu_wake): sum of partStretching."""
+        sg = np.ascontiguousarray(sigma, dtype=np.float64)
+        m_ = np.ascontiguousarray(mu, dtype=np.float64)
+        if sg.shape != (self.nb,) or m_.shape != (self.nb,):
+            raise ValueError("sigma and mu need one value per body panel")
+        mw = None if mu_wake is None else np.ascontiguousarray(mu_wake, dtype=np.float64)
+        q = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+        al = np.ascontiguousarray(alpha, dtype=np.float64).reshape(-1, 3)
+        if al.shape != q.shape:
+            raise ValueError("pos and alpha must describe the same particles")
+        out = np.zeros_like(q)
+        self.ctx.check(abi.lib().cp_panel_stretching(self.h, _p(sg), _p(m_), _p(mw), len(q), _p(q), _p(al), _p(out)),
+                       "cp_panel_stretching")
+        return out
+
     def solver_report(self):
         import json
         buf = C.create_string_buffer(1 << 16)

@@ -1132,3 +1132,37 @@ cp_status cp_solve_rhs(cp_geom* g, cp_prec prec, int32_t k, const double* sigma,
     });
 }
 
+cp_status cp_panel_stretching(cp_geom* g, const double* sigma, const double* mu, const double* mu_wake, int32_t m,
+                              const double* pos, const double* alpha, double* out) {
+    if (!g || !sigma || !mu || m < 0 || (m > 0 && (!pos || !alpha || !out))) return CP_BAD_ARG;
+    cp_ctx* c = g->ctx;
+    if (c->device < 0) return fail(c, CP_NO_DEVICE, "host-only context");
+    if (m == 0) return CP_OK;
+    return guarded_g(g, [&]() -> cp_status {
+        hipStream_t s = g->stream;
+        const int nb = g->dev.n, nw = g->dev.nw;
+        std::vector<double> str((size_t)nb * 2 + std::max(nw, 1), 0.0);
+        std::memcpy(str.data(), sigma, nb * sizeof(double));
+        std::memcpy(str.data() + nb, mu, nb * sizeof(double));
+        if (mu_wake && nw > 0) std::memcpy(str.data() + 2 * nb, mu_wake, nw * sizeof(double));
+        const int KS = p2p_slices(m, nb + nw);
+        std::vector<double> pa(6 * (size_t)m);
+        std::memcpy(pa.data(), pos, 3 * (size_t)m * sizeof(double));
+        std::memcpy(pa.data() + 3 * (size_t)m, alpha, 3 * (size_t)m * sizeof(double));
+        DevBuf dstr, dpa, dpart, dout;
+        upload(c, dstr, str);
+        upload(c, dpa, pa);
+        dalloc(c, dpart, (size_t)3 * m * KS * sizeof(double));
+        dalloc(c, dout, (size_t)3 * m * sizeof(double));
+        {
+            TimedScope ts("panel_stretching", s);
+            launch_panel_stretching(g->dev, dstr.as<double>(), dstr.as<double>() + nb, dstr.as<double>() + 2 * nb, m,
+                                    dpa.as<double>(), dpa.as<double>() + 3 * (size_t)m, KS, dpart.as<double>(),
+                                    dout.as<double>(), s);
+        }
+        CP_HIP(hipMemcpyAsync(out, dout.p, (size_t)3 * m * sizeof(double), hipMemcpyDeviceToHost, s));
+        CP_HIP(hipStreamSynchronize(s));
+        return CP_OK;
+    });
+}
+

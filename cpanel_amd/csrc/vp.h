@@ -32,6 +32,11 @@ int p2p_slices(int m, int npanels);
 void launch_panel_to_points(const DevGeom& g, const double* sigma, const double* mu, const double* muw, int m,
                             const double* pts, int KS, double* part, double* out, hipStream_t s);
 
+// out[3m] = sum over body and wake panels of partStretching at the particles
+// (pos, alpha: 3 per particle); part: KS * 3m scratch
+void launch_panel_stretching(const DevGeom& g, const double* sigma, const double* mu, const double* muw, int m,
+                             const double* pos, const double* alpha, int KS, double* part, double* out,
+                             hipStream_t s);
 // y[0..n) += a[0..n)
 void launch_add_vec(int n, const double* a, double* y, hipStream_t s);
 // out (column-major, ld) = -C mu for k columns: C is lda x m (FP64), mu m x k

@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include "device.h"
+#include "stretch.h"
 #include "vp.h"
 
 namespace cpgpu {
@@ -224,6 +225,33 @@ __global__ __launch_bounds__(kT) void p2p_kernel(const PanelRec* __restrict__ bo
     }
 }
 
+// panel -> particle stretching over the panels [p0, p1): per particle the
+// velocity-gradient matrices of the slice are summed, then applied to alpha
+__global__ __launch_bounds__(kT) void stretch_kernel(const PanelRec* __restrict__ body, int nb,
+                                                     const PanelRec* __restrict__ wake, int nw,
+                                                     const double* __restrict__ sigma, const double* __restrict__ mu,
+                                                     const double* __restrict__ muw, int m,
+                                                     const double* __restrict__ pos,
+                                                     const double* __restrict__ alpha, int chunk,
+                                                     double* __restrict__ part) {
+    const int t = blockIdx.x * kT + threadIdx.x;
+    const int ks = blockIdx.y;
+    const int p0 = ks * chunk, p1 = min(nb + nw, p0 + chunk);
+    if (t >= m) return;
+    const double p[3] = {pos[3 * t], pos[3 * t + 1], pos[3 * t + 2]};
+    Mat3 G;
+    mat3_zero(G);
+    for (int j = p0; j < p1; j++) {
+        if (j < nb)
+            part_stretch_add(body[j], true, sigma[j], mu[j], p, G);
+        else
+            part_stretch_add(wake[j - nb], false, 0.0, muw[j - nb], p, G);
+    }
+    const double a0 = alpha[3 * t], a1 = alpha[3 * t + 1], a2 = alpha[3 * t + 2];
+    double* o = part + ((int64_t)ks * m + t) * 3;
+    for (int r = 0; r < 3; r++) o[r] = G.a[r][0] * a0 + G.a[r][1] * a1 + G.a[r][2] * a2;
+}
+
 __global__ void p2p_reduce_kernel(const double* __restrict__ part, int KS, int m, double* __restrict__ out) {
     int e = blockIdx.x * kT + threadIdx.x;
     if (e >= 3 * m) return;
@@ -308,6 +336,21 @@ void launch_panel_to_points(const DevGeom& g, const double* sigma, const double*
     dim3 grid((unsigned)((m + kT - 1) / kT), (unsigned)KS);
     hipLaunchKernelGGL(p2p_kernel, grid, dim3(kT), 0, s, g.body, g.n, g.wake, g.nw, sigma, mu, muw, m, pts, chunk,
                        part);
+    CP_LAUNCH_CHECK();
+    hipLaunchKernelGGL(p2p_reduce_kernel, dim3((unsigned)((3 * m + kT - 1) / kT)), dim3(kT), 0, s,
+                       (const double*)part, KS, m, out);
+    CP_LAUNCH_CHECK();
+}
+
+void launch_panel_stretching(const DevGeom& g, const double* sigma, const double* mu, const double* muw, int m,
+                             const double* pos, const double* alpha, int KS, double* part, double* out,
+                             hipStream_t s) {
+    if (m <= 0) return;
+    const int np = g.n + g.nw;
+    const int chunk = (np + KS - 1) / KS;
+    dim3 grid((unsigned)((m + kT - 1) / kT), (unsigned)KS);
+    hipLaunchKernelGGL(stretch_kernel, grid, dim3(kT), 0, s, g.body, g.n, g.wake, g.nw, sigma, mu, muw, m, pos, alpha,
+                       chunk, part);
     CP_LAUNCH_CHECK();
     hipLaunchKernelGGL(p2p_reduce_kernel, dim3((unsigned)((3 * m + kT - 1) / kT)), dim3(kT), 0, s,
                        (const double*)part, KS, m, out);

@@ -183,6 +183,14 @@ void cp_bh_release(cp_tree* t);
 cp_status cp_panel_to_points(cp_geom* g, const double* sigma, const double* mu, const double* mu_wake, int32_t m,
                              const double* pts, double* vel);
 
+/* ---- panel -> particle stretching (bodyPanel::partStretching,
+ *      bodyPanel.cpp:697-729; wakePanel::partStretching, wakePanel.cpp:266-289)
+ * out[3m] = sum over body panels (source sigma, doublet mu) and wake panels
+ * (doublet mu_wake, NULL = zero) of the velocity-gradient matrix at each
+ * particle applied to its strength: pos, alpha: 3 per particle. */
+cp_status cp_panel_stretching(cp_geom* g, const double* sigma, const double* mu, const double* mu_wake, int32_t m,
+                              const double* pos, const double* alpha, double* out);
+
 /* ---- vortex-particle solve (cpCaseVP::solveVPmatrixEq, cpCaseVP.cpp:265-301)
  * The VP case's second buffer-wake row: m panels (nv[j] = 3 or 4 vertices,
  * verts[12 j + 3 q + d]) whose doublet influence on the body collocation
