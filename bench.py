@@ -96,6 +96,12 @@ def vp_bench(ctx, log):
     ms, n = ctx.timing("panel_to_points")
     out["panel_to_points_ms"] = ms
     out["panel_to_points_gpairs_per_s"] = len(R) * (g.nb + g.nw) / (ms * 1e-3) / 1e9
+    g.panel_stretching(sig, mu, X[:1000], A[:1000])
+    ctx.timing_reset()
+    g.panel_stretching(sig, mu, X, A, mu_wake=rng.normal(size=g.nw))
+    ms, n = ctx.timing("panel_stretching")
+    out["panel_stretching_ms"] = ms
+    out["panel_stretching_gpairs_per_s"] = len(R) * (g.nb + g.nw) / (ms * 1e-3) / 1e9
     log("vp: %s" % json.dumps(out))
     return out
 
