@@ -283,3 +283,45 @@ def test_4412ar6_stability_derivatives_on_gpu(ctx, golden, fixtures_json, prec):
         assert abs(v - fx[k]) <= 1e-4, (k, v, fx[k])
     txt = cpout.case_block(1, p["V"], p["mach"], p["alpha"], p["beta"], r, d)
     assert "CL = 0.65503" in txt
+
+
+@pytest.fixture(scope="module")
+def c1(ctx):
+    """The plain steady case at its configured size: NACA 4412, AR 6, 80 x 30,
+    flat wake 20 c (4,956 body panels)."""
+    m = meshgen.config_c1()
+    og, pg = build_pair(ctx, m)
+    return og, pg, m
+
+
+def test_c1_fp64_matches_oracle(c1):
+    """configs' plain case (alpha 5, beta 0) at full size: GPU FP64 mu within
+    1e-8 of the FP64 oracle (both to ||r||/||b|| <= 1e-12), CL/CDi to 1e-8."""
+    og, pg, m = c1
+    say("[gpu] C1 fp64 vs oracle n=%d" % pg.nb)
+    A, B = og.assemble()
+    pg.assemble(CP_FP64)
+    r = pg.solve([(1.0, 5.0, 0.0)], CP_FP64, tol=1e-12, **P4412)[0]
+    ref = og.run_case(A, B, V=1.0, alpha=5.0, beta=0.0, tol=1e-12, **P4412)
+    rel = np.linalg.norm(r["mu"] - ref["mu"]) / np.linalg.norm(ref["mu"])
+    say("[gpu]   rel dmu %.2e, dCL %.2e" % (rel, abs(r["CL"] - ref["CL"])))
+    assert rel <= 1e-8
+    assert abs(r["CL"] - ref["CL"]) <= 1e-8 and abs(r["CDi"] - ref["CDi"]) <= 1e-8
+
+
+def test_c1_mixed_64_conditions(c1):
+    """The 64-condition grid on the full-size plain case: every column of the
+    mixed batched solve within |dCL| <= 1e-5 and |dCp| <= 1e-4 of the plain
+    FP64 GPU solve."""
+    og, pg, m = c1
+    conds = [(1.0, float(a), float(b)) for b in (-5.0, 0.0, 2.5, 5.0) for a in np.linspace(-4, 10, 16)]
+    say("[gpu] C1 mixed 64 conditions")
+    pg.assemble(CP_FP64)
+    pg.assemble(CP_FP32_REFINE)
+    r64 = pg.solve(conds, CP_FP64, tol=1e-12, **P4412)
+    r32 = pg.solve(conds, CP_FP32_REFINE, tol=1e-10, **P4412)
+    dcl = max(abs(a["CL"] - b["CL"]) for a, b in zip(r64, r32))
+    dcp = max((np.abs(a["cp"] - b["cp"]) / np.maximum(1.0, np.abs(a["cp"]))).max() for a, b in zip(r64, r32))
+    say("[gpu]   max dCL %.2e, max dCp/max(1,|Cp|) %.2e" % (dcl, dcp))
+    assert dcl <= 1e-5 and dcp <= 1e-4
+    assert all(b["rel_residual"] <= 1e-10 for b in r32)
