@@ -2,5 +2,5 @@
 
 Native library: cpanel_amd/libcpgpu.so (C ABI in include/cpgpu.h).
 """
-from .abi import CP_FP32_REFINE, CP_FP64, lib  # noqa: F401
+from .abi import CP_FP32_REFINE, CP_FP32_REFINE64, CP_FP64, lib  # noqa: F401
 from .case import Context, CPanelError, Geometry  # noqa: F401

@@ -12,7 +12,7 @@ LIB_PATH = os.path.join(HERE, "libcpgpu.so")
 
 CP_OK, CP_NOT_CONVERGED, CP_OUT_OF_BUDGET, CP_BAD_MESH, CP_HIP_ERROR, CP_BAD_ARG, CP_NO_DEVICE, \
     CP_NOT_ASSEMBLED = range(8)
-CP_FP64, CP_FP32_REFINE = 0, 1
+CP_FP64, CP_FP32_REFINE, CP_FP32_REFINE64 = 0, 1, 2
 
 _d = C.POINTER(C.c_double)
 _i32 = C.POINTER(C.c_int32)

@@ -70,7 +70,14 @@ public:
     // Y = Badd + sign * (A32 + D) X64 with FP64 accumulation; D = the sparse
     // near-field correction given by corr (values on the set_near pattern)
     void matvec64acc(const float* A32, const double* X, double* Y, const double* Badd, double sign, hipStream_t s,
-                     const double* corr = nullptr);
+                     const double* corr = nullptr, const double* A64 = nullptr);
+    // FP64 matrices for the refinement residual and the right-hand side (the
+    // refined solution is then the FP64 system's); null = FP32 + near-field
+    // correction
+    void set_f64(const double* A64, const double* B64) {
+        a64_ = A64;
+        b64_ = B64;
+    }
     // strip block-Jacobi preconditioner for the inner solve (device arrays;
     // null perm = Jacobi).  inv: FP32 inverses, column-major per block.
     void set_block_jacobi(const int* perm, const int* ptr, const int64_t* off, const float* inv, int nblocks,
@@ -103,6 +110,8 @@ private:
                     const double* sig_cm = nullptr, const double* add_cm = nullptr, const double* x0_cm = nullptr);
     void launch_gemm32(const GemmShape& sh, const float* A32, const float* P, hipStream_t s);
     void apply_block_jacobi(const float* P, float* Y, hipStream_t s);
+    const double* a64_ = nullptr;
+    const double* b64_ = nullptr;
     const void* apk_ = nullptr;
     int nkb_ = 0;
     void* ppk_ = nullptr;  // bf16 hi/lo tiles of the multivector being multiplied

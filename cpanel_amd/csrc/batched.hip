@@ -535,10 +535,22 @@ __global__ __launch_bounds__(256) void gemm_f64acc_kernel(const float* __restric
 // 4(l%16)..+3 at column k + l/16 and its components feed the 4 row tiles.
 // U K steps of 4 per chunk; the next chunk's loads are issued before the
 // current chunk's MFMAs.
-template <int U>
-__global__ __launch_bounds__(256) void gemm_f64acc_p_kernel(const float* __restrict__ A, int64_t lda, int n,
+template <class T>
+struct Vec4Of;
+template <>
+struct Vec4Of<float> {
+    typedef float4 type;
+};
+template <>
+struct Vec4Of<double> {
+    typedef double4 type;
+};
+
+template <int U, class T = float>
+__global__ __launch_bounds__(256) void gemm_f64acc_p_kernel(const T* __restrict__ A, int64_t lda, int n,
                                                             const double* __restrict__ X, double* __restrict__ part,
                                                             int kchunk, int KS) {
+    typedef typename Vec4Of<T>::type V4;
     const int bid = blockIdx.x;
     const int ks = bid % KS, rb = bid / KS;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -550,18 +562,19 @@ __global__ __launch_bounds__(256) void gemm_f64acc_p_kernel(const float* __restr
     for (int a = 0; a < 4; a++)
 #pragma unroll
         for (int b = 0; b < 4; b++) acc[a][b] = doublex4{0, 0, 0, 0};
-    const float* Ab = A + row0 + 4 * m;
+    const T* Ab = A + row0 + 4 * m;
     const double* Xb = X + m;
-    float4 a[U], an[U];
+    V4 a[U], an[U];
     double b[U][4], bn[U][4];
-    auto load = [&](int k, float4 (&aa)[U], double (&bb)[U][4]) {
+    auto load = [&](int k, V4 (&aa)[U], double (&bb)[U][4]) {
 #pragma unroll
         for (int u = 0; u < U; u++) {
             const int64_t kk = k + 4 * u + lk;
             const bool ok = kk < k1;
             const int64_t kc = ok ? kk : 0;
-            float4 v = *reinterpret_cast<const float4*>(Ab + kc * lda);
-            aa[u] = ok ? v : make_float4(0.f, 0.f, 0.f, 0.f);
+            V4 v = *reinterpret_cast<const V4*>(Ab + kc * lda);
+            if (!ok) v.x = v.y = v.z = v.w = 0;
+            aa[u] = v;
 #pragma unroll
             for (int c = 0; c < 4; c++) bb[u][c] = ok ? Xb[kc * KP + 16 * c] : 0.0;
         }
@@ -1321,7 +1334,7 @@ void BatchedSolver::matvec32(const float* A32, const float* P, float* Y, hipStre
 }
 
 void BatchedSolver::matvec64acc(const float* A32, const double* X, double* Y, const double* Badd, double sign,
-                                hipStream_t s, const double* corr) {
+                                hipStream_t s, const double* corr, const double* A64) {
     // K slices: enough workgroups for two per CU, no more (each slice adds an
     // lda x 64 FP64 partial to write and reduce)
     const int rb = (int)(lda_ / 256);
@@ -1331,13 +1344,16 @@ void BatchedSolver::matvec64acc(const float* A32, const double* X, double* Y, co
     dim3 grid((unsigned)(rb * KS));
     {
         TimedScope ts("resid_f64", s);
-        if (f64_pipelined_)
+        if (A64)
+            hipLaunchKernelGGL((gemm_f64acc_p_kernel<2, double>), grid, dim3(256), 0, s, A64, lda_, n_, X, part64_,
+                               kchunk, KS);
+        else if (f64_pipelined_)
             hipLaunchKernelGGL(gemm_f64acc_p_kernel<2>, grid, dim3(256), 0, s, A32, lda_, n_, X, part64_, kchunk, KS);
         else
             hipLaunchKernelGGL(gemm_f64acc_kernel, grid, dim3(256), 0, s, A32, lda_, n_, X, part64_, kchunk, KS);
     }
     CP_LAUNCH_CHECK();
-    const double* vals = (corr && near_rowptr_) ? corr : nullptr;
+    const double* vals = (corr && near_rowptr_ && !A64) ? corr : nullptr;
     hipLaunchKernelGGL(reduce64_kernel, dim3((unsigned)nrb_), dim3(kT), 0, s, (const double*)part64_, KS, lda_, n_,
                        Badd, sign, Y, dpart_, near_rowptr_, near_cols_, vals, X);
     CP_LAUNCH_CHECK();
@@ -1400,7 +1416,7 @@ void BatchedSolver::tile_solve(const float* A32, const float* B32, const double*
     else
         hipLaunchKernelGGL(sigma_rows_kernel, dim3(nel), dim3(kT), 0, s, n_, lda_, kc, normals, vinf4, S_);
     if (add_cm) hipLaunchKernelGGL(gather_cols_kernel, dim3(nel), dim3(kT), 0, s, n_, lda_, kc, add_cm, ldx, R_);
-    matvec64acc(B32, S_, Bv_, add_cm ? R_ : nullptr, -1.0, s, near_dB_);
+    matvec64acc(B32, S_, Bv_, add_cm ? R_ : nullptr, -1.0, s, near_dB_, b64_);
     auto colsum = [&](const double* dp, int nq, int q0, int q1) {
         hipLaunchKernelGGL(colsum_kernel, dim3(1), dim3(1024), 0, s, dp, nrb_, nq, q0, q1, sc_);
         CP_LAUNCH_CHECK();
@@ -1428,7 +1444,7 @@ void BatchedSolver::tile_solve(const float* A32, const float* B32, const double*
     std::vector<double> rel(KP, 0.0);
     while (true) {
         // residual R = b - A32 X in FP64
-        matvec64acc(A32, X_, R_, Bv_, -1.0, s, near_dA_);
+        matvec64acc(A32, X_, R_, Bv_, -1.0, s, near_dA_, a64_);
         colsum(dpart_, 1, Q_RR64, 0);
         CP_HIP(hipMemcpyAsync(host_sc_, sc_, KP * Q_NQ * sizeof(double), hipMemcpyDeviceToHost, s));
         CP_HIP(hipStreamSynchronize(s));

@@ -566,7 +566,11 @@ cp_status cp_geometry_clusters(const cp_geom* g, int32_t* cl16, int32_t* ts_orde
 }
 
 cp_status cp_assemble(cp_geom* g, cp_prec prec) {
-    if (!g || (prec != CP_FP64 && prec != CP_FP32_REFINE)) return CP_BAD_ARG;
+    if (!g || (prec != CP_FP64 && prec != CP_FP32_REFINE && prec != CP_FP32_REFINE64)) return CP_BAD_ARG;
+    if (prec == CP_FP32_REFINE64) {  // both sets: FP64 for refinement, FP32 for the Krylov iterations
+        cp_status st = cp_assemble(g, CP_FP64);
+        return st == CP_OK ? cp_assemble(g, CP_FP32_REFINE) : st;
+    }
     cp_ctx* c = g->ctx;
     if (c->device < 0) return fail(c, CP_NO_DEVICE, "host-only context");
     return guarded_g(g, [&] {
@@ -659,13 +663,15 @@ cp_status cp_read_matrices(cp_geom* g, cp_prec prec, double* A, double* B, int64
 cp_status cp_solve(cp_geom* g, cp_prec prec, int32_t k, const cp_freestream* fs, const cp_solve_opts* opts,
                    double* mu, double* cpout, cp_solve_info* info) {
     if (!g || !fs || !opts || k <= 0) return CP_BAD_ARG;
-    if (prec != CP_FP64 && prec != CP_FP32_REFINE) return CP_BAD_ARG;
+    if (prec != CP_FP64 && prec != CP_FP32_REFINE && prec != CP_FP32_REFINE64) return CP_BAD_ARG;
     cp_ctx* c = g->ctx;
     if (c->device < 0) return fail(c, CP_NO_DEVICE, "host-only context");
     return guarded_g(g, [&]() -> cp_status {
         if (prec == CP_FP64 && !g->have64) return fail(c, CP_NOT_ASSEMBLED, "call cp_assemble(CP_FP64) first");
         if (prec == CP_FP32_REFINE && !g->have32)
             return fail(c, CP_NOT_ASSEMBLED, "call cp_assemble(CP_FP32_REFINE) first");
+        if (prec == CP_FP32_REFINE64 && !(g->have32 && g->have64))
+            return fail(c, CP_NOT_ASSEMBLED, "call cp_assemble(CP_FP32_REFINE64) first");
         ensure_post(g);
         const int n = g->dev.n;
         const int64_t lda = g->dev.lda;
@@ -709,6 +715,10 @@ cp_status cp_solve(cp_geom* g, cp_prec prec, int32_t k, const cp_freestream* fs,
             g->batched->set_near(g->near_rowptr.as<int>(), g->near_cols.as<int>(), g->near_dA.as<double>(),
                                  g->near_dB.as<double>());
             g->batched->set_packed(g->A16.p, packed_kblocks(n));
+            if (prec == CP_FP32_REFINE64)
+                g->batched->set_f64(g->A64.as<double>(), g->B64.as<double>());
+            else
+                g->batched->set_f64(nullptr, nullptr);
             if (opts->precond != 1) {
                 ensure_block_jacobi(g);
             }
@@ -764,7 +774,7 @@ cp_status cp_solve(cp_geom* g, cp_prec prec, int32_t k, const cp_freestream* fs,
 }
 
 cp_status cp_matvec(cp_geom* g, int kernel, int32_t k, const double* X, double* Y) {
-    if (!g || !X || !Y || k <= 0 || kernel < 0 || kernel > 3) return CP_BAD_ARG;
+    if (!g || !X || !Y || k <= 0 || kernel < 0 || kernel > 4) return CP_BAD_ARG;
     if (kernel > 0 && k > kRhsTile) return CP_BAD_ARG;
     cp_ctx* c = g->ctx;
     if (c->device < 0) return fail(c, CP_NO_DEVICE, "host-only context");
@@ -789,6 +799,7 @@ cp_status cp_matvec(cp_geom* g, int kernel, int32_t k, const double* X, double* 
             return CP_OK;
         }
         if (!g->have32) return fail(c, CP_NOT_ASSEMBLED, "FP32 matrices not assembled");
+        if (kernel == 4 && !g->have64) return fail(c, CP_NOT_ASSEMBLED, "FP64 matrices not assembled");
         if (!g->batched) {
             const int64_t need = (int64_t)BatchedSolver::workspace_bytes(lda);
             g->batched.reset(new BatchedSolver(n, lda, kRhsTile));
@@ -820,7 +831,8 @@ cp_status cp_matvec(cp_geom* g, int kernel, int32_t k, const double* X, double* 
             upload(c, dx, xr);
             dalloc(c, dy, elems * sizeof(double));
             g->batched->matvec64acc(g->A32.as<float>(), dx.as<double>(), dy.as<double>(), nullptr, 1.0, s,
-                                    kernel == 3 ? g->near_dA.as<double>() : nullptr);
+                                    kernel == 3 ? g->near_dA.as<double>() : nullptr,
+                                    kernel == 4 ? g->A64.as<double>() : nullptr);
             CP_HIP(hipMemcpyAsync(yr.data(), dy.p, elems * sizeof(double), hipMemcpyDeviceToHost, s));
             CP_HIP(hipStreamSynchronize(s));
             for (int q = 0; q < k; q++)
@@ -1043,7 +1055,8 @@ cp_status cp_set_buffer_wake(cp_geom* g, int32_t m, const int32_t* nv, const dou
 
 cp_status cp_solve_rhs(cp_geom* g, cp_prec prec, int32_t k, const double* sigma, const double* mu_w2,
                        const double* x0, const cp_solve_opts* opts, double* mu, cp_solve_info* info) {
-    if (!g || !sigma || !opts || k <= 0 || (prec != CP_FP64 && prec != CP_FP32_REFINE)) return CP_BAD_ARG;
+    if (!g || !sigma || !opts || k <= 0 || (prec != CP_FP64 && prec != CP_FP32_REFINE && prec != CP_FP32_REFINE64))
+        return CP_BAD_ARG;
     cp_ctx* c = g->ctx;
     if (c->device < 0) return fail(c, CP_NO_DEVICE, "host-only context");
     if (mu_w2 && g->nw2 == 0) return fail(c, CP_BAD_ARG, "mu_w2 given but no buffer wake set");
@@ -1051,6 +1064,8 @@ cp_status cp_solve_rhs(cp_geom* g, cp_prec prec, int32_t k, const double* sigma,
         if (prec == CP_FP64 && !g->have64) return fail(c, CP_NOT_ASSEMBLED, "call cp_assemble(CP_FP64) first");
         if (prec == CP_FP32_REFINE && !g->have32)
             return fail(c, CP_NOT_ASSEMBLED, "call cp_assemble(CP_FP32_REFINE) first");
+        if (prec == CP_FP32_REFINE64 && !(g->have32 && g->have64))
+            return fail(c, CP_NOT_ASSEMBLED, "call cp_assemble(CP_FP32_REFINE64) first");
         const int n = g->dev.n;
         const int64_t lda = g->dev.lda;
         hipStream_t s = g->stream;
@@ -1097,6 +1112,10 @@ cp_status cp_solve_rhs(cp_geom* g, cp_prec prec, int32_t k, const double* sigma,
             g->batched->set_near(g->near_rowptr.as<int>(), g->near_cols.as<int>(), g->near_dA.as<double>(),
                                  g->near_dB.as<double>());
             g->batched->set_packed(g->A16.p, packed_kblocks(n));
+            if (prec == CP_FP32_REFINE64)
+                g->batched->set_f64(g->A64.as<double>(), g->B64.as<double>());
+            else
+                g->batched->set_f64(nullptr, nullptr);
             if (opts->precond != 1) ensure_block_jacobi(g);
             if (opts->precond != 1 && !g->bj_plan.perm.empty())
                 g->batched->set_block_jacobi(g->bj_perm.as<int>(), g->bj_ptr.as<int>(), g->bj_off.as<int64_t>(),

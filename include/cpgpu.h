@@ -48,7 +48,9 @@ typedef enum {
 
 typedef enum {
     CP_FP64 = 0,            /* plain FP64 assembly and FP64 BiCGSTAB */
-    CP_FP32_REFINE = 1      /* FP32 A,B (FP64 near field), FP32 Krylov, FP64 refinement */
+    CP_FP32_REFINE = 1,     /* FP32 A,B (FP64 near field), FP32 Krylov, FP64 refinement */
+    CP_FP32_REFINE64 = 2    /* FP32 Krylov refined against the resident FP64 A,B: the
+                               refined solution is the FP64 system's (both sets assembled) */
 } cp_prec;
 
 typedef struct cp_ctx cp_ctx;
@@ -139,6 +141,7 @@ cp_status cp_solve(cp_geom* g, cp_prec prec, int32_t k, const cp_freestream* fs,
  *   kernel 2: FP64-accumulated MFMA GEMM on the FP32 A (k <= 64)
  *   kernel 3: kernel 2 plus the FP64 near-field correction (entries with
  *             |c_i - c_j| <= longSide_j carry their FP64 value)
+ *   kernel 4: FP64 MFMA GEMM on the FP64 A (k <= 64; needs both sets)
  * X, Y: host, n_body x k column-major. */
 cp_status cp_matvec(cp_geom* g, int kernel, int32_t k, const double* X, double* Y);
 
