@@ -119,6 +119,7 @@ private:
     bool bj_mfma_ = true;  // matrix-core block product (else vector ALU)
     bool bf16x3_ = true;   // bf16x3 GEMM is a tuner candidate
     bool f64_pipelined_ = true;  // pipelined FP64-accumulated product
+    int f64_depth_ = 2;          // K steps per stage of the FP64-matrix product
     const int* bj_perm_ = nullptr;
     const int* bj_iperm_ = nullptr;  // body index -> position in block order
     float* pp_ = nullptr;            // multivector in block order

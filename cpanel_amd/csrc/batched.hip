@@ -547,7 +547,7 @@ struct Vec4Of<double> {
 };
 
 template <int U, class T = float>
-__global__ __launch_bounds__(256) void gemm_f64acc_p_kernel(const T* __restrict__ A, int64_t lda, int n,
+__global__ __launch_bounds__(256, 2) void gemm_f64acc_p_kernel(const T* __restrict__ A, int64_t lda, int n,
                                                             const double* __restrict__ X, double* __restrict__ part,
                                                             int kchunk, int KS) {
     typedef typename Vec4Of<T>::type V4;
@@ -1146,6 +1146,11 @@ BatchedSolver::BatchedSolver(int n, int64_t lda, int capacity) : n_(n), cap_(cap
     int rb = (int)(lda / 256);
     shape_.rt = 2;
     shape_.kslices = rb >= 64 ? 8 : (rb >= 16 ? 16 : 32);
+    // CPGPU_F64_DEPTH=2|3: K steps of 4 per load stage of the FP64-matrix residual
+    if (const char* e = std::getenv("CPGPU_F64_DEPTH")) {
+        int d = std::atoi(e);
+        if (d >= 2 && d <= 3) f64_depth_ = d;
+    }
     // CPGPU_F64_SIMPLE=1 selects the unpipelined FP64-accumulated product (tests)
     if (const char* e = std::getenv("CPGPU_F64_SIMPLE")) f64_pipelined_ = e[0] != '1';
     // CPGPU_NO_BF16X3=1 keeps the inner product in FP32 matrix-core arithmetic
@@ -1345,7 +1350,10 @@ void BatchedSolver::matvec64acc(const float* A32, const double* X, double* Y, co
     dim3 grid((unsigned)(rb * KS));
     {
         TimedScope ts("resid_f64", s);
-        if (A64)
+        if (A64 && f64_depth_ == 3)
+            hipLaunchKernelGGL((gemm_f64acc_p_kernel<3, double>), grid, dim3(256), 0, s, A64, lda_, n_, X, part64_,
+                               kchunk, KS);
+        else if (A64)
             hipLaunchKernelGGL((gemm_f64acc_p_kernel<2, double>), grid, dim3(256), 0, s, A64, lda_, n_, X, part64_,
                                kchunk, KS);
         else if (f64_pipelined_)
