@@ -174,3 +174,45 @@ def summarize(results, wall):
                 p50_ms=1e3 * float(np.percentile(lat, 50)), p99_ms=1e3 * float(np.percentile(lat, 99)),
                 mean_queue_ms=1e3 * float(np.mean([r["stats"]["queue_s"] for r in results])),
                 cache_hit_frac=float(np.mean([r["stats"]["cache_hit"] for r in results])))
+
+
+def main(argv=None):
+    """python -m cpanel_amd.serve --requests N --qps Q [--slots S] [--seed K]
+    [--budget-gb G]: runs the seeded mixed-size stream on device 0 and prints
+    one JSON summary line (p50/p99 latency, solves/s, cache behaviour)."""
+    import argparse
+    import json
+    import sys
+
+    from .case import Context
+    ap = argparse.ArgumentParser(prog="python -m cpanel_amd.serve")
+    ap.add_argument("--requests", type=int, default=2000)
+    ap.add_argument("--qps", type=float, default=10.0)
+    ap.add_argument("--slots", type=int, default=4)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--budget-gb", type=float, default=200.0)
+    ap.add_argument("--min-panels", type=int, default=500)
+    ap.add_argument("--max-panels", type=int, default=20000)
+    a = ap.parse_args(argv)
+    t0 = time.perf_counter()
+    stream, geoms = make_stream(a.requests, a.qps, seed=a.seed, n_lo=a.min_panels, n_hi=a.max_panels)
+    print("stream: %d requests, %d geometries generated in %.1fs" % (len(stream), len(geoms),
+                                                                     time.perf_counter() - t0),
+          file=sys.stderr, flush=True)
+    ctx = Context(0)
+    S = Server(ctx, slots=a.slots, hbm_budget=int(a.budget_gb * (1 << 30)))
+    res, wall = run_stream(S, stream, geoms)
+    st = S.stats()
+    S.close()
+    out = summarize(res, wall)
+    out.update(dict(qps=a.qps, slots=a.slots, seed=a.seed, geometries=len(geoms), evictions=st["evictions"],
+                    batches=st["batches"], budget_gb=a.budget_gb,
+                    p90_ms=1e3 * float(np.percentile([r["stats"]["latency_s"] for r in res], 90)),
+                    mean_prepare_ms_new_geometry=1e3 * float(np.mean(
+                        [r["stats"]["prepare_s"] for r in res if not r["stats"]["cache_hit"]] or [0])),
+                    mean_solve_ms=1e3 * float(np.mean([r["stats"]["solve_s"] for r in res]))))
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
