@@ -85,6 +85,8 @@ SIGNATURES = {
                            C.POINTER(SolveInfo)]),
     "cp_matvec": (C.c_int, [C.c_void_p, C.c_int, C.c_int32, _d, _d]),
     "cp_solver_report": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int32]),
+    "cp_block_jacobi_read": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int32),
+                                       C.POINTER(C.c_int32), C.POINTER(C.c_float)]),
     "cp_bh_build": (C.c_int, [C.c_void_p, C.c_int32, _d, _d, _d, C.POINTER(BhOpts), C.POINTER(C.c_void_p)]),
     "cp_bh_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), _d, _d]),
     "cp_bh_eval": (C.c_int, [C.c_void_p, C.c_int, C.c_int32, _d, _d]),

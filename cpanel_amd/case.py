@@ -223,6 +223,28 @@ class Geometry:
                        "cp_panel_stretching")
         return out
 
+    def block_jacobi(self):
+        """(ptr, perm, [inverse of each block as a b x b FP32 array]) of the
+        mixed-precision solve's strip block-Jacobi preconditioner, or None
+        when the point-Jacobi fallback is in use."""
+        sz = (C.c_int64 * 3)()
+        lib = abi.lib()
+        self.ctx.check(lib.cp_block_jacobi_read(self.h, sz, None, None, None), "block jacobi")
+        nb, npan, tot = sz[0], sz[1], sz[2]
+        if nb == 0:
+            return None
+        ptr = np.zeros(nb + 1, np.int32)
+        perm = np.zeros(npan, np.int32)
+        inv = np.zeros(tot, np.float32)
+        self.ctx.check(lib.cp_block_jacobi_read(self.h, sz, _p(ptr, C.c_int32), _p(perm, C.c_int32),
+                                                _p(inv, C.c_float)), "block jacobi")
+        blocks, o = [], 0
+        for q in range(nb):
+            b = int(ptr[q + 1] - ptr[q])
+            blocks.append(inv[o:o + b * b].reshape(b, b).T)  # stored column-major
+            o += b * b
+        return ptr, perm, blocks
+
     def solver_report(self):
         import json
         buf = C.create_string_buffer(1 << 16)

@@ -38,9 +38,14 @@ struct TuneRecord {
 
 class BatchedSolver {
 public:
-    BatchedSolver(int n, int64_t lda, int capacity);
+    // ws: device workspace of workspace_bytes(lda) bytes; host: pinned host
+    // memory of host_bytes() bytes.  Both stay owned by the caller (pooled
+    // there: freeing device or pinned memory waits for the whole device).
+    // The workspace's zero-initialised parts are cleared on stream s.
+    BatchedSolver(int n, int64_t lda, int capacity, void* ws, void* host, hipStream_t s);
     // device bytes of the workspace of a solver with leading dimension lda
     static size_t workspace_bytes(int64_t lda);
+    static size_t host_bytes();
     ~BatchedSolver();
     BatchedSolver(const BatchedSolver&) = delete;
     BatchedSolver& operator=(const BatchedSolver&) = delete;
@@ -157,8 +162,12 @@ void launch_sigma(int n, int64_t lda, const double* normals, const double* vinf4
 int packed_kblocks(int n);
 size_t packed_a_bytes(int64_t lda, int n);
 void launch_pack_a(const float* A32, int64_t lda, int n, void* out, hipStream_t s);
-// copies the diagonal blocks of A32 into packed column-major storage
-void launch_bj_gather(const float* A32, int64_t lda, const int* perm, const int* ptr, const int64_t* off, int nblocks,
-                      int max_b, float* out, hipStream_t s);
+// Inverts the diagonal blocks of A32 named by the block plan (perm, ptr,
+// off; rows per block <= 1024) in FP64 on the device and stores the inverses
+// in FP32, packed column-major, in out.  Work buffers: X (off[nblocks]
+// doubles), used and piv (n ints each, n = ptr[nblocks]), status (nblocks
+// ints; status[q] != 0 marks a singular block, whose inverse is not written).
+void launch_bj_invert(const float* A32, int64_t lda, const int* perm, const int* ptr, const int64_t* off, int nblocks,
+                      int max_b, double* X, int* used, int* piv, int* status, float* out, hipStream_t s);
 
 }  // namespace cpgpu

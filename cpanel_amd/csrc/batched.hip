@@ -941,18 +941,6 @@ __global__ void sigma_kernel(int n, const double* __restrict__ nrm, int64_t lda,
 }
 
 // ---------------------------------------------------------------- block Jacobi
-// gather the diagonal blocks of A32 (column-major per block, FP32)
-__global__ __launch_bounds__(256) void bj_gather_kernel(const float* __restrict__ A, int64_t lda,
-                                                        const int* __restrict__ perm, const int* __restrict__ ptr,
-                                                        const int64_t* __restrict__ off, float* __restrict__ out) {
-    const int q = blockIdx.y;
-    const int b0 = ptr[q], b = ptr[q + 1] - b0;
-    const int64_t e = (int64_t)blockIdx.x * kT + threadIdx.x;
-    if (e >= (int64_t)b * b) return;
-    const int r = (int)(e % b), c = (int)(e / b);
-    out[off[q] + e] = A[perm[b0 + r] + (int64_t)perm[b0 + c] * lda];
-}
-
 constexpr int kBjCols = 8;  // multivector columns per workgroup
 
 // Y[rows of block q] = inv_q * P[rows of block q]; the block's gathered rows of
@@ -1110,12 +1098,15 @@ size_t BatchedSolver::workspace_bytes(int64_t lda) {
            (size_t)4 * nrb * KP * sizeof(double) + KP * Q_NQ * sizeof(double) + KP * sizeof(int) + 21 * 256;
 }
 
-BatchedSolver::BatchedSolver(int n, int64_t lda, int capacity) : n_(n), cap_(capacity), lda_(lda) {
+size_t BatchedSolver::host_bytes() { return KP * Q_NQ * sizeof(double) + 2 * KP * sizeof(int); }
+
+BatchedSolver::BatchedSolver(int n, int64_t lda, int capacity, void* ws, void* host, hipStream_t s)
+    : n_(n), cap_(capacity), lda_(lda) {
     nrb_ = (int)(lda / kRedRows);
     const size_t mv64 = (size_t)lda * KP * sizeof(double);
     const size_t mv32 = (size_t)lda * KP * sizeof(float);
     const int KSmax = 32;
-    CP_HIP(hipMalloc(&ws_, workspace_bytes(lda)));
+    ws_ = ws;
     char* p = (char*)ws_;
     auto take = [&](size_t b) {
         char* q = p;
@@ -1136,11 +1127,11 @@ BatchedSolver::BatchedSolver(int n, int64_t lda, int capacity) : n_(n), cap_(cap
     dpart_ = (double*)take((size_t)4 * nrb_ * KP * sizeof(double));
     sc_ = (double*)take(KP * Q_NQ * sizeof(double));
     act_ = (int*)take(KP * sizeof(int));
-    CP_HIP(hipMemset(y_, 0, mv32));
-    CP_HIP(hipMemset(z_, 0, mv32));
-    CP_HIP(hipMemset(pp_, 0, mv32));
-    CP_HIP(hipHostMalloc(&host_sc_, KP * Q_NQ * sizeof(double), hipHostMallocDefault));
-    CP_HIP(hipHostMalloc(&host_act_, 2 * KP * sizeof(int), hipHostMallocDefault));
+    CP_HIP(hipMemsetAsync(y_, 0, mv32, s));
+    CP_HIP(hipMemsetAsync(z_, 0, mv32, s));
+    CP_HIP(hipMemsetAsync(pp_, 0, mv32, s));
+    host_sc_ = static_cast<double*>(host);
+    host_act_ = reinterpret_cast<int*>(host_sc_ + KP * Q_NQ);
     // default shape before tuning: 8 K slices (one per XCD), more for small
     // matrices
     int rb = (int)(lda / 256);
@@ -1170,11 +1161,7 @@ BatchedSolver::BatchedSolver(int n, int64_t lda, int capacity) : n_(n), cap_(cap
     }
 }
 
-BatchedSolver::~BatchedSolver() {
-    if (ws_) (void)hipFree(ws_);
-    if (host_sc_) (void)hipHostFree(host_sc_);
-    if (host_act_) (void)hipHostFree(host_act_);
-}
+BatchedSolver::~BatchedSolver() = default;
 
 void BatchedSolver::launch_gemm32(const GemmShape& sh, const float* A32, const float* P, hipStream_t s) {
     const int KS = sh.kslices;
@@ -1382,13 +1369,6 @@ void BatchedSolver::apply_block_jacobi(const float* P, float* Y, hipStream_t s) 
         hipLaunchKernelGGL(bj_apply_kernel, grid, dim3(kT), (size_t)bj_maxb_ * kBjCols * sizeof(float), s, bj_perm_,
                            bj_ptr_, bj_off_, bj_inv_, P, Y);
     }
-    CP_LAUNCH_CHECK();
-}
-
-void launch_bj_gather(const float* A32, int64_t lda, const int* perm, const int* ptr, const int64_t* off, int nblocks,
-                      int max_b, float* out, hipStream_t s) {
-    dim3 grid((unsigned)(((int64_t)max_b * max_b + kT - 1) / kT), (unsigned)nblocks);
-    hipLaunchKernelGGL(bj_gather_kernel, grid, dim3(kT), 0, s, A32, lda, perm, ptr, off, out);
     CP_LAUNCH_CHECK();
 }
 

@@ -6,6 +6,7 @@
 //             FP64 and/or FP32, post-processing tables, solver workspaces.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -44,6 +45,11 @@ struct cp_ctx {
     // device, which would stall the other geometries' streams
     std::multimap<size_t, void*> pool;
     uint64_t pooled = 0;
+    // pinned host blocks of BatchedSolver::host_bytes() and idle
+    // non-blocking streams of released geometries, reused for the same
+    // reason (hipHostFree and stream teardown synchronise)
+    std::vector<void*> pinned;
+    std::vector<hipStream_t> streams;
 };
 
 namespace {
@@ -147,6 +153,8 @@ struct cp_geom {
     hipStream_t stream = nullptr;  // every device call on this geometry
     std::mutex mu;                 // serialises calls on this geometry
     int64_t solver_bytes = 0;      // workspaces counted in ctx->in_use
+    DevBuf solver_ws;              // BatchedSolver workspace
+    void* solver_host = nullptr;   // BatchedSolver pinned host block (ctx pool)
     Geometry host;
     DevGeom dev;
     DevBuf body, wake, cxyz, fold_ptr, fold_w, fold_c, normals;
@@ -264,7 +272,14 @@ cp_status build_geom(cp_ctx* c, const cp_mesh* m, uint64_t key, cp_geom** out) {
         *out = g.release();
         return CP_OK;
     }
-    CP_HIP(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
+    {
+        std::lock_guard<std::mutex> lk(c->alloc_mu);
+        if (!c->streams.empty()) {
+            g->stream = c->streams.back();
+            c->streams.pop_back();
+        }
+    }
+    if (!g->stream) CP_HIP(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
     std::vector<PanelRec> br(n), wr(std::max(nw, 1));
     for (int i = 0; i < n; i++) br[i] = make_panel_rec(G.body[i]);
     for (int j = 0; j < nw; j++) wr[j] = make_panel_rec(G.wake[j]);
@@ -370,8 +385,29 @@ void ensure_post(cp_geom* g) {
     g->post_ready = true;
 }
 
-// Strip block-Jacobi inverses of the FP32 matrix: plan on the host, gather the
-// diagonal blocks on the device, invert in FP64 on host threads, upload FP32.
+// The batched mixed-precision solver of a geometry, built on first use: its
+// workspace comes from the context's device pool, its pinned host block from
+// the context's pinned pool.
+void ensure_batched(cp_geom* g) {
+    if (g->batched) return;
+    cp_ctx* c = g->ctx;
+    const int64_t lda = g->dev.lda;
+    dalloc(c, g->solver_ws, BatchedSolver::workspace_bytes(lda));
+    if (!g->solver_host) {
+        {
+            std::lock_guard<std::mutex> lk(c->alloc_mu);
+            if (!c->pinned.empty()) {
+                g->solver_host = c->pinned.back();
+                c->pinned.pop_back();
+            }
+        }
+        if (!g->solver_host) CP_HIP(hipHostMalloc(&g->solver_host, BatchedSolver::host_bytes(), hipHostMallocDefault));
+    }
+    g->batched.reset(new BatchedSolver(g->dev.n, lda, kRhsTile, g->solver_ws.p, g->solver_host, g->stream));
+}
+
+// Strip block-Jacobi inverses of the FP32 matrix: plan on the host, invert
+// the diagonal blocks on the device (FP64 Gauss-Jordan, FP32 result).
 void ensure_block_jacobi(cp_geom* g) {
     if (g->bj_ready) return;
     cp_ctx* c = g->ctx;
@@ -386,19 +422,25 @@ void ensure_block_jacobi(cp_geom* g) {
         upload(c, g->bj_off, P.off);
         dalloc(c, g->bj_inv, std::max<int64_t>(P.off.back(), 1) * sizeof(float));
     }
-    const int64_t total = P.off.back();
-    launch_bj_gather(g->A32.as<float>(), g->dev.lda, g->bj_perm.as<int>(), g->bj_ptr.as<int>(), g->bj_off.as<int64_t>(),
-                     P.nblocks(), P.max_b, g->bj_inv.as<float>(), g->stream);
-    std::vector<float> f(total);
-    CP_HIP(hipMemcpyAsync(f.data(), g->bj_inv.p, total * sizeof(float), hipMemcpyDeviceToHost, g->stream));
-    CP_HIP(hipStreamSynchronize(g->stream));
-    std::vector<double> d(f.begin(), f.end());
-    bool ok = invert_blocks(P, d);
-    for (int64_t e = 0; e < total; e++) f[e] = (float)d[e];
-    if (ok) CP_HIP(hipMemcpyAsync(g->bj_inv.p, f.data(), total * sizeof(float), hipMemcpyHostToDevice, g->stream));
+    const int nb = P.nblocks();
+    const int np = (int)P.perm.size();
+    DevBuf X, flags, st;
+    dalloc(c, X, (size_t)std::max<int64_t>(P.off.back(), 1) * sizeof(double));
+    dalloc(c, flags, 2 * (size_t)std::max(np, 1) * sizeof(int));
+    dalloc(c, st, (size_t)std::max(nb, 1) * sizeof(int));
+    {
+        TimedScope ts("bj_invert", g->stream);
+        launch_bj_invert(g->A32.as<float>(), g->dev.lda, g->bj_perm.as<int>(), g->bj_ptr.as<int>(),
+                         g->bj_off.as<int64_t>(), nb, P.max_b, X.as<double>(), flags.as<int>(), flags.as<int>() + np,
+                         st.as<int>(), g->bj_inv.as<float>(), g->stream);
+    }
+    std::vector<int> sv(std::max(nb, 1), 0);
+    if (nb > 0)
+        CP_HIP(hipMemcpyAsync(sv.data(), st.p, (size_t)nb * sizeof(int), hipMemcpyDeviceToHost, g->stream));
     CP_HIP(hipStreamSynchronize(g->stream));
     g->bj_ready = true;
-    if (!ok) g->bj_plan.perm.clear();  // singular block: fall back to Jacobi
+    // a singular block: fall back to Jacobi
+    if (std::any_of(sv.begin(), sv.end(), [](int v) { return v != 0; })) g->bj_plan.perm.clear();
 }
 
 void fill_info(const double* r, const double T[9], cp_solve_info& inf) {
@@ -460,6 +502,8 @@ void cp_destroy(cp_ctx* c) {
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     (void)hipStreamDestroy(c->stream);
+    for (hipStream_t st : c->streams) (void)hipStreamDestroy(st);
+    for (void* p : c->pinned) (void)hipHostFree(p);
     for (auto& e : c->pool) (void)hipFree(e.second);
     delete c;
 }
@@ -505,8 +549,12 @@ void cp_geometry_release(cp_geom* g) {
         if (g->solver_bytes) account(c, -g->solver_bytes);
         st = g->stream;
     }
+    void* host = g->solver_host;
+    g->batched.reset();
     delete g;
-    if (st) (void)hipStreamDestroy(st);
+    std::lock_guard<std::mutex> lk(c->alloc_mu);
+    if (host) c->pinned.push_back(host);
+    if (st) c->streams.push_back(st);
 }
 
 cp_status cp_geometry_info(const cp_geom* g, cp_geom_info* info) {
@@ -705,13 +753,7 @@ cp_status cp_solve(cp_geom* g, cp_prec prec, int32_t k, const cp_freestream* fs,
                                           opts->max_iter, s);
             }
         } else {
-            if (!g->batched) {
-                const int64_t need = (int64_t)BatchedSolver::workspace_bytes(lda);
-                if (!fits(c, (uint64_t)need)) throw BudgetError{(uint64_t)need};
-                g->batched.reset(new BatchedSolver(n, lda, kRhsTile));
-                g->solver_bytes += need;
-                account(c, need);
-            }
+            ensure_batched(g);
             g->batched->set_near(g->near_rowptr.as<int>(), g->near_cols.as<int>(), g->near_dA.as<double>(),
                                  g->near_dB.as<double>());
             g->batched->set_packed(g->A16.p, packed_kblocks(n));
@@ -800,12 +842,7 @@ cp_status cp_matvec(cp_geom* g, int kernel, int32_t k, const double* X, double* 
         }
         if (!g->have32) return fail(c, CP_NOT_ASSEMBLED, "FP32 matrices not assembled");
         if (kernel == 4 && !g->have64) return fail(c, CP_NOT_ASSEMBLED, "FP64 matrices not assembled");
-        if (!g->batched) {
-            const int64_t need = (int64_t)BatchedSolver::workspace_bytes(lda);
-            g->batched.reset(new BatchedSolver(n, lda, kRhsTile));
-            g->solver_bytes += need;
-            account(c, need);
-        }
+        ensure_batched(g);
         g->batched->set_near(g->near_rowptr.as<int>(), g->near_cols.as<int>(), g->near_dA.as<double>(),
                              g->near_dB.as<double>());
         g->batched->set_packed(g->A16.p, packed_kblocks(n));
@@ -864,6 +901,30 @@ cp_status cp_solver_report(cp_geom* g, char* buf, int32_t len) {
          std::to_string(g->near_nnz) + "}";
     std::snprintf(buf, (size_t)len, "%s", r.c_str());
     return (int32_t)r.size() < len ? CP_OK : CP_BAD_ARG;
+}
+
+cp_status cp_block_jacobi_read(cp_geom* g, int64_t* sizes, int32_t* ptr, int32_t* perm, float* inv) {
+    if (!g || !sizes) return CP_BAD_ARG;
+    cp_ctx* c = g->ctx;
+    if (c->device < 0) return fail(c, CP_NO_DEVICE, "host-only context");
+    return guarded_g(g, [&] {
+        if (!g->have32) return fail(c, CP_NOT_ASSEMBLED, "call cp_assemble(CP_FP32_REFINE) first");
+        ensure_block_jacobi(g);
+        const BlockPlan& P = g->bj_plan;
+        const bool on = !P.perm.empty();
+        sizes[0] = on ? P.nblocks() : 0;
+        sizes[1] = on ? (int64_t)P.perm.size() : 0;
+        sizes[2] = on ? P.off.back() : 0;
+        if (!on) return CP_OK;
+        if (ptr) std::copy(P.ptr.begin(), P.ptr.end(), ptr);
+        if (perm) std::copy(P.perm.begin(), P.perm.end(), perm);
+        if (inv) {
+            CP_HIP(hipMemcpyAsync(inv, g->bj_inv.p, (size_t)P.off.back() * sizeof(float), hipMemcpyDeviceToHost,
+                                  g->stream));
+            CP_HIP(hipStreamSynchronize(g->stream));
+        }
+        return CP_OK;
+    });
 }
 
 cp_status cp_timing_reset(cp_ctx* c) {
@@ -1102,13 +1163,7 @@ cp_status cp_solve_rhs(cp_geom* g, cp_prec prec, int32_t k, const double* sigma,
                                           opts->max_iter, s, nullptr, x0 ? dx0.as<double>() + lda * q : nullptr);
             }
         } else {
-            if (!g->batched) {
-                const int64_t need = (int64_t)BatchedSolver::workspace_bytes(lda);
-                if (!fits(c, (uint64_t)need)) throw BudgetError{(uint64_t)need};
-                g->batched.reset(new BatchedSolver(n, lda, kRhsTile));
-                g->solver_bytes += need;
-                account(c, need);
-            }
+            ensure_batched(g);
             g->batched->set_near(g->near_rowptr.as<int>(), g->near_cols.as<int>(), g->near_dA.as<double>(),
                                  g->near_dB.as<double>());
             g->batched->set_packed(g->A16.p, packed_kblocks(n));

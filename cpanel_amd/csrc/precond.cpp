@@ -63,69 +63,4 @@ void build_strip_blocks(const Geometry& G, int target, int hard_max, BlockPlan& 
     }
 }
 
-namespace {
-
-// Gauss-Jordan inversion with partial pivoting of a row-major b x b matrix.
-bool gauss_jordan(int b, double* M) {
-    std::vector<std::pair<int, int>> swaps;
-    for (int k = 0; k < b; k++) {
-        int p = k;
-        double best = std::fabs(M[(size_t)k * b + k]);
-        for (int i = k + 1; i < b; i++) {
-            double v = std::fabs(M[(size_t)i * b + k]);
-            if (v > best) {
-                best = v;
-                p = i;
-            }
-        }
-        if (!(best > 0)) return false;
-        if (p != k) {
-            std::swap_ranges(M + (size_t)k * b, M + (size_t)k * b + b, M + (size_t)p * b);
-            swaps.push_back({k, p});
-        }
-        double* rk = M + (size_t)k * b;
-        const double piv = rk[k];
-        rk[k] = 1.0;
-        const double inv = 1.0 / piv;
-        for (int j = 0; j < b; j++) rk[j] *= inv;
-        for (int i = 0; i < b; i++) {
-            if (i == k) continue;
-            double* ri = M + (size_t)i * b;
-            const double f = ri[k];
-            if (f == 0) continue;
-            ri[k] = 0.0;
-            for (int j = 0; j < b; j++) ri[j] -= f * rk[j];
-        }
-    }
-    // inverse of the row-permuted matrix -> undo with column swaps, reversed
-    for (auto it = swaps.rbegin(); it != swaps.rend(); ++it)
-        for (int i = 0; i < b; i++) std::swap(M[(size_t)i * b + it->first], M[(size_t)i * b + it->second]);
-    return true;
-}
-
-}  // namespace
-
-bool invert_blocks(const BlockPlan& plan, std::vector<double>& blocks) {
-    std::atomic<bool> ok{true};
-    const int nb = plan.nblocks();
-    std::atomic<int> next{0};
-    parallel_for(std::max(nb, 256), 0, [&](int, int) {
-        // dynamic assignment: blocks differ in size
-        for (;;) {
-            int q = next.fetch_add(1);
-            if (q >= nb) break;
-            int b = plan.ptr[q + 1] - plan.ptr[q];
-            double* M = blocks.data() + plan.off[q];
-            // column-major in, row-major work, column-major out
-            std::vector<double> R((size_t)b * b);
-            for (int r = 0; r < b; r++)
-                for (int s = 0; s < b; s++) R[(size_t)r * b + s] = M[(size_t)s * b + r];
-            if (!gauss_jordan(b, R.data())) ok = false;
-            for (int r = 0; r < b; r++)
-                for (int s = 0; s < b; s++) M[(size_t)s * b + r] = R[(size_t)r * b + s];
-        }
-    });
-    return ok;
-}
-
 }  // namespace cpgpu

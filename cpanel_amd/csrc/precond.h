@@ -28,8 +28,4 @@ struct BlockPlan {
 // single strip larger than target stays whole, up to `hard_max`).
 void build_strip_blocks(const Geometry& G, int target, int hard_max, BlockPlan& plan);
 
-// In-place inversion of packed column-major blocks (FP64 Gauss-Jordan with
-// partial pivoting), host threads.  Returns false if a block is singular.
-bool invert_blocks(const BlockPlan& plan, std::vector<double>& blocks);
-
 }  // namespace cpgpu

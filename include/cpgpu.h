@@ -254,6 +254,14 @@ void cp_server_destroy(cp_server* s);
  * and the autotuner's timings, block-Jacobi blocks, near-field entries. */
 cp_status cp_solver_report(cp_geom* g, char* buf, int32_t len);
 
+/* Diagnostic, no counterpart in the reference: the strip block-Jacobi
+ * preconditioner of the mixed-precision solve (built on first use after
+ * cp_assemble(CP_FP32_REFINE)).  sizes[0] = blocks, sizes[1] = panels
+ * covered, sizes[2] = packed inverse entries (all 0 when the point-Jacobi
+ * fallback is in use).  ptr (blocks+1), perm (panels) and inv (FP32 inverse of
+ * block q, column-major, at sum of earlier b^2) may be NULL. */
+cp_status cp_block_jacobi_read(cp_geom* g, int64_t* sizes, int32_t* ptr, int32_t* perm, float* inv);
+
 /* ---- timing: HIP-event totals of the named kernel on the context stream -- */
 cp_status cp_timing_reset(cp_ctx* ctx);
 /* kernel: "assemble_f64", "assemble_f32", "gemv_f64", "gemm_f32", "resid_f64", ...

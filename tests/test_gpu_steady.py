@@ -209,6 +209,36 @@ def test_block_jacobi_same_solution_fewer_iterations(c1s):
     assert sum(r["iters"] for r in rb) < sum(r["iters"] for r in rj)
 
 
+def check_block_inverses(pg):
+    """Every strip block-Jacobi inverse (device Gauss-Jordan in FP64, stored
+    FP32) against LAPACK's FP64 inverse of the same FP32 block, rounded to
+    FP32: |inv - fl32(inv_ref)| <= 4 ulp of the block's largest entry (the two
+    FP64 inverses differ by rounding only; the FP32 store can then differ by
+    one rounding step)."""
+    A32, _ = pg.matrices(CP_FP32_REFINE)
+    bj = pg.block_jacobi()
+    assert bj is not None
+    ptr, perm, blocks = bj
+    assert sorted(perm.tolist()) == list(range(pg.nb))
+    worst = 0.0
+    for q, inv in enumerate(blocks):
+        idx = perm[ptr[q]:ptr[q + 1]]
+        ref = np.linalg.inv(A32[np.ix_(idx, idx)]).astype(np.float32)
+        scale = np.abs(ref).max()
+        err = np.abs(inv.astype(np.float64) - ref).max() / scale
+        worst = max(worst, err)
+        assert err <= 4 * 2.0 ** -23, (q, len(idx), err)
+    return len(blocks), worst
+
+
+def test_block_jacobi_inverses_match_lapack(c1s, c1):
+    for name, pg in (("c1 small", c1s[1]), ("c1", c1[1])):
+        pg.assemble(CP_FP32_REFINE)
+        say("[gpu] block jacobi inverses %s n=%d" % (name, pg.nb))
+        nb, worst = check_block_inverses(pg)
+        say("[gpu]   %d blocks, max rel err %.2e" % (nb, worst))
+
+
 @pytest.mark.parametrize("shape", ["0,1,8", "0,2,16", "0,4,4", "1,4,8", "2,4,32", "3,4,16", "4,4,32", "3,4,4",
                                    "5,4,4", "5,4,32", "6,4,8", "7,4,4", "8,4,16", "9,4,32"])
 def test_gemm_shapes(ctx, shape, monkeypatch):
