@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <future>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -180,6 +181,10 @@ struct cp_geom {
     DevBuf p_nbr, p_w1, p_w2, p_mode, p_te, p_rphi, p_rv, p_rw, p_simp, p_ctr, p_bez, p_area;
     std::unique_ptr<Bicgstab64> bicg;
     std::unique_ptr<BatchedSolver> batched;
+    // host build of post_host, started at upload so that it runs beside the
+    // assembly and the solve; declared last so that it is joined before the
+    // members it writes are destroyed
+    std::future<void> post_job;
 };
 
 namespace {
@@ -325,6 +330,10 @@ cp_status build_geom(cp_ctx* c, const cp_mesh* m, uint64_t key, cp_geom** out) {
     d.fold_ptr = g->fold_ptr.as<int>();
     d.fold_w = g->fold_w.as<int>();
     d.fold_c = g->fold_c.as<double>();
+    // the post tables depend on the host geometry only (the CHTLS fits are the
+    // longest host step of a new geometry)
+    cp_geom* gp = g.get();
+    gp->post_job = std::async(std::launch::async, [gp] { build_post_tables(gp->host, gp->post_host); });
     *out = g.release();
     return CP_OK;
 }
@@ -339,7 +348,10 @@ void ensure_wake_influence(cp_geom* g) {
 void ensure_post(cp_geom* g) {
     if (g->post_ready) return;
     cp_ctx* c = g->ctx;
-    build_post_tables(g->host, g->post_host);
+    if (g->post_job.valid())
+        g->post_job.get();
+    else
+        build_post_tables(g->host, g->post_host);
     const PostHost& H = g->post_host;
     const int n = H.n;
     upload(c, g->p_nbr, H.nbr);

@@ -3,6 +3,7 @@
 #include <cmath>
 #include <limits>
 #include <map>
+#include <mutex>
 
 #include "panelrec.h"
 #include "post.h"
@@ -38,6 +39,31 @@ void monomials(int q, int N, std::vector<std::vector<int>>& out) {
         }
     }
 }
+
+// The monomial lists of chtls_weights, sorted by total degree with the
+// constant term removed, built once per (order, N) and kept.
+const std::vector<std::vector<int>>& sorted_monomials(int order, int N) {
+    static std::mutex mu;
+    static std::map<std::pair<int, int>, std::vector<std::vector<int>>> cache;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find({order, N});
+    if (it != cache.end()) return it->second;
+    std::vector<std::vector<int>> ms;
+    monomials(order, N, ms);
+    std::stable_sort(ms.begin(), ms.end(), [](const std::vector<int>& a, const std::vector<int>& b) {
+        int sa = 0, sb = 0;
+        for (int v : a) sa += v;
+        for (int v : b) sb += v;
+        return sa < sb;
+    });
+    ms.erase(ms.begin());
+    return cache.emplace(std::make_pair(order, N), std::move(ms)).first->second;
+}
+
+// x^e for the small exponents of the Taylor monomials.  pow returns x^0 = 1
+// and x^1 = x exactly, so answering those two without the call changes no
+// bit of the fit; larger exponents go through pow as before.
+inline double ipow(double x, int e) { return e == 0 ? 1.0 : (e == 1 ? x : std::pow(x, e)); }
 
 int factorial_i(int i) {
     int r = 1;
@@ -77,15 +103,7 @@ void chtls_weights(int N, const double* X0, int nf, const double* Xf, int order,
         for (int k = 0; k < N; k++) s += dXb[i * N + k] * dXb[i * N + k];
         Wb[i] = 1 / std::sqrt(s);
     }
-    std::vector<std::vector<int>> ms;
-    monomials(order, N, ms);
-    std::stable_sort(ms.begin(), ms.end(), [](const std::vector<int>& a, const std::vector<int>& b) {
-        int sa = 0, sb = 0;
-        for (int v : a) sa += v;
-        for (int v : b) sb += v;
-        return sa < sb;
-    });
-    ms.erase(ms.begin());
+    const std::vector<std::vector<int>>& ms = sorted_monomials(order, N);
     t = (int)ms.size();
     std::vector<int> deg(t);
     std::vector<double> Am((size_t)nf * t), Bm((size_t)nb * t);
@@ -99,7 +117,7 @@ void chtls_weights(int N, const double* X0, int nf, const double* Xf, int order,
         deg[k] = dsum;
         for (int i = 0; i < nf; i++) {
             double prod = 1;
-            for (int a = 0; a < N; a++) prod *= std::pow(dXf[i * N + a], ms[k][a]);
+            for (int a = 0; a < N; a++) prod *= ipow(dXf[i * N + a], ms[k][a]);
             Am[i + (size_t)k * nf] = fct * prod;
         }
         for (int i = 0; i < nb; i++) {
@@ -107,7 +125,7 @@ void chtls_weights(int N, const double* X0, int nf, const double* Xf, int order,
             for (int a = 0; a < N; a++) {
                 if (ms[k][a] == 0) continue;
                 double prod = 1;
-                for (int b = 0; b < N; b++) prod *= std::pow(dXb[i * N + b], ms[k][b] - (b == a ? 1 : 0));
+                for (int b = 0; b < N; b++) prod *= ipow(dXb[i * N + b], ms[k][b] - (b == a ? 1 : 0));
                 acc += Vb[i * N + a] * ms[k][a] * prod;
             }
             Bm[i + (size_t)k * nb] = fct * acc;
