@@ -25,7 +25,14 @@ class Mesh(C.Structure):
 
 class GeomInfo(C.Structure):
     _fields_ = [("n_body", C.c_int32), ("n_wake", C.c_int32), ("n_wakelines", C.c_int32),
-                ("n_nodes", C.c_int32), ("lda", C.c_int64)]
+                ("n_nodes", C.c_int32), ("lda", C.c_int64), ("n_wakes", C.c_int32), ("n_stations", C.c_int32)]
+
+
+class CaseFields(C.Structure):
+    _fields_ = [("sigma", C.POINTER(C.c_double)), ("velocity", C.POINTER(C.c_double)),
+                ("potential", C.POINTER(C.c_double)),
+                ("cp", C.POINTER(C.c_double)), ("wake_mu", C.POINTER(C.c_double)),
+                ("wake_potential", C.POINTER(C.c_double)), ("spanwise", C.POINTER(C.c_double))]
 
 
 class Freestream(C.Structure):
@@ -85,6 +92,7 @@ SIGNATURES = {
                            C.POINTER(SolveInfo)]),
     "cp_matvec": (C.c_int, [C.c_void_p, C.c_int, C.c_int32, _d, _d]),
     "cp_solver_report": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int32]),
+    "cp_case_fields_eval": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, _d, C.POINTER(CaseFields)]),
     "cp_block_jacobi_read": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int32),
                                        C.POINTER(C.c_int32), C.POINTER(C.c_float)]),
     "cp_bh_build": (C.c_int, [C.c_void_p, C.c_int32, _d, _d, _d, C.POINTER(BhOpts), C.POINTER(C.c_void_p)]),

@@ -223,6 +223,31 @@ class Geometry:
                        "cp_panel_stretching")
         return out
 
+    def case_fields(self, conditions, mu, mach=0.1, Sref=1.0, bref=1.0, cref=1.0, cg=(0, 0, 0)):
+        """Per-panel and per-station outputs of solved conditions, from their
+        body doublet strengths mu (k x n_body): the arrays the reference writes
+        to surfaceData.vtu, wakeData.vtu and spanwiseData_Wake<i>.csv.  Returns
+        one dict per condition."""
+        k = len(conditions)
+        mu = np.ascontiguousarray(np.asarray(mu, dtype=np.float64).reshape(k, self.nb))
+        info = abi.GeomInfo()
+        self.ctx.check(abi.lib().cp_geometry_info(self.h, C.byref(info)))
+        nwk, nst = info.n_wakes, info.n_stations
+        fs = (abi.Freestream * k)(*[abi.Freestream(*map(float, c)) for c in conditions])
+        o = abi.SolveOpts(mach, Sref, bref, cref, (C.c_double * 3)(*cg), 0.0, 0, 0, 0.0, 0, 0)
+        sig = np.zeros((k, self.nb))
+        vel = np.zeros((k, self.nb, 3))
+        pot = np.zeros((k, self.nb))
+        cp = np.zeros((k, self.nb))
+        wmu = np.zeros((k, max(self.nw, 1)))
+        wpot = np.zeros((k, max(self.nw, 1)))
+        span = np.zeros((k, max(nwk, 1), nst, 3))
+        f = abi.CaseFields(_p(sig), _p(vel), _p(pot), _p(cp), _p(wmu), _p(wpot), _p(span))
+        self.ctx.check(abi.lib().cp_case_fields_eval(self.h, k, fs, C.byref(o), _p(mu), C.byref(f)), "case fields")
+        return [dict(mu=mu[q], sigma=sig[q], potential=pot[q], velocity=vel[q], cp=cp[q],
+                     wake_mu=wmu[q, :self.nw], wake_potential=wpot[q, :self.nw], spanwise=span[q, :nwk])
+                for q in range(k)]
+
     def block_jacobi(self):
         """(ptr, perm, [inverse of each block as a b x b FP32 array]) of the
         mixed-precision solve's strip block-Jacobi preconditioner, or None

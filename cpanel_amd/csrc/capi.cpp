@@ -181,6 +181,9 @@ struct cp_geom {
     DevBuf p_nbr, p_w1, p_w2, p_mode, p_te, p_rphi, p_rv, p_rw, p_simp, p_ctr, p_bez, p_area;
     std::unique_ptr<Bicgstab64> bicg;
     std::unique_ptr<BatchedSolver> batched;
+    // wake fold tables for the wake output fields: U1, L1, U2, L2 and t per
+    // wake panel, wake centres (SoA)
+    DevBuf wf_idx, wf_t, wf_ctr;
     // host build of post_host, started at upload so that it runs beside the
     // assembly and the solve; declared last so that it is joined before the
     // members it writes are destroyed
@@ -578,6 +581,8 @@ cp_status cp_geometry_info(const cp_geom* g, cp_geom_info* info) {
     info->n_wakelines = nl;
     info->n_nodes = g->host.n_nodes;
     info->lda = g->dev.lda;
+    info->n_wakes = (int32_t)g->host.wakes.size();
+    info->n_stations = 165;  // wake::trefftzPlane: 164 intervals (wake.cpp:148-156)
     return CP_OK;
 }
 
@@ -913,6 +918,109 @@ cp_status cp_solver_report(cp_geom* g, char* buf, int32_t len) {
          std::to_string(g->near_nnz) + "}";
     std::snprintf(buf, (size_t)len, "%s", r.c_str());
     return (int32_t)r.size() < len ? CP_OK : CP_BAD_ARG;
+}
+
+cp_status cp_case_fields_eval(cp_geom* g, int32_t k, const cp_freestream* fs, const cp_solve_opts* opts,
+                              const double* mu, cp_case_fields* out) {
+    if (!g || !fs || !opts || !mu || !out || k <= 0) return CP_BAD_ARG;
+    cp_ctx* c = g->ctx;
+    if (c->device < 0) return fail(c, CP_NO_DEVICE, "host-only context");
+    return guarded_g(g, [&]() -> cp_status {
+        ensure_post(g);
+        const int n = g->dev.n, nw = g->dev.nw;
+        const int64_t lda = g->dev.lda;
+        hipStream_t s = g->stream;
+        std::vector<double> vinf4(4 * (size_t)k);
+        for (int q = 0; q < k; q++) {
+            double T[9], Vb[3];
+            wind_to_body(fs[q], T, Vb);
+            for (int a = 0; a < 3; a++) vinf4[4 * q + a] = Vb[a];
+            vinf4[4 * q + 3] = fs[q].V;
+        }
+        DevBuf dv4, dmu, dpart, dres, dvel, dphi, dcp, dspan, dwmu, dwphi;
+        upload(c, dv4, vinf4);
+        dalloc(c, dmu, (size_t)lda * k * sizeof(double));
+        CP_HIP(hipMemcpy2DAsync(dmu.p, lda * sizeof(double), mu, n * sizeof(double), n * sizeof(double), k,
+                                hipMemcpyHostToDevice, s));
+        const int nblk = post_partial_blocks(n);
+        dalloc(c, dpart, (size_t)nblk * k * 6 * sizeof(double));
+        dalloc(c, dres, (size_t)16 * k * sizeof(double));
+        const PostHost& H = g->post_host;
+        const size_t nspan = (size_t)H.n_wakes * H.n_st;
+        PostFields F;
+        if (out->velocity) {
+            dalloc(c, dvel, (size_t)3 * n * k * sizeof(double));
+            F.vel = dvel.as<double>();
+            F.ldvel = 3 * (int64_t)n;
+        }
+        if (out->potential) {
+            dalloc(c, dphi, (size_t)n * k * sizeof(double));
+            F.phi = dphi.as<double>();
+            F.ldphi = n;
+        }
+        if (out->cp) dalloc(c, dcp, (size_t)n * k * sizeof(double));
+        DevBuf dsig;
+        if (out->sigma) {
+            dalloc(c, dsig, (size_t)lda * k * sizeof(double));
+            for (int q = 0; q < k; q++)
+                launch_sigma(n, lda, g->normals.as<double>(), dv4.as<double>() + 4 * q, dsig.as<double>() + lda * q, s);
+        }
+        if (out->spanwise && nspan) {
+            dalloc(c, dspan, nspan * 2 * k * sizeof(double));
+            F.span = dspan.as<double>();
+        }
+        PostParams pp;
+        pp.PG = std::sqrt(1 - opts->mach * opts->mach);
+        pp.Sref = opts->Sref;
+        pp.bref = opts->bref;
+        pp.cref = opts->cref;
+        for (int a = 0; a < 3; a++) pp.cg[a] = opts->cg[a];
+        launch_post(g->post, pp, k, dmu.as<double>(), lda, dv4.as<double>(), out->cp ? dcp.as<double>() : nullptr, n,
+                    dpart.as<double>(), nblk, dres.as<double>(), s, &F);
+        if ((out->wake_mu || out->wake_potential) && nw > 0) {
+            if (!g->wf_idx.p) {
+                const Geometry& G = g->host;
+                std::vector<int32_t> idx(G.fold_idx.begin(), G.fold_idx.end());
+                std::vector<double> ctr(3 * (size_t)nw);
+                for (int j = 0; j < nw; j++) {
+                    ctr[j] = G.wake[j].center.x;
+                    ctr[nw + j] = G.wake[j].center.y;
+                    ctr[2 * (size_t)nw + j] = G.wake[j].center.z;
+                }
+                upload(c, g->wf_idx, idx);
+                upload(c, g->wf_t, G.fold_coef);
+                upload(c, g->wf_ctr, ctr);
+            }
+            if (out->wake_mu) dalloc(c, dwmu, (size_t)nw * k * sizeof(double));
+            if (out->wake_potential) dalloc(c, dwphi, (size_t)nw * k * sizeof(double));
+            launch_post_wake(nw, g->wf_idx.as<int32_t>(), g->wf_t.as<double>(), g->wf_ctr.as<double>(), k,
+                             dmu.as<double>(), lda, dv4.as<double>(), out->wake_mu ? dwmu.as<double>() : nullptr,
+                             out->wake_potential ? dwphi.as<double>() : nullptr, s);
+        }
+        auto d2h = [&](double* dst, const DevBuf& src, size_t count) {
+            if (dst && src.p) CP_HIP(hipMemcpyAsync(dst, src.p, count * sizeof(double), hipMemcpyDeviceToHost, s));
+        };
+        if (out->sigma)
+            CP_HIP(hipMemcpy2DAsync(out->sigma, n * sizeof(double), dsig.p, lda * sizeof(double), n * sizeof(double), k,
+                                    hipMemcpyDeviceToHost, s));
+        d2h(out->velocity, dvel, (size_t)3 * n * k);
+        d2h(out->potential, dphi, (size_t)n * k);
+        d2h(out->cp, dcp, (size_t)n * k);
+        d2h(out->wake_mu, dwmu, (size_t)nw * k);
+        d2h(out->wake_potential, dwphi, (size_t)nw * k);
+        std::vector<double> span(nspan * 2 * k);
+        if (out->spanwise && nspan) d2h(span.data(), dspan, span.size());
+        CP_HIP(hipStreamSynchronize(s));
+        if (out->spanwise)
+            for (int q = 0; q < k; q++)
+                for (size_t e = 0; e < nspan; e++) {
+                    double* o = out->spanwise + ((size_t)q * nspan + e) * 3;
+                    o[0] = H.yloc[e];
+                    o[1] = span[((size_t)q * nspan + e) * 2];
+                    o[2] = span[((size_t)q * nspan + e) * 2 + 1];
+                }
+        return CP_OK;
+    });
 }
 
 cp_status cp_block_jacobi_read(cp_geom* g, int64_t* sizes, int32_t* ptr, int32_t* perm, float* inv) {

@@ -66,12 +66,31 @@ struct PostParams {
     double PG, Sref, bref, cref, cg[3];
 };
 
+// Optional outputs of launch_post (device pointers, all may be null):
+//   vel   3 per panel (x, y, z; V_x divided by the PG factor), condition c at
+//         vel + c*ldvel;
+//   phi   potential Vinf . c_i - mu_i, condition c at phi + c*ldphi;
+//   span  per condition, wake and Trefftz station: (Cl / PG, Cdi / PG^2),
+//         at span + ((c*n_wakes + w)*n_st + st)*2.
+struct PostFields {
+    double* vel = nullptr;
+    int64_t ldvel = 0;
+    double* phi = nullptr;
+    int64_t ldphi = 0;
+    double* span = nullptr;
+};
+
 // mu: device, column c at mu + c*ldmu (n_body rows); Vinf3: device 4*k
 // (Vx, Vy, Vz, |V|).  Outputs: cp (device, n*k, may be null), res (device,
 // 16*k: CL, CDi, Fbody[3], Fwind[3] filled on host, CM[3]).
 void launch_post(const PostDev& P, const PostParams& pp, int k, const double* mu, int64_t ldmu,
                  const double* vinf4, double* cp, int64_t ldcp, double* partials, int npart_blocks,
-                 double* res, hipStream_t s);
+                 double* res, hipStream_t s, const PostFields* fields = nullptr);
+// Wake doublet strengths (Kutta fold of the body mu: fidx 4 body indices and
+// ft the interpolation coefficient t per wake panel) and wake potentials
+// (wctr: wake centres SoA, 3*nw) for k conditions; outputs nw per condition.
+void launch_post_wake(int nw, const int32_t* fidx, const double* ft, const double* wctr, int k, const double* mu,
+                      int64_t ldmu, const double* vinf4, double* wmu, double* wphi, hipStream_t s);
 int post_partial_blocks(int n);
 
 }  // namespace cpgpu

@@ -34,7 +34,8 @@ __global__ __launch_bounds__(256) void post_panel_kernel(
     int n, const int32_t* __restrict__ nbr, const double* __restrict__ w1, const double* __restrict__ w2,
     const int32_t* __restrict__ mode, const double* __restrict__ ctr, const double* __restrict__ bez,
     const double* __restrict__ area, PostParams pp, const double* __restrict__ mu, int64_t ldmu,
-    const double* __restrict__ vinf4, double* __restrict__ cp, int64_t ldcp, double* __restrict__ partials) {
+    const double* __restrict__ vinf4, double* __restrict__ cp, int64_t ldcp, double* __restrict__ partials,
+    double* __restrict__ vel, int64_t ldvel, double* __restrict__ phi, int64_t ldphi) {
     __shared__ double sh[kT];
     const int c = blockIdx.y;
     const int i = blockIdx.x * kT + threadIdx.x;
@@ -80,6 +81,13 @@ __global__ __launch_bounds__(256) void post_panel_kernel(
             v2 = q0 * by - q1 * bx;
         }
         v0 /= pp.PG;
+        if (vel) {
+            double* vo = vel + (int64_t)c * ldvel + 3 * (int64_t)i;
+            vo[0] = v0;
+            vo[1] = v1;
+            vo[2] = v2;
+        }
+        if (phi) phi[(int64_t)c * ldphi + i] = phi_i;
         double vn = sqrt(v0 * v0 + v1 * v1 + v2 * v2) / Vm;
         double Cp = 1 - vn * vn;
         if (cp) cp[(int64_t)c * ldcp + i] = Cp;
@@ -104,7 +112,8 @@ __global__ __launch_bounds__(256) void post_final_kernel(
     int nblk, const double* __restrict__ partials, int n_wakes, int n_st, int n_te,
     const int32_t* __restrict__ te_idx, const double* __restrict__ rphi, const double* __restrict__ rv,
     const double* __restrict__ rw, const double* __restrict__ simpson, PostParams pp,
-    const double* __restrict__ mu, int64_t ldmu, const double* __restrict__ vinf4, double* __restrict__ res) {
+    const double* __restrict__ mu, int64_t ldmu, const double* __restrict__ vinf4, double* __restrict__ res,
+    double* __restrict__ span) {
     __shared__ double sh[kT];
     const int c = blockIdx.x;
     const double* m = mu + (int64_t)c * ldmu;
@@ -132,6 +141,11 @@ __global__ __launch_bounds__(256) void post_final_kernel(
             double Cd = dphi * sqrt(v * v + ww * ww) / (Vm * Vm * pp.Sref);
             cl += sw * Cl;
             cd += sw * Cd;
+            if (span) {
+                double* so = span + (((int64_t)c * n_wakes + w) * n_st + st) * 2;
+                so[0] = Cl / pp.PG;
+                so[1] = Cd / (pp.PG * pp.PG);
+            }
         }
         cl = block_sum_post(cl, sh);
         cd = block_sum_post(cd, sh);
@@ -151,19 +165,48 @@ __global__ __launch_bounds__(256) void post_final_kernel(
     }
 }
 
+// wakePanel::setMu (wakePanel.cpp:94-100) and panel::setPotential
+// (panel.cpp:106-111) for every wake panel and condition
+__global__ __launch_bounds__(256) void post_wake_kernel(int nw, const int32_t* __restrict__ fidx,
+                                                        const double* __restrict__ ft,
+                                                        const double* __restrict__ wctr, const double* __restrict__ mu,
+                                                        int64_t ldmu, const double* __restrict__ vinf4,
+                                                        double* __restrict__ wmu, double* __restrict__ wphi) {
+    const int c = blockIdx.y;
+    const int j = blockIdx.x * kT + threadIdx.x;
+    if (j >= nw) return;
+    const double* m = mu + (int64_t)c * ldmu;
+    const double t = ft[j];
+    const int32_t* q = fidx + 4 * (int64_t)j;
+    const double u = (1 - t) * m[q[0]] + (t - 1) * m[q[1]] + t * m[q[2]] - t * m[q[3]];
+    const double Vx = vinf4[4 * c], Vy = vinf4[4 * c + 1], Vz = vinf4[4 * c + 2];
+    if (wmu) wmu[(int64_t)c * nw + j] = u;
+    if (wphi) wphi[(int64_t)c * nw + j] = Vx * wctr[j] + Vy * wctr[nw + j] + Vz * wctr[2 * (int64_t)nw + j] - u;
+}
+
 }  // namespace
+
+void launch_post_wake(int nw, const int32_t* fidx, const double* ft, const double* wctr, int k, const double* mu,
+                      int64_t ldmu, const double* vinf4, double* wmu, double* wphi, hipStream_t s) {
+    if (nw <= 0 || k <= 0) return;
+    dim3 g((unsigned)((nw + kT - 1) / kT), (unsigned)k);
+    hipLaunchKernelGGL(post_wake_kernel, g, dim3(kT), 0, s, nw, fidx, ft, wctr, mu, ldmu, vinf4, wmu, wphi);
+    CP_LAUNCH_CHECK();
+}
 
 int post_partial_blocks(int n) { return (n + kT - 1) / kT; }
 
 void launch_post(const PostDev& P, const PostParams& pp, int k, const double* mu, int64_t ldmu,
                  const double* vinf4, double* cp, int64_t ldcp, double* partials, int nblk, double* res,
-                 hipStream_t s) {
+                 hipStream_t s, const PostFields* f) {
+    const PostFields none;
+    if (!f) f = &none;
     dim3 g1((unsigned)nblk, (unsigned)k);
     hipLaunchKernelGGL(post_panel_kernel, g1, dim3(kT), 0, s, P.n, P.nbr, P.w1, P.w2, P.mode, P.ctr, P.bez, P.area,
-                       pp, mu, ldmu, vinf4, cp, ldcp, partials);
+                       pp, mu, ldmu, vinf4, cp, ldcp, partials, f->vel, f->ldvel, f->phi, f->ldphi);
     CP_LAUNCH_CHECK();
     hipLaunchKernelGGL(post_final_kernel, dim3((unsigned)k), dim3(kT), 0, s, nblk, partials, P.n_wakes, P.n_st,
-                       P.n_te, P.te_idx, P.rphi, P.rv, P.rw, P.simpson, pp, mu, ldmu, vinf4, res);
+                       P.n_te, P.te_idx, P.rphi, P.rv, P.rw, P.simpson, pp, mu, ldmu, vinf4, res, f->span);
     CP_LAUNCH_CHECK();
 }
 

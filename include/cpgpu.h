@@ -82,6 +82,8 @@ typedef struct {
     int32_t n_wakelines;
     int32_t n_nodes;
     int64_t lda;            /* device leading dimension of A, B */
+    int32_t n_wakes;        /* wakes (one Trefftz plane and spanwise file each) */
+    int32_t n_stations;     /* Trefftz stations per wake (165) */
 } cp_geom_info;
 
 cp_status cp_geometry_upload(cp_ctx* ctx, const cp_mesh* mesh, uint64_t key, cp_geom** out);
@@ -253,6 +255,29 @@ void cp_server_destroy(cp_server* s);
 /* JSON text describing the batched solver of a geometry: tuned GEMM shape
  * and the autotuner's timings, block-Jacobi blocks, near-field entries. */
 cp_status cp_solver_report(cp_geom* g, char* buf, int32_t len);
+
+/* ---- per-case output fields ----------------------------------------------
+ * The arrays the reference writes for every case (cpCase::writeFiles,
+ * cpCase.cpp:667-709): surfaceData.vtu cell data (writeBodyData, :711-758),
+ * wakeData.vtu (writeWakeData, :883-908) and spanwiseData_Wake<i>.csv
+ * (writeSpanwiseData, :910-935).  Every pointer may be NULL; arrays are
+ * condition-major (condition q's block follows condition q-1's). */
+typedef struct {
+    double* sigma;          /* k x n_body: source strengths -Vinf . n_i */
+    double* velocity;       /* k x n_body x 3: panel velocity (V_x / PG) */
+    double* potential;      /* k x n_body: Vinf . c_i - mu_i */
+    double* cp;             /* k x n_body */
+    double* wake_mu;        /* k x n_wake: wakePanel::setMu */
+    double* wake_potential; /* k x n_wake */
+    double* spanwise;       /* k x n_wakes x n_stations x 3: y, Cl / PG,
+                               Cdi / PG^2 (wake::trefftzPlane, wake.cpp:146-188) */
+} cp_case_fields;
+
+/* Evaluates the fields of k conditions on the device from their body doublet
+ * strengths mu (k x n_body, as cp_solve returns them); opts supplies the
+ * Mach number and reference quantities (tolerances are not used). */
+cp_status cp_case_fields_eval(cp_geom* g, int32_t k, const cp_freestream* fs, const cp_solve_opts* opts,
+                              const double* mu, cp_case_fields* out);
 
 /* Diagnostic, no counterpart in the reference: the strip block-Jacobi
  * preconditioner of the mixed-precision solve (built on first use after

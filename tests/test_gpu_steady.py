@@ -149,6 +149,61 @@ def test_4412_fixture_on_gpu(ctx, golden, fixtures_json):
     assert np.all(np.abs(r["cp"] - s["cp"]) <= 5e-6 * np.maximum(np.abs(s["cp"]), 0.1))
 
 
+def close6(a, b):
+    """Agreement to the 6 significant digits the reference's files carry."""
+    a, b = np.asarray(a), np.asarray(b)
+    return bool(np.all(np.abs(a - b) <= 5e-6 * np.maximum(np.abs(b), 1e-1) + 1e-12))
+
+
+def test_4412_case_files_on_gpu(ctx, golden, tmp_path):
+    """surfaceData.vtu, wakeData.vtu and spanwiseData_Wake1.csv of the 4412
+    case, evaluated on the device and written in the reference's formats,
+    against the arrays of the reference's own files for that case (6
+    significant digits)."""
+    from cpanel_amd import casefiles, meshgen
+    say("[gpu] 4412 case files")
+    mesh = meshgen.read_tri(os.path.join(golden, "4412.tri"))
+    g = ctx.geometry(mesh)
+    g.assemble(CP_FP64)
+    cond = (1.0, 5.0, 0.0)
+    r = g.solve([cond], CP_FP64, tol=1e-12, **P4412)[0]
+    f = g.case_fields([cond], r["mu"][None, :], **P4412)[0]
+    s = np.load(os.path.join(golden, "4412_surface.npz"))
+    assert close6(f["mu"], s["mu"])
+    assert close6(f["sigma"], s["sigma"])
+    assert close6(f["potential"], s["phi"])
+    assert close6(f["velocity"], s["vel"])
+    assert close6(f["cp"], s["cp"])
+    assert np.array_equal(f["cp"], r["cp"])  # the same post kernel as the solve
+    assert close6(f["wake_mu"], np.load(os.path.join(golden, "4412_wake_mu.npy")))
+    sp = np.load(os.path.join(golden, "4412_spanwise.npy"))
+    assert f["spanwise"].shape == (1, 165, 3)
+    assert np.all(np.abs(2 * f["spanwise"][0, :, 0] / P4412["bref"] - sp[:, 0]) <= 5e-6)
+    assert close6(f["spanwise"][0, :, 1], sp[:, 1])
+    assert close6(f["spanwise"][0, :, 2], sp[:, 2])
+    # the Simpson sum of the station values is the case's CL
+    PG = np.sqrt(1 - P4412["mach"] ** 2)
+    y = f["spanwise"][0, :, 0]
+    h = y[1] - y[0]
+    cl = f["spanwise"][0, :, 1]
+    assert abs(h / 3 * (cl[0] + 4 * cl[1:-1:2].sum() + 2 * cl[2:-1:2].sum() + cl[-1]) - r["CL"]) <= 1e-9
+    # the files, read back
+    d = casefiles.write_case(str(tmp_path), g, mesh, cond, P4412["mach"], f, P4412["bref"])
+    assert os.path.basename(d) == "V1_Mach0.1_alpha5_beta0"
+    v = casefiles.read_vtu_arrays(os.path.join(d, "surfaceData.vtu"))
+    assert list(v)[:9] == ["Doublet Strengths", "Source Strengths", "Velocity Potential", "Velocity", "Cp",
+                           "bezNormals", "xPosition", "yPosition", "zPostition"]
+    assert close6(v["Doublet Strengths"][:, 0], s["mu"])
+    assert close6(v["Velocity"], s["vel"])
+    assert close6(v["Cp"][:, 0], s["cp"])
+    assert close6(np.column_stack([v["xPosition"][:, 0], v["yPosition"][:, 0], v["zPostition"][:, 0]]), s["center"])
+    w = casefiles.read_vtu_arrays(os.path.join(d, "wakeData.vtu"))
+    assert close6(w["Doublet Strengths"][:, 0], np.load(os.path.join(golden, "4412_wake_mu.npy")))
+    csv = np.loadtxt(os.path.join(d, "spanwiseData_Wake1.csv"), delimiter=",", skiprows=1)
+    assert close6(csv[:, 1], sp[:, 1]) and close6(csv[:, 2], sp[:, 2])
+    g.release()
+
+
 def test_mixed_precision_against_fp64(c1s):
     og, pg, A, B = c1s
     pg.assemble(CP_FP64)
