@@ -9,6 +9,11 @@ writes small fixtures:
   4412AR6_surface.npz               - per-panel arrays at alpha 5, beta 0.5 (see fixtures.json)
   fixtures.json                     - .CPout numbers and the kernel known-answer table
   4412AR6_results.txt               - the RESULTS section of 4412AR6.CPout (output text format)
+  steadyTest.tri                    - vortex-particle case mesh (copied data file)
+  steadyTest_vp.npz                 - per-step arrays of that case (surfaceData-k, wakeData-k,
+                                      bufferWake2Data-k, particleData-k .vtu; FP32, 6 digits):
+                                      buffer-wake cell vertices, body mu/sigma, wake rows' mu,
+                                      particle positions/strengths/shed steps
 
 Run: python tests/golden/make_golden.py [/root/reference]
 """
@@ -34,7 +39,37 @@ def vtu_arrays(path, names):
     return out
 
 
+VP_STEPS = [1, 2, 3, 4, 5, 6, 8, 10, 20, 40]
+
+
+def steady_test_vp():
+    d = os.path.join(CASES, "steadyTest")
+    shutil.copy(os.path.join(d, "steadyTest.tri"), os.path.join(OUT, "steadyTest.tri"))
+    r = os.path.join(d, "V1_Mach0.1_alpha10_beta0")
+    out = {}
+    for row, fname in (("w1", "wakeData-1.vtu"), ("w2", "bufferWake2Data-1.vtu")):
+        a = vtu_arrays(os.path.join(r, fname), ["Position", "connectivity"])
+        pts = a["Position"].reshape(-1, 3)
+        con = a["connectivity"].astype(int).reshape(-1, 4)
+        out[row + "_cells"] = pts[con].astype(np.float32)
+    for k in VP_STEPS:
+        s = vtu_arrays(os.path.join(r, "surfaceData-%d.vtu" % k), ["Doublet Strengths", "Source Strengths"])
+        out["mu_%d" % k] = s["Doublet Strengths"].astype(np.float32)
+        out["sigma_%d" % k] = s["Source Strengths"].astype(np.float32)
+        out["w1mu_%d" % k] = vtu_arrays(os.path.join(r, "wakeData-%d.vtu" % k),
+                                        ["Doublet Strengths"])["Doublet Strengths"].astype(np.float32)
+        out["w2mu_%d" % k] = vtu_arrays(os.path.join(r, "bufferWake2Data-%d.vtu" % k),
+                                        ["Doublet Strengths"])["Doublet Strengths"].astype(np.float32)
+        if k >= 3:
+            p = vtu_arrays(os.path.join(r, "particleData-%d.vtu" % k), ["Strength", "Time Step Shed", "Position"])
+            out["ppos_%d" % k] = p["Position"].reshape(-1, 3).astype(np.float32)
+            out["pstr_%d" % k] = p["Strength"].reshape(-1, 3).astype(np.float32)
+            out["pshed_%d" % k] = p["Time Step Shed"].astype(np.int16)
+    np.savez_compressed(os.path.join(OUT, "steadyTest_vp.npz"), **out)
+
+
 def main():
+    steady_test_vp()
     shutil.copy(os.path.join(CASES, "4412", "4412.tri"), os.path.join(OUT, "4412.tri"))
     shutil.copy(os.path.join(CASES, "4412AR6", "4412AR6.tri"), os.path.join(OUT, "4412AR6.tri"))
     txt = open(os.path.join(CASES, "4412AR6", "4412AR6.CPout")).read()
@@ -76,6 +111,15 @@ def main():
             "Cn_alpha": 0.01123, "CL_beta": -0.00297, "CY_beta": 0.03141, "Cm_beta": 0.00068,
             "Cl_beta": 0.04724, "Cn_beta": -0.00311,
             "vtu_state": "alpha 5, beta 0.5 (stability-derivative run overwrites panel state, cpCase.cpp:635-665)",
+            "print_decimals": 5,
+        },
+        "steadyTest": {
+            "source": "Connor Cases/steadyTest/steadyTest.CPin, steadyTest.CPout (vortex-particle wake)",
+            "params": {"V": 1, "alpha": 10, "beta": 0, "mach": 0.1, "Sref": 6, "bref": 6, "cref": 1,
+                       "cg": [0.416, 0, 0.03], "steps": 40, "time_step": 0, "accelerate": 1},
+            "steps_stored": VP_STEPS,
+            "CL": 1.10020, "CDi": 0.06869, "CN": 1.11715, "CA": -0.10571, "CY": 0.00034,
+            "CL_surface": 1.11853, "CD_surface": 0.08989, "Cm": 0.11546, "Cl": -0.00117, "Cn": -0.00052,
             "print_decimals": 5,
         },
         "kat_triangle": {
