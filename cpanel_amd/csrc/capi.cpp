@@ -4,86 +4,13 @@
 //   cp_ctx  - one GPU: device id, stream, HBM budget accounting, timers.
 //   cp_geom - one mesh: host topology, device panel tables, resident A/B in
 //             FP64 and/or FP32, post-processing tables, solver workspaces.
-#include <hip/hip_runtime.h>
-
-#include <algorithm>
-#include <cmath>
-#include <cstring>
-#include <future>
-#include <map>
-#include <memory>
-#include <mutex>
-#include <new>
-#include <string>
-#include <vector>
-
-#include "../../include/cpgpu.h"
-#include "batched.h"
-#include "device.h"
-#include "geometry.h"
-#include "krylov.h"
-#include "panelrec.h"
-#include "post.h"
-#include "precond.h"
-#include "timer.h"
-#include "vp.h"
-#include "vp_tree.h"
+#include "capi_internal.h"
 
 using namespace cpgpu;
+using namespace cpgpu::capi;
 
-struct cp_ctx {
-    int device = 0;
-    uint64_t budget = 0;
-    uint64_t in_use = 0;          // device bytes held (live + pooled), under alloc_mu
-    hipStream_t stream = nullptr; // context-level calls (Barnes-Hut trees)
-    std::string err;              // last error, under err_mu
-    int hip_err = 0;
-    KernelTimer timer;
-    std::mutex mu;                // serialises context-level calls
-    std::mutex alloc_mu;
-    std::mutex err_mu;
-    // freed device blocks kept for reuse: a hipFree waits for the whole
-    // device, which would stall the other geometries' streams
-    std::multimap<size_t, void*> pool;
-    uint64_t pooled = 0;
-    // pinned host blocks of BatchedSolver::host_bytes() and idle
-    // non-blocking streams of released geometries, reused for the same
-    // reason (hipHostFree and stream teardown synchronise)
-    std::vector<void*> pinned;
-    std::vector<hipStream_t> streams;
-};
-
-namespace {
-
-struct DevBuf {
-    cp_ctx* ctx = nullptr;
-    void* p = nullptr;
-    size_t bytes = 0;
-    DevBuf() = default;
-    DevBuf(const DevBuf&) = delete;
-    DevBuf& operator=(const DevBuf&) = delete;
-    ~DevBuf() { release(); }
-    // returns the block to the context's pool; callers release only buffers
-    // whose last use on the device has completed (every API call
-    // synchronises its stream before its temporaries go)
-    void release() {
-        if (p) {
-            std::lock_guard<std::mutex> lk(ctx->alloc_mu);
-            ctx->pool.emplace(bytes, p);
-            ctx->pooled += bytes;
-            p = nullptr;
-            bytes = 0;
-        }
-    }
-    template <class T>
-    T* as() const {
-        return reinterpret_cast<T*>(p);
-    }
-};
-
-struct BudgetError {
-    uint64_t need;
-};
+namespace cpgpu {
+namespace capi {
 
 // frees pooled blocks (largest first) until `need` more bytes fit the budget
 void trim_pool(cp_ctx* c, uint64_t need) {
@@ -140,57 +67,7 @@ bool fits(cp_ctx* c, uint64_t bytes) {
     return !c->budget || c->in_use - c->pooled + bytes <= c->budget;
 }
 
-template <class T>
-void upload(cp_ctx* c, DevBuf& b, const std::vector<T>& v) {
-    dalloc(c, b, std::max<size_t>(v.size(), 1) * sizeof(T));
-    if (!v.empty()) CP_HIP(hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
-}
 
-}  // namespace
-
-struct cp_geom {
-    cp_ctx* ctx = nullptr;
-    uint64_t key = 0;
-    hipStream_t stream = nullptr;  // every device call on this geometry
-    std::mutex mu;                 // serialises calls on this geometry
-    int64_t solver_bytes = 0;      // workspaces counted in ctx->in_use
-    DevBuf solver_ws;              // BatchedSolver workspace
-    void* solver_host = nullptr;   // BatchedSolver pinned host block (ctx pool)
-    Geometry host;
-    DevGeom dev;
-    DevBuf body, wake, cxyz, fold_ptr, fold_w, fold_c, normals;
-    DevBuf W;          // wake influence N x Nw (FP64)
-    DevBuf A64, B64, A32, B32;
-    DevBuf A16;        // bf16 hi/lo tiles of A32 (bf16x3 inner products)
-    // second buffer-wake row of the vortex-particle case: panels and C
-    DevBuf w2, Cw;
-    int nw2 = 0;
-    bool have64 = false, have32 = false;
-    // FP64 near-field corrections of A32/B32 (entries within one panel
-    // diameter: |c_i - c_j| <= longSide_j)
-    DevBuf near_rowptr, near_rows, near_cols, near_dA, near_dB;
-    int near_nnz = 0;
-    // strip block-Jacobi preconditioner of the batched FP32 inner solve
-    BlockPlan bj_plan;
-    DevBuf bj_perm, bj_ptr, bj_off, bj_inv, bj_iperm;
-    bool bj_ready = false;
-    // post tables
-    bool post_ready = false;
-    PostHost post_host;
-    PostDev post;
-    DevBuf p_nbr, p_w1, p_w2, p_mode, p_te, p_rphi, p_rv, p_rw, p_simp, p_ctr, p_bez, p_area;
-    std::unique_ptr<Bicgstab64> bicg;
-    std::unique_ptr<BatchedSolver> batched;
-    // wake fold tables for the wake output fields: U1, L1, U2, L2 and t per
-    // wake panel, wake centres (SoA)
-    DevBuf wf_idx, wf_t, wf_ctr;
-    // host build of post_host, started at upload so that it runs beside the
-    // assembly and the solve; declared last so that it is joined before the
-    // members it writes are destroyed
-    std::future<void> post_job;
-};
-
-namespace {
 
 cp_status fail(cp_ctx* c, cp_status s, const std::string& msg) {
     if (c) {
@@ -205,45 +82,6 @@ void set_err(cp_ctx* c, const std::string& msg) {
     c->err = msg;
 }
 
-// runs f with HIP and budget errors turned into cp_status; `lock` (if any)
-// is held for the call
-template <class F>
-cp_status guarded_impl(cp_ctx* c, std::mutex* lock, F&& f) {
-    if (!c) return CP_BAD_ARG;
-    std::unique_lock<std::mutex> lk;
-    if (lock) lk = std::unique_lock<std::mutex>(*lock);
-    set_current_timer(&c->timer);
-    try {
-        if (c->device >= 0) CP_HIP(hipSetDevice(c->device));
-        cp_status s = f();
-        set_current_timer(nullptr);
-        return s;
-    } catch (const HipError& e) {
-        set_current_timer(nullptr);
-        c->hip_err = (int)e.code;
-        set_err(c, std::string("HIP error ") + hipGetErrorString(e.code) + " in " + e.where);
-        return CP_HIP_ERROR;
-    } catch (const BudgetError& e) {
-        set_current_timer(nullptr);
-        set_err(c, "HBM budget exceeded: need " + std::to_string(e.need) + " more bytes, " +
-                       std::to_string(c->in_use) + " of " + std::to_string(c->budget) + " in use");
-        return CP_OUT_OF_BUDGET;
-    } catch (const std::bad_alloc&) {
-        set_current_timer(nullptr);
-        set_err(c, "host allocation failed");
-        return CP_OUT_OF_BUDGET;
-    }
-}
-
-template <class F>
-cp_status guarded(cp_ctx* c, F&& f) {
-    return guarded_impl(c, c ? &c->mu : nullptr, std::forward<F>(f));
-}
-
-template <class F>
-cp_status guarded_g(cp_geom* g, F&& f) {
-    return guarded_impl(g->ctx, &g->mu, std::forward<F>(f));
-}
 
 // cpCase::windToBody (cpCase.cpp:142-166): body-axis freestream and the
 // wind<-body transform T (row major)
@@ -263,12 +101,12 @@ void wind_to_body(const cp_freestream& f, double T[9], double Vb[3]) {
     Vb[2] = T[6] * f.V;
 }
 
-cp_status build_geom(cp_ctx* c, const cp_mesh* m, uint64_t key, cp_geom** out) {
+cp_status build_geom(cp_ctx* c, const cp_mesh* m, uint64_t key, cp_geom** out, const VpParams* vp) {
     std::unique_ptr<cp_geom> g(new cp_geom());
     g->ctx = c;
     g->key = key;
     std::string err;
-    if (!build_geometry(m->n_nodes, m->xyz, m->n_tris, m->tri, m->ids, g->host, err))
+    if (!build_geometry(m->n_nodes, m->xyz, m->n_tris, m->tri, m->ids, g->host, err, vp))
         return fail(c, CP_BAD_MESH, err);
     const Geometry& G = g->host;
     const int n = (int)G.body.size(), nw = (int)G.wake.size();
@@ -467,7 +305,9 @@ void fill_info(const double* r, const double T[9], cp_solve_info& inf) {
     for (int a = 0; a < 3; a++) inf.Fwind[a] = T[0 * 3 + a] * r[2] + T[1 * 3 + a] * r[3] + T[2 * 3 + a] * r[4];
 }
 
-}  // namespace
+}  // namespace capi
+}  // namespace cpgpu
+
 
 extern "C" {
 
@@ -1077,13 +917,6 @@ cp_status cp_timing_get(cp_ctx* c, const char* kernel, double* total_ms, int64_t
 }  // extern "C"
 
 // ---------------------------------------------------------------- VP kernels
-struct cp_tree {
-    cp_ctx* ctx = nullptr;
-    BhTreeHost host;
-    DevBhTree dev;
-    DevBuf nodes, mpos, mstr, mrad, order;
-    double theta = 0.5;
-};
 
 cp_status cp_bh_build(cp_ctx* c, int32_t n, const double* xyz, const double* alpha, const double* radius,
                       const cp_bh_opts* opts, cp_tree** out) {

@@ -273,8 +273,210 @@ bool read_tri_file(const std::string& path, std::vector<double>& xyz, std::vecto
     return true;
 }
 
+namespace {
+
+// The two buffer rows of the vortex-particle mode (geometry.cpp:267-397,
+// 1276-1342), appended after the body panels: first-row quads become wake
+// panels of one new wake (their attachment to the trailing-edge edge makes
+// the wake lines and upper/lower flags through te_check, as edge::addWakePan
+// does), second-row quads go to W with the topology the step loop needs.
+bool build_vp_rows(Builder& B, const VpParams& vp, VpBuffer& W, std::vector<int>& vp_wake, std::string& err) {
+    std::vector<int> te;
+    for (int e = 0; e < (int)B.edges.size(); e++)
+        if (B.edges[e].te && B.edges[e].nbody == 2) te.push_back(e);
+    if (te.size() < 2) {
+        err = "vortex-particle mode: fewer than two trailing-edge edges";
+        return false;
+    }
+    double dt = vp.dt;
+    if (!(dt > 0)) {  // geometry::calcTimeStep
+        double sum = 0;
+        for (int e : te) sum += len(sub(B.nodes[B.edges[e].n2], B.nodes[B.edges[e].n1]));
+        double avg = sum / (double)te.size();
+        dt = avg / vp.V;
+    }
+    W = VpBuffer();
+    W.dt = dt;
+    const int nn0 = (int)B.nodes.size();
+    // a node's trailing edges in edge creation order (cpNode::getTrailingEdges)
+    std::vector<std::vector<int>> node_te(nn0);
+    for (int e : te) {
+        node_te[B.edges[e].n1].push_back(e);
+        node_te[B.edges[e].n2].push_back(e);
+    }
+    // cpNode::nodeWakeProjAngle: normals of the trailing edges' body panels,
+    // the running sum divided by 3 after each edge, normalised
+    auto bisector = [&](int n) {
+        Vec3 avg{0, 0, 0};
+        for (int e : node_te[n]) {
+            const EdgeRec& E = B.edges[e];
+            for (int k = 0; k < E.nbody; k++) avg = add(avg, B.pan[E.body[k]].normal);
+            avg = {avg.x / 3.0, avg.y / 3.0, avg.z / 3.0};
+        }
+        return unit(avg);
+    };
+    const double cw = 1.3;  // cpNode::c_w
+    std::vector<int> f1(nn0, -1), f2(nn0, -1);
+    auto project = [&](int n) {  // firstProjNode / secProjNode, first use only
+        if (f1[n] >= 0) return;
+        const Vec3 b = bisector(n), p = B.nodes[n];
+        double s1 = cw * dt;
+        s1 = s1 * vp.V;
+        double s2 = (cw + 1) * dt;
+        s2 = s2 * vp.V;
+        f1[n] = (int)B.nodes.size();
+        B.nodes.push_back({p.x + s1 * b.x, p.y + s1 * b.y, p.z + s1 * b.z});
+        f2[n] = (int)B.nodes.size();
+        B.nodes.push_back({p.x + s2 * b.x, p.y + s2 * b.y, p.z + s2 * b.z});
+    };
+    Builder::WakeBuild wk;
+    B.wk.push_back(wk);
+    const int grp = (int)B.wk.size() - 1;
+    // wake::addPanel bounds over both rows, in creation order
+    auto bounds = [&](const PanelGeom& p) {
+        Builder::WakeBuild& w = B.wk[grp];
+        if (w.panels.empty() && W.panels.empty() && vp_wake.empty()) {
+            Vec3 q = p.v[0];
+            w.yMin = w.yMax = q.y;
+            w.x0 = w.xf = q.x;
+            w.z0 = w.zf = q.z;
+        }
+        for (int k = 0; k < p.nv; k++) {
+            Vec3 q = p.v[k];
+            if (q.y > w.yMax) w.yMax = q.y;
+            else if (q.y < w.yMin) w.yMin = q.y;
+            if (q.x > w.xf) w.xf = q.x;
+            else if (q.x < w.x0) w.x0 = q.x;
+            if (q.z > w.zf) w.zf = q.z;
+            else if (q.z < w.z0) w.z0 = q.z;
+        }
+    };
+    auto quad = [&](std::array<int, 4> q) {
+        PanelGeom p;
+        p.nv = 4;
+        for (int k = 0; k < 4; k++) {
+            p.node[k] = q[k];
+            p.v[k] = B.nodes[q[k]];
+        }
+        setup_panel(p);
+        return p;
+    };
+    std::vector<std::vector<int>> edge_w2;  // second-row panels on each edge
+    std::vector<std::array<int, 4>> w2_edges;
+    for (int e : te) {
+        const int n1 = B.edges[e].n1, n2 = B.edges[e].n2;
+        project(n1);
+        project(n2);
+        const int a1 = f1[n1], a2 = f1[n2], b1 = f2[n1], b2 = f2[n2];
+        // first row (n1, n2, n2', n1')
+        PanelGeom p1 = quad({n1, n2, a2, a1});
+        bounds(p1);
+        const int pi = (int)B.pan.size();
+        B.pan.push_back(p1);
+        B.is_wake.push_back(1);
+        B.group.push_back(grp);
+        B.pedge.push_back({-1, -1, -1});
+        B.upper.push_back(0);
+        B.lower.push_back(0);
+        B.te_panel.push_back(0);
+        B.up_pan.push_back(-1);
+        B.lo_pan.push_back(-1);
+        B.wte.push_back(0);
+        B.nbr.emplace_back();
+        vp_wake.push_back(pi);
+        B.wk[grp].panels.push_back(pi);
+        for (int k = 0; k < 4; k++) {
+            const int ei = B.edge_of(p1.node[k], p1.node[(k + 1) % 4]);
+            EdgeRec& E = B.edges[ei];
+            if (E.nwake == 0) E.wake0 = pi;
+            E.nwake++;
+            B.te_check(ei);
+        }
+        // second row (n1', n2', n2'', n1'')
+        PanelGeom p2 = quad({a1, a2, b2, b1});
+        bounds(p2);
+        const int wi = (int)W.panels.size();
+        W.panels.push_back(p2);
+        W.parent.push_back((int)vp_wake.size() - 1);
+        std::array<int, 4> pe;
+        for (int k = 0; k < 4; k++) {
+            pe[k] = B.edge_of(p2.node[k], p2.node[(k + 1) % 4]);
+            if ((int)edge_w2.size() < (int)B.edges.size()) edge_w2.resize(B.edges.size());
+            edge_w2[pe[k]].push_back(wi);
+            B.edges[pe[k]].nwake++;
+        }
+        w2_edges.push_back(pe);
+    }
+    // second-row topology: wakePanel::edgesInOrder / pointsInOrder,
+    // getOtherWakePan across the side edges, edge first use
+    auto mid = [&](int ei) {
+        const Vec3 a = B.nodes[B.edges[ei].n1], b = B.nodes[B.edges[ei].n2];
+        const Vec3 v = sub(b, a);
+        return add(a, scale(0.5, v));
+    };
+    auto other = [&](int ei, int self) {
+        for (int o : edge_w2[ei])
+            if (o != self) return o;
+        return -1;
+    };
+    std::vector<char> used(B.edges.size(), 0);
+    const int m = (int)W.panels.size();
+    W.pio.resize(m);
+    W.eio_nbr.assign(m, {-1, -1, -1, -1});
+    W.eio_first.assign(m, {0, 0, 0, 0});
+    W.rad_nbr.resize(m);
+    for (int i = 0; i < m; i++) {
+        const PanelGeom& p = W.panels[i];
+        const std::array<int, 4>& pe = w2_edges[i];
+        std::array<int, 4> eio{pe[0], -1, pe[2], -1};
+        const Vec3 a = sub(mid(pe[0]), p.center);
+        const Vec3 c = cross(a, p.normal);
+        const Vec3 d = add(p.center, c);
+        const double d1 = len(sub(mid(pe[1]), d)), d3 = len(sub(mid(pe[3]), d));
+        if (d3 < d1) {
+            eio[3] = pe[3];
+            eio[1] = pe[1];
+        } else {
+            eio[3] = pe[1];
+            eio[1] = pe[3];
+        }
+        auto has = [&](int ei, int n) { return B.edges[ei].n1 == n || B.edges[ei].n2 == n; };
+        int q[4];
+        {
+            const int x1 = B.edges[eio[0]].n1, x2 = B.edges[eio[0]].n2;
+            if (has(eio[3], x1)) {
+                q[0] = x1;
+                q[1] = x2;
+            } else {
+                q[0] = x2;
+                q[1] = x1;
+            }
+            const int y1 = B.edges[eio[2]].n1, y2 = B.edges[eio[2]].n2;
+            if (has(eio[1], y1)) {
+                q[2] = y1;
+                q[3] = y2;
+            } else {
+                q[2] = y2;
+                q[3] = y1;
+            }
+        }
+        for (int k = 0; k < 4; k++) W.pio[i][k] = B.nodes[q[k]];
+        for (int j = 1; j < 4; j++) {
+            if (j != 2) W.eio_nbr[i][j] = other(eio[j], i);
+            if (!used[eio[j]]) {
+                used[eio[j]] = 1;
+                W.eio_first[i][j] = 1;
+            }
+        }
+        W.rad_nbr[i] = {other(pe[1], i), other(pe[3], i)};
+    }
+    return true;
+}
+
+}  // namespace
+
 bool build_geometry(int nn, const double* xyz, int nt, const int32_t* tri_in, const int32_t* ids, Geometry& G,
-                    std::string& err) {
+                    std::string& err, const VpParams* vp) {
     Builder B;
     B.ids = ids;
     B.nodes.resize(nn);
@@ -411,6 +613,16 @@ bool build_geometry(int nn, const double* xyz, int nt, const int32_t* tri_in, co
             B.nbr[e.body[0]].push_back(e.body[1]);
             B.nbr[e.body[1]].push_back(e.body[0]);
         }
+    std::vector<int> vp_wake;  // first-row panels of the vortex-particle mode
+    if (vp) {
+        if (!B.wk.empty()) {
+            err = "vortex-particle mode needs a mesh without prescribed wake panels";
+            return false;
+        }
+        if (!build_vp_rows(B, *vp, G.w2, vp_wake, err)) return false;
+        G.vp = true;
+    }
+    const int np = (int)B.pan.size();  // file triangles plus first-row quads
 
     // --- wake merge (geometry.cpp:424-447): same x0, z0, xf, zf within 1e-2
     std::vector<int> wake_final(B.wk.size(), -1);
@@ -445,7 +657,7 @@ bool build_geometry(int nn, const double* xyz, int nt, const int32_t* tri_in, co
 
     // --- panel order: surfaces prepended, wakes in order
     std::vector<std::vector<int>> surfPanels(B.surf_lifting.size());
-    for (int i = 0; i < nt; i++)
+    for (int i = 0; i < np; i++)
         if (!B.is_wake[i]) surfPanels[B.group[i]].push_back(i);
     std::vector<int> border;
     for (int s = (int)surfPanels.size() - 1; s >= 0; s--)
@@ -457,7 +669,7 @@ bool build_geometry(int nn, const double* xyz, int nt, const int32_t* tri_in, co
         err = "mesh has no body panels";
         return false;
     }
-    std::vector<int> bidx(nt, -1), widx(nt, -1);
+    std::vector<int> bidx(np, -1), widx(np, -1);
     for (int i = 0; i < nb; i++) bidx[border[i]] = i;
     for (int j = 0; j < nw; j++) widx[worder[j]] = j;
     for (auto& w : wakes)
@@ -467,7 +679,7 @@ bool build_geometry(int nn, const double* xyz, int nt, const int32_t* tri_in, co
         }
 
     // --- tip flags (bodyPanel::setTipFlag, bodyPanel.cpp:58-95)
-    std::vector<char> tip(nt, 0);
+    std::vector<char> tip(np, 0);
     for (int fi : border) {
         if (!B.surf_lifting[B.group[fi]]) continue;
         int count = 0;
@@ -536,7 +748,7 @@ bool build_geometry(int nn, const double* xyz, int nt, const int32_t* tri_in, co
     }
 
     // --- outputs
-    G.n_nodes = nn;
+    G.n_nodes = (int)B.nodes.size();
     G.nodes = B.nodes;
     G.body.resize(nb);
     G.body_tri = border;
@@ -560,6 +772,8 @@ bool build_geometry(int nn, const double* xyz, int nt, const int32_t* tri_in, co
     }
     G.wake.resize(nw);
     G.wake_tri = worder;
+    for (int& t : G.wake_tri)
+        if (t >= nt) t = -1;  // first-row quads of the vortex-particle mode
     G.wake_te.resize(nw);
     G.wake_upper.resize(nw);
     for (int j = 0; j < nw; j++) {

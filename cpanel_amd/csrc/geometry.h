@@ -41,6 +41,30 @@ struct WakeInfo {
     double yMin = 0, yMax = 0, x0 = 0, xf = 0, z0 = 0, zf = 0;
 };
 
+// Vortex-particle mode (geometry::readTri with the vortex-particle flag,
+// geometry.cpp:267-397; calcTimeStep :1276-1304; createVPWakeSurfaces
+// :1307-1342): the mesh has no prescribed wake; every trailing-edge edge gets
+// a first-row buffer quad (the wake whose doublets fold into A) and a
+// second-row quad behind it (the columns of C).
+struct VpParams {
+    double V = 1;   // freestream speed of the node projections and of dt
+    double dt = 0;  // time step; <= 0: mean trailing-edge edge length / V
+};
+
+// The second buffer row, in trailing-edge edge order.
+struct VpBuffer {
+    double dt = 0;
+    std::vector<PanelGeom> panels;
+    std::vector<int> parent;                    // first-row (wake) panel
+    std::vector<std::array<Vec3, 4>> pio;       // wakePanel::pointsInOrder
+    // per edge of wakePanel::edgesInOrder (j = 1..3): the second-row panel
+    // across it (-1: none) and whether this panel is the first to use it
+    // in panel order (cpCaseVP::collapseBufferWake counts an edge once)
+    std::vector<std::array<int, 4>> eio_nbr;
+    std::vector<std::array<char, 4>> eio_first;
+    std::vector<std::array<int, 2>> rad_nbr;    // across getEdges()[1], [3]
+};
+
 // Fully resolved geometry of one .tri mesh.
 struct Geometry {
     int n_nodes = 0;
@@ -62,12 +86,16 @@ struct Geometry {
     std::vector<int> wake_upper;
     // per body panel: edges flagged TE (node pairs), up to 3
     std::vector<int> body_te_edges;  // 6 per body panel (n1,n2)*3, -1 if not TE
+    // vortex-particle mode: the wake above is the first buffer row (wake_tri
+    // is -1 for its quads, which no file triangle backs)
+    bool vp = false;
+    VpBuffer w2;
 };
 
 // Builds the geometry from 0-based connectivity.  Returns false with a
 // message for meshes the reference would reject or mis-handle.
 bool build_geometry(int n_nodes, const double* xyz, int n_tris, const int32_t* tri, const int32_t* ids,
-                    Geometry& out, std::string& err);
+                    Geometry& out, std::string& err, const VpParams* vp = nullptr);
 
 bool read_tri_file(const std::string& path, std::vector<double>& xyz, std::vector<int32_t>& tri,
                    std::vector<int32_t>& ids, std::string& err);

@@ -25,6 +25,20 @@ void launch_bh_points(const DevBhTree& T, double theta, int m, const double* pts
 // modes CP_BH_VEL / CP_BH_STRETCH / CP_BH_DIFF on the tree's particles -> out[3n]
 void launch_bh_particles(const DevBhTree& T, int mode, double theta, double* out, hipStream_t s);
 
+// modes CP_BH_VEL / CP_BH_STRETCH / CP_BH_DIFF on m caller-given targets
+// (tpos, tstr: 3 per target; trad; tslot: the target's slot in T or -1)
+void launch_bh_targets(const DevBhTree& T, int mode, double theta, int m, const double* tpos, const double* tstr,
+                       const double* trad, const int* tslot, double* out, hipStream_t s);
+// T's members take the current state of their particles (particle-indexed
+// pos, str: 3 each; rad), keeping the tree's nodes and pseudo-particles
+void launch_bh_refresh(const DevBhTree& T, const double* pos, const double* str, const double* rad, hipStream_t s);
+// direct Gaussian point velocity of n particles at m points
+void launch_gauss_points(int n, const double* pos, const double* str, const double* rad, int m, const double* pts,
+                         double* out, hipStream_t s);
+// vortex-filament velocity (ends: 6 per filament, fstr) at m points
+void launch_filament_points(int nf, const double* ends, const double* fstr, int m, const double* pts, double* out,
+                            hipStream_t s);
+
 // K slices of the panel -> point evaluation for m points and np panels
 int p2p_slices(int m, int npanels);
 // out[3m] = velocity at pts of the body (sigma, mu) and wake (muw) panels;

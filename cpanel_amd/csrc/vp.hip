@@ -189,6 +189,131 @@ __global__ __launch_bounds__(kT) void bh_particles_kernel(const BhNode* __restri
     out[3 * o + 2] = acc[2];
 }
 
+// Barnes-Hut sums on caller-given particle targets.  The vortex-particle
+// step loop evaluates the previous step's tree on the current particles,
+// including those shed since it was built (cpCaseVP.cpp:58-83 rebuild after
+// the strength update): tslot is the target's slot in this tree (its own
+// member is skipped, node.h:272) or -1.
+template <int MODE>
+__global__ __launch_bounds__(kT) void bh_targets_kernel(const BhNode* __restrict__ nodes, int n_nodes,
+                                                        const double* __restrict__ mpos,
+                                                        const double* __restrict__ mstr,
+                                                        const double* __restrict__ mrad, double theta, int m,
+                                                        const double* __restrict__ tpos,
+                                                        const double* __restrict__ tstr,
+                                                        const double* __restrict__ trad,
+                                                        const int* __restrict__ tslot, double* __restrict__ out) {
+    const int t = blockIdx.x * kT + threadIdx.x;
+    if (t >= m) return;
+    const Src tg{tpos[3 * t], tpos[3 * t + 1], tpos[3 * t + 2], tstr[3 * t], tstr[3 * t + 1], tstr[3 * t + 2], trad[t]};
+    const int self = tslot[t];
+    double acc[3] = {0, 0, 0};
+    int i = 1;
+    while (i < n_nodes) {
+        const BhNode nd = nodes[i];
+        if (nd.count >= 0) {
+            for (int k = 0; k < nd.count; k++) {
+                const int ms = nd.first + k;
+                if (ms == self) continue;
+                const Src sp = member(mpos, mstr, mrad, ms);
+                if (MODE == CP_BH_VEL)
+                    k_gauss(sp, tg, acc);
+                else if (MODE == CP_BH_STRETCH)
+                    k_stretch(sp, tg, acc);
+                else
+                    k_diff(sp, tg, acc);
+            }
+            i = nd.skip;
+        } else if (far_cell(nd, tg.x, tg.y, tg.z, theta)) {
+            const Src sp = pseudo(nd);
+            if (MODE == CP_BH_VEL)
+                k_gauss(sp, tg, acc);
+            else if (MODE == CP_BH_STRETCH)
+                k_stretch(sp, tg, acc);
+            else
+                k_diff(sp, tg, acc);
+            i = nd.skip;
+        } else {
+            i = i + 1;
+        }
+    }
+    out[3 * t] = acc[0];
+    out[3 * t + 1] = acc[1];
+    out[3 * t + 2] = acc[2];
+}
+
+// the tree's members take the current state of the particles they hold
+// (the reference's octree holds particle pointers: leaves see the particles
+// as they are now, pseudo-particles stay as they were at the build)
+__global__ __launch_bounds__(kT) void bh_refresh_kernel(int n, const int* __restrict__ order,
+                                                        const double* __restrict__ pos,
+                                                        const double* __restrict__ str,
+                                                        const double* __restrict__ rad, double* __restrict__ mpos,
+                                                        double* __restrict__ mstr, double* __restrict__ mrad) {
+    const int sl = blockIdx.x * kT + threadIdx.x;
+    if (sl >= n) return;
+    const int i = order[sl];
+    for (int q = 0; q < 3; q++) {
+        mpos[3 * sl + q] = pos[3 * i + q];
+        mstr[3 * sl + q] = str[3 * i + q];
+    }
+    mrad[sl] = rad[i];
+}
+
+// particle::velInfl(POI) (particle.cpp:60-77) summed over all particles in
+// their order: Gaussian kernel, sigma = coreOverlap * radius, a particle at
+// the point itself contributes nothing
+__global__ __launch_bounds__(kT) void gauss_points_kernel(int n, const double* __restrict__ ppos,
+                                                          const double* __restrict__ pstr,
+                                                          const double* __restrict__ prad, int m,
+                                                          const double* __restrict__ pts, double* __restrict__ out) {
+    const int t = blockIdx.x * kT + threadIdx.x;
+    if (t >= m) return;
+    const double px = pts[3 * t], py = pts[3 * t + 1], pz = pts[3 * t + 2];
+    double acc[3] = {0, 0, 0};
+    for (int j = 0; j < n; j++) {
+        const double dx = px - ppos[3 * j], dy = py - ppos[3 * j + 1], dz = pz - ppos[3 * j + 2];
+        const double d = nrm3(dx, dy, dz);
+        if (d == 0) continue;
+        const double sigma = kCoreOverlap * prad[j];
+        const double rho = d / sigma;
+        const double K = (1 / (4 * kPI * rho) * erf(rho / sqrt(2.0)) - 1 / pow(2 * kPI, 1.5) * exp(-0.5 * rho * rho)) /
+                         (rho * rho);
+        const double f = -1 / (sigma * sigma * sigma) * K;
+        const double ax = pstr[3 * j], ay = pstr[3 * j + 1], az = pstr[3 * j + 2];
+        acc[0] += f * (dy * az - dz * ay);
+        acc[1] += f * (dz * ax - dx * az);
+        acc[2] += f * (dx * ay - dy * ax);
+    }
+    out[3 * t] = acc[0];
+    out[3 * t + 1] = acc[1];
+    out[3 * t + 2] = acc[2];
+}
+
+// vortexFil::velInfl (vortexFil.cpp:26-40): strength / (4 pi) * vortexV over
+// the filaments (ends p1, p2: 6 per filament), in order
+__global__ __launch_bounds__(kT) void filament_points_kernel(int nf, const double* __restrict__ ends,
+                                                             const double* __restrict__ fstr, int m,
+                                                             const double* __restrict__ pts,
+                                                             double* __restrict__ out) {
+    const int t = blockIdx.x * kT + threadIdx.x;
+    if (t >= m) return;
+    const double p[3] = {pts[3 * t], pts[3 * t + 1], pts[3 * t + 2]};
+    double acc[3] = {0, 0, 0};
+    for (int f = 0; f < nf; f++) {
+        const double* e = ends + 6 * (int64_t)f;
+        const double a[3] = {p[0] - e[0], p[1] - e[1], p[2] - e[2]};
+        const double b[3] = {p[0] - e[3], p[1] - e[4], p[2] - e[5]};
+        const double sv[3] = {e[3] - e[0], e[4] - e[1], e[5] - e[2]};
+        double w[3];
+        vortex_v(a, b, sv, w);
+        for (int q = 0; q < 3; q++) acc[q] += w[q] * fstr[f] / (4 * kPI);
+    }
+    out[3 * t] = acc[0];
+    out[3 * t + 1] = acc[1];
+    out[3 * t + 2] = acc[2];
+}
+
 // panel -> point velocity over the panels [p0, p1) of the combined list
 // (body panels, then wake panels); partial sums per K slice
 __global__ __launch_bounds__(kT) void p2p_kernel(const PanelRec* __restrict__ body, int nb,
@@ -318,6 +443,50 @@ void launch_bh_particles(const DevBhTree& T, int mode, double theta, double* out
                                T.mstr, T.mrad, T.n, theta, T.order, out);
             break;
     }
+    CP_LAUNCH_CHECK();
+}
+
+void launch_bh_targets(const DevBhTree& T, int mode, double theta, int m, const double* tpos, const double* tstr,
+                       const double* trad, const int* tslot, double* out, hipStream_t s) {
+    if (m <= 0) return;
+    dim3 grid((unsigned)((m + kT - 1) / kT));
+    switch (mode) {
+        case CP_BH_VEL:
+            hipLaunchKernelGGL(bh_targets_kernel<CP_BH_VEL>, grid, dim3(kT), 0, s, T.nodes, T.n_nodes, T.mpos, T.mstr,
+                               T.mrad, theta, m, tpos, tstr, trad, tslot, out);
+            break;
+        case CP_BH_STRETCH:
+            hipLaunchKernelGGL(bh_targets_kernel<CP_BH_STRETCH>, grid, dim3(kT), 0, s, T.nodes, T.n_nodes, T.mpos,
+                               T.mstr, T.mrad, theta, m, tpos, tstr, trad, tslot, out);
+            break;
+        default:
+            hipLaunchKernelGGL(bh_targets_kernel<CP_BH_DIFF>, grid, dim3(kT), 0, s, T.nodes, T.n_nodes, T.mpos,
+                               T.mstr, T.mrad, theta, m, tpos, tstr, trad, tslot, out);
+            break;
+    }
+    CP_LAUNCH_CHECK();
+}
+
+void launch_bh_refresh(const DevBhTree& T, const double* pos, const double* str, const double* rad, hipStream_t s) {
+    if (T.n <= 0) return;
+    hipLaunchKernelGGL(bh_refresh_kernel, dim3((unsigned)((T.n + kT - 1) / kT)), dim3(kT), 0, s, T.n, T.order, pos,
+                       str, rad, T.mpos, T.mstr, T.mrad);
+    CP_LAUNCH_CHECK();
+}
+
+void launch_gauss_points(int n, const double* pos, const double* str, const double* rad, int m, const double* pts,
+                         double* out, hipStream_t s) {
+    if (m <= 0) return;
+    hipLaunchKernelGGL(gauss_points_kernel, dim3((unsigned)((m + kT - 1) / kT)), dim3(kT), 0, s, n, pos, str, rad, m,
+                       pts, out);
+    CP_LAUNCH_CHECK();
+}
+
+void launch_filament_points(int nf, const double* ends, const double* fstr, int m, const double* pts, double* out,
+                            hipStream_t s) {
+    if (m <= 0) return;
+    hipLaunchKernelGGL(filament_points_kernel, dim3((unsigned)((m + kT - 1) / kT)), dim3(kT), 0, s, nf, ends, fstr, m,
+                       pts, out);
     CP_LAUNCH_CHECK();
 }
 
