@@ -65,6 +65,18 @@ class ServerStats(C.Structure):
                 ("resident_bytes_est", C.c_uint64), ("device_bytes", C.c_uint64)]
 
 
+class VpOpts(C.Structure):
+    _fields_ = [("fs", Freestream), ("mach", C.c_double), ("Sref", C.c_double), ("bref", C.c_double),
+                ("cref", C.c_double), ("cg", C.c_double * 3), ("prec", C.c_int32), ("accelerate", C.c_int32),
+                ("warm_start", C.c_int32), ("max_members", C.c_int32), ("theta", C.c_double), ("tol", C.c_double)]
+
+
+class VpStepInfo(C.Structure):
+    _fields_ = [("step", C.c_int32), ("n_particles", C.c_int32), ("converged", C.c_int32),
+                ("solve_iters", C.c_int32), ("CL_trefftz", C.c_double), ("CDi_trefftz", C.c_double),
+                ("rel_residual", C.c_double), ("ms", C.c_double)]
+
+
 class BhOpts(C.Structure):
     _fields_ = [("max_members", C.c_int32), ("pad", C.c_int32), ("theta", C.c_double)]
 
@@ -92,6 +104,15 @@ SIGNATURES = {
                            C.POINTER(SolveInfo)]),
     "cp_matvec": (C.c_int, [C.c_void_p, C.c_int, C.c_int32, _d, _d]),
     "cp_solver_report": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int32]),
+    "cp_geometry_upload_vp": (C.c_int, [C.c_void_p, C.POINTER(Mesh), C.c_double, C.c_double, C.c_uint64,
+                                        C.POINTER(C.c_void_p)]),
+    "cp_vp_buffer_rows": (C.c_int, [C.c_void_p, _i32, _d, _d, _d]),
+    "cp_vp_create": (C.c_int, [C.c_void_p, C.POINTER(VpOpts), C.POINTER(C.c_void_p)]),
+    "cp_vp_step": (C.c_int, [C.c_void_p, C.POINTER(VpStepInfo)]),
+    "cp_vp_finish": (C.c_int, [C.c_void_p, C.POINTER(SolveInfo), _d]),
+    "cp_vp_particles": (C.c_int, [C.c_void_p, _i32, _d, _d, _d, _i32]),
+    "cp_vp_state": (C.c_int, [C.c_void_p, _d, _d, _d, _d]),
+    "cp_vp_release": (None, [C.c_void_p]),
     "cp_case_fields_eval": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, _d, C.POINTER(CaseFields)]),
     "cp_block_jacobi_read": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int32),
                                        C.POINTER(C.c_int32), C.POINTER(C.c_float)]),

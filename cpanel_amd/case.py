@@ -54,6 +54,11 @@ class Context:
     def geometry(self, mesh, key=0):
         return Geometry(self, mesh, key)
 
+    def geometry_vp(self, mesh, V=1.0, dt=0.0, key=0):
+        """Vortex-particle geometry: no prescribed wake, two buffer rows of
+        quads behind the trailing edge (time step dt; <= 0: computed)."""
+        return Geometry(self, mesh, key, vp=(V, dt))
+
     def bh_tree(self, xyz, alpha, radius, max_members=10, theta=0.5):
         return BHTree(self, xyz, alpha, radius, max_members, theta)
 
@@ -74,11 +79,21 @@ class Context:
 
 
 class Geometry:
-    def __init__(self, ctx, mesh, key=0):
+    def __init__(self, ctx, mesh, key=0, vp=None):
         self.ctx = ctx
         L = abi.lib()
         h = C.c_void_p()
-        if isinstance(mesh, str):
+        if vp is not None:
+            if isinstance(mesh, str):
+                from . import meshgen
+                mesh = meshgen.read_tri(mesh)
+            self._xyz = np.ascontiguousarray(mesh["xyz"], dtype=np.float64)
+            self._tri = np.ascontiguousarray(mesh["tri"], dtype=np.int32)
+            self._ids = np.ascontiguousarray(mesh["ids"], dtype=np.int32)
+            m = abi.Mesh(len(self._xyz), _p(self._xyz), len(self._tri), _p(self._tri, C.c_int32),
+                         _p(self._ids, C.c_int32))
+            st = L.cp_geometry_upload_vp(ctx.h, C.byref(m), float(vp[0]), float(vp[1]), key, C.byref(h))
+        elif isinstance(mesh, str):
             st = L.cp_geometry_from_tri(ctx.h, mesh.encode(), key, C.byref(h))
         else:
             self._xyz = np.ascontiguousarray(mesh["xyz"], dtype=np.float64)
@@ -248,6 +263,23 @@ class Geometry:
                      wake_mu=wmu[q, :self.nw], wake_potential=wpot[q, :self.nw], spanwise=span[q, :nwk])
                 for q in range(k)]
 
+    def vp_rows(self):
+        """(dt, first-row quads, second-row quads) of a vortex-particle
+        geometry; quads as (m, 4, 3) vertex arrays in node order."""
+        m = C.c_int32()
+        dt = C.c_double()
+        L = abi.lib()
+        self.ctx.check(L.cp_vp_buffer_rows(self.h, C.byref(m), C.byref(dt), None, None), "buffer rows")
+        r1 = np.zeros((m.value, 4, 3))
+        r2 = np.zeros((m.value, 4, 3))
+        self.ctx.check(L.cp_vp_buffer_rows(self.h, None, None, _p(r1), _p(r2)), "buffer rows")
+        return dt.value, r1, r2
+
+    def vp_case(self, condition, prec=CP_FP64, mach=0.1, Sref=1.0, bref=1.0, cref=1.0, cg=(0, 0, 0), accelerate=True,
+                warm_start=False, max_members=10, theta=0.5, tol=1e-12):
+        return VpCase(self, condition, prec, mach, Sref, bref, cref, cg, accelerate, warm_start, max_members, theta,
+                      tol)
+
     def block_jacobi(self):
         """(ptr, perm, [inverse of each block as a b x b FP32 array]) of the
         mixed-precision solve's strip block-Jacobi preconditioner, or None
@@ -298,6 +330,81 @@ class Geometry:
                             rel_residual=r.rel_residual, iters=r.iters, refinements=r.refinements,
                             converged=bool(r.converged)))
         return out
+
+
+class VpCase:
+    """The vortex-particle case of cpCaseVP (cpCaseVP.cpp:26-158) on a
+    vortex-particle geometry: step() advances one time step, run() loops to
+    the step count or to the Trefftz convergence and returns finish()'s
+    forces."""
+
+    def __init__(self, geom, condition, prec, mach, Sref, bref, cref, cg, accelerate, warm_start, max_members,
+                 theta, tol):
+        self.g = geom
+        o = abi.VpOpts()
+        o.fs = abi.Freestream(*map(float, condition))
+        o.mach, o.Sref, o.bref, o.cref = mach, Sref, bref, cref
+        o.cg = (C.c_double * 3)(*cg)
+        o.prec, o.accelerate, o.warm_start, o.max_members = prec, int(accelerate), int(warm_start), max_members
+        o.theta, o.tol = theta, tol
+        h = C.c_void_p()
+        geom.ctx.check(abi.lib().cp_vp_create(geom.h, C.byref(o), C.byref(h)), "vp create")
+        self.h = h
+
+    def step(self):
+        i = abi.VpStepInfo()
+        st = abi.lib().cp_vp_step(self.h, C.byref(i))
+        if st not in (abi.CP_OK, abi.CP_NOT_CONVERGED):
+            self.g.ctx.check(st, "vp step")
+        return dict(step=i.step, n_particles=i.n_particles, converged=bool(i.converged), iters=i.solve_iters,
+                    CL=i.CL_trefftz, CDi=i.CDi_trefftz, rel_residual=i.rel_residual, ms=i.ms,
+                    solved=st == abi.CP_OK)
+
+    def run(self, steps, on_step=None):
+        """cpCaseVP::run's loop: steps time steps, stopping early when the
+        Trefftz values converge; then finish()."""
+        for _ in range(steps):
+            s = self.step()
+            if on_step:
+                on_step(s)
+            if s["converged"]:
+                break
+        return self.finish()
+
+    def finish(self):
+        r = abi.SolveInfo()
+        cp = np.zeros(self.g.nb)
+        self.g.ctx.check(abi.lib().cp_vp_finish(self.h, C.byref(r), _p(cp)), "vp finish")
+        return dict(CL=r.CL, CDi=r.CDi, Fbody=np.array(r.Fbody[:]), Fwind=np.array(r.Fwind[:]),
+                    CM=np.array(r.CM[:]), cp=cp)
+
+    def particles(self):
+        n = C.c_int32()
+        L = abi.lib()
+        L.cp_vp_particles(self.h, C.byref(n), None, None, None, None)
+        k = n.value
+        pos, st, rad = np.zeros((k, 3)), np.zeros((k, 3)), np.zeros(k)
+        shed = np.zeros(k, np.int32)
+        L.cp_vp_particles(self.h, None, _p(pos), _p(st), _p(rad), _p(shed, C.c_int32))
+        return dict(pos=pos, strength=st, radius=rad, shed=shed)
+
+    def state(self):
+        m = self.g.nw
+        mu, sig = np.zeros(self.g.nb), np.zeros(self.g.nb)
+        m1, m2 = np.zeros(max(m, 1)), np.zeros(max(m, 1))
+        abi.lib().cp_vp_state(self.h, _p(mu), _p(sig), _p(m1), _p(m2))
+        return dict(mu=mu, sigma=sig, mu_row1=m1[:m], mu_row2=m2[:m])
+
+    def close(self):
+        if self.h:
+            abi.lib().cp_vp_release(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class BHTree:

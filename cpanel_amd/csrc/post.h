@@ -72,7 +72,10 @@ struct PostParams {
 //   phi   potential Vinf . c_i - mu_i, condition c at phi + c*ldphi;
 //   span  per condition, wake and Trefftz station: (Cl / PG, Cdi / PG^2),
 //         at span + ((c*n_wakes + w)*n_st + st)*2.
+//   vpot  input: the velocity each panel's potential is formed with (3 per
+//         panel, condition c at vpot + 3*c*n) instead of the freestream.
 struct PostFields {
+    const double* vpot = nullptr;
     double* vel = nullptr;
     int64_t ldvel = 0;
     double* phi = nullptr;

@@ -35,7 +35,8 @@ __global__ __launch_bounds__(256) void post_panel_kernel(
     const int32_t* __restrict__ mode, const double* __restrict__ ctr, const double* __restrict__ bez,
     const double* __restrict__ area, PostParams pp, const double* __restrict__ mu, int64_t ldmu,
     const double* __restrict__ vinf4, double* __restrict__ cp, int64_t ldcp, double* __restrict__ partials,
-    double* __restrict__ vel, int64_t ldvel, double* __restrict__ phi, int64_t ldphi) {
+    double* __restrict__ vel, int64_t ldvel, double* __restrict__ phi, int64_t ldphi,
+    const double* __restrict__ vpot) {
     __shared__ double sh[kT];
     const int c = blockIdx.y;
     const int i = blockIdx.x * kT + threadIdx.x;
@@ -44,15 +45,23 @@ __global__ __launch_bounds__(256) void post_panel_kernel(
     const double* cx = ctr;
     const double* cy = ctr + n;
     const double* cz = ctr + 2 * (int64_t)n;
+    // the velocity a panel's potential is formed with: the freestream, or a
+    // per-panel one (the vortex-particle case adds the particle and filament
+    // velocity at the centre, cpCaseVP::VinfPlusVecPot, cpCaseVP.cpp:601-623)
+    const double* vp = vpot ? vpot + (int64_t)c * 3 * n : nullptr;
+    auto pot = [&](int j) {
+        if (vp) return vp[3 * j] * cx[j] + vp[3 * j + 1] * cy[j] + vp[3 * j + 2] * cz[j] - m[j];
+        return Vx * cx[j] + Vy * cy[j] + Vz * cz[j] - m[j];
+    };
     double f[kNP] = {0, 0, 0, 0, 0, 0};
     if (i < n) {
-        double phi_i = Vx * cx[i] + Vy * cy[i] + Vz * cz[i] - m[i];
+        double phi_i = pot(i);
         const int32_t* nb = nbr + 16 * (int64_t)i;
         int cnt = nb[0];
         double df[kClusterMax];
         for (int k = 0; k < cnt; k++) {
             int j = nb[1 + k];
-            df[k] = (Vx * cx[j] + Vy * cy[j] + Vz * cz[j] - m[j]) - phi_i;
+            df[k] = pot(j) - phi_i;
         }
         double v0 = 0, v1 = 0, v2 = 0;
         const double* w = w1 + 45 * (int64_t)i;
@@ -203,7 +212,7 @@ void launch_post(const PostDev& P, const PostParams& pp, int k, const double* mu
     if (!f) f = &none;
     dim3 g1((unsigned)nblk, (unsigned)k);
     hipLaunchKernelGGL(post_panel_kernel, g1, dim3(kT), 0, s, P.n, P.nbr, P.w1, P.w2, P.mode, P.ctr, P.bez, P.area,
-                       pp, mu, ldmu, vinf4, cp, ldcp, partials, f->vel, f->ldvel, f->phi, f->ldphi);
+                       pp, mu, ldmu, vinf4, cp, ldcp, partials, f->vel, f->ldvel, f->phi, f->ldphi, f->vpot);
     CP_LAUNCH_CHECK();
     hipLaunchKernelGGL(post_final_kernel, dim3((unsigned)k), dim3(kT), 0, s, nblk, partials, P.n_wakes, P.n_st,
                        P.n_te, P.te_idx, P.rphi, P.rv, P.rw, P.simpson, pp, mu, ldmu, vinf4, res, f->span);

@@ -32,6 +32,10 @@ void launch_bh_targets(const DevBhTree& T, int mode, double theta, int m, const 
 // T's members take the current state of their particles (particle-indexed
 // pos, str: 3 each; rad), keeping the tree's nodes and pseudo-particles
 void launch_bh_refresh(const DevBhTree& T, const double* pos, const double* str, const double* rad, hipStream_t s);
+// direct pairwise stretching (CP_BH_STRETCH) or diffusion (CP_BH_DIFF) of n
+// particles, each particle the called object of the pair functions
+void launch_pairs_direct(int mode, int n, const double* pos, const double* str, const double* rad, double* out,
+                         hipStream_t s);
 // direct Gaussian point velocity of n particles at m points
 void launch_gauss_points(int n, const double* pos, const double* str, const double* rad, int m, const double* pts,
                          double* out, hipStream_t s);

@@ -242,6 +242,32 @@ __global__ __launch_bounds__(kT) void bh_targets_kernel(const BhNode* __restrict
     out[3 * t + 2] = acc[2];
 }
 
+// direct pairwise stretching or diffusion on every particle i from every
+// other particle j, in j order, as cpCaseVP::particleStrengthUpdate's
+// direct branch calls them (cpCaseVP.cpp:489-496, 507-514): particle i is
+// the called object (particles[i]->vortexStretching(particles[j])), the
+// roles the tree walk gives to the other particle
+template <int MODE>
+__global__ __launch_bounds__(kT) void pairs_direct_kernel(int n, const double* __restrict__ pos,
+                                                          const double* __restrict__ str,
+                                                          const double* __restrict__ rad, double* __restrict__ out) {
+    const int i = blockIdx.x * kT + threadIdx.x;
+    if (i >= n) return;
+    const Src me = member(pos, str, rad, i);
+    double acc[3] = {0, 0, 0};
+    for (int j = 0; j < n; j++) {
+        if (j == i) continue;
+        const Src o = member(pos, str, rad, j);
+        if (MODE == CP_BH_STRETCH)
+            k_stretch(me, o, acc);
+        else
+            k_diff(me, o, acc);
+    }
+    out[3 * i] = acc[0];
+    out[3 * i + 1] = acc[1];
+    out[3 * i + 2] = acc[2];
+}
+
 // the tree's members take the current state of the particles they hold
 // (the reference's octree holds particle pointers: leaves see the particles
 // as they are now, pseudo-particles stay as they were at the build)
@@ -471,6 +497,17 @@ void launch_bh_refresh(const DevBhTree& T, const double* pos, const double* str,
     if (T.n <= 0) return;
     hipLaunchKernelGGL(bh_refresh_kernel, dim3((unsigned)((T.n + kT - 1) / kT)), dim3(kT), 0, s, T.n, T.order, pos,
                        str, rad, T.mpos, T.mstr, T.mrad);
+    CP_LAUNCH_CHECK();
+}
+
+void launch_pairs_direct(int mode, int n, const double* pos, const double* str, const double* rad, double* out,
+                         hipStream_t s) {
+    if (n <= 0) return;
+    dim3 grid((unsigned)((n + kT - 1) / kT));
+    if (mode == CP_BH_STRETCH)
+        hipLaunchKernelGGL(pairs_direct_kernel<CP_BH_STRETCH>, grid, dim3(kT), 0, s, n, pos, str, rad, out);
+    else
+        hipLaunchKernelGGL(pairs_direct_kernel<CP_BH_DIFF>, grid, dim3(kT), 0, s, n, pos, str, rad, out);
     CP_LAUNCH_CHECK();
 }
 

@@ -211,6 +211,64 @@ cp_status cp_set_buffer_wake(cp_geom* g, int32_t m, const int32_t* nv, const dou
 cp_status cp_solve_rhs(cp_geom* g, cp_prec prec, int32_t k, const double* sigma, const double* mu_w2,
                        const double* x0, const cp_solve_opts* opts, double* mu, cp_solve_info* info);
 
+/* ---- vortex-particle case (cpCaseVP::run, cpCaseVP.cpp:26-158) -----------
+ * A geometry in vortex-particle mode has no prescribed wake: every
+ * trailing-edge edge gets a first buffer-row quad (the wake that folds into A)
+ * and a second-row quad behind it (geometry::readTri with the VP flag,
+ * geometry.cpp:267-397; createVPWakeSurfaces :1307-1342).  The rows start at
+ * the trailing edge and extend along the trailing-edge bisector by 1.3 V dt
+ * and V dt; dt <= 0 takes the mean trailing-edge edge length / V
+ * (geometry::calcTimeStep, :1276-1304). */
+cp_status cp_geometry_upload_vp(cp_ctx* ctx, const cp_mesh* mesh, double V, double dt, uint64_t key, cp_geom** out);
+/* m quads per row, the time step, and the quads' vertices (m x 4 x 3 per
+ * row, in node order); any output may be NULL. */
+cp_status cp_vp_buffer_rows(const cp_geom* g, int32_t* m, double* dt, double* row1, double* row2);
+
+typedef struct cp_vp cp_vp;
+
+typedef struct {
+    cp_freestream fs;       /* V, alpha, beta (steady freestream) */
+    double mach, Sref, bref, cref, cg[3];
+    int32_t prec;           /* cp_prec of the matrix solves */
+    int32_t accelerate;     /* 1: Barnes-Hut sums ("Accelerate Code"), 0: direct */
+    int32_t warm_start;     /* 1: each solve starts from the previous mu (the
+                               reference starts from zero) */
+    int32_t max_members;    /* tree leaf capacity (<= 0: 10, cpCaseVP.cpp:79) */
+    double theta;           /* opening angle (< 0: 0.5) */
+    double tol;             /* solve tolerance (<= 0: 1e-12) */
+} cp_vp_opts;
+
+typedef struct {
+    int32_t step;           /* time step just completed (1-based) */
+    int32_t n_particles;
+    int32_t converged;      /* cpCaseVP::solutionConvergence: relative change
+                               of the Trefftz CL and CDi < 1e-4 (steps >= 5) */
+    int32_t solve_iters;
+    double CL_trefftz, CDi_trefftz;  /* latest wake::trefftzPlaneVP values */
+    double rel_residual;
+    double ms;              /* wall time of the step */
+} cp_vp_step_info;
+
+/* Starts a case on a vortex-particle geometry (assembles what opts->prec
+ * needs and the second row's influence C). */
+cp_status cp_vp_create(cp_geom* g, const cp_vp_opts* opts, cp_vp** out);
+/* One time step of cpCaseVP::run: step 1 solves the first-row system, step 2
+ * convects the buffer row and re-solves, later steps convect the particles
+ * (Adams-Bashforth), shed one particle per second-row panel (RK4 seed),
+ * update the strengths (stretching, diffusion, panel stretching), rebuild
+ * the tree, set the sources from the particle and filament velocities and
+ * re-solve; from step 5 the Trefftz convergence test runs. */
+cp_status cp_vp_step(cp_vp* v, cp_vp_step_info* info);
+/* After the last step: surface velocities, Cp and forces with the
+ * particle-induced potentials (cpCaseVP::compVelocity) and the final
+ * wake::trefftzPlaneVP CL, CDi in info; cp: n_body or NULL. */
+cp_status cp_vp_finish(cp_vp* v, cp_solve_info* info, double* cp);
+/* particles in creation order: pos, strength (3 each), radius, shed step */
+cp_status cp_vp_particles(const cp_vp* v, int32_t* n, double* pos, double* strength, double* radius, int32_t* shed);
+/* body mu and sigma (n_body), first- and second-row doublet strengths (m) */
+cp_status cp_vp_state(const cp_vp* v, double* mu, double* sigma, double* mu_row1, double* mu_row2);
+void cp_vp_release(cp_vp* v);
+
 /* ---- serving: continuous batching of (geometry, conditions) requests -----
  * Replaces the reference's sequential case loop (CPanelMgr.cpp:49-88).
  * Worker threads (slots) take requests in arrival order; each geometry is

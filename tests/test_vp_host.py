@@ -42,3 +42,33 @@ def test_host_only_context_refuses_eval(hctx):
     t = hctx.bh_tree(x, x, np.full(27, 0.1))
     with pytest.raises(Exception):
         t.eval(1)
+
+
+def test_vp_geometry_buffer_rows_match_reference(hctx):
+    """Vortex-particle geometry of the reference's steadyTest case: the two
+    buffer rows (one quad per trailing-edge edge, node projections along the
+    trailing-edge bisector, computed time step) against the cells of the
+    reference's own wakeData-1.vtu and bufferWake2Data-1.vtu (6 digits)."""
+    import os
+
+    from cpanel_amd import meshgen
+    golden = os.path.join(os.path.dirname(__file__), "golden")
+    f = np.load(os.path.join(golden, "steadyTest_vp.npz"))
+    g = hctx.geometry_vp(meshgen.read_tri(os.path.join(golden, "steadyTest.tri")), V=1.0, dt=0.0)
+    dt, r1, r2 = g.vp_rows()
+    assert g.nb == 3046 and g.nw == 60 and r1.shape == (60, 4, 3)
+    # geometry::calcTimeStep: the 60 trailing-edge edges span 6 chords
+    assert abs(dt - 0.1) < 1e-12
+    tol = 5e-6 * np.maximum(np.abs(f["w1_cells"]), 0.1)
+    assert np.all(np.abs(r1 - f["w1_cells"]) <= tol)
+    assert np.all(np.abs(r2 - f["w2_cells"]) <= 5e-6 * np.maximum(np.abs(f["w2_cells"]), 0.1))
+    # row 2 starts where row 1 ends
+    assert np.array_equal(r2[:, 0], r1[:, 3]) and np.array_equal(r2[:, 1], r1[:, 2])
+
+
+def test_vp_geometry_refuses_prescribed_wake(hctx):
+    from cpanel_amd import meshgen
+    from cpanel_amd.case import CPanelError
+    m = meshgen.small_wing()
+    with pytest.raises(CPanelError):
+        hctx.geometry_vp(m, V=1.0)
