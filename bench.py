@@ -52,6 +52,9 @@ def gemm_kernel_name(shape):
         return "gemm_bf16x3_kernel"
     if shape["variant"] in (6, 7, 8, 9):
         return "gemm_bf16x3_s_kernel<%d, %d>" % {6: (2, 3), 7: (2, 4), 8: (4, 3), 9: (4, 2)}[shape["variant"]]
+    if shape["variant"] in (10, 11, 12, 13, 14):
+        return "gemm_bf16x2_s_kernel<%d, %d>" % {10: (4, 2), 11: (4, 3), 12: (4, 4), 13: (2, 2),
+                                                 14: (2, 3)}[shape["variant"]]
     return "gemm_f32_kernel<%d>" % shape["rt"]
 
 
@@ -249,7 +252,19 @@ def main():
     log("solver: %s" % json.dumps(rep))
     kname = gemm_kernel_name(rep.get("gemm_shape"))
     traffic = profiled_traffic("mixed_solve_naca0012_ar10_64cond", kname) if not args.small else None
-    if rep.get("gemm_shape", {}).get("variant", 0) >= 5:
+    variant = rep.get("gemm_shape", {}).get("variant", 0)
+    if variant >= 10:
+        # bf16x2: the product streams the bf16 hi plane of A (2 bytes per
+        # entry) plus the packed multivector and the output; the near-field
+        # restoration (near16) is a separate small kernel
+        alg_bytes = 2.0 * n * n + 2 * 4.0 * n * 64
+        gbs = alg_bytes / (gemm_avg * 1e-3) / 1e9
+        nr_ms, nr_n = ctx.timing("near16")
+        roof = {"bound": "hbm", "achieved": gbs, "peak": MI355X_HBM_GBS, "unit": "GB/s", "frac": gbs / MI355X_HBM_GBS,
+                "traffic": traffic,
+                "kernel": "%s (A32 as bf16 tiles x 64-column multivector, %.0f MB per launch; near-field FP32 "
+                          "restoration %.3f ms per product)" % (kname, alg_bytes / 1e6, nr_ms / max(nr_n, 1))}
+    elif variant >= 5:
         # bf16x3: the product needs a third of the FP32 matrix-core time and is
         # bound by streaming the packed matrix (4 bytes per entry: bf16 hi + lo)
         # plus the packed multivector and the output
@@ -304,13 +319,14 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": ("fp32(bf16x3)" if rep.get("gemm_shape", {}).get("variant", 0) >= 5 else "fp32") +
+        "dtype": ("fp32(bf16x2+near fp32)" if variant >= 10 else ("fp32(bf16x3)" if variant >= 5 else "fp32")) +
                  ("+fp64-refine(fp64 A,B)" if args.prec == "refine64" else "+fp64-refine"),
         "data": "synthetic (generated NACA wing mesh)",
         "config": {"workload": ("naca4412_ar6_4956tri_64cond" if args.small else "naca0012_ar10_25916tri_64cond"),
                    "n_body": n, "conditions": k,
                    "precision": "FP32 A (FP64 near field) in the Krylov iterations%s; FP64 refinement against %s" % (
-                       " (bf16x3 matrix products)" if rep.get("gemm_shape", {}).get("variant", 0) >= 5 else "",
+                       " (bf16 matrix products, near-field entries in FP32)" if variant >= 10 else
+                       (" (bf16x3 matrix products)" if variant >= 5 else ""),
                        "the FP64 A, B" if args.prec == "refine64" else "FP32 A, B + FP64 near field"),
                    "tol": 1e-10},
         "roofline": roof,

@@ -26,7 +26,10 @@ struct GemmShape {
                       // tiles, 16/8 K values per chunk, two register stages; 3/4: the same with
                       // 8/4 K values per chunk and three stages; 5: bf16x3 on packed tiles
                       // (4 row tiles, 2 stages); 6/7: bf16x3, 2 row tiles, 3/4 stages;
-                      // 8/9: bf16x3, 4 row tiles, 3/2 stages (generic form)
+                      // 8/9: bf16x3, 4 row tiles, 3/2 stages (generic form);
+                      // 10/11/12: bf16x2 (A as bf16, near field restored in
+                      // FP32), 4 row tiles, 2/3/4 stages; 13/14: bf16x2, 2 row
+                      // tiles, 2/3 stages
     int rt = 2;
     int kslices = 8;
 };
@@ -96,10 +99,11 @@ public:
         bj_nblocks_ = nblocks;
         bj_maxb_ = max_b;
     }
-    // bf16 hi/lo tiles of A32 (pack_a_bf16x2); enables GEMM variant 5
+    // bf16 hi/lo tile planes of A32 (pack_a_bf16x2); enables GEMM variants 5-9
     void set_packed(const void* apk, int nkb) {
         apk_ = apk;
         nkb_ = nkb;
+        plane_ = (lda_ / 32) * (int64_t)nkb * 64;
     }
     // FP64 near-field corrections of A32 and B32 (device CSR by row)
     void set_near(const int* rowptr, const int* cols, const double* dA, const double* dB) {
@@ -108,6 +112,11 @@ public:
         near_dA_ = dA;
         near_dB_ = dB;
     }
+    // A32 - bf16(A32) on the near-field pattern of set_near; enables the
+    // bf16x2 GEMM variants 10-12
+    void set_near16(const float* d16) { near16_ = d16; }
+    // partial slices a product of shape g leaves for the reduction
+    static int slices(const GemmShape& g) { return g.kslices + (g.variant >= 10 ? 1 : 0); }
 
 private:
     void tile_solve(const float* A32, const float* B32, const double* normals, const double* vinf4, int c0,
@@ -123,6 +132,9 @@ private:
     bool bj_ = false;
     bool bj_mfma_ = true;  // matrix-core block product (else vector ALU)
     bool bf16x3_ = true;   // bf16x3 GEMM is a tuner candidate
+    bool bf16x2_ = true;   // bf16x2 GEMM is a tuner candidate
+    int64_t plane_ = 0;    // uint4 elements from the hi plane to the lo plane
+    const float* near16_ = nullptr;
     bool f64_pipelined_ = true;  // pipelined FP64-accumulated product
     int f64_depth_ = 2;          // K steps per stage of the FP64-matrix product
     const int* bj_perm_ = nullptr;
@@ -162,6 +174,9 @@ void launch_sigma(int n, int64_t lda, const double* normals, const double* vinf4
 int packed_kblocks(int n);
 size_t packed_a_bytes(int64_t lda, int n);
 void launch_pack_a(const float* A32, int64_t lda, int n, void* out, hipStream_t s);
+// d16[e] = A32(rows[e], cols[e]) - bf16(A32(rows[e], cols[e])) for e < nnz
+void launch_near16_values(const float* A32, int64_t lda, int nnz, const int* rows, const int* cols, float* d16,
+                          hipStream_t s);
 // Inverts the diagonal blocks of A32 named by the block plan (perm, ptr,
 // off; rows per block <= 1024) in FP64 on the device and stores the inverses
 // in FP32, packed column-major, in out.  Work buffers: X (off[nblocks]

@@ -267,11 +267,14 @@ __global__ __launch_bounds__(256) void gemm_f32_x4s3_kernel(const float* __restr
 // refined solution is unchanged; the product costs a third of the FP32
 // matrix-core time and becomes bound by reading the matrix from HBM.
 //
-// Packed tiles (one per 32 rows x 16 K values): 64 lanes x 32 bytes, lane l
-// holding row l%32, K values 8(l/32)..8(l/32)+7 as 8 bf16 hi then 8 bf16 lo:
-// the operand layout of the MFMA, so each lane loads its operands with two
-// 16-byte loads and a wavefront reads one contiguous 2 KiB tile.
+// Packed tiles (one per 32 rows x 16 K values) in two planes, hi and lo: in
+// each plane a tile is 64 lanes x 16 bytes, lane l holding row l%32, K values
+// 8(l/32)..8(l/32)+7 as 8 bf16: the operand layout of the MFMA, so a
+// wavefront reads one contiguous 1 KiB tile per plane with one 16-byte load
+// per lane.  The bf16x3 product reads both planes; the bf16x2 product (below)
+// reads only the hi plane.
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 __device__ inline uint32_t bf16_bits_rne(float x) {
     uint32_t u = __float_as_uint(x);
@@ -291,7 +294,8 @@ __device__ inline void split8(const float (&v)[8], uint4& hi, uint4& lo) {
     lo = make_uint4(l[0] | (l[1] << 16), l[2] | (l[3] << 16), l[4] | (l[5] << 16), l[6] | (l[7] << 16));
 }
 
-// A (column-major, lda) -> tiles [row tile][K block]; K values >= n are 0
+// A (column-major, lda) -> tiles [row tile][K block] of the hi plane (out)
+// and the lo plane (out + ntiles * 64); K values >= n are 0
 __global__ __launch_bounds__(256) void pack_a_bf16x2_kernel(const float* __restrict__ A, int64_t lda, int n,
                                                             int nkb, int64_t ntiles, uint4* __restrict__ out) {
     const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -310,8 +314,8 @@ __global__ __launch_bounds__(256) void pack_a_bf16x2_kernel(const float* __restr
     }
     uint4 hi, lo;
     split8(v, hi, lo);
-    out[2 * g] = hi;
-    out[2 * g + 1] = lo;
+    out[g] = hi;
+    out[ntiles * 64 + g] = lo;
 }
 
 // P (row-major lda x 64) -> tiles [K block][column tile 0/1]; rows >= n are 0
@@ -338,7 +342,7 @@ __global__ __launch_bounds__(256) void pack_p_bf16x2_kernel(const float* __restr
 
 // Y_part[ks] = A~ P~ over the K blocks of slice ks.  Wavefront: 4 row tiles
 // (128 rows) x 2 column tiles; workgroup: 512 rows.  Two register stages.
-__global__ __launch_bounds__(256) void gemm_bf16x3_kernel(const uint4* __restrict__ Apk, int nkb,
+__global__ __launch_bounds__(256) void gemm_bf16x3_kernel(const uint4* __restrict__ Apk, int64_t plane, int nkb,
                                                           const uint4* __restrict__ Ppk, float* __restrict__ part,
                                                           int64_t lda, int kbchunk, int KS) {
     const int bid = blockIdx.x;
@@ -355,9 +359,9 @@ __global__ __launch_bounds__(256) void gemm_bf16x3_kernel(const uint4* __restric
         const int kc = ok ? kb : kb0;
 #pragma unroll
         for (int t = 0; t < 4; t++) {
-            const uint4* src = Apk + (((rt0 + t) * nkb + kc) * 64 + lane) * 2;
-            h[t] = src[0];
-            l[t] = src[1];
+            const int64_t o = ((rt0 + t) * nkb + kc) * 64 + lane;
+            h[t] = Apk[o];
+            l[t] = Apk[plane + o];
         }
 #pragma unroll
         for (int c = 0; c < 2; c++) {
@@ -412,9 +416,10 @@ __global__ __launch_bounds__(256) void gemm_bf16x3_kernel(const uint4* __restric
 // x 32 rows), S register stages (the loads for K block kb + S - 1 are issued
 // before block kb is multiplied).  K blocks past the slice load zeros.
 template <int RT, int S>
-__global__ __launch_bounds__(256) void gemm_bf16x3_s_kernel(const uint4* __restrict__ Apk, int nkb,
-                                                            const uint4* __restrict__ Ppk, float* __restrict__ part,
-                                                            int64_t lda, int kbchunk, int KS) {
+__global__ __launch_bounds__(256) void gemm_bf16x3_s_kernel(const uint4* __restrict__ Apk, int64_t plane,
+                                                            int nkb, const uint4* __restrict__ Ppk,
+                                                            float* __restrict__ part, int64_t lda, int kbchunk,
+                                                            int KS) {
     const int bid = blockIdx.x;
     const int ks = bid % KS, rb = bid / KS;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -429,9 +434,9 @@ __global__ __launch_bounds__(256) void gemm_bf16x3_s_kernel(const uint4* __restr
         if (kb < kb1) {
 #pragma unroll
             for (int t = 0; t < RT; t++) {
-                const uint4* src = Apk + (((rt0 + t) * nkb + kb) * 64 + lane) * 2;
-                h[t] = src[0];
-                l[t] = src[1];
+                const int64_t o = ((rt0 + t) * nkb + kb) * 64 + lane;
+                h[t] = Apk[o];
+                l[t] = Apk[plane + o];
             }
 #pragma unroll
             for (int c = 0; c < 2; c++) {
@@ -483,6 +488,134 @@ __global__ __launch_bounds__(256) void gemm_bf16x3_s_kernel(const uint4* __restr
             out[r * KP + m] = acc[t][0][j];
             out[r * KP + 32 + m] = acc[t][1][j];
         }
+}
+
+// bf16x2 product: A rounded once to bf16 (the hi plane alone, 2 bytes per
+// entry), the multivector split hi + lo as above, so a*p ~ hi_a (hi_p + lo_p)
+// with FP32 accumulation.  Outside the near field the inner Krylov solve
+// sees A to bf16 precision (|a - hi_a| <= 2^-9 |a|); the near-field entries,
+// which carry the solution, get their FP32 value back through near16_kernel
+// ((A32 - bf16(A32)) P on the near-field pattern, one more partial slice of
+// the same fixed-order reduction).  Half the matrix bytes of bf16x3; the FP64
+// refinement residual is unchanged, so the refined solution is too.
+// RT row tiles per wavefront (workgroup = 4 x RT x 32 rows), S register stages.
+template <int RT, int S>
+__global__ __launch_bounds__(256) void gemm_bf16x2_s_kernel(const uint4* __restrict__ Apk, int nkb,
+                                                            const uint4* __restrict__ Ppk, float* __restrict__ part,
+                                                            int64_t lda, int kbchunk, int KS) {
+    const int bid = blockIdx.x;
+    const int ks = bid % KS, rb = bid / KS;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t rt0 = ((int64_t)rb * 4 + wave) * RT;
+    const int kb0 = ks * kbchunk, kb1 = min(nkb, kb0 + kbchunk);
+    floatx16 acc[RT][2];
+#pragma unroll
+    for (int t = 0; t < RT; t++) acc[t][0] = acc[t][1] = floatx16{};
+    uint4 ah[S][RT], ph[S][2], pl[S][2];
+    const uint4 z = make_uint4(0u, 0u, 0u, 0u);
+    auto load = [&](int kb, uint4 (&h)[RT], uint4 (&qh)[2], uint4 (&ql)[2]) {
+        if (kb < kb1) {
+#pragma unroll
+            for (int t = 0; t < RT; t++) {
+                // each A tile is read once per product: stream it past the caches
+                const u32x4 v = __builtin_nontemporal_load(
+                    reinterpret_cast<const u32x4*>(Apk + ((rt0 + t) * nkb + kb) * 64 + lane));
+                h[t] = make_uint4(v.x, v.y, v.z, v.w);
+            }
+#pragma unroll
+            for (int c = 0; c < 2; c++) {
+                const uint4* src = Ppk + (((int64_t)kb * 2 + c) * 64 + lane) * 2;
+                qh[c] = src[0];
+                ql[c] = src[1];
+            }
+        } else {
+#pragma unroll
+            for (int t = 0; t < RT; t++) h[t] = z;
+#pragma unroll
+            for (int c = 0; c < 2; c++) qh[c] = ql[c] = z;
+        }
+    };
+    auto mma = [&](const uint4 (&h)[RT], const uint4 (&qh)[2], const uint4 (&ql)[2]) {
+#pragma unroll
+        for (int t = 0; t < RT; t++) {
+            const bf16x8 xh = __builtin_bit_cast(bf16x8, h[t]);
+#pragma unroll
+            for (int c = 0; c < 2; c++) {
+                const bf16x8 yh = __builtin_bit_cast(bf16x8, qh[c]);
+                const bf16x8 yl = __builtin_bit_cast(bf16x8, ql[c]);
+                acc[t][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xh, yl, acc[t][c], 0, 0, 0);
+                acc[t][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xh, yh, acc[t][c], 0, 0, 0);
+            }
+        }
+    };
+    const int nsteps = kb1 - kb0;
+    const int kbend = kb0 + (nsteps > 0 ? (nsteps + S - 1) / S * S : 0);
+#pragma unroll
+    for (int st = 0; st < S - 1; st++) load(kb0 + st, ah[st], ph[st], pl[st]);
+    for (int kb = kb0; kb < kbend; kb += S) {
+#pragma unroll
+        for (int st = 0; st < S; st++) {
+            const int nx = (st + S - 1) % S;
+            load(kb + st + S - 1, ah[nx], ph[nx], pl[nx]);
+            mma(ah[st], ph[st], pl[st]);
+        }
+    }
+    float* out = part + (int64_t)ks * lda * KP;
+    const int m = lane & 31, kh = lane >> 5;
+#pragma unroll
+    for (int t = 0; t < RT; t++)
+#pragma unroll
+        for (int j = 0; j < 16; j++) {
+            const int64_t r = (rt0 + t) * 32 + 8 * (j >> 2) + 4 * kh + (j & 3);
+            out[r * KP + m] = acc[t][0][j];
+            out[r * KP + 32 + m] = acc[t][1][j];
+        }
+}
+
+// Near-field restoration of the bf16x2 product: out(i, c) = sum over the
+// near-field entries e of row i (CSR, columns ascending) of d16[e] P(col, c),
+// d16 = A32 - bf16(A32).  One wavefront per row, lane = column; the entry
+// indices and values are wave-uniform (scalar loads) and eight 256-byte rows
+// of P are gathered at a time into four accumulators (fixed order), so the
+// gathers overlap instead of waiting on each other.  Rows without entries
+// and rows >= n get 0.
+__global__ __launch_bounds__(256) void near16_kernel(const int* __restrict__ rowptr, const int* __restrict__ cols,
+                                                     const float* __restrict__ d16, int n, int64_t lda,
+                                                     const float* __restrict__ P, float* __restrict__ out) {
+    const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int c = threadIdx.x & 63;
+    if (row >= lda) return;
+    float acc[4] = {0.f, 0.f, 0.f, 0.f};
+    if (row < n) {
+        int e = rowptr[row];
+        const int e1 = rowptr[row + 1];
+        for (; e + 8 <= e1; e += 8) {
+            int j[8];
+            float d[8], p[8];
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                j[q] = cols[e + q];
+                d[q] = d16[e + q];
+            }
+#pragma unroll
+            for (int q = 0; q < 8; q++) p[q] = P[(int64_t)j[q] * KP + c];
+#pragma unroll
+            for (int q = 0; q < 8; q++) acc[q & 3] += d[q] * p[q];
+        }
+        for (; e < e1; e++) acc[0] += d16[e] * P[(int64_t)cols[e] * KP + c];
+    }
+    out[row * KP + c] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+}
+
+// d16[e] = A32(i, j) - bf16(A32(i, j)) on the near-field entries (exact in
+// FP32: the rounding error of a rounding to fewer bits)
+__global__ __launch_bounds__(256) void near16_values_kernel(const float* __restrict__ A32, int64_t lda, int nnz,
+                                                            const int* __restrict__ rows,
+                                                            const int* __restrict__ cols, float* __restrict__ d16) {
+    const int e = blockIdx.x * kT + threadIdx.x;
+    if (e >= nnz) return;
+    const float a = A32[rows[e] + (int64_t)cols[e] * lda];
+    d16[e] = a - __uint_as_float(bf16_bits_rne(a) << 16);
 }
 
 __global__ __launch_bounds__(256) void gemm_f64acc_kernel(const float* __restrict__ A, int64_t lda, int n,
@@ -1083,6 +1216,14 @@ void launch_pack_a(const float* A32, int64_t lda, int n, void* out, hipStream_t 
     CP_LAUNCH_CHECK();
 }
 
+void launch_near16_values(const float* A32, int64_t lda, int nnz, const int* rows, const int* cols, float* d16,
+                          hipStream_t s) {
+    if (nnz <= 0) return;
+    hipLaunchKernelGGL(near16_values_kernel, dim3((unsigned)((nnz + kT - 1) / kT)), dim3(kT), 0, s, A32, lda, nnz,
+                       rows, cols, d16);
+    CP_LAUNCH_CHECK();
+}
+
 void launch_sigma(int n, int64_t lda, const double* normals, const double* vinf4, double* sigma, hipStream_t s) {
     hipLaunchKernelGGL(sigma_kernel, dim3(nblocks(lda)), dim3(kT), 0, s, n, normals, lda, vinf4, sigma);
     CP_LAUNCH_CHECK();
@@ -1093,8 +1234,9 @@ size_t BatchedSolver::workspace_bytes(int64_t lda) {
     const size_t mv32 = (size_t)lda * KP * sizeof(float);
     const int KSmax = 32;
     const int nrb = (int)(lda / kRedRows);
-    // 19 sub-buffers, each rounded up to 256 bytes
-    return 4 * mv64 + 11 * mv32 + (size_t)lda * sizeof(float) + KSmax * mv32 + KSmax * mv64 +
+    // 19 sub-buffers, each rounded up to 256 bytes; the FP32 partials hold
+    // one slice more than the K split (the bf16x2 near-field slice)
+    return 4 * mv64 + 11 * mv32 + (size_t)lda * sizeof(float) + (KSmax + 1) * mv32 + KSmax * mv64 +
            (size_t)4 * nrb * KP * sizeof(double) + KP * Q_NQ * sizeof(double) + KP * sizeof(int) + 21 * 256;
 }
 
@@ -1122,7 +1264,7 @@ BatchedSolver::BatchedSolver(int n, int64_t lda, int capacity, void* ws, void* h
     invd_ = (float*)take((size_t)lda * sizeof(float));
     ppk_ = (void*)take(mv32);  // packed_kblocks(n) * 4 KiB <= lda / 16 * 4 KiB
     pp_ = (float*)take(mv32);
-    part32_ = (float*)take(KSmax * mv32);
+    part32_ = (float*)take((KSmax + 1) * mv32);
     part64_ = (double*)take(KSmax * mv64);
     dpart_ = (double*)take((size_t)4 * nrb_ * KP * sizeof(double));
     sc_ = (double*)take(KP * Q_NQ * sizeof(double));
@@ -1146,12 +1288,14 @@ BatchedSolver::BatchedSolver(int n, int64_t lda, int capacity, void* ws, void* h
     if (const char* e = std::getenv("CPGPU_F64_SIMPLE")) f64_pipelined_ = e[0] != '1';
     // CPGPU_NO_BF16X3=1 keeps the inner product in FP32 matrix-core arithmetic
     if (const char* e = std::getenv("CPGPU_NO_BF16X3")) bf16x3_ = e[0] != '1';
+    // CPGPU_NO_BF16X2=1 keeps the bf16x2 product out of the tuner's candidates
+    if (const char* e = std::getenv("CPGPU_NO_BF16X2")) bf16x2_ = e[0] != '1';
     // CPGPU_BJ_VALU=1 selects the vector-ALU block-Jacobi product (tests)
     if (const char* e = std::getenv("CPGPU_BJ_VALU")) bj_mfma_ = e[0] != '1';
     // CPGPU_GEMM_SHAPE="variant,rt,kslices" pins the shape (tests, tuning runs)
     if (const char* e = std::getenv("CPGPU_GEMM_SHAPE")) {
         int v = 0, r = 2, k = 8;
-        if (std::sscanf(e, "%d,%d,%d", &v, &r, &k) == 3 && v >= 0 && v <= 9 && (r == 1 || r == 2 || r == 4) &&
+        if (std::sscanf(e, "%d,%d,%d", &v, &r, &k) == 3 && v >= 0 && v <= 14 && (r == 1 || r == 2 || r == 4) &&
             (k == 4 || k == 8 || k == 16 || k == 32)) {
             shape_.variant = v;
             shape_.rt = r;
@@ -1178,8 +1322,8 @@ void BatchedSolver::launch_gemm32(const GemmShape& sh, const float* A32, const f
         const int kbchunk = (nkb + KS - 1) / KS;
         dim3 grid((unsigned)((lda_ / 512) * KS));
         TimedScope ts("gemm_f32", s);
-        hipLaunchKernelGGL(gemm_bf16x3_kernel, grid, dim3(256), 0, s, (const uint4*)apk_, nkb, (const uint4*)ppk_,
-                           part32_, lda_, kbchunk, KS);
+        hipLaunchKernelGGL(gemm_bf16x3_kernel, grid, dim3(256), 0, s, (const uint4*)apk_, plane_, nkb,
+                           (const uint4*)ppk_, part32_, lda_, kbchunk, KS);
         CP_LAUNCH_CHECK();
         return;
     }
@@ -1200,23 +1344,65 @@ void BatchedSolver::launch_gemm32(const GemmShape& sh, const float* A32, const f
         const uint4* Pp = (const uint4*)ppk_;
         switch (sh.variant) {
             case 6:
-                hipLaunchKernelGGL((gemm_bf16x3_s_kernel<2, 3>), grid, dim3(256), 0, s, Ap, nkb, Pp, part32_, lda_,
-                                   kbchunk, KS);
+                hipLaunchKernelGGL((gemm_bf16x3_s_kernel<2, 3>), grid, dim3(256), 0, s, Ap, plane_, nkb, Pp, part32_,
+                                   lda_, kbchunk, KS);
                 break;
             case 7:
-                hipLaunchKernelGGL((gemm_bf16x3_s_kernel<2, 4>), grid, dim3(256), 0, s, Ap, nkb, Pp, part32_, lda_,
-                                   kbchunk, KS);
+                hipLaunchKernelGGL((gemm_bf16x3_s_kernel<2, 4>), grid, dim3(256), 0, s, Ap, plane_, nkb, Pp, part32_,
+                                   lda_, kbchunk, KS);
                 break;
             case 8:
-                hipLaunchKernelGGL((gemm_bf16x3_s_kernel<4, 3>), grid, dim3(256), 0, s, Ap, nkb, Pp, part32_, lda_,
-                                   kbchunk, KS);
+                hipLaunchKernelGGL((gemm_bf16x3_s_kernel<4, 3>), grid, dim3(256), 0, s, Ap, plane_, nkb, Pp, part32_,
+                                   lda_, kbchunk, KS);
                 break;
             default:
-                hipLaunchKernelGGL((gemm_bf16x3_s_kernel<4, 2>), grid, dim3(256), 0, s, Ap, nkb, Pp, part32_, lda_,
-                                   kbchunk, KS);
+                hipLaunchKernelGGL((gemm_bf16x3_s_kernel<4, 2>), grid, dim3(256), 0, s, Ap, plane_, nkb, Pp, part32_,
+                                   lda_, kbchunk, KS);
                 break;
         }
         CP_LAUNCH_CHECK();
+        return;
+    }
+    if (sh.variant >= 10 && sh.variant <= 14) {  // bf16x2 + near-field restoration slice
+        if (!apk_ || !near16_ || !near_rowptr_)
+            throw HipError{hipErrorInvalidValue, "bf16x2 GEMM without packed A or near-field values"};
+        const int nkb = nkb_;
+        {
+            TimedScope tp("pack_p", s);
+            hipLaunchKernelGGL(pack_p_bf16x2_kernel, dim3((unsigned)((2 * (int64_t)nkb * 64 + 255) / 256)),
+                               dim3(256), 0, s, P, n_, nkb, (uint4*)ppk_);
+        }
+        CP_LAUNCH_CHECK();
+        const int kbchunk = (nkb + KS - 1) / KS;
+        const int rows_wg = sh.variant >= 13 ? 256 : 512;
+        dim3 grid((unsigned)((lda_ / rows_wg) * KS));
+        const uint4* Ap = (const uint4*)apk_;
+        const uint4* Pp = (const uint4*)ppk_;
+        {
+            TimedScope ts("gemm_f32", s);
+            if (sh.variant == 13)
+                hipLaunchKernelGGL((gemm_bf16x2_s_kernel<2, 2>), grid, dim3(256), 0, s, Ap, nkb, Pp, part32_, lda_,
+                                   kbchunk, KS);
+            else if (sh.variant == 14)
+                hipLaunchKernelGGL((gemm_bf16x2_s_kernel<2, 3>), grid, dim3(256), 0, s, Ap, nkb, Pp, part32_, lda_,
+                                   kbchunk, KS);
+            else if (sh.variant == 10)
+                hipLaunchKernelGGL((gemm_bf16x2_s_kernel<4, 2>), grid, dim3(256), 0, s, Ap, nkb, Pp, part32_, lda_,
+                                   kbchunk, KS);
+            else if (sh.variant == 11)
+                hipLaunchKernelGGL((gemm_bf16x2_s_kernel<4, 3>), grid, dim3(256), 0, s, Ap, nkb, Pp, part32_, lda_,
+                                   kbchunk, KS);
+            else
+                hipLaunchKernelGGL((gemm_bf16x2_s_kernel<4, 4>), grid, dim3(256), 0, s, Ap, nkb, Pp, part32_, lda_,
+                                   kbchunk, KS);
+            CP_LAUNCH_CHECK();
+        }
+        {
+            TimedScope tn("near16", s);
+            hipLaunchKernelGGL(near16_kernel, dim3((unsigned)(lda_ / 4)), dim3(256), 0, s, near_rowptr_, near_cols_,
+                               near16_, n_, lda_, P, part32_ + (int64_t)KS * lda_ * KP);
+            CP_LAUNCH_CHECK();
+        }
         return;
     }
     TimedScope ts("gemm_f32", s);
@@ -1250,7 +1436,8 @@ void BatchedSolver::tune(const float* A32, hipStream_t s) {
     tuned_ = true;
     static std::mutex mu;
     static std::map<int64_t, GemmShape> cache;  // per leading dimension (and bf16x3 use), process wide
-    const int64_t ckey = 2 * lda_ + ((apk_ && bf16x3_) ? 1 : 0);
+    const bool use3 = apk_ && bf16x3_, use2 = use3 && bf16x2_ && near16_ && near_rowptr_;
+    const int64_t ckey = 4 * lda_ + (use3 ? 1 : 0) + (use2 ? 2 : 0);
     {
         std::lock_guard<std::mutex> lk(mu);
         auto it = cache.find(ckey);
@@ -1271,12 +1458,13 @@ void BatchedSolver::tune(const float* A32, hipStream_t s) {
             cand.push_back(g);
         }
     // (variant 5 is the same tiling as 9; 9 is the one timed)
-    for (int v : {1, 2, 3, 4, 6, 7, 8, 9})
+    for (int v : {1, 2, 3, 4, 6, 7, 8, 9, 10, 11, 12, 13, 14})
         for (int ks : {4, 8, 16, 32}) {
-            if (v >= 5 && (!apk_ || !bf16x3_)) continue;
+            if (v >= 5 && !use3) continue;
+            if (v >= 10 && !use2) continue;
             GemmShape g;
             g.variant = v;
-            g.rt = 4;
+            g.rt = v >= 13 ? 2 : 4;
             g.kslices = ks;
             cand.push_back(g);
         }
@@ -1291,7 +1479,7 @@ void BatchedSolver::tune(const float* A32, hipStream_t s) {
     auto run = [&](const GemmShape& g) {
         launch_gemm32(g, A32, p_, s);
         hipLaunchKernelGGL(reduce32_kernel, dim3(nblocks(lda_ * KP)), dim3(kT), 0, s, (const float*)part32_,
-                           g.kslices, lda_, n_, v_);
+                           slices(g), lda_, n_, v_);
         CP_LAUNCH_CHECK();
     };
     for (const GemmShape& g : cand) {
@@ -1317,7 +1505,7 @@ void BatchedSolver::tune(const float* A32, hipStream_t s) {
 }
 
 void BatchedSolver::matvec32(const float* A32, const float* P, float* Y, hipStream_t s) {
-    const int KS = shape_.kslices;
+    const int KS = slices(shape_);
     launch_gemm32(shape_, A32, P, s);
     if (Y) {
         hipLaunchKernelGGL(reduce32_kernel, dim3(nblocks(lda_ * KP)), dim3(kT), 0, s, (const float*)part32_, KS, lda_,
@@ -1413,7 +1601,7 @@ void BatchedSolver::tile_solve(const float* A32, const float* B32, const double*
     colsum(dpart_, 1, Q_BB, 0);
     hipLaunchKernelGGL(invd32_kernel, dim3(nblocks(lda_)), dim3(kT), 0, s, n_, A32, lda_, invd_);
     if (autotune_) tune(A32, s);
-    const int KS = shape_.kslices;
+    const int KS = slices(shape_);  // partial slices the reductions sum
     if (x0_cm)
         hipLaunchKernelGGL(gather_cols_kernel, dim3(nel), dim3(kT), 0, s, n_, lda_, kc, x0_cm, ldx, X_);
     else

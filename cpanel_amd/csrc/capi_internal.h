@@ -117,7 +117,7 @@ struct cp_geom {
     DevBuf body, wake, cxyz, fold_ptr, fold_w, fold_c, normals;
     DevBuf W;          // wake influence N x Nw (FP64)
     DevBuf A64, B64, A32, B32;
-    DevBuf A16;        // bf16 hi/lo tiles of A32 (bf16x3 inner products)
+    DevBuf A16;        // bf16 hi and lo tile planes of A32 (bf16x3 / bf16x2 inner products)
     // second buffer-wake row of the vortex-particle case: panels and C
     DevBuf w2, Cw;
     int nw2 = 0;
@@ -125,6 +125,7 @@ struct cp_geom {
     // FP64 near-field corrections of A32/B32 (entries within one panel
     // diameter: |c_i - c_j| <= longSide_j)
     DevBuf near_rowptr, near_rows, near_cols, near_dA, near_dB;
+    DevBuf near_d16;   // A32 - bf16(A32) on the same entries (bf16x2 product)
     int near_nnz = 0;
     // strip block-Jacobi preconditioner of the batched FP32 inner solve
     BlockPlan bj_plan;

@@ -294,11 +294,32 @@ def test_block_jacobi_inverses_match_lapack(c1s, c1):
         say("[gpu]   %d blocks, max rel err %.2e" % (nb, worst))
 
 
+def bf16_round(x):
+    """Round-to-nearest-even of FP32 values to bf16 (kept as float64)."""
+    u = np.asarray(x, np.float32).view(np.uint32).astype(np.uint64)
+    u = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def near_field_mask(g, m):
+    """|c_i - c_j| <= longSide_j: the entries the mixed path keeps beside its
+    FP32 matrices (longSide = the panel's longest edge)."""
+    b = g.body()
+    V = m["xyz"][m["tri"][b["tri_index"]]]
+    ls = np.max(np.linalg.norm(V - np.roll(V, 1, axis=1), axis=2), axis=1)
+    c = b["center"]
+    d2 = ((c[:, None, :] - c[None, :, :]) ** 2).sum(-1)
+    return d2 <= ls[None, :] ** 2
+
+
 @pytest.mark.parametrize("shape", ["0,1,8", "0,2,16", "0,4,4", "1,4,8", "2,4,32", "3,4,16", "4,4,32", "3,4,4",
-                                   "5,4,4", "5,4,32", "6,4,8", "7,4,4", "8,4,16", "9,4,32"])
+                                   "5,4,4", "5,4,32", "6,4,8", "7,4,4", "8,4,16", "9,4,32",
+                                   "10,4,8", "11,4,16", "12,4,4", "10,4,32", "13,2,8", "14,2,4"])
 def test_gemm_shapes(ctx, shape, monkeypatch):
-    """Every GEMM variant/shape the autotuner can pick gives the FP32 product
-    (FP32 accumulation bound as in test_matvec_kernels)."""
+    """Every GEMM variant/shape the autotuner can pick gives its product
+    (FP32 accumulation bound as in test_matvec_kernels): the FP32 product for
+    variants 0-9, and for the bf16x2 variants 10-12 the product of A32
+    rounded to bf16 outside the near field and kept in FP32 inside it."""
     monkeypatch.setenv("CPGPU_GEMM_SHAPE", shape)
     m = meshgen.wing("4412", 1.0, 6.0, 40, 12, "uniform", 20.0)
     g = ctx.geometry(m)
@@ -307,6 +328,21 @@ def test_gemm_shapes(ctx, shape, monkeypatch):
     say("[gpu] gemm shape %s" % shape)
     X = np.random.default_rng(9).normal(size=(g.nb, 64))
     Y = g.matvec(X, 1)
+    if int(shape.split(",")[0]) >= 10:
+        near = near_field_mask(g, m)
+        Aq = np.where(near, A32, bf16_round(A32))
+        ref = Aq @ X.astype(np.float32).astype(np.float64)
+        scale = np.abs(Aq) @ np.abs(X)
+        # the multivector is split hi + lo (16 significant bits), FP32 sums
+        err = np.abs(Y - ref) / (scale + 1e-300)
+        say("[gpu]   max error / (|Aq||x|) = %.2e (near-field entries %d)" % (err.max(), near.sum()))
+        assert np.all(np.abs(Y - ref) <= 4e-5 * scale + 1e-30)
+        # and it is not the FP32 product: far-field entries carry bf16 rounding
+        full = A32 @ X.astype(np.float32).astype(np.float64)
+        assert np.max(np.abs(Y - full) / (np.abs(A32) @ np.abs(X))) > 1e-4
+        rep = g.solver_report()
+        assert rep["gemm_shape"] == dict(zip(("variant", "rt", "kslices"), map(int, shape.split(","))))
+        return
     ref = A32 @ X.astype(np.float32).astype(np.float64)
     scale = np.abs(A32) @ np.abs(X)
     # FP32 accumulation; the bf16x3 form also rounds each operand to 16
@@ -320,6 +356,32 @@ def test_gemm_shapes(ctx, shape, monkeypatch):
         assert err.max() > 1e-9  # it really is the bf16 path
     rep = g.solver_report()
     assert rep["gemm_shape"] == dict(zip(("variant", "rt", "kslices"), map(int, shape.split(","))))
+
+
+def test_bf16x2_product_same_refined_solution(ctx, monkeypatch):
+    """The bf16x2 inner product (A as bf16 outside the near field) changes
+    only the inner iterations: refined against the FP64 matrices, its
+    solutions equal the bf16x3 product's to the refinement tolerance."""
+    from cpanel_amd import CP_FP32_REFINE64
+    m = meshgen.config_c1()
+    conds = [(1.0, a, b) for a in (-4.0, 3.0, 10.0) for b in (-5.0, 0.0, 5.0)]
+    kw = dict(mach=0.1, Sref=6.0, bref=6.0, cref=1.0, cg=(0.25, 0.0, 0.0), tol=1e-12)
+    out = {}
+    for shape in ("9,4,8", "10,4,8"):
+        monkeypatch.setenv("CPGPU_GEMM_SHAPE", shape)
+        g = ctx.geometry(m)
+        g.assemble(CP_FP32_REFINE64)
+        out[shape] = g.solve(conds, CP_FP32_REFINE64, **kw)
+        say("[gpu] shape %s: inner iterations %s refinements %s" % (
+            shape, [r["iters"] for r in out[shape]], [r["refinements"] for r in out[shape]]))
+        g.release()
+    for a, b in zip(out["9,4,8"], out["10,4,8"]):
+        assert a["rel_residual"] <= 1e-12 and b["rel_residual"] <= 1e-12
+        assert np.linalg.norm(a["mu"] - b["mu"]) / np.linalg.norm(a["mu"]) <= 1e-10
+        assert abs(a["CL"] - b["CL"]) <= 1e-10
+        # bf16 rounding of the far field costs at most a modest number of
+        # extra inner iterations
+        assert b["iters"] <= 2 * a["iters"] + 10
 
 
 def test_block_jacobi_mfma_matches_valu(ctx, monkeypatch):
