@@ -68,9 +68,11 @@ def profiled_traffic(workload, kernel):
     except (OSError, ValueError):
         return None
     k = ks.get(kernel)
-    if k is None and kernel and kernel.startswith("gemm_bf16x3"):
-        # the bf16x3 tilings differ only in pipeline depth: same bytes
-        k = next((v for name, v in ks.items() if name.startswith("gemm_bf16x3")), None)
+    for fam in ("gemm_bf16x3", "gemm_bf16x2"):
+        if k is None and kernel and kernel.startswith(fam):
+            # the tilings of one family differ only in pipeline depth and
+            # row tiles: the same bytes
+            k = next((v for name, v in ks.items() if name.startswith(fam)), None)
     return None if k is None else k["read_bytes"] + k["write_bytes"]
 
 
