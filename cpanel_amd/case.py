@@ -309,12 +309,17 @@ class Geometry:
         return json.loads(buf.value.decode())
 
     def solve(self, conditions, prec=CP_FP64, mach=0.1, Sref=1.0, bref=1.0, cref=1.0, cg=(0, 0, 0), tol=1e-10,
-              max_iter=0, max_refine=0, inner_tol=0.0, want_cp=True, allow_unconverged=False, precond=0):
+              max_iter=0, max_refine=0, inner_tol=0.0, want_cp=True, allow_unconverged=False, precond=0,
+              want_mu=True):
+        """Solves k freestream conditions (V, alpha_deg, beta_deg) on the
+        assembled geometry (cpCase::run per condition).  want_mu / want_cp
+        False leave the per-panel arrays on the device (the forces, CL and
+        CDi still come back)."""
         k = len(conditions)
         fs = (abi.Freestream * k)(*[abi.Freestream(*map(float, c)) for c in conditions])
         o = abi.SolveOpts(mach, Sref, bref, cref, (C.c_double * 3)(*cg), tol, max_iter, max_refine, inner_tol,
                           precond, 0)
-        mu = np.zeros((k, self.nb))
+        mu = np.zeros((k, self.nb)) if want_mu else None
         cp = np.zeros((k, self.nb)) if want_cp else None
         info = (abi.SolveInfo * k)()
         st = abi.lib().cp_solve(self.h, prec, k, fs, C.byref(o), _p(mu), _p(cp), info)
@@ -325,7 +330,7 @@ class Geometry:
         out = []
         for q in range(k):
             r = info[q]
-            out.append(dict(mu=mu[q], cp=None if cp is None else cp[q], CL=r.CL, CDi=r.CDi,
+            out.append(dict(mu=None if mu is None else mu[q], cp=None if cp is None else cp[q], CL=r.CL, CDi=r.CDi,
                             Fbody=np.array(r.Fbody[:]), Fwind=np.array(r.Fwind[:]), CM=np.array(r.CM[:]),
                             rel_residual=r.rel_residual, iters=r.iters, refinements=r.refinements,
                             converged=bool(r.converged)))

@@ -112,9 +112,13 @@ public:
         near_dA_ = dA;
         near_dB_ = dB;
     }
-    // A32 - bf16(A32) on the near-field pattern of set_near; enables the
-    // bf16x2 GEMM variants 10-12
-    void set_near16(const float* d16) { near16_ = d16; }
+    // near-field tiles of the packed A (CSR by 32-row tile: the 16-wide K
+    // blocks holding an entry |c_i - c_j| <= longSide_j); enables the bf16x2
+    // GEMM variants 10-14
+    void set_near_tiles(const int* tptr, const int* tkb) {
+        tptr_ = tptr;
+        tkb_ = tkb;
+    }
     // partial slices a product of shape g leaves for the reduction
     static int slices(const GemmShape& g) { return g.kslices + (g.variant >= 10 ? 1 : 0); }
 
@@ -134,7 +138,8 @@ private:
     bool bf16x3_ = true;   // bf16x3 GEMM is a tuner candidate
     bool bf16x2_ = true;   // bf16x2 GEMM is a tuner candidate
     int64_t plane_ = 0;    // uint4 elements from the hi plane to the lo plane
-    const float* near16_ = nullptr;
+    const int* tptr_ = nullptr;  // near-field tile lists (set_near_tiles)
+    const int* tkb_ = nullptr;
     bool f64_pipelined_ = true;  // pipelined FP64-accumulated product
     int f64_depth_ = 2;          // K steps per stage of the FP64-matrix product
     const int* bj_perm_ = nullptr;
@@ -167,6 +172,10 @@ private:
     const double* near_dB_ = nullptr;
     double* host_sc_ = nullptr;
     int* host_act_ = nullptr;
+    hipEvent_t act_ev_[2] = {nullptr, nullptr};  // active-flag read-backs of the inner loop
+    // active flags of the inner iteration being queued (null outside it): the
+    // heavy kernels of an iteration with no active column return at once
+    const int* gate_ = nullptr;
 };
 
 void launch_sigma(int n, int64_t lda, const double* normals, const double* vinf4, double* sigma, hipStream_t s);
@@ -174,9 +183,10 @@ void launch_sigma(int n, int64_t lda, const double* normals, const double* vinf4
 int packed_kblocks(int n);
 size_t packed_a_bytes(int64_t lda, int n);
 void launch_pack_a(const float* A32, int64_t lda, int n, void* out, hipStream_t s);
-// d16[e] = A32(rows[e], cols[e]) - bf16(A32(rows[e], cols[e])) for e < nnz
-void launch_near16_values(const float* A32, int64_t lda, int nnz, const int* rows, const int* cols, float* d16,
-                          hipStream_t s);
+// near-field tile lists of the packed A: for each 32-row tile, the ascending
+// 16-wide K blocks holding an entry of the near-field CSR (rowptr, cols; n rows)
+void near_tile_lists(int n, int64_t lda, const std::vector<int>& rowptr, const std::vector<int>& cols,
+                     std::vector<int>& tptr, std::vector<int>& tkb);
 // Inverts the diagonal blocks of A32 named by the block plan (perm, ptr,
 // off; rows per block <= 1024) in FP64 on the device and stores the inverses
 // in FP32, packed column-major, in out.  Work buffers: X (off[nblocks]

@@ -49,6 +49,11 @@ enum {
 // solve has reached its attainable accuracy and refinement takes over)
 constexpr int kStagWin = 60;
 
+// true when none of the 64 columns is active: the launch belongs to an inner
+// iteration queued ahead of the host's convergence check and past the end of
+// its sweep (every lane of the wavefront reads one flag)
+__device__ inline bool gate_idle(const int* gate) { return __ballot(gate[threadIdx.x & 63] != 0) == 0; }
+
 // ---------------------------------------------------------------- GEMMs
 // Y_part[ks] = A32[:, K slice ks] * P[K slice ks, :] for 64 columns.
 // RT row tiles of 32 rows per wavefront (4 wavefronts per workgroup), the
@@ -61,7 +66,8 @@ constexpr int kStagWin = 60;
 template <int RT>
 __global__ __launch_bounds__(256) void gemm_f32_kernel(const float* __restrict__ A, int64_t lda, int n,
                                                        const float* __restrict__ P, float* __restrict__ part,
-                                                       int kchunk, int KS) {
+                                                       int kchunk, int KS, const int* __restrict__ gate) {
+    if (gate && gate_idle(gate)) return;
     constexpr int U = 8;  // K steps of 2 per chunk (16 K values)
     const int bid = blockIdx.x;
     const int ks = bid % KS, rb = bid / KS;
@@ -127,7 +133,8 @@ __global__ __launch_bounds__(256) void gemm_f32_kernel(const float* __restrict__
 template <int U>
 __global__ __launch_bounds__(256) void gemm_f32_x4_kernel(const float* __restrict__ A, int64_t lda, int n,
                                                           const float* __restrict__ P, float* __restrict__ part,
-                                                          int kchunk, int KS) {
+                                                          int kchunk, int KS, const int* __restrict__ gate) {
+    if (gate && gate_idle(gate)) return;
     const int bid = blockIdx.x;
     const int ks = bid % KS, rb = bid / KS;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -194,7 +201,8 @@ __global__ __launch_bounds__(256) void gemm_f32_x4_kernel(const float* __restric
 template <int U>
 __global__ __launch_bounds__(256) void gemm_f32_x4s3_kernel(const float* __restrict__ A, int64_t lda, int n,
                                                             const float* __restrict__ P, float* __restrict__ part,
-                                                            int kchunk, int KS) {
+                                                            int kchunk, int KS, const int* __restrict__ gate) {
+    if (gate && gate_idle(gate)) return;
     const int bid = blockIdx.x;
     const int ks = bid % KS, rb = bid / KS;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -320,7 +328,8 @@ __global__ __launch_bounds__(256) void pack_a_bf16x2_kernel(const float* __restr
 
 // P (row-major lda x 64) -> tiles [K block][column tile 0/1]; rows >= n are 0
 __global__ __launch_bounds__(256) void pack_p_bf16x2_kernel(const float* __restrict__ P, int n, int nkb,
-                                                            uint4* __restrict__ out) {
+                                                            uint4* __restrict__ out, const int* __restrict__ gate) {
+    if (gate && gate_idle(gate)) return;
     const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
     const int64_t tile = g >> 6;
     if (tile >= 2 * (int64_t)nkb) return;
@@ -344,7 +353,8 @@ __global__ __launch_bounds__(256) void pack_p_bf16x2_kernel(const float* __restr
 // (128 rows) x 2 column tiles; workgroup: 512 rows.  Two register stages.
 __global__ __launch_bounds__(256) void gemm_bf16x3_kernel(const uint4* __restrict__ Apk, int64_t plane, int nkb,
                                                           const uint4* __restrict__ Ppk, float* __restrict__ part,
-                                                          int64_t lda, int kbchunk, int KS) {
+                                                          int64_t lda, int kbchunk, int KS, const int* __restrict__ gate) {
+    if (gate && gate_idle(gate)) return;
     const int bid = blockIdx.x;
     const int ks = bid % KS, rb = bid / KS;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -419,7 +429,8 @@ template <int RT, int S>
 __global__ __launch_bounds__(256) void gemm_bf16x3_s_kernel(const uint4* __restrict__ Apk, int64_t plane,
                                                             int nkb, const uint4* __restrict__ Ppk,
                                                             float* __restrict__ part, int64_t lda, int kbchunk,
-                                                            int KS) {
+                                                            int KS, const int* __restrict__ gate) {
+    if (gate && gate_idle(gate)) return;
     const int bid = blockIdx.x;
     const int ks = bid % KS, rb = bid / KS;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -490,20 +501,63 @@ __global__ __launch_bounds__(256) void gemm_bf16x3_s_kernel(const uint4* __restr
         }
 }
 
+// Near-field restoration of the bf16x2 product, per 32 x 16 tile: for row
+// tile rt and each K block kb listed for it (the tiles holding a near-field
+// entry |c_i - c_j| <= longSide_j, about 1% of them), the lo plane tile
+// bf16(A32 - bf16(A32)) times the hi plane of P.  Those tiles then carry A32
+// to 2^-16 like the bf16x3 product, the rest A32 to bf16.  One wavefront per
+// row tile writes its 32 rows of the extra partial slice (zeros when nothing
+// is listed), so the slice is complete for the reduction.
+__device__ inline void lo_tile_row(const uint4* __restrict__ Alo, int nkb, const int* __restrict__ tptr,
+                                   const int* __restrict__ tkb, const uint4* __restrict__ Ppk,
+                                   float* __restrict__ out, int64_t rt, int64_t nrt) {
+    if (rt >= nrt) return;
+    const int lane = threadIdx.x & 63;
+    floatx16 acc0 = floatx16{}, acc1 = floatx16{};
+    const int e1 = tptr[rt + 1];
+    for (int e = tptr[rt]; e < e1; e++) {
+        const int kb = tkb[e];
+        const bf16x8 xl = __builtin_bit_cast(bf16x8, Alo[(rt * nkb + kb) * 64 + lane]);
+        const bf16x8 y0 = __builtin_bit_cast(bf16x8, Ppk[(((int64_t)kb * 2 + 0) * 64 + lane) * 2]);
+        const bf16x8 y1 = __builtin_bit_cast(bf16x8, Ppk[(((int64_t)kb * 2 + 1) * 64 + lane) * 2]);
+        acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xl, y0, acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xl, y1, acc1, 0, 0, 0);
+    }
+    const int m = lane & 31, kh = lane >> 5;
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+        const int64_t r = rt * 32 + 8 * (j >> 2) + 4 * kh + (j & 3);
+        out[r * KP + m] = acc0[j];
+        out[r * KP + 32 + m] = acc1[j];
+    }
+}
+
 // bf16x2 product: A rounded once to bf16 (the hi plane alone, 2 bytes per
 // entry), the multivector split hi + lo as above, so a*p ~ hi_a (hi_p + lo_p)
 // with FP32 accumulation.  Outside the near field the inner Krylov solve
 // sees A to bf16 precision (|a - hi_a| <= 2^-9 |a|); the near-field entries,
-// which carry the solution, get their FP32 value back through near16_kernel
-// ((A32 - bf16(A32)) P on the near-field pattern, one more partial slice of
-// the same fixed-order reduction).  Half the matrix bytes of bf16x3; the FP64
+// which carry the solution, get A32 back to 2^-16 through lo_tile_row (the
+// lo plane on the tiles that hold them, one more partial slice of the same
+// fixed-order reduction).  Half the matrix bytes of bf16x3; the FP64
 // refinement residual is unchanged, so the refined solution is too.
 // RT row tiles per wavefront (workgroup = 4 x RT x 32 rows), S register stages.
+// Workgroups past the product's (lda / (128 RT)) * KS run the near-field
+// restoration (lo_tile_row, 4 row tiles each) on the CUs and wave slots the
+// product leaves free, writing the extra partial slice in the same launch.
 template <int RT, int S>
 __global__ __launch_bounds__(256) void gemm_bf16x2_s_kernel(const uint4* __restrict__ Apk, int nkb,
                                                             const uint4* __restrict__ Ppk, float* __restrict__ part,
-                                                            int64_t lda, int kbchunk, int KS) {
+                                                            int64_t lda, int kbchunk, int KS, int ngemm,
+                                                            int64_t plane, const int* __restrict__ tptr,
+                                                            const int* __restrict__ tkb,
+                                                            const int* __restrict__ gate) {
+    if (gate && gate_idle(gate)) return;
     const int bid = blockIdx.x;
+    if (bid >= ngemm) {
+        lo_tile_row(Apk + plane, nkb, tptr, tkb, Ppk, part + (int64_t)KS * lda * KP,
+                    (int64_t)(bid - ngemm) * 4 + (threadIdx.x >> 6), lda / 32);
+        return;
+    }
     const int ks = bid % KS, rb = bid / KS;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int64_t rt0 = ((int64_t)rb * 4 + wave) * RT;
@@ -570,52 +624,6 @@ __global__ __launch_bounds__(256) void gemm_bf16x2_s_kernel(const uint4* __restr
             out[r * KP + m] = acc[t][0][j];
             out[r * KP + 32 + m] = acc[t][1][j];
         }
-}
-
-// Near-field restoration of the bf16x2 product: out(i, c) = sum over the
-// near-field entries e of row i (CSR, columns ascending) of d16[e] P(col, c),
-// d16 = A32 - bf16(A32).  One wavefront per row, lane = column; the entry
-// indices and values are wave-uniform (scalar loads) and eight 256-byte rows
-// of P are gathered at a time into four accumulators (fixed order), so the
-// gathers overlap instead of waiting on each other.  Rows without entries
-// and rows >= n get 0.
-__global__ __launch_bounds__(256) void near16_kernel(const int* __restrict__ rowptr, const int* __restrict__ cols,
-                                                     const float* __restrict__ d16, int n, int64_t lda,
-                                                     const float* __restrict__ P, float* __restrict__ out) {
-    const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int c = threadIdx.x & 63;
-    if (row >= lda) return;
-    float acc[4] = {0.f, 0.f, 0.f, 0.f};
-    if (row < n) {
-        int e = rowptr[row];
-        const int e1 = rowptr[row + 1];
-        for (; e + 8 <= e1; e += 8) {
-            int j[8];
-            float d[8], p[8];
-#pragma unroll
-            for (int q = 0; q < 8; q++) {
-                j[q] = cols[e + q];
-                d[q] = d16[e + q];
-            }
-#pragma unroll
-            for (int q = 0; q < 8; q++) p[q] = P[(int64_t)j[q] * KP + c];
-#pragma unroll
-            for (int q = 0; q < 8; q++) acc[q & 3] += d[q] * p[q];
-        }
-        for (; e < e1; e++) acc[0] += d16[e] * P[(int64_t)cols[e] * KP + c];
-    }
-    out[row * KP + c] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
-}
-
-// d16[e] = A32(i, j) - bf16(A32(i, j)) on the near-field entries (exact in
-// FP32: the rounding error of a rounding to fewer bits)
-__global__ __launch_bounds__(256) void near16_values_kernel(const float* __restrict__ A32, int64_t lda, int nnz,
-                                                            const int* __restrict__ rows,
-                                                            const int* __restrict__ cols, float* __restrict__ d16) {
-    const int e = blockIdx.x * kT + threadIdx.x;
-    if (e >= nnz) return;
-    const float a = A32[rows[e] + (int64_t)cols[e] * lda];
-    d16[e] = a - __uint_as_float(bf16_bits_rne(a) << 16);
 }
 
 __global__ __launch_bounds__(256) void gemm_f64acc_kernel(const float* __restrict__ A, int64_t lda, int n,
@@ -810,11 +818,52 @@ __global__ __launch_bounds__(256) void reduce32_kernel(const float* __restrict__
     Y[idx] = (idx / KP) < n ? s : 0.f;
 }
 
-// sum of per-block column partials (fixed order) into sc[c][q]
+// rho/beta step at the top of an inner iteration; deactivates converged or
+// broken-down columns (breakdown is left to the next refinement sweep)
+__device__ inline void scal_beta_col(double* __restrict__ s, int* __restrict__ act_c, double tol2) {
+    if (!*act_c) return;
+    if (s[Q_RR] <= tol2 * s[Q_RR0]) {
+        *act_c = 0;
+        return;
+    }
+    const double it = s[Q_IT];
+    s[Q_MIN] = fmin(s[Q_MIN], s[Q_RR]);
+    if (it - s[Q_CHKIT] >= kStagWin) {
+        if (s[Q_MIN] > 0.25 * s[Q_CHK]) {  // best |r| fell by less than 2x over the window
+            *act_c = 0;
+            return;
+        }
+        s[Q_CHK] = s[Q_MIN];
+        s[Q_CHKIT] = it;
+    }
+    s[Q_IT] = it + 1;
+    double rho_old = s[Q_RHO];
+    double rho = s[Q_R0R];
+    const double eps = 1.1920928955078125e-07;
+    if (!(fabs(rho) > eps * eps * s[Q_RR0]) || !(s[Q_OMEGA] != 0)) {
+        *act_c = 0;
+        return;
+    }
+    s[Q_BETA] = (rho / rho_old) * (s[Q_ALPHA] / s[Q_OMEGA]);
+    s[Q_RHO] = rho;
+}
+
+__global__ void scal_beta_kernel(double* __restrict__ sc, int* __restrict__ act, double tol2) {
+    int c = threadIdx.x;
+    if (c >= KP) return;
+    scal_beta_col(sc + c * Q_NQ, act + c, tol2);
+}
+
+// sum of per-block column partials (fixed order) into sc[c][q], then the
+// scalar step that consumes them on the active columns (op: COLSUM_ALPHA
+// alpha = rho / (r0 . v); COLSUM_OMEGA omega = (t . s) / (t . t);
+// COLSUM_BETA the next iteration's rho/beta step)
 // 1024 threads: column c = t % 64, group g = t / 64 sums blocks b = g mod 16;
 // the 16 group sums are then added in a fixed order
+enum { COLSUM_NONE = 0, COLSUM_ALPHA = 1, COLSUM_OMEGA = 2, COLSUM_BETA = 3 };
 __global__ __launch_bounds__(1024) void colsum_kernel(const double* __restrict__ dpart, int nblk, int nq, int q0,
-                                                      int q1, double* __restrict__ sc) {
+                                                      int q1, double* __restrict__ sc, int op,
+                                                      int* __restrict__ act, double tol2) {
     __shared__ double sh[1024];
     const int c = threadIdx.x & 63, g = threadIdx.x >> 6;
     for (int q = 0; q < nq; q++) {
@@ -830,6 +879,14 @@ __global__ __launch_bounds__(1024) void colsum_kernel(const double* __restrict__
         }
         __syncthreads();
     }
+    if (g != 0 || op == COLSUM_NONE) return;
+    double* s = sc + c * Q_NQ;
+    if (op == COLSUM_BETA)
+        scal_beta_col(s, act + c, tol2);
+    else if (op == COLSUM_ALPHA && act[c])
+        s[Q_ALPHA] = s[Q_RHO] / s[Q_R0V];
+    else if (op == COLSUM_OMEGA && act[c])
+        s[Q_OMEGA] = s[Q_TT] > 0 ? s[Q_TS] / s[Q_TT] : 0.0;
 }
 
 // inner init: r = R/|R| (FP32), r0 = r, p = v = d = 0, partial r.r
@@ -871,53 +928,6 @@ __global__ void scal_init_kernel(double* __restrict__ sc) {
     s[Q_CHK] = s[Q_RR];
     s[Q_CHKIT] = 0;
     s[Q_MIN] = s[Q_RR];
-}
-
-// rho/beta step at the top of an inner iteration; deactivates converged or
-// broken-down columns (breakdown is left to the next refinement sweep)
-__global__ void scal_beta_kernel(double* __restrict__ sc, int* __restrict__ act, double tol2) {
-    int c = threadIdx.x;
-    if (c >= KP) return;
-    double* s = sc + c * Q_NQ;
-    if (!act[c]) return;
-    if (s[Q_RR] <= tol2 * s[Q_RR0]) {
-        act[c] = 0;
-        return;
-    }
-    const double it = s[Q_IT];
-    s[Q_MIN] = fmin(s[Q_MIN], s[Q_RR]);
-    if (it - s[Q_CHKIT] >= kStagWin) {
-        if (s[Q_MIN] > 0.25 * s[Q_CHK]) {  // best |r| fell by less than 2x over the window
-            act[c] = 0;
-            return;
-        }
-        s[Q_CHK] = s[Q_MIN];
-        s[Q_CHKIT] = it;
-    }
-    s[Q_IT] = it + 1;
-    double rho_old = s[Q_RHO];
-    double rho = s[Q_R0R];
-    const double eps = 1.1920928955078125e-07;
-    if (!(fabs(rho) > eps * eps * s[Q_RR0]) || !(s[Q_OMEGA] != 0)) {
-        act[c] = 0;
-        return;
-    }
-    s[Q_BETA] = (rho / rho_old) * (s[Q_ALPHA] / s[Q_OMEGA]);
-    s[Q_RHO] = rho;
-}
-
-__global__ void scal_alpha_kernel(double* __restrict__ sc, const int* __restrict__ act) {
-    int c = threadIdx.x;
-    if (c >= KP || !act[c]) return;
-    double* s = sc + c * Q_NQ;
-    s[Q_ALPHA] = s[Q_RHO] / s[Q_R0V];
-}
-
-__global__ void scal_omega_kernel(double* __restrict__ sc, const int* __restrict__ act) {
-    int c = threadIdx.x;
-    if (c >= KP || !act[c]) return;
-    double* s = sc + c * Q_NQ;
-    s[Q_OMEGA] = s[Q_TT] > 0 ? s[Q_TS] / s[Q_TT] : 0.0;
 }
 
 // p = r + beta (p - omega v); y = invd p (Jacobi), or p also written in
@@ -1081,7 +1091,8 @@ constexpr int kBjCols = 8;  // multivector columns per workgroup
 // column-major so consecutive lanes (rows) read consecutive addresses.
 __global__ __launch_bounds__(256) void bj_apply_kernel(const int* __restrict__ perm, const int* __restrict__ ptr,
                                                        const int64_t* __restrict__ off, const float* __restrict__ inv,
-                                                       const float* __restrict__ P, float* __restrict__ Y) {
+                                                       const float* __restrict__ P, float* __restrict__ Y, const int* __restrict__ gate) {
+    if (gate && gate_idle(gate)) return;
     extern __shared__ float Pl[];  // b x kBjCols
     const int q = blockIdx.x, c0 = blockIdx.y * kBjCols;
     const int b0 = ptr[q], b = ptr[q + 1] - b0;
@@ -1115,7 +1126,8 @@ __global__ __launch_bounds__(256) void bj_apply_kernel(const int* __restrict__ p
 __global__ __launch_bounds__(256) void bj_apply_mfma_kernel(const int* __restrict__ perm, const int* __restrict__ ptr,
                                                             const int64_t* __restrict__ off,
                                                             const float* __restrict__ inv, const float* __restrict__ P,
-                                                            float* __restrict__ Y) {
+                                                            float* __restrict__ Y, const int* __restrict__ gate) {
+    if (gate && gate_idle(gate)) return;
     constexpr int U = 8;  // K steps of 2 per chunk
     const int q = blockIdx.y;
     const int b0 = ptr[q], b = ptr[q + 1] - b0;
@@ -1155,7 +1167,8 @@ template <int U>
 __global__ __launch_bounds__(256) void bj_apply_pp_kernel(const int* __restrict__ perm, const int* __restrict__ ptr,
                                                           const int64_t* __restrict__ off,
                                                           const float* __restrict__ inv,
-                                                          const float* __restrict__ PP, float* __restrict__ Y) {
+                                                          const float* __restrict__ PP, float* __restrict__ Y, const int* __restrict__ gate) {
+    if (gate && gate_idle(gate)) return;
     const int q = blockIdx.y;
     const int b0 = ptr[q], b = ptr[q + 1] - b0;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -1216,12 +1229,21 @@ void launch_pack_a(const float* A32, int64_t lda, int n, void* out, hipStream_t 
     CP_LAUNCH_CHECK();
 }
 
-void launch_near16_values(const float* A32, int64_t lda, int nnz, const int* rows, const int* cols, float* d16,
-                          hipStream_t s) {
-    if (nnz <= 0) return;
-    hipLaunchKernelGGL(near16_values_kernel, dim3((unsigned)((nnz + kT - 1) / kT)), dim3(kT), 0, s, A32, lda, nnz,
-                       rows, cols, d16);
-    CP_LAUNCH_CHECK();
+void near_tile_lists(int n, int64_t lda, const std::vector<int>& rowptr, const std::vector<int>& cols,
+                     std::vector<int>& tptr, std::vector<int>& tkb) {
+    const int64_t nrt = lda / 32;
+    const int nkb = packed_kblocks(n);
+    std::vector<uint8_t> mark((size_t)nrt * nkb, 0);
+    for (int i = 0; i < n; i++)
+        for (int e = rowptr[i]; e < rowptr[i + 1]; e++) mark[(size_t)(i / 32) * nkb + cols[e] / 16] = 1;
+    tptr.assign((size_t)nrt + 1, 0);
+    tkb.clear();
+    for (int64_t rt = 0; rt < nrt; rt++) {
+        tptr[rt] = (int)tkb.size();
+        for (int kb = 0; kb < nkb; kb++)
+            if (mark[(size_t)rt * nkb + kb]) tkb.push_back(kb);
+    }
+    tptr[nrt] = (int)tkb.size();
 }
 
 void launch_sigma(int n, int64_t lda, const double* normals, const double* vinf4, double* sigma, hipStream_t s) {
@@ -1240,7 +1262,9 @@ size_t BatchedSolver::workspace_bytes(int64_t lda) {
            (size_t)4 * nrb * KP * sizeof(double) + KP * Q_NQ * sizeof(double) + KP * sizeof(int) + 21 * 256;
 }
 
-size_t BatchedSolver::host_bytes() { return KP * Q_NQ * sizeof(double) + 2 * KP * sizeof(int); }
+// per-column scalars, then 64-int blocks: the sweep flags and two read-back
+// buffers of the active flags (the inner loop checks one iteration behind)
+size_t BatchedSolver::host_bytes() { return KP * Q_NQ * sizeof(double) + 3 * KP * sizeof(int); }
 
 BatchedSolver::BatchedSolver(int n, int64_t lda, int capacity, void* ws, void* host, hipStream_t s)
     : n_(n), cap_(capacity), lda_(lda) {
@@ -1274,6 +1298,7 @@ BatchedSolver::BatchedSolver(int n, int64_t lda, int capacity, void* ws, void* h
     CP_HIP(hipMemsetAsync(pp_, 0, mv32, s));
     host_sc_ = static_cast<double*>(host);
     host_act_ = reinterpret_cast<int*>(host_sc_ + KP * Q_NQ);
+    for (hipEvent_t& e : act_ev_) CP_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     // default shape before tuning: 8 K slices (one per XCD), more for small
     // matrices
     int rb = (int)(lda / 256);
@@ -1296,7 +1321,7 @@ BatchedSolver::BatchedSolver(int n, int64_t lda, int capacity, void* ws, void* h
     if (const char* e = std::getenv("CPGPU_GEMM_SHAPE")) {
         int v = 0, r = 2, k = 8;
         if (std::sscanf(e, "%d,%d,%d", &v, &r, &k) == 3 && v >= 0 && v <= 14 && (r == 1 || r == 2 || r == 4) &&
-            (k == 4 || k == 8 || k == 16 || k == 32)) {
+            (k == 4 || k == 5 || k == 6 || k == 8 || k == 10 || k == 12 || k == 16 || k == 32)) {
             shape_.variant = v;
             shape_.rt = r;
             shape_.kslices = k;
@@ -1305,7 +1330,10 @@ BatchedSolver::BatchedSolver(int n, int64_t lda, int capacity, void* ws, void* h
     }
 }
 
-BatchedSolver::~BatchedSolver() = default;
+BatchedSolver::~BatchedSolver() {
+    for (hipEvent_t& e : act_ev_)
+        if (e) (void)hipEventDestroy(e);
+}
 
 void BatchedSolver::launch_gemm32(const GemmShape& sh, const float* A32, const float* P, hipStream_t s) {
     const int KS = sh.kslices;
@@ -1316,14 +1344,14 @@ void BatchedSolver::launch_gemm32(const GemmShape& sh, const float* A32, const f
         {
             TimedScope tp("pack_p", s);
             hipLaunchKernelGGL(pack_p_bf16x2_kernel, dim3((unsigned)((2 * (int64_t)nkb * 64 + 255) / 256)),
-                               dim3(256), 0, s, P, n_, nkb, (uint4*)ppk_);
+                               dim3(256), 0, s, P, n_, nkb, (uint4*)ppk_, gate_);
         }
         CP_LAUNCH_CHECK();
         const int kbchunk = (nkb + KS - 1) / KS;
         dim3 grid((unsigned)((lda_ / 512) * KS));
         TimedScope ts("gemm_f32", s);
         hipLaunchKernelGGL(gemm_bf16x3_kernel, grid, dim3(256), 0, s, (const uint4*)apk_, plane_, nkb,
-                           (const uint4*)ppk_, part32_, lda_, kbchunk, KS);
+                           (const uint4*)ppk_, part32_, lda_, kbchunk, KS, gate_);
         CP_LAUNCH_CHECK();
         return;
     }
@@ -1333,7 +1361,7 @@ void BatchedSolver::launch_gemm32(const GemmShape& sh, const float* A32, const f
         {
             TimedScope tp("pack_p", s);
             hipLaunchKernelGGL(pack_p_bf16x2_kernel, dim3((unsigned)((2 * (int64_t)nkb * 64 + 255) / 256)),
-                               dim3(256), 0, s, P, n_, nkb, (uint4*)ppk_);
+                               dim3(256), 0, s, P, n_, nkb, (uint4*)ppk_, gate_);
         }
         CP_LAUNCH_CHECK();
         const int kbchunk = (nkb + KS - 1) / KS;
@@ -1345,88 +1373,77 @@ void BatchedSolver::launch_gemm32(const GemmShape& sh, const float* A32, const f
         switch (sh.variant) {
             case 6:
                 hipLaunchKernelGGL((gemm_bf16x3_s_kernel<2, 3>), grid, dim3(256), 0, s, Ap, plane_, nkb, Pp, part32_,
-                                   lda_, kbchunk, KS);
+                                   lda_, kbchunk, KS, gate_);
                 break;
             case 7:
                 hipLaunchKernelGGL((gemm_bf16x3_s_kernel<2, 4>), grid, dim3(256), 0, s, Ap, plane_, nkb, Pp, part32_,
-                                   lda_, kbchunk, KS);
+                                   lda_, kbchunk, KS, gate_);
                 break;
             case 8:
                 hipLaunchKernelGGL((gemm_bf16x3_s_kernel<4, 3>), grid, dim3(256), 0, s, Ap, plane_, nkb, Pp, part32_,
-                                   lda_, kbchunk, KS);
+                                   lda_, kbchunk, KS, gate_);
                 break;
             default:
                 hipLaunchKernelGGL((gemm_bf16x3_s_kernel<4, 2>), grid, dim3(256), 0, s, Ap, plane_, nkb, Pp, part32_,
-                                   lda_, kbchunk, KS);
+                                   lda_, kbchunk, KS, gate_);
                 break;
         }
         CP_LAUNCH_CHECK();
         return;
     }
     if (sh.variant >= 10 && sh.variant <= 14) {  // bf16x2 + near-field restoration slice
-        if (!apk_ || !near16_ || !near_rowptr_)
-            throw HipError{hipErrorInvalidValue, "bf16x2 GEMM without packed A or near-field values"};
+        if (!apk_ || !tptr_)
+            throw HipError{hipErrorInvalidValue, "bf16x2 GEMM without packed A or near-field tile lists"};
         const int nkb = nkb_;
         {
             TimedScope tp("pack_p", s);
             hipLaunchKernelGGL(pack_p_bf16x2_kernel, dim3((unsigned)((2 * (int64_t)nkb * 64 + 255) / 256)),
-                               dim3(256), 0, s, P, n_, nkb, (uint4*)ppk_);
+                               dim3(256), 0, s, P, n_, nkb, (uint4*)ppk_, gate_);
         }
         CP_LAUNCH_CHECK();
         const int kbchunk = (nkb + KS - 1) / KS;
         const int rows_wg = sh.variant >= 13 ? 256 : 512;
-        dim3 grid((unsigned)((lda_ / rows_wg) * KS));
+        const int ngemm = (int)((lda_ / rows_wg) * KS);
+        dim3 grid((unsigned)(ngemm + lda_ / 128));  // + 4 row tiles per workgroup
         const uint4* Ap = (const uint4*)apk_;
         const uint4* Pp = (const uint4*)ppk_;
-        {
-            TimedScope ts("gemm_f32", s);
-            if (sh.variant == 13)
-                hipLaunchKernelGGL((gemm_bf16x2_s_kernel<2, 2>), grid, dim3(256), 0, s, Ap, nkb, Pp, part32_, lda_,
-                                   kbchunk, KS);
-            else if (sh.variant == 14)
-                hipLaunchKernelGGL((gemm_bf16x2_s_kernel<2, 3>), grid, dim3(256), 0, s, Ap, nkb, Pp, part32_, lda_,
-                                   kbchunk, KS);
-            else if (sh.variant == 10)
-                hipLaunchKernelGGL((gemm_bf16x2_s_kernel<4, 2>), grid, dim3(256), 0, s, Ap, nkb, Pp, part32_, lda_,
-                                   kbchunk, KS);
-            else if (sh.variant == 11)
-                hipLaunchKernelGGL((gemm_bf16x2_s_kernel<4, 3>), grid, dim3(256), 0, s, Ap, nkb, Pp, part32_, lda_,
-                                   kbchunk, KS);
-            else
-                hipLaunchKernelGGL((gemm_bf16x2_s_kernel<4, 4>), grid, dim3(256), 0, s, Ap, nkb, Pp, part32_, lda_,
-                                   kbchunk, KS);
-            CP_LAUNCH_CHECK();
+        TimedScope ts("gemm_f32", s);
+#define CP_BF16X2(RT, S)                                                                                         \
+    hipLaunchKernelGGL((gemm_bf16x2_s_kernel<RT, S>), grid, dim3(256), 0, s, Ap, nkb, Pp, part32_, lda_, kbchunk, \
+                       KS, ngemm, plane_, tptr_, tkb_, gate_)
+        switch (sh.variant) {
+            case 10: CP_BF16X2(4, 2); break;
+            case 11: CP_BF16X2(4, 3); break;
+            case 12: CP_BF16X2(4, 4); break;
+            case 13: CP_BF16X2(2, 2); break;
+            default: CP_BF16X2(2, 3); break;
         }
-        {
-            TimedScope tn("near16", s);
-            hipLaunchKernelGGL(near16_kernel, dim3((unsigned)(lda_ / 4)), dim3(256), 0, s, near_rowptr_, near_cols_,
-                               near16_, n_, lda_, P, part32_ + (int64_t)KS * lda_ * KP);
-            CP_LAUNCH_CHECK();
-        }
+#undef CP_BF16X2
+        CP_LAUNCH_CHECK();
         return;
     }
     TimedScope ts("gemm_f32", s);
     if (sh.variant >= 1) {  // 16-byte A loads, 512 rows per workgroup
         dim3 grid((unsigned)((lda_ / 512) * KS));
         if (sh.variant == 1)
-            hipLaunchKernelGGL(gemm_f32_x4_kernel<8>, grid, dim3(256), 0, s, A32, lda_, n_, P, part32_, kchunk, KS);
+            hipLaunchKernelGGL(gemm_f32_x4_kernel<8>, grid, dim3(256), 0, s, A32, lda_, n_, P, part32_, kchunk, KS, gate_);
         else if (sh.variant == 2)
-            hipLaunchKernelGGL(gemm_f32_x4_kernel<4>, grid, dim3(256), 0, s, A32, lda_, n_, P, part32_, kchunk, KS);
+            hipLaunchKernelGGL(gemm_f32_x4_kernel<4>, grid, dim3(256), 0, s, A32, lda_, n_, P, part32_, kchunk, KS, gate_);
         else if (sh.variant == 3)
             hipLaunchKernelGGL(gemm_f32_x4s3_kernel<4>, grid, dim3(256), 0, s, A32, lda_, n_, P, part32_, kchunk,
-                               KS);
+                               KS, gate_);
         else
             hipLaunchKernelGGL(gemm_f32_x4s3_kernel<2>, grid, dim3(256), 0, s, A32, lda_, n_, P, part32_, kchunk,
-                               KS);
+                               KS, gate_);
         CP_LAUNCH_CHECK();
         return;
     }
     const int rows_per_block = 4 * 32 * sh.rt;
     dim3 grid((unsigned)((lda_ / rows_per_block) * KS));
     switch (sh.rt) {
-        case 1: hipLaunchKernelGGL(gemm_f32_kernel<1>, grid, dim3(256), 0, s, A32, lda_, n_, P, part32_, kchunk, KS); break;
-        case 2: hipLaunchKernelGGL(gemm_f32_kernel<2>, grid, dim3(256), 0, s, A32, lda_, n_, P, part32_, kchunk, KS); break;
-        default: hipLaunchKernelGGL(gemm_f32_kernel<4>, grid, dim3(256), 0, s, A32, lda_, n_, P, part32_, kchunk, KS); break;
+        case 1: hipLaunchKernelGGL(gemm_f32_kernel<1>, grid, dim3(256), 0, s, A32, lda_, n_, P, part32_, kchunk, KS, gate_); break;
+        case 2: hipLaunchKernelGGL(gemm_f32_kernel<2>, grid, dim3(256), 0, s, A32, lda_, n_, P, part32_, kchunk, KS, gate_); break;
+        default: hipLaunchKernelGGL(gemm_f32_kernel<4>, grid, dim3(256), 0, s, A32, lda_, n_, P, part32_, kchunk, KS, gate_); break;
     }
     CP_LAUNCH_CHECK();
 }
@@ -1436,7 +1453,7 @@ void BatchedSolver::tune(const float* A32, hipStream_t s) {
     tuned_ = true;
     static std::mutex mu;
     static std::map<int64_t, GemmShape> cache;  // per leading dimension (and bf16x3 use), process wide
-    const bool use3 = apk_ && bf16x3_, use2 = use3 && bf16x2_ && near16_ && near_rowptr_;
+    const bool use3 = apk_ && bf16x3_, use2 = use3 && bf16x2_ && tptr_;
     const int64_t ckey = 4 * lda_ + (use3 ? 1 : 0) + (use2 ? 2 : 0);
     {
         std::lock_guard<std::mutex> lk(mu);
@@ -1458,8 +1475,11 @@ void BatchedSolver::tune(const float* A32, hipStream_t s) {
             cand.push_back(g);
         }
     // (variant 5 is the same tiling as 9; 9 is the one timed)
+    // K splits: powers of two, plus 5, 6, 10, 12, which let the row-block
+    // count times the split land near one or two workgroups per CU
     for (int v : {1, 2, 3, 4, 6, 7, 8, 9, 10, 11, 12, 13, 14})
-        for (int ks : {4, 8, 16, 32}) {
+        for (int ks : {4, 5, 6, 8, 10, 12, 16, 32}) {
+            if (v < 10 && (ks == 5 || ks == 6 || ks == 10 || ks == 12)) continue;
             if (v >= 5 && !use3) continue;
             if (v >= 10 && !use2) continue;
             GemmShape g;
@@ -1548,14 +1568,14 @@ void BatchedSolver::apply_block_jacobi(const float* P, float* Y, hipStream_t s) 
     if (bj_mfma_ && bj_iperm_) {  // P was also written in block order to pp_
         dim3 grid((unsigned)((bj_maxb_ + 63) / 64), (unsigned)bj_nblocks_);
         hipLaunchKernelGGL(bj_apply_pp_kernel<8>, grid, dim3(kT), 0, s, bj_perm_, bj_ptr_, bj_off_, bj_inv_,
-                           (const float*)pp_, Y);
+                           (const float*)pp_, Y, gate_);
     } else if (bj_mfma_) {
         dim3 grid((unsigned)((bj_maxb_ + 63) / 64), (unsigned)bj_nblocks_);
-        hipLaunchKernelGGL(bj_apply_mfma_kernel, grid, dim3(kT), 0, s, bj_perm_, bj_ptr_, bj_off_, bj_inv_, P, Y);
+        hipLaunchKernelGGL(bj_apply_mfma_kernel, grid, dim3(kT), 0, s, bj_perm_, bj_ptr_, bj_off_, bj_inv_, P, Y, gate_);
     } else {
         dim3 grid((unsigned)bj_nblocks_, (unsigned)(KP / kBjCols));
         hipLaunchKernelGGL(bj_apply_kernel, grid, dim3(kT), (size_t)bj_maxb_ * kBjCols * sizeof(float), s, bj_perm_,
-                           bj_ptr_, bj_off_, bj_inv_, P, Y);
+                           bj_ptr_, bj_off_, bj_inv_, P, Y, gate_);
     }
     CP_LAUNCH_CHECK();
 }
@@ -1594,8 +1614,8 @@ void BatchedSolver::tile_solve(const float* A32, const float* B32, const double*
         hipLaunchKernelGGL(sigma_rows_kernel, dim3(nel), dim3(kT), 0, s, n_, lda_, kc, normals, vinf4, S_);
     if (add_cm) hipLaunchKernelGGL(gather_cols_kernel, dim3(nel), dim3(kT), 0, s, n_, lda_, kc, add_cm, ldx, R_);
     matvec64acc(B32, S_, Bv_, add_cm ? R_ : nullptr, -1.0, s, near_dB_, b64_);
-    auto colsum = [&](const double* dp, int nq, int q0, int q1) {
-        hipLaunchKernelGGL(colsum_kernel, dim3(1), dim3(1024), 0, s, dp, nrb_, nq, q0, q1, sc_);
+    auto colsum = [&](const double* dp, int nq, int q0, int q1, int op = COLSUM_NONE, double tol2 = 0) {
+        hipLaunchKernelGGL(colsum_kernel, dim3(1), dim3(1024), 0, s, dp, nrb_, nq, q0, q1, sc_, op, act_, tol2);
         CP_LAUNCH_CHECK();
     };
     colsum(dpart_, 1, Q_BB, 0);
@@ -1652,14 +1672,18 @@ void BatchedSolver::tile_solve(const float* A32, const float* B32, const double*
         hipLaunchKernelGGL(scal_init_kernel, dim3(1), dim3(KP), 0, s, sc_);
         int it = 0;
         const int maxit = o.max_iter > 0 ? o.max_iter : 2 * n_;
-        while (it < maxit) {
-            hipLaunchKernelGGL(scal_beta_kernel, dim3(1), dim3(KP), 0, s, sc_, act_, itol2);
-            // check for all columns finished (one small read per iteration)
-            CP_HIP(hipMemcpyAsync(host_act_, act_, KP * sizeof(int), hipMemcpyDeviceToHost, s));
-            CP_HIP(hipStreamSynchronize(s));
-            int nact = 0;
-            for (int c = 0; c < KP; c++) nact += host_act_[c] != 0;
-            if (nact == 0) break;
+        // The host reads iteration it's active flags while iteration it is
+        // already queued behind them, so the device never waits for the
+        // host's check.  An iteration queued past the end of its sweep finds
+        // no active column: its heavy kernels return at once (gate_) and the
+        // per-column updates are masked as for any finished column.
+        int* hact[2] = {host_act_, host_act_ + 2 * KP};  // host_act_ + KP holds the sweep flags
+        auto top = [&](int k) {  // flag read-back of iteration k (its rho/beta step ends iteration k - 1)
+            if (k == 0) hipLaunchKernelGGL(scal_beta_kernel, dim3(1), dim3(KP), 0, s, sc_, act_, itol2);
+            CP_HIP(hipMemcpyAsync(hact[k & 1], act_, KP * sizeof(int), hipMemcpyDeviceToHost, s));
+            CP_HIP(hipEventRecord(act_ev_[k & 1], s));
+        };
+        auto body = [&]() {  // one BiCGSTAB iteration on the active columns
             const int* ip = (bj_ && bj_mfma_) ? bj_iperm_ : nullptr;
             hipLaunchKernelGGL(inner_p_kernel, dim3(nel), dim3(kT), 0, s, rows, n_, (const double*)sc_,
                                (const int*)act_, (const float*)r_, p_, (const float*)v_,
@@ -1671,8 +1695,7 @@ void BatchedSolver::tile_solve(const float* A32, const float* B32, const double*
                 hipLaunchKernelGGL(inner_v_kernel, dim3((unsigned)nrb_), dim3(kT), 0, s, (const float*)part32_, KS,
                                    lda_, n_, (const int*)act_, v_, (const float*)r0_, dpart_);
             }
-            colsum(dpart_, 1, Q_R0V, 0);
-            hipLaunchKernelGGL(scal_alpha_kernel, dim3(1), dim3(KP), 0, s, sc_, (const int*)act_);
+            colsum(dpart_, 1, Q_R0V, 0, COLSUM_ALPHA);
             hipLaunchKernelGGL(inner_s_kernel, dim3(nel), dim3(kT), 0, s, rows, n_, (const double*)sc_,
                                (const int*)act_, (const float*)r_, (const float*)v_,
                                bj_ ? (const float*)nullptr : (const float*)invd_, sv_, z_, ip, pp_);
@@ -1684,17 +1707,38 @@ void BatchedSolver::tile_solve(const float* A32, const float* B32, const double*
                                    lda_, n_, (const int*)act_, t_, (const float*)sv_, dpart_,
                                    dpart_ + (size_t)nrb_ * KP);
             }
-            colsum(dpart_, 2, Q_TT, Q_TS);
-            hipLaunchKernelGGL(scal_omega_kernel, dim3(1), dim3(KP), 0, s, sc_, (const int*)act_);
+            colsum(dpart_, 2, Q_TT, Q_TS, COLSUM_OMEGA);
             hipLaunchKernelGGL(inner_x_kernel, dim3((unsigned)nrb_), dim3(kT), 0, s, n_, (const double*)sc_,
                                (const int*)act_, d_, (const float*)y_, (const float*)z_, r_, (const float*)sv_,
                                (const float*)t_, (const float*)r0_, dpart_, dpart_ + (size_t)nrb_ * KP);
-            colsum(dpart_, 2, Q_RR, Q_R0R);
+            colsum(dpart_, 2, Q_RR, Q_R0R, COLSUM_BETA, itol2);  // and the next iteration's rho/beta step
             CP_LAUNCH_CHECK();
-            ++it;
+        };
+        gate_ = act_;
+        top(0);
+        for (;;) {
+            const bool more = it < maxit;
+            if (more) {
+                body();
+                top(it + 1);
+            }
+            CP_HIP(hipEventSynchronize(act_ev_[it & 1]));
+            const int* fl = hact[it & 1];
+            int nact = 0;
+            for (int c = 0; c < KP; c++) nact += fl[c] != 0;
+            if (!more || nact == 0) {
+                // the iteration queued behind this check found no active
+                // column: its launches did no work, so they are not timed
+                if (more)
+                    if (KernelTimer* t = current_timer())
+                        for (const char* nm : {"gemm_f32", "pack_p", "reduce_f32"}) t->discard(nm, 2);
+                break;
+            }
             for (int c = 0; c < kc; c++)
-                if (host_act_[c]) stats[c].iters++;
+                if (fl[c]) stats[c].iters++;
+            ++it;
         }
+        gate_ = nullptr;
         // X += |R| d on the columns of this sweep
         CP_HIP(hipMemcpyAsync(act_, host_sweep, KP * sizeof(int), hipMemcpyHostToDevice, s));
         hipLaunchKernelGGL(correct_kernel, dim3(nel), dim3(kT), 0, s, rows, (const double*)sc_, (const int*)act_, X_,

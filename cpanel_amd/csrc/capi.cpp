@@ -514,7 +514,10 @@ cp_status cp_assemble(cp_geom* g, cp_prec prec) {
                 upload(c, g->near_rows, rows);
                 dalloc(c, g->near_dA, std::max<size_t>(cl.size(), 1) * sizeof(double));
                 dalloc(c, g->near_dB, std::max<size_t>(cl.size(), 1) * sizeof(double));
-                dalloc(c, g->near_d16, std::max<size_t>(cl.size(), 1) * sizeof(float));
+                std::vector<int> tptr, tkb;
+                near_tile_lists(g->dev.n, lda, rp, cl, tptr, tkb);
+                upload(c, g->near_tptr, tptr);
+                upload(c, g->near_tkb, tkb);
             }
             {
                 TimedScope ts("near_f64", g->stream);
@@ -524,8 +527,6 @@ cp_status cp_assemble(cp_geom* g, cp_prec prec) {
             }
             if (!g->A16.p) dalloc(c, g->A16, packed_a_bytes(lda, g->dev.n));
             launch_pack_a(g->A32.as<float>(), lda, g->dev.n, g->A16.p, g->stream);
-            launch_near16_values(g->A32.as<float>(), lda, g->near_nnz, g->near_rows.as<int>(),
-                                 g->near_cols.as<int>(), g->near_d16.as<float>(), g->stream);
             g->have32 = true;
             g->bj_ready = false;  // inverses follow the new A32
         }
@@ -617,7 +618,7 @@ cp_status cp_solve(cp_geom* g, cp_prec prec, int32_t k, const cp_freestream* fs,
             g->batched->set_near(g->near_rowptr.as<int>(), g->near_cols.as<int>(), g->near_dA.as<double>(),
                                  g->near_dB.as<double>());
             g->batched->set_packed(g->A16.p, packed_kblocks(n));
-            g->batched->set_near16(g->near_d16.as<float>());
+            g->batched->set_near_tiles(g->near_tptr.as<int>(), g->near_tkb.as<int>());
             if (prec == CP_FP32_REFINE64)
                 g->batched->set_f64(g->A64.as<double>(), g->B64.as<double>());
             else
@@ -707,7 +708,7 @@ cp_status cp_matvec(cp_geom* g, int kernel, int32_t k, const double* X, double* 
         g->batched->set_near(g->near_rowptr.as<int>(), g->near_cols.as<int>(), g->near_dA.as<double>(),
                              g->near_dB.as<double>());
         g->batched->set_packed(g->A16.p, packed_kblocks(n));
-        g->batched->set_near16(g->near_d16.as<float>());
+        g->batched->set_near_tiles(g->near_tptr.as<int>(), g->near_tkb.as<int>());
         // row-major lda x 64 multivectors
         const size_t elems = (size_t)lda * kRhsTile;
         if (kernel == 1) {
@@ -1125,7 +1126,7 @@ cp_status cp_solve_rhs(cp_geom* g, cp_prec prec, int32_t k, const double* sigma,
             g->batched->set_near(g->near_rowptr.as<int>(), g->near_cols.as<int>(), g->near_dA.as<double>(),
                                  g->near_dB.as<double>());
             g->batched->set_packed(g->A16.p, packed_kblocks(n));
-            g->batched->set_near16(g->near_d16.as<float>());
+            g->batched->set_near_tiles(g->near_tptr.as<int>(), g->near_tkb.as<int>());
             if (prec == CP_FP32_REFINE64)
                 g->batched->set_f64(g->A64.as<double>(), g->B64.as<double>());
             else

@@ -125,7 +125,7 @@ struct cp_geom {
     // FP64 near-field corrections of A32/B32 (entries within one panel
     // diameter: |c_i - c_j| <= longSide_j)
     DevBuf near_rowptr, near_rows, near_cols, near_dA, near_dB;
-    DevBuf near_d16;   // A32 - bf16(A32) on the same entries (bf16x2 product)
+    DevBuf near_tptr, near_tkb;  // the packed 32 x 16 tiles holding them (bf16x2 product)
     int near_nnz = 0;
     // strip block-Jacobi preconditioner of the batched FP32 inner solve
     BlockPlan bj_plan;

@@ -51,6 +51,17 @@ void KernelTimer::collect_locked() {
     pending_.clear();
 }
 
+void KernelTimer::discard(const char* name, int count) {
+    std::lock_guard<std::mutex> lk(mu_);
+    for (size_t i = pending_.size(); i-- > 0 && count > 0;) {
+        if (pending_[i].name != name) continue;
+        pool_.push_back(pending_[i].a);
+        pool_.push_back(pending_[i].b);
+        pending_.erase(pending_.begin() + (std::ptrdiff_t)i);
+        --count;
+    }
+}
+
 void KernelTimer::collect() {
     std::lock_guard<std::mutex> lk(mu_);
     collect_locked();

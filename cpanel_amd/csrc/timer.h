@@ -21,6 +21,9 @@ public:
     // waits for pending events and accumulates them
     void collect();
     void reset();
+    // drops the last `count` not yet collected intervals recorded under
+    // `name` (launches found afterwards to have done no work)
+    void discard(const char* name, int count);
     bool get(const std::string& name, double& ms, long long& n);
 
 private:

@@ -319,7 +319,8 @@ def test_gemm_shapes(ctx, shape, monkeypatch):
     """Every GEMM variant/shape the autotuner can pick gives its product
     (FP32 accumulation bound as in test_matvec_kernels): the FP32 product for
     variants 0-9, and for the bf16x2 variants 10-12 the product of A32
-    rounded to bf16 outside the near field and kept in FP32 inside it."""
+    rounded to bf16 outside the packed tiles holding near-field entries and
+    kept to 2^-16 inside them."""
     monkeypatch.setenv("CPGPU_GEMM_SHAPE", shape)
     m = meshgen.wing("4412", 1.0, 6.0, 40, 12, "uniform", 20.0)
     g = ctx.geometry(m)
@@ -330,12 +331,19 @@ def test_gemm_shapes(ctx, shape, monkeypatch):
     Y = g.matvec(X, 1)
     if int(shape.split(",")[0]) >= 10:
         near = near_field_mask(g, m)
-        Aq = np.where(near, A32, bf16_round(A32))
+        # the restoration works per packed 32 x 16 tile: every tile holding a
+        # near-field entry carries A32 (to 2^-16), the rest bf16(A32)
+        ti, tj = np.nonzero(near)
+        T = np.zeros(((g.nb + 31) // 32, (g.nb + 15) // 16), bool)
+        T[ti // 32, tj // 16] = True
+        tiles = T[np.arange(g.nb)[:, None] // 32, np.arange(g.nb)[None, :] // 16]
+        Aq = np.where(tiles, A32, bf16_round(A32))
         ref = Aq @ X.astype(np.float32).astype(np.float64)
         scale = np.abs(Aq) @ np.abs(X)
         # the multivector is split hi + lo (16 significant bits), FP32 sums
         err = np.abs(Y - ref) / (scale + 1e-300)
-        say("[gpu]   max error / (|Aq||x|) = %.2e (near-field entries %d)" % (err.max(), near.sum()))
+        say("[gpu]   max error / (|Aq||x|) = %.2e (near-field entries %d, tiles %d)" % (err.max(), near.sum(),
+                                                                                  T.sum()))
         assert np.all(np.abs(Y - ref) <= 4e-5 * scale + 1e-30)
         # and it is not the FP32 product: far-field entries carry bf16 rounding
         full = A32 @ X.astype(np.float32).astype(np.float64)
