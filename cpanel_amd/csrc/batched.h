@@ -128,6 +128,8 @@ private:
                     const double* sig_cm = nullptr, const double* add_cm = nullptr, const double* x0_cm = nullptr);
     void launch_gemm32(const GemmShape& sh, const float* A32, const float* P, hipStream_t s);
     void apply_block_jacobi(const float* P, float* Y, hipStream_t s);
+    // Y = B (rows >= n zeroed) with the column partials of ||Y_c||^2 in dpart_
+    void copy64(const double* B, double* Y, hipStream_t s);
     const double* a64_ = nullptr;
     const double* b64_ = nullptr;
     const void* apk_ = nullptr;
@@ -164,6 +166,8 @@ private:
     float* part32_;     // kslices x lda x 64
     double* part64_;    // kslices x lda x 64
     double* dpart_;     // nrb x 64 x 4 partial column dots
+    double* bn_;        // lda x 4: -B n_x, -B n_y, -B n_z (steady right-hand sides)
+    int bn_mode_ = -1;  // matrix bn_ was formed with: 1 FP64 B, 0 B32 + near field, -1 none
     double* sc_;        // 64 x 16 per-column scalars
     int* act_;          // 64 active flags
     const int* near_rowptr_ = nullptr;

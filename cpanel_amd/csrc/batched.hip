@@ -780,6 +780,37 @@ __global__ void sigma_rows_kernel(int n, int64_t lda, int kc, const double* __re
     S[idx] = v;
 }
 
+// S(i, d) = n_d(i) for d < 3, 0 elsewhere (the normals as three columns)
+__global__ void normal_cols_kernel(int n, int64_t lda, const double* __restrict__ nrm, double* __restrict__ S) {
+    int64_t idx = (int64_t)blockIdx.x * kT + threadIdx.x;
+    if (idx >= lda * KP) return;
+    int64_t i = idx / KP;
+    int d = (int)(idx % KP);
+    S[idx] = (i < n && d < 3) ? nrm[d * lda + i] : 0.0;
+}
+
+// bn(i, d) = Y(i, d) for d < 3 (Y = -B N, lda x 64)
+__global__ void keep3_kernel(int64_t lda, const double* __restrict__ Y, double* __restrict__ bn) {
+    int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+    if (i >= lda) return;
+    for (int d = 0; d < 3; d++) bn[4 * i + d] = Y[i * KP + d];
+    bn[4 * i + 3] = 0.0;
+}
+
+// R(i, c) = -(bn(i, :) . Vinf_c) = -B sigma_c for the steady source
+// strengths sigma_c = -Vinf_c . n (cpCase::setSourceStrengths), c < kc
+__global__ void rhs_bn_kernel(int n, int64_t lda, int kc, const double* __restrict__ bn,
+                              const double* __restrict__ vinf4, double* __restrict__ R) {
+    int64_t idx = (int64_t)blockIdx.x * kT + threadIdx.x;
+    if (idx >= lda * KP) return;
+    int64_t i = idx / KP;
+    int c = (int)(idx % KP);
+    double v = 0;
+    if (i < n && c < kc)
+        v = -(bn[4 * i] * vinf4[4 * c] + bn[4 * i + 1] * vinf4[4 * c + 1] + bn[4 * i + 2] * vinf4[4 * c + 2]);
+    R[idx] = v;
+}
+
 // Y = Badd + sign * (sum_ks part + D X) ; partial ||Y_c||^2.  D is the
 // optional sparse FP64 near-field correction (CSR by row).
 __global__ __launch_bounds__(256) void reduce64_kernel(const double* __restrict__ part, int KS, int64_t lda, int n,
@@ -1256,10 +1287,11 @@ size_t BatchedSolver::workspace_bytes(int64_t lda) {
     const size_t mv32 = (size_t)lda * KP * sizeof(float);
     const int KSmax = 32;
     const int nrb = (int)(lda / kRedRows);
-    // 19 sub-buffers, each rounded up to 256 bytes; the FP32 partials hold
+    // 20 sub-buffers, each rounded up to 256 bytes; the FP32 partials hold
     // one slice more than the K split (the bf16x2 near-field slice)
     return 4 * mv64 + 11 * mv32 + (size_t)lda * sizeof(float) + (KSmax + 1) * mv32 + KSmax * mv64 +
-           (size_t)4 * nrb * KP * sizeof(double) + KP * Q_NQ * sizeof(double) + KP * sizeof(int) + 21 * 256;
+           (size_t)lda * 4 * sizeof(double) +
+           (size_t)4 * nrb * KP * sizeof(double) + KP * Q_NQ * sizeof(double) + KP * sizeof(int) + 22 * 256;
 }
 
 // per-column scalars, then 64-int blocks: the sweep flags and two read-back
@@ -1291,6 +1323,7 @@ BatchedSolver::BatchedSolver(int n, int64_t lda, int capacity, void* ws, void* h
     part32_ = (float*)take((KSmax + 1) * mv32);
     part64_ = (double*)take(KSmax * mv64);
     dpart_ = (double*)take((size_t)4 * nrb_ * KP * sizeof(double));
+    bn_ = (double*)take((size_t)lda * 4 * sizeof(double));
     sc_ = (double*)take(KP * Q_NQ * sizeof(double));
     act_ = (int*)take(KP * sizeof(int));
     CP_HIP(hipMemsetAsync(y_, 0, mv32, s));
@@ -1534,6 +1567,15 @@ void BatchedSolver::matvec32(const float* A32, const float* P, float* Y, hipStre
     }
 }
 
+void BatchedSolver::copy64(const double* B, double* Y, hipStream_t s) {
+    // the reduction with no K slices: Y = B (rows >= n zeroed) and the
+    // per-block column partials of ||Y_c||^2
+    hipLaunchKernelGGL(reduce64_kernel, dim3((unsigned)nrb_), dim3(kT), 0, s, (const double*)nullptr, 0, lda_, n_, B,
+                       1.0, Y, dpart_, (const int*)nullptr, (const int*)nullptr, (const double*)nullptr,
+                       (const double*)nullptr);
+    CP_LAUNCH_CHECK();
+}
+
 void BatchedSolver::matvec64acc(const float* A32, const double* X, double* Y, const double* Badd, double sign,
                                 hipStream_t s, const double* corr, const double* A64) {
     // K slices: enough workgroups for two per CU, no more (each slice adds an
@@ -1607,13 +1649,30 @@ void BatchedSolver::tile_solve(const float* A32, const float* B32, const double*
     const unsigned nel = nblocks(rows * KP);
     int* host_sweep = host_act_ + KP;
 
-    // sigma multivector and RHS b = add - (B32 + dB) sigma (FP64 accumulation)
-    if (sig_cm)
-        hipLaunchKernelGGL(gather_cols_kernel, dim3(nel), dim3(kT), 0, s, n_, lda_, kc, sig_cm, ldx, S_);
-    else
-        hipLaunchKernelGGL(sigma_rows_kernel, dim3(nel), dim3(kT), 0, s, n_, lda_, kc, normals, vinf4, S_);
-    if (add_cm) hipLaunchKernelGGL(gather_cols_kernel, dim3(nel), dim3(kT), 0, s, n_, lda_, kc, add_cm, ldx, R_);
-    matvec64acc(B32, S_, Bv_, add_cm ? R_ : nullptr, -1.0, s, near_dB_, b64_);
+    // RHS b = add - B sigma with FP64 accumulation (B: the FP64 matrix, or
+    // B32 + the near-field correction)
+    if (!sig_cm && !add_cm) {
+        // steady sources sigma_c = -Vinf_c . n: b_c is a combination of the
+        // three columns -B n_x, -B n_y, -B n_z, computed once per matrix
+        const int mode = b64_ ? 1 : 0;
+        if (bn_mode_ != mode) {
+            hipLaunchKernelGGL(normal_cols_kernel, dim3(nel), dim3(kT), 0, s, n_, lda_, normals, S_);
+            matvec64acc(B32, S_, R_, nullptr, -1.0, s, near_dB_, b64_);
+            hipLaunchKernelGGL(keep3_kernel, dim3(nblocks(lda_)), dim3(kT), 0, s, lda_, (const double*)R_, bn_);
+            CP_LAUNCH_CHECK();
+            bn_mode_ = mode;
+        }
+        hipLaunchKernelGGL(rhs_bn_kernel, dim3(nel), dim3(kT), 0, s, n_, lda_, kc, (const double*)bn_, vinf4, R_);
+        copy64(R_, Bv_, s);
+    } else {
+        if (sig_cm)
+            hipLaunchKernelGGL(gather_cols_kernel, dim3(nel), dim3(kT), 0, s, n_, lda_, kc, sig_cm, ldx, S_);
+        else
+            hipLaunchKernelGGL(sigma_rows_kernel, dim3(nel), dim3(kT), 0, s, n_, lda_, kc, normals, vinf4, S_);
+        if (add_cm)
+            hipLaunchKernelGGL(gather_cols_kernel, dim3(nel), dim3(kT), 0, s, n_, lda_, kc, add_cm, ldx, R_);
+        matvec64acc(B32, S_, Bv_, add_cm ? R_ : nullptr, -1.0, s, near_dB_, b64_);
+    }
     auto colsum = [&](const double* dp, int nq, int q0, int q1, int op = COLSUM_NONE, double tol2 = 0) {
         hipLaunchKernelGGL(colsum_kernel, dim3(1), dim3(1024), 0, s, dp, nrb_, nq, q0, q1, sc_, op, act_, tol2);
         CP_LAUNCH_CHECK();
@@ -1640,8 +1699,11 @@ void BatchedSolver::tile_solve(const float* A32, const float* B32, const double*
     int sweep = 0;
     std::vector<double> rel(KP, 0.0);
     while (true) {
-        // residual R = b - A32 X in FP64
-        matvec64acc(A32, X_, R_, Bv_, -1.0, s, near_dA_, a64_);
+        // residual R = b - A X in FP64 (R = b while X is still zero)
+        if (sweep == 0 && !x0_cm)
+            copy64(Bv_, R_, s);
+        else
+            matvec64acc(A32, X_, R_, Bv_, -1.0, s, near_dA_, a64_);
         colsum(dpart_, 1, Q_RR64, 0);
         CP_HIP(hipMemcpyAsync(host_sc_, sc_, KP * Q_NQ * sizeof(double), hipMemcpyDeviceToHost, s));
         CP_HIP(hipStreamSynchronize(s));
