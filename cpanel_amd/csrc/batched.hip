@@ -1191,48 +1191,57 @@ __global__ __launch_bounds__(256) void bj_apply_mfma_kernel(const int* __restric
     }
 }
 
-// Block product from the block-ordered multivector PP (row b0 + s of PP is
-// row perm[b0 + s] of P): same tiling as bj_apply_mfma_kernel, contiguous
-// operand loads, next chunk loaded before the current chunk's MFMAs.
+// Block-Jacobi product from the block-ordered multivector PP (inner_p /
+// inner_s write it): Y(perm row) = inv_q * PP rows of block q, on FP32
+// matrix cores (v_mfma_f32_32x32x2f32).  A wavefront owns 32 rows of one
+// block and all 64 columns (two 32 x 32 accumulators), so every inverse entry
+// is read once per product; a workgroup covers 128 rows of a block.  The
+// K loop loads 2U columns of the inverse and the matching PP rows one step
+// ahead of the matrix-core ops.
 template <int U>
 __global__ __launch_bounds__(256) void bj_apply_pp_kernel(const int* __restrict__ perm, const int* __restrict__ ptr,
                                                           const int64_t* __restrict__ off,
                                                           const float* __restrict__ inv,
-                                                          const float* __restrict__ PP, float* __restrict__ Y, const int* __restrict__ gate) {
+                                                          const float* __restrict__ PP, float* __restrict__ Y,
+                                                          const int* __restrict__ gate) {
     if (gate && gate_idle(gate)) return;
     const int q = blockIdx.y;
     const int b0 = ptr[q], b = ptr[q + 1] - b0;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int r0 = blockIdx.x * 64 + 32 * (wave >> 1);
+    const int r0 = blockIdx.x * 128 + 32 * wave;
     if (r0 >= b) return;  // no LDS, no barrier: idle waves leave
-    const int c = 32 * (wave & 1) + (lane & 31);
-    const int kh = lane >> 5;
-    const int r = r0 + (lane & 31);
+    const int cl = lane & 31, kh = lane >> 5;
+    const int r = r0 + cl;
     const bool rok = r < b;
     const float* M = inv + off[q];
     const float* Pq = PP + (int64_t)b0 * KP;
-    floatx16 acc = floatx16{};
-    float a[U], x[U], an[U], xn[U];
-    auto load = [&](int k, float (&aa)[U], float (&xx)[U]) {
+    floatx16 acc0 = floatx16{}, acc1 = floatx16{};
+    float a[U], x0[U], x1[U], an[U], xn0[U], xn1[U];
+    auto load = [&](int k, float (&aa)[U], float (&y0)[U], float (&y1)[U]) {
 #pragma unroll
         for (int u = 0; u < U; u++) {
             const int kk = k + 2 * u + kh;
             const bool ok = kk < b;
             aa[u] = (ok && rok) ? M[(int64_t)kk * b + r] : 0.f;
-            xx[u] = ok ? Pq[(int64_t)kk * KP + c] : 0.f;
+            y0[u] = ok ? Pq[(int64_t)kk * KP + cl] : 0.f;
+            y1[u] = ok ? Pq[(int64_t)kk * KP + 32 + cl] : 0.f;
         }
     };
-    load(0, a, x);
+    load(0, a, x0, x1);
     for (int k = 0; k < b; k += 2 * U) {
         const bool more = k + 2 * U < b;
-        if (more) load(k + 2 * U, an, xn);
+        if (more) load(k + 2 * U, an, xn0, xn1);
 #pragma unroll
-        for (int u = 0; u < U; u++) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a[u], x[u], acc, 0, 0, 0);
+        for (int u = 0; u < U; u++) {
+            acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a[u], x0[u], acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a[u], x1[u], acc1, 0, 0, 0);
+        }
         if (more) {
 #pragma unroll
             for (int u = 0; u < U; u++) {
                 a[u] = an[u];
-                x[u] = xn[u];
+                x0[u] = xn0[u];
+                x1[u] = xn1[u];
             }
         }
     }
@@ -1240,7 +1249,10 @@ __global__ __launch_bounds__(256) void bj_apply_pp_kernel(const int* __restrict_
 #pragma unroll
     for (int j = 0; j < 16; j++) {
         const int rr = r0 + 8 * (j >> 2) + 4 * kh + (j & 3);
-        if (rr < b) Y[(int64_t)pq[rr] * KP + c] = acc[j];
+        if (rr < b) {
+            Y[(int64_t)pq[rr] * KP + cl] = acc0[j];
+            Y[(int64_t)pq[rr] * KP + 32 + cl] = acc1[j];
+        }
     }
 }
 
@@ -1608,7 +1620,7 @@ void BatchedSolver::matvec64acc(const float* A32, const double* X, double* Y, co
 void BatchedSolver::apply_block_jacobi(const float* P, float* Y, hipStream_t s) {
     TimedScope ts("bj_apply", s);
     if (bj_mfma_ && bj_iperm_) {  // P was also written in block order to pp_
-        dim3 grid((unsigned)((bj_maxb_ + 63) / 64), (unsigned)bj_nblocks_);
+        dim3 grid((unsigned)((bj_maxb_ + 127) / 128), (unsigned)bj_nblocks_);
         hipLaunchKernelGGL(bj_apply_pp_kernel<8>, grid, dim3(kT), 0, s, bj_perm_, bj_ptr_, bj_off_, bj_inv_,
                            (const float*)pp_, Y, gate_);
     } else if (bj_mfma_) {
