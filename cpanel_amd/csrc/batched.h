@@ -130,6 +130,8 @@ private:
     void apply_block_jacobi(const float* P, float* Y, hipStream_t s);
     // Y = B (rows >= n zeroed) with the column partials of ||Y_c||^2 in dpart_
     void copy64(const double* B, double* Y, hipStream_t s);
+    // inner solve of one refinement sweep by right-preconditioned GMRES
+    void gmres_sweep(const float* A32, const BatchedOpts& o, int kc, SolveStats* stats, hipStream_t s);
     const double* a64_ = nullptr;
     const double* b64_ = nullptr;
     const void* apk_ = nullptr;
@@ -168,6 +170,14 @@ private:
     double* dpart_;     // nrb x 64 x 4 partial column dots
     double* bn_;        // lda x 4: -B n_x, -B n_y, -B n_z (steady right-hand sides)
     int bn_mode_ = -1;  // matrix bn_ was formed with: 1 FP64 B, 0 B32 + near field, -1 none
+    // GMRES inner solve (with the block-Jacobi preconditioner): basis in
+    // block order, per-column state, dot partials; Krylov dimension gm_m_
+    bool gmres_ = true;
+    int gm_m_ = 40;
+    float* gm_u_ = nullptr;
+    double* gm_st_ = nullptr;
+    double* gm_dpart_ = nullptr;
+    float* gm_coef_ = nullptr;  // [t][column] h_{t,j} / s_t of the current iteration
     double* sc_;        // 64 x 16 per-column scalars
     int* act_;          // 64 active flags
     const int* near_rowptr_ = nullptr;

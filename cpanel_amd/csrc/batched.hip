@@ -1072,6 +1072,248 @@ __global__ __launch_bounds__(256) void inner_x_kernel(int n, const double* __res
 }
 
 // X += scale_c * d  (FP64 accumulation of the FP32 correction)
+// ---------------------------------------------------------------- GMRES
+// Right-preconditioned GMRES per column, the inner solve used with the strip
+// block-Jacobi preconditioner: one product per iteration against BiCGSTAB's
+// two, and the minimal residual over the Krylov space (a numpy restatement
+// of the 4,956-panel case needs 86 products per column for the four
+// refinement sweeps where the batched BiCGSTAB takes 182).  The basis
+// vectors u_q are stored unnormalised in block order (the preconditioner's
+// row order) with their norms s_q, v_q = u_q / s_q.  Per column a state
+// block of kGmStride doubles holds the Hessenberg columns (already rotated),
+// the Givens rotations, the rotated right-hand side g, the norms, the
+// solution coefficients y and the iteration count.  A sweep that reaches
+// the Krylov dimension simply ends: the FP64 refinement is the restart.
+constexpr int kGmMax = 48;                      // Krylov dimension bound per sweep
+constexpr int GM_H = 0;                         // column j of H at GM_H + j * (kGmMax + 1)
+constexpr int GM_CS = GM_H + (kGmMax + 1) * kGmMax;
+constexpr int GM_SN = GM_CS + kGmMax;
+constexpr int GM_G = GM_SN + kGmMax;            // kGmMax + 1
+constexpr int GM_S = GM_G + kGmMax + 1;         // kGmMax + 1 norms s_q
+constexpr int GM_Y = GM_S + kGmMax + 1;         // kGmMax coefficients
+constexpr int GM_C = GM_Y + kGmMax;             // kGmMax + 1: h_{q,j} / s_q of the current j
+constexpr int GM_NJ = GM_C + kGmMax + 1;        // iterations taken
+constexpr int kGmStride = GM_NJ + 1;
+
+// u_0 = R / scale in block order on the sweeping columns; ||u_0||^2 partials
+__global__ __launch_bounds__(256) void gm_init_kernel(int n, const double* __restrict__ R,
+                                                      const double* __restrict__ sc, const int* __restrict__ act,
+                                                      const int* __restrict__ iperm, float* __restrict__ U0,
+                                                      double* __restrict__ dpart) {
+    __shared__ double sh[kT];
+    const int c = threadIdx.x & 63, g = threadIdx.x >> 6;
+    const double scale = sc[c * Q_NQ + Q_SCALE];
+    const double inv = scale > 0 ? 1.0 / scale : 0.0;
+    double acc = 0;
+    for (int rr = g; rr < kRedRows; rr += 4) {
+        const int64_t i = (int64_t)blockIdx.x * kRedRows + rr;
+        if (i >= n) break;
+        const float u = act[c] ? (float)(R[i * KP + c] * inv) : 0.f;
+        U0[(int64_t)iperm[i] * KP + c] = u;
+        acc += (double)u * u;
+    }
+    col_partial(acc, sh, dpart + (int64_t)blockIdx.x * KP);
+}
+
+// sum of the per-block partials of quantity q (fixed order) for column c,
+// with 1024 threads: g = t / 64 sums blocks g, g + 16, ...; thread group 0
+// adds the 16 group sums and returns the total (other threads: 0)
+__device__ inline double colsum1024(const double* __restrict__ d, int nblk, double* sh) {
+    const int c = threadIdx.x & 63, g = threadIdx.x >> 6;
+    double s = 0;
+    for (int b = g; b < nblk; b += 16) s += d[(int64_t)b * KP + c];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    double t = 0;
+    if (g == 0)
+        for (int k = 0; k < 16; k++) t += sh[k * 64 + c];
+    __syncthreads();
+    return t;
+}
+
+// s_0 = ||u_0||, g_0 = s_0, no iterations yet
+__global__ __launch_bounds__(1024) void gm_start_kernel(const double* __restrict__ dpart, int nblk,
+                                                        double* __restrict__ st) {
+    __shared__ double sh[1024];
+    const double t = colsum1024(dpart, nblk, sh);
+    if (threadIdx.x >= KP) return;
+    double* S = st + threadIdx.x * kGmStride;
+    S[GM_S] = sqrt(t);
+    S[GM_G] = sqrt(t);
+    S[GM_NJ] = 0;
+}
+
+// w = sum_ks part (the product A~ z_j, row order) stored as u_{j+1} in block
+// order on the active columns
+__global__ __launch_bounds__(256) void gm_w_kernel(const float* __restrict__ part, int KS, int64_t lda, int n,
+                                                   const int* __restrict__ act, const int* __restrict__ iperm,
+                                                   float* __restrict__ W) {
+    const int64_t idx = (int64_t)blockIdx.x * kT + threadIdx.x;
+    if (idx >= (int64_t)n * KP) return;
+    const int c = (int)(idx & 63);
+    if (!act[c]) return;
+    float v = 0.f;
+#pragma unroll 8
+    for (int q = 0; q < KS; q++) v += part[(int64_t)q * lda * KP + idx];
+    W[(int64_t)iperm[idx >> 6] * KP + c] = v;
+}
+
+// dots u_t . u_{j+1}: one wavefront per (chunk of kGmDotRows block-order
+// positions, t <= j), lane = column; eight position pairs in flight, each
+// wavefront writes its own partial (no block reduction)
+constexpr int kGmDotRows = 256;
+__global__ __launch_bounds__(256) void gm_dots_kernel(int n, const int* __restrict__ act, const float* __restrict__ U,
+                                                      int64_t us, int j, double* __restrict__ dpart) {
+    const int c = threadIdx.x & 63, g = threadIdx.x >> 6;
+    const int t = blockIdx.y * 4 + g;
+    if (t > j) return;  // whole wavefront
+    const int64_t chunk = blockIdx.x;
+    double acc = 0;
+    if (act[c]) {
+        const float* ut = U + (int64_t)t * us + c;
+        const float* w = U + (int64_t)(j + 1) * us + c;
+        const int64_t p1 = std::min((int64_t)n, (chunk + 1) * kGmDotRows);
+        int64_t p = chunk * kGmDotRows;
+        for (; p + 8 <= p1; p += 8) {
+            float a[8], b[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                a[k] = ut[(p + k) * KP];
+                b[k] = w[(p + k) * KP];
+            }
+#pragma unroll
+            for (int k = 0; k < 8; k++) acc += (double)a[k] * b[k];
+        }
+        for (; p < p1; p++) acc += (double)ut[p * KP] * w[p * KP];
+    }
+    dpart[((int64_t)t * gridDim.x + chunk) * KP + c] = acc;
+}
+
+// Hessenberg entries h_{q,j} = (u_q . w) / s_q, one quantity q per block
+__global__ __launch_bounds__(1024) void gm_hcol_kernel(const double* __restrict__ dpart, int nblk,
+                                                       const int* __restrict__ act, int j, double* __restrict__ st,
+                                                       float* __restrict__ coef) {
+    __shared__ double sh[1024];
+    const int q = blockIdx.x;
+    const double t = colsum1024(dpart + (int64_t)q * nblk * KP, nblk, sh);
+    if (threadIdx.x >= KP || !act[threadIdx.x]) return;
+    double* S = st + threadIdx.x * kGmStride;
+    const double h = t / S[GM_S + q];
+    S[GM_H + j * (kGmMax + 1) + q] = h;
+    coef[q * KP + threadIdx.x] = (float)(h / S[GM_S + q]);
+}
+
+// u_{j+1} = w - sum_{q <= j} h_{q,j} v_q on the active columns (coef[t][c] =
+// h_{t,j} / s_t), and the partials of ||u_{j+1}||^2: kGmUpPos block-order
+// positions per thread, four basis vectors at a time (16 loads in flight),
+// 4 * kGmUpPos positions per workgroup
+constexpr int kGmUpPos = 4;
+__global__ __launch_bounds__(256) void gm_update_kernel(int n, const int* __restrict__ act, float* __restrict__ U,
+                                                        int64_t us, int j, const float* __restrict__ coef,
+                                                        double* __restrict__ dpart) {
+    __shared__ double sh[kT];
+    const int c = threadIdx.x & 63, g = threadIdx.x >> 6;
+    const int64_t p0 = ((int64_t)blockIdx.x * 4 + g) * kGmUpPos;
+    double acc = 0;
+    if (act[c]) {
+        float* w = U + (int64_t)(j + 1) * us + c;
+        float v[kGmUpPos];
+#pragma unroll
+        for (int k = 0; k < kGmUpPos; k++) v[k] = p0 + k < n ? w[(p0 + k) * KP] : 0.f;
+        int t = 0;
+        for (; t + 4 <= j + 1; t += 4) {
+            float f[4], x[4][kGmUpPos];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                f[q] = coef[(t + q) * KP + c];
+                const float* ut = U + (int64_t)(t + q) * us + c;
+#pragma unroll
+                for (int k = 0; k < kGmUpPos; k++) x[q][k] = p0 + k < n ? ut[(p0 + k) * KP] : 0.f;
+            }
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+#pragma unroll
+                for (int k = 0; k < kGmUpPos; k++) v[k] -= f[q] * x[q][k];
+        }
+        for (; t <= j; t++) {
+            const float f = coef[t * KP + c];
+            const float* ut = U + (int64_t)t * us + c;
+#pragma unroll
+            for (int k = 0; k < kGmUpPos; k++)
+                if (p0 + k < n) v[k] -= f * ut[(p0 + k) * KP];
+        }
+#pragma unroll
+        for (int k = 0; k < kGmUpPos; k++)
+            if (p0 + k < n) {
+                w[(p0 + k) * KP] = v[k];
+                acc += (double)v[k] * v[k];
+            }
+    }
+    col_partial(acc, sh, dpart + (int64_t)blockIdx.x * KP);
+}
+
+// h_{j+1,j} = ||u_{j+1}||; the earlier rotations and a new one applied to
+// column j; g rotated; a column stops when |g_{j+1}| <= tol s_0, at the
+// Krylov bound m, or on a breakdown (h_{j+1,j} = 0: the solution is exact)
+__global__ __launch_bounds__(1024) void gm_givens_kernel(const double* __restrict__ dpart, int nblk,
+                                                         int* __restrict__ act, int j, int m, double tol,
+                                                         double* __restrict__ st) {
+    __shared__ double sh[1024];
+    const double t = colsum1024(dpart, nblk, sh);
+    if (threadIdx.x >= KP || !act[threadIdx.x]) return;
+    double* S = st + threadIdx.x * kGmStride;
+    double* h = S + GM_H + j * (kGmMax + 1);
+    const double hn = sqrt(t);
+    for (int i = 0; i < j; i++) {
+        const double cs = S[GM_CS + i], sn = S[GM_SN + i];
+        const double a = cs * h[i] + sn * h[i + 1];
+        h[i + 1] = -sn * h[i] + cs * h[i + 1];
+        h[i] = a;
+    }
+    const double den = sqrt(h[j] * h[j] + hn * hn);
+    const double cs = den > 0 ? h[j] / den : 1.0, sn = den > 0 ? hn / den : 0.0;
+    S[GM_CS + j] = cs;
+    S[GM_SN + j] = sn;
+    h[j] = den;
+    h[j + 1] = 0;
+    const double gj = S[GM_G + j];
+    S[GM_G + j + 1] = -sn * gj;
+    S[GM_G + j] = cs * gj;
+    S[GM_S + j + 1] = hn;
+    S[GM_NJ] = j + 1;
+    if (!(fabs(S[GM_G + j + 1]) > tol * S[GM_S]) || j + 1 >= m || !(hn > 0)) act[threadIdx.x] = 0;
+}
+
+// y = H^-1 g over the iterations each column took (upper triangular)
+__global__ void gm_backsub_kernel(double* __restrict__ st) {
+    const int c = threadIdx.x;
+    if (c >= KP) return;
+    double* S = st + c * kGmStride;
+    const int nj = (int)S[GM_NJ];
+    for (int i = nj - 1; i >= 0; i--) {
+        double v = S[GM_G + i];
+        for (int k = i + 1; k < nj; k++) v -= S[GM_H + k * (kGmMax + 1) + i] * S[GM_Y + k];
+        const double d = S[GM_H + i * (kGmMax + 1) + i];
+        S[GM_Y + i] = d != 0 ? v / d : 0.0;
+    }
+    // the combination reads y_q / s_q
+    for (int i = 0; i < nj; i++) S[GM_Y + i] /= S[GM_S + i];
+}
+
+// T = sum_q y_q v_q in block order (zero on columns without iterations;
+// GM_Y holds y_q / s_q after gm_backsub_kernel)
+__global__ __launch_bounds__(256) void gm_combine_kernel(int n, const float* __restrict__ U, int64_t us,
+                                                         const double* __restrict__ st, float* __restrict__ T) {
+    const int64_t idx = (int64_t)blockIdx.x * kT + threadIdx.x;
+    if (idx >= (int64_t)n * KP) return;
+    const int c = (int)(idx & 63);
+    const double* S = st + c * kGmStride;
+    const int nj = (int)S[GM_NJ];
+    float v = 0.f;
+    for (int q = 0; q < nj; q++) v += (float)S[GM_Y + q] * U[(int64_t)q * us + idx];
+    T[idx] = v;
+}
+
 __global__ __launch_bounds__(256) void correct_kernel(int64_t rows, const double* __restrict__ sc,
                                                       const int* __restrict__ sweep, double* __restrict__ X,
                                                       const float* __restrict__ d) {
@@ -1203,6 +1445,7 @@ __global__ __launch_bounds__(256) void bj_apply_pp_kernel(const int* __restrict_
                                                           const int64_t* __restrict__ off,
                                                           const float* __restrict__ inv,
                                                           const float* __restrict__ PP, float* __restrict__ Y,
+                                                          const double* __restrict__ cscale, int cstride,
                                                           const int* __restrict__ gate) {
     if (gate && gate_idle(gate)) return;
     const int q = blockIdx.y;
@@ -1245,13 +1488,20 @@ __global__ __launch_bounds__(256) void bj_apply_pp_kernel(const int* __restrict_
             }
         }
     }
+    // optional per-column factor 1 / cscale[c * cstride] (0 where that is 0)
+    float f0 = 1.f, f1 = 1.f;
+    if (cscale) {
+        const double s0 = cscale[(int64_t)cl * cstride], s1 = cscale[(int64_t)(32 + cl) * cstride];
+        f0 = s0 != 0 ? (float)(1.0 / s0) : 0.f;
+        f1 = s1 != 0 ? (float)(1.0 / s1) : 0.f;
+    }
     const int* pq = perm + b0;
 #pragma unroll
     for (int j = 0; j < 16; j++) {
         const int rr = r0 + 8 * (j >> 2) + 4 * kh + (j & 3);
         if (rr < b) {
-            Y[(int64_t)pq[rr] * KP + cl] = acc0[j];
-            Y[(int64_t)pq[rr] * KP + 32 + cl] = acc1[j];
+            Y[(int64_t)pq[rr] * KP + cl] = acc0[j] * f0;
+            Y[(int64_t)pq[rr] * KP + 32 + cl] = acc1[j] * f1;
         }
     }
 }
@@ -1299,11 +1549,12 @@ size_t BatchedSolver::workspace_bytes(int64_t lda) {
     const size_t mv32 = (size_t)lda * KP * sizeof(float);
     const int KSmax = 32;
     const int nrb = (int)(lda / kRedRows);
-    // 20 sub-buffers, each rounded up to 256 bytes; the FP32 partials hold
+    // 24 sub-buffers, each rounded up to 256 bytes; the FP32 partials hold
     // one slice more than the K split (the bf16x2 near-field slice)
     return 4 * mv64 + 11 * mv32 + (size_t)lda * sizeof(float) + (KSmax + 1) * mv32 + KSmax * mv64 +
-           (size_t)lda * 4 * sizeof(double) +
-           (size_t)4 * nrb * KP * sizeof(double) + KP * Q_NQ * sizeof(double) + KP * sizeof(int) + 22 * 256;
+           (size_t)lda * 4 * sizeof(double) + (kGmMax + 1) * mv32 + (size_t)KP * kGmStride * sizeof(double) +
+           (size_t)(kGmMax + 1) * nrb * KP * sizeof(double) + (size_t)(kGmMax + 1) * KP * sizeof(float) +
+           (size_t)4 * nrb * KP * sizeof(double) + KP * Q_NQ * sizeof(double) + KP * sizeof(int) + 26 * 256;
 }
 
 // per-column scalars, then 64-int blocks: the sweep flags and two read-back
@@ -1336,6 +1587,10 @@ BatchedSolver::BatchedSolver(int n, int64_t lda, int capacity, void* ws, void* h
     part64_ = (double*)take(KSmax * mv64);
     dpart_ = (double*)take((size_t)4 * nrb_ * KP * sizeof(double));
     bn_ = (double*)take((size_t)lda * 4 * sizeof(double));
+    gm_u_ = (float*)take((kGmMax + 1) * mv32);
+    gm_st_ = (double*)take((size_t)KP * kGmStride * sizeof(double));
+    gm_dpart_ = (double*)take((size_t)(kGmMax + 1) * nrb_ * KP * sizeof(double));
+    gm_coef_ = (float*)take((size_t)(kGmMax + 1) * KP * sizeof(float));
     sc_ = (double*)take(KP * Q_NQ * sizeof(double));
     act_ = (int*)take(KP * sizeof(int));
     CP_HIP(hipMemsetAsync(y_, 0, mv32, s));
@@ -1358,6 +1613,13 @@ BatchedSolver::BatchedSolver(int n, int64_t lda, int capacity, void* ws, void* h
     if (const char* e = std::getenv("CPGPU_F64_SIMPLE")) f64_pipelined_ = e[0] != '1';
     // CPGPU_NO_BF16X3=1 keeps the inner product in FP32 matrix-core arithmetic
     if (const char* e = std::getenv("CPGPU_NO_BF16X3")) bf16x3_ = e[0] != '1';
+    // CPGPU_INNER=bicgstab keeps BiCGSTAB as the inner solver with the
+    // block-Jacobi preconditioner; CPGPU_GMRES_M sets the Krylov dimension
+    if (const char* e = std::getenv("CPGPU_INNER")) gmres_ = std::string(e) != "bicgstab";
+    if (const char* e = std::getenv("CPGPU_GMRES_M")) {
+        const int v = std::atoi(e);
+        if (v >= 4 && v <= kGmMax) gm_m_ = v;
+    }
     // CPGPU_NO_BF16X2=1 keeps the bf16x2 product out of the tuner's candidates
     if (const char* e = std::getenv("CPGPU_NO_BF16X2")) bf16x2_ = e[0] != '1';
     // CPGPU_BJ_VALU=1 selects the vector-ALU block-Jacobi product (tests)
@@ -1622,7 +1884,7 @@ void BatchedSolver::apply_block_jacobi(const float* P, float* Y, hipStream_t s) 
     if (bj_mfma_ && bj_iperm_) {  // P was also written in block order to pp_
         dim3 grid((unsigned)((bj_maxb_ + 127) / 128), (unsigned)bj_nblocks_);
         hipLaunchKernelGGL(bj_apply_pp_kernel<8>, grid, dim3(kT), 0, s, bj_perm_, bj_ptr_, bj_off_, bj_inv_,
-                           (const float*)pp_, Y, gate_);
+                           (const float*)pp_, Y, (const double*)nullptr, 0, gate_);
     } else if (bj_mfma_) {
         dim3 grid((unsigned)((bj_maxb_ + 63) / 64), (unsigned)bj_nblocks_);
         hipLaunchKernelGGL(bj_apply_mfma_kernel, grid, dim3(kT), 0, s, bj_perm_, bj_ptr_, bj_off_, bj_inv_, P, Y, gate_);
@@ -1632,6 +1894,79 @@ void BatchedSolver::apply_block_jacobi(const float* P, float* Y, hipStream_t s) 
                            bj_ptr_, bj_off_, bj_inv_, P, Y, gate_);
     }
     CP_LAUNCH_CHECK();
+}
+
+// One refinement sweep's inner solve by right-preconditioned GMRES (see the
+// gm_* kernels): R / |R| on the sweeping columns (act_), at most gm_m_
+// iterations, d_ = M^-1 V y on return.  The host reads iteration it's active
+// flags while iteration it is queued behind them, as in the BiCGSTAB loop.
+void BatchedSolver::gmres_sweep(const float* A32, const BatchedOpts& o, int kc, SolveStats* stats, hipStream_t s) {
+    const int64_t us = lda_ * KP;
+    int m = gm_m_;
+    if (o.max_iter > 0) m = std::min(m, o.max_iter);
+    hipLaunchKernelGGL(gm_init_kernel, dim3((unsigned)nrb_), dim3(kT), 0, s, n_, (const double*)R_,
+                       (const double*)sc_, (const int*)act_, bj_iperm_, gm_u_, dpart_);
+    hipLaunchKernelGGL(gm_start_kernel, dim3(1), dim3(1024), 0, s, (const double*)dpart_, nrb_, gm_st_);
+    CP_LAUNCH_CHECK();
+    int* hact[2] = {host_act_, host_act_ + 2 * KP};  // host_act_ + KP holds the sweep flags
+    auto top = [&](int k) {
+        CP_HIP(hipMemcpyAsync(hact[k & 1], act_, KP * sizeof(int), hipMemcpyDeviceToHost, s));
+        CP_HIP(hipEventRecord(act_ev_[k & 1], s));
+    };
+    auto body = [&](int j) {
+        {  // z = M^-1 v_j (row order)
+            TimedScope tb("bj_apply", s);
+            dim3 grid((unsigned)((bj_maxb_ + 127) / 128), (unsigned)bj_nblocks_);
+            hipLaunchKernelGGL(bj_apply_pp_kernel<8>, grid, dim3(kT), 0, s, bj_perm_, bj_ptr_, bj_off_, bj_inv_,
+                               (const float*)(gm_u_ + j * us), y_, (const double*)(gm_st_ + GM_S + j), kGmStride,
+                               gate_);
+        }
+        matvec32(A32, y_, nullptr, s);  // w = A~ z
+        const int KS = slices(shape_);
+        const int ndc = (int)((n_ + kGmDotRows - 1) / kGmDotRows);     // position chunks of the dots
+        const int nuc = (int)((n_ + 4 * kGmUpPos - 1) / (4 * kGmUpPos));  // workgroups of the update
+        hipLaunchKernelGGL(gm_w_kernel, dim3(nblocks((int64_t)n_ * KP)), dim3(kT), 0, s, (const float*)part32_, KS,
+                           lda_, n_, (const int*)act_, bj_iperm_, gm_u_ + (j + 1) * us);
+        hipLaunchKernelGGL(gm_dots_kernel, dim3((unsigned)ndc, (unsigned)((j + 4) / 4)), dim3(kT), 0, s, n_,
+                           (const int*)act_, (const float*)gm_u_, us, j, gm_dpart_);
+        hipLaunchKernelGGL(gm_hcol_kernel, dim3((unsigned)(j + 1)), dim3(1024), 0, s, (const double*)gm_dpart_, ndc,
+                           (const int*)act_, j, gm_st_, gm_coef_);
+        hipLaunchKernelGGL(gm_update_kernel, dim3((unsigned)nuc), dim3(kT), 0, s, n_, (const int*)act_, gm_u_, us, j,
+                           (const float*)gm_coef_, dpart_);
+        hipLaunchKernelGGL(gm_givens_kernel, dim3(1), dim3(1024), 0, s, (const double*)dpart_, nuc, act_, j, m,
+                           o.inner_tol, gm_st_);
+        CP_LAUNCH_CHECK();
+    };
+    gate_ = act_;
+    top(0);
+    int it = 0;
+    for (;;) {
+        const bool more = it < m;
+        if (more) {
+            body(it);
+            top(it + 1);
+        }
+        CP_HIP(hipEventSynchronize(act_ev_[it & 1]));
+        const int* fl = hact[it & 1];
+        int nact = 0;
+        for (int c = 0; c < KP; c++) nact += fl[c] != 0;
+        if (!more || nact == 0) {
+            if (more)
+                if (KernelTimer* t = current_timer())
+                    for (const char* nm : {"gemm_f32", "pack_p", "bj_apply"}) t->discard(nm, 1);
+            break;
+        }
+        for (int c = 0; c < kc; c++)
+            if (fl[c]) stats[c].iters++;
+        ++it;
+    }
+    gate_ = nullptr;
+    // d = M^-1 sum_q y_q v_q
+    hipLaunchKernelGGL(gm_backsub_kernel, dim3(1), dim3(KP), 0, s, gm_st_);
+    hipLaunchKernelGGL(gm_combine_kernel, dim3(nblocks((int64_t)n_ * KP)), dim3(kT), 0, s, n_, (const float*)gm_u_,
+                       us, (const double*)gm_st_, pp_);
+    CP_LAUNCH_CHECK();
+    apply_block_jacobi(nullptr, d_, s);
 }
 
 void BatchedSolver::solve(const float* A32, const float* B32, const double* normals, const double* vinf4, int k,
@@ -1740,79 +2075,83 @@ void BatchedSolver::tile_solve(const float* A32, const float* B32, const double*
         // inner FP32 solve of A32 d = R / |R| on the sweeping columns
         CP_HIP(hipMemcpyAsync(sc_, host_sc_, KP * Q_NQ * sizeof(double), hipMemcpyHostToDevice, s));
         CP_HIP(hipMemcpyAsync(act_, host_sweep, KP * sizeof(int), hipMemcpyHostToDevice, s));
-        hipLaunchKernelGGL(inner_init_kernel, dim3((unsigned)nrb_), dim3(kT), 0, s, n_, (const double*)R_,
-                           (const double*)sc_, (const int*)act_, r_, r0_, p_, v_, d_, dpart_);
-        colsum(dpart_, 1, Q_RR, 0);
-        hipLaunchKernelGGL(scal_init_kernel, dim3(1), dim3(KP), 0, s, sc_);
-        int it = 0;
-        const int maxit = o.max_iter > 0 ? o.max_iter : 2 * n_;
-        // The host reads iteration it's active flags while iteration it is
-        // already queued behind them, so the device never waits for the
-        // host's check.  An iteration queued past the end of its sweep finds
-        // no active column: its heavy kernels return at once (gate_) and the
-        // per-column updates are masked as for any finished column.
-        int* hact[2] = {host_act_, host_act_ + 2 * KP};  // host_act_ + KP holds the sweep flags
-        auto top = [&](int k) {  // flag read-back of iteration k (its rho/beta step ends iteration k - 1)
-            if (k == 0) hipLaunchKernelGGL(scal_beta_kernel, dim3(1), dim3(KP), 0, s, sc_, act_, itol2);
-            CP_HIP(hipMemcpyAsync(hact[k & 1], act_, KP * sizeof(int), hipMemcpyDeviceToHost, s));
-            CP_HIP(hipEventRecord(act_ev_[k & 1], s));
-        };
-        auto body = [&]() {  // one BiCGSTAB iteration on the active columns
-            const int* ip = (bj_ && bj_mfma_) ? bj_iperm_ : nullptr;
-            hipLaunchKernelGGL(inner_p_kernel, dim3(nel), dim3(kT), 0, s, rows, n_, (const double*)sc_,
-                               (const int*)act_, (const float*)r_, p_, (const float*)v_,
-                               bj_ ? (const float*)nullptr : (const float*)invd_, y_, ip, pp_);
-            if (bj_) apply_block_jacobi(p_, y_, s);
-            matvec32(A32, y_, nullptr, s);
-            {
-                TimedScope tr("reduce_f32", s);
-                hipLaunchKernelGGL(inner_v_kernel, dim3((unsigned)nrb_), dim3(kT), 0, s, (const float*)part32_, KS,
-                                   lda_, n_, (const int*)act_, v_, (const float*)r0_, dpart_);
+        if (gmres_ && bj_ && bj_mfma_ && bj_iperm_) {
+            gmres_sweep(A32, o, kc, stats, s);
+        } else {
+            hipLaunchKernelGGL(inner_init_kernel, dim3((unsigned)nrb_), dim3(kT), 0, s, n_, (const double*)R_,
+                               (const double*)sc_, (const int*)act_, r_, r0_, p_, v_, d_, dpart_);
+            colsum(dpart_, 1, Q_RR, 0);
+            hipLaunchKernelGGL(scal_init_kernel, dim3(1), dim3(KP), 0, s, sc_);
+            int it = 0;
+            const int maxit = o.max_iter > 0 ? o.max_iter : 2 * n_;
+            // The host reads iteration it's active flags while iteration it is
+            // already queued behind them, so the device never waits for the
+            // host's check.  An iteration queued past the end of its sweep finds
+            // no active column: its heavy kernels return at once (gate_) and the
+            // per-column updates are masked as for any finished column.
+            int* hact[2] = {host_act_, host_act_ + 2 * KP};  // host_act_ + KP holds the sweep flags
+            auto top = [&](int k) {  // flag read-back of iteration k (its rho/beta step ends iteration k - 1)
+                if (k == 0) hipLaunchKernelGGL(scal_beta_kernel, dim3(1), dim3(KP), 0, s, sc_, act_, itol2);
+                CP_HIP(hipMemcpyAsync(hact[k & 1], act_, KP * sizeof(int), hipMemcpyDeviceToHost, s));
+                CP_HIP(hipEventRecord(act_ev_[k & 1], s));
+            };
+            auto body = [&]() {  // one BiCGSTAB iteration on the active columns
+                const int* ip = (bj_ && bj_mfma_) ? bj_iperm_ : nullptr;
+                hipLaunchKernelGGL(inner_p_kernel, dim3(nel), dim3(kT), 0, s, rows, n_, (const double*)sc_,
+                                   (const int*)act_, (const float*)r_, p_, (const float*)v_,
+                                   bj_ ? (const float*)nullptr : (const float*)invd_, y_, ip, pp_);
+                if (bj_) apply_block_jacobi(p_, y_, s);
+                matvec32(A32, y_, nullptr, s);
+                {
+                    TimedScope tr("reduce_f32", s);
+                    hipLaunchKernelGGL(inner_v_kernel, dim3((unsigned)nrb_), dim3(kT), 0, s, (const float*)part32_, KS,
+                                       lda_, n_, (const int*)act_, v_, (const float*)r0_, dpart_);
+                }
+                colsum(dpart_, 1, Q_R0V, 0, COLSUM_ALPHA);
+                hipLaunchKernelGGL(inner_s_kernel, dim3(nel), dim3(kT), 0, s, rows, n_, (const double*)sc_,
+                                   (const int*)act_, (const float*)r_, (const float*)v_,
+                                   bj_ ? (const float*)nullptr : (const float*)invd_, sv_, z_, ip, pp_);
+                if (bj_) apply_block_jacobi(sv_, z_, s);
+                matvec32(A32, z_, nullptr, s);
+                {
+                    TimedScope tr("reduce_f32", s);
+                    hipLaunchKernelGGL(inner_t_kernel, dim3((unsigned)nrb_), dim3(kT), 0, s, (const float*)part32_, KS,
+                                       lda_, n_, (const int*)act_, t_, (const float*)sv_, dpart_,
+                                       dpart_ + (size_t)nrb_ * KP);
+                }
+                colsum(dpart_, 2, Q_TT, Q_TS, COLSUM_OMEGA);
+                hipLaunchKernelGGL(inner_x_kernel, dim3((unsigned)nrb_), dim3(kT), 0, s, n_, (const double*)sc_,
+                                   (const int*)act_, d_, (const float*)y_, (const float*)z_, r_, (const float*)sv_,
+                                   (const float*)t_, (const float*)r0_, dpart_, dpart_ + (size_t)nrb_ * KP);
+                colsum(dpart_, 2, Q_RR, Q_R0R, COLSUM_BETA, itol2);  // and the next iteration's rho/beta step
+                CP_LAUNCH_CHECK();
+            };
+            gate_ = act_;
+            top(0);
+            for (;;) {
+                const bool more = it < maxit;
+                if (more) {
+                    body();
+                    top(it + 1);
+                }
+                CP_HIP(hipEventSynchronize(act_ev_[it & 1]));
+                const int* fl = hact[it & 1];
+                int nact = 0;
+                for (int c = 0; c < KP; c++) nact += fl[c] != 0;
+                if (!more || nact == 0) {
+                    // the iteration queued behind this check found no active
+                    // column: its launches did no work, so they are not timed
+                    if (more)
+                        if (KernelTimer* t = current_timer())
+                            for (const char* nm : {"gemm_f32", "pack_p", "reduce_f32", "bj_apply"}) t->discard(nm, 2);
+                    break;
+                }
+                for (int c = 0; c < kc; c++)
+                    if (fl[c]) stats[c].iters++;
+                ++it;
             }
-            colsum(dpart_, 1, Q_R0V, 0, COLSUM_ALPHA);
-            hipLaunchKernelGGL(inner_s_kernel, dim3(nel), dim3(kT), 0, s, rows, n_, (const double*)sc_,
-                               (const int*)act_, (const float*)r_, (const float*)v_,
-                               bj_ ? (const float*)nullptr : (const float*)invd_, sv_, z_, ip, pp_);
-            if (bj_) apply_block_jacobi(sv_, z_, s);
-            matvec32(A32, z_, nullptr, s);
-            {
-                TimedScope tr("reduce_f32", s);
-                hipLaunchKernelGGL(inner_t_kernel, dim3((unsigned)nrb_), dim3(kT), 0, s, (const float*)part32_, KS,
-                                   lda_, n_, (const int*)act_, t_, (const float*)sv_, dpart_,
-                                   dpart_ + (size_t)nrb_ * KP);
-            }
-            colsum(dpart_, 2, Q_TT, Q_TS, COLSUM_OMEGA);
-            hipLaunchKernelGGL(inner_x_kernel, dim3((unsigned)nrb_), dim3(kT), 0, s, n_, (const double*)sc_,
-                               (const int*)act_, d_, (const float*)y_, (const float*)z_, r_, (const float*)sv_,
-                               (const float*)t_, (const float*)r0_, dpart_, dpart_ + (size_t)nrb_ * KP);
-            colsum(dpart_, 2, Q_RR, Q_R0R, COLSUM_BETA, itol2);  // and the next iteration's rho/beta step
-            CP_LAUNCH_CHECK();
-        };
-        gate_ = act_;
-        top(0);
-        for (;;) {
-            const bool more = it < maxit;
-            if (more) {
-                body();
-                top(it + 1);
-            }
-            CP_HIP(hipEventSynchronize(act_ev_[it & 1]));
-            const int* fl = hact[it & 1];
-            int nact = 0;
-            for (int c = 0; c < KP; c++) nact += fl[c] != 0;
-            if (!more || nact == 0) {
-                // the iteration queued behind this check found no active
-                // column: its launches did no work, so they are not timed
-                if (more)
-                    if (KernelTimer* t = current_timer())
-                        for (const char* nm : {"gemm_f32", "pack_p", "reduce_f32"}) t->discard(nm, 2);
-                break;
-            }
-            for (int c = 0; c < kc; c++)
-                if (fl[c]) stats[c].iters++;
-            ++it;
+            gate_ = nullptr;
         }
-        gate_ = nullptr;
         // X += |R| d on the columns of this sweep
         CP_HIP(hipMemcpyAsync(act_, host_sweep, KP * sizeof(int), hipMemcpyHostToDevice, s));
         hipLaunchKernelGGL(correct_kernel, dim3(nel), dim3(kT), 0, s, rows, (const double*)sc_, (const int*)act_, X_,
