@@ -246,6 +246,7 @@ def main():
     t_step = float(np.mean(times))
     mm_ms, mm_n = ctx.timing("gemm_f32")
     rs_ms, rs_n = ctx.timing("resid_f64")
+    bj_ms, bj_n = ctx.timing("bj_apply")
     # the same step with mu and Cp of all 64 conditions copied to host memory
     # (PCIe-inclusive; reported beside value, never as value)
     t0 = time.perf_counter()
@@ -255,7 +256,9 @@ def main():
     gemm_flops = 2.0 * n * n * 64
     achieved = gemm_flops / (gemm_avg * 1e-3) / 1e12
     log("mixed batched: %.1f ms/step (%d conditions); gemm_f32 avg %.3f ms x %d (%.1f TFLOP/s), resid_f64 avg %.3f"
-        " ms x %d" % (1e3 * t_step, k, gemm_avg, mm_n, achieved, rs_ms / max(rs_n, 1), rs_n))
+        " ms x %d, bj_apply avg %.1f us x %d; refinements max %d, inner iterations max %d" % (
+            1e3 * t_step, k, gemm_avg, mm_n, achieved, rs_ms / max(rs_n, 1), rs_n, 1e3 * bj_ms / max(bj_n, 1), bj_n,
+            max(r["refinements"] for r in res), max(r["iters"] for r in res)))
 
     rep = g.solver_report()
     log("solver: %s" % json.dumps(rep))

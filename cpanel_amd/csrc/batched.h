@@ -12,11 +12,19 @@ namespace cpgpu {
 
 constexpr int kRhsTile = 64;  // right-hand sides per batched launch
 
+// Default inner reduction per refinement sweep.  How far one sweep lowers
+// the FP64 residual is limited by the bf16x2 inner operator: on the
+// 25,916-panel wing a 2e-4 target reaches tol = 1e-10 in three sweeps, a 5e-4
+// target needs a fourth and 1e-4 or 3e-5 only add inner iterations.
+constexpr double kInnerTolTarget = 2e-4;
+
 struct BatchedOpts {
     double tol = 1e-10;       // outer: ||b - A x|| / ||b|| in FP64
     int max_iter = 0;         // inner FP32 iterations per sweep
     int max_refine = 20;      // FP64 refinement sweeps
-    double inner_tol = 1e-3;  // inner residual reduction per sweep
+    // inner residual reduction per sweep; <= 0: kInnerTolTarget, relaxed on
+    // a sweep that can finish to what the outer tolerance still needs
+    double inner_tol = 0;
 };
 
 // GEMM tile shape chosen by the autotuner: rt = 32-row tiles per wavefront,
@@ -87,11 +95,15 @@ public:
         b64_ = B64;
     }
     // strip block-Jacobi preconditioner for the inner solve (device arrays;
-    // null perm = Jacobi).  inv: FP32 inverses, column-major per block.
+    // null perm = Jacobi).  inv: FP32 inverses, column-major per block; pk /
+    // poff: the same in matrix-core operand order (launch_bj_pack).
     void set_block_jacobi(const int* perm, const int* ptr, const int64_t* off, const float* inv, int nblocks,
-                          int max_b, const int* iperm = nullptr) {
+                          int max_b, const int* iperm = nullptr, const float* pk = nullptr,
+                          const int64_t* poff = nullptr) {
         bj_ = perm != nullptr;
-        bj_iperm_ = iperm;
+        bj_pk_ = pk;
+        bj_poff_ = poff;
+        bj_iperm_ = pk && poff ? iperm : nullptr;  // block-ordered products need the packed inverses
         bj_perm_ = perm;
         bj_ptr_ = ptr;
         bj_off_ = off;
@@ -131,6 +143,7 @@ private:
     // Y = B (rows >= n zeroed) with the column partials of ||Y_c||^2 in dpart_
     void copy64(const double* B, double* Y, hipStream_t s);
     // inner solve of one refinement sweep by right-preconditioned GMRES
+    void bj_apply_pk(const float* PP, float* Y, const double* cscale, int cstride, hipStream_t s);
     void gmres_sweep(const float* A32, const BatchedOpts& o, int kc, SolveStats* stats, hipStream_t s);
     const double* a64_ = nullptr;
     const double* b64_ = nullptr;
@@ -152,6 +165,9 @@ private:
     const int* bj_ptr_ = nullptr;
     const int64_t* bj_off_ = nullptr;
     const float* bj_inv_ = nullptr;
+    const float* bj_pk_ = nullptr;
+    const int64_t* bj_poff_ = nullptr;
+    int bj_u_ = 1;  // K chunks per load batch of the packed block-Jacobi product
     int bj_nblocks_ = 0, bj_maxb_ = 0;
     bool tuned_ = false, autotune_ = true;
     std::vector<TuneRecord> tune_log_;
@@ -196,6 +212,12 @@ void launch_sigma(int n, int64_t lda, const double* normals, const double* vinf4
 // bf16 hi/lo tiles of A32 for the bf16x3 GEMM: K blocks of 16 and bytes
 int packed_kblocks(int n);
 size_t packed_a_bytes(int64_t lda, int n);
+// floats of one b x b block-Jacobi inverse in matrix-core operand order
+int64_t bj_packed_floats(int b);
+// block q's inverse (column-major at inv + off[q]) to operand order at
+// pk + poff[q] (poff[q + 1] - poff[q] = bj_packed_floats(b_q))
+void launch_bj_pack(const float* inv, const int* ptr, const int64_t* off, const int64_t* poff, int nblocks, int max_b,
+                    float* pk, hipStream_t s);
 void launch_pack_a(const float* A32, int64_t lda, int n, void* out, hipStream_t s);
 // near-field tile lists of the packed A: for each 32-row tile, the ascending
 // 16-wide K blocks holding an entry of the near-field CSR (rowptr, cols; n rows)

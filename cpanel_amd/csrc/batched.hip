@@ -32,6 +32,7 @@ namespace cpgpu {
 namespace {
 
 typedef float floatx16 __attribute__((ext_vector_type(16)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef double doublex4 __attribute__((ext_vector_type(4)));
 
 constexpr int KP = kRhsTile;   // 64 columns
@@ -40,7 +41,7 @@ constexpr int kRedRows = 64;   // rows per reduction block
 
 enum {
     Q_RHO = 0, Q_ALPHA, Q_OMEGA, Q_RR, Q_R0R, Q_R0V, Q_TT, Q_TS, Q_RR0, Q_BB, Q_RR64, Q_SCALE, Q_BETA, Q_IT, Q_CHK,
-    Q_CHKIT, Q_MIN, Q_NQ = 20
+    Q_CHKIT, Q_MIN, Q_ITOL, Q_NQ = 20  // Q_ITOL: inner residual reduction of the current sweep
 };
 
 // stagnation window of the inner FP32 solve: a column stops when the best
@@ -851,9 +852,9 @@ __global__ __launch_bounds__(256) void reduce32_kernel(const float* __restrict__
 
 // rho/beta step at the top of an inner iteration; deactivates converged or
 // broken-down columns (breakdown is left to the next refinement sweep)
-__device__ inline void scal_beta_col(double* __restrict__ s, int* __restrict__ act_c, double tol2) {
+__device__ inline void scal_beta_col(double* __restrict__ s, int* __restrict__ act_c) {
     if (!*act_c) return;
-    if (s[Q_RR] <= tol2 * s[Q_RR0]) {
+    if (s[Q_RR] <= s[Q_ITOL] * s[Q_ITOL] * s[Q_RR0]) {
         *act_c = 0;
         return;
     }
@@ -879,10 +880,10 @@ __device__ inline void scal_beta_col(double* __restrict__ s, int* __restrict__ a
     s[Q_RHO] = rho;
 }
 
-__global__ void scal_beta_kernel(double* __restrict__ sc, int* __restrict__ act, double tol2) {
+__global__ void scal_beta_kernel(double* __restrict__ sc, int* __restrict__ act) {
     int c = threadIdx.x;
     if (c >= KP) return;
-    scal_beta_col(sc + c * Q_NQ, act + c, tol2);
+    scal_beta_col(sc + c * Q_NQ, act + c);
 }
 
 // sum of per-block column partials (fixed order) into sc[c][q], then the
@@ -894,7 +895,7 @@ __global__ void scal_beta_kernel(double* __restrict__ sc, int* __restrict__ act,
 enum { COLSUM_NONE = 0, COLSUM_ALPHA = 1, COLSUM_OMEGA = 2, COLSUM_BETA = 3 };
 __global__ __launch_bounds__(1024) void colsum_kernel(const double* __restrict__ dpart, int nblk, int nq, int q0,
                                                       int q1, double* __restrict__ sc, int op,
-                                                      int* __restrict__ act, double tol2) {
+                                                      int* __restrict__ act) {
     __shared__ double sh[1024];
     const int c = threadIdx.x & 63, g = threadIdx.x >> 6;
     for (int q = 0; q < nq; q++) {
@@ -913,7 +914,7 @@ __global__ __launch_bounds__(1024) void colsum_kernel(const double* __restrict__
     if (g != 0 || op == COLSUM_NONE) return;
     double* s = sc + c * Q_NQ;
     if (op == COLSUM_BETA)
-        scal_beta_col(s, act + c, tol2);
+        scal_beta_col(s, act + c);
     else if (op == COLSUM_ALPHA && act[c])
         s[Q_ALPHA] = s[Q_RHO] / s[Q_R0V];
     else if (op == COLSUM_OMEGA && act[c])
@@ -1116,12 +1117,21 @@ __global__ __launch_bounds__(256) void gm_init_kernel(int n, const double* __res
 }
 
 // sum of the per-block partials of quantity q (fixed order) for column c,
-// with 1024 threads: g = t / 64 sums blocks g, g + 16, ...; thread group 0
-// adds the 16 group sums and returns the total (other threads: 0)
+// with 1024 threads: g = t / 64 sums blocks g, g + 16, ... (four running sums
+// over blocks g + 64 i + 16 r, r = 0..3, so four loads are in flight);
+// thread group 0 adds the 16 group sums and returns the total (others: 0)
 __device__ inline double colsum1024(const double* __restrict__ d, int nblk, double* sh) {
     const int c = threadIdx.x & 63, g = threadIdx.x >> 6;
-    double s = 0;
-    for (int b = g; b < nblk; b += 16) s += d[(int64_t)b * KP + c];
+    double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+    int b = g;
+    for (; b + 48 < nblk; b += 64) {
+        s0 += d[(int64_t)b * KP + c];
+        s1 += d[(int64_t)(b + 16) * KP + c];
+        s2 += d[(int64_t)(b + 32) * KP + c];
+        s3 += d[(int64_t)(b + 48) * KP + c];
+    }
+    for (; b < nblk; b += 16) s0 += d[(int64_t)b * KP + c];
+    const double s = (s0 + s1) + (s2 + s3);
     sh[threadIdx.x] = s;
     __syncthreads();
     double t = 0;
@@ -1205,9 +1215,9 @@ __global__ __launch_bounds__(1024) void gm_hcol_kernel(const double* __restrict_
 
 // u_{j+1} = w - sum_{q <= j} h_{q,j} v_q on the active columns (coef[t][c] =
 // h_{t,j} / s_t), and the partials of ||u_{j+1}||^2: kGmUpPos block-order
-// positions per thread, four basis vectors at a time (16 loads in flight),
+// positions per thread, four basis vectors at a time (32 loads in flight),
 // 4 * kGmUpPos positions per workgroup
-constexpr int kGmUpPos = 4;
+constexpr int kGmUpPos = 8;
 __global__ __launch_bounds__(256) void gm_update_kernel(int n, const int* __restrict__ act, float* __restrict__ U,
                                                         int64_t us, int j, const float* __restrict__ coef,
                                                         double* __restrict__ dpart) {
@@ -1254,10 +1264,11 @@ __global__ __launch_bounds__(256) void gm_update_kernel(int n, const int* __rest
 
 // h_{j+1,j} = ||u_{j+1}||; the earlier rotations and a new one applied to
 // column j; g rotated; a column stops when |g_{j+1}| <= tol s_0, at the
-// Krylov bound m, or on a breakdown (h_{j+1,j} = 0: the solution is exact)
+// Krylov bound m, or on a breakdown (h_{j+1,j} = 0: the solution is exact);
+// tol is the column's sc[Q_ITOL]
 __global__ __launch_bounds__(1024) void gm_givens_kernel(const double* __restrict__ dpart, int nblk,
-                                                         int* __restrict__ act, int j, int m, double tol,
-                                                         double* __restrict__ st) {
+                                                         int* __restrict__ act, int j, int m,
+                                                         const double* __restrict__ sc, double* __restrict__ st) {
     __shared__ double sh[1024];
     const double t = colsum1024(dpart, nblk, sh);
     if (threadIdx.x >= KP || !act[threadIdx.x]) return;
@@ -1281,6 +1292,7 @@ __global__ __launch_bounds__(1024) void gm_givens_kernel(const double* __restric
     S[GM_G + j] = cs * gj;
     S[GM_S + j + 1] = hn;
     S[GM_NJ] = j + 1;
+    const double tol = sc[threadIdx.x * Q_NQ + Q_ITOL];
     if (!(fabs(S[GM_G + j + 1]) > tol * S[GM_S]) || j + 1 >= m || !(hn > 0)) act[threadIdx.x] = 0;
 }
 
@@ -1433,63 +1445,100 @@ __global__ __launch_bounds__(256) void bj_apply_mfma_kernel(const int* __restric
     }
 }
 
+// Block-Jacobi inverses in matrix-core operand order: block q's b x b
+// inverse (column-major at inv + off[q]) as ceil(b/32) row tiles by
+// ceil(b/8) K chunks of 256 floats at pk + poff[q]; in tile (t, c) lane l
+// holds, as one float4, rows 32 t + l % 32 and columns 8 c + 2 u + l / 32
+// (u = 0..3), zero outside the block.
+__global__ __launch_bounds__(256) void bj_pack_kernel(const float* __restrict__ inv, const int* __restrict__ ptr,
+                                                      const int64_t* __restrict__ off,
+                                                      const int64_t* __restrict__ poff, float* __restrict__ pk) {
+    const int q = blockIdx.y;
+    const int b = ptr[q + 1] - ptr[q];
+    const int nkc = (b + 7) / 8;
+    const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= (int64_t)((b + 31) / 32) * nkc * 256) return;
+    const int t = (int)(idx >> 8), e = (int)(idx & 255);
+    const int lane = e >> 2, u = e & 3;
+    const int row = (t / nkc) * 32 + (lane & 31);
+    const int k = (t % nkc) * 8 + 2 * u + (lane >> 5);
+    pk[poff[q] + idx] = (row < b && k < b) ? inv[off[q] + (int64_t)k * b + row] : 0.f;
+}
+
 // Block-Jacobi product from the block-ordered multivector PP (inner_p /
-// inner_s write it): Y(perm row) = inv_q * PP rows of block q, on FP32
-// matrix cores (v_mfma_f32_32x32x2f32).  A wavefront owns 32 rows of one
-// block and all 64 columns (two 32 x 32 accumulators), so every inverse entry
-// is read once per product; a workgroup covers 128 rows of a block.  The
-// K loop loads 2U columns of the inverse and the matching PP rows one step
-// ahead of the matrix-core ops.
+// inner_s / the GMRES basis hold it): Y(perm row) = inv_q * PP rows of block
+// q, on FP32 matrix cores (v_mfma_f32_32x32x2f32).  A workgroup owns 32 rows
+// of one block and all 64 columns; its four wavefronts take every fourth K
+// chunk of 8 (one 1 KB float4 load of the packed inverse each), U chunks
+// ahead of the matrix-core ops, and add their two 32 x 32 accumulators
+// through LDS.  Every inverse entry is read once per product.
 template <int U>
-__global__ __launch_bounds__(256) void bj_apply_pp_kernel(const int* __restrict__ perm, const int* __restrict__ ptr,
-                                                          const int64_t* __restrict__ off,
-                                                          const float* __restrict__ inv,
+__global__ __launch_bounds__(256) void bj_apply_pk_kernel(const int* __restrict__ perm, const int* __restrict__ ptr,
+                                                          const int64_t* __restrict__ poff,
+                                                          const float* __restrict__ pk,
                                                           const float* __restrict__ PP, float* __restrict__ Y,
                                                           const double* __restrict__ cscale, int cstride,
                                                           const int* __restrict__ gate) {
     if (gate && gate_idle(gate)) return;
+    __shared__ float red[4][32][64];
     const int q = blockIdx.y;
     const int b0 = ptr[q], b = ptr[q + 1] - b0;
+    const int rt = blockIdx.x;
+    if (rt * 32 >= b) return;  // whole workgroup, before any barrier
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int r0 = blockIdx.x * 128 + 32 * wave;
-    if (r0 >= b) return;  // no LDS, no barrier: idle waves leave
     const int cl = lane & 31, kh = lane >> 5;
-    const int r = r0 + cl;
-    const bool rok = r < b;
-    const float* M = inv + off[q];
+    const int nkc = (b + 7) / 8;
+    const floatx4* T = (const floatx4*)(pk + poff[q]) + (int64_t)rt * nkc * 64 + lane;
     const float* Pq = PP + (int64_t)b0 * KP;
     floatx16 acc0 = floatx16{}, acc1 = floatx16{};
-    float a[U], x0[U], x1[U], an[U], xn0[U], xn1[U];
-    auto load = [&](int k, float (&aa)[U], float (&y0)[U], float (&y1)[U]) {
+    floatx4 a[U], an[U];
+    float x0[U][4], x1[U][4], xn0[U][4], xn1[U][4];
+    auto load = [&](int c0, floatx4 (&aa)[U], float (&y0)[U][4], float (&y1)[U][4]) {
 #pragma unroll
         for (int u = 0; u < U; u++) {
-            const int kk = k + 2 * u + kh;
-            const bool ok = kk < b;
-            aa[u] = (ok && rok) ? M[(int64_t)kk * b + r] : 0.f;
-            y0[u] = ok ? Pq[(int64_t)kk * KP + cl] : 0.f;
-            y1[u] = ok ? Pq[(int64_t)kk * KP + 32 + cl] : 0.f;
+            const int c = c0 + 4 * u;
+            const bool cok = c < nkc;
+            aa[u] = cok ? __builtin_nontemporal_load(T + (int64_t)c * 64) : floatx4{};
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int k = 8 * c + 2 * v + kh;
+                const bool ok = cok && k < b;
+                y0[u][v] = ok ? Pq[(int64_t)k * KP + cl] : 0.f;
+                y1[u][v] = ok ? Pq[(int64_t)k * KP + 32 + cl] : 0.f;
+            }
         }
     };
-    load(0, a, x0, x1);
-    for (int k = 0; k < b; k += 2 * U) {
-        const bool more = k + 2 * U < b;
-        if (more) load(k + 2 * U, an, xn0, xn1);
+    load(wave, a, x0, x1);
+    for (int c = wave; c < nkc; c += 4 * U) {
+        const bool more = c + 4 * U < nkc;
+        if (more) load(c + 4 * U, an, xn0, xn1);
 #pragma unroll
-        for (int u = 0; u < U; u++) {
-            acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a[u], x0[u], acc0, 0, 0, 0);
-            acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a[u], x1[u], acc1, 0, 0, 0);
-        }
+        for (int u = 0; u < U; u++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a[u][v], x0[u][v], acc0, 0, 0, 0);
+                acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a[u][v], x1[u][v], acc1, 0, 0, 0);
+            }
         if (more) {
 #pragma unroll
             for (int u = 0; u < U; u++) {
                 a[u] = an[u];
-                x0[u] = xn0[u];
-                x1[u] = xn1[u];
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    x0[u][v] = xn0[u][v];
+                    x1[u][v] = xn1[u][v];
+                }
             }
         }
     }
-    // optional per-column factor 1 / cscale[c * cstride] (0 where that is 0)
-    float f0 = 1.f, f1 = 1.f;
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+        red[wave][j][lane] = acc0[j];
+        red[wave][16 + j][lane] = acc1[j];
+    }
+    __syncthreads();
+    // wavefront w finishes accumulator entries 8 w .. 8 w + 7
+    float f0 = 1.f, f1 = 1.f;  // optional per-column factor 1 / cscale[c * cstride] (0 where that is 0)
     if (cscale) {
         const double s0 = cscale[(int64_t)cl * cstride], s1 = cscale[(int64_t)(32 + cl) * cstride];
         f0 = s0 != 0 ? (float)(1.0 / s0) : 0.f;
@@ -1497,11 +1546,16 @@ __global__ __launch_bounds__(256) void bj_apply_pp_kernel(const int* __restrict_
     }
     const int* pq = perm + b0;
 #pragma unroll
-    for (int j = 0; j < 16; j++) {
-        const int rr = r0 + 8 * (j >> 2) + 4 * kh + (j & 3);
+    for (int i = 0; i < 8; i++) {
+        const int jj = 8 * wave + i;  // 0..15: acc0 entry jj, 16..31: acc1 entry jj - 16
+        const float v = red[0][jj][lane] + red[1][jj][lane] + red[2][jj][lane] + red[3][jj][lane];
+        const int j = jj & 15;
+        const int rr = rt * 32 + 8 * (j >> 2) + 4 * kh + (j & 3);
         if (rr < b) {
-            Y[(int64_t)pq[rr] * KP + cl] = acc0[j] * f0;
-            Y[(int64_t)pq[rr] * KP + 32 + cl] = acc1[j] * f1;
+            if (jj < 16)
+                Y[(int64_t)pq[rr] * KP + cl] = v * f0;
+            else
+                Y[(int64_t)pq[rr] * KP + 32 + cl] = v * f1;
         }
     }
 }
@@ -1513,6 +1567,16 @@ inline unsigned nblocks(int64_t work) { return (unsigned)((work + kT - 1) / kT);
 int packed_kblocks(int n) { return (n + 15) / 16; }
 
 size_t packed_a_bytes(int64_t lda, int n) { return (size_t)(lda / 32) * packed_kblocks(n) * 2048; }
+
+int64_t bj_packed_floats(int b) { return (int64_t)((b + 31) / 32) * ((b + 7) / 8) * 256; }
+
+void launch_bj_pack(const float* inv, const int* ptr, const int64_t* off, const int64_t* poff, int nblocks, int max_b,
+                    float* pk, hipStream_t s) {
+    if (nblocks <= 0) return;
+    const dim3 grid((unsigned)(bj_packed_floats(max_b) / 256), (unsigned)nblocks);
+    hipLaunchKernelGGL(bj_pack_kernel, grid, dim3(256), 0, s, inv, ptr, off, poff, pk);
+    CP_LAUNCH_CHECK();
+}
 
 void launch_pack_a(const float* A32, int64_t lda, int n, void* out, hipStream_t s) {
     const int nkb = packed_kblocks(n);
@@ -1624,6 +1688,12 @@ BatchedSolver::BatchedSolver(int n, int64_t lda, int capacity, void* ws, void* h
     if (const char* e = std::getenv("CPGPU_NO_BF16X2")) bf16x2_ = e[0] != '1';
     // CPGPU_BJ_VALU=1 selects the vector-ALU block-Jacobi product (tests)
     if (const char* e = std::getenv("CPGPU_BJ_VALU")) bj_mfma_ = e[0] != '1';
+    // CPGPU_BJ_U: K chunks per wavefront per load batch of the block-Jacobi
+    // product (1..3)
+    if (const char* e = std::getenv("CPGPU_BJ_U")) {
+        const int u = std::atoi(e);
+        if (u >= 1 && u <= 3) bj_u_ = u;
+    }
     // CPGPU_GEMM_SHAPE="variant,rt,kslices" pins the shape (tests, tuning runs)
     if (const char* e = std::getenv("CPGPU_GEMM_SHAPE")) {
         int v = 0, r = 2, k = 8;
@@ -1879,12 +1949,27 @@ void BatchedSolver::matvec64acc(const float* A32, const double* X, double* Y, co
     CP_LAUNCH_CHECK();
 }
 
+// Y = M^-1 PP (PP block-ordered, Y row-ordered), column c scaled by
+// 1 / cscale[c * cstride] when cscale is given
+void BatchedSolver::bj_apply_pk(const float* PP, float* Y, const double* cscale, int cstride, hipStream_t s) {
+    const dim3 grid((unsigned)((bj_maxb_ + 31) / 32), (unsigned)bj_nblocks_);
+    auto go = [&](auto kern) {
+        hipLaunchKernelGGL(kern, grid, dim3(kT), 0, s, bj_perm_, bj_ptr_, bj_poff_, bj_pk_, PP, Y, cscale, cstride,
+                           gate_);
+    };
+    if (bj_u_ == 1)
+        go(bj_apply_pk_kernel<1>);
+    else if (bj_u_ == 3)
+        go(bj_apply_pk_kernel<3>);
+    else
+        go(bj_apply_pk_kernel<2>);
+    CP_LAUNCH_CHECK();
+}
+
 void BatchedSolver::apply_block_jacobi(const float* P, float* Y, hipStream_t s) {
     TimedScope ts("bj_apply", s);
     if (bj_mfma_ && bj_iperm_) {  // P was also written in block order to pp_
-        dim3 grid((unsigned)((bj_maxb_ + 127) / 128), (unsigned)bj_nblocks_);
-        hipLaunchKernelGGL(bj_apply_pp_kernel<8>, grid, dim3(kT), 0, s, bj_perm_, bj_ptr_, bj_off_, bj_inv_,
-                           (const float*)pp_, Y, (const double*)nullptr, 0, gate_);
+        bj_apply_pk((const float*)pp_, Y, nullptr, 0, s);
     } else if (bj_mfma_) {
         dim3 grid((unsigned)((bj_maxb_ + 63) / 64), (unsigned)bj_nblocks_);
         hipLaunchKernelGGL(bj_apply_mfma_kernel, grid, dim3(kT), 0, s, bj_perm_, bj_ptr_, bj_off_, bj_inv_, P, Y, gate_);
@@ -1916,10 +2001,7 @@ void BatchedSolver::gmres_sweep(const float* A32, const BatchedOpts& o, int kc, 
     auto body = [&](int j) {
         {  // z = M^-1 v_j (row order)
             TimedScope tb("bj_apply", s);
-            dim3 grid((unsigned)((bj_maxb_ + 127) / 128), (unsigned)bj_nblocks_);
-            hipLaunchKernelGGL(bj_apply_pp_kernel<8>, grid, dim3(kT), 0, s, bj_perm_, bj_ptr_, bj_off_, bj_inv_,
-                               (const float*)(gm_u_ + j * us), y_, (const double*)(gm_st_ + GM_S + j), kGmStride,
-                               gate_);
+            bj_apply_pk((const float*)(gm_u_ + j * us), y_, (const double*)(gm_st_ + GM_S + j), kGmStride, s);
         }
         matvec32(A32, y_, nullptr, s);  // w = A~ z
         const int KS = slices(shape_);
@@ -1934,7 +2016,7 @@ void BatchedSolver::gmres_sweep(const float* A32, const BatchedOpts& o, int kc, 
         hipLaunchKernelGGL(gm_update_kernel, dim3((unsigned)nuc), dim3(kT), 0, s, n_, (const int*)act_, gm_u_, us, j,
                            (const float*)gm_coef_, dpart_);
         hipLaunchKernelGGL(gm_givens_kernel, dim3(1), dim3(1024), 0, s, (const double*)dpart_, nuc, act_, j, m,
-                           o.inner_tol, gm_st_);
+                           (const double*)sc_, gm_st_);
         CP_LAUNCH_CHECK();
     };
     gate_ = act_;
@@ -2020,8 +2102,8 @@ void BatchedSolver::tile_solve(const float* A32, const float* B32, const double*
             hipLaunchKernelGGL(gather_cols_kernel, dim3(nel), dim3(kT), 0, s, n_, lda_, kc, add_cm, ldx, R_);
         matvec64acc(B32, S_, Bv_, add_cm ? R_ : nullptr, -1.0, s, near_dB_, b64_);
     }
-    auto colsum = [&](const double* dp, int nq, int q0, int q1, int op = COLSUM_NONE, double tol2 = 0) {
-        hipLaunchKernelGGL(colsum_kernel, dim3(1), dim3(1024), 0, s, dp, nrb_, nq, q0, q1, sc_, op, act_, tol2);
+    auto colsum = [&](const double* dp, int nq, int q0, int q1, int op = COLSUM_NONE) {
+        hipLaunchKernelGGL(colsum_kernel, dim3(1), dim3(1024), 0, s, dp, nrb_, nq, q0, q1, sc_, op, act_);
         CP_LAUNCH_CHECK();
     };
     colsum(dpart_, 1, Q_BB, 0);
@@ -2042,7 +2124,6 @@ void BatchedSolver::tile_solve(const float* A32, const float* B32, const double*
         stats[c] = SolveStats();
     }
     const double tol2 = o.tol * o.tol;
-    const double itol2 = o.inner_tol * o.inner_tol;
     int sweep = 0;
     std::vector<double> rel(KP, 0.0);
     while (true) {
@@ -2066,6 +2147,11 @@ void BatchedSolver::tile_solve(const float* A32, const float* B32, const double*
                 host_sweep[c] = 1;
                 any = true;
                 host_sc_[c * Q_NQ + Q_SCALE] = std::sqrt(rr);
+                // the caller's inner reduction, or kInnerTolTarget relaxed to
+                // what this column still needs (half of tol / rel)
+                host_sc_[c * Q_NQ + Q_ITOL] = o.inner_tol > 0
+                                                  ? o.inner_tol
+                                                  : std::min(0.5, std::max(kInnerTolTarget, 0.5 * o.tol / rel[c]));
             } else {
                 host_sc_[c * Q_NQ + Q_SCALE] = 0;
             }
@@ -2091,7 +2177,7 @@ void BatchedSolver::tile_solve(const float* A32, const float* B32, const double*
             // per-column updates are masked as for any finished column.
             int* hact[2] = {host_act_, host_act_ + 2 * KP};  // host_act_ + KP holds the sweep flags
             auto top = [&](int k) {  // flag read-back of iteration k (its rho/beta step ends iteration k - 1)
-                if (k == 0) hipLaunchKernelGGL(scal_beta_kernel, dim3(1), dim3(KP), 0, s, sc_, act_, itol2);
+                if (k == 0) hipLaunchKernelGGL(scal_beta_kernel, dim3(1), dim3(KP), 0, s, sc_, act_);
                 CP_HIP(hipMemcpyAsync(hact[k & 1], act_, KP * sizeof(int), hipMemcpyDeviceToHost, s));
                 CP_HIP(hipEventRecord(act_ev_[k & 1], s));
             };
@@ -2123,7 +2209,7 @@ void BatchedSolver::tile_solve(const float* A32, const float* B32, const double*
                 hipLaunchKernelGGL(inner_x_kernel, dim3((unsigned)nrb_), dim3(kT), 0, s, n_, (const double*)sc_,
                                    (const int*)act_, d_, (const float*)y_, (const float*)z_, r_, (const float*)sv_,
                                    (const float*)t_, (const float*)r0_, dpart_, dpart_ + (size_t)nrb_ * KP);
-                colsum(dpart_, 2, Q_RR, Q_R0R, COLSUM_BETA, itol2);  // and the next iteration's rho/beta step
+                colsum(dpart_, 2, Q_RR, Q_R0R, COLSUM_BETA);  // and the next iteration's rho/beta step
                 CP_LAUNCH_CHECK();
             };
             gate_ = act_;

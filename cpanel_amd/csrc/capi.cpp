@@ -274,6 +274,11 @@ void ensure_block_jacobi(cp_geom* g) {
         upload(c, g->bj_ptr, P.ptr);
         upload(c, g->bj_off, P.off);
         dalloc(c, g->bj_inv, std::max<int64_t>(P.off.back(), 1) * sizeof(float));
+        std::vector<int64_t> poff(P.ptr.size(), 0);
+        for (int q = 0; q + 1 < (int)P.ptr.size(); q++)
+            poff[q + 1] = poff[q] + bj_packed_floats(P.ptr[q + 1] - P.ptr[q]);
+        upload(c, g->bj_poff, poff);
+        dalloc(c, g->bj_pk, std::max<int64_t>(poff.back(), 1) * sizeof(float));
     }
     const int nb = P.nblocks();
     const int np = (int)P.perm.size();
@@ -286,6 +291,8 @@ void ensure_block_jacobi(cp_geom* g) {
         launch_bj_invert(g->A32.as<float>(), g->dev.lda, g->bj_perm.as<int>(), g->bj_ptr.as<int>(),
                          g->bj_off.as<int64_t>(), nb, P.max_b, X.as<double>(), flags.as<int>(), flags.as<int>() + np,
                          st.as<int>(), g->bj_inv.as<float>(), g->stream);
+        launch_bj_pack(g->bj_inv.as<float>(), g->bj_ptr.as<int>(), g->bj_off.as<int64_t>(), g->bj_poff.as<int64_t>(),
+                       nb, P.max_b, g->bj_pk.as<float>(), g->stream);
     }
     std::vector<int> sv(std::max(nb, 1), 0);
     if (nb > 0)
@@ -629,14 +636,15 @@ cp_status cp_solve(cp_geom* g, cp_prec prec, int32_t k, const cp_freestream* fs,
             if (opts->precond != 1 && !g->bj_plan.perm.empty())
                 g->batched->set_block_jacobi(g->bj_perm.as<int>(), g->bj_ptr.as<int>(), g->bj_off.as<int64_t>(),
                                              g->bj_inv.as<float>(), g->bj_plan.nblocks(), g->bj_plan.max_b,
-                                             g->bj_iperm.as<int>());
+                                             g->bj_iperm.as<int>(), g->bj_pk.as<float>(),
+                                             g->bj_poff.as<int64_t>());
             else
                 g->batched->set_block_jacobi(nullptr, nullptr, nullptr, nullptr, 0, 0);
             BatchedOpts bo;
             bo.tol = tol;
             bo.max_iter = opts->max_iter > 0 ? opts->max_iter : 2 * n;
             bo.max_refine = opts->max_refine > 0 ? opts->max_refine : 20;
-            bo.inner_tol = opts->inner_tol > 0 ? opts->inner_tol : 1e-3;
+            bo.inner_tol = opts->inner_tol > 0 ? opts->inner_tol : 0;  // 0: adaptive
             g->batched->solve(g->A32.as<float>(), g->B32.as<float>(), g->normals.as<double>(), dv4.as<double>(), k,
                               bo, dmu.as<double>(), lda, stats.data(), s);
         }
@@ -1135,14 +1143,15 @@ cp_status cp_solve_rhs(cp_geom* g, cp_prec prec, int32_t k, const double* sigma,
             if (opts->precond != 1 && !g->bj_plan.perm.empty())
                 g->batched->set_block_jacobi(g->bj_perm.as<int>(), g->bj_ptr.as<int>(), g->bj_off.as<int64_t>(),
                                              g->bj_inv.as<float>(), g->bj_plan.nblocks(), g->bj_plan.max_b,
-                                             g->bj_iperm.as<int>());
+                                             g->bj_iperm.as<int>(), g->bj_pk.as<float>(),
+                                             g->bj_poff.as<int64_t>());
             else
                 g->batched->set_block_jacobi(nullptr, nullptr, nullptr, nullptr, 0, 0);
             BatchedOpts bo;
             bo.tol = tol;
             bo.max_iter = opts->max_iter > 0 ? opts->max_iter : 2 * n;
             bo.max_refine = opts->max_refine > 0 ? opts->max_refine : 20;
-            bo.inner_tol = opts->inner_tol > 0 ? opts->inner_tol : 1e-3;
+            bo.inner_tol = opts->inner_tol > 0 ? opts->inner_tol : 0;  // 0: adaptive
             g->batched->solve_rhs(g->A32.as<float>(), g->B32.as<float>(), dsig.as<double>(),
                                   mu_w2 ? dadd.as<double>() : nullptr, x0 ? dx0.as<double>() : nullptr, k, bo,
                                   dmu.as<double>(), lda, stats.data(), s);

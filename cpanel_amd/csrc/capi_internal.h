@@ -129,7 +129,7 @@ struct cp_geom {
     int near_nnz = 0;
     // strip block-Jacobi preconditioner of the batched FP32 inner solve
     BlockPlan bj_plan;
-    DevBuf bj_perm, bj_ptr, bj_off, bj_inv, bj_iperm;
+    DevBuf bj_perm, bj_ptr, bj_off, bj_inv, bj_iperm, bj_poff, bj_pk;
     bool bj_ready = false;
     // post tables
     bool post_ready = false;

@@ -114,7 +114,8 @@ typedef struct {
     double tol;             /* target ||r||/||b|| (<=0: 1e-10) */
     int32_t max_iter;       /* Krylov iterations per solve (<=0: 2N) */
     int32_t max_refine;     /* FP64 refinement sweeps, CP_FP32_REFINE (<=0: 20) */
-    double inner_tol;       /* FP32 inner solve reduction per sweep (<=0: 1e-3) */
+    double inner_tol;       /* FP32 inner solve reduction per sweep (<=0: 2e-4, relaxed on the
+                               last sweep to what tol still needs) */
     int32_t precond;        /* CP_FP32_REFINE inner solve: 0 = strip block Jacobi
                                (default), 1 = Jacobi (the reference's); the FP64
                                path always uses Jacobi */

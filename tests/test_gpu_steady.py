@@ -392,6 +392,32 @@ def test_bf16x2_product_same_refined_solution(ctx, monkeypatch):
         assert b["iters"] <= 2 * a["iters"] + 10
 
 
+@pytest.mark.parametrize("krylov_dim", ["4", "40"])
+def test_gmres_inner_same_refined_solution(ctx, monkeypatch, krylov_dim):
+    """The GMRES inner solver (restarted every `krylov_dim` iterations, one
+    restart per refinement sweep) and the BiCGSTAB one give the same refined
+    solutions: both are refined against the FP64 matrices to 1e-12."""
+    from cpanel_amd import CP_FP32_REFINE64
+    m = meshgen.config_c1()
+    conds = [(1.0, a, b) for a in (-4.0, 3.0, 10.0) for b in (-5.0, 0.0, 5.0)]
+    kw = dict(mach=0.1, Sref=6.0, bref=6.0, cref=1.0, cg=(0.25, 0.0, 0.0), tol=1e-12)
+    out = {}
+    for inner in ("bicgstab", "gmres"):
+        monkeypatch.setenv("CPGPU_INNER", inner)
+        monkeypatch.setenv("CPGPU_GMRES_M", krylov_dim)
+        g = ctx.geometry(m)
+        g.assemble(CP_FP32_REFINE64)
+        out[inner] = g.solve(conds, CP_FP32_REFINE64, **kw)
+        say("[gpu] inner %s m=%s: inner iterations %s refinements %s" % (
+            inner, krylov_dim, [r["iters"] for r in out[inner]], [r["refinements"] for r in out[inner]]))
+        g.release()
+    for a, b in zip(out["bicgstab"], out["gmres"]):
+        assert a["rel_residual"] <= 1e-12 and b["rel_residual"] <= 1e-12
+        assert np.linalg.norm(a["mu"] - b["mu"]) / np.linalg.norm(a["mu"]) <= 1e-10
+        assert abs(a["CL"] - b["CL"]) <= 1e-10
+        assert abs(a["CDi"] - b["CDi"]) <= 1e-10
+
+
 def test_block_jacobi_mfma_matches_valu(ctx, monkeypatch):
     """The matrix-core block-Jacobi product and the vector-ALU one give the
     same refined solutions (both FP32 products of the same inverses)."""
