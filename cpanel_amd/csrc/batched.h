@@ -167,6 +167,7 @@ private:
     const float* bj_inv_ = nullptr;
     const float* bj_pk_ = nullptr;
     const int64_t* bj_poff_ = nullptr;
+    int cus_ = 256;  // compute units of the device
     int bj_u_ = 1;  // K chunks per load batch of the packed block-Jacobi product
     int bj_nblocks_ = 0, bj_maxb_ = 0;
     bool tuned_ = false, autotune_ = true;

@@ -1687,6 +1687,12 @@ BatchedSolver::BatchedSolver(int n, int64_t lda, int capacity, void* ws, void* h
     // CPGPU_NO_BF16X2=1 keeps the bf16x2 product out of the tuner's candidates
     if (const char* e = std::getenv("CPGPU_NO_BF16X2")) bf16x2_ = e[0] != '1';
     // CPGPU_BJ_VALU=1 selects the vector-ALU block-Jacobi product (tests)
+    {
+        int dev = 0, cus = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && cus > 0)
+            cus_ = cus;
+    }
     if (const char* e = std::getenv("CPGPU_BJ_VALU")) bj_mfma_ = e[0] != '1';
     // CPGPU_BJ_U: K chunks per wavefront per load batch of the block-Jacobi
     // product (1..3)
@@ -1779,7 +1785,7 @@ void BatchedSolver::launch_gemm32(const GemmShape& sh, const float* A32, const f
         }
         CP_LAUNCH_CHECK();
         const int kbchunk = (nkb + KS - 1) / KS;
-        const int rows_wg = sh.variant >= 13 ? 256 : 512;
+        const int rows_wg = (sh.variant == 13 || sh.variant == 14) ? 256 : 512;
         const int ngemm = (int)((lda_ / rows_wg) * KS);
         dim3 grid((unsigned)(ngemm + lda_ / 128));  // + 4 row tiles per workgroup
         const uint4* Ap = (const uint4*)apk_;
@@ -1861,7 +1867,7 @@ void BatchedSolver::tune(const float* A32, hipStream_t s) {
             if (v >= 10 && !use2) continue;
             GemmShape g;
             g.variant = v;
-            g.rt = v >= 13 ? 2 : 4;
+            g.rt = (v == 13 || v == 14) ? 2 : 4;
             g.kslices = ks;
             cand.push_back(g);
         }
@@ -1896,9 +1902,16 @@ void BatchedSolver::tune(const float* A32, hipStream_t s) {
     set_current_timer(saved);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    shape_ = bestg;
+    // the first solver to finish tuning a key fixes its shape for the process:
+    // one that tuned the same key concurrently takes that shape too, so
+    // every solve of a leading dimension uses one split-K order (results do
+    // not depend on which solver ran it)
     std::lock_guard<std::mutex> lk(mu);
-    cache[ckey] = bestg;
+    auto it = cache.find(ckey);
+    if (it != cache.end())
+        shape_ = it->second;
+    else
+        shape_ = cache[ckey] = bestg;
 }
 
 void BatchedSolver::matvec32(const float* A32, const float* P, float* Y, hipStream_t s) {
@@ -1922,11 +1935,19 @@ void BatchedSolver::copy64(const double* B, double* Y, hipStream_t s) {
 
 void BatchedSolver::matvec64acc(const float* A32, const double* X, double* Y, const double* Badd, double sign,
                                 hipStream_t s, const double* corr, const double* A64) {
-    // K slices: enough workgroups for two per CU, no more (each slice adds an
-    // lda x 64 FP64 partial to write and reduce)
+    // K slices: the fewest whose workgroups (two resident per CU) fill the
+    // slots of every round to within 3% of the best count up to 32 (each
+    // slice adds an lda x 64 FP64 partial to write and reduce)
     const int rb = (int)(lda_ / 256);
-    int KS = 4;
-    while (KS < 32 && rb * KS < 512) KS *= 2;
+    const int slots = 2 * cus_;
+    auto fill = [&](int ks) {
+        const int64_t wg = (int64_t)rb * ks;
+        return (double)wg / (double)(((wg + slots - 1) / slots) * slots);
+    };
+    double best = 0;
+    for (int ks = 1; ks <= 32; ks++) best = std::max(best, fill(ks));
+    int KS = 1;
+    while (KS < 32 && fill(KS) < best - 0.03) KS++;
     int kchunk = (int)round_up((n_ + KS - 1) / KS, 16);
     dim3 grid((unsigned)(rb * KS));
     {
