@@ -18,6 +18,10 @@ constexpr int kRhsTile = 64;  // right-hand sides per batched launch
 // target needs a fourth and 1e-4 or 3e-5 only add inner iterations.
 constexpr double kInnerTolTarget = 2e-4;
 
+// block-Jacobi inverses in matrix-core operand order: FP32 (FP32 MFMA) or
+// bf16 hi/lo pairs (bf16 MFMA)
+enum { kBjPackFp32 = 0, kBjPackBf16 = 1 };
+
 struct BatchedOpts {
     double tol = 1e-10;       // outer: ||b - A x|| / ||b|| in FP64
     int max_iter = 0;         // inner FP32 iterations per sweep
@@ -99,10 +103,11 @@ public:
     // poff: the same in matrix-core operand order (launch_bj_pack).
     void set_block_jacobi(const int* perm, const int* ptr, const int64_t* off, const float* inv, int nblocks,
                           int max_b, const int* iperm = nullptr, const float* pk = nullptr,
-                          const int64_t* poff = nullptr) {
+                          const int64_t* poff = nullptr, int pk_kind = kBjPackBf16) {
         bj_ = perm != nullptr;
         bj_pk_ = pk;
         bj_poff_ = poff;
+        bj_kind_ = pk_kind;
         bj_iperm_ = pk && poff ? iperm : nullptr;  // block-ordered products need the packed inverses
         bj_perm_ = perm;
         bj_ptr_ = ptr;
@@ -168,6 +173,8 @@ private:
     const float* bj_pk_ = nullptr;
     const int64_t* bj_poff_ = nullptr;
     int cus_ = 256;  // compute units of the device
+    int bj_kind_ = kBjPackBf16;  // layout of bj_pk_
+    bool bj_lo_ = false;         // bf16 product: read the lo plane of the inverses too
     int bj_u_ = 1;  // K chunks per load batch of the packed block-Jacobi product
     int bj_nblocks_ = 0, bj_maxb_ = 0;
     bool tuned_ = false, autotune_ = true;
@@ -213,12 +220,12 @@ void launch_sigma(int n, int64_t lda, const double* normals, const double* vinf4
 // bf16 hi/lo tiles of A32 for the bf16x3 GEMM: K blocks of 16 and bytes
 int packed_kblocks(int n);
 size_t packed_a_bytes(int64_t lda, int n);
-// floats of one b x b block-Jacobi inverse in matrix-core operand order
+// floats reserved for one b x b inverse in operand order (either kind)
 int64_t bj_packed_floats(int b);
-// block q's inverse (column-major at inv + off[q]) to operand order at
-// pk + poff[q] (poff[q + 1] - poff[q] = bj_packed_floats(b_q))
+// block q's inverse (column-major at inv + off[q]) to operand order of the
+// given kind at pk + poff[q] (poff[q + 1] - poff[q] = bj_packed_floats(b_q))
 void launch_bj_pack(const float* inv, const int* ptr, const int64_t* off, const int64_t* poff, int nblocks, int max_b,
-                    float* pk, hipStream_t s);
+                    float* pk, int kind, hipStream_t s);
 void launch_pack_a(const float* A32, int64_t lda, int n, void* out, hipStream_t s);
 // near-field tile lists of the packed A: for each 32-row tile, the ascending
 // 16-wide K blocks holding an entry of the near-field CSR (rowptr, cols; n rows)

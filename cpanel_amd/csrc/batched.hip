@@ -1117,21 +1117,18 @@ __global__ __launch_bounds__(256) void gm_init_kernel(int n, const double* __res
 }
 
 // sum of the per-block partials of quantity q (fixed order) for column c,
-// with 1024 threads: g = t / 64 sums blocks g, g + 16, ... (four running sums
-// over blocks g + 64 i + 16 r, r = 0..3, so four loads are in flight);
+// with 1024 threads: g = t / 64 sums blocks g, g + 16, ... (eight running sums
+// over blocks g + 128 i + 16 r, r = 0..7, so eight loads are in flight);
 // thread group 0 adds the 16 group sums and returns the total (others: 0)
 __device__ inline double colsum1024(const double* __restrict__ d, int nblk, double* sh) {
     const int c = threadIdx.x & 63, g = threadIdx.x >> 6;
-    double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+    double r[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     int b = g;
-    for (; b + 48 < nblk; b += 64) {
-        s0 += d[(int64_t)b * KP + c];
-        s1 += d[(int64_t)(b + 16) * KP + c];
-        s2 += d[(int64_t)(b + 32) * KP + c];
-        s3 += d[(int64_t)(b + 48) * KP + c];
-    }
-    for (; b < nblk; b += 16) s0 += d[(int64_t)b * KP + c];
-    const double s = (s0 + s1) + (s2 + s3);
+    for (; b + 112 < nblk; b += 128)
+#pragma unroll
+        for (int k = 0; k < 8; k++) r[k] += d[(int64_t)(b + 16 * k) * KP + c];
+    for (; b < nblk; b += 16) r[0] += d[(int64_t)b * KP + c];
+    const double s = ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
     sh[threadIdx.x] = s;
     __syncthreads();
     double t = 0;
@@ -1154,18 +1151,20 @@ __global__ __launch_bounds__(1024) void gm_start_kernel(const double* __restrict
 }
 
 // w = sum_ks part (the product A~ z_j, row order) stored as u_{j+1} in block
-// order on the active columns
+// order; a thread sums four columns (skipped when none of them is active;
+// an inactive column's u_{j+1} is never read)
 __global__ __launch_bounds__(256) void gm_w_kernel(const float* __restrict__ part, int KS, int64_t lda, int n,
                                                    const int* __restrict__ act, const int* __restrict__ iperm,
                                                    float* __restrict__ W) {
-    const int64_t idx = (int64_t)blockIdx.x * kT + threadIdx.x;
-    if (idx >= (int64_t)n * KP) return;
-    const int c = (int)(idx & 63);
-    if (!act[c]) return;
-    float v = 0.f;
+    const int64_t idx = (int64_t)blockIdx.x * kT + threadIdx.x;  // float4 index
+    if (idx >= (int64_t)n * (KP / 4)) return;
+    const int c = 4 * (int)(idx & 15);
+    if (!(act[c] | act[c + 1] | act[c + 2] | act[c + 3])) return;
+    const floatx4* P4 = reinterpret_cast<const floatx4*>(part);
+    floatx4 v = floatx4{};
 #pragma unroll 8
-    for (int q = 0; q < KS; q++) v += part[(int64_t)q * lda * KP + idx];
-    W[(int64_t)iperm[idx >> 6] * KP + c] = v;
+    for (int q = 0; q < KS; q++) v += P4[(int64_t)q * lda * (KP / 4) + idx];
+    *reinterpret_cast<floatx4*>(W + (int64_t)iperm[idx >> 4] * KP + c) = v;
 }
 
 // dots u_t . u_{j+1}: one wavefront per (chunk of kGmDotRows block-order
@@ -1560,6 +1559,128 @@ __global__ __launch_bounds__(256) void bj_apply_pk_kernel(const int* __restrict_
     }
 }
 
+// The same inverses as bf16 hi/lo pairs in the bf16 matrix-core operand
+// order: block q as ceil(b/32) row tiles by ceil(b/16) K chunks of 2 KiB at
+// pk + poff[q] (floats), each chunk the hi plane then the lo plane (64 lanes
+// x 16 bytes each); lane l holds row 32 t + l % 32, columns 16 c + 8 (l / 32)
+// + 0..7, zero outside the block.
+__global__ __launch_bounds__(256) void bj_pack_bf_kernel(const float* __restrict__ inv, const int* __restrict__ ptr,
+                                                         const int64_t* __restrict__ off,
+                                                         const int64_t* __restrict__ poff, float* __restrict__ pk) {
+    const int q = blockIdx.y;
+    const int b = ptr[q + 1] - ptr[q];
+    const int nkc = (b + 15) / 16;
+    const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;  // (chunk, lane)
+    if (g >= (int64_t)((b + 31) / 32) * nkc * 64) return;
+    const int t = (int)(g >> 6), lane = (int)(g & 63);
+    const int row = (t / nkc) * 32 + (lane & 31);
+    const int kbase = (t % nkc) * 16 + 8 * (lane >> 5);
+    float v[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        const int k = kbase + i;
+        v[i] = (row < b && k < b) ? inv[off[q] + (int64_t)k * b + row] : 0.f;
+    }
+    uint4 hi, lo;
+    split8(v, hi, lo);
+    uint4* T = reinterpret_cast<uint4*>(pk + poff[q]) + (int64_t)t * 128 + lane;
+    T[0] = hi;
+    T[64] = lo;
+}
+
+// Block-Jacobi product on bf16 matrix cores (v_mfma_f32_32x32x16_bf16): the
+// PP rows are split hi + lo as they are loaded, so inv * PP ~ hi_inv (hi_p +
+// lo_p) (+ lo_inv hi_p when LO): LO gives FP32-level accuracy from the same
+// bytes as the FP32 inverses, !LO reads half of them (the inverse to bf16).
+// Work split and reduction as in bj_apply_pk_kernel, with K chunks of 16.
+template <bool LO>
+__global__ __launch_bounds__(256) void bj_apply_bf_kernel(const int* __restrict__ perm, const int* __restrict__ ptr,
+                                                          const int64_t* __restrict__ poff,
+                                                          const float* __restrict__ pk,
+                                                          const float* __restrict__ PP, float* __restrict__ Y,
+                                                          const double* __restrict__ cscale, int cstride,
+                                                          const int* __restrict__ gate) {
+    if (gate && gate_idle(gate)) return;
+    __shared__ float red[4][32][64];
+    const int q = blockIdx.y;
+    const int b0 = ptr[q], b = ptr[q + 1] - b0;
+    const int rt = blockIdx.x;
+    if (rt * 32 >= b) return;  // whole workgroup, before any barrier
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int cl = lane & 31, kh = lane >> 5;
+    const int nkc = (b + 15) / 16;
+    const u32x4* T = reinterpret_cast<const u32x4*>(pk + poff[q]) + (int64_t)rt * nkc * 128 + lane;
+    const float* Pq = PP + (int64_t)b0 * KP;
+    floatx16 acc0 = floatx16{}, acc1 = floatx16{};
+    u32x4 ah, al, nah, nal;
+    float x0[8], x1[8], nx0[8], nx1[8];
+    auto load = [&](int c, u32x4& h, u32x4& l, float (&y0)[8], float (&y1)[8]) {
+        const bool cok = c < nkc;
+        h = cok ? __builtin_nontemporal_load(T + (int64_t)c * 128) : u32x4{};
+        if (LO) l = cok ? __builtin_nontemporal_load(T + (int64_t)c * 128 + 64) : u32x4{};
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const int k = 16 * c + 8 * kh + i;
+            const bool ok = cok && k < b;
+            y0[i] = ok ? Pq[(int64_t)k * KP + cl] : 0.f;
+            y1[i] = ok ? Pq[(int64_t)k * KP + 32 + cl] : 0.f;
+        }
+    };
+    load(wave, ah, al, x0, x1);
+    for (int c = wave; c < nkc; c += 4) {
+        const bool more = c + 4 < nkc;
+        if (more) load(c + 4, nah, nal, nx0, nx1);
+        uint4 h0, l0, h1, l1;
+        split8(x0, h0, l0);
+        split8(x1, h1, l1);
+        const bf16x8 a = __builtin_bit_cast(bf16x8, ah);
+        acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, __builtin_bit_cast(bf16x8, l0), acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, __builtin_bit_cast(bf16x8, l1), acc1, 0, 0, 0);
+        if (LO) {
+            const bf16x8 alo = __builtin_bit_cast(bf16x8, al);
+            acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo, __builtin_bit_cast(bf16x8, h0), acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo, __builtin_bit_cast(bf16x8, h1), acc1, 0, 0, 0);
+        }
+        acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, __builtin_bit_cast(bf16x8, h0), acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, __builtin_bit_cast(bf16x8, h1), acc1, 0, 0, 0);
+        if (more) {
+            ah = nah;
+            if (LO) al = nal;
+#pragma unroll
+            for (int i = 0; i < 8; i++) {
+                x0[i] = nx0[i];
+                x1[i] = nx1[i];
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+        red[wave][j][lane] = acc0[j];
+        red[wave][16 + j][lane] = acc1[j];
+    }
+    __syncthreads();
+    float f0 = 1.f, f1 = 1.f;  // optional per-column factor 1 / cscale[c * cstride] (0 where that is 0)
+    if (cscale) {
+        const double s0 = cscale[(int64_t)cl * cstride], s1 = cscale[(int64_t)(32 + cl) * cstride];
+        f0 = s0 != 0 ? (float)(1.0 / s0) : 0.f;
+        f1 = s1 != 0 ? (float)(1.0 / s1) : 0.f;
+    }
+    const int* pq = perm + b0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        const int jj = 8 * wave + i;  // 0..15: acc0 entry jj, 16..31: acc1 entry jj - 16
+        const float v = red[0][jj][lane] + red[1][jj][lane] + red[2][jj][lane] + red[3][jj][lane];
+        const int j = jj & 15;
+        const int rr = rt * 32 + 8 * (j >> 2) + 4 * kh + (j & 3);
+        if (rr < b) {
+            if (jj < 16)
+                Y[(int64_t)pq[rr] * KP + cl] = v * f0;
+            else
+                Y[(int64_t)pq[rr] * KP + 32 + cl] = v * f1;
+        }
+    }
+}
+
 inline unsigned nblocks(int64_t work) { return (unsigned)((work + kT - 1) / kT); }
 
 }  // namespace
@@ -1568,13 +1689,19 @@ int packed_kblocks(int n) { return (n + 15) / 16; }
 
 size_t packed_a_bytes(int64_t lda, int n) { return (size_t)(lda / 32) * packed_kblocks(n) * 2048; }
 
-int64_t bj_packed_floats(int b) { return (int64_t)((b + 31) / 32) * ((b + 7) / 8) * 256; }
+int64_t bj_packed_floats(int b) { return (int64_t)((b + 31) / 32) * ((b + 15) / 16) * 512; }
 
 void launch_bj_pack(const float* inv, const int* ptr, const int64_t* off, const int64_t* poff, int nblocks, int max_b,
-                    float* pk, hipStream_t s) {
+                    float* pk, int kind, hipStream_t s) {
     if (nblocks <= 0) return;
-    const dim3 grid((unsigned)(bj_packed_floats(max_b) / 256), (unsigned)nblocks);
-    hipLaunchKernelGGL(bj_pack_kernel, grid, dim3(256), 0, s, inv, ptr, off, poff, pk);
+    const int64_t tiles32 = (int64_t)((max_b + 31) / 32);
+    if (kind == kBjPackFp32) {
+        const dim3 grid((unsigned)(tiles32 * ((max_b + 7) / 8)), (unsigned)nblocks);  // 256 floats per chunk
+        hipLaunchKernelGGL(bj_pack_kernel, grid, dim3(256), 0, s, inv, ptr, off, poff, pk);
+    } else {
+        const dim3 grid((unsigned)((tiles32 * ((max_b + 15) / 16) * 64 + 255) / 256), (unsigned)nblocks);
+        hipLaunchKernelGGL(bj_pack_bf_kernel, grid, dim3(256), 0, s, inv, ptr, off, poff, pk);
+    }
     CP_LAUNCH_CHECK();
 }
 
@@ -1694,8 +1821,13 @@ BatchedSolver::BatchedSolver(int n, int64_t lda, int capacity, void* ws, void* h
             cus_ = cus;
     }
     if (const char* e = std::getenv("CPGPU_BJ_VALU")) bj_mfma_ = e[0] != '1';
-    // CPGPU_BJ_U: K chunks per wavefront per load batch of the block-Jacobi
-    // product (1..3)
+    // CPGPU_BJ_PREC=bf16x3: the bf16 block-Jacobi product reads the lo plane
+    // of the inverses too (FP32-level accuracy; default: the hi plane alone,
+    // the inverses to bf16 -- a preconditioner changes only the inner
+    // iterations, and on the 25,916-panel wing not even those)
+    if (const char* e = std::getenv("CPGPU_BJ_PREC")) bj_lo_ = std::string(e) == "bf16x3";
+    // CPGPU_BJ_U: K chunks per wavefront per load batch of the FP32
+    // block-Jacobi product (1..3)
     if (const char* e = std::getenv("CPGPU_BJ_U")) {
         const int u = std::atoi(e);
         if (u >= 1 && u <= 3) bj_u_ = u;
@@ -1978,7 +2110,9 @@ void BatchedSolver::bj_apply_pk(const float* PP, float* Y, const double* cscale,
         hipLaunchKernelGGL(kern, grid, dim3(kT), 0, s, bj_perm_, bj_ptr_, bj_poff_, bj_pk_, PP, Y, cscale, cstride,
                            gate_);
     };
-    if (bj_u_ == 1)
+    if (bj_kind_ == kBjPackBf16)
+        go(bj_lo_ ? bj_apply_bf_kernel<true> : bj_apply_bf_kernel<false>);
+    else if (bj_u_ == 1)
         go(bj_apply_pk_kernel<1>);
     else if (bj_u_ == 3)
         go(bj_apply_pk_kernel<3>);
@@ -2028,7 +2162,7 @@ void BatchedSolver::gmres_sweep(const float* A32, const BatchedOpts& o, int kc, 
         const int KS = slices(shape_);
         const int ndc = (int)((n_ + kGmDotRows - 1) / kGmDotRows);     // position chunks of the dots
         const int nuc = (int)((n_ + 4 * kGmUpPos - 1) / (4 * kGmUpPos));  // workgroups of the update
-        hipLaunchKernelGGL(gm_w_kernel, dim3(nblocks((int64_t)n_ * KP)), dim3(kT), 0, s, (const float*)part32_, KS,
+        hipLaunchKernelGGL(gm_w_kernel, dim3(nblocks((int64_t)n_ * (KP / 4))), dim3(kT), 0, s, (const float*)part32_, KS,
                            lda_, n_, (const int*)act_, bj_iperm_, gm_u_ + (j + 1) * us);
         hipLaunchKernelGGL(gm_dots_kernel, dim3((unsigned)ndc, (unsigned)((j + 4) / 4)), dim3(kT), 0, s, n_,
                            (const int*)act_, (const float*)gm_u_, us, j, gm_dpart_);

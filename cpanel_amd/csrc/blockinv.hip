@@ -1,15 +1,16 @@
 // Batched inversion of the strip block-Jacobi diagonal blocks on the device.
 //
-// Each block M (b x b, b <= 1024, column-major, FP64) is inverted in place by
+// Each block M (b x b, b <= 2048, column-major, FP64) is inverted in place by
 // Gauss-Jordan elimination with partial pivoting, in a blocked right-looking
 // form over panels of kPw pivot columns and without physical row swaps: the
 // pivot of step k is the row of largest |M[i][k]| among the rows not yet
 // used as pivots (the set of candidates is the same as after the swaps of
 // the textbook form, so the pivot choice is partial pivoting).
 //
-//   panel kernel  one 1024-thread workgroup per block, one row per thread,
-//                 the row's kPw panel entries in registers: runs the kPw
-//                 pivot steps on the panel columns only.  Afterwards column
+//   panel kernel  one 1024-thread workgroup per block, one row per thread
+//                 (two above 1024 rows), the rows' kPw panel entries in
+//                 registers: runs the kPw pivot steps on the panel columns
+//                 only.  Afterwards column
 //                 k0+t holds column p_t of the accumulated transform T, which
 //                 differs from I only in the pivot-row columns p_t.
 //   update kernel every other column j:  M[:, j] += (T - I)[:, p] * M[p, j]
@@ -31,8 +32,9 @@ namespace cpgpu {
 
 namespace {
 
-constexpr int kPw = 32;        // pivot columns per panel
-constexpr int kMaxB = 1024;    // rows per block (one per panel-kernel thread)
+constexpr int kPw = 16;        // pivot columns per panel
+constexpr int kPanelT = 1024;  // panel-kernel workgroup: one or two rows per thread
+constexpr int kMaxB = 2048;    // rows per block
 constexpr int kUc = 16;        // columns per update workgroup
 constexpr int kUt = 256;       // update / elementwise workgroup size
 
@@ -67,31 +69,47 @@ __device__ inline void wave_argmax(double& v, int& i) {
     }
 }
 
-__global__ __launch_bounds__(kMaxB) void bjinv_panel_kernel(double* __restrict__ X, const int* __restrict__ ptr,
-                                                            const int64_t* __restrict__ off, int k0,
-                                                            int* __restrict__ used, int* __restrict__ piv,
-                                                            int* __restrict__ status) {
+template <int RPT>
+__global__ __launch_bounds__(kPanelT) void bjinv_panel_kernel(double* __restrict__ X, const int* __restrict__ ptr,
+                                                              const int64_t* __restrict__ off, int k0,
+                                                              int* __restrict__ used, int* __restrict__ piv,
+                                                              int* __restrict__ status) {
     const int q = blockIdx.x;
     const int b0 = ptr[q], b = ptr[q + 1] - b0;
     if (k0 >= b || status[q]) return;
     const int w = min(kPw, b - k0);
     double* M = X + off[q];
-    const int i = threadIdx.x;
-    const bool row = i < b;
-    bool usd = row ? used[b0 + i] != 0 : true;
-    double r[kPw];
+    const int tid = threadIdx.x;
+    // rows tid + kPanelT * k, k < RPT
+    bool usd[RPT];
+    double r[RPT][kPw];
 #pragma unroll
-    for (int t = 0; t < kPw; t++) r[t] = (row && t < w) ? M[i + (int64_t)(k0 + t) * b] : 0.0;
+    for (int k = 0; k < RPT; k++) {
+        const int i = tid + kPanelT * k;
+        const bool row = i < b;
+        usd[k] = row ? used[b0 + i] != 0 : true;
+#pragma unroll
+        for (int t = 0; t < kPw; t++) r[k][t] = (row && t < w) ? M[i + (int64_t)(k0 + t) * b] : 0.0;
+    }
 
-    __shared__ double sv[kMaxB / 64];
-    __shared__ int si[kMaxB / 64];
+    __shared__ double sv[kPanelT / 64];
+    __shared__ int si[kPanelT / 64];
     __shared__ double prow[kPw];
-    const int lane = i & 63, wv = i >> 6;
+    const int lane = tid & 63, wv = tid >> 6;
 #pragma unroll
     for (int t = 0; t < kPw; t++) {
         if (t >= w) break;
-        double v = usd ? -1.0 : fabs(r[t]);
-        int vi = i;
+        // this thread's best candidate (larger |value|, then smaller row)
+        double v = -1.0;
+        int vi = tid;
+#pragma unroll
+        for (int k = 0; k < RPT; k++) {
+            const double a = usd[k] ? -1.0 : fabs(r[k][t]);
+            if (a > v) {
+                v = a;
+                vi = tid + kPanelT * k;
+            }
+        }
         wave_argmax(v, vi);
         if (lane == 0) {
             sv[wv] = v;
@@ -100,7 +118,7 @@ __global__ __launch_bounds__(kMaxB) void bjinv_panel_kernel(double* __restrict__
         __syncthreads();
         double best = sv[0];
         int p = si[0];
-        for (int u = 1; u < kMaxB / 64; u++) {
+        for (int u = 1; u < kPanelT / 64; u++) {
             const double ov = sv[u];
             const int oi = si[u];
             if (ov > best || (ov == best && oi < p)) {
@@ -109,33 +127,43 @@ __global__ __launch_bounds__(kMaxB) void bjinv_panel_kernel(double* __restrict__
             }
         }
         if (!(best > 0)) {  // singular (or NaN) block: the caller falls back
-            if (i == 0) status[q] = 1;
+            if (tid == 0) status[q] = 1;
             return;
         }
-        if (i == p) {
-            usd = true;
-            const double inv = 1.0 / r[t];
-            r[t] = 1.0;
 #pragma unroll
-            for (int j = 0; j < kPw; j++) {
-                r[j] *= inv;
-                prow[j] = r[j];
+        for (int k = 0; k < RPT; k++)
+            if (tid + kPanelT * k == p) {
+                usd[k] = true;
+                const double inv = 1.0 / r[k][t];
+                r[k][t] = 1.0;
+#pragma unroll
+                for (int j = 0; j < kPw; j++) {
+                    r[k][j] *= inv;
+                    prow[j] = r[k][j];
+                }
+            }
+        if (tid == 0) piv[b0 + k0 + t] = p;
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < RPT; k++) {
+            const int i = tid + kPanelT * k;
+            if (i < b && i != p) {
+                const double f = r[k][t];
+                r[k][t] = 0.0;
+#pragma unroll
+                for (int j = 0; j < kPw; j++) r[k][j] = fma(-f, prow[j], r[k][j]);
             }
         }
-        if (i == 0) piv[b0 + k0 + t] = p;
-        __syncthreads();
-        if (row && i != p) {
-            const double f = r[t];
-            r[t] = 0.0;
-#pragma unroll
-            for (int j = 0; j < kPw; j++) r[j] = fma(-f, prow[j], r[j]);
-        }
     }
-    if (row) {
 #pragma unroll
-        for (int t = 0; t < kPw; t++)
-            if (t < w) M[i + (int64_t)(k0 + t) * b] = r[t];
-        used[b0 + i] = usd ? 1 : 0;
+    for (int k = 0; k < RPT; k++) {
+        const int i = tid + kPanelT * k;
+        if (i < b) {
+#pragma unroll
+            for (int t = 0; t < kPw; t++)
+                if (t < w) M[i + (int64_t)(k0 + t) * b] = r[k][t];
+            used[b0 + i] = usd[k] ? 1 : 0;
+        }
     }
 }
 
@@ -198,15 +226,19 @@ __global__ __launch_bounds__(kUt) void bjinv_finish_kernel(const double* __restr
 
 void launch_bj_invert(const float* A32, int64_t lda, const int* perm, const int* ptr, const int64_t* off, int nblocks,
                       int max_b, double* X, int* used, int* piv, int* status, float* out, hipStream_t s) {
-    if (max_b > kMaxB) throw HipError{hipErrorInvalidValue, "block-Jacobi block larger than 1024 rows"};
+    if (max_b > kMaxB) throw HipError{hipErrorInvalidValue, "block-Jacobi block larger than 2048 rows"};
     if (nblocks <= 0) return;
     const dim3 gsq((unsigned)(((int64_t)max_b * max_b + kUt - 1) / kUt), (unsigned)nblocks);
     hipLaunchKernelGGL(bjinv_gather_kernel, gsq, dim3(kUt), 0, s, A32, lda, perm, ptr, off, X, used, piv, status);
     CP_LAUNCH_CHECK();
     const dim3 gup((unsigned)((max_b + kUc - 1) / kUc), (unsigned)nblocks);
     for (int k0 = 0; k0 < max_b; k0 += kPw) {
-        hipLaunchKernelGGL(bjinv_panel_kernel, dim3((unsigned)nblocks), dim3(kMaxB), 0, s, X, ptr, off, k0, used, piv,
-                           status);
+        if (max_b <= kPanelT)
+            hipLaunchKernelGGL(bjinv_panel_kernel<1>, dim3((unsigned)nblocks), dim3(kPanelT), 0, s, X, ptr, off, k0,
+                               used, piv, status);
+        else
+            hipLaunchKernelGGL(bjinv_panel_kernel<2>, dim3((unsigned)nblocks), dim3(kPanelT), 0, s, X, ptr, off, k0,
+                               used, piv, status);
         CP_LAUNCH_CHECK();
         hipLaunchKernelGGL(bjinv_update_kernel, gup, dim3(kUt), 0, s, X, ptr, off, k0, piv, status);
         CP_LAUNCH_CHECK();

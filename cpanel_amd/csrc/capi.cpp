@@ -6,6 +6,8 @@
 //             FP64 and/or FP32, post-processing tables, solver workspaces.
 #include "capi_internal.h"
 
+#include <cstdlib>
+
 using namespace cpgpu;
 using namespace cpgpu::capi;
 
@@ -266,7 +268,14 @@ void ensure_block_jacobi(cp_geom* g) {
     cp_ctx* c = g->ctx;
     BlockPlan& P = g->bj_plan;
     if (P.perm.empty()) {
-        build_strip_blocks(g->host, 768, 1024, P);
+        // CPGPU_BJ_TARGET: panels per block before a strip starts a new one
+        // (512..2048; the inverter's limit is 2048 rows)
+        int target = 2048;
+        if (const char* e = std::getenv("CPGPU_BJ_TARGET")) {
+            const int t = std::atoi(e);
+            if (t >= 512 && t <= 2048) target = t;
+        }
+        build_strip_blocks(g->host, target, 2048, P);
         upload(c, g->bj_perm, P.perm);
         std::vector<int> ip(g->dev.n, 0);
         for (size_t q = 0; q < P.perm.size(); q++) ip[P.perm[q]] = (int)q;
@@ -291,8 +300,11 @@ void ensure_block_jacobi(cp_geom* g) {
         launch_bj_invert(g->A32.as<float>(), g->dev.lda, g->bj_perm.as<int>(), g->bj_ptr.as<int>(),
                          g->bj_off.as<int64_t>(), nb, P.max_b, X.as<double>(), flags.as<int>(), flags.as<int>() + np,
                          st.as<int>(), g->bj_inv.as<float>(), g->stream);
+        // CPGPU_BJ_PREC=fp32: FP32 matrix-core product (default: bf16 operands)
+        const char* e = std::getenv("CPGPU_BJ_PREC");
+        g->bj_pk_kind = (e && std::string(e) == "fp32") ? kBjPackFp32 : kBjPackBf16;
         launch_bj_pack(g->bj_inv.as<float>(), g->bj_ptr.as<int>(), g->bj_off.as<int64_t>(), g->bj_poff.as<int64_t>(),
-                       nb, P.max_b, g->bj_pk.as<float>(), g->stream);
+                       nb, P.max_b, g->bj_pk.as<float>(), g->bj_pk_kind, g->stream);
     }
     std::vector<int> sv(std::max(nb, 1), 0);
     if (nb > 0)
@@ -637,7 +649,7 @@ cp_status cp_solve(cp_geom* g, cp_prec prec, int32_t k, const cp_freestream* fs,
                 g->batched->set_block_jacobi(g->bj_perm.as<int>(), g->bj_ptr.as<int>(), g->bj_off.as<int64_t>(),
                                              g->bj_inv.as<float>(), g->bj_plan.nblocks(), g->bj_plan.max_b,
                                              g->bj_iperm.as<int>(), g->bj_pk.as<float>(),
-                                             g->bj_poff.as<int64_t>());
+                                             g->bj_poff.as<int64_t>(), g->bj_pk_kind);
             else
                 g->batched->set_block_jacobi(nullptr, nullptr, nullptr, nullptr, 0, 0);
             BatchedOpts bo;
@@ -1144,7 +1156,7 @@ cp_status cp_solve_rhs(cp_geom* g, cp_prec prec, int32_t k, const double* sigma,
                 g->batched->set_block_jacobi(g->bj_perm.as<int>(), g->bj_ptr.as<int>(), g->bj_off.as<int64_t>(),
                                              g->bj_inv.as<float>(), g->bj_plan.nblocks(), g->bj_plan.max_b,
                                              g->bj_iperm.as<int>(), g->bj_pk.as<float>(),
-                                             g->bj_poff.as<int64_t>());
+                                             g->bj_poff.as<int64_t>(), g->bj_pk_kind);
             else
                 g->batched->set_block_jacobi(nullptr, nullptr, nullptr, nullptr, 0, 0);
             BatchedOpts bo;

@@ -131,6 +131,7 @@ struct cp_geom {
     BlockPlan bj_plan;
     DevBuf bj_perm, bj_ptr, bj_off, bj_inv, bj_iperm, bj_poff, bj_pk;
     bool bj_ready = false;
+    int bj_pk_kind = kBjPackBf16;  // layout of bj_pk (launch_bj_pack)
     // post tables
     bool post_ready = false;
     PostHost post_host;

@@ -294,6 +294,30 @@ def test_block_jacobi_inverses_match_lapack(c1s, c1):
         say("[gpu]   %d blocks, max rel err %.2e" % (nb, worst))
 
 
+def test_block_jacobi_blocks_above_1024_rows(ctx, monkeypatch):
+    """Blocks of more than 1024 rows (the panel kernel's two-rows-per-thread
+    form): inverses against LAPACK as above, and the solve refined against
+    the FP64 matrices with them equals the FP64 solve."""
+    from cpanel_amd import CP_FP32_REFINE64
+    monkeypatch.setenv("CPGPU_BJ_TARGET", "2048")
+    m = meshgen.config_c1()
+    g = ctx.geometry(m)
+    g.assemble(CP_FP32_REFINE)
+    g.assemble(CP_FP64)
+    ptr = g.block_jacobi()[0]
+    say("[gpu] block jacobi with target 2048: block sizes %s" % sorted(set(np.diff(ptr).tolist())))
+    assert np.diff(ptr).max() > 1024
+    nb, worst = check_block_inverses(g)
+    say("[gpu]   %d blocks, max rel err %.2e" % (nb, worst))
+    conds = [(1.0, 4.0, 0.0), (1.0, -3.0, 5.0)]
+    rm = g.solve(conds, CP_FP32_REFINE64, tol=1e-12, **P4412)
+    r64 = g.solve(conds, CP_FP64, tol=1e-12, **P4412)
+    for a, b in zip(rm, r64):
+        assert a["rel_residual"] <= 1e-12
+        assert abs(a["CL"] - b["CL"]) <= 1e-9
+    g.release()
+
+
 def bf16_round(x):
     """Round-to-nearest-even of FP32 values to bf16 (kept as float64)."""
     u = np.asarray(x, np.float32).view(np.uint32).astype(np.uint64)
@@ -418,16 +442,21 @@ def test_gmres_inner_same_refined_solution(ctx, monkeypatch, krylov_dim):
         assert abs(a["CDi"] - b["CDi"]) <= 1e-10
 
 
-def test_block_jacobi_mfma_matches_valu(ctx, monkeypatch):
-    """The matrix-core block-Jacobi product and the vector-ALU one give the
-    same refined solutions (both FP32 products of the same inverses)."""
+@pytest.mark.parametrize("bj_prec", ["bf16x3", "bf16x2", "fp32"])
+def test_block_jacobi_mfma_matches_valu(ctx, monkeypatch, bj_prec):
+    """The matrix-core block-Jacobi products (bf16 hi/lo pairs of the
+    inverses, their hi plane alone, or FP32 operands) and the vector-ALU FP32
+    one give the same refined solutions: the preconditioner changes only the
+    inner iterations."""
     m = meshgen.wing("4412", 1.0, 6.0, 40, 12, "uniform", 20.0)
     conds = [(1.0, a, b) for a in (-4.0, 6.0) for b in (-5.0, 2.5)]
     kw = dict(mach=0.1, Sref=6.0, bref=6.0, cref=1.0, cg=(0.25, 0.0, 0.0), tol=1e-12)
-    say("[gpu] block jacobi mfma vs valu")
+    say("[gpu] block jacobi mfma (%s) vs valu" % bj_prec)
+    monkeypatch.setenv("CPGPU_BJ_PREC", bj_prec)
     g1 = ctx.geometry(m)
     g1.assemble(CP_FP32_REFINE)
     r1 = g1.solve(conds, CP_FP32_REFINE, **kw)
+    say("[gpu]   inner iterations %s" % [r["iters"] for r in r1])
     monkeypatch.setenv("CPGPU_BJ_VALU", "1")
     g2 = ctx.geometry(m)
     g2.assemble(CP_FP32_REFINE)
