@@ -268,9 +268,11 @@ void ensure_block_jacobi(cp_geom* g) {
     cp_ctx* c = g->ctx;
     BlockPlan& P = g->bj_plan;
     if (P.perm.empty()) {
-        // CPGPU_BJ_TARGET: panels per block before a strip starts a new one
-        // (512..2048; the inverter's limit is 2048 rows)
-        int target = 2048;
+        // panels per block before a strip starts a new one: n / 12 within
+        // [768, 2048] (larger blocks cut the inner iterations but cost more to
+        // invert, once per geometry: about b / 16 sequential panel steps);
+        // CPGPU_BJ_TARGET (512..2048) sets it
+        int target = std::min(2048, std::max(768, g->dev.n / 12));
         if (const char* e = std::getenv("CPGPU_BJ_TARGET")) {
             const int t = std::atoi(e);
             if (t >= 512 && t <= 2048) target = t;
