@@ -1,6 +1,6 @@
 // Batched inversion of the strip block-Jacobi diagonal blocks on the device.
 //
-// Each block M (b x b, b <= 2048, column-major, FP64) is inverted in place by
+// Each block M (b x b, b <= 3072, column-major, FP64) is inverted in place by
 // Gauss-Jordan elimination with partial pivoting, in a blocked right-looking
 // form over panels of kPw pivot columns and without physical row swaps: the
 // pivot of step k is the row of largest |M[i][k]| among the rows not yet
@@ -8,7 +8,7 @@
 // the textbook form, so the pivot choice is partial pivoting).
 //
 //   panel kernel  one 1024-thread workgroup per block, one row per thread
-//                 (two above 1024 rows), the rows' kPw panel entries in
+//                 (two or three above 1024 rows), the rows' kPw panel entries in
 //                 registers: runs the kPw pivot steps on the panel columns
 //                 only.  Afterwards column
 //                 k0+t holds column p_t of the accumulated transform T, which
@@ -34,7 +34,7 @@ namespace {
 
 constexpr int kPw = 16;        // pivot columns per panel
 constexpr int kPanelT = 1024;  // panel-kernel workgroup: one or two rows per thread
-constexpr int kMaxB = 2048;    // rows per block
+constexpr int kMaxB = 3072;    // rows per block
 constexpr int kUc = 16;        // columns per update workgroup
 constexpr int kUt = 256;       // update / elementwise workgroup size
 
@@ -226,7 +226,7 @@ __global__ __launch_bounds__(kUt) void bjinv_finish_kernel(const double* __restr
 
 void launch_bj_invert(const float* A32, int64_t lda, const int* perm, const int* ptr, const int64_t* off, int nblocks,
                       int max_b, double* X, int* used, int* piv, int* status, float* out, hipStream_t s) {
-    if (max_b > kMaxB) throw HipError{hipErrorInvalidValue, "block-Jacobi block larger than 2048 rows"};
+    if (max_b > kMaxB) throw HipError{hipErrorInvalidValue, "block-Jacobi block larger than 3072 rows"};
     if (nblocks <= 0) return;
     const dim3 gsq((unsigned)(((int64_t)max_b * max_b + kUt - 1) / kUt), (unsigned)nblocks);
     hipLaunchKernelGGL(bjinv_gather_kernel, gsq, dim3(kUt), 0, s, A32, lda, perm, ptr, off, X, used, piv, status);
@@ -236,8 +236,11 @@ void launch_bj_invert(const float* A32, int64_t lda, const int* perm, const int*
         if (max_b <= kPanelT)
             hipLaunchKernelGGL(bjinv_panel_kernel<1>, dim3((unsigned)nblocks), dim3(kPanelT), 0, s, X, ptr, off, k0,
                                used, piv, status);
-        else
+        else if (max_b <= 2 * kPanelT)
             hipLaunchKernelGGL(bjinv_panel_kernel<2>, dim3((unsigned)nblocks), dim3(kPanelT), 0, s, X, ptr, off, k0,
+                               used, piv, status);
+        else
+            hipLaunchKernelGGL(bjinv_panel_kernel<3>, dim3((unsigned)nblocks), dim3(kPanelT), 0, s, X, ptr, off, k0,
                                used, piv, status);
         CP_LAUNCH_CHECK();
         hipLaunchKernelGGL(bjinv_update_kernel, gup, dim3(kUt), 0, s, X, ptr, off, k0, piv, status);

@@ -294,19 +294,20 @@ def test_block_jacobi_inverses_match_lapack(c1s, c1):
         say("[gpu]   %d blocks, max rel err %.2e" % (nb, worst))
 
 
-def test_block_jacobi_blocks_above_1024_rows(ctx, monkeypatch):
-    """Blocks of more than 1024 rows (the panel kernel's two-rows-per-thread
-    form): inverses against LAPACK as above, and the solve refined against
-    the FP64 matrices with them equals the FP64 solve."""
+@pytest.mark.parametrize("target,rows", [(2048, (1024, 2048)), (3072, (2048, 3072))])
+def test_block_jacobi_blocks_above_1024_rows(ctx, monkeypatch, target, rows):
+    """Blocks of more than 1024 rows (the panel kernel's two- and
+    three-rows-per-thread forms): inverses against LAPACK as above, and the
+    solve refined against the FP64 matrices with them equals the FP64 solve."""
     from cpanel_amd import CP_FP32_REFINE64
-    monkeypatch.setenv("CPGPU_BJ_TARGET", "2048")
+    monkeypatch.setenv("CPGPU_BJ_TARGET", str(target))
     m = meshgen.config_c1()
     g = ctx.geometry(m)
     g.assemble(CP_FP32_REFINE)
     g.assemble(CP_FP64)
     ptr = g.block_jacobi()[0]
-    say("[gpu] block jacobi with target 2048: block sizes %s" % sorted(set(np.diff(ptr).tolist())))
-    assert np.diff(ptr).max() > 1024
+    say("[gpu] block jacobi with target %d: block sizes %s" % (target, sorted(set(np.diff(ptr).tolist()))))
+    assert rows[0] < np.diff(ptr).max() <= rows[1]
     nb, worst = check_block_inverses(g)
     say("[gpu]   %d blocks, max rel err %.2e" % (nb, worst))
     conds = [(1.0, 4.0, 0.0), (1.0, -3.0, 5.0)]
