@@ -268,16 +268,7 @@ void ensure_block_jacobi(cp_geom* g) {
     cp_ctx* c = g->ctx;
     BlockPlan& P = g->bj_plan;
     if (P.perm.empty()) {
-        // panels per block before a strip starts a new one: n / 10 within
-        // [768, 3072] (larger blocks cut the inner iterations but cost more to
-        // apply and, once per geometry, to invert: about b / 16 sequential
-        // panel steps); CPGPU_BJ_TARGET (512..3072) sets it
-        int target = std::min(3072, std::max(768, g->dev.n / 10));
-        if (const char* e = std::getenv("CPGPU_BJ_TARGET")) {
-            const int t = std::atoi(e);
-            if (t >= 512 && t <= 3072) target = t;
-        }
-        build_strip_blocks(g->host, target, 3072, P);
+        build_strip_blocks(g->host, bj_block_target(g->dev.n), kBjHardMax, P);
         upload(c, g->bj_perm, P.perm);
         std::vector<int> ip(g->dev.n, 0);
         for (size_t q = 0; q < P.perm.size(); q++) ip[P.perm[q]] = (int)q;

@@ -4,8 +4,18 @@
 #include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <cstdlib>
 
 namespace cpgpu {
+
+int bj_block_target(int n) {
+    int target = std::min(kBjHardMax, std::max(768, n / 10));
+    if (const char* e = std::getenv("CPGPU_BJ_TARGET")) {
+        const int t = std::atoi(e);
+        if (t >= 512 && t <= kBjHardMax) target = t;
+    }
+    return target;
+}
 
 void build_strip_blocks(const Geometry& G, int target, int hard_max, BlockPlan& plan) {
     const int n = (int)G.body.size();

@@ -24,6 +24,12 @@ struct BlockPlan {
     int nblocks() const { return (int)ptr.size() - 1; }
 };
 
+// Default strip-block target for n body panels: n / 10 within [768, 3072]
+// (larger blocks cut the inner iterations but cost more to apply and, once
+// per geometry, to invert); CPGPU_BJ_TARGET (512..3072) overrides it.
+int bj_block_target(int n);
+constexpr int kBjHardMax = 3072;  // rows of the largest block the inverter takes
+
 // Groups the body panels into strip blocks of at most `target` panels (a
 // single strip larger than target stays whole, up to `hard_max`).
 void build_strip_blocks(const Geometry& G, int target, int hard_max, BlockPlan& plan);

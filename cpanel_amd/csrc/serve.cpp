@@ -33,6 +33,7 @@
 #include "../../include/cpgpu.h"
 #include "batched.h"
 #include "device.h"
+#include "precond.h"
 
 using namespace cpgpu;
 
@@ -81,7 +82,9 @@ uint64_t estimate_bytes(const Req& r) {
     uint64_t b = 3ull * lda * (nb + 15) * 4;               // A32, B32, bf16 tiles of A32
     b += (uint64_t)lda * std::max<int64_t>(nw, 1) * 8;     // wake block W
     b += (uint64_t)nb * 120 * 28;                          // near-field CSR (~120 entries per row)
-    b += (uint64_t)nb * 768 * 4;                           // block-Jacobi inverses
+    // block-Jacobi: FP32 inverses + their packed copy + the FP64 inversion
+    // workspace, blocks of about bj_block_target(nb) rows
+    b += (uint64_t)nb * std::min<int64_t>(nb, bj_block_target((int)nb)) * (4 + 4 + 8);
     b += (uint64_t)nb * (16 * 4 + 90 * 8 + 64);            // post tables
     b += (uint64_t)BatchedSolver::workspace_bytes(lda);    // solver workspace
     b += (uint64_t)lda * 64 * 8 * 3;                       // per-call temporaries
