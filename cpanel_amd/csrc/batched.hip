@@ -1117,18 +1117,25 @@ __global__ __launch_bounds__(256) void gm_init_kernel(int n, const double* __res
 }
 
 // sum of the per-block partials of quantity q (fixed order) for column c,
-// with 1024 threads: g = t / 64 sums blocks g, g + 16, ... (eight running sums
-// over blocks g + 128 i + 16 r, r = 0..7, so eight loads are in flight);
-// thread group 0 adds the 16 group sums and returns the total (others: 0)
+// with 1024 threads: g = t / 64 sums blocks g, g + 16, ... (sixteen running
+// sums over blocks g + 256 i + 16 r, r = 0..15, so sixteen loads are in
+// flight); thread group 0 adds the 16 group sums and returns the total
+// (others: 0)
 __device__ inline double colsum1024(const double* __restrict__ d, int nblk, double* sh) {
     const int c = threadIdx.x & 63, g = threadIdx.x >> 6;
-    double r[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    int b = g;
-    for (; b + 112 < nblk; b += 128)
+    double r[16];
 #pragma unroll
-        for (int k = 0; k < 8; k++) r[k] += d[(int64_t)(b + 16 * k) * KP + c];
+    for (int k = 0; k < 16; k++) r[k] = 0;
+    int b = g;
+    for (; b + 240 < nblk; b += 256)
+#pragma unroll
+        for (int k = 0; k < 16; k++) r[k] += d[(int64_t)(b + 16 * k) * KP + c];
     for (; b < nblk; b += 16) r[0] += d[(int64_t)b * KP + c];
-    const double s = ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
+#pragma unroll
+    for (int w = 8; w >= 1; w >>= 1)
+#pragma unroll
+        for (int k = 0; k < w; k++) r[k] += r[k + w];
+    const double s = r[0];
     sh[threadIdx.x] = s;
     __syncthreads();
     double t = 0;
@@ -1213,18 +1220,20 @@ __global__ __launch_bounds__(1024) void gm_hcol_kernel(const double* __restrict_
 }
 
 // u_{j+1} = w - sum_{q <= j} h_{q,j} v_q on the active columns (coef[t][c] =
-// h_{t,j} / s_t), and the partials of ||u_{j+1}||^2: kGmUpPos block-order
-// positions per thread, four basis vectors at a time (32 loads in flight),
-// 4 * kGmUpPos positions per workgroup
+// h_{t,j} / s_t), and the partials of ||u_{j+1}||^2: kGmUpRounds groups of
+// kGmUpPos block-order positions per thread, four basis vectors at a time (32
+// loads in flight), 4 * kGmUpRounds * kGmUpPos positions and one partial per
+// workgroup
 constexpr int kGmUpPos = 8;
+constexpr int kGmUpRounds = 2;
 __global__ __launch_bounds__(256) void gm_update_kernel(int n, const int* __restrict__ act, float* __restrict__ U,
                                                         int64_t us, int j, const float* __restrict__ coef,
                                                         double* __restrict__ dpart) {
     __shared__ double sh[kT];
     const int c = threadIdx.x & 63, g = threadIdx.x >> 6;
-    const int64_t p0 = ((int64_t)blockIdx.x * 4 + g) * kGmUpPos;
     double acc = 0;
-    if (act[c]) {
+    for (int round = 0; round < kGmUpRounds && act[c]; round++) {
+        const int64_t p0 = (((int64_t)blockIdx.x * kGmUpRounds + round) * 4 + g) * kGmUpPos;
         float* w = U + (int64_t)(j + 1) * us + c;
         float v[kGmUpPos];
 #pragma unroll
@@ -2161,7 +2170,7 @@ void BatchedSolver::gmres_sweep(const float* A32, const BatchedOpts& o, int kc, 
         matvec32(A32, y_, nullptr, s);  // w = A~ z
         const int KS = slices(shape_);
         const int ndc = (int)((n_ + kGmDotRows - 1) / kGmDotRows);     // position chunks of the dots
-        const int nuc = (int)((n_ + 4 * kGmUpPos - 1) / (4 * kGmUpPos));  // workgroups of the update
+        const int nuc = (int)((n_ + 4 * kGmUpRounds * kGmUpPos - 1) / (4 * kGmUpRounds * kGmUpPos));  // update WGs
         hipLaunchKernelGGL(gm_w_kernel, dim3(nblocks((int64_t)n_ * (KP / 4))), dim3(kT), 0, s, (const float*)part32_, KS,
                            lda_, n_, (const int*)act_, bj_iperm_, gm_u_ + (j + 1) * us);
         hipLaunchKernelGGL(gm_dots_kernel, dim3((unsigned)ndc, (unsigned)((j + 4) / 4)), dim3(kT), 0, s, n_,
