@@ -175,6 +175,7 @@ private:
     int cus_ = 256;  // compute units of the device
     int bj_kind_ = kBjPackBf16;  // layout of bj_pk_
     bool bj_lo_ = false;         // bf16 product: read the lo plane of the inverses too
+    int bj_rt_ = 2;              // bf16 product: 32-row tiles per workgroup
     int bj_u_ = 1;  // K chunks per load batch of the packed block-Jacobi product
     int bj_nblocks_ = 0, bj_maxb_ = 0;
     bool tuned_ = false, autotune_ = true;

@@ -1601,8 +1601,10 @@ __global__ __launch_bounds__(256) void bj_pack_bf_kernel(const float* __restrict
 // PP rows are split hi + lo as they are loaded, so inv * PP ~ hi_inv (hi_p +
 // lo_p) (+ lo_inv hi_p when LO): LO gives FP32-level accuracy from the same
 // bytes as the FP32 inverses, !LO reads half of them (the inverse to bf16).
-// Work split and reduction as in bj_apply_pk_kernel, with K chunks of 16.
-template <bool LO>
+// A workgroup owns RT 32-row tiles of one block and all 64 columns; its four
+// wavefronts take every fourth K chunk of 16 (each multivector load feeds the
+// RT tiles) and add their accumulators through LDS.
+template <bool LO, int RT>
 __global__ __launch_bounds__(256) void bj_apply_bf_kernel(const int* __restrict__ perm, const int* __restrict__ ptr,
                                                           const int64_t* __restrict__ poff,
                                                           const float* __restrict__ pk,
@@ -1610,23 +1612,29 @@ __global__ __launch_bounds__(256) void bj_apply_bf_kernel(const int* __restrict_
                                                           const double* __restrict__ cscale, int cstride,
                                                           const int* __restrict__ gate) {
     if (gate && gate_idle(gate)) return;
-    __shared__ float red[4][32][64];
+    __shared__ float red[4][RT * 32][64];
     const int q = blockIdx.y;
     const int b0 = ptr[q], b = ptr[q + 1] - b0;
-    const int rt = blockIdx.x;
-    if (rt * 32 >= b) return;  // whole workgroup, before any barrier
+    const int rt0 = blockIdx.x * RT;
+    if (rt0 * 32 >= b) return;  // whole workgroup, before any barrier
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int cl = lane & 31, kh = lane >> 5;
-    const int nkc = (b + 15) / 16;
-    const u32x4* T = reinterpret_cast<const u32x4*>(pk + poff[q]) + (int64_t)rt * nkc * 128 + lane;
+    const int nkc = (b + 15) / 16, ntile = (b + 31) / 32;
+    const u32x4* T = reinterpret_cast<const u32x4*>(pk + poff[q]) + (int64_t)rt0 * nkc * 128 + lane;
     const float* Pq = PP + (int64_t)b0 * KP;
-    floatx16 acc0 = floatx16{}, acc1 = floatx16{};
-    u32x4 ah, al, nah, nal;
+    floatx16 acc[RT][2];
+#pragma unroll
+    for (int r = 0; r < RT; r++) acc[r][0] = acc[r][1] = floatx16{};
+    u32x4 ah[RT], al[RT], nah[RT], nal[RT];
     float x0[8], x1[8], nx0[8], nx1[8];
-    auto load = [&](int c, u32x4& h, u32x4& l, float (&y0)[8], float (&y1)[8]) {
+    auto load = [&](int c, u32x4 (&h)[RT], u32x4 (&l)[RT], float (&y0)[8], float (&y1)[8]) {
         const bool cok = c < nkc;
-        h = cok ? __builtin_nontemporal_load(T + (int64_t)c * 128) : u32x4{};
-        if (LO) l = cok ? __builtin_nontemporal_load(T + (int64_t)c * 128 + 64) : u32x4{};
+#pragma unroll
+        for (int r = 0; r < RT; r++) {
+            const bool ok = cok && rt0 + r < ntile;  // tiles past the block's last are not stored
+            h[r] = ok ? __builtin_nontemporal_load(T + ((int64_t)r * nkc + c) * 128) : u32x4{};
+            if (LO) l[r] = ok ? __builtin_nontemporal_load(T + ((int64_t)r * nkc + c) * 128 + 64) : u32x4{};
+        }
 #pragma unroll
         for (int i = 0; i < 8; i++) {
             const int k = 16 * c + 8 * kh + i;
@@ -1642,19 +1650,27 @@ __global__ __launch_bounds__(256) void bj_apply_bf_kernel(const int* __restrict_
         uint4 h0, l0, h1, l1;
         split8(x0, h0, l0);
         split8(x1, h1, l1);
-        const bf16x8 a = __builtin_bit_cast(bf16x8, ah);
-        acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, __builtin_bit_cast(bf16x8, l0), acc0, 0, 0, 0);
-        acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, __builtin_bit_cast(bf16x8, l1), acc1, 0, 0, 0);
-        if (LO) {
-            const bf16x8 alo = __builtin_bit_cast(bf16x8, al);
-            acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo, __builtin_bit_cast(bf16x8, h0), acc0, 0, 0, 0);
-            acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo, __builtin_bit_cast(bf16x8, h1), acc1, 0, 0, 0);
+#pragma unroll
+        for (int r = 0; r < RT; r++) {
+            const bf16x8 a = __builtin_bit_cast(bf16x8, ah[r]);
+            acc[r][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, __builtin_bit_cast(bf16x8, l0), acc[r][0], 0, 0, 0);
+            acc[r][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, __builtin_bit_cast(bf16x8, l1), acc[r][1], 0, 0, 0);
+            if (LO) {
+                const bf16x8 alo = __builtin_bit_cast(bf16x8, al[r]);
+                acc[r][0] =
+                    __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo, __builtin_bit_cast(bf16x8, h0), acc[r][0], 0, 0, 0);
+                acc[r][1] =
+                    __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo, __builtin_bit_cast(bf16x8, h1), acc[r][1], 0, 0, 0);
+            }
+            acc[r][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, __builtin_bit_cast(bf16x8, h0), acc[r][0], 0, 0, 0);
+            acc[r][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, __builtin_bit_cast(bf16x8, h1), acc[r][1], 0, 0, 0);
         }
-        acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, __builtin_bit_cast(bf16x8, h0), acc0, 0, 0, 0);
-        acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, __builtin_bit_cast(bf16x8, h1), acc1, 0, 0, 0);
         if (more) {
-            ah = nah;
-            if (LO) al = nal;
+#pragma unroll
+            for (int r = 0; r < RT; r++) {
+                ah[r] = nah[r];
+                if (LO) al[r] = nal[r];
+            }
 #pragma unroll
             for (int i = 0; i < 8; i++) {
                 x0[i] = nx0[i];
@@ -1663,10 +1679,12 @@ __global__ __launch_bounds__(256) void bj_apply_bf_kernel(const int* __restrict_
         }
     }
 #pragma unroll
-    for (int j = 0; j < 16; j++) {
-        red[wave][j][lane] = acc0[j];
-        red[wave][16 + j][lane] = acc1[j];
-    }
+    for (int r = 0; r < RT; r++)
+#pragma unroll
+        for (int j = 0; j < 16; j++) {
+            red[wave][r * 32 + j][lane] = acc[r][0][j];
+            red[wave][r * 32 + 16 + j][lane] = acc[r][1][j];
+        }
     __syncthreads();
     float f0 = 1.f, f1 = 1.f;  // optional per-column factor 1 / cscale[c * cstride] (0 where that is 0)
     if (cscale) {
@@ -1675,14 +1693,16 @@ __global__ __launch_bounds__(256) void bj_apply_bf_kernel(const int* __restrict_
         f1 = s1 != 0 ? (float)(1.0 / s1) : 0.f;
     }
     const int* pq = perm + b0;
+    // wavefront w finishes entries 8 RT w .. 8 RT w + 8 RT - 1 of the 32 RT
+    // (tile r: entries 32 r + 0..15 of acc[r][0], 32 r + 16..31 of acc[r][1])
 #pragma unroll
-    for (int i = 0; i < 8; i++) {
-        const int jj = 8 * wave + i;  // 0..15: acc0 entry jj, 16..31: acc1 entry jj - 16
+    for (int i = 0; i < 8 * RT; i++) {
+        const int jj = 8 * RT * wave + i;
         const float v = red[0][jj][lane] + red[1][jj][lane] + red[2][jj][lane] + red[3][jj][lane];
-        const int j = jj & 15;
-        const int rr = rt * 32 + 8 * (j >> 2) + 4 * kh + (j & 3);
+        const int r = jj >> 5, e = jj & 31, j = e & 15;
+        const int rr = (rt0 + r) * 32 + 8 * (j >> 2) + 4 * kh + (j & 3);
         if (rr < b) {
-            if (jj < 16)
+            if (e < 16)
                 Y[(int64_t)pq[rr] * KP + cl] = v * f0;
             else
                 Y[(int64_t)pq[rr] * KP + 32 + cl] = v * f1;
@@ -1835,6 +1855,8 @@ BatchedSolver::BatchedSolver(int n, int64_t lda, int capacity, void* ws, void* h
     // the inverses to bf16 -- a preconditioner changes only the inner
     // iterations, and on the 25,916-panel wing not even those)
     if (const char* e = std::getenv("CPGPU_BJ_PREC")) bj_lo_ = std::string(e) == "bf16x3";
+    // CPGPU_BJ_RT=1|2: 32-row tiles per workgroup of the bf16 block-Jacobi product
+    if (const char* e = std::getenv("CPGPU_BJ_RT")) bj_rt_ = std::atoi(e) == 1 ? 1 : 2;
     // CPGPU_BJ_U: K chunks per wavefront per load batch of the FP32
     // block-Jacobi product (1..3)
     if (const char* e = std::getenv("CPGPU_BJ_U")) {
@@ -2119,8 +2141,13 @@ void BatchedSolver::bj_apply_pk(const float* PP, float* Y, const double* cscale,
         hipLaunchKernelGGL(kern, grid, dim3(kT), 0, s, bj_perm_, bj_ptr_, bj_poff_, bj_pk_, PP, Y, cscale, cstride,
                            gate_);
     };
-    if (bj_kind_ == kBjPackBf16)
-        go(bj_lo_ ? bj_apply_bf_kernel<true> : bj_apply_bf_kernel<false>);
+    if (bj_kind_ == kBjPackBf16 && bj_rt_ == 2) {
+        const dim3 grid2((unsigned)((bj_maxb_ + 63) / 64), (unsigned)bj_nblocks_);
+        auto kern = bj_lo_ ? bj_apply_bf_kernel<true, 2> : bj_apply_bf_kernel<false, 2>;
+        hipLaunchKernelGGL(kern, grid2, dim3(kT), 0, s, bj_perm_, bj_ptr_, bj_poff_, bj_pk_, PP, Y, cscale, cstride,
+                           gate_);
+    } else if (bj_kind_ == kBjPackBf16)
+        go(bj_lo_ ? bj_apply_bf_kernel<true, 1> : bj_apply_bf_kernel<false, 1>);
     else if (bj_u_ == 1)
         go(bj_apply_pk_kernel<1>);
     else if (bj_u_ == 3)
