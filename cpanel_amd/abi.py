@@ -100,6 +100,7 @@ SIGNATURES = {
     "cp_geometry_clusters": (C.c_int, [C.c_void_p, _i32, _i32, _i32]),
     "cp_assemble": (C.c_int, [C.c_void_p, C.c_int]),
     "cp_read_matrices": (C.c_int, [C.c_void_p, C.c_int, _d, _d, C.c_int64]),
+    "cp_load_matrices": (C.c_int, [C.c_void_p, _d, _d, C.c_int64]),
     "cp_solve": (C.c_int, [C.c_void_p, C.c_int, C.c_int32, C.POINTER(Freestream), C.POINTER(SolveOpts), _d, _d,
                            C.POINTER(SolveInfo)]),
     "cp_matvec": (C.c_int, [C.c_void_p, C.c_int, C.c_int32, _d, _d]),

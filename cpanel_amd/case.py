@@ -156,6 +156,17 @@ class Geometry:
         self.ctx.check(abi.lib().cp_read_matrices(self.h, prec, _p(A), _p(B), n), "read matrices")
         return A, B
 
+    def load_matrices(self, A, B):
+        """Set the FP64 A and B (n x n) in place of assemble(CP_FP64), as the
+        reference does when it reads a .infCoeff file (geometry::readInfCoeff,
+        geometry.cpp:1125-1166; cpanel_amd.infcoeff reads the file)."""
+        n = self.nb
+        A = np.asfortranarray(A, dtype=np.float64)
+        B = np.asfortranarray(B, dtype=np.float64)
+        if A.shape != (n, n) or B.shape != (n, n):
+            raise ValueError("A and B must be %d x %d" % (n, n))
+        self.ctx.check(abi.lib().cp_load_matrices(self.h, _p(A), _p(B), n), "load matrices")
+
     def matvec(self, X, kernel=0):
         """Y = A X through one device kernel (0: FP64 GEMV, 1: FP32 MFMA, 2: FP64-acc MFMA,
         3: FP64-acc MFMA + FP64 near-field correction)."""

@@ -94,6 +94,8 @@ public:
     // FP64 matrices for the refinement residual and the right-hand side (the
     // refined solution is then the FP64 system's); null = FP32 + near-field
     // correction
+    // forget the cached -B n columns (the matrices were replaced)
+    void invalidate_rhs_cache() { bn_mode_ = -1; }
     void set_f64(const double* A64, const double* B64) {
         a64_ = A64;
         b64_ = B64;

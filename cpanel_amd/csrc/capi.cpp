@@ -547,6 +547,32 @@ cp_status cp_assemble(cp_geom* g, cp_prec prec) {
     });
 }
 
+cp_status cp_load_matrices(cp_geom* g, const double* A, const double* B, int64_t ld) {
+    if (!g || !A || !B || ld < g->dev.n) return CP_BAD_ARG;
+    cp_ctx* c = g->ctx;
+    if (c->device < 0) return fail(c, CP_NO_DEVICE, "host-only context");
+    return guarded_g(g, [&] {
+        const int n = g->dev.n;
+        const int64_t lda = g->dev.lda;
+        const size_t elems = (size_t)lda * n;
+        if (!g->have64) {
+            dalloc(c, g->A64, elems * sizeof(double));
+            dalloc(c, g->B64, elems * sizeof(double));
+        }
+        // rows n..lda of each column stay zero, as the assembly leaves them
+        CP_HIP(hipMemsetAsync(g->A64.p, 0, elems * sizeof(double), g->stream));
+        CP_HIP(hipMemsetAsync(g->B64.p, 0, elems * sizeof(double), g->stream));
+        CP_HIP(hipMemcpy2DAsync(g->A64.p, lda * sizeof(double), A, ld * sizeof(double), n * sizeof(double), n,
+                                hipMemcpyHostToDevice, g->stream));
+        CP_HIP(hipMemcpy2DAsync(g->B64.p, lda * sizeof(double), B, ld * sizeof(double), n * sizeof(double), n,
+                                hipMemcpyHostToDevice, g->stream));
+        CP_HIP(hipStreamSynchronize(g->stream));
+        g->have64 = true;
+        if (g->batched) g->batched->invalidate_rhs_cache();  // its -B n columns came from the old B
+        return CP_OK;
+    });
+}
+
 cp_status cp_read_matrices(cp_geom* g, cp_prec prec, double* A, double* B, int64_t ld) {
     if (!g || ld < g->dev.n) return CP_BAD_ARG;
     cp_ctx* c = g->ctx;

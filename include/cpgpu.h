@@ -101,6 +101,13 @@ cp_status cp_geometry_clusters(const cp_geom* g, int32_t* cluster16, int32_t* ts
 cp_status cp_assemble(cp_geom* g, cp_prec prec);
 /* copy A and B (as FP64) to host arrays with leading dimension ld */
 cp_status cp_read_matrices(cp_geom* g, cp_prec prec, double* A, double* B, int64_t ld);
+/* Sets the FP64 matrices A and B (n x n, column-major, leading dimension
+   ld >= n) of a geometry to caller-given ones, in place of
+   cp_assemble(CP_FP64): replaces geometry::readInfCoeff (geometry.cpp:1125-1166),
+   which loads A and B from a .infCoeff file instead of computing them
+   (cpanel_amd/infcoeff.py reads and writes that format). The FP32 matrices of
+   the mixed paths are not touched (cp_assemble(CP_FP32_REFINE) builds them). */
+cp_status cp_load_matrices(cp_geom* g, const double* A, const double* B, int64_t ld);
 
 /* ---- steady solve (cpCase::run, cpCase.cpp:31-139) ----------------------- */
 typedef struct {

@@ -572,3 +572,36 @@ def test_c2_refine64_cp_all_panels(ctx):
     dcp = max((np.abs(a["cp"] - b["cp"]) / np.maximum(1.0, np.abs(a["cp"]))).max() for a, b in zip(r64, r32))
     say("[gpu]   C2 max dCL %.2e, max dCp/max(1,|Cp|) %.2e" % (dcl, dcp))
     assert dcl <= 1e-5 and dcp <= 1e-4
+
+
+def test_load_matrices_from_infcoeff(ctx, tmp_path):
+    """A and B written to a .infCoeff file and loaded back (cp_load_matrices,
+    the counterpart of geometry::readInfCoeff) solve exactly as the assembled
+    ones; loading 2 B doubles every solution (the solver's cached -B n
+    columns follow the new B)."""
+    from cpanel_amd import CP_FP32_REFINE64, infcoeff
+    m = meshgen.wing("4412", 1.0, 6.0, 40, 12, "uniform", 20.0)
+    conds = [(1.0, 4.0, 0.0), (1.0, -2.0, 3.0)]
+    kw = dict(tol=1e-12, **P4412)
+    g1 = ctx.geometry(m)
+    g1.assemble(CP_FP32_REFINE64)
+    A, B = g1.matrices(CP_FP64)
+    r64 = g1.solve(conds, CP_FP64, **kw)
+    rmx = g1.solve(conds, CP_FP32_REFINE64, **kw)
+    path = tmp_path / "wing.infCoeff"
+    infcoeff.write_infcoeff(path, A, B, precision=17)
+    assert infcoeff.matches(path, g1.nb)
+    A2, B2, _ = infcoeff.read_infcoeff(path)
+    say("[gpu] load matrices n=%d" % g1.nb)
+    g2 = ctx.geometry(m)
+    g2.load_matrices(A2, B2)
+    g2.assemble(CP_FP32_REFINE)
+    for a, b in zip(g2.solve(conds, CP_FP64, **kw), r64):
+        assert np.array_equal(a["mu"], b["mu"])
+    for a, b in zip(g2.solve(conds, CP_FP32_REFINE64, **kw), rmx):
+        assert np.array_equal(a["mu"], b["mu"])
+    g2.load_matrices(A2, 2.0 * B2)
+    for a, b in zip(g2.solve(conds, CP_FP32_REFINE64, **kw), rmx):
+        assert np.linalg.norm(a["mu"] - 2.0 * b["mu"]) <= 1e-10 * np.linalg.norm(b["mu"])
+    g1.release()
+    g2.release()
