@@ -1092,8 +1092,7 @@ constexpr int GM_SN = GM_CS + kGmMax;
 constexpr int GM_G = GM_SN + kGmMax;            // kGmMax + 1
 constexpr int GM_S = GM_G + kGmMax + 1;         // kGmMax + 1 norms s_q
 constexpr int GM_Y = GM_S + kGmMax + 1;         // kGmMax coefficients
-constexpr int GM_C = GM_Y + kGmMax;             // kGmMax + 1: h_{q,j} / s_q of the current j
-constexpr int GM_NJ = GM_C + kGmMax + 1;        // iterations taken
+constexpr int GM_NJ = GM_Y + kGmMax;            // iterations taken
 constexpr int kGmStride = GM_NJ + 1;
 
 // u_0 = R / scale in block order on the sweeping columns; ||u_0||^2 partials
