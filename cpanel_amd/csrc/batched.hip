@@ -2009,11 +2009,15 @@ void BatchedSolver::tune(const float* A32, hipStream_t s) {
         }
     }
     // candidates: row tiles per wavefront x K slices; the row-block count must
-    // divide lda (a multiple of 512)
+    // divide lda (a multiple of 512).  With the bf16x2 product available only
+    // its variants are timed: it reads half the bytes of bf16x3 and a quarter
+    // of FP32 and has won at every size measured, so timing the others only
+    // lengthens the first solve of every new size (a serving stream meets
+    // many); CPGPU_NO_BF16X2=1 brings them back.
     std::vector<GemmShape> cand;
     for (int rt : {1, 2, 4})
         for (int ks : {4, 8, 16, 32}) {
-            if (lda_ % (128 * rt) != 0) continue;
+            if (use2 || lda_ % (128 * rt) != 0) continue;
             GemmShape g;
             g.rt = rt;
             g.kslices = ks;
@@ -2026,7 +2030,7 @@ void BatchedSolver::tune(const float* A32, hipStream_t s) {
         for (int ks : {4, 5, 6, 8, 10, 12, 16, 32}) {
             if (v < 10 && (ks == 5 || ks == 6 || ks == 10 || ks == 12)) continue;
             if (v >= 5 && !use3) continue;
-            if (v >= 10 && !use2) continue;
+            if ((v >= 10) != use2) continue;
             GemmShape g;
             g.variant = v;
             g.rt = (v == 13 || v == 14) ? 2 : 4;
