@@ -169,10 +169,10 @@ CP_HD void vortex_v(const double* a, const double* b, const double* s, double* o
     double nb = sqrt(sq_norm3(b[0], b[1], b[2]));
     double cs = kCore * sqrt(sq_norm3(s[0], s[1], s[2]));
     double den = na * nb * (na * nb + (a[0] * b[0] + a[1] * b[1] + a[2] * b[2])) + cs * cs;
-    double f = na + nb;
-    out[0] = (a[1] * b[2] - a[2] * b[1]) * f / den;
-    out[1] = (a[2] * b[0] - a[0] * b[2]) * f / den;
-    out[2] = (a[0] * b[1] - a[1] * b[0]) * f / den;
+    const double f = (na + nb) / den;  // one division for the three components
+    out[0] = (a[1] * b[2] - a[2] * b[1]) * f;
+    out[1] = (a[2] * b[0] - a[0] * b[2]) * f;
+    out[2] = (a[0] * b[1] - a[1] * b[0]) * f;
 }
 
 // panel::pntDubV (panel.cpp:773-776): area (3 (pjk.n) pjk - r^2 n) / (4 pi r^5)
@@ -180,8 +180,9 @@ CP_HD void pnt_dub_v(const PanelRec& P, const double* pjk, double r, double* out
 #pragma clang fp contract(off)
     double pn3 = 3 * (pjk[0] * P.n[0] + pjk[1] * P.n[1] + pjk[2] * P.n[2]);
     double r2 = r * r;
-    double den = 4 * kPI * pow(r, 5);
-    for (int k = 0; k < 3; k++) out[k] = P.area * (pn3 * pjk[k] - r2 * P.n[k]) / den;
+    double den = 4 * kPI * (r2 * r2 * r);  // r^5 (the reference's pow(r, 5), to a few ulp)
+    const double f = P.area / den;         // one division for the three components
+    for (int k = 0; k < 3; k++) out[k] = (pn3 * pjk[k] - r2 * P.n[k]) * f;
 }
 
 // bodyPanel::panelVInf (bodyPanel.cpp:305-348) in FP64: source and doublet
@@ -193,8 +194,9 @@ CP_HD void panel_v_inf_f64(const PanelRec& P, const double* p, double* vs, doubl
     double r2 = sq_norm3(pjk[0], pjk[1], pjk[2]);
     if (is_far(P, r2)) {
         double r = sqrt(r2);
-        double den = 4 * kPI * pow(r, 3);
-        for (int k = 0; k < 3; k++) vs[k] = -1.0 * (P.area * pjk[k] / den);
+        double den = 4 * kPI * (r * r * r);  // r^3 (the reference's pow(r, 3), to a few ulp)
+        const double f = P.area / den;
+        for (int k = 0; k < 3; k++) vs[k] = -1.0 * (pjk[k] * f);
         pnt_dub_v(P, pjk, r, vd);
         return;
     }

@@ -37,17 +37,19 @@ CP_HD void vg_point_source_add(const PanelRec& P, double sigma, const double* po
 #pragma clang fp contract(off)
     double x = P.c[0], y = P.c[1], z = P.c[2], x0 = poi[0], y0 = poi[1], z0 = poi[2];
     double dx = x - x0, dy = y - y0;
-    double sdd = pow(dx * dx + dy * dy + z * z, 2.5);
+    const double q2 = dx * dx + dy * dy + z * z;
+    double sdd = q2 * q2 * sqrt(q2);  // q^2.5 (the reference's pow(q, 2.5), to a few ulp)
+    const double isdd = 1 / sdd;  // one division for the nine entries
     double c = sigma * P.area / (4 * kPI);
-    G.a[0][0] += c * ((-2 * dx * dx + dy * dy + z * z) / sdd);
-    G.a[1][0] += c * (-3 * dx * dy / sdd);
-    G.a[2][0] += c * (-3 * dx * z / sdd);
-    G.a[0][1] += c * (-3 * dx * dy / sdd);
-    G.a[1][1] += c * ((dx * dx - dy * dy + z * z) / sdd);
-    G.a[2][1] += c * (-3 * z * dy / sdd);
-    G.a[0][2] += c * (-3 * dx * (z - z0) / sdd);
-    G.a[1][2] += c * (-3 * dy * (z - z0) / sdd);
-    G.a[2][2] += c * ((-3 * z * (z - z0) + (dx * dx + dy * dy + z * z)) / sdd);
+    G.a[0][0] += c * ((-2 * dx * dx + dy * dy + z * z) * isdd);
+    G.a[1][0] += c * (-3 * dx * dy * isdd);
+    G.a[2][0] += c * (-3 * dx * z * isdd);
+    G.a[0][1] += c * (-3 * dx * dy * isdd);
+    G.a[1][1] += c * ((dx * dx - dy * dy + z * z) * isdd);
+    G.a[2][1] += c * (-3 * z * dy * isdd);
+    G.a[0][2] += c * (-3 * dx * (z - z0) * isdd);
+    G.a[1][2] += c * (-3 * dy * (z - z0) * isdd);
+    G.a[2][2] += c * ((-3 * z * (z - z0) + (dx * dx + dy * dy + z * z)) * isdd);
 }
 
 // bodyPanel::velocityGradientTriSource / QuadSource (bodyPanel.cpp:766-1063):
@@ -108,18 +110,20 @@ CP_HD void vg_point_doublet_add(const PanelRec& P, double mu, const double* poi,
 #pragma clang fp contract(off)
     double x = P.c[0], y = P.c[1], z = P.c[2], x0 = poi[0], y0 = poi[1];
     double dx2 = (x - x0) * (x - x0), dy2 = (y - y0) * (y - y0);
-    double ddd = pow(dx2 + dy2 + z * z, 3.5);
+    const double q2 = dx2 + dy2 + z * z;
+    double ddd = q2 * q2 * q2 * sqrt(q2);  // q^3.5 (the reference's pow(q, 3.5), to a few ulp)
+    const double iddd = 1 / ddd;  // one division for the nine entries
     double uv = 3 * mu * P.area / (4 * kPI);
     double w = -mu * P.area / (4 * kPI);
-    G.a[0][0] += uv * z * (-4 * dx2 + dy2 + z * z) / ddd;
-    G.a[1][0] += uv * (-5) * z * (y - y0) * (x - x0) / ddd;
-    G.a[2][0] += uv * (x - x0) * (dx2 + dy2 - 4 * z * z) / ddd;
-    G.a[0][1] += uv * (-5) * (x - x0) * (y - y0) * z / ddd;
-    G.a[1][1] += uv * z * (dx2 - 4 * dy2 + z * z) / ddd;
-    G.a[2][1] += uv * (y - y0) * (dx2 + dy2 - 4 * z * z) / ddd;
-    G.a[0][2] += w * (-3) * (x - x0) * (dx2 + dy2 - 4 * z * z) / ddd;
-    G.a[1][2] += w * (-3) * (y - y0) * (dx2 + dy2 - 4 * z * z) / ddd;
-    G.a[2][2] += w * (-3) * z * (3 * dx2 + 3 * dy2 - 2 * z * z) / ddd;
+    G.a[0][0] += uv * z * (-4 * dx2 + dy2 + z * z) * iddd;
+    G.a[1][0] += uv * (-5) * z * (y - y0) * (x - x0) * iddd;
+    G.a[2][0] += uv * (x - x0) * (dx2 + dy2 - 4 * z * z) * iddd;
+    G.a[0][1] += uv * (-5) * (x - x0) * (y - y0) * z * iddd;
+    G.a[1][1] += uv * z * (dx2 - 4 * dy2 + z * z) * iddd;
+    G.a[2][1] += uv * (y - y0) * (dx2 + dy2 - 4 * z * z) * iddd;
+    G.a[0][2] += w * (-3) * (x - x0) * (dx2 + dy2 - 4 * z * z) * iddd;
+    G.a[1][2] += w * (-3) * (y - y0) * (dx2 + dy2 - 4 * z * z) * iddd;
+    G.a[2][2] += w * (-3) * z * (3 * dx2 + 3 * dy2 - 2 * z * z) * iddd;
 }
 
 // panel::gradDoub (panel.cpp:876-923) scaled by s and added to G; only

@@ -40,7 +40,8 @@ __device__ inline void k_alg(const Src& s, const double* p, double* acc) {
     double dx = p[0] - s.x, dy = p[1] - s.y, dz = p[2] - s.z;
     double r = nrm3(dx, dy, dz);
     double r2 = r * r, s2 = sigma * sigma;
-    double f = -1 / (4 * kPI) * (r2 + 2.5 * s2) / pow(r2 + s2, 2.5);
+    const double q = r2 + s2;
+    double f = -1 / (4 * kPI) * (r2 + 2.5 * s2) / (q * q * sqrt(q));  // (r^2 + s^2)^2.5
     acc[0] += f * (dy * s.az - dz * s.ay);
     acc[1] += f * (dz * s.ax - dx * s.az);
     acc[2] += f * (dx * s.ay - dy * s.ax);
