@@ -236,7 +236,11 @@ def main():
     # condition come back (what the validity check needs)
     for _ in range(args.warmup):
         g.solve(conds, mprec, tol=1e-10, inner_tol=args.inner_tol, want_mu=False, want_cp=False, **kw)
+    # in the timed steps only the dominant kernel's launches carry HIP events
+    # (the roofline's duration); the other kernels' timings come from the
+    # host-copy step below, which records them all
     ctx.timing_reset()
+    ctx.timing_only("gemm_f32")
     times = []
     res = None
     for _ in range(args.steps):
@@ -245,13 +249,17 @@ def main():
         times.append(time.perf_counter() - t0)
     t_step = float(np.mean(times))
     mm_ms, mm_n = ctx.timing("gemm_f32")
-    rs_ms, rs_n = ctx.timing("resid_f64")
-    bj_ms, bj_n = ctx.timing("bj_apply")
     # the same step with mu and Cp of all 64 conditions copied to host memory
     # (PCIe-inclusive; reported beside value, never as value)
     t0 = time.perf_counter()
     res_full = g.solve(conds, mprec, tol=1e-10, inner_tol=args.inner_tol, **kw)
     t_step_host = time.perf_counter() - t0
+    # one more step outside both measurements with every kernel timed (log only)
+    ctx.timing_only(None)
+    ctx.timing_reset()
+    g.solve(conds, mprec, tol=1e-10, inner_tol=args.inner_tol, want_mu=False, want_cp=False, **kw)
+    rs_ms, rs_n = ctx.timing("resid_f64")
+    bj_ms, bj_n = ctx.timing("bj_apply")
     gemm_avg = mm_ms / max(mm_n, 1)
     gemm_flops = 2.0 * n * n * 64
     achieved = gemm_flops / (gemm_avg * 1e-3) / 1e12

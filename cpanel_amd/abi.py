@@ -134,6 +134,7 @@ SIGNATURES = {
     "cp_server_destroy": (None, [C.c_void_p]),
     "cp_timing_reset": (C.c_int, [C.c_void_p]),
     "cp_timing_enable": (C.c_int, [C.c_void_p, C.c_int]),
+    "cp_timing_only": (C.c_int, [C.c_void_p, C.c_char_p]),
     "cp_timing_get": (C.c_int, [C.c_void_p, C.c_char_p, _d, _i64]),
 }
 

@@ -74,6 +74,10 @@ class Context:
     def timing_enable(self, on=True):
         abi.lib().cp_timing_enable(self.h, 1 if on else 0)
 
+    def timing_only(self, name=None):
+        """Record only `name`'s intervals while timing is on (None: all)."""
+        self.check(abi.lib().cp_timing_only(self.h, None if name is None else name.encode()))
+
     def timing_reset(self):
         self.check(abi.lib().cp_timing_reset(self.h))
 

@@ -946,6 +946,12 @@ cp_status cp_timing_enable(cp_ctx* c, int on) {
     return CP_OK;
 }
 
+cp_status cp_timing_only(cp_ctx* c, const char* kernel) {
+    if (!c) return CP_BAD_ARG;
+    c->timer.only = kernel ? kernel : "";
+    return CP_OK;
+}
+
 cp_status cp_timing_get(cp_ctx* c, const char* kernel, double* total_ms, int64_t* launches) {
     if (!c || !kernel) return CP_BAD_ARG;
     return guarded(c, [&] {

@@ -13,6 +13,10 @@ namespace cpgpu {
 class KernelTimer {
 public:
     bool enabled = false;
+    // when not empty, only intervals under this name are recorded (the others
+    // cost their event records for nothing when one kernel is being measured)
+    std::string only;
+    bool wants(const char* name) const { return enabled && (only.empty() || only == name); }
     ~KernelTimer();
     // an event from the pool (thread safe)
     hipEvent_t take();
@@ -49,7 +53,7 @@ struct TimedScope {
     const char* name;
     hipEvent_t a = nullptr;
     TimedScope(const char* nm, hipStream_t st) : t(current_timer()), s(st), name(nm) {
-        if (t && t->enabled) {
+        if (t && t->wants(nm)) {
             a = t->take();
             (void)hipEventRecord(a, s);
         }

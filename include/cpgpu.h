@@ -359,6 +359,10 @@ cp_status cp_timing_reset(cp_ctx* ctx);
  * returns total milliseconds and launch count since the last reset */
 cp_status cp_timing_get(cp_ctx* ctx, const char* kernel, double* total_ms, int64_t* launches);
 cp_status cp_timing_enable(cp_ctx* ctx, int on);
+/* records only the named kernel's intervals while timing is on (NULL or "":
+ * every kernel); no counterpart in the reference: for measuring one kernel
+ * without the event records of the others in the timed region */
+cp_status cp_timing_only(cp_ctx* ctx, const char* kernel);
 
 #ifdef __cplusplus
 }
