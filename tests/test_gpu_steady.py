@@ -443,7 +443,7 @@ def test_gmres_inner_same_refined_solution(ctx, monkeypatch, krylov_dim):
         assert abs(a["CDi"] - b["CDi"]) <= 1e-10
 
 
-@pytest.mark.parametrize("bj_prec", ["bf16x3", "bf16x2", "fp32"])
+@pytest.mark.parametrize("bj_prec", ["bf16x3", "bf16x2", "bf16x2-rt1", "fp32"])
 def test_block_jacobi_mfma_matches_valu(ctx, monkeypatch, bj_prec):
     """The matrix-core block-Jacobi products (bf16 hi/lo pairs of the
     inverses, their hi plane alone, or FP32 operands) and the vector-ALU FP32
@@ -453,7 +453,9 @@ def test_block_jacobi_mfma_matches_valu(ctx, monkeypatch, bj_prec):
     conds = [(1.0, a, b) for a in (-4.0, 6.0) for b in (-5.0, 2.5)]
     kw = dict(mach=0.1, Sref=6.0, bref=6.0, cref=1.0, cg=(0.25, 0.0, 0.0), tol=1e-12)
     say("[gpu] block jacobi mfma (%s) vs valu" % bj_prec)
-    monkeypatch.setenv("CPGPU_BJ_PREC", bj_prec)
+    if bj_prec.endswith("-rt1"):  # one 32-row tile per workgroup
+        monkeypatch.setenv("CPGPU_BJ_RT", "1")
+    monkeypatch.setenv("CPGPU_BJ_PREC", bj_prec.split("-")[0])
     g1 = ctx.geometry(m)
     g1.assemble(CP_FP32_REFINE)
     r1 = g1.solve(conds, CP_FP32_REFINE, **kw)
@@ -605,3 +607,28 @@ def test_load_matrices_from_infcoeff(ctx, tmp_path):
         assert np.linalg.norm(a["mu"] - 2.0 * b["mu"]) <= 1e-10 * np.linalg.norm(b["mu"])
     g1.release()
     g2.release()
+
+
+def test_timing_only_records_the_named_kernel(ctx):
+    """cp_timing_only: with a name set only that kernel's launches are timed;
+    cleared, every timed kernel is recorded again."""
+    from cpanel_amd import CP_FP32_REFINE64
+    m = meshgen.wing("4412", 1.0, 6.0, 40, 12, "uniform", 20.0)
+    g = ctx.geometry(m)
+    g.assemble(CP_FP32_REFINE64)
+    conds = [(1.0, 4.0, 0.0), (1.0, -2.0, 3.0)]
+    ctx.timing_enable(True)
+    try:
+        ctx.timing_reset()
+        ctx.timing_only("gemm_f32")
+        g.solve(conds, CP_FP32_REFINE64, tol=1e-10, **P4412)
+        assert ctx.timing("gemm_f32")[1] > 0
+        assert ctx.timing("bj_apply")[1] == 0 and ctx.timing("resid_f64")[1] == 0
+        ctx.timing_only(None)
+        ctx.timing_reset()
+        g.solve(conds, CP_FP32_REFINE64, tol=1e-10, **P4412)
+        assert ctx.timing("gemm_f32")[1] > 0 and ctx.timing("bj_apply")[1] > 0 and ctx.timing("resid_f64")[1] > 0
+    finally:
+        ctx.timing_only(None)
+        ctx.timing_enable(False)
+        g.release()
