@@ -22,6 +22,13 @@ class CPanelError(RuntimeError):
         self.status = status
 
 
+# cp_solve_info as a NumPy record (the layout of abi.SolveInfo)
+_SOLVE_INFO = np.dtype([("CL", "<f8"), ("CDi", "<f8"), ("Fbody", "<f8", (3,)), ("Fwind", "<f8", (3,)),
+                        ("CM", "<f8", (3,)), ("rel_residual", "<f8"), ("iters", "<i4"), ("refinements", "<i4"),
+                        ("converged", "<i4"), ("pad", "<i4")])
+assert _SOLVE_INFO.itemsize == C.sizeof(abi.SolveInfo)
+
+
 def _p(a, t=C.c_double):
     return None if a is None else a.ctypes.data_as(C.POINTER(t))
 
@@ -331,7 +338,8 @@ class Geometry:
         False leave the per-panel arrays on the device (the forces, CL and
         CDi still come back)."""
         k = len(conditions)
-        fs = (abi.Freestream * k)(*[abi.Freestream(*map(float, c)) for c in conditions])
+        # the k cp_freestream records as one (k, 3) FP64 buffer
+        fs = (abi.Freestream * k).from_buffer_copy(np.ascontiguousarray(conditions, dtype=np.float64).reshape(k, 3))
         o = abi.SolveOpts(mach, Sref, bref, cref, (C.c_double * 3)(*cg), tol, max_iter, max_refine, inner_tol,
                           precond, 0)
         mu = np.zeros((k, self.nb)) if want_mu else None
@@ -342,14 +350,14 @@ class Geometry:
             pass
         else:
             self.ctx.check(st, "solve")
-        out = []
-        for q in range(k):
-            r = info[q]
-            out.append(dict(mu=None if mu is None else mu[q], cp=None if cp is None else cp[q], CL=r.CL, CDi=r.CDi,
-                            Fbody=np.array(r.Fbody[:]), Fwind=np.array(r.Fwind[:]), CM=np.array(r.CM[:]),
-                            rel_residual=r.rel_residual, iters=r.iters, refinements=r.refinements,
-                            converged=bool(r.converged)))
-        return out
+        # the k cp_solve_info records read through one structured view
+        rec = np.frombuffer(info, dtype=_SOLVE_INFO)
+        CL, CDi, rel = rec["CL"].tolist(), rec["CDi"].tolist(), rec["rel_residual"].tolist()
+        iters, refs, conv = rec["iters"].tolist(), rec["refinements"].tolist(), rec["converged"].tolist()
+        Fb, Fw, CM = rec["Fbody"].copy(), rec["Fwind"].copy(), rec["CM"].copy()
+        return [dict(mu=None if mu is None else mu[q], cp=None if cp is None else cp[q], CL=CL[q], CDi=CDi[q],
+                     Fbody=Fb[q], Fwind=Fw[q], CM=CM[q], rel_residual=rel[q], iters=iters[q], refinements=refs[q],
+                     converged=bool(conv[q])) for q in range(k)]
 
 
 class VpCase:
