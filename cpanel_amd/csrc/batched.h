@@ -166,6 +166,7 @@ private:
     const int* tkb_ = nullptr;
     bool f64_pipelined_ = true;  // pipelined FP64-accumulated product
     int f64_depth_ = 2;          // K steps per stage of the FP64-matrix product
+    int f64_ks_ = 0;             // pinned K split of the FP64-accumulated products (0: by fill)
     const int* bj_perm_ = nullptr;
     const int* bj_iperm_ = nullptr;  // body index -> position in block order
     float* pp_ = nullptr;            // multivector in block order

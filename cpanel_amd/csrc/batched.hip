@@ -1828,6 +1828,11 @@ BatchedSolver::BatchedSolver(int n, int64_t lda, int capacity, void* ws, void* h
         int d = std::atoi(e);
         if (d >= 2 && d <= 3) f64_depth_ = d;
     }
+    // CPGPU_F64_KS=1..32 pins the K split of the FP64-accumulated products
+    if (const char* e = std::getenv("CPGPU_F64_KS")) {
+        const int k = std::atoi(e);
+        if (k >= 1 && k <= 32) f64_ks_ = k;
+    }
     // CPGPU_F64_SIMPLE=1 selects the unpipelined FP64-accumulated product (tests)
     if (const char* e = std::getenv("CPGPU_F64_SIMPLE")) f64_pipelined_ = e[0] != '1';
     // CPGPU_NO_BF16X3=1 keeps the inner product in FP32 matrix-core arithmetic
@@ -2114,6 +2119,7 @@ void BatchedSolver::matvec64acc(const float* A32, const double* X, double* Y, co
     for (int ks = 1; ks <= 32; ks++) best = std::max(best, fill(ks));
     int KS = 1;
     while (KS < 32 && fill(KS) < best - 0.03) KS++;
+    if (f64_ks_ > 0) KS = f64_ks_;  // CPGPU_F64_KS
     int kchunk = (int)round_up((n_ + KS - 1) / KS, 16);
     dim3 grid((unsigned)(rb * KS));
     {
