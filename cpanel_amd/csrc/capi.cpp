@@ -84,6 +84,51 @@ void set_err(cp_ctx* c, const std::string& msg) {
     c->err = msg;
 }
 
+void ctx_ref(cp_ctx* c) { c->refs.fetch_add(1, std::memory_order_relaxed); }
+
+// the last reference frees the context: its streams, pinned blocks and pooled
+// device blocks (every child has returned its buffers by then)
+void ctx_unref(cp_ctx* c) {
+    if (c->refs.fetch_sub(1, std::memory_order_acq_rel) != 1) return;
+    if (c->device >= 0) {
+        (void)hipSetDevice(c->device);
+        (void)hipStreamSynchronize(c->stream);
+        (void)hipStreamDestroy(c->stream);
+        for (hipStream_t st : c->streams) (void)hipStreamDestroy(st);
+        for (void* p : c->pinned) (void)hipHostFree(p);
+        for (auto& e : c->pool) (void)hipFree(e.second);
+    }
+    delete c;
+}
+
+void geom_ref(cp_geom* g) { g->refs.fetch_add(1, std::memory_order_relaxed); }
+
+// the last reference frees the geometry; its buffers, pinned block and
+// stream go back to the context, which it then releases
+void geom_unref(cp_geom* g) {
+    if (g->refs.fetch_sub(1, std::memory_order_acq_rel) != 1) return;
+    cp_ctx* c = g->ctx;
+    hipStream_t st = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g->mu);
+        if (c->device >= 0) {
+            (void)hipSetDevice(c->device);
+            if (g->stream) (void)hipStreamSynchronize(g->stream);
+        }
+        if (g->solver_bytes) account(c, -g->solver_bytes);
+        st = g->stream;
+    }
+    void* host = g->solver_host;
+    g->batched.reset();
+    delete g;
+    {
+        std::lock_guard<std::mutex> lk(c->alloc_mu);
+        if (host) c->pinned.push_back(host);
+        if (st) c->streams.push_back(st);
+    }
+    ctx_unref(c);
+}
+
 
 // cpCase::windToBody (cpCase.cpp:142-166): body-axis freestream and the
 // wind<-body transform T (row major)
@@ -117,6 +162,7 @@ cp_status build_geom(cp_ctx* c, const cp_mesh* m, uint64_t key, cp_geom** out, c
     d.nw = nw;
     d.lda = round_up(n, 512);
     if (c->device < 0) {  // host-only context: topology tables only
+        ctx_ref(c);
         *out = g.release();
         return CP_OK;
     }
@@ -177,6 +223,7 @@ cp_status build_geom(cp_ctx* c, const cp_mesh* m, uint64_t key, cp_geom** out, c
     // longest host step of a new geometry)
     cp_geom* gp = g.get();
     gp->post_job = std::async(std::launch::async, [gp] { build_post_tables(gp->host, gp->post_host); });
+    ctx_ref(c);
     *out = g.release();
     return CP_OK;
 }
@@ -361,18 +408,7 @@ cp_status cp_create(int device, uint64_t budget, cp_ctx** out) {
 }
 
 void cp_destroy(cp_ctx* c) {
-    if (!c) return;
-    if (c->device < 0) {
-        delete c;
-        return;
-    }
-    (void)hipSetDevice(c->device);
-    (void)hipStreamSynchronize(c->stream);
-    (void)hipStreamDestroy(c->stream);
-    for (hipStream_t st : c->streams) (void)hipStreamDestroy(st);
-    for (void* p : c->pinned) (void)hipHostFree(p);
-    for (auto& e : c->pool) (void)hipFree(e.second);
-    delete c;
+    if (c) ctx_unref(c);
 }
 
 const char* cp_last_error(const cp_ctx* c) {
@@ -404,24 +440,7 @@ cp_status cp_geometry_from_tri(cp_ctx* c, const char* path, uint64_t key, cp_geo
 }
 
 void cp_geometry_release(cp_geom* g) {
-    if (!g) return;
-    cp_ctx* c = g->ctx;
-    hipStream_t st = nullptr;
-    {
-        std::lock_guard<std::mutex> lk(g->mu);
-        if (c->device >= 0) {
-            (void)hipSetDevice(c->device);
-            if (g->stream) (void)hipStreamSynchronize(g->stream);
-        }
-        if (g->solver_bytes) account(c, -g->solver_bytes);
-        st = g->stream;
-    }
-    void* host = g->solver_host;
-    g->batched.reset();
-    delete g;
-    std::lock_guard<std::mutex> lk(c->alloc_mu);
-    if (host) c->pinned.push_back(host);
-    if (st) c->streams.push_back(st);
+    if (g) geom_unref(g);
 }
 
 cp_status cp_geometry_info(const cp_geom* g, cp_geom_info* info) {
@@ -708,9 +727,9 @@ cp_status cp_solve(cp_geom* g, cp_prec prec, int32_t k, const cp_freestream* fs,
                 info[q].rel_residual = stats[q].rel_residual;
                 info[q].iters = stats[q].iters;
                 info[q].refinements = stats[q].refinements;
-                info[q].converged = stats[q].rel_residual <= 1e-10;
+                info[q].converged = stats[q].rel_residual <= tol;
             }
-            all_conv = all_conv && stats[q].rel_residual <= 1e-10;
+            all_conv = all_conv && stats[q].rel_residual <= tol;
         }
         return all_conv ? CP_OK : CP_NOT_CONVERGED;
     });
@@ -948,7 +967,7 @@ cp_status cp_timing_enable(cp_ctx* c, int on) {
 
 cp_status cp_timing_only(cp_ctx* c, const char* kernel) {
     if (!c) return CP_BAD_ARG;
-    c->timer.only = kernel ? kernel : "";
+    c->timer.set_only(kernel);
     return CP_OK;
 }
 
@@ -997,7 +1016,10 @@ cp_status cp_bh_build(cp_ctx* c, int32_t n, const double* xyz, const double* alp
         t->dev.order = t->order.as<int>();
         return CP_OK;
     });
-    if (st == CP_OK) *out = t.release();
+    if (st == CP_OK) {
+        ctx_ref(c);
+        *out = t.release();
+    }
     return st;
 }
 
@@ -1046,9 +1068,12 @@ cp_status cp_bh_eval(cp_tree* t, cp_bh_mode mode, int32_t m, const double* pts, 
 void cp_bh_release(cp_tree* t) {
     if (!t) return;
     cp_ctx* c = t->ctx;
-    std::lock_guard<std::mutex> lk(c->mu);
-    if (c->device >= 0) (void)hipSetDevice(c->device);
-    delete t;
+    {
+        std::lock_guard<std::mutex> lk(c->mu);
+        if (c->device >= 0) (void)hipSetDevice(c->device);
+        delete t;
+    }
+    ctx_unref(c);
 }
 
 cp_status cp_panel_to_points(cp_geom* g, const double* sigma, const double* mu, const double* mu_wake, int32_t m,
@@ -1204,9 +1229,9 @@ cp_status cp_solve_rhs(cp_geom* g, cp_prec prec, int32_t k, const double* sigma,
                 info[q].rel_residual = stats[q].rel_residual;
                 info[q].iters = stats[q].iters;
                 info[q].refinements = stats[q].refinements;
-                info[q].converged = stats[q].rel_residual <= 1e-10;
+                info[q].converged = stats[q].rel_residual <= tol;
             }
-            all_conv = all_conv && stats[q].rel_residual <= 1e-10;
+            all_conv = all_conv && stats[q].rel_residual <= tol;
         }
         return all_conv ? CP_OK : CP_NOT_CONVERGED;
     });

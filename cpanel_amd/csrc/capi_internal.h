@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <future>
@@ -33,6 +34,10 @@
 using namespace cpgpu;
 
 struct cp_ctx {
+    // one reference for the caller's handle and one per live geometry, tree
+    // and server: cp_destroy drops the handle's, and the context is freed
+    // when the last child is released, whatever order a caller tears down in
+    std::atomic<int> refs{1};
     int device = 0;
     uint64_t budget = 0;
     uint64_t in_use = 0;          // device bytes held (live + pooled), under alloc_mu
@@ -106,6 +111,9 @@ using cpgpu::capi::DevBuf;
 
 struct cp_geom {
     cp_ctx* ctx = nullptr;
+    // the caller's handle plus one per vortex-particle case built on it;
+    // the geometry is freed when the last goes
+    std::atomic<int> refs{1};
     uint64_t key = 0;
     hipStream_t stream = nullptr;  // every device call on this geometry
     std::mutex mu;                 // serialises calls on this geometry
@@ -160,6 +168,11 @@ namespace cpgpu {
 namespace capi {
 
 cp_status fail(cp_ctx* c, cp_status s, const std::string& msg);
+// context and geometry lifetimes (see cp_ctx::refs, cp_geom::refs)
+void ctx_ref(cp_ctx* c);
+void ctx_unref(cp_ctx* c);
+void geom_ref(cp_geom* g);
+void geom_unref(cp_geom* g);
 void set_err(cp_ctx* c, const std::string& msg);
 
 // runs f with HIP and budget errors turned into cp_status; `lock` (if any)

@@ -37,6 +37,14 @@
 
 using namespace cpgpu;
 
+namespace cpgpu {
+namespace capi {
+// context lifetime (capi.cpp): the server holds one reference
+void ctx_ref(cp_ctx* c);
+void ctx_unref(cp_ctx* c);
+}  // namespace capi
+}  // namespace cpgpu
+
 namespace {
 
 double now_s() {
@@ -260,6 +268,7 @@ cp_status cp_server_create(cp_ctx* ctx, int32_t slots, uint64_t hbm_budget, cp_s
     *out = nullptr;
     cp_server* S = new cp_server();
     S->ctx = ctx;
+    capi::ctx_ref(ctx);  // released by cp_server_destroy
     S->slots = slots;
     S->budget = hbm_budget;
     for (int i = 0; i < slots; i++) S->workers.emplace_back([S] { S->worker(); });
@@ -334,7 +343,9 @@ void cp_server_destroy(cp_server* S) {
     for (auto& t : S->workers) t.join();
     for (auto& kv : S->cache)
         if (kv.second.g) cp_geometry_release(kv.second.g);
+    cp_ctx* c = S->ctx;
     delete S;
+    capi::ctx_unref(c);
 }
 
 }  // extern "C"

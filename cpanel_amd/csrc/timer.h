@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <map>
 #include <mutex>
 #include <string>
@@ -12,11 +13,19 @@ namespace cpgpu {
 
 class KernelTimer {
 public:
-    bool enabled = false;
+    std::atomic<bool> enabled{false};
     // when not empty, only intervals under this name are recorded (the others
-    // cost their event records for nothing when one kernel is being measured)
-    std::string only;
-    bool wants(const char* name) const { return enabled && (only.empty() || only == name); }
+    // cost their event records for nothing when one kernel is being measured);
+    // read and set under the timer's lock (any thread may time a launch)
+    void set_only(const char* name) {
+        std::lock_guard<std::mutex> lk(mu_);
+        only_ = name ? name : "";
+    }
+    bool wants(const char* name) {
+        if (!enabled.load(std::memory_order_relaxed)) return false;
+        std::lock_guard<std::mutex> lk(mu_);
+        return only_.empty() || only_ == name;
+    }
     ~KernelTimer();
     // an event from the pool (thread safe)
     hipEvent_t take();
@@ -37,6 +46,7 @@ private:
     };
     void collect_locked();
     std::mutex mu_;
+    std::string only_;
     std::vector<Pending> pending_;
     std::vector<hipEvent_t> pool_;
     std::map<std::string, std::pair<double, long long>> totals_;

@@ -790,6 +790,7 @@ cp_status cp_vp_create(cp_geom* g, const cp_vp_opts* o, cp_vp** out) {
         return CP_OK;
     });
     if (st != CP_OK) return st;
+    geom_ref(g);  // the case keeps its geometry (and through it the context) alive
     *out = v.release();
     return CP_OK;
 }
@@ -923,8 +924,10 @@ cp_status cp_vp_state(const cp_vp* v, double* mu, double* sigma, double* mu_row1
 
 void cp_vp_release(cp_vp* v) {
     if (!v) return;
-    if (v->g && v->g->stream) (void)hipStreamSynchronize(v->g->stream);
+    cp_geom* g = v->g;
+    if (g->stream) (void)hipStreamSynchronize(g->stream);
     delete v;
+    geom_unref(g);
 }
 
 }  // extern "C"

@@ -37,7 +37,8 @@ extern "C" {
 
 typedef enum {
     CP_OK = 0,
-    CP_NOT_CONVERGED = 1,   /* solve ran but ||r||/||b|| > 1e-10 (cpCase.cpp:199-202) */
+    CP_NOT_CONVERGED = 1,   /* solve ran but ||r||/||b|| > the requested tol (1e-10 by
+                               default, the reference's flag, cpCase.cpp:199-202) */
     CP_OUT_OF_BUDGET = 2,   /* request exceeds the context's HBM budget */
     CP_BAD_MESH = 3,        /* topology the reference cannot handle */
     CP_HIP_ERROR = 4,       /* a HIP runtime call failed; see cp_last_hip_error */
@@ -135,7 +136,7 @@ typedef struct {
     double rel_residual;    /* final ||b - A mu|| / ||b|| in FP64 */
     int32_t iters;          /* Krylov iterations */
     int32_t refinements;    /* FP64 refinement sweeps */
-    int32_t converged;      /* rel_residual <= 1e-10 */
+    int32_t converged;      /* rel_residual <= the requested tol (default 1e-10) */
     int32_t pad;
 } cp_solve_info;
 

@@ -632,3 +632,34 @@ def test_timing_only_records_the_named_kernel(ctx):
         ctx.timing_only(None)
         ctx.timing_enable(False)
         g.release()
+
+
+def test_handles_close_in_any_order():
+    """A geometry, a tree and a vortex-particle case keep their context (and
+    the case its geometry) alive: closing the parents first must leave the
+    children usable and their release clean."""
+    m = meshgen.small_wing("4412", n_chord=20, n_span=6)
+    c = Context(0)
+    g = c.geometry(m)
+    g.assemble(CP_FP64)
+    t = c.bh_tree(np.random.default_rng(0).normal(size=(64, 3)), np.ones((64, 3)), np.full(64, 0.1))
+    c.close()
+    say("[gpu] solve after the context handle was closed")
+    r = g.solve([(1.0, 5.0, 0.0)], CP_FP64, tol=1e-12, **P4412)[0]
+    assert np.isfinite(r["CL"]) and r["converged"]
+    assert np.all(np.isfinite(t.eval(1)))
+    g.release()
+    t.close()
+
+
+def test_converged_flag_follows_requested_tol(c1s):
+    """cp_solve reports convergence against the caller's tol (the reference's
+    1e-10 flag is the default), on both paths."""
+    og, pg, A, B = c1s
+    pg.assemble(CP_FP64)
+    pg.assemble(CP_FP32_REFINE)
+    for prec in (CP_FP64, CP_FP32_REFINE):
+        r = pg.solve([(1.0, 5.0, 0.0)], prec, tol=1e-6, **P4412)[0]
+        assert r["converged"] and r["rel_residual"] <= 1e-6
+        r = pg.solve([(1.0, 5.0, 0.0)], prec, tol=1e-12, **P4412)[0]
+        assert r["converged"] and r["rel_residual"] <= 1e-12

@@ -8,15 +8,16 @@ and source strengths, both buffer rows' doublet strengths, particle positions,
 strengths and shed steps, then the surface force and moment coefficients of
 steadyTest.CPout.
 
-Skipped: the one GPU run of this case printed every step in agreement with
-the reference arrays (mu within 3e-6 of max |mu|, positions within 5.2e-6,
-strengths within 1.9e-6, surface CL 1.11853, CD 0.08989, Cm 0.11546 as in the
-.CPout) but the process then aborted in glibc's heap check
-("malloc_consolidate(): unaligned fastbin chunk detected").  A CPU build of
-the same host code under AddressSanitizer (device memory as host memory, each
-kernel replaced by a host loop writing the kernel's index range) ran all 40
-steps clean, so the cause is not identified and the case stays off the GPU
-until it is.
+The GPU run that first matched every stored step then aborted in glibc's heap
+check at interpreter exit ("malloc_consolidate(): unaligned fastbin chunk
+detected").  Cause, shown on the CPU by replaying the same host code and
+kernels under AddressSanitizer: with the handles left to the interpreter, the
+garbage collector finalized the Context before the Geometry (objects reachable
+only from a reference cycle are finalized in arbitrary order), so
+cp_geometry_release read the freed context and pushed its device blocks into
+the freed context's pool.  cp_destroy and cp_geometry_release now hold
+references (a context lives while a geometry, tree or server does; a geometry
+while a vortex-particle case does), so any teardown order is safe.
 """
 import os
 
@@ -25,8 +26,7 @@ import pytest
 
 from cpanel_amd import CP_FP64, meshgen
 
-pytestmark = [pytest.mark.gpu,
-              pytest.mark.skip(reason="aborted in a heap check after finishing on the GPU; cause not identified")]
+pytestmark = pytest.mark.gpu
 
 STORED = [1, 2, 3, 4, 5, 6, 8, 10, 20, 40]
 
