@@ -1386,6 +1386,27 @@ void or_assemble(const or_geom* g, double* A, double* B, int64_t lda, int nthrea
     }
 }
 
+// The body-body loop of or_assemble (geometry.cpp:823-841) over columns
+// j0..j1-1 only: a bounded sample of the assembly for timing.
+void or_assemble_cols(const or_geom* g, double* A, double* B, int64_t lda, int j0, int j1, int nthreads) {
+    int nb = (int)g->bPanels.size();
+    int nth = nthreads_or_default(nthreads);
+    std::vector<V3> ctr(nb);
+    for (int i = 0; i < nb; i++) ctr[i] = g->P[g->bPanels[i]].g.center;
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nth)
+    for (int j = j0; j < j1; j++) {
+        const PanelGeo& pg = g->P[g->bPanels[j]].g;
+        double* Aj = A + (int64_t)(j - j0) * lda;
+        double* Bj = B + (int64_t)(j - j0) * lda;
+        for (int i = 0; i < nb; i++) {
+            double s = 0, d = 0;
+            panel_phi_inf(pg, ctr[i], s, d);
+            Bj[i] = s;
+            Aj[i] = d;
+        }
+    }
+}
+
 // Eigen BiCGSTAB (Eigen/src/IterativeLinearSolvers/BiCGSTAB.h), restated:
 // diagonal preconditioner, x0 = 0, restart when rho becomes too small.
 int or_bicgstab(int n, const double* A, int64_t lda, const double* b, double* x, double tol, int maxit,

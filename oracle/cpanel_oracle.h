@@ -65,6 +65,10 @@ void or_geom_clusters(const or_geom* g, int32_t* out16, int32_t* ts_order);
 /* A, B column-major nb x nb with leading dimension lda. */
 void or_assemble(const or_geom* g, double* A, double* B, int64_t lda,
                  int nthreads);
+/* Columns j0..j1-1 of the body-body loop of or_assemble (no wake folding):
+ * a bounded sample of the assembly for single-thread timing. */
+void or_assemble_cols(const or_geom* g, double* A, double* B, int64_t lda, int j0, int j1,
+                      int nthreads);
 
 /* ---- solve -------------------------------------------------------------- */
 /* Eigen::BiCGSTAB<MatrixXd> with its default Jacobi preconditioner and x0=0

@@ -30,6 +30,32 @@ class CaseResult(C.Structure):
 _lib = None
 
 
+def use_library(path):
+    """Loads the oracle from `path` (a build of the same sources with other
+    flags, e.g. build_native's) in place of liboracle.so."""
+    global _lib, LIB_PATH
+    LIB_PATH = path
+    _lib = None
+    return lib()
+
+
+def build_native(out_dir):
+    """Builds the oracle with -march=native for this host's CPU into out_dir
+    (the CPU baseline is timed with the host's own instruction set; the
+    in-tree liboracle.so is built for x86-64-v3 so that it loads anywhere).
+    Returns the library path or None when no compiler is available."""
+    import shutil
+    import subprocess
+    gxx = shutil.which("g++")
+    if not gxx:
+        return None
+    srcs = [os.path.join(_HERE, f) for f in sorted(os.listdir(_HERE)) if f.endswith(".cpp")]
+    out = os.path.join(out_dir, "liboracle_native.so")
+    r = subprocess.run([gxx, "-O3", "-march=native", "-ffp-contract=off", "-fopenmp", "-fPIC", "-shared",
+                        "-std=c++17", "-o", out] + srcs, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
+    return out if r.returncode == 0 else None
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -48,6 +74,7 @@ def lib():
         L.or_geom_wake_info.argtypes = [C.c_void_p, _i, _i, _d]
         L.or_geom_clusters.argtypes = [C.c_void_p, _i, _i]
         L.or_assemble.argtypes = [C.c_void_p, _d, _d, C.c_int64, C.c_int]
+        L.or_assemble_cols.argtypes = [C.c_void_p, _d, _d, C.c_int64, C.c_int, C.c_int, C.c_int]
         L.or_bicgstab.argtypes = [C.c_int, _d, C.c_int64, _d, _d, C.c_double, C.c_int, C.c_int, _d]
         L.or_bicgstab.restype = C.c_int
         L.or_case_run.argtypes = [C.c_void_p, _d, _d, C.c_int64, C.POINTER(CaseParams), C.c_double,
@@ -164,6 +191,14 @@ class Geom:
         A = np.zeros((n, n), order="F")
         B = np.zeros((n, n), order="F")
         lib().or_assemble(self.h, _p(A), _p(B), n, nthreads)
+        return A, B
+
+    def assemble_cols(self, j0, j1, nthreads=0):
+        """Columns j0..j1-1 of the body-body block (no wake folding)."""
+        n = self.nb
+        A = np.zeros((n, j1 - j0), order="F")
+        B = np.zeros((n, j1 - j0), order="F")
+        lib().or_assemble_cols(self.h, _p(A), _p(B), n, j0, j1, nthreads)
         return A, B
 
     def run_case(self, A, B, V=1.0, alpha=0.0, beta=0.0, mach=0.1, Sref=1.0, bref=1.0, cref=1.0,
