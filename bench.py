@@ -292,9 +292,11 @@ def main():
     g.assemble(CP_FP32_REFINE)
     t_a64, _ = ctx.timing("assemble_f64")
     t_a32, _ = ctx.timing("assemble_f32")
+    asm_parts = {nm: ctx.timing(nm)[0] for nm in ("assemble_far_f64", "assemble_near_f64", "assemble_far_f32",
+                                                  "assemble_near_f32", "near_f64")}
     pairs = float(n) * n
-    log("assembly: fp64 %.2f ms (%.1f Ginfl/s), fp32 %.2f ms (%.1f Ginfl/s)" %
-        (t_a64, pairs / t_a64 / 1e6, t_a32, pairs / t_a32 / 1e6))
+    log("assembly: fp64 %.2f ms (%.1f Ginfl/s), fp32 %.2f ms (%.1f Ginfl/s); passes (ms) %s" %
+        (t_a64, pairs / t_a64 / 1e6, t_a32, pairs / t_a32 / 1e6, json.dumps(asm_parts)))
     tip = g.clusters()[2].astype(bool)
 
     # plain FP64 GPU path, every condition solved alone (reference
@@ -496,6 +498,7 @@ def main():
             "validity": vinfo,
             "assembly_ginfl_per_s_fp64": pairs / t_a64 / 1e6, "assembly_ms_fp64": t_a64,
             "assembly_ginfl_per_s_fp32": pairs / t_a32 / 1e6, "assembly_ms_fp32": t_a32,
+            "assembly_passes_ms": asm_parts,
             "plain_fp64_solves_per_s": (k / t_fp64) if t_fp64 else None,
             "plain_fp64_tol": args.cmp_tol,
             "solves_per_s_with_host_copy_of_mu_cp": valid / t_step_host,

@@ -191,6 +191,35 @@ cp_status build_geom(cp_ctx* c, const cp_mesh* m, uint64_t key, cp_geom** out, c
     }
     upload(c, g->cxyz, cxyz);
     upload(c, g->normals, nrm);
+    // bounding sphere of each 64-row tile of collocation points (box centre,
+    // largest distance to it, widened by a relative 1e-12): the near-field
+    // pass of the assembly skips the tiles it proves far
+    d.ntiles = (n + 63) / 64;
+    std::vector<double> ts(4 * (size_t)d.ntiles);
+    for (int t = 0; t < d.ntiles; t++) {
+        const int i0 = 64 * t, i1 = std::min(n, i0 + 64);
+        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+        for (int i = i0; i < i1; i++) {
+            const Vec3& p = G.body[i].center;
+            const double v[3] = {p.x, p.y, p.z};
+            for (int a = 0; a < 3; a++) {
+                lo[a] = std::min(lo[a], v[a]);
+                hi[a] = std::max(hi[a], v[a]);
+            }
+        }
+        double ctr[3], r2 = 0;
+        for (int a = 0; a < 3; a++) ctr[a] = 0.5 * (lo[a] + hi[a]);
+        for (int i = i0; i < i1; i++) {
+            const Vec3& p = G.body[i].center;
+            const double dx = p.x - ctr[0], dy = p.y - ctr[1], dz = p.z - ctr[2];
+            r2 = std::max(r2, dx * dx + dy * dy + dz * dz);
+        }
+        ts[4 * t] = ctr[0];
+        ts[4 * t + 1] = ctr[1];
+        ts[4 * t + 2] = ctr[2];
+        ts[4 * t + 3] = std::sqrt(r2) * (1 + 1e-12) + 1e-300;
+    }
+    upload(c, g->tsph, ts);
     // fold lists per body column in (wake panel, slot) order
     std::vector<std::vector<std::pair<int, double>>> cols(n);
     for (int j = 0; j < nw; j++) {
@@ -219,6 +248,7 @@ cp_status build_geom(cp_ctx* c, const cp_mesh* m, uint64_t key, cp_geom** out, c
     d.fold_ptr = g->fold_ptr.as<int>();
     d.fold_w = g->fold_w.as<int>();
     d.fold_c = g->fold_c.as<double>();
+    d.tsph = g->tsph.as<double>();
     // the post tables depend on the host geometry only (the CHTLS fits are the
     // longest host step of a new geometry)
     cp_geom* gp = g.get();

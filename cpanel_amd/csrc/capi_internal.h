@@ -123,6 +123,7 @@ struct cp_geom {
     Geometry host;
     DevGeom dev;
     DevBuf body, wake, cxyz, fold_ptr, fold_w, fold_c, normals;
+    DevBuf tsph;       // bounding spheres of the 64-row collocation tiles
     DevBuf W;          // wake influence N x Nw (FP64)
     DevBuf A64, B64, A32, B32;
     DevBuf A16;        // bf16 hi and lo tile planes of A32 (bf16x3 / bf16x2 inner products)

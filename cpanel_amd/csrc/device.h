@@ -41,6 +41,10 @@ struct DevGeom {
     int* fold_ptr = nullptr;
     int* fold_w = nullptr;
     double* fold_c = nullptr;
+    // bounding spheres of the 64-row tiles of collocation points (centre,
+    // radius: 4 per tile), for the near-field pass of the assembly
+    double* tsph = nullptr;
+    int ntiles = 0;
 };
 
 // ---- kernel launchers (assemble.hip) ----

@@ -30,89 +30,122 @@ __device__ double block_sum_post(double v, double* sh) {
     return r;
 }
 
+// sum over the 64 lanes of a wavefront with a fixed butterfly (every lane
+// gets the total)
+__device__ inline double wave_sum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+constexpr int kPP = 64;   // panels per workgroup (one per lane)
+constexpr int kPC = 16;   // conditions per workgroup (each wavefront every 4th)
+
+// Per (panel, condition): potential, CHTLS velocity, Cp and the force and
+// moment contributions.  A workgroup owns 64 consecutive panels and 16
+// conditions: the panels' weights and neighbour lists (contiguous, 45 + 16
+// words per panel) are staged in LDS once and serve every condition, so each
+// weight is read from memory once per 16 conditions instead of once per
+// condition.  Partial sums per (tile, condition) come from a wavefront sum.
 __global__ __launch_bounds__(256) void post_panel_kernel(
-    int n, const int32_t* __restrict__ nbr, const double* __restrict__ w1, const double* __restrict__ w2,
+    int n, int k, const int32_t* __restrict__ nbr, const double* __restrict__ w1, const double* __restrict__ w2,
     const int32_t* __restrict__ mode, const double* __restrict__ ctr, const double* __restrict__ bez,
     const double* __restrict__ area, PostParams pp, const double* __restrict__ mu, int64_t ldmu,
     const double* __restrict__ vinf4, double* __restrict__ cp, int64_t ldcp, double* __restrict__ partials,
     double* __restrict__ vel, int64_t ldvel, double* __restrict__ phi, int64_t ldphi,
     const double* __restrict__ vpot) {
-    __shared__ double sh[kT];
-    const int c = blockIdx.y;
-    const int i = blockIdx.x * kT + threadIdx.x;
-    const double Vx = vinf4[4 * c], Vy = vinf4[4 * c + 1], Vz = vinf4[4 * c + 2], Vm = vinf4[4 * c + 3];
-    const double* m = mu + (int64_t)c * ldmu;
+    __shared__ double sw[kPP * 45];
+    __shared__ int32_t snb[kPP * 16];
+    const int i0 = blockIdx.x * kPP;
+    const int np = min(kPP, n - i0);
+    for (int e = threadIdx.x; e < np * 45; e += 256) sw[e] = w1[45 * (int64_t)i0 + e];
+    for (int e = threadIdx.x; e < np * 16; e += 256) snb[e] = nbr[16 * (int64_t)i0 + e];
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const int i = i0 + lane;
+    const bool live = lane < np;
     const double* cx = ctr;
     const double* cy = ctr + n;
     const double* cz = ctr + 2 * (int64_t)n;
-    // the velocity a panel's potential is formed with: the freestream, or a
-    // per-panel one (the vortex-particle case adds the particle and filament
-    // velocity at the centre, cpCaseVP::VinfPlusVecPot, cpCaseVP.cpp:601-623)
-    const double* vp = vpot ? vpot + (int64_t)c * 3 * n : nullptr;
-    auto pot = [&](int j) {
-        if (vp) return vp[3 * j] * cx[j] + vp[3 * j + 1] * cy[j] + vp[3 * j + 2] * cz[j] - m[j];
-        return Vx * cx[j] + Vy * cy[j] + Vz * cz[j] - m[j];
-    };
-    double f[kNP] = {0, 0, 0, 0, 0, 0};
-    if (i < n) {
-        double phi_i = pot(i);
-        const int32_t* nb = nbr + 16 * (int64_t)i;
-        int cnt = nb[0];
-        double df[kClusterMax];
-        for (int k = 0; k < cnt; k++) {
-            int j = nb[1 + k];
-            df[k] = pot(j) - phi_i;
-        }
-        double v0 = 0, v1 = 0, v2 = 0;
-        const double* w = w1 + 45 * (int64_t)i;
-        for (int k = 0; k < cnt; k++) {
-            v0 += w[3 * k] * df[k];
-            v1 += w[3 * k + 1] * df[k];
-            v2 += w[3 * k + 2] * df[k];
-        }
-        bool bad = cnt == 0 || (v0 != v0) || (v1 != v1) || (v2 != v2);
-        if (bad && mode[i] == 1) {
-            const double* ww = w2 + 45 * (int64_t)i;
-            v0 = v1 = v2 = 0;
-            for (int k = 0; k < cnt; k++) {
-                v0 += ww[3 * k] * df[k];
-                v1 += ww[3 * k + 1] * df[k];
-                v2 += ww[3 * k + 2] * df[k];
-            }
-            bad = cnt == 0 || (v0 != v0) || (v1 != v1) || (v2 != v2);
-        }
-        const double bx = bez[i], by = bez[n + i], bz = bez[2 * (int64_t)n + i];
-        if (bad) {
-            // (bez x Vinf) x bez   (bodyPanel.cpp:516-520)
-            double q0 = by * Vz - bz * Vy, q1 = bz * Vx - bx * Vz, q2 = bx * Vy - by * Vx;
-            v0 = q1 * bz - q2 * by;
-            v1 = q2 * bx - q0 * bz;
-            v2 = q0 * by - q1 * bx;
-        }
-        v0 /= pp.PG;
-        if (vel) {
-            double* vo = vel + (int64_t)c * ldvel + 3 * (int64_t)i;
-            vo[0] = v0;
-            vo[1] = v1;
-            vo[2] = v2;
-        }
-        if (phi) phi[(int64_t)c * ldphi + i] = phi_i;
-        double vn = sqrt(v0 * v0 + v1 * v1 + v2 * v2) / Vm;
-        double Cp = 1 - vn * vn;
-        if (cp) cp[(int64_t)c * ldcp + i] = Cp;
-        double a = area[i];
-        double F0 = -Cp * a * bx, F1 = -Cp * a * by, F2 = -Cp * a * bz;
-        f[0] = F0 / pp.Sref;
-        f[1] = F1 / pp.Sref;
-        f[2] = F2 / pp.Sref;
-        double r0 = cx[i] - pp.cg[0], r1 = cy[i] - pp.cg[1], r2 = cz[i] - pp.cg[2];
-        f[3] = (r1 * F2 - r2 * F1) / (pp.Sref * pp.bref);
-        f[4] = (r2 * F0 - r0 * F2) / (pp.Sref * pp.cref);
-        f[5] = (r0 * F1 - r1 * F0) / (pp.Sref * pp.bref);
+    double bx = 0, by = 0, bz = 0, a = 0, r0 = 0, r1 = 0, r2 = 0;
+    int cnt = 0, md = 0;
+    if (live) {
+        bx = bez[i];
+        by = bez[n + i];
+        bz = bez[2 * (int64_t)n + i];
+        a = area[i];
+        r0 = cx[i] - pp.cg[0];
+        r1 = cy[i] - pp.cg[1];
+        r2 = cz[i] - pp.cg[2];
+        cnt = snb[16 * lane];
+        md = mode[i];
     }
-    for (int q = 0; q < kNP; q++) {
-        double s = block_sum_post(f[q], sh);
-        if (threadIdx.x == 0) partials[((int64_t)c * gridDim.x + blockIdx.x) * kNP + q] = s;
+    const double* w = sw + 45 * lane;
+    const int32_t* nb = snb + 16 * lane + 1;
+    const int c1 = min(k, (int)(blockIdx.y + 1) * kPC);
+    for (int c = blockIdx.y * kPC + (threadIdx.x >> 6); c < c1; c += 4) {
+        const double Vx = vinf4[4 * c], Vy = vinf4[4 * c + 1], Vz = vinf4[4 * c + 2], Vm = vinf4[4 * c + 3];
+        const double* m = mu + (int64_t)c * ldmu;
+        // the velocity a panel's potential is formed with: the freestream,
+        // or a per-panel one (the vortex-particle case adds the particle and
+        // filament velocity at the centre, cpCaseVP::VinfPlusVecPot,
+        // cpCaseVP.cpp:601-623)
+        const double* vp = vpot ? vpot + (int64_t)c * 3 * n : nullptr;
+        auto pot = [&](int j) {
+            if (vp) return vp[3 * j] * cx[j] + vp[3 * j + 1] * cy[j] + vp[3 * j + 2] * cz[j] - m[j];
+            return Vx * cx[j] + Vy * cy[j] + Vz * cz[j] - m[j];
+        };
+        double f[kNP] = {0, 0, 0, 0, 0, 0};
+        if (live) {
+            const double phi_i = pot(i);
+            double df[kClusterMax];
+            for (int q = 0; q < cnt; q++) df[q] = pot(nb[q]) - phi_i;
+            double v0 = 0, v1 = 0, v2 = 0;
+            for (int q = 0; q < cnt; q++) {
+                v0 += w[3 * q] * df[q];
+                v1 += w[3 * q + 1] * df[q];
+                v2 += w[3 * q + 2] * df[q];
+            }
+            bool bad = cnt == 0 || (v0 != v0) || (v1 != v1) || (v2 != v2);
+            if (bad && md == 1) {
+                const double* ww = w2 + 45 * (int64_t)i;
+                v0 = v1 = v2 = 0;
+                for (int q = 0; q < cnt; q++) {
+                    v0 += ww[3 * q] * df[q];
+                    v1 += ww[3 * q + 1] * df[q];
+                    v2 += ww[3 * q + 2] * df[q];
+                }
+                bad = cnt == 0 || (v0 != v0) || (v1 != v1) || (v2 != v2);
+            }
+            if (bad) {
+                // (bez x Vinf) x bez   (bodyPanel.cpp:516-520)
+                double q0 = by * Vz - bz * Vy, q1 = bz * Vx - bx * Vz, q2 = bx * Vy - by * Vx;
+                v0 = q1 * bz - q2 * by;
+                v1 = q2 * bx - q0 * bz;
+                v2 = q0 * by - q1 * bx;
+            }
+            v0 /= pp.PG;
+            if (vel) {
+                double* vo = vel + (int64_t)c * ldvel + 3 * (int64_t)i;
+                vo[0] = v0;
+                vo[1] = v1;
+                vo[2] = v2;
+            }
+            if (phi) phi[(int64_t)c * ldphi + i] = phi_i;
+            double vn = sqrt(v0 * v0 + v1 * v1 + v2 * v2) / Vm;
+            double Cp = 1 - vn * vn;
+            if (cp) cp[(int64_t)c * ldcp + i] = Cp;
+            double F0 = -Cp * a * bx, F1 = -Cp * a * by, F2 = -Cp * a * bz;
+            f[0] = F0 / pp.Sref;
+            f[1] = F1 / pp.Sref;
+            f[2] = F2 / pp.Sref;
+            f[3] = (r1 * F2 - r2 * F1) / (pp.Sref * pp.bref);
+            f[4] = (r2 * F0 - r0 * F2) / (pp.Sref * pp.cref);
+            f[5] = (r0 * F1 - r1 * F0) / (pp.Sref * pp.bref);
+        }
+        for (int q = 0; q < kNP; q++) {
+            const double t = wave_sum(f[q]);
+            if (lane == 0) partials[((int64_t)c * gridDim.x + blockIdx.x) * kNP + q] = t;
+        }
     }
 }
 
@@ -203,16 +236,16 @@ void launch_post_wake(int nw, const int32_t* fidx, const double* ft, const doubl
     CP_LAUNCH_CHECK();
 }
 
-int post_partial_blocks(int n) { return (n + kT - 1) / kT; }
+int post_partial_blocks(int n) { return (n + kPP - 1) / kPP; }
 
 void launch_post(const PostDev& P, const PostParams& pp, int k, const double* mu, int64_t ldmu,
                  const double* vinf4, double* cp, int64_t ldcp, double* partials, int nblk, double* res,
                  hipStream_t s, const PostFields* f) {
     const PostFields none;
     if (!f) f = &none;
-    dim3 g1((unsigned)nblk, (unsigned)k);
-    hipLaunchKernelGGL(post_panel_kernel, g1, dim3(kT), 0, s, P.n, P.nbr, P.w1, P.w2, P.mode, P.ctr, P.bez, P.area,
-                       pp, mu, ldmu, vinf4, cp, ldcp, partials, f->vel, f->ldvel, f->phi, f->ldphi, f->vpot);
+    dim3 g1((unsigned)nblk, (unsigned)((k + kPC - 1) / kPC));
+    hipLaunchKernelGGL(post_panel_kernel, g1, dim3(kT), 0, s, P.n, k, P.nbr, P.w1, P.w2, P.mode, P.ctr, P.bez,
+                       P.area, pp, mu, ldmu, vinf4, cp, ldcp, partials, f->vel, f->ldvel, f->phi, f->ldphi, f->vpot);
     CP_LAUNCH_CHECK();
     hipLaunchKernelGGL(post_final_kernel, dim3((unsigned)k), dim3(kT), 0, s, nblk, partials, P.n_wakes, P.n_st,
                        P.n_te, P.te_idx, P.rphi, P.rv, P.rw, P.simpson, pp, mu, ldmu, vinf4, res, f->span);

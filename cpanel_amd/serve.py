@@ -170,7 +170,19 @@ def summarize(results, wall):
     lat = np.array([r["stats"]["latency_s"] for r in results])
     ok = [r for r in results if r["status"] == abi.CP_OK]
     conds = sum(len(r["info"]) for r in ok)
-    return dict(requests=len(results), ok=len(ok), conditions=conds, wall_s=wall, solves_per_s=conds / wall,
+    failed = {}
+    failed_detail = []
+    for r in results:
+        if r["status"] != abi.CP_OK:
+            name = abi.lib().cp_status_string(r["status"]).decode()
+            failed[name] = failed.get(name, 0) + 1
+            if len(failed_detail) < 8:
+                failed_detail.append(dict(status=name, n_body=r["stats"]["n_body"], conditions=len(r["info"]),
+                                          max_rel_residual=max((i["rel_residual"] for i in r["info"]), default=0),
+                                          refinements=max((i["refinements"] for i in r["info"]), default=0)))
+    return dict(requests=len(results), ok=len(ok), failed=failed, failed_detail=failed_detail, conditions=conds,
+                wall_s=wall,
+                solves_per_s=conds / wall,
                 p50_ms=1e3 * float(np.percentile(lat, 50)), p99_ms=1e3 * float(np.percentile(lat, 99)),
                 mean_queue_ms=1e3 * float(np.mean([r["stats"]["queue_s"] for r in results])),
                 cache_hit_frac=float(np.mean([r["stats"]["cache_hit"] for r in results])))

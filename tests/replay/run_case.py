@@ -1,0 +1,114 @@
+"""Scenarios run against the CPU replay of libcpgpu in a child process with
+the sanitizer runtime preloaded (tests/test_replay.py).
+
+    python tests/replay/run_case.py LIB assembly
+    python tests/replay/run_case.py LIB vp STEPS
+    python tests/replay/run_case.py LIB teardown
+
+Each prints "ok ..." lines; any sanitizer report aborts the process.
+"""
+import os
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+from cpanel_amd import abi  # noqa: E402
+
+abi.LIB_PATH = sys.argv[1]
+
+from cpanel_amd import CP_FP32_REFINE, CP_FP64, meshgen  # noqa: E402
+from cpanel_amd.case import Context  # noqa: E402
+
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def rel(a, b):
+    scale = float(np.max(np.abs(b)))
+    return float(np.max(np.abs(a - b))) / (scale if scale > 0 else 1.0)
+
+
+def assembly():
+    """The far and near passes of the assembly (lane-permute compaction
+    included) against the FP64 oracle on a small wing."""
+    from oracle import oracle as O
+    m = meshgen.wing("4412", 1.0, 6.0, 40, 12, "uniform", 20.0)
+    og = O.Geom.from_arrays(m["xyz"], m["tri"], m["ids"])
+    A, B = og.assemble()
+    ctx = Context(0)
+    g = ctx.geometry(m)
+    g.assemble(CP_FP64)
+    Ag, Bg = g.matrices(CP_FP64)
+    ea, eb = np.abs(Ag - A).max() / np.abs(A).max(), np.abs(Bg - B).max() / np.abs(B).max()
+    assert ea <= 1e-12 and eb <= 1e-12, (ea, eb)
+    g.assemble(CP_FP32_REFINE)
+    A32, B32 = g.matrices(CP_FP32_REFINE)
+    ea32, eb32 = np.abs(A32 - A).max() / np.abs(A).max(), np.abs(B32 - B).max() / np.abs(B).max()
+    assert ea32 <= 4e-7 and eb32 <= 4e-7, (ea32, eb32)
+    print("ok assembly n=%d fp64 %.1e %.1e fp32 %.1e %.1e" % (g.nb, ea, eb, ea32, eb32), flush=True)
+    g.release()
+    ctx.close()
+
+
+def vp(steps):
+    """The vortex-particle driver on the reference's steadyTest case against
+    its stored steps; the handles are left to the interpreter, as a script
+    that never closes them would (this module's functions hold its globals,
+    so at exit the handles are finalized together, in no fixed order)."""
+    global ctx, geom, case
+    f = np.load(os.path.join(GOLDEN, "steadyTest_vp.npz"))
+    ctx = Context(0)
+    geom = ctx.geometry_vp(meshgen.read_tri(os.path.join(GOLDEN, "steadyTest.tri")), V=1.0, dt=0.0)
+    case = geom.vp_case((1.0, 10.0, 0.0), prec=CP_FP64, mach=0.1, Sref=6, bref=6, cref=1, cg=(0.416, 0, 0.03),
+                        accelerate=True, tol=1e-12)
+    for k in range(1, steps + 1):
+        case.step()
+        if ("mu_%d" % k) in f:
+            st = case.state()
+            d = rel(st["mu"], f["mu_%d" % k])
+            assert d < 2e-5, (k, d)
+            if k >= 3:
+                p = case.particles()
+                assert np.max(np.abs(p["pos"] - f["ppos_%d" % k])) < 2e-5
+            print("ok vp step %d dmu %.2e" % (k, d), flush=True)
+    fin = case.finish()
+    assert np.isfinite(fin["CL"])
+    print("ok vp finish", flush=True)
+
+
+def teardown():
+    """Parents closed before their children: the context before a geometry,
+    a tree and a vortex-particle case, the geometry before its case."""
+    m = meshgen.small_wing("4412", n_chord=12, n_span=4)
+    c = Context(0)
+    g = c.geometry(m)
+    g.assemble(CP_FP64)
+    t = c.bh_tree(np.random.default_rng(0).normal(size=(40, 3)), np.ones((40, 3)), np.full(40, 0.1))
+    c.close()
+    r = g.solve([(1.0, 5.0, 0.0)], CP_FP64, tol=1e-12, mach=0.1, Sref=4, bref=4, cref=1)[0]
+    assert np.isfinite(r["CL"])
+    assert np.all(np.isfinite(t.eval(abi.CP_BH_VEL)))
+    g.release()
+    t.close()
+    c2 = Context(0)
+    gv = c2.geometry_vp(meshgen.read_tri(os.path.join(GOLDEN, "steadyTest.tri")), V=1.0, dt=0.0)
+    case = gv.vp_case((1.0, 10.0, 0.0), prec=CP_FP64, mach=0.1, Sref=6, bref=6, cref=1, cg=(0.416, 0, 0.03),
+                      accelerate=True, tol=1e-10)
+    gv.release()
+    c2.close()
+    case.step()
+    case.close()
+    print("ok teardown", flush=True)
+
+
+if __name__ == "__main__":
+    what = sys.argv[2]
+    if what == "assembly":
+        assembly()
+    elif what == "vp":
+        vp(int(sys.argv[3]))
+    elif what == "teardown":
+        teardown()

@@ -1,0 +1,37 @@
+// The batched mixed-precision path (batched.hip, blockinv.hip: matrix-core
+// builtins) is not replayed: its solver entry points throw, which the C ABI
+// reports as CP_HIP_ERROR.  launch_sigma (used by the FP64 solve) is restated
+// on the host; the bf16 packing is skipped.
+#include "capi_internal.h"
+namespace cpgpu {
+[[noreturn]] static void nope(const char* w) { throw HipError{1, std::string("replay: not emulated: ") + w}; }
+BatchedSolver::BatchedSolver(int, int64_t, int, void*, void*, hipStream_t) { nope("BatchedSolver"); }
+size_t BatchedSolver::workspace_bytes(int64_t) { return 256; }
+size_t BatchedSolver::host_bytes() { return 256; }
+BatchedSolver::~BatchedSolver() {}
+void BatchedSolver::solve(const float*, const float*, const double*, const double*, int, const BatchedOpts&, double*,
+                          int64_t, SolveStats*, hipStream_t) { nope("solve"); }
+void BatchedSolver::solve_rhs(const float*, const float*, const double*, const double*, const double*, int,
+                              const BatchedOpts&, double*, int64_t, SolveStats*, hipStream_t) { nope("solve_rhs"); }
+void BatchedSolver::matvec32(const float*, const float*, float*, hipStream_t) { nope("matvec32"); }
+void BatchedSolver::matvec64acc(const float*, const double*, double*, const double*, double, hipStream_t,
+                                const double*, const double*) { nope("matvec64acc"); }
+void launch_sigma(int n, int64_t lda, const double* nrm, const double* v4, double* sig, hipStream_t) {
+    for (int64_t i = 0; i < lda; i++)
+        sig[i] = i < n ? -(v4[0] * nrm[i] + v4[1] * nrm[lda + i] + v4[2] * nrm[2 * lda + i]) : 0.0;
+}
+int packed_kblocks(int n) { return (n + 15) / 16; }
+size_t packed_a_bytes(int64_t, int) { return 256; }
+int64_t bj_packed_floats(int b) { return (int64_t)b * b; }
+void launch_bj_pack(const float*, const int*, const int64_t*, const int64_t*, int, int, float*, int, hipStream_t) { nope("bj_pack"); }
+// the bf16 tile planes and their near-field tile lists feed only the batched
+// products, which are not replayed
+void launch_pack_a(const float*, int64_t, int, void*, hipStream_t) {}
+void near_tile_lists(int, int64_t, const std::vector<int>&, const std::vector<int>&, std::vector<int>& tptr,
+                     std::vector<int>& tkb) {
+    tptr.clear();
+    tkb.clear();
+}
+void launch_bj_invert(const float*, int64_t, const int*, const int*, const int64_t*, int, int, double*, int*, int*, int*,
+                      float*, hipStream_t) { nope("bj_invert"); }
+}  // namespace cpgpu
