@@ -79,14 +79,19 @@ __global__ __launch_bounds__(256) void far_f64_kernel(
         const PanelRec& P = pan[j];
         double d0 = px - P.c[0], d1 = py - P.c[1], d2 = pz - P.c[2];
         double r2 = sq_norm3(d0, d1, d2);
-        double r = sqrt(r2);
+        // one reciprocal for both point formulas (area / (4 pi r) and
+        // PN area / (4 pi r^3), bodyPanel.cpp:392-395, panel.cpp:737-740):
+        // a few ulp from the reference's two divisions
+        double inv = 1.0 / sqrt(r2);
         double PN = d0 * P.n[0] + d1 * P.n[1] + d2 * P.n[2];
-        double src = P.area / (4 * kPI * r);
-        double dub = PN * P.area / (4 * kPI * (r * r * r));
+        double src = P.area_4pi * inv;
+        double dub = PN * P.area_4pi * (inv * inv * inv);
         dub = fold_in(dub, i, j, fold_ptr, fold_w, fold_c, W, ldw);
         const int64_t o = i + (int64_t)j * lda;
-        A[o] = live ? dub : 0.0;
-        B[o] = live ? src : 0.0;
+        // the matrices are written once here and read by later kernels:
+        // streaming stores keep them out of the caches
+        __builtin_nontemporal_store(live ? dub : 0.0, &A[o]);
+        __builtin_nontemporal_store(live ? src : 0.0, &B[o]);
     }
 }
 
@@ -118,8 +123,8 @@ __global__ __launch_bounds__(256) void far_f32_kernel(
             dubf = (float)dub64;
         }
         const int64_t o = i + (int64_t)j * lda;
-        A[o] = live ? dubf : 0.0f;
-        B[o] = live ? srcf : 0.0f;
+        __builtin_nontemporal_store(live ? dubf : 0.0f, &A[o]);
+        __builtin_nontemporal_store(live ? srcf : 0.0f, &B[o]);
     }
 }
 
@@ -203,7 +208,7 @@ __global__ __launch_bounds__(256) void near_f64_kernel(
     auto eval = [&](int i, bool ok) {
         if (!ok) return;
         double src, dub;
-        near_phi_f64(P, cx[i], cy[i], cz[i], src, dub);
+        near_phi_tri_f64(P, cx[i], cy[i], cz[i], src, dub);
         dub = fold_in(dub, i, j, fold_ptr, fold_w, fold_c, W, ldw);
         const int64_t o = i + (int64_t)j * lda;
         A[o] = dub;
@@ -231,7 +236,7 @@ __global__ __launch_bounds__(256) void near_f32_kernel(
     auto eval = [&](int i, bool ok) {
         if (!ok) return;
         double src, dub;
-        near_phi_f64(P, cx[i], cy[i], cz[i], src, dub);
+        near_phi_tri_f64(P, cx[i], cy[i], cz[i], src, dub);
         dub = fold_in(dub, i, j, fold_ptr, fold_w, fold_c, W, ldw);
         const int64_t o = i + (int64_t)j * lda;
         A[o] = (float)dub;

@@ -84,6 +84,8 @@ public:
     // leading dimension and process)
     void tune(const float* A32, hipStream_t s);
     const std::vector<TuneRecord>& tune_log() const { return tune_log_; }
+    // 0, 1, 2: see bj_level_
+    int block_jacobi_level() const { return bj_level_; }
     const GemmShape& shape() const { return shape_; }
     // Y = A32 * P for one 64-column tile (exposed for tests / tuning)
     void matvec32(const float* A32, const float* P, float* Y, hipStream_t s);
@@ -106,7 +108,7 @@ public:
     void set_block_jacobi(const int* perm, const int* ptr, const int64_t* off, const float* inv, int nblocks,
                           int max_b, const int* iperm = nullptr, const float* pk = nullptr,
                           const int64_t* poff = nullptr, int pk_kind = kBjPackBf16) {
-        bj_ = perm != nullptr;
+        bj_ = perm != nullptr && bj_level_ < 2;
         bj_pk_ = pk;
         bj_poff_ = poff;
         bj_kind_ = pk_kind;
@@ -178,6 +180,12 @@ private:
     int cus_ = 256;  // compute units of the device
     int bj_kind_ = kBjPackBf16;  // layout of bj_pk_
     bool bj_lo_ = false;         // bf16 product: read the lo plane of the inverses too
+    // preconditioner strength after stalled refinement sweeps (sticky for
+    // this solver): 0 as configured, 1 bf16 product with the lo plane,
+    // 2 point Jacobi
+    int bj_level_ = 0;
+    int bj_escalations_ = 0;
+    void escalate_block_jacobi();
     int bj_rt_ = 2;              // bf16 product: 32-row tiles per workgroup
     int bj_u_ = 1;  // K chunks per load batch of the packed block-Jacobi product
     int bj_nblocks_ = 0, bj_maxb_ = 0;

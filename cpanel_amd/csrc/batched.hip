@@ -1998,6 +1998,15 @@ void BatchedSolver::launch_gemm32(const GemmShape& sh, const float* A32, const f
     CP_LAUNCH_CHECK();
 }
 
+void BatchedSolver::escalate_block_jacobi() {
+    if (bj_level_ == 0 && bj_kind_ == kBjPackBf16 && !bj_lo_)
+        bj_level_ = 1;  // the lo plane of the inverses: FP32-accurate operands
+    else
+        bj_level_ = 2;  // point Jacobi (the BiCGSTAB inner solve)
+    if (bj_level_ == 2) bj_ = false;
+    ++bj_escalations_;
+}
+
 void BatchedSolver::tune(const float* A32, hipStream_t s) {
     if (tuned_) return;
     tuned_ = true;
@@ -2152,11 +2161,11 @@ void BatchedSolver::bj_apply_pk(const float* PP, float* Y, const double* cscale,
     };
     if (bj_kind_ == kBjPackBf16 && bj_rt_ == 2) {
         const dim3 grid2((unsigned)((bj_maxb_ + 63) / 64), (unsigned)bj_nblocks_);
-        auto kern = bj_lo_ ? bj_apply_bf_kernel<true, 2> : bj_apply_bf_kernel<false, 2>;
+        auto kern = (bj_lo_ || bj_level_ >= 1) ? bj_apply_bf_kernel<true, 2> : bj_apply_bf_kernel<false, 2>;
         hipLaunchKernelGGL(kern, grid2, dim3(kT), 0, s, bj_perm_, bj_ptr_, bj_poff_, bj_pk_, PP, Y, cscale, cstride,
                            gate_);
     } else if (bj_kind_ == kBjPackBf16)
-        go(bj_lo_ ? bj_apply_bf_kernel<true, 1> : bj_apply_bf_kernel<false, 1>);
+        go((bj_lo_ || bj_level_ >= 1) ? bj_apply_bf_kernel<true, 1> : bj_apply_bf_kernel<false, 1>);
     else if (bj_u_ == 1)
         go(bj_apply_pk_kernel<1>);
     else if (bj_u_ == 3)
@@ -2325,7 +2334,7 @@ void BatchedSolver::tile_solve(const float* A32, const float* B32, const double*
     }
     const double tol2 = o.tol * o.tol;
     int sweep = 0;
-    std::vector<double> rel(KP, 0.0);
+    std::vector<double> rel(KP, 0.0), prev(KP, 0.0);
     while (true) {
         // residual R = b - A X in FP64 (R = b while X is still zero)
         if (sweep == 0 && !x0_cm)
@@ -2356,6 +2365,18 @@ void BatchedSolver::tile_solve(const float* A32, const float* B32, const double*
                 host_sc_[c * Q_NQ + Q_SCALE] = 0;
             }
         }
+        // a sweep whose inner solve did not halve the residual of every column
+        // it ran on: the preconditioner is too weak for this geometry (bf16
+        // rounding of an ill-conditioned block's inverse can leave the
+        // preconditioned operator with eigenvalues near zero, and GMRES then
+        // stagnates); strengthen it for the remaining sweeps and later solves
+        if (sweep > 0 && bj_ && any) {
+            bool stalled = false;
+            for (int c = 0; c < kc; c++)
+                if (host_sweep[c] && !(rel[c] <= 0.5 * prev[c])) stalled = true;
+            if (stalled) escalate_block_jacobi();
+        }
+        for (int c = 0; c < kc; c++) prev[c] = rel[c];
         if (!any || sweep >= o.max_refine) break;
         ++sweep;
         // inner FP32 solve of A32 d = R / |R| on the sweeping columns

@@ -14,9 +14,16 @@ using namespace cpgpu::capi;
 namespace cpgpu {
 namespace capi {
 
-// frees pooled blocks (largest first) until `need` more bytes fit the budget
+// the bytes the context may hold: the caller's budget within the device cap
+static uint64_t limit(const cp_ctx* c) {
+    if (c->budget && c->cap) return std::min(c->budget, c->cap);
+    return c->budget ? c->budget : c->cap;
+}
+
+// frees pooled blocks (largest first) until `need` more bytes fit the limit
 void trim_pool(cp_ctx* c, uint64_t need) {
-    while (!c->pool.empty() && c->budget && c->in_use + need > c->budget) {
+    const uint64_t lim = limit(c);
+    while (!c->pool.empty() && lim && c->in_use + need > lim) {
         auto it = std::prev(c->pool.end());
         (void)hipFree(it->second);
         c->in_use -= it->first;
@@ -433,6 +440,11 @@ cp_status cp_create(int device, uint64_t budget, cp_ctx** out) {
         delete c;
         return CP_HIP_ERROR;
     }
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && total_b > 0) {
+        const uint64_t reserve = std::max<uint64_t>((uint64_t)8 << 30, total_b / 20);
+        c->cap = free_b > 2 * reserve ? free_b - reserve : free_b / 2;
+    }
     *out = c;
     return CP_OK;
 }
@@ -847,6 +859,7 @@ cp_status cp_solver_report(cp_geom* g, char* buf, int32_t len) {
         }
         r += "], ";
     }
+    if (g->batched) r += "\"block_jacobi_level\": " + std::to_string(g->batched->block_jacobi_level()) + ", ";
     r += "\"block_jacobi_blocks\": " + std::to_string(g->bj_plan.perm.empty() ? 0 : g->bj_plan.nblocks()) +
          ", \"block_jacobi_max\": " + std::to_string(g->bj_plan.max_b) + ", \"near_nnz\": " +
          std::to_string(g->near_nnz) + "}";

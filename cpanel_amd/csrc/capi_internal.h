@@ -40,6 +40,13 @@ struct cp_ctx {
     std::atomic<int> refs{1};
     int device = 0;
     uint64_t budget = 0;
+    // device bytes this context lets itself hold (live + pooled): the free
+    // memory at creation less a reserve for the runtime's own allocations
+    // (kernel scratch, queues) and other users of the device.  Pooled blocks
+    // are freed to stay under it even without a caller budget: a pool that
+    // grew to the physical limit left the runtime unable to allocate a
+    // queue's scratch, which aborts the queue.
+    uint64_t cap = 0;
     uint64_t in_use = 0;          // device bytes held (live + pooled), under alloc_mu
     hipStream_t stream = nullptr; // context-level calls (Barnes-Hut trees)
     std::string err;              // last error, under err_mu

@@ -74,14 +74,17 @@ CP_HD double vortex_phi(double PN, double Al, const double* a, const double* s, 
 
 // Near-field (edge formula) source and doublet potential of a panel at a
 // point; the caller has already excluded the far field.  itself: the point
-// is the panel centre (|pjk| < 1e-10), doublet -> -0.5.
-CP_HD void near_phi_f64(const PanelRec& P, double px, double py, double pz, double& src, double& dub) {
+// is the panel centre (|pjk| < 1e-10), doublet -> -0.5.  NV: the vertex
+// count when it is known at compile time (3: a body triangle, so the
+// per-vertex arrays are plain registers), 0 to read it from the record.
+template <int NV>
+CP_HD void near_phi_nv(const PanelRec& P, double px, double py, double pz, double& src, double& dub) {
 #pragma clang fp contract(off)
     double pjk0 = px - P.c[0], pjk1 = py - P.c[1], pjk2 = pz - P.c[2];
     double r = sqrt(sq_norm3(pjk0, pjk1, pjk2));
     bool itself = r < 1e-10;
     double PN = pjk0 * P.n[0] + pjk1 * P.n[1] + pjk2 * P.n[2];
-    const int nv = P.nv;
+    const int nv = NV > 0 ? NV : P.nv;
     double d[4][3], nd[4];
     for (int k = 0; k < nv; k++) {
         d[k][0] = px - P.v[3 * k];
@@ -110,6 +113,16 @@ CP_HD void near_phi_f64(const PanelRec& P, double px, double py, double pz, doub
     }
     src = s_src / (4 * kPI);
     dub = itself ? -0.5 : s_dub / (4 * kPI);
+}
+
+CP_HD void near_phi_f64(const PanelRec& P, double px, double py, double pz, double& src, double& dub) {
+    near_phi_nv<0>(P, px, py, pz, src, dub);
+}
+
+// a body triangle (every body panel: the .tri reader's panels are
+// triangles), bit-identical to near_phi_f64 on it
+CP_HD void near_phi_tri_f64(const PanelRec& P, double px, double py, double pz, double& src, double& dub) {
+    near_phi_nv<3>(P, px, py, pz, src, dub);
 }
 
 // bodyPanel::panelPhiInf in FP64 (far field via the point formulas).

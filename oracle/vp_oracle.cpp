@@ -52,6 +52,27 @@ V3 vel_alg(const Part& p, V3 poi) {
            cross(dist, p.str);
 }
 
+// The configured unsteady case's particle kernel: velInflAlgSmooth's
+// algebraic smoothing (particle.cpp:31-38) with one core radius `core` for
+// every particle in place of 1.5 x the particle's radius.
+V3 vel_alg_core(V3 spos, V3 sstr, V3 poi, double core) {
+    V3 dist = poi - spos;
+    double r2 = dot(dist, dist), s2 = core * core;
+    return (-1 / (4 * PI) * (r2 + 2.5 * s2) / std::pow(r2 + s2, 2.5)) * cross(dist, sstr);
+}
+
+// (alpha_t . grad) of vel_alg_core at the target: with x = poi - spos,
+// u = f(r) x x alpha_s, f = -(r^2 + 2.5 s^2) / (4 pi (r^2 + s^2)^2.5),
+// d u / d x_k = f'(r) x_k / r (x x alpha_s) + f e_k x alpha_s, and
+// f'(r) / r = (3 r^2 + 10.5 s^2) / (4 pi (r^2 + s^2)^3.5)
+V3 stretch_alg_core(V3 spos, V3 sstr, V3 tpos, V3 tstr, double core) {
+    V3 x = tpos - spos;
+    double r2 = dot(x, x), s2 = core * core;
+    double f = -1 / (4 * PI) * (r2 + 2.5 * s2) / std::pow(r2 + s2, 2.5);
+    double h = 1 / (4 * PI) * (3 * r2 + 10.5 * s2) / std::pow(r2 + s2, 3.5);
+    return (h * dot(tstr, x)) * cross(x, sstr) + f * cross(tstr, sstr);
+}
+
 // particle::velInfl(particle*) (particle.cpp:42-58): source s on target t
 V3 vel_gauss(const Part& s, const Part& t) {
     double sigma = std::sqrt(std::pow(CORE_OVERLAP * s.radius, 2) + std::pow(CORE_OVERLAP * t.radius, 2)) /
@@ -398,6 +419,42 @@ void or_vp_direct(int mode, int n, const double* xyz, const double* alpha, const
         out[3 * t + 1] = v.y;
         out[3 * t + 2] = v.z;
     }
+}
+
+/* Direct sums of the configured case's kernel (vel_alg_core,
+ * stretch_alg_core; one core radius): mode 0 velocity at m points, 1
+ * velocity on the n particles, 2 stretching on the n particles (a particle
+ * excludes itself). */
+void or_vp_direct_core(int mode, double core, int n, const double* xyz, const double* alpha, int m, const double* pts,
+                       double* out) {
+    int nt = mode == 0 ? m : n;
+#pragma omp parallel for schedule(static)
+    for (int t = 0; t < nt; t++) {
+        V3 v = {0, 0, 0};
+        const double* tp = mode == 0 ? pts + 3 * t : xyz + 3 * t;
+        V3 tpos = {tp[0], tp[1], tp[2]};
+        V3 tstr = mode == 2 ? V3{alpha[3 * t], alpha[3 * t + 1], alpha[3 * t + 2]} : V3{0, 0, 0};
+        for (int j = 0; j < n; j++) {
+            if (mode != 0 && j == t) continue;
+            V3 sp = {xyz[3 * j], xyz[3 * j + 1], xyz[3 * j + 2]};
+            V3 sa = {alpha[3 * j], alpha[3 * j + 1], alpha[3 * j + 2]};
+            v = v + (mode == 2 ? stretch_alg_core(sp, sa, tpos, tstr, core) : vel_alg_core(sp, sa, tpos, core));
+        }
+        out[3 * t] = v.x;
+        out[3 * t + 1] = v.y;
+        out[3 * t + 2] = v.z;
+    }
+}
+
+/* One pair of the configured case's kernel: mode 0 velocity, 2 stretching. */
+void or_vp_pair_core(int mode, double core, const double* xs, const double* as, const double* xt, const double* at,
+                     double* out) {
+    V3 sp = {xs[0], xs[1], xs[2]}, sa = {as[0], as[1], as[2]}, tp = {xt[0], xt[1], xt[2]},
+       ta = {at[0], at[1], at[2]};
+    V3 v = mode == 2 ? stretch_alg_core(sp, sa, tp, ta, core) : vel_alg_core(sp, sa, tp, core);
+    out[0] = v.x;
+    out[1] = v.y;
+    out[2] = v.z;
 }
 
 /* One pair: kernel `mode` of source (xs, as, rs) at target (xt, at, rt);

@@ -24,6 +24,8 @@ def _lib():
         L.or_bh_eval.argtypes = [C.c_void_p, C.c_int, C.c_int, _d, _d]
         L.or_vp_direct.argtypes = [C.c_int, C.c_int, _d, _d, _d, C.c_int, _d, _d]
         L.or_vp_pair.argtypes = [C.c_int, _d, _d, C.c_double, _d, _d, C.c_double, _d]
+        L.or_vp_direct_core.argtypes = [C.c_int, C.c_double, C.c_int, _d, _d, C.c_int, _d, _d]
+        L.or_vp_pair_core.argtypes = [C.c_int, C.c_double, _d, _d, _d, _d, _d]
         L.or_panels_to_points.argtypes = [C.c_int, _i, _d, _d, _d, C.c_int, _i, _d, _d, C.c_int, _d, _d]
         L.or_velgrad.argtypes = [C.c_int, _d, C.c_int, C.c_int, C.c_double, C.c_double, _d, _d]
         L.or_panels_stretching.argtypes = [C.c_int, _i, _d, _d, _d, C.c_int, _i, _d, _d, C.c_int, _d, _d, _d]
@@ -78,6 +80,25 @@ def direct(mode, xyz, alpha, radius, pts=None):
         q = np.zeros((len(radius), 3))
     out = np.zeros_like(q)
     _lib().or_vp_direct(mode, len(radius), _p(xyz), _p(alpha), _p(radius), len(q), _p(q), _p(out))
+    return out
+
+
+CORE_VEL_POINTS, CORE_VEL, CORE_STRETCH = 0, 1, 2
+
+
+def direct_core(mode, core, xyz, alpha, pts=None):
+    """Direct sums of the configured case's kernel (algebraic, one core
+    radius): CORE_VEL_POINTS at pts, CORE_VEL / CORE_STRETCH on the particles."""
+    xyz, alpha = _a(xyz).reshape(-1, 3), _a(alpha).reshape(-1, 3)
+    q = _a(pts).reshape(-1, 3) if mode == CORE_VEL_POINTS else np.zeros((len(xyz), 3))
+    out = np.zeros_like(q)
+    _lib().or_vp_direct_core(mode, float(core), len(xyz), _p(xyz), _p(alpha), len(q), _p(q), _p(out))
+    return out
+
+
+def pair_core(mode, core, xs, as_, xt, at=(0.0, 0.0, 0.0)):
+    out = np.zeros(3)
+    _lib().or_vp_pair_core(mode, float(core), _p(_a(xs)), _p(_a(as_)), _p(_a(xt)), _p(_a(at)), _p(out))
     return out
 
 

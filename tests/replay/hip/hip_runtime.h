@@ -26,6 +26,8 @@
 #define __launch_bounds__(...)
 #define __shared__ static
 #define __restrict__ __restrict
+#define __align__(n) alignas(n)
+#define HIP_SYMBOL(x) (&(x))
 
 struct dim3 {
     unsigned x, y, z;
@@ -37,6 +39,8 @@ extern dim3 blockDim, gridDim;
 
 struct double2 { double x, y; };
 inline double2 make_double2(double a, double b) { return double2{a, b}; }
+template <class T>
+inline void __builtin_nontemporal_store(T v, T* p) { *p = v; }
 
 using std::min;
 using std::max;
@@ -106,6 +110,8 @@ hipError_t hipDeviceSynchronize();
 hipError_t hipSetDevice(int d);
 hipError_t hipGetDevice(int* d);
 hipError_t hipGetDeviceCount(int* n);
+hipError_t hipMemGetInfo(size_t* free_b, size_t* total_b);
+hipError_t hipGetSymbolAddress(void** p, const void* sym);
 hipError_t hipGetLastError();
 const char* hipGetErrorString(hipError_t e);
 hipError_t hipEventCreate(hipEvent_t* e);

@@ -183,6 +183,15 @@ hipError_t hipDeviceSynchronize() { return hipSuccess; }
 hipError_t hipSetDevice(int) { return hipSuccess; }
 hipError_t hipGetDevice(int* d) { *d = 0; return hipSuccess; }
 hipError_t hipGetDeviceCount(int* n) { *n = 1; return hipSuccess; }
+hipError_t hipGetSymbolAddress(void** p, const void* sym) {
+    *p = const_cast<void*>(sym);
+    return hipSuccess;
+}
+// a 16 GiB device
+hipError_t hipMemGetInfo(size_t* free_b, size_t* total_b) {
+    *free_b = *total_b = (size_t)16 << 30;
+    return hipSuccess;
+}
 hipError_t hipGetLastError() { int e = last_err; last_err = 0; return e; }
 const char* hipGetErrorString(hipError_t) { return "replay error"; }
 hipError_t hipEventCreate(hipEvent_t* e) { *e = new rp_event{0}; return hipSuccess; }

@@ -25,7 +25,10 @@ def say(*a):
 @pytest.fixture(scope="module")
 def ctx():
     say("[gpu-serve] creating context on device 0")
-    return Context(0)
+    c = Context(0)
+    yield c
+    # the geometries of this module's fixtures keep it alive until they go
+    c.close()
 
 
 def test_mix_equals_alone_and_fp64(ctx):
@@ -77,3 +80,34 @@ def test_budget_evicts_and_results_hold(ctx):
     assert all(r["status"] == abi.CP_OK for r in res)
     assert st["evictions"] > 0
     assert st["resident_bytes_est"] <= int(5e8)
+
+
+@pytest.mark.timeout(600)
+def test_block_jacobi_escalates_on_stalled_sweeps():
+    """A geometry of the serving stream (NACA 0009, AR 5, 126 x 78, 19,904
+    panels) on which GMRES with the bf16 hi plane of the strip block-Jacobi
+    inverses stagnates (the bf16 rounding of the ill-conditioned tip block's
+    inverse spoils the preconditioner): the refinement detects the stalled
+    sweep, reads the inverses' lo plane (then falls back to point Jacobi if
+    that stalls too), and every condition converges to the FP64 solution."""
+    from cpanel_amd import CP_FP32_REFINE, CP_FP64, serve
+    from cpanel_amd.case import Context
+    stream, geoms = serve.make_stream(1132, 20.0, seed=0, n_lo=500, n_hi=20000, reuse=0.3, max_cond=32)
+    r = stream[1131]
+    mesh, prm = geoms[r["key"]]
+    ctx = Context(0)
+    g = ctx.geometry(mesh)
+    g.assemble(CP_FP32_REFINE)
+    kw = dict(mach=0.1, Sref=prm["Sref"], bref=prm["bref"], cref=prm["cref"], cg=(0.25, 0.0, 0.0), tol=1e-10)
+    res = g.solve(r["conds"], CP_FP32_REFINE, want_cp=False, **kw)
+    rep = g.solver_report()
+    print("[gpu] n=%d refinements %s, block-Jacobi level %s" % (g.nb, sorted(set(x["refinements"] for x in res)),
+                                                               rep.get("block_jacobi_level")), flush=True)
+    assert all(x["converged"] and x["rel_residual"] <= 1e-10 for x in res)
+    assert rep.get("block_jacobi_level", 0) >= 1
+    g.assemble(CP_FP64)
+    r64 = g.solve(r["conds"][:3], CP_FP64, want_cp=False, **kw)
+    for a, b in zip(res, r64):
+        assert abs(a["CL"] - b["CL"]) <= 1e-6
+    g.release()
+    ctx.close()

@@ -32,7 +32,10 @@ def say(*a):
 @pytest.fixture(scope="module")
 def ctx():
     say("[gpu] creating context on device 0")
-    return Context(0)
+    c = Context(0)
+    yield c
+    # the geometries of this module's fixtures keep it alive until they go
+    c.close()
 
 
 def build_pair(ctx, mesh):

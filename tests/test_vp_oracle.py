@@ -270,3 +270,47 @@ def test_doublet_gradient_exclusions():
     Mv = V.velgrad(tri, near_vertex, V.VG_PANEL_DOUBLET, mu=4 * np.pi)
     e12 = V.velgrad(np.roll(tri, -1, axis=0), near_vertex, V.VG_EDGE)
     assert np.allclose(Mv, e12, rtol=1e-13, atol=0)
+
+
+def test_core_kernel_velocity_is_alg_smooth_with_one_core():
+    """The configured case's velocity kernel is the reference's
+    velInflAlgSmooth (particle.cpp:31-38) with sigma = core for every
+    particle: equal to the oracle's mode-0 kernel of a particle whose radius
+    is core / 1.5 (coreOverlap)."""
+    rng = np.random.default_rng(3)
+    core = 0.07
+    for _ in range(20):
+        xs, a, xt = rng.normal(size=3), rng.normal(size=3), rng.normal(size=3) * 0.2
+        v1 = V.pair_core(0, core, xs, a, xt)
+        v0 = V.pair(0, xs, a, core / 1.5, xt)
+        assert np.allclose(v1, v0, rtol=1e-13, atol=1e-16)
+
+
+def test_core_kernel_stretching_is_directional_derivative():
+    """(alpha_t . grad) u of the core kernel: central differences of the
+    velocity along alpha_t (h = 1e-6, error O(h^2))."""
+    rng = np.random.default_rng(4)
+    core = 0.05
+    for _ in range(30):
+        xs, a = rng.normal(size=3), rng.normal(size=3)
+        xt = xs + rng.normal(size=3) * rng.choice([0.02, 0.1, 1.0])
+        at = rng.normal(size=3)
+        h = 1e-6
+        fd = (V.pair_core(0, core, xs, a, xt + h * at) - V.pair_core(0, core, xs, a, xt - h * at)) / (2 * h)
+        st = V.pair_core(2, core, xs, a, xt, at)
+        assert np.allclose(st, fd, rtol=2e-6, atol=1e-9 * np.abs(fd).max() + 1e-12)
+
+
+def test_core_direct_sums_excludes_self_and_matches_pairs():
+    rng = np.random.default_rng(5)
+    X, A = rng.normal(size=(40, 3)), rng.normal(size=(40, 3))
+    core = 0.1
+    vel = V.direct_core(V.CORE_VEL, core, X, A)
+    st = V.direct_core(V.CORE_STRETCH, core, X, A)
+    for t in (0, 17, 39):
+        ev = sum(V.pair_core(0, core, X[j], A[j], X[t]) for j in range(40) if j != t)
+        es = sum(V.pair_core(2, core, X[j], A[j], X[t], A[t]) for j in range(40) if j != t)
+        assert np.allclose(vel[t], ev, rtol=1e-12) and np.allclose(st[t], es, rtol=1e-12)
+    pts = rng.normal(size=(5, 3))
+    vp = V.direct_core(V.CORE_VEL_POINTS, core, X, A, pts)
+    assert np.allclose(vp[2], sum(V.pair_core(0, core, X[j], A[j], pts[2]) for j in range(40)), rtol=1e-12)
