@@ -104,6 +104,7 @@ SIGNATURES = {
     "cp_solve": (C.c_int, [C.c_void_p, C.c_int, C.c_int32, C.POINTER(Freestream), C.POINTER(SolveOpts), _d, _d,
                            C.POINTER(SolveInfo)]),
     "cp_matvec": (C.c_int, [C.c_void_p, C.c_int, C.c_int32, _d, _d]),
+    "cp_matvec_f64_noalloc": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64]),
     "cp_solver_report": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int32]),
     "cp_geometry_upload_vp": (C.c_int, [C.c_void_p, C.POINTER(Mesh), C.c_double, C.c_double, C.c_uint64,
                                         C.POINTER(C.c_void_p)]),

@@ -156,6 +156,18 @@ cp_status cp_solve(cp_geom* g, cp_prec prec, int32_t k, const cp_freestream* fs,
  * X, Y: host, n_body x k column-major. */
 cp_status cp_matvec(cp_geom* g, int kernel, int32_t k, const double* X, double* Y);
 
+/* y = A x (FP64 A, the product of cp_matvec kernel 0) through caller-given
+ * DEVICE buffers, allocating nothing: no counterpart in the reference; for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  X: k columns of
+ * n_body entries (column stride ldx >= n_body); Y: k columns of lda entries
+ * (cp_geometry_info's lda, stride ldy >= lda; rows n_body..lda-1 are set to
+ * zero).  The split-K partial sums live in a static 20 MiB device array per
+ * GPU (CP_BAD_ARG if a geometry's split would exceed it, which needs a
+ * leading dimension above 524,288); calls that overlap on one GPU race on
+ * that array.  Results equal cp_matvec kernel 0 bit for bit.  Synchronous. */
+cp_status cp_matvec_f64_noalloc(cp_geom* g, int32_t k, const double* X, int64_t ldx, double* Y, int64_t ldy);
+
 /* ---- vortex particles: Barnes-Hut (particleFMM::build / barnesHutVel /
  *      barnesHutStretch / barnesHutDiff, CPanel/particleFMM.h:52-56) ------- */
 typedef struct cp_tree cp_tree;
