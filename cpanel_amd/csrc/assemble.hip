@@ -36,7 +36,6 @@ namespace cpgpu {
 namespace {
 
 constexpr int kRows = 256;   // rows per workgroup (4 wavefronts)
-constexpr int kCols = 32;    // columns per workgroup
 
 __global__ __launch_bounds__(256) void wake_influence_kernel(const PanelRec* __restrict__ wake, int nw,
                                                              const double* __restrict__ cx,
@@ -63,35 +62,55 @@ __device__ inline double fold_in(double dub, int i, int j, const int* __restrict
     return dub;
 }
 
-// far-field pass, FP64: every (i, j) of the workgroup's 256 rows x 32 columns
+// far-field passes: a workgroup owns 1024 rows (4 per lane, 256 apart) and
+// 16 columns, so each column receives 8 KiB of contiguous rows per matrix
+// from one workgroup (longer write runs than one 256-row tile) and each lane
+// has four independent pairs in flight; rows from n to lda are zeroed, rows
+// past lda do not exist
+constexpr int kFarR = 4;
+constexpr int kFarRows = kFarR * 256;
+constexpr int kFarCols = 16;
+
+// far-field pass, FP64
 __global__ __launch_bounds__(256) void far_f64_kernel(
     const PanelRec* __restrict__ pan, const double* __restrict__ cx, const double* __restrict__ cy,
     const double* __restrict__ cz, int n, int64_t lda, double* __restrict__ A, double* __restrict__ B,
     const int* __restrict__ fold_ptr, const int* __restrict__ fold_w, const double* __restrict__ fold_c,
     const double* __restrict__ W, int64_t ldw) {
-    const int i = blockIdx.x * kRows + threadIdx.x;
-    const int j0 = blockIdx.y * kCols;
-    const int j1 = min(n, j0 + kCols);
-    const bool live = i < n;
-    const double px = cx[i], py = cy[i], pz = cz[i];
+    const int i0 = blockIdx.x * kFarRows + threadIdx.x;
+    const int j0 = blockIdx.y * kFarCols;
+    const int j1 = min(n, j0 + kFarCols);
+    double px[kFarR], py[kFarR], pz[kFarR];
+    for (int r = 0; r < kFarR; r++) {
+        const int i = i0 + r * 256;
+        const bool in = i < lda;
+        px[r] = in ? cx[i] : 0.0;
+        py[r] = in ? cy[i] : 0.0;
+        pz[r] = in ? cz[i] : 0.0;
+    }
     for (int j = j0; j < j1; j++) {
 #pragma clang fp contract(off)
         const PanelRec& P = pan[j];
-        double d0 = px - P.c[0], d1 = py - P.c[1], d2 = pz - P.c[2];
-        double r2 = sq_norm3(d0, d1, d2);
-        // one reciprocal for both point formulas (area / (4 pi r) and
-        // PN area / (4 pi r^3), bodyPanel.cpp:392-395, panel.cpp:737-740):
-        // a few ulp from the reference's two divisions
-        double inv = 1.0 / sqrt(r2);
-        double PN = d0 * P.n[0] + d1 * P.n[1] + d2 * P.n[2];
-        double src = P.area_4pi * inv;
-        double dub = PN * P.area_4pi * (inv * inv * inv);
-        dub = fold_in(dub, i, j, fold_ptr, fold_w, fold_c, W, ldw);
-        const int64_t o = i + (int64_t)j * lda;
-        // the matrices are written once here and read by later kernels:
-        // streaming stores keep them out of the caches
-        __builtin_nontemporal_store(live ? dub : 0.0, &A[o]);
-        __builtin_nontemporal_store(live ? src : 0.0, &B[o]);
+        for (int r = 0; r < kFarR; r++) {
+            const int i = i0 + r * 256;
+            if (i >= lda) break;
+            double d0 = px[r] - P.c[0], d1 = py[r] - P.c[1], d2 = pz[r] - P.c[2];
+            double r2 = sq_norm3(d0, d1, d2);
+            // one reciprocal for both point formulas (area / (4 pi r) and
+            // PN area / (4 pi r^3), bodyPanel.cpp:392-395, panel.cpp:737-740):
+            // a few ulp from the reference's two divisions
+            double inv = 1.0 / sqrt(r2);
+            double PN = d0 * P.n[0] + d1 * P.n[1] + d2 * P.n[2];
+            double src = P.area_4pi * inv;
+            double dub = PN * P.area_4pi * (inv * inv * inv);
+            dub = fold_in(dub, i, j, fold_ptr, fold_w, fold_c, W, ldw);
+            const int64_t o = i + (int64_t)j * lda;
+            const bool live = i < n;
+            // the matrices are written once here and read by later kernels:
+            // streaming stores keep them out of the caches
+            __builtin_nontemporal_store(live ? dub : 0.0, &A[o]);
+            __builtin_nontemporal_store(live ? src : 0.0, &B[o]);
+        }
     }
 }
 
@@ -102,29 +121,40 @@ __global__ __launch_bounds__(256) void far_f32_kernel(
     const double* __restrict__ cz, int n, int64_t lda, float* __restrict__ A, float* __restrict__ B,
     const int* __restrict__ fold_ptr, const int* __restrict__ fold_w, const double* __restrict__ fold_c,
     const double* __restrict__ W, int64_t ldw) {
-    const int i = blockIdx.x * kRows + threadIdx.x;
-    const int j0 = blockIdx.y * kCols;
-    const int j1 = min(n, j0 + kCols);
-    const bool live = i < n;
-    const double px = cx[i], py = cy[i], pz = cz[i];
+    const int i0 = blockIdx.x * kFarRows + threadIdx.x;
+    const int j0 = blockIdx.y * kFarCols;
+    const int j1 = min(n, j0 + kFarCols);
+    double px[kFarR], py[kFarR], pz[kFarR];
+    for (int r = 0; r < kFarR; r++) {
+        const int i = i0 + r * 256;
+        const bool in = i < lda;
+        px[r] = in ? cx[i] : 0.0;
+        py[r] = in ? cy[i] : 0.0;
+        pz[r] = in ? cz[i] : 0.0;
+    }
     for (int j = j0; j < j1; j++) {
         const PanelRec& P = pan[j];
-        double d0 = px - P.c[0], d1 = py - P.c[1], d2 = pz - P.c[2];
-        float f0 = (float)d0, f1 = (float)d1, f2 = (float)d2;
-        float rinv = rsqrtf((float)sq_norm3(d0, d1, d2));
-        float a4 = (float)P.area_4pi;
-        float PN = f0 * (float)P.n[0] + f1 * (float)P.n[1] + f2 * (float)P.n[2];
-        float srcf = a4 * rinv;
-        float dubf = PN * a4 * (rinv * rinv * rinv);
         const int e0 = fold_ptr[j], e1 = fold_ptr[j + 1];
-        if (e0 != e1) {
-            double dub64 = dubf;
-            for (int e = e0; e < e1; e++) dub64 += W[i + (int64_t)fold_w[e] * ldw] * fold_c[e];
-            dubf = (float)dub64;
+        for (int r = 0; r < kFarR; r++) {
+            const int i = i0 + r * 256;
+            if (i >= lda) break;
+            double d0 = px[r] - P.c[0], d1 = py[r] - P.c[1], d2 = pz[r] - P.c[2];
+            float f0 = (float)d0, f1 = (float)d1, f2 = (float)d2;
+            float rinv = rsqrtf((float)sq_norm3(d0, d1, d2));
+            float a4 = (float)P.area_4pi;
+            float PN = f0 * (float)P.n[0] + f1 * (float)P.n[1] + f2 * (float)P.n[2];
+            float srcf = a4 * rinv;
+            float dubf = PN * a4 * (rinv * rinv * rinv);
+            if (e0 != e1) {
+                double dub64 = dubf;
+                for (int e = e0; e < e1; e++) dub64 += W[i + (int64_t)fold_w[e] * ldw] * fold_c[e];
+                dubf = (float)dub64;
+            }
+            const int64_t o = i + (int64_t)j * lda;
+            const bool live = i < n;
+            __builtin_nontemporal_store(live ? dubf : 0.0f, &A[o]);
+            __builtin_nontemporal_store(live ? srcf : 0.0f, &B[o]);
         }
-        const int64_t o = i + (int64_t)j * lda;
-        __builtin_nontemporal_store(live ? dubf : 0.0f, &A[o]);
-        __builtin_nontemporal_store(live ? srcf : 0.0f, &B[o]);
     }
 }
 
@@ -293,7 +323,7 @@ void launch_wake_influence(const DevGeom& g, double* W, int64_t ldw, hipStream_t
 }
 
 void launch_assemble_f64(const DevGeom& g, const double* W, int64_t ldw, double* A, double* B, hipStream_t s) {
-    dim3 grid((unsigned)(g.lda / kRows), (unsigned)((g.n + kCols - 1) / kCols));
+    dim3 grid((unsigned)((g.lda + kFarRows - 1) / kFarRows), (unsigned)((g.n + kFarCols - 1) / kFarCols));
     {
         TimedScope ts("assemble_far_f64", s);
         hipLaunchKernelGGL(far_f64_kernel, grid, dim3(kRows), 0, s, g.body, g.cx, g.cy, g.cz, g.n, g.lda, A, B,
@@ -307,7 +337,7 @@ void launch_assemble_f64(const DevGeom& g, const double* W, int64_t ldw, double*
 }
 
 void launch_assemble_f32(const DevGeom& g, const double* W, int64_t ldw, float* A, float* B, hipStream_t s) {
-    dim3 grid((unsigned)(g.lda / kRows), (unsigned)((g.n + kCols - 1) / kCols));
+    dim3 grid((unsigned)((g.lda + kFarRows - 1) / kFarRows), (unsigned)((g.n + kFarCols - 1) / kFarCols));
     {
         TimedScope ts("assemble_far_f32", s);
         hipLaunchKernelGGL(far_f32_kernel, grid, dim3(kRows), 0, s, g.body, g.cx, g.cy, g.cz, g.n, g.lda, A, B,
