@@ -68,7 +68,9 @@ class ServerStats(C.Structure):
 class VpOpts(C.Structure):
     _fields_ = [("fs", Freestream), ("mach", C.c_double), ("Sref", C.c_double), ("bref", C.c_double),
                 ("cref", C.c_double), ("cg", C.c_double * 3), ("prec", C.c_int32), ("accelerate", C.c_int32),
-                ("warm_start", C.c_int32), ("max_members", C.c_int32), ("theta", C.c_double), ("tol", C.c_double)]
+                ("warm_start", C.c_int32), ("max_members", C.c_int32), ("theta", C.c_double), ("tol", C.c_double),
+                ("kernel", C.c_int32), ("per_edge", C.c_int32), ("core", C.c_double), ("fp32", C.c_int32),
+                ("pad", C.c_int32)]
 
 
 class VpStepInfo(C.Structure):
@@ -78,10 +80,12 @@ class VpStepInfo(C.Structure):
 
 
 class BhOpts(C.Structure):
-    _fields_ = [("max_members", C.c_int32), ("pad", C.c_int32), ("theta", C.c_double)]
+    _fields_ = [("max_members", C.c_int32), ("kernel", C.c_int32), ("theta", C.c_double), ("core", C.c_double),
+                ("fp32", C.c_int32), ("pad", C.c_int32)]
 
 
 CP_BH_VEL_POINTS, CP_BH_VEL, CP_BH_STRETCH, CP_BH_DIFF = 0, 1, 2, 3
+CP_VP_REFERENCE, CP_VP_ALGEBRAIC = 0, 1
 
 # Every symbol declared in include/cpgpu.h with (restype, argtypes).
 SIGNATURES = {
@@ -122,8 +126,8 @@ SIGNATURES = {
     "cp_bh_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), _d, _d]),
     "cp_bh_eval": (C.c_int, [C.c_void_p, C.c_int, C.c_int32, _d, _d]),
     "cp_bh_release": (None, [C.c_void_p]),
-    "cp_panel_to_points": (C.c_int, [C.c_void_p, _d, _d, _d, C.c_int32, _d, _d]),
-    "cp_panel_stretching": (C.c_int, [C.c_void_p, _d, _d, _d, C.c_int32, _d, _d, _d]),
+    "cp_panel_to_points": (C.c_int, [C.c_void_p, _d, _d, _d, C.c_int32, _d, _d, C.c_int32]),
+    "cp_panel_stretching": (C.c_int, [C.c_void_p, _d, _d, _d, C.c_int32, _d, _d, _d, C.c_int32]),
     "cp_set_buffer_wake": (C.c_int, [C.c_void_p, C.c_int32, _i32, _d]),
     "cp_solve_rhs": (C.c_int, [C.c_void_p, C.c_int, C.c_int32, _d, _d, _d, C.POINTER(SolveOpts), _d,
                                C.POINTER(SolveInfo)]),

@@ -66,8 +66,8 @@ class Context:
         quads behind the trailing edge (time step dt; <= 0: computed)."""
         return Geometry(self, mesh, key, vp=(V, dt))
 
-    def bh_tree(self, xyz, alpha, radius, max_members=10, theta=0.5):
-        return BHTree(self, xyz, alpha, radius, max_members, theta)
+    def bh_tree(self, xyz, alpha, radius, max_members=10, theta=0.5, kernel=0, core=0.0, fp32=False):
+        return BHTree(self, xyz, alpha, radius, max_members, theta, kernel, core, fp32)
 
     def bytes_in_use(self):
         return abi.lib().cp_bytes_in_use(self.h)
@@ -186,9 +186,10 @@ class Geometry:
         self.ctx.check(abi.lib().cp_matvec(self.h, kernel, X.shape[1], _p(X), _p(Y)), "matvec")
         return Y
 
-    def panel_to_points(self, sigma, mu, pts, mu_wake=None):
+    def panel_to_points(self, sigma, mu, pts, mu_wake=None, fp32=False):
         """Velocity induced at pts (m x 3) by the body panels (source sigma,
-        doublet mu, solve order) and the wake panels (doublet mu_wake)."""
+        doublet mu, solve order) and the wake panels (doublet mu_wake);
+        fp32: far-field pairs in FP32 arithmetic."""
         sg = np.ascontiguousarray(sigma, dtype=np.float64)
         m_ = np.ascontiguousarray(mu, dtype=np.float64)
         if sg.shape != (self.nb,) or m_.shape != (self.nb,):
@@ -198,8 +199,8 @@ class Geometry:
             raise ValueError("mu_wake needs one value per wake panel")
         q = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 3)
         out = np.zeros_like(q)
-        self.ctx.check(abi.lib().cp_panel_to_points(self.h, _p(sg), _p(m_), _p(mw), len(q), _p(q), _p(out)),
-                       "cp_panel_to_points")
+        self.ctx.check(abi.lib().cp_panel_to_points(self.h, _p(sg), _p(m_), _p(mw), len(q), _p(q), _p(out),
+                                                    int(fp32)), "cp_panel_to_points")
         return out
 
     def solve_with_derivatives(self, condition, prec=CP_FP64, delta_deg=0.5, **kw):
@@ -243,9 +244,10 @@ class Geometry:
                       converged=bool(i.converged)) for i in info]
         return (mu[0] if one else mu.T), infos
 
-    def panel_stretching(self, sigma, mu, pos, alpha, mu_wake=None):
+    def panel_stretching(self, sigma, mu, pos, alpha, mu_wake=None, fp32=False):
         """Stretching of particles (pos, alpha: m x 3) by the body panels
-        (sigma, mu) and wake panels (mu_wake): sum of partStretching."""
+        (sigma, mu) and wake panels (mu_wake): sum of partStretching;
+        fp32: far-field pairs in FP32 arithmetic."""
         sg = np.ascontiguousarray(sigma, dtype=np.float64)
         m_ = np.ascontiguousarray(mu, dtype=np.float64)
         if sg.shape != (self.nb,) or m_.shape != (self.nb,):
@@ -256,8 +258,8 @@ class Geometry:
         if al.shape != q.shape:
             raise ValueError("pos and alpha must describe the same particles")
         out = np.zeros_like(q)
-        self.ctx.check(abi.lib().cp_panel_stretching(self.h, _p(sg), _p(m_), _p(mw), len(q), _p(q), _p(al), _p(out)),
-                       "cp_panel_stretching")
+        self.ctx.check(abi.lib().cp_panel_stretching(self.h, _p(sg), _p(m_), _p(mw), len(q), _p(q), _p(al), _p(out),
+                                                     int(fp32)), "cp_panel_stretching")
         return out
 
     def case_fields(self, conditions, mu, mach=0.1, Sref=1.0, bref=1.0, cref=1.0, cg=(0, 0, 0)):
@@ -298,9 +300,13 @@ class Geometry:
         return dt.value, r1, r2
 
     def vp_case(self, condition, prec=CP_FP64, mach=0.1, Sref=1.0, bref=1.0, cref=1.0, cg=(0, 0, 0), accelerate=True,
-                warm_start=False, max_members=10, theta=0.5, tol=1e-12):
+                warm_start=False, max_members=10, theta=0.5, tol=1e-12, kernel=0, per_edge=1, core=0.0, fp32=False):
+        """kernel=abi.CP_VP_ALGEBRAIC, per_edge, core, fp32: the configured
+        form (algebraic fixed-core particle kernel without diffusion, q
+        particles per second-row panel and step, FP32 particle and far-field
+        panel sums); the defaults are the reference's case."""
         return VpCase(self, condition, prec, mach, Sref, bref, cref, cg, accelerate, warm_start, max_members, theta,
-                      tol)
+                      tol, kernel, per_edge, core, fp32)
 
     def block_jacobi(self):
         """(ptr, perm, [inverse of each block as a b x b FP32 array]) of the
@@ -367,7 +373,7 @@ class VpCase:
     forces."""
 
     def __init__(self, geom, condition, prec, mach, Sref, bref, cref, cg, accelerate, warm_start, max_members,
-                 theta, tol):
+                 theta, tol, kernel=0, per_edge=1, core=0.0, fp32=False):
         self.g = geom
         o = abi.VpOpts()
         o.fs = abi.Freestream(*map(float, condition))
@@ -375,6 +381,7 @@ class VpCase:
         o.cg = (C.c_double * 3)(*cg)
         o.prec, o.accelerate, o.warm_start, o.max_members = prec, int(accelerate), int(warm_start), max_members
         o.theta, o.tol = theta, tol
+        o.kernel, o.per_edge, o.core, o.fp32 = kernel, per_edge, core, int(fp32)
         h = C.c_void_p()
         geom.ctx.check(abi.lib().cp_vp_create(geom.h, C.byref(o), C.byref(h)), "vp create")
         self.h = h
@@ -440,7 +447,9 @@ class BHTree:
     (particleFMM::build, particleFMM.cpp:22-27); eval() runs the Barnes-Hut
     sums of particleFMM::barnesHutVel/Stretch/Diff (particleFMM.cpp:65-116)."""
 
-    def __init__(self, ctx, xyz, alpha, radius, max_members=10, theta=0.5):
+    def __init__(self, ctx, xyz, alpha, radius, max_members=10, theta=0.5, kernel=0, core=0.0, fp32=False):
+        """kernel=abi.CP_VP_ALGEBRAIC: the algebraic kernel of one core
+        radius `core` for every mode (no CP_BH_DIFF), in FP32 with fp32."""
         self.ctx = ctx
         self.xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
         self.alpha = np.ascontiguousarray(alpha, dtype=np.float64).reshape(-1, 3)
@@ -449,7 +458,7 @@ class BHTree:
         if self.xyz.shape[0] != self.n or self.alpha.shape[0] != self.n:
             raise ValueError("xyz, alpha and radius must describe the same particles")
         h = C.c_void_p()
-        o = abi.BhOpts(max_members, 0, theta)
+        o = abi.BhOpts(max_members, kernel, theta, core, int(fp32), 0)
         ctx.check(abi.lib().cp_bh_build(ctx.h, self.n, _p(self.xyz), _p(self.alpha), _p(self.radius), C.byref(o),
                                         C.byref(h)), "cp_bh_build")
         self.h = h

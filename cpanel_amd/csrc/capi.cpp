@@ -1037,9 +1037,17 @@ cp_status cp_bh_build(cp_ctx* c, int32_t n, const double* xyz, const double* alp
     *out = nullptr;
     int maxm = (opts && opts->max_members > 0) ? opts->max_members : 10;
     double theta = (opts && opts->theta >= 0) ? opts->theta : 0.5;
+    const int kernel = opts ? opts->kernel : CP_VP_REFERENCE;
+    if (kernel != CP_VP_REFERENCE && kernel != CP_VP_ALGEBRAIC) return CP_BAD_ARG;
+    if (kernel == CP_VP_ALGEBRAIC && !(opts->core > 0)) return CP_BAD_ARG;
     std::unique_ptr<cp_tree> t(new cp_tree);
     t->ctx = c;
     t->theta = theta;
+    t->kernel = kernel;
+    if (kernel == CP_VP_ALGEBRAIC) {
+        t->core = opts->core;
+        t->fp32 = opts->fp32 != 0;
+    }
     cp_status st = guarded(c, [&]() -> cp_status {
         build_bh_tree(n, xyz, alpha, radius, maxm, t->host);
         t->dev.n = n;
@@ -1079,6 +1087,8 @@ cp_status cp_bh_info(const cp_tree* t, int32_t* n_nodes, int32_t* levels, double
 cp_status cp_bh_eval(cp_tree* t, cp_bh_mode mode, int32_t m, const double* pts, double* out) {
     if (!t || !out || mode < CP_BH_VEL_POINTS || mode > CP_BH_DIFF) return CP_BAD_ARG;
     if (mode == CP_BH_VEL_POINTS && (m < 0 || (m > 0 && !pts))) return CP_BAD_ARG;
+    const bool alg = t->kernel == CP_VP_ALGEBRAIC;
+    if (alg && mode == CP_BH_DIFF) return fail(t->ctx, CP_BAD_ARG, "the algebraic kernel has no diffusion term");
     cp_ctx* c = t->ctx;
     if (c->device < 0) return fail(c, CP_NO_DEVICE, "host-only context");
     return guarded(c, [&]() -> cp_status {
@@ -1097,7 +1107,15 @@ cp_status cp_bh_eval(cp_tree* t, cp_bh_mode mode, int32_t m, const double* pts, 
             dalloc(c, dpts, (size_t)3 * m * sizeof(double));
             CP_HIP(hipMemcpyAsync(dpts.p, pts, (size_t)3 * m * sizeof(double), hipMemcpyHostToDevice, s));
             TimedScope ts("bh_eval", s);
-            launch_bh_points(t->dev, t->theta, m, dpts.as<double>(), dout.as<double>(), s);
+            if (alg)
+                launch_core_bh(t->dev, CP_BH_VEL, t->theta, t->core, t->fp32, m, dpts.as<double>(), nullptr, nullptr,
+                               nullptr, dout.as<double>(), s);
+            else
+                launch_bh_points(t->dev, t->theta, m, dpts.as<double>(), dout.as<double>(), s);
+        } else if (alg) {
+            TimedScope ts("bh_eval", s);
+            launch_core_bh(t->dev, (int)mode, t->theta, t->core, t->fp32, t->dev.n, t->dev.mpos, t->dev.mstr,
+                           nullptr, t->dev.order, dout.as<double>(), s);
         } else {
             TimedScope ts("bh_eval", s);
             launch_bh_particles(t->dev, (int)mode, t->theta, dout.as<double>(), s);
@@ -1120,7 +1138,7 @@ void cp_bh_release(cp_tree* t) {
 }
 
 cp_status cp_panel_to_points(cp_geom* g, const double* sigma, const double* mu, const double* mu_wake, int32_t m,
-                             const double* pts, double* vel) {
+                             const double* pts, double* vel, int32_t fp32) {
     if (!g || !sigma || !mu || m < 0 || (m > 0 && (!pts || !vel))) return CP_BAD_ARG;
     cp_ctx* c = g->ctx;
     if (c->device < 0) return fail(c, CP_NO_DEVICE, "host-only context");
@@ -1142,7 +1160,7 @@ cp_status cp_panel_to_points(cp_geom* g, const double* sigma, const double* mu, 
         {
             TimedScope ts("panel_to_points", s);
             launch_panel_to_points(g->dev, dstr.as<double>(), dstr.as<double>() + nb, dstr.as<double>() + 2 * nb, m,
-                                   dpts.as<double>(), KS, dpart.as<double>(), dout.as<double>(), s);
+                                   dpts.as<double>(), KS, dpart.as<double>(), dout.as<double>(), s, fp32 != 0);
         }
         CP_HIP(hipMemcpyAsync(vel, dout.p, (size_t)3 * m * sizeof(double), hipMemcpyDeviceToHost, s));
         CP_HIP(hipStreamSynchronize(s));
@@ -1281,7 +1299,7 @@ cp_status cp_solve_rhs(cp_geom* g, cp_prec prec, int32_t k, const double* sigma,
 }
 
 cp_status cp_panel_stretching(cp_geom* g, const double* sigma, const double* mu, const double* mu_wake, int32_t m,
-                              const double* pos, const double* alpha, double* out) {
+                              const double* pos, const double* alpha, double* out, int32_t fp32) {
     if (!g || !sigma || !mu || m < 0 || (m > 0 && (!pos || !alpha || !out))) return CP_BAD_ARG;
     cp_ctx* c = g->ctx;
     if (c->device < 0) return fail(c, CP_NO_DEVICE, "host-only context");
@@ -1306,7 +1324,7 @@ cp_status cp_panel_stretching(cp_geom* g, const double* sigma, const double* mu,
             TimedScope ts("panel_stretching", s);
             launch_panel_stretching(g->dev, dstr.as<double>(), dstr.as<double>() + nb, dstr.as<double>() + 2 * nb, m,
                                     dpa.as<double>(), dpa.as<double>() + 3 * (size_t)m, KS, dpart.as<double>(),
-                                    dout.as<double>(), s);
+                                    dout.as<double>(), s, fp32 != 0);
         }
         CP_HIP(hipMemcpyAsync(out, dout.p, (size_t)3 * m * sizeof(double), hipMemcpyDeviceToHost, s));
         CP_HIP(hipStreamSynchronize(s));

@@ -170,6 +170,9 @@ struct cp_tree {
     DevBhTree dev;
     DevBuf nodes, mpos, mstr, mrad, order;
     double theta = 0.5;
+    int kernel = CP_VP_REFERENCE;  // cp_vp_kernel
+    double core = 0;               // CP_VP_ALGEBRAIC core radius
+    bool fp32 = false;
 };
 
 namespace cpgpu {

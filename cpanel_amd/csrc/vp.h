@@ -47,14 +47,29 @@ void launch_filament_points(int nf, const double* ends, const double* fstr, int 
 int p2p_slices(int m, int npanels);
 // out[3m] = velocity at pts of the body (sigma, mu) and wake (muw) panels;
 // part: KS * 3m scratch
+// f32far: far-field pairs in FP32 arithmetic (near field FP64)
 void launch_panel_to_points(const DevGeom& g, const double* sigma, const double* mu, const double* muw, int m,
-                            const double* pts, int KS, double* part, double* out, hipStream_t s);
+                            const double* pts, int KS, double* part, double* out, hipStream_t s, bool f32far = false);
 
 // out[3m] = sum over body and wake panels of partStretching at the particles
 // (pos, alpha: 3 per particle); part: KS * 3m scratch
 void launch_panel_stretching(const DevGeom& g, const double* sigma, const double* mu, const double* muw, int m,
                              const double* pos, const double* alpha, int KS, double* part, double* out,
-                             hipStream_t s);
+                             hipStream_t s, bool f32far = false);
+
+// The algebraic particle kernel (velInflAlgSmooth with one core radius for
+// every particle): mode CP_BH_VEL (velocity; tstr unused) or CP_BH_STRETCH
+// ((alpha . grad) u), in FP32 (f32) or FP64 arithmetic.
+// Barnes-Hut on tree T at m targets: caller-given (tpos, tstr; tslot[t] the
+// target's slot in T, -1 none, or tslot null) with out[3m] in target order,
+// or T's own members (tpos = T.mpos, tstr = T.mstr, m = T.n, own_order =
+// T.order) with out[3n] in particle order.
+void launch_core_bh(const DevBhTree& T, int mode, double theta, double core, bool f32, int m, const double* tpos,
+                    const double* tstr, const int* tslot, const int* own_order, double* out, hipStream_t s);
+// direct sums over n particles at m targets; own: the targets are the
+// particles themselves (m = n), each skipping itself
+void launch_core_direct(int mode, double core, bool f32, int n, const double* pos, const double* str, int m,
+                        const double* tpos, const double* tstr, bool own, double* out, hipStream_t s);
 // y[0..n) += a[0..n)
 void launch_add_vec(int n, const double* a, double* y, hipStream_t s);
 // out (column-major, ld) = -C mu for k columns: C is lda x m (FP64), mu m x k

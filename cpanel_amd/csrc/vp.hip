@@ -404,6 +404,265 @@ __global__ __launch_bounds__(kT) void stretch_kernel(const PanelRec* __restrict_
     for (int r = 0; r < 3; r++) o[r] = G.a[r][0] * a0 + G.a[r][1] * a1 + G.a[r][2] * a2;
 }
 
+// ---------------------------------------------------------------------------
+// The configured unsteady case's particle kernel: velInflAlgSmooth's
+// algebraic smoothing (particle.cpp:31-38) with one core radius for every
+// particle (s2 = core^2), velocity and (alpha_t . grad) u stretching.  The
+// pair offset is formed in FP64 (particles sit ~1e-2 apart at coordinates of
+// order 10, below FP32's resolution of an absolute position); the rest runs
+// in T (float: the FP32 path; double: the FP64 path the FP32 one is checked
+// against).
+template <class T>
+__device__ inline void kc_vel(const double* sp, const T* sa, const double* tp, T s2, T* acc) {
+    const T dx = (T)(tp[0] - sp[0]), dy = (T)(tp[1] - sp[1]), dz = (T)(tp[2] - sp[2]);
+    const T r2 = dx * dx + dy * dy + dz * dz;
+    const T q = r2 + s2;
+    const T f = (T)(-1.0 / (4 * kPI)) * (r2 + (T)2.5 * s2) / (q * q * sqrt(q));
+    acc[0] += f * (dy * sa[2] - dz * sa[1]);
+    acc[1] += f * (dz * sa[0] - dx * sa[2]);
+    acc[2] += f * (dx * sa[1] - dy * sa[0]);
+}
+
+// u = f(r) x x a_s with x = target - source: (a_t . grad) u =
+// f'(r)/r (a_t . x)(x x a_s) + f (a_t x a_s), f'/r = (3r^2 + 10.5 s^2) /
+// (4 pi (r^2 + s^2)^3.5)
+template <class T>
+__device__ inline void kc_stretch(const double* sp, const T* sa, const double* tp, const T* ta, T s2, T* acc) {
+    const T dx = (T)(tp[0] - sp[0]), dy = (T)(tp[1] - sp[1]), dz = (T)(tp[2] - sp[2]);
+    const T r2 = dx * dx + dy * dy + dz * dz;
+    const T q = r2 + s2;
+    const T sq = sqrt(q);
+    const T f = (T)(-1.0 / (4 * kPI)) * (r2 + (T)2.5 * s2) / (q * q * sq);
+    const T h = (T)(1.0 / (4 * kPI)) * ((T)3 * r2 + (T)10.5 * s2) / (q * q * q * sq);
+    const T hd = h * (ta[0] * dx + ta[1] * dy + ta[2] * dz);
+    acc[0] += hd * (dy * sa[2] - dz * sa[1]) + f * (ta[1] * sa[2] - ta[2] * sa[1]);
+    acc[1] += hd * (dz * sa[0] - dx * sa[2]) + f * (ta[2] * sa[0] - ta[0] * sa[2]);
+    acc[2] += hd * (dx * sa[1] - dy * sa[0]) + f * (ta[0] * sa[1] - ta[1] * sa[0]);
+}
+
+// Barnes-Hut with the configured kernel on m targets (the reference's walk:
+// root excluded, far iff s/d < theta, decided in FP64 so both precisions
+// visit the same cells; leaves skip the target's own slot).  KC_VEL at
+// points / particles (tstr unused), KC_STRETCH on particles.  Targets are
+// either caller-given (tslot[t]: the target's slot or -1; tslot null: no
+// slot; result t to out[t]) or, with own_order, the tree's own members in
+// slot order (target t is slot t; result to out[own_order[t]]).
+enum { KC_VEL = 0, KC_STRETCH = 1 };
+template <int MODE, class T>
+__global__ __launch_bounds__(kT) void bhc_kernel(const BhNode* __restrict__ nodes, int n_nodes,
+                                                 const double* __restrict__ mpos, const double* __restrict__ mstr,
+                                                 double theta, double s2d, int m, const double* __restrict__ tpos,
+                                                 const double* __restrict__ tstr, const int* __restrict__ tslot,
+                                                 const int* __restrict__ own_order, double* __restrict__ out) {
+    const int t = blockIdx.x * kT + threadIdx.x;
+    if (t >= m) return;
+    const double tp[3] = {tpos[3 * t], tpos[3 * t + 1], tpos[3 * t + 2]};
+    T ta[3] = {0, 0, 0};
+    if (MODE == KC_STRETCH) {
+        ta[0] = (T)tstr[3 * t];
+        ta[1] = (T)tstr[3 * t + 1];
+        ta[2] = (T)tstr[3 * t + 2];
+    }
+    const int self = own_order ? t : (tslot ? tslot[t] : -1);
+    const int o = own_order ? own_order[t] : t;
+    const T s2 = (T)s2d;
+    T acc[3] = {0, 0, 0};
+    int i = 1;
+    while (i < n_nodes) {
+        const BhNode nd = nodes[i];
+        if (nd.count >= 0) {
+            for (int k = 0; k < nd.count; k++) {
+                const int ms = nd.first + k;
+                if (ms == self) continue;
+                const T sa[3] = {(T)mstr[3 * ms], (T)mstr[3 * ms + 1], (T)mstr[3 * ms + 2]};
+                if (MODE == KC_VEL)
+                    kc_vel<T>(mpos + 3 * ms, sa, tp, s2, acc);
+                else
+                    kc_stretch<T>(mpos + 3 * ms, sa, tp, ta, s2, acc);
+            }
+            i = nd.skip;
+        } else if (far_cell(nd, tp[0], tp[1], tp[2], theta)) {
+            const double sp[3] = {nd.ex, nd.ey, nd.ez};
+            const T sa[3] = {(T)nd.sx, (T)nd.sy, (T)nd.sz};
+            if (MODE == KC_VEL)
+                kc_vel<T>(sp, sa, tp, s2, acc);
+            else
+                kc_stretch<T>(sp, sa, tp, ta, s2, acc);
+            i = nd.skip;
+        } else {
+            i = i + 1;
+        }
+    }
+    out[3 * o] = acc[0];
+    out[3 * o + 1] = acc[1];
+    out[3 * o + 2] = acc[2];
+}
+
+// direct sums of the configured kernel over n particles at m targets, in
+// particle order; with own, target t is particle t and skips itself
+template <int MODE, class T>
+__global__ __launch_bounds__(kT) void dc_kernel(int n, const double* __restrict__ pos, const double* __restrict__ str,
+                                                double s2d, int m, const double* __restrict__ tpos,
+                                                const double* __restrict__ tstr, int own, double* __restrict__ out) {
+    const int t = blockIdx.x * kT + threadIdx.x;
+    if (t >= m) return;
+    const double tp[3] = {tpos[3 * t], tpos[3 * t + 1], tpos[3 * t + 2]};
+    T ta[3] = {0, 0, 0};
+    if (MODE == KC_STRETCH) {
+        ta[0] = (T)tstr[3 * t];
+        ta[1] = (T)tstr[3 * t + 1];
+        ta[2] = (T)tstr[3 * t + 2];
+    }
+    const int self = own ? t : -1;
+    const T s2 = (T)s2d;
+    T acc[3] = {0, 0, 0};
+    for (int j = 0; j < n; j++) {
+        if (j == self) continue;
+        const T sa[3] = {(T)str[3 * j], (T)str[3 * j + 1], (T)str[3 * j + 2]};
+        if (MODE == KC_VEL)
+            kc_vel<T>(pos + 3 * j, sa, tp, s2, acc);
+        else
+            kc_stretch<T>(pos + 3 * j, sa, tp, ta, s2, acc);
+    }
+    out[3 * t] = acc[0];
+    out[3 * t + 1] = acc[1];
+    out[3 * t + 2] = acc[2];
+}
+
+// ---- FP32 far field of the panel -> particle sums (FP64 near field)
+// point source (negated, bodyPanel.cpp:314) and point doublet velocity of a
+// far panel times their strengths, added to acc
+__device__ inline void p2p_far_f32(const PanelRec& P, const double* p, float sg, float mj, bool body, float* acc) {
+    const float x = (float)(p[0] - P.c[0]), y = (float)(p[1] - P.c[1]), z = (float)(p[2] - P.c[2]);
+    const float ri = rsqrtf(x * x + y * y + z * z);
+    const float ri2 = ri * ri;
+    const float ari3 = (float)P.area_4pi * ri2 * ri;  // area / (4 pi r^3)
+    const float nx = (float)P.n[0], ny = (float)P.n[1], nz = (float)P.n[2];
+    const float t = 3.0f * (x * nx + y * ny + z * nz) * ri2;
+    const float fd = mj * ari3;
+    acc[0] += fd * (t * x - nx);
+    acc[1] += fd * (t * y - ny);
+    acc[2] += fd * (t * z - nz);
+    if (body) {
+        const float fs = -sg * ari3;
+        acc[0] += fs * x;
+        acc[1] += fs * y;
+        acc[2] += fs * z;
+    }
+}
+
+// the point source and point doublet gradients of vg_point_source_add /
+// vg_point_doublet_add (bodyPanel.cpp:733-763, panel.cpp:813-840) in FP32,
+// with the reference's frames (the panel centre's z), added to G
+__device__ inline void vg_far_f32(const PanelRec& P, bool body, float sigma, float mu, const double* poi,
+                                  float G[3][3]) {
+    const float z = (float)P.c[2], zz0 = (float)(P.c[2] - poi[2]);
+    const float dx = (float)(P.c[0] - poi[0]), dy = (float)(P.c[1] - poi[1]);
+    const float dx2 = dx * dx, dy2 = dy * dy;
+    const float q2 = dx2 + dy2 + z * z;
+    const float sq = sqrtf(q2);
+    const float area = (float)P.area;
+    const float i4p = (float)(1.0 / (4 * kPI));
+    if (body) {
+        const float isdd = 1.0f / (q2 * q2 * sq);
+        const float c = sigma * area * i4p;
+        G[0][0] += c * ((-2 * dx2 + dy2 + z * z) * isdd);
+        G[1][0] += c * (-3 * dx * dy * isdd);
+        G[2][0] += c * (-3 * dx * z * isdd);
+        G[0][1] += c * (-3 * dx * dy * isdd);
+        G[1][1] += c * ((dx2 - dy2 + z * z) * isdd);
+        G[2][1] += c * (-3 * z * dy * isdd);
+        G[0][2] += c * (-3 * dx * zz0 * isdd);
+        G[1][2] += c * (-3 * dy * zz0 * isdd);
+        G[2][2] += c * ((-3 * z * zz0 + (dx2 + dy2 + z * z)) * isdd);
+    }
+    const float iddd = 1.0f / (q2 * q2 * q2 * sq);
+    const float uv = 3 * mu * area * i4p;
+    const float w = -mu * area * i4p;
+    G[0][0] += uv * z * (-4 * dx2 + dy2 + z * z) * iddd;
+    G[1][0] += uv * (-5) * z * dy * dx * iddd;
+    G[2][0] += uv * dx * (dx2 + dy2 - 4 * z * z) * iddd;
+    G[0][1] += uv * (-5) * dx * dy * z * iddd;
+    G[1][1] += uv * z * (dx2 - 4 * dy2 + z * z) * iddd;
+    G[2][1] += uv * dy * (dx2 + dy2 - 4 * z * z) * iddd;
+    G[0][2] += w * (-3) * dx * (dx2 + dy2 - 4 * z * z) * iddd;
+    G[1][2] += w * (-3) * dy * (dx2 + dy2 - 4 * z * z) * iddd;
+    G[2][2] += w * (-3) * z * (3 * dx2 + 3 * dy2 - 2 * z * z) * iddd;
+}
+
+// panel -> point velocity, far field in FP32 and near field in FP64 (one
+// accumulator each, added at the end)
+__global__ __launch_bounds__(kT) void p2p_f32_kernel(const PanelRec* __restrict__ body, int nb,
+                                                     const PanelRec* __restrict__ wake, int nw,
+                                                     const double* __restrict__ sigma, const double* __restrict__ mu,
+                                                     const double* __restrict__ muw, int m,
+                                                     const double* __restrict__ pts, int chunk,
+                                                     double* __restrict__ part) {
+    const int t = blockIdx.x * kT + threadIdx.x;
+    const int ks = blockIdx.y;
+    const int p0 = ks * chunk, p1 = min(nb + nw, p0 + chunk);
+    if (t >= m) return;
+    const double p[3] = {pts[3 * t], pts[3 * t + 1], pts[3 * t + 2]};
+    double acc[3] = {0, 0, 0};
+    float accf[3] = {0, 0, 0};
+    for (int j = p0; j < p1; j++) {
+        const bool isb = j < nb;
+        const PanelRec& P = isb ? body[j] : wake[j - nb];
+        const double d0 = p[0] - P.c[0], d1 = p[1] - P.c[1], d2 = p[2] - P.c[2];
+        if (is_far(P, sq_norm3(d0, d1, d2))) {
+            p2p_far_f32(P, p, isb ? (float)sigma[j] : 0.0f, isb ? (float)mu[j] : (float)muw[j - nb], isb, accf);
+        } else if (isb) {
+            double vs[3], vd[3];
+            panel_v_inf_f64(P, p, vs, vd);
+            const double sg = sigma[j], mj = mu[j];
+            for (int q = 0; q < 3; q++) acc[q] += sg * vs[q] + mj * vd[q];
+        } else {
+            double vd[3];
+            dub_v_inf_f64(P, p, vd);
+            const double mw = muw[j - nb];
+            for (int q = 0; q < 3; q++) acc[q] += mw * vd[q];
+        }
+    }
+    double* o = part + ((int64_t)ks * m + t) * 3;
+    for (int q = 0; q < 3; q++) o[q] = acc[q] + (double)accf[q];
+}
+
+// panel -> particle stretching, far field in FP32 and near field in FP64
+__global__ __launch_bounds__(kT) void stretch_f32_kernel(const PanelRec* __restrict__ body, int nb,
+                                                         const PanelRec* __restrict__ wake, int nw,
+                                                         const double* __restrict__ sigma,
+                                                         const double* __restrict__ mu,
+                                                         const double* __restrict__ muw, int m,
+                                                         const double* __restrict__ pos,
+                                                         const double* __restrict__ alpha, int chunk,
+                                                         double* __restrict__ part) {
+    const int t = blockIdx.x * kT + threadIdx.x;
+    const int ks = blockIdx.y;
+    const int p0 = ks * chunk, p1 = min(nb + nw, p0 + chunk);
+    if (t >= m) return;
+    const double p[3] = {pos[3 * t], pos[3 * t + 1], pos[3 * t + 2]};
+    Mat3 G;
+    mat3_zero(G);
+    float Gf[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    for (int j = p0; j < p1; j++) {
+        const bool isb = j < nb;
+        const PanelRec& P = isb ? body[j] : wake[j - nb];
+        const double d0 = p[0] - P.c[0], d1 = p[1] - P.c[1], d2 = p[2] - P.c[2];
+        if (is_far(P, sq_norm3(d0, d1, d2)))
+            vg_far_f32(P, isb, isb ? (float)sigma[j] : 0.0f, isb ? (float)mu[j] : (float)muw[j - nb], p, Gf);
+        else if (isb)
+            part_stretch_add(P, true, sigma[j], mu[j], p, G);
+        else
+            part_stretch_add(P, false, 0.0, muw[j - nb], p, G);
+    }
+    const double a0 = alpha[3 * t], a1 = alpha[3 * t + 1], a2 = alpha[3 * t + 2];
+    double* o = part + ((int64_t)ks * m + t) * 3;
+    for (int r = 0; r < 3; r++) {
+        const double g0 = G.a[r][0] + Gf[r][0], g1 = G.a[r][1] + Gf[r][1], g2 = G.a[r][2] + Gf[r][2];
+        o[r] = g0 * a0 + g1 * a1 + g2 * a2;
+    }
+}
+
 __global__ void p2p_reduce_kernel(const double* __restrict__ part, int KS, int m, double* __restrict__ out) {
     int e = blockIdx.x * kT + threadIdx.x;
     if (e >= 3 * m) return;
@@ -536,13 +795,17 @@ int p2p_slices(int m, int npanels) {
 }
 
 void launch_panel_to_points(const DevGeom& g, const double* sigma, const double* mu, const double* muw, int m,
-                            const double* pts, int KS, double* part, double* out, hipStream_t s) {
+                            const double* pts, int KS, double* part, double* out, hipStream_t s, bool f32far) {
     if (m <= 0) return;
     const int np = g.n + g.nw;
     const int chunk = (np + KS - 1) / KS;
     dim3 grid((unsigned)((m + kT - 1) / kT), (unsigned)KS);
-    hipLaunchKernelGGL(p2p_kernel, grid, dim3(kT), 0, s, g.body, g.n, g.wake, g.nw, sigma, mu, muw, m, pts, chunk,
-                       part);
+    if (f32far)
+        hipLaunchKernelGGL(p2p_f32_kernel, grid, dim3(kT), 0, s, g.body, g.n, g.wake, g.nw, sigma, mu, muw, m, pts,
+                           chunk, part);
+    else
+        hipLaunchKernelGGL(p2p_kernel, grid, dim3(kT), 0, s, g.body, g.n, g.wake, g.nw, sigma, mu, muw, m, pts, chunk,
+                           part);
     CP_LAUNCH_CHECK();
     hipLaunchKernelGGL(p2p_reduce_kernel, dim3((unsigned)((3 * m + kT - 1) / kT)), dim3(kT), 0, s,
                        (const double*)part, KS, m, out);
@@ -551,16 +814,66 @@ void launch_panel_to_points(const DevGeom& g, const double* sigma, const double*
 
 void launch_panel_stretching(const DevGeom& g, const double* sigma, const double* mu, const double* muw, int m,
                              const double* pos, const double* alpha, int KS, double* part, double* out,
-                             hipStream_t s) {
+                             hipStream_t s, bool f32far) {
     if (m <= 0) return;
     const int np = g.n + g.nw;
     const int chunk = (np + KS - 1) / KS;
     dim3 grid((unsigned)((m + kT - 1) / kT), (unsigned)KS);
-    hipLaunchKernelGGL(stretch_kernel, grid, dim3(kT), 0, s, g.body, g.n, g.wake, g.nw, sigma, mu, muw, m, pos, alpha,
-                       chunk, part);
+    if (f32far)
+        hipLaunchKernelGGL(stretch_f32_kernel, grid, dim3(kT), 0, s, g.body, g.n, g.wake, g.nw, sigma, mu, muw, m,
+                           pos, alpha, chunk, part);
+    else
+        hipLaunchKernelGGL(stretch_kernel, grid, dim3(kT), 0, s, g.body, g.n, g.wake, g.nw, sigma, mu, muw, m, pos,
+                           alpha, chunk, part);
     CP_LAUNCH_CHECK();
     hipLaunchKernelGGL(p2p_reduce_kernel, dim3((unsigned)((3 * m + kT - 1) / kT)), dim3(kT), 0, s,
                        (const double*)part, KS, m, out);
+    CP_LAUNCH_CHECK();
+}
+
+void launch_core_bh(const DevBhTree& T, int mode, double theta, double core, bool f32, int m, const double* tpos,
+                    const double* tstr, const int* tslot, const int* own_order, double* out, hipStream_t s) {
+    if (m <= 0) return;
+    dim3 grid((unsigned)((m + kT - 1) / kT));
+    const double s2 = core * core;
+#define CP_BHC(MODE, TY)                                                                                      \
+    hipLaunchKernelGGL((bhc_kernel<MODE, TY>), grid, dim3(kT), 0, s, T.nodes, T.n_nodes, T.mpos, T.mstr, theta, \
+                       s2, m, tpos, tstr, tslot, own_order, out)
+    if (mode == CP_BH_STRETCH) {
+        if (f32)
+            CP_BHC(KC_STRETCH, float);
+        else
+            CP_BHC(KC_STRETCH, double);
+    } else {
+        if (f32)
+            CP_BHC(KC_VEL, float);
+        else
+            CP_BHC(KC_VEL, double);
+    }
+#undef CP_BHC
+    CP_LAUNCH_CHECK();
+}
+
+void launch_core_direct(int mode, double core, bool f32, int n, const double* pos, const double* str, int m,
+                        const double* tpos, const double* tstr, bool own, double* out, hipStream_t s) {
+    if (m <= 0) return;
+    dim3 grid((unsigned)((m + kT - 1) / kT));
+    const double s2 = core * core;
+#define CP_DC(MODE, TY)                                                                                    \
+    hipLaunchKernelGGL((dc_kernel<MODE, TY>), grid, dim3(kT), 0, s, n, pos, str, s2, m, tpos, tstr, (int)own, \
+                       out)
+    if (mode == CP_BH_STRETCH) {
+        if (f32)
+            CP_DC(KC_STRETCH, float);
+        else
+            CP_DC(KC_STRETCH, double);
+    } else {
+        if (f32)
+            CP_DC(KC_VEL, float);
+        else
+            CP_DC(KC_VEL, double);
+    }
+#undef CP_DC
     CP_LAUNCH_CHECK();
 }
 

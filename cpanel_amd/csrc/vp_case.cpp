@@ -13,6 +13,10 @@
 //     fixed at the build), or the direct sums where the reference uses them
 //     (time steps <= 2 or 3, or with acceleration off);
 //   * panel -> particle stretching: stretch_kernel.
+// The configured form (cp_vp_opts kernel / per_edge / core / fp32) swaps the
+// particle kernel for the algebraic one with a fixed core (no diffusion
+// term), sheds q particles per second-row panel, and evaluates the particle
+// sums and the far-field panel sums in FP32; the step order is the same.
 // The order of the steps, the far-field cut-offs (12 and 20 reference
 // chords), the Adams-Bashforth updates and the RK4 seeding of new particles
 // follow cpCaseVP.cpp:160-734; wake::trefftzPlaneVP (wake.cpp:191-325) gives
@@ -56,6 +60,14 @@ struct cp_vp {
     // particles, in creation order
     std::vector<double> ppos, pstr, prad, pvel0, pdst0;
     std::vector<int> pshed, pparent;
+    // where each particle was seeded: its second-row panel's centre, or the
+    // point along the panel's span of a split panel (3 per particle)
+    std::vector<double> pseed;
+    // the configured form: algebraic kernel of core radius `core`, q
+    // particles per panel and step, FP32 sums
+    bool alg = false, f32 = false;
+    int q = 1;
+    double core = 0;
     // tree of the last build and each particle's slot in it
     std::unique_ptr<cp_tree> tree;
     std::vector<int> slot;
@@ -101,7 +113,7 @@ std::vector<double> vel_panels(cp_vp* v, const std::vector<double>& pts) {
     {
         TimedScope ts("panel_to_points", s);
         launch_panel_to_points(g->dev, dsig.as<double>(), dmu.as<double>(), dmw.as<double>(), m, dpts.as<double>(), KS,
-                               dpart.as<double>(), dout.as<double>(), s);
+                               dpart.as<double>(), dout.as<double>(), s, v->f32);
     }
     from_dev(out, dout, (size_t)3 * m, s);
     return out;
@@ -122,7 +134,7 @@ std::vector<double> vel_row2(cp_vp* v, const std::vector<double>& pts) {
     dalloc(c, dpart, (size_t)KS * 3 * m * sizeof(double));
     dalloc(c, dout, (size_t)3 * m * sizeof(double));
     launch_panel_to_points(v->row2, nullptr, nullptr, dmw.as<double>(), m, dpts.as<double>(), KS, dpart.as<double>(),
-                           dout.as<double>(), s);
+                           dout.as<double>(), s, v->f32);
     from_dev(out, dout, (size_t)3 * m, s);
     return out;
 }
@@ -144,8 +156,9 @@ std::vector<double> vel_filaments(cp_vp* v, const std::vector<double>& pts) {
     return out;
 }
 
-// direct Gaussian point velocity of every particle (particle::velInfl(POI))
-std::vector<double> vel_gauss(cp_vp* v, const std::vector<double>& pts) {
+// direct point velocity of every particle: the Gaussian particle::velInfl(
+// POI), or the algebraic kernel in the configured form
+std::vector<double> vel_direct(cp_vp* v, const std::vector<double>& pts) {
     const int m = (int)(pts.size() / 3), np = (int)v->prad.size();
     std::vector<double> out((size_t)3 * m, 0.0);
     if (m == 0 || np == 0) return out;
@@ -155,11 +168,16 @@ std::vector<double> vel_gauss(cp_vp* v, const std::vector<double>& pts) {
     DevBuf dp, ds, dr, dpts, dout;
     to_dev(c, dp, v->ppos, s);
     to_dev(c, ds, v->pstr, s);
-    to_dev(c, dr, v->prad, s);
     to_dev(c, dpts, pts, s);
     dalloc(c, dout, (size_t)3 * m * sizeof(double));
-    launch_gauss_points(np, dp.as<double>(), ds.as<double>(), dr.as<double>(), m, dpts.as<double>(), dout.as<double>(),
-                        s);
+    if (v->alg) {
+        launch_core_direct(CP_BH_VEL, v->core, v->f32, np, dp.as<double>(), ds.as<double>(), m, dpts.as<double>(),
+                           nullptr, false, dout.as<double>(), s);
+    } else {
+        to_dev(c, dr, v->prad, s);
+        launch_gauss_points(np, dp.as<double>(), ds.as<double>(), dr.as<double>(), m, dpts.as<double>(),
+                            dout.as<double>(), s);
+    }
     from_dev(out, dout, (size_t)3 * m, s);
     return out;
 }
@@ -195,7 +213,11 @@ std::vector<double> vel_bh_points(cp_vp* v, const std::vector<double>& pts) {
     dalloc(c, dout, (size_t)3 * m * sizeof(double));
     {
         TimedScope ts("bh_eval", s);
-        launch_bh_points(v->tree->dev, v->tree->theta, m, dpts.as<double>(), dout.as<double>(), s);
+        if (v->alg)
+            launch_core_bh(v->tree->dev, CP_BH_VEL, v->tree->theta, v->core, v->f32, m, dpts.as<double>(), nullptr,
+                           nullptr, nullptr, dout.as<double>(), s);
+        else
+            launch_bh_points(v->tree->dev, v->tree->theta, m, dpts.as<double>(), dout.as<double>(), s);
     }
     from_dev(out, dout, (size_t)3 * m, s);
     return out;
@@ -221,8 +243,12 @@ std::vector<double> bh_on_particles(cp_vp* v, int mode) {
     dalloc(c, dout, (size_t)3 * np * sizeof(double));
     {
         TimedScope ts("bh_eval", s);
-        launch_bh_targets(v->tree->dev, mode, v->tree->theta, np, dp.as<double>(), ds.as<double>(), dr.as<double>(),
-                          dsl.as<int>(), dout.as<double>(), s);
+        if (v->alg)
+            launch_core_bh(v->tree->dev, mode, v->tree->theta, v->core, v->f32, np, dp.as<double>(), ds.as<double>(),
+                           dsl.as<int>(), nullptr, dout.as<double>(), s);
+        else
+            launch_bh_targets(v->tree->dev, mode, v->tree->theta, np, dp.as<double>(), ds.as<double>(),
+                              dr.as<double>(), dsl.as<int>(), dout.as<double>(), s);
     }
     from_dev(out, dout, (size_t)3 * np, s);
     return out;
@@ -240,7 +266,11 @@ std::vector<double> pairs_direct(cp_vp* v, int mode) {
     to_dev(c, ds, v->pstr, s);
     to_dev(c, dr, v->prad, s);
     dalloc(c, dout, (size_t)3 * np * sizeof(double));
-    launch_pairs_direct(mode, np, dp.as<double>(), ds.as<double>(), dr.as<double>(), dout.as<double>(), s);
+    if (v->alg)
+        launch_core_direct(mode, v->core, v->f32, np, dp.as<double>(), ds.as<double>(), np, dp.as<double>(),
+                           ds.as<double>(), true, dout.as<double>(), s);
+    else
+        launch_pairs_direct(mode, np, dp.as<double>(), ds.as<double>(), dr.as<double>(), dout.as<double>(), s);
     from_dev(out, dout, (size_t)3 * np, s);
     return out;
 }
@@ -265,7 +295,7 @@ std::vector<double> panel_stretch(cp_vp* v) {
     {
         TimedScope ts("panel_stretching", s);
         launch_panel_stretching(g->dev, dsig.as<double>(), dmu.as<double>(), dmw.as<double>(), np, dp.as<double>(),
-                                ds.as<double>(), KS, dpart.as<double>(), dout.as<double>(), s);
+                                ds.as<double>(), KS, dpart.as<double>(), dout.as<double>(), s, v->f32);
     }
     from_dev(out, dout, (size_t)3 * np, s);
     return out;
@@ -278,6 +308,11 @@ void build_tree(cp_vp* v) {
     std::unique_ptr<cp_tree> t(new cp_tree);
     t->ctx = c;
     t->theta = v->o.theta >= 0 ? v->o.theta : 0.5;
+    if (v->alg) {
+        t->kernel = CP_VP_ALGEBRAIC;
+        t->core = v->core;
+        t->fp32 = v->f32;
+    }
     const int maxm = v->o.max_members > 0 ? v->o.max_members : 10;
     build_bh_tree(np, v->ppos.data(), v->pstr.data(), v->prad.data(), maxm, t->host);
     upload(c, t->nodes, t->host.nodes);
@@ -352,7 +387,7 @@ std::vector<double> particle_velocities(cp_vp* v) {
     if (v->o.accelerate && v->ts > 2)
         part = bh_on_particles(v, CP_BH_VEL);
     else
-        part = vel_gauss(v, v->ppos);
+        part = vel_direct(v, v->ppos);
     const std::vector<double> pan = vel_panels(v, v->ppos);
     const std::vector<double> r2 = vel_row2(v, v->ppos);
     const std::vector<double> fil = vel_filaments(v, v->ppos);
@@ -375,7 +410,7 @@ std::vector<double> particle_velocities(cp_vp* v) {
 // particle term from the tree (algebraic) or the direct Gaussian sum
 std::vector<double> everything_at(cp_vp* v, const std::vector<double>& pts, bool bh) {
     const int m = (int)(pts.size() / 3);
-    const std::vector<double> part = bh ? vel_bh_points(v, pts) : vel_gauss(v, pts);
+    const std::vector<double> part = bh ? vel_bh_points(v, pts) : vel_direct(v, pts);
     const std::vector<double> pan = vel_panels(v, pts);
     const std::vector<double> r2 = vel_row2(v, pts);
     const std::vector<double> fil = vel_filaments(v, pts);
@@ -448,46 +483,70 @@ void collapse_buffer(cp_vp* v) {
         }
         str[i] = s;
     }
-    const bool bh = v->o.accelerate && v->ts > 3;
-    std::vector<double> pos((size_t)3 * m);
-    if (bh) {  // the tree does not hold this step's new particles: one batch
-        std::vector<double> P((size_t)3 * m);
-        for (int i = 0; i < m; i++) {
-            P[3 * i] = W.panels[i].center.x;
-            P[3 * i + 1] = W.panels[i].center.y;
-            P[3 * i + 2] = W.panels[i].center.z;
+    // seeds: the panel's centre, or with q > 1 particles per panel the
+    // points (k + 0.5) / q along its span (the mean of its two spanwise
+    // edges, pointsInOrder 0 -> 1 and 3 -> 2), each carrying 1/q of the
+    // panel's edge strength
+    const int q = v->q;
+    std::vector<double> seed((size_t)3 * m * q);
+    for (int i = 0; i < m; i++) {
+        const Vec3 c = W.panels[i].center;
+        const std::array<Vec3, 4>& p = W.pio[i];
+        const double sx = 0.5 * ((p[1].x - p[0].x) + (p[2].x - p[3].x));
+        const double sy = 0.5 * ((p[1].y - p[0].y) + (p[2].y - p[3].y));
+        const double sz = 0.5 * ((p[1].z - p[0].z) + (p[2].z - p[3].z));
+        for (int k = 0; k < q; k++) {
+            double* o = &seed[3 * ((size_t)i * q + k)];
+            const double u = q == 1 ? 0.0 : (k + 0.5) / q - 0.5;
+            o[0] = q == 1 ? c.x : c.x + u * sx;
+            o[1] = q == 1 ? c.y : c.y + u * sy;
+            o[2] = q == 1 ? c.z : c.z + u * sz;
         }
-        pos = rk4(v, P, true);
     }
+    const bool bh = v->o.accelerate && v->ts > 3;
+    std::vector<double> pos;
+    if (bh) pos = rk4(v, seed, true);  // the tree does not hold this step's new particles: one batch
     const double dt = v->dt;
     for (int i = 0; i < m; i++) {
-        if (!bh) {  // direct sums see the particles added so far
-            std::vector<double> P{W.panels[i].center.x, W.panels[i].center.y, W.panels[i].center.z};
-            std::vector<double> x = rk4(v, P, false);
-            for (int q = 0; q < 3; q++) pos[3 * i + q] = x[q];
+        double radius = v->core;
+        if (!v->alg) {
+            // wakePanel::getPartRadius (wakePanel.cpp:228-249)
+            const Vec3 c = W.panels[i].center;
+            const double cx = c.x + dt * v->Vb[0], cy = c.y + dt * v->Vb[1], cz = c.z + dt * v->Vb[2];
+            double sum = 0;
+            int cnt = 0;
+            for (int k = 0; k < 2; k++) {
+                const int o = W.rad_nbr[i][k];
+                if (o < 0) continue;
+                const Vec3 d = W.panels[o].center;
+                const double dx = cx - (d.x + dt * v->Vb[0]), dy = cy - (d.y + dt * v->Vb[1]),
+                             dz = cz - (d.z + dt * v->Vb[2]);
+                sum += std::fabs(std::sqrt(dx * dx + dy * dy + dz * dz));
+                cnt++;
+            }
+            radius = sum / (double)cnt;
         }
-        // wakePanel::getPartRadius (wakePanel.cpp:228-249)
-        const Vec3 c = W.panels[i].center;
-        const double cx = c.x + dt * v->Vb[0], cy = c.y + dt * v->Vb[1], cz = c.z + dt * v->Vb[2];
-        double sum = 0;
-        int cnt = 0;
-        for (int k = 0; k < 2; k++) {
-            const int o = W.rad_nbr[i][k];
-            if (o < 0) continue;
-            const Vec3 d = W.panels[o].center;
-            const double dx = cx - (d.x + dt * v->Vb[0]), dy = cy - (d.y + dt * v->Vb[1]),
-                         dz = cz - (d.z + dt * v->Vb[2]);
-            sum += std::fabs(std::sqrt(dx * dx + dy * dy + dz * dz));
-            cnt++;
+        for (int k = 0; k < q; k++) {
+            const double* sd = &seed[3 * ((size_t)i * q + k)];
+            double x[3];
+            if (bh) {
+                for (int a = 0; a < 3; a++) x[a] = pos[3 * ((size_t)i * q + k) + a];
+            } else {  // direct sums see the particles added so far
+                const std::vector<double> r = rk4(v, std::vector<double>{sd[0], sd[1], sd[2]}, false);
+                for (int a = 0; a < 3; a++) x[a] = r[a];
+            }
+            v->ppos.insert(v->ppos.end(), {x[0], x[1], x[2]});
+            if (q == 1)
+                v->pstr.insert(v->pstr.end(), {str[i].x, str[i].y, str[i].z});
+            else
+                v->pstr.insert(v->pstr.end(), {str[i].x / q, str[i].y / q, str[i].z / q});
+            v->prad.push_back(radius);
+            v->pvel0.insert(v->pvel0.end(), {0.0, 0.0, 0.0});
+            v->pdst0.insert(v->pdst0.end(), {0.0, 0.0, 0.0});
+            v->pshed.push_back(v->ts);
+            v->pparent.push_back(i);
+            v->pseed.insert(v->pseed.end(), {sd[0], sd[1], sd[2]});
         }
-        const double radius = sum / (double)cnt;
-        v->ppos.insert(v->ppos.end(), {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]});
-        v->pstr.insert(v->pstr.end(), {str[i].x, str[i].y, str[i].z});
-        v->prad.push_back(radius);
-        v->pvel0.insert(v->pvel0.end(), {0.0, 0.0, 0.0});
-        v->pdst0.insert(v->pdst0.end(), {0.0, 0.0, 0.0});
-        v->pshed.push_back(v->ts);
-        v->pparent.push_back(i);
     }
     if (!v->have_fil) {  // vortexFil from pointsInOrder 2 -> 3
         v->fil_ends.resize((size_t)6 * m);
@@ -524,13 +583,14 @@ void strength_update(cp_vp* v) {
         if (ax > c12) (accel ? need_bh : need_direct) = true;
         else (bh_normal ? need_bh : need_direct) = true;
     }
+    const bool diffusion = !v->alg;  // the algebraic kernel carries no diffusion term
     if (need_bh) {
         bh_st = bh_on_particles(v, CP_BH_STRETCH);
-        bh_df = bh_on_particles(v, CP_BH_DIFF);
+        if (diffusion) bh_df = bh_on_particles(v, CP_BH_DIFF);
     }
     if (need_direct) {
         d_st = pairs_direct(v, CP_BH_STRETCH);
-        d_df = pairs_direct(v, CP_BH_DIFF);
+        if (diffusion) d_df = pairs_direct(v, CP_BH_DIFF);
     }
     const std::vector<double> pst = panel_stretch(v);
     std::vector<double> upd((size_t)3 * np, 0.0);
@@ -542,7 +602,7 @@ void strength_update(cp_vp* v) {
         const std::vector<double>& st = use_bh ? bh_st : d_st;
         const std::vector<double>& df = use_bh ? bh_df : d_df;
         for (int q = 0; q < 3; q++) {
-            double diff = df[3 * i + q];
+            double diff = diffusion ? df[3 * i + q] : 0.0;
             if (!far1) diff += pst[3 * i + q];
             upd[3 * i + q] = diff + st[3 * i + q];
         }
@@ -572,7 +632,7 @@ void sources_from_particles(cp_vp* v) {
         ctr[3 * i + 1] = G.body[i].center.y;
         ctr[3 * i + 2] = G.body[i].center.z;
     }
-    const std::vector<double> part = v->o.accelerate ? vel_bh_points(v, ctr) : vel_gauss(v, ctr);
+    const std::vector<double> part = v->o.accelerate ? vel_bh_points(v, ctr) : vel_direct(v, ctr);
     const std::vector<double> fil = vel_filaments(v, ctr);
     v->vpot.resize((size_t)3 * n);
     for (int i = 0; i < n; i++) {
@@ -605,7 +665,6 @@ double wake_strength(const cp_vp* v, const WakeInfo& W, double y) {
 // CDi / PG^2 (cpCaseVP::trefftzPlaneAnalysisVP, cpCaseVP.cpp:418-429)
 void trefftz_vp(cp_vp* v) {
     const Geometry& G = v->g->host;
-    const VpBuffer& W = G.w2;
     int steps = v->ts;
     if (steps % 2 != 0) steps++;
     const int row = steps / 2 + 2;
@@ -614,12 +673,18 @@ void trefftz_vp(cp_vp* v) {
         std::vector<int> un;
         for (int i = 0; i < (int)v->pshed.size(); i++)
             if (v->pshed[i] == row) un.push_back(i);
-        std::vector<int> sp;  // by the shedding panel's centre y, first minimum
+        // the shedding position of particle k (the reference's shedding
+        // panel centre)
+        auto seed = [&](int k) {
+            const double* p = &v->pseed[3 * (size_t)k];
+            return V3{p[0], p[1], p[2]};
+        };
+        std::vector<int> sp;  // by the shedding position's y, first minimum
         while (!un.empty()) {
             size_t lo = 0;
-            double ly = W.panels[v->pparent[un[0]]].center.y;
+            double ly = seed(un[0]).y;
             for (size_t i = 0; i < un.size(); i++) {
-                const double y = W.panels[v->pparent[un[i]]].center.y;
+                const double y = seed(un[i]).y;
                 if (y < ly) {
                     ly = y;
                     lo = i;
@@ -673,12 +738,13 @@ void trefftz_vp(cp_vp* v) {
                 pts[3 * i + 1] = S[i].y;
                 pts[3 * i + 2] = S[i].z;
             }
-            const std::vector<double> pv = vel_gauss(v, pts);
+            const std::vector<double> pv = vel_direct(v, pts);
             std::vector<double> Cl(nP + 1, 0.0), Cd(nP + 1, 0.0);
             for (size_t i = 1; i < S.size(); i++) {
                 const int a = sp[S1[i]], b = sp[S2[i]];
                 const V3 pa = P(S1[i]), pb = P(S2[i]);
-                const double ya = W.panels[v->pparent[a]].center.y, yb = W.panels[v->pparent[b]].center.y;
+                const V3 ca = seed(a), cb = seed(b);
+                const double ya = ca.y, yb = cb.y;
                 // wake::particlePntInWakeY
                 const double dd = dist(pa, pb), da = dist(pa, S[i]), db = dist(S[i], pb);
                 double y;
@@ -686,7 +752,6 @@ void trefftz_vp(cp_vp* v) {
                 else if (db < 1e-10) y = yb;
                 else y = ya + da / dd * (yb - ya);
                 // wake::stretchFactor
-                const Vec3 ca = W.panels[v->pparent[a]].center, cb = W.panels[v->pparent[b]].center;
                 const double dshed = std::sqrt((cb.x - ca.x) * (cb.x - ca.x) + (cb.y - ca.y) * (cb.y - ca.y) +
                                                (cb.z - ca.z) * (cb.z - ca.z));
                 const double st = dshed / dd;
@@ -749,6 +814,7 @@ cp_status cp_vp_buffer_rows(const cp_geom* g, int32_t* m, double* dt, double* ro
 cp_status cp_vp_create(cp_geom* g, const cp_vp_opts* o, cp_vp** out) {
     if (!g || !o || !out || !g->host.vp || !(o->fs.V > 0)) return CP_BAD_ARG;
     if (o->prec != CP_FP64 && o->prec != CP_FP32_REFINE && o->prec != CP_FP32_REFINE64) return CP_BAD_ARG;
+    if (o->kernel != CP_VP_REFERENCE && o->kernel != CP_VP_ALGEBRAIC) return CP_BAD_ARG;
     *out = nullptr;
     cp_ctx* c = g->ctx;
     if (c->device < 0) return fail(c, CP_NO_DEVICE, "host-only context");
@@ -760,6 +826,17 @@ cp_status cp_vp_create(cp_geom* g, const cp_vp_opts* o, cp_vp** out) {
     v->dt = g->host.w2.dt;
     v->Vmag = o->fs.V;
     v->PG = std::sqrt(1 - o->mach * o->mach);
+    v->alg = o->kernel == CP_VP_ALGEBRAIC;
+    v->f32 = o->fp32 != 0;
+    v->q = std::max(1, (int)o->per_edge);
+    if (v->alg) {
+        v->core = o->core;
+        if (!(v->core > 0)) {  // 0.1 x the mean panel size
+            double sum = 0;
+            for (const PanelGeom& P : g->host.body) sum += std::sqrt(P.area);
+            v->core = 0.1 * sum / (double)std::max<size_t>(1, g->host.body.size());
+        }
+    }
     double T[9];
     wind_to_body(o->fs, T, v->Vb);
     v->mu.assign(v->n, 0.0);

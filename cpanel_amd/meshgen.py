@@ -175,11 +175,13 @@ def config_c2():
     return wing("0012", 1.0, 10.0, 160, 80, "cosine", 30.0)
 
 
-def config_c4():
+def config_c4(with_wake=True):
     """Swept (20 deg at the quarter chord), tapered (0.5) NACA 4412, span 8,
-    60 panels around x 50 strips (6,116 body tris); one buffer wake row of
-    one chord behind the trailing edge."""
-    return wing("4412", 1.0, 8.0, 60, 50, "uniform", 1.0, taper=0.5, sweep_deg=20.0)
+    60 panels around x 50 strips (6,116 body tris, 50 trailing-edge edges);
+    one buffer wake row of one chord behind the trailing edge, or none
+    (with_wake=False: the vortex-particle geometry, whose buffer rows follow
+    from the time step)."""
+    return wing("4412", 1.0, 8.0, 60, 50, "uniform", 1.0, taper=0.5, sweep_deg=20.0, with_wake=with_wake)
 
 
 def c4_particles(mesh, steps=400, per_edge=4, dt=0.02, seed=0):

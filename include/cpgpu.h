@@ -181,10 +181,25 @@ typedef enum {
     CP_BH_DIFF = 3          /* viscous diffusion (barnesHutDiff, node.h:333-359) */
 } cp_bh_mode;
 
+/* particle kernel of a tree or a vortex-particle case */
+typedef enum {
+    CP_VP_REFERENCE = 0,    /* the reference's kernels: Gaussian particle-particle
+                               velocity, stretching and diffusion with per-particle
+                               radii, algebraic velocity at points (particle.cpp:31-140) */
+    CP_VP_ALGEBRAIC = 1     /* one fixed core for everything: velInflAlgSmooth's
+                               algebraic kernel (particle.cpp:31-38) with core radius
+                               `core` for velocity at points and on particles, and its
+                               (alpha . grad) u for stretching; no diffusion term */
+} cp_vp_kernel;
+
 typedef struct {
     int32_t max_members;    /* leaf capacity (<=0: 10, cpCaseVP.cpp:79) */
-    int32_t pad;
+    int32_t kernel;         /* cp_vp_kernel (0: the reference's) */
     double theta;           /* opening angle s/d (<0: 0.5, octree.h:37) */
+    double core;            /* CP_VP_ALGEBRAIC: core radius (> 0) */
+    int32_t fp32;           /* CP_VP_ALGEBRAIC: 1 = pair arithmetic in FP32 (pair
+                               offsets formed in FP64), 0 = FP64 */
+    int32_t pad;
 } cp_bh_opts;
 
 /* Builds the particle octree and its pseudo-particles (octree::addData,
@@ -197,7 +212,8 @@ cp_status cp_bh_build(cp_ctx* ctx, int32_t n, const double* xyz, const double* a
 cp_status cp_bh_info(const cp_tree* t, int32_t* n_nodes, int32_t* levels, double* origin, double* half);
 /* Barnes-Hut sums from the root's children (root excluded,
  * particleFMM.cpp:65-116).  CP_BH_VEL_POINTS: m points pts[3m] -> out[3m];
- * the other modes: the tree's own n particles -> out[3n] (pts ignored). */
+ * the other modes: the tree's own n particles -> out[3n] (pts ignored).
+ * A CP_VP_ALGEBRAIC tree has no CP_BH_DIFF (CP_BAD_ARG). */
 cp_status cp_bh_eval(cp_tree* t, cp_bh_mode mode, int32_t m, const double* pts, double* out);
 void cp_bh_release(cp_tree* t);
 
@@ -205,17 +221,20 @@ void cp_bh_release(cp_tree* t);
  *      wakePanel::panelV, wakePanel.cpp:89-92) ------------------------------
  * vel[3m] = sum over body panels of sigma_j vSrc_j + mu_j vDub_j plus wake
  * panels mu_w vDub_w at the points pts[3m]; sigma, mu: n_body (solve order),
- * mu_wake: n_wake or NULL (zero). */
+ * mu_wake: n_wake or NULL (zero).  fp32 = 1: the far-field pairs (the
+ * point source and doublet of bodyPanel.cpp:305-348 beyond 5 longSide) in
+ * FP32 arithmetic, the near field in FP64; 0: everything FP64. */
 cp_status cp_panel_to_points(cp_geom* g, const double* sigma, const double* mu, const double* mu_wake, int32_t m,
-                             const double* pts, double* vel);
+                             const double* pts, double* vel, int32_t fp32);
 
 /* ---- panel -> particle stretching (bodyPanel::partStretching,
  *      bodyPanel.cpp:697-729; wakePanel::partStretching, wakePanel.cpp:266-289)
  * out[3m] = sum over body panels (source sigma, doublet mu) and wake panels
  * (doublet mu_wake, NULL = zero) of the velocity-gradient matrix at each
- * particle applied to its strength: pos, alpha: 3 per particle. */
+ * particle applied to its strength: pos, alpha: 3 per particle.  fp32 as for
+ * cp_panel_to_points (far-field point source and doublet gradients). */
 cp_status cp_panel_stretching(cp_geom* g, const double* sigma, const double* mu, const double* mu_wake, int32_t m,
-                              const double* pos, const double* alpha, double* out);
+                              const double* pos, const double* alpha, double* out, int32_t fp32);
 
 /* ---- vortex-particle solve (cpCaseVP::solveVPmatrixEq, cpCaseVP.cpp:265-301)
  * The VP case's second buffer-wake row: m panels (nv[j] = 3 or 4 vertices,
@@ -257,6 +276,19 @@ typedef struct {
     int32_t max_members;    /* tree leaf capacity (<= 0: 10, cpCaseVP.cpp:79) */
     double theta;           /* opening angle (< 0: 0.5) */
     double tol;             /* solve tolerance (<= 0: 1e-12) */
+    /* the configured form of the case (0 everywhere: the reference's) */
+    int32_t kernel;         /* cp_vp_kernel of the particle sums */
+    int32_t per_edge;       /* particles shed per second-row panel and step
+                               (<= 1: one, cpCaseVP.cpp:319-341); q > 1 splits the
+                               panel's edge strength into q equal particles seeded
+                               at (k + 0.5) / q along its span */
+    double core;            /* CP_VP_ALGEBRAIC core radius (<= 0: 0.1 x the mean
+                               sqrt(area) of the body panels) */
+    int32_t fp32;           /* 1: far-field panel -> particle sums (velocity,
+                               stretching) and, with CP_VP_ALGEBRAIC, the particle
+                               sums in FP32 arithmetic; solves, near field and
+                               forces stay FP64 */
+    int32_t pad;
 } cp_vp_opts;
 
 typedef struct {

@@ -3,6 +3,7 @@ the sanitizer runtime preloaded (tests/test_replay.py).
 
     python tests/replay/run_case.py LIB assembly
     python tests/replay/run_case.py LIB vp STEPS
+    python tests/replay/run_case.py LIB vpcfg STEPS
     python tests/replay/run_case.py LIB teardown
 
 Each prints "ok ..." lines; any sanitizer report aborts the process.
@@ -79,6 +80,50 @@ def vp(steps):
     print("ok vp finish", flush=True)
 
 
+def vp_config(steps):
+    """The configured form of the vortex-particle case (algebraic fixed-core
+    kernel, two particles per second-row panel, FP64 and FP32 sums) on a
+    small swept, tapered wing; after the steps, the Barnes-Hut sums of a
+    theta = 0 tree on the case's particles against the oracle's direct
+    sums."""
+    from oracle import vp_oracle as VO
+    m = meshgen.wing("4412", 1.0, 4.0, 16, 6, "uniform", 1.0, taper=0.5, sweep_deg=20.0, with_wake=False)
+    c = Context(0)
+    g = c.geometry_vp(m, V=1.0, dt=0.05)
+    out = {}
+    for f32 in (False, True):
+        case = g.vp_case((1.0, 6.0, 0.0), prec=CP_FP64, mach=0.1, Sref=3, bref=4, cref=0.75, accelerate=True,
+                         warm_start=True, max_members=32, theta=0.5, tol=1e-12, kernel=abi.CP_VP_ALGEBRAIC,
+                         per_edge=2, fp32=f32)
+        for k in range(1, steps + 1):
+            s = case.step()
+            assert s["rel_residual"] <= 1e-12, (k, s)
+        p = case.particles()
+        assert len(p["radius"]) == 2 * 6 * (steps - 2), len(p["radius"])
+        out[f32] = (case.state()["mu"], p)
+        case.close()
+    mu64, p64 = out[False]
+    mu32, p32 = out[True]
+    dmu = rel(mu32, mu64)
+    dpos = float(np.max(np.abs(p32["pos"] - p64["pos"])))
+    assert dmu < 1e-4 and dpos < 1e-4, (dmu, dpos)
+    core = float(p64["radius"][0])
+    x, a = p64["pos"], p64["strength"]
+    for f32, tol in ((False, 1e-12), (True, 2e-5)):
+        t = c.bh_tree(x, a, p64["radius"], max_members=32, theta=0.0, kernel=abi.CP_VP_ALGEBRAIC, core=core,
+                      fp32=f32)
+        for mode, omode in ((abi.CP_BH_VEL, VO.CORE_VEL), (abi.CP_BH_STRETCH, VO.CORE_STRETCH)):
+            d = rel(t.eval(mode), VO.direct_core(omode, core, x, a))
+            assert d <= tol, (f32, mode, d)
+        pts = x[:7] + 0.01
+        d = rel(t.eval(abi.CP_BH_VEL_POINTS, pts), VO.direct_core(VO.CORE_VEL_POINTS, core, x, a, pts))
+        assert d <= tol, (f32, d)
+        t.close()
+    print("ok vp config n=%d dmu %.1e dpos %.1e" % (len(x), dmu, dpos), flush=True)
+    g.release()
+    c.close()
+
+
 def teardown():
     """Parents closed before their children: the context before a geometry,
     a tree and a vortex-particle case, the geometry before its case."""
@@ -110,5 +155,7 @@ if __name__ == "__main__":
         assembly()
     elif what == "vp":
         vp(int(sys.argv[3]))
+    elif what == "vpcfg":
+        vp_config(int(sys.argv[3]))
     elif what == "teardown":
         teardown()

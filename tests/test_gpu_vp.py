@@ -122,6 +122,12 @@ def test_panel_to_points_matches_oracle(ctx):
     got0 = g.panel_to_points(sigma, mu, pts)
     ref0 = V.panels_to_points(bv, sigma, mu, None, None, pts)
     assert close(got0, ref0, 1e-10)[0]
+    # far-field pairs in FP32 (near field FP64): a pair's point source and
+    # doublet carry a few FP32 roundings (~1e-6 relative), summed over up to
+    # ~1,100 far panels of both signs
+    ok, rel32 = close(g.panel_to_points(sigma, mu, pts, mu_wake=muw, fp32=True), ref, 1e-5)
+    say("[gpu-vp]   fp32 far field: max rel err %.2e" % rel32)
+    assert ok, rel32
 
 
 @pytest.fixture(scope="module")
@@ -211,3 +217,9 @@ def test_panel_stretching_matches_oracle(ctx):
     err = np.abs(got[ok] - ref[ok]).max()
     say("[gpu-vp]   max rel err %.2e" % (err / scale))
     assert err <= 1e-8 * scale
+    # far-field gradients in FP32 (bound as for the FP32 panel velocity)
+    got32 = g.panel_stretching(sigma, mu, pos, alpha, mu_wake=muw, fp32=True)
+    assert np.array_equal(np.isnan(got32), np.isnan(ref))
+    err32 = np.abs(got32[ok] - ref[ok]).max()
+    say("[gpu-vp]   fp32 far field: max rel err %.2e" % (err32 / scale))
+    assert err32 <= 1e-5 * scale

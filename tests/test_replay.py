@@ -11,6 +11,9 @@ ThreadSanitizer.
 * vp: the vortex-particle driver on the reference's steadyTest case against
   its stored steps, handles left to the interpreter (finalized together, in
   no fixed order) - the teardown that aborted.
+* vpcfg: the configured form of the case (algebraic fixed-core kernel,
+  particles split along each shedding panel, FP32 sums) on a small swept,
+  tapered wing, and theta = 0 trees of that kernel against the oracle.
 * assembly: the far and near passes (tile culling, lane-permute compaction)
   against the FP64 oracle, 1e-12 of max |A| and max |B|.
 
@@ -53,6 +56,12 @@ def test_assembly_passes_match_oracle_asan():
 def test_vp_steps_and_interpreter_teardown_asan():
     out = run("asan", "vp", "6")
     assert "ok vp step 6" in out and "ok vp finish" in out
+
+
+def test_vp_config_asan():
+    """The configured form (algebraic core, two particles per panel, FP64 and
+    FP32 sums) for 10 steps, then theta = 0 trees against the oracle."""
+    assert "ok vp config" in run("asan", "vpcfg", "10")
 
 
 def test_vp_steps_tsan():
