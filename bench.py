@@ -78,11 +78,68 @@ def profiled_traffic(workload, kernel):
     return None if k is None else k["read_bytes"] + k["write_bytes"]
 
 
-def vp_bench(ctx, log):
+def pair_flops():
+    """Per-pair flop counts (constexpr int kName = N) of the kernels'
+    flop-count header."""
+    import re
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpanel_amd", "csrc", "flopcount.h")) as f:
+        txt = f.read()
+    return {m.group(1): int(m.group(2)) for m in re.finditer(r"constexpr int (k\w+) = (\d+);", txt)}
+
+
+def vp_case_bench(ctx, log, steps, faithful=False):
+    """The unsteady vortex-particle case on the swept tapered 4412 wing
+    (6,116 tris, 50 trailing-edge edges) from the impulsive start at alpha
+    6 deg, dt 0.02: the configured form (algebraic core 0.2, 4 particles per
+    edge per step, 32 per leaf, FP32 particle and far-field panel sums) or the reference's (Gaussian kernels with
+    diffusion, one particle per edge per step, 10 per leaf, FP64).  Warm
+    starts, solves refined to 1e-12 against the FP64 matrices."""
+    from cpanel_amd import CP_FP32_REFINE64, abi, meshgen
+    g = ctx.geometry_vp(meshgen.config_c4(with_wake=False), V=1.0, dt=0.02)
+    # core 0.2: the specified 0.1 x mean panel size (0.0043) leaves the
+    # particles without overlap and the run diverges (FP64 and FP32) within
+    # 40-70 steps; 0.2 is the smallest core probed that stays bounded over
+    # 400 steps
+    kw = dict(max_members=10) if faithful else dict(kernel=abi.CP_VP_ALGEBRAIC, per_edge=4, fp32=True,
+                                                    max_members=32, core=0.2)
+    case = g.vp_case((1.0, 6.0, 0.0), prec=CP_FP32_REFINE64, mach=0.1, Sref=6.0, bref=8.0, cref=0.75,
+                     accelerate=True, warm_start=True, theta=0.5, tol=1e-12, **kw)
+    names = ("bh_eval", "panel_to_points", "panel_stretching")
+    ctx.timing_reset()
+    infos = []
+    t0 = time.perf_counter()
+    for k in range(steps):
+        infos.append(case.step())
+        if (k + 1) % 50 == 0:
+            i = infos[-1]
+            log("vp case (%s): step %d, %d particles, %.1f ms, CL_trefftz %.5f" %
+                ("reference" if faithful else "configured", i["step"], i["n_particles"], i["ms"], i["CL"]))
+    wall = time.perf_counter() - t0
+    fin = case.finish()
+    phases = {nm: ctx.timing(nm)[0] for nm in names}
+    p = case.particles()
+    case.close()
+    g.release()
+    last = infos[-10:]
+    return {"form": "reference" if faithful else "configured", "core": None if faithful else 0.2,
+            "steps": steps, "wall_s": wall,
+            "ms_per_step_mean": 1e3 * wall / steps, "ms_per_step_last10": float(np.mean([i["ms"] for i in last])),
+            "particles": infos[-1]["n_particles"], "CL_trefftz": infos[-1]["CL"], "CDi_trefftz": infos[-1]["CDi"],
+            "CL": fin["CL"], "CDi": fin["CDi"], "max_rel_residual": max(i["rel_residual"] for i in infos),
+            "solve_iters_last": infos[-1]["iters"], "finite": bool(np.isfinite(p["pos"]).all() and
+                                                                     np.isfinite(p["strength"]).all()),
+            "phase_ms_total": phases}
+
+
+def vp_bench(ctx, log, case_steps=0, faithful_steps=0):
     """Vortex-particle kernels on the swept tapered 4412 case: 80,000
     particles (4 per trailing-edge edge per step over 400 steps) and 6,116
-    body panels.  Barnes-Hut (theta 0.5, 10 per leaf) in each mode, and the
-    panel -> particle velocity of every particle against every panel."""
+    body panels.  Barnes-Hut (theta 0.5, 10 per leaf) in each mode, the
+    configured form's algebraic-core tree (32 per leaf) in FP64 and FP32,
+    and the panel -> particle velocity and stretching of every particle
+    against every panel in FP64 and with the far field in FP32; then the
+    case itself grown over case_steps (configured) and faithful_steps
+    (reference form) steps."""
     from cpanel_amd import abi, meshgen
     mesh = meshgen.config_c4()
     X, A, R = meshgen.c4_particles(mesh)
@@ -99,6 +156,19 @@ def vp_bench(ctx, log):
             tree.eval(mode)
         ms, n = ctx.timing("bh_eval")
         out["bh_%s_ms" % nm] = ms / max(n, 1)
+    tree.close()
+    for f32 in (False, True):
+        tree = ctx.bh_tree(X, A, R, max_members=32, theta=0.5, kernel=abi.CP_VP_ALGEBRAIC, core=float(R[0]),
+                           fp32=f32)
+        for mode, nm in ((abi.CP_BH_VEL, "vel"), (abi.CP_BH_STRETCH, "stretch")):
+            tree.eval(mode)
+            ctx.timing_reset()
+            for _ in range(3):
+                tree.eval(mode)
+            ms, n = ctx.timing("bh_eval")
+            out["alg_core_%s_%s_ms" % (nm, "fp32" if f32 else "fp64")] = ms / max(n, 1)
+        tree.close()
+    fl = pair_flops()
     g = ctx.geometry(mesh)
     rng = np.random.default_rng(1)
     sig, mu = rng.normal(size=g.nb), rng.normal(size=g.nb)
@@ -114,7 +184,36 @@ def vp_bench(ctx, log):
     ms, n = ctx.timing("panel_stretching")
     out["panel_stretching_ms"] = ms
     out["panel_stretching_gpairs_per_s"] = len(R) * (g.nb + g.nw) / (ms * 1e-3) / 1e9
+    # far field in FP32 (near field FP64); the flop rate counts every pair
+    # at the far-field count, a lower bound (near pairs cost ~13x more)
+    muw = rng.normal(size=g.nw)
+    g.panel_to_points(sig, mu, X[:1000], fp32=True)
+    ctx.timing_reset()
+    g.panel_to_points(sig, mu, X, mu_wake=muw, fp32=True)
+    ms, n = ctx.timing("panel_to_points")
+    pairs = len(R) * (g.nb + g.nw)
+    out["panel_to_points_fp32_ms"] = ms
+    out["panel_to_points_fp32_gpairs_per_s"] = pairs / (ms * 1e-3) / 1e9
+    tf = pairs * fl["kP2PFarPairF32"] / (ms * 1e-3) / 1e12
+    out["panel_to_points_fp32_tflops_lower_bound"] = tf
+    out["panel_to_points_fp32_frac_of_fp32_valu_peak"] = tf / MI355X_FP32_MFMA_TFLOPS
+    tf64 = pairs * fl["kP2PFarPair"] / (out["panel_to_points_ms"] * 1e-3) / 1e12
+    out["panel_to_points_fp64_tflops_lower_bound"] = tf64
+    out["panel_to_points_fp64_frac_of_fp64_valu_peak"] = tf64 / MI355X_FP64_TFLOPS
+    g.panel_stretching(sig, mu, X[:1000], A[:1000], fp32=True)
+    ctx.timing_reset()
+    g.panel_stretching(sig, mu, X, A, mu_wake=muw, fp32=True)
+    ms, n = ctx.timing("panel_stretching")
+    out["panel_stretching_fp32_ms"] = ms
+    out["panel_stretching_fp32_gpairs_per_s"] = pairs / (ms * 1e-3) / 1e9
+    g.release()
     log("vp: %s" % json.dumps(out))
+    if case_steps > 0:
+        out["case_configured"] = vp_case_bench(ctx, log, case_steps)
+        log("vp case: %s" % json.dumps(out["case_configured"]))
+    if faithful_steps > 0:
+        out["case_reference_form"] = vp_case_bench(ctx, log, faithful_steps, faithful=True)
+        log("vp case: %s" % json.dumps(out["case_reference_form"]))
     return out
 
 
@@ -248,6 +347,10 @@ def main():
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-seed", type=int, default=0)
     ap.add_argument("--no-vp", action="store_true", help="skip the vortex-particle kernels")
+    ap.add_argument("--vp-steps", type=int, default=400,
+                    help="steps of the configured vortex-particle case (0: skip)")
+    ap.add_argument("--vp-reference-steps", type=int, default=100,
+                    help="steps of the vortex-particle case in the reference's form (0: skip)")
     ap.add_argument("--no-variants", action="store_true",
                     help="skip the amortised-setup and FP32-operand measurements")
     ap.add_argument("--copy-calib", action="store_true",
@@ -434,7 +537,7 @@ def main():
             del os.environ["CPGPU_NO_BF16X2"]
             del os.environ["CPGPU_NO_BF16X3"]
 
-    vp = None if args.no_vp else vp_bench(ctx, log)
+    vp = None if args.no_vp else vp_bench(ctx, log, args.vp_steps, args.vp_reference_steps)
     srv = serve_bench(ctx, args.serve_requests, args.serve_qps, args.serve_slots, args.serve_seed, log) \
         if args.serve_requests > 0 else None
 

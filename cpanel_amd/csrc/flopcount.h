@@ -66,5 +66,21 @@ constexpr int kStretchPairTrans = 4;
 constexpr int kDiffPair = 50;
 constexpr int kDiffPairTrans = 3;
 
+// ---- the configured case's forms (vp.hip), FP32 or FP64 arithmetic
+// algebraic fixed-core velocity (kc_vel): offset 3, r^2 5, q 1, r^2 + 2.5
+// s^2 2, q^2 sqrt(q) 2 + sqrt, divide 1, constant 1, cross product 9,
+// scale 3, accumulation 3
+constexpr int kAlgCorePair = 31;
+constexpr int kAlgCorePairTrans = 1;
+// its stretching (kc_stretch): offset 3, r^2 5, q 1, sqrt, f 6, h 8,
+// (alpha_t . x) h 6, two cross products with their scales and the sum and
+// accumulation 30
+constexpr int kStretchCorePair = 60;
+constexpr int kStretchCorePairTrans = 1;
+// far panel -> point velocity in FP32 (p2p_far_f32): offset 3, r^2 5,
+// rsqrt, 1/r^2 1, area / r^3 2, 3 (x . n) / r^2 7, doublet 13, source 8
+constexpr int kP2PFarPairF32 = 40;
+constexpr int kP2PFarPairF32Trans = 1;
+
 }  // namespace flops
 }  // namespace cpgpu
