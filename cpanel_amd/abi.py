@@ -81,7 +81,7 @@ class VpStepInfo(C.Structure):
 
 class BhOpts(C.Structure):
     _fields_ = [("max_members", C.c_int32), ("kernel", C.c_int32), ("theta", C.c_double), ("core", C.c_double),
-                ("fp32", C.c_int32), ("pad", C.c_int32)]
+                ("fp32", C.c_int32), ("host_build", C.c_int32)]
 
 
 CP_BH_VEL_POINTS, CP_BH_VEL, CP_BH_STRETCH, CP_BH_DIFF = 0, 1, 2, 3
@@ -123,6 +123,7 @@ SIGNATURES = {
     "cp_block_jacobi_read": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int32),
                                        C.POINTER(C.c_int32), C.POINTER(C.c_float)]),
     "cp_bh_build": (C.c_int, [C.c_void_p, C.c_int32, _d, _d, _d, C.POINTER(BhOpts), C.POINTER(C.c_void_p)]),
+    "cp_bh_read": (C.c_int, [C.c_void_p, _d, _i32, _i32]),
     "cp_bh_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), _d, _d]),
     "cp_bh_eval": (C.c_int, [C.c_void_p, C.c_int, C.c_int32, _d, _d]),
     "cp_bh_release": (None, [C.c_void_p]),

@@ -66,8 +66,9 @@ class Context:
         quads behind the trailing edge (time step dt; <= 0: computed)."""
         return Geometry(self, mesh, key, vp=(V, dt))
 
-    def bh_tree(self, xyz, alpha, radius, max_members=10, theta=0.5, kernel=0, core=0.0, fp32=False):
-        return BHTree(self, xyz, alpha, radius, max_members, theta, kernel, core, fp32)
+    def bh_tree(self, xyz, alpha, radius, max_members=10, theta=0.5, kernel=0, core=0.0, fp32=False,
+                host_build=False):
+        return BHTree(self, xyz, alpha, radius, max_members, theta, kernel, core, fp32, host_build)
 
     def bytes_in_use(self):
         return abi.lib().cp_bytes_in_use(self.h)
@@ -447,9 +448,11 @@ class BHTree:
     (particleFMM::build, particleFMM.cpp:22-27); eval() runs the Barnes-Hut
     sums of particleFMM::barnesHutVel/Stretch/Diff (particleFMM.cpp:65-116)."""
 
-    def __init__(self, ctx, xyz, alpha, radius, max_members=10, theta=0.5, kernel=0, core=0.0, fp32=False):
+    def __init__(self, ctx, xyz, alpha, radius, max_members=10, theta=0.5, kernel=0, core=0.0, fp32=False,
+                 host_build=False):
         """kernel=abi.CP_VP_ALGEBRAIC: the algebraic kernel of one core
-        radius `core` for every mode (no CP_BH_DIFF), in FP32 with fp32."""
+        radius `core` for every mode (no CP_BH_DIFF), in FP32 with fp32.
+        host_build: build the tree on the host (the device build's check)."""
         self.ctx = ctx
         self.xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
         self.alpha = np.ascontiguousarray(alpha, dtype=np.float64).reshape(-1, 3)
@@ -458,7 +461,7 @@ class BHTree:
         if self.xyz.shape[0] != self.n or self.alpha.shape[0] != self.n:
             raise ValueError("xyz, alpha and radius must describe the same particles")
         h = C.c_void_p()
-        o = abi.BhOpts(max_members, kernel, theta, core, int(fp32), 0)
+        o = abi.BhOpts(max_members, kernel, theta, core, int(fp32), int(host_build))
         ctx.check(abi.lib().cp_bh_build(ctx.h, self.n, _p(self.xyz), _p(self.alpha), _p(self.radius), C.byref(o),
                                         C.byref(h)), "cp_bh_build")
         self.h = h
@@ -468,6 +471,18 @@ class BHTree:
         o, h = np.zeros(3), C.c_double()
         abi.lib().cp_bh_info(self.h, C.byref(nn), C.byref(lv), _p(o), C.byref(h))
         return dict(n_nodes=nn.value, levels=lv.value, origin=o, half=np.full(3, h.value))
+
+    def read(self):
+        """The flattened tree: nodes (n_nodes x 8: pseudo-particle position,
+        cell width, strength, radius), links (n_nodes x 3: skip, first
+        member slot, member count or -1) and order (slot -> particle)."""
+        nn = self.info()["n_nodes"]
+        nodes = np.zeros((max(nn, 1), 8))
+        links = np.zeros((max(nn, 1), 3), np.int32)
+        order = np.zeros(max(self.n, 1), np.int32)
+        self.ctx.check(abi.lib().cp_bh_read(self.h, _p(nodes), _p(links, C.c_int32), _p(order, C.c_int32)),
+                       "cp_bh_read")
+        return nodes[:nn], links[:nn], order[:self.n]
 
     def eval(self, mode, pts=None):
         if mode == abi.CP_BH_VEL_POINTS:

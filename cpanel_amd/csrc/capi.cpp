@@ -1031,6 +1031,31 @@ cp_status cp_timing_get(cp_ctx* c, const char* kernel, double* total_ms, int64_t
 
 // ---------------------------------------------------------------- VP kernels
 
+namespace cpgpu {
+namespace capi {
+
+void build_tree_host(cp_ctx* c, int n, const double* xyz, const double* alpha, const double* radius,
+                     int max_members, cp_tree* t) {
+    build_bh_tree(n, xyz, alpha, radius, max_members, t->host);
+    t->dev.n = n;
+    t->dev.n_nodes = (int)t->host.nodes.size();
+    if (c->device < 0) return;  // host-only context: tree queries only
+    upload(c, t->nodes, t->host.nodes);
+    upload(c, t->mpos, t->host.mpos);
+    upload(c, t->mstr, t->host.mstr);
+    upload(c, t->mrad, t->host.mrad);
+    upload(c, t->order, t->host.order);
+    upload(c, t->slot, t->host.slot);
+    t->dev.nodes = t->nodes.as<BhNode>();
+    t->dev.mpos = t->mpos.as<double>();
+    t->dev.mstr = t->mstr.as<double>();
+    t->dev.mrad = t->mrad.as<double>();
+    t->dev.order = t->order.as<int>();
+}
+
+}  // namespace capi
+}  // namespace cpgpu
+
 cp_status cp_bh_build(cp_ctx* c, int32_t n, const double* xyz, const double* alpha, const double* radius,
                       const cp_bh_opts* opts, cp_tree** out) {
     if (!c || !out || n < 0 || (n > 0 && (!xyz || !alpha || !radius))) return CP_BAD_ARG;
@@ -1040,6 +1065,7 @@ cp_status cp_bh_build(cp_ctx* c, int32_t n, const double* xyz, const double* alp
     const int kernel = opts ? opts->kernel : CP_VP_REFERENCE;
     if (kernel != CP_VP_REFERENCE && kernel != CP_VP_ALGEBRAIC) return CP_BAD_ARG;
     if (kernel == CP_VP_ALGEBRAIC && !(opts->core > 0)) return CP_BAD_ARG;
+    const bool host_build = opts && opts->host_build != 0;
     std::unique_ptr<cp_tree> t(new cp_tree);
     t->ctx = c;
     t->theta = theta;
@@ -1049,22 +1075,21 @@ cp_status cp_bh_build(cp_ctx* c, int32_t n, const double* xyz, const double* alp
         t->fp32 = opts->fp32 != 0;
     }
     cp_status st = guarded(c, [&]() -> cp_status {
-        build_bh_tree(n, xyz, alpha, radius, maxm, t->host);
-        t->dev.n = n;
-        t->dev.n_nodes = (int)t->host.nodes.size();
-        if (c->device < 0) return CP_OK;  // host-only context: tree queries only
-        upload(c, t->nodes, t->host.nodes);
-        upload(c, t->mpos, t->host.mpos);
-        upload(c, t->mstr, t->host.mstr);
-        upload(c, t->mrad, t->host.mrad);
-        upload(c, t->order, t->host.order);
-        t->dev.n = n;
-        t->dev.n_nodes = (int)t->host.nodes.size();
-        t->dev.nodes = t->nodes.as<BhNode>();
-        t->dev.mpos = t->mpos.as<double>();
-        t->dev.mstr = t->mstr.as<double>();
-        t->dev.mrad = t->mrad.as<double>();
-        t->dev.order = t->order.as<int>();
+        if (c->device >= 0 && !host_build && n > 0) {
+            DevBuf dx, da, dr;
+            dalloc(c, dx, (size_t)3 * n * sizeof(double));
+            dalloc(c, da, (size_t)3 * n * sizeof(double));
+            dalloc(c, dr, (size_t)n * sizeof(double));
+            CP_HIP(hipMemcpyAsync(dx.p, xyz, (size_t)3 * n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+            CP_HIP(hipMemcpyAsync(da.p, alpha, (size_t)3 * n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+            CP_HIP(hipMemcpyAsync(dr.p, radius, (size_t)n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+            TimedScope ts("bh_build", c->stream);
+            const bool ok = build_tree_device(c, n, dx.as<double>(), da.as<double>(), dr.as<double>(), maxm,
+                                              c->stream, t.get());
+            CP_HIP(hipStreamSynchronize(c->stream));
+            if (ok) return CP_OK;
+        }
+        build_tree_host(c, n, xyz, alpha, radius, maxm, t.get());
         return CP_OK;
     });
     if (st == CP_OK) {
@@ -1076,12 +1101,48 @@ cp_status cp_bh_build(cp_ctx* c, int32_t n, const double* xyz, const double* alp
 
 cp_status cp_bh_info(const cp_tree* t, int32_t* n_nodes, int32_t* levels, double* origin, double* half) {
     if (!t) return CP_BAD_ARG;
-    if (n_nodes) *n_nodes = (int32_t)t->host.nodes.size();
+    if (n_nodes) *n_nodes = (int32_t)t->dev.n_nodes;
     if (levels) *levels = t->host.levels;
     if (origin)
         for (int d = 0; d < 3; d++) origin[d] = t->host.origin[d];
     if (half) *half = t->host.half;
     return CP_OK;
+}
+
+cp_status cp_bh_read(const cp_tree* t, double* nodes, int32_t* links, int32_t* order) {
+    if (!t) return CP_BAD_ARG;
+    cp_ctx* c = t->ctx;
+    const int nn = t->dev.n_nodes, n = t->dev.n;
+    return guarded(c, [&]() -> cp_status {
+        std::vector<BhNode> nd;
+        std::vector<int32_t> od;
+        if (!t->host.nodes.empty() || nn == 0) {
+            nd = t->host.nodes;
+            od = t->host.order;
+        } else {
+            nd.resize(nn);
+            od.resize(n);
+            CP_HIP(hipMemcpyAsync(nd.data(), t->nodes.p, (size_t)nn * sizeof(BhNode), hipMemcpyDeviceToHost,
+                                  c->stream));
+            CP_HIP(hipMemcpyAsync(od.data(), t->order.p, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost,
+                                  c->stream));
+            CP_HIP(hipStreamSynchronize(c->stream));
+        }
+        for (int i = 0; i < nn; i++) {
+            const BhNode& b = nd[i];
+            if (nodes) {
+                const double v[8] = {b.ex, b.ey, b.ez, b.width, b.sx, b.sy, b.sz, b.radius};
+                std::memcpy(nodes + 8 * (size_t)i, v, sizeof(v));
+            }
+            if (links) {
+                links[3 * i] = b.skip;
+                links[3 * i + 1] = b.first;
+                links[3 * i + 2] = b.count;
+            }
+        }
+        if (order) std::copy(od.begin(), od.end(), order);
+        return CP_OK;
+    });
 }
 
 cp_status cp_bh_eval(cp_tree* t, cp_bh_mode mode, int32_t m, const double* pts, double* out) {

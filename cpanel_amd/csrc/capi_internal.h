@@ -169,6 +169,7 @@ struct cp_tree {
     BhTreeHost host;
     DevBhTree dev;
     DevBuf nodes, mpos, mstr, mrad, order;
+    DevBuf slot;                   // particle index -> member slot (device)
     double theta = 0.5;
     int kernel = CP_VP_REFERENCE;  // cp_vp_kernel
     double core = 0;               // CP_VP_ALGEBRAIC core radius
@@ -179,6 +180,16 @@ namespace cpgpu {
 namespace capi {
 
 cp_status fail(cp_ctx* c, cp_status s, const std::string& msg);
+// Builds t's octree on the device from n particles in device memory (the
+// tree build_bh_tree makes on the host, node for node, in t->nodes, mpos,
+// mstr, mrad, order, slot, t->dev and t->host's levels / origin / half).
+// false: a node would split below the deepest level its keys hold (or the
+// node capacity ran out); nothing is built and the caller builds on the host.
+bool build_tree_device(cp_ctx* c, int n, const double* dxyz, const double* dalpha, const double* drad,
+                       int max_members, hipStream_t s, cp_tree* t);
+// the host build of t (build_bh_tree) uploaded; the device copy included
+void build_tree_host(cp_ctx* c, int n, const double* xyz, const double* alpha, const double* radius,
+                     int max_members, cp_tree* t);
 // context and geometry lifetimes (see cp_ctx::refs, cp_geom::refs)
 void ctx_ref(cp_ctx* c);
 void ctx_unref(cp_ctx* c);

@@ -68,9 +68,9 @@ struct cp_vp {
     bool alg = false, f32 = false;
     int q = 1;
     double core = 0;
-    // tree of the last build and each particle's slot in it
+    // tree of the last build (its slot array maps the particles of that
+    // build to their member slots)
     std::unique_ptr<cp_tree> tree;
-    std::vector<int> slot;
     double CL = 0, CD = 0;
     bool converged = false;
     DevBuf w2rec, fil_dev;
@@ -232,14 +232,16 @@ std::vector<double> bh_on_particles(cp_vp* v, int mode) {
     cp_geom* g = v->g;
     cp_ctx* c = g->ctx;
     hipStream_t s = g->stream;
-    std::vector<int> sl(np, -1);
-    for (int i = 0; i < np && i < (int)v->slot.size(); i++) sl[i] = v->slot[i];
     DevBuf dp, ds, dr, dsl, dout;
     to_dev(c, dp, v->ppos, s);
     to_dev(c, ds, v->pstr, s);
     to_dev(c, dr, v->prad, s);
+    // each particle's slot in the tree; particles shed since the build have
+    // none (-1)
     dalloc(c, dsl, (size_t)np * sizeof(int));
-    CP_HIP(hipMemcpyAsync(dsl.p, sl.data(), (size_t)np * sizeof(int), hipMemcpyHostToDevice, s));
+    const int nb = std::min(np, v->tree->dev.n);
+    if (nb > 0) CP_HIP(hipMemcpyAsync(dsl.p, v->tree->slot.p, (size_t)nb * sizeof(int), hipMemcpyDeviceToDevice, s));
+    if (np > nb) CP_HIP(hipMemsetAsync(dsl.as<int>() + nb, 0xFF, (size_t)(np - nb) * sizeof(int), s));
     dalloc(c, dout, (size_t)3 * np * sizeof(double));
     {
         TimedScope ts("bh_eval", s);
@@ -314,21 +316,18 @@ void build_tree(cp_vp* v) {
         t->fp32 = v->f32;
     }
     const int maxm = v->o.max_members > 0 ? v->o.max_members : 10;
-    build_bh_tree(np, v->ppos.data(), v->pstr.data(), v->prad.data(), maxm, t->host);
-    upload(c, t->nodes, t->host.nodes);
-    upload(c, t->mpos, t->host.mpos);
-    upload(c, t->mstr, t->host.mstr);
-    upload(c, t->mrad, t->host.mrad);
-    upload(c, t->order, t->host.order);
-    t->dev.n = np;
-    t->dev.n_nodes = (int)t->host.nodes.size();
-    t->dev.nodes = t->nodes.as<BhNode>();
-    t->dev.mpos = t->mpos.as<double>();
-    t->dev.mstr = t->mstr.as<double>();
-    t->dev.mrad = t->mrad.as<double>();
-    t->dev.order = t->order.as<int>();
-    v->slot.assign(np, -1);
-    for (int sl = 0; sl < np; sl++) v->slot[t->host.order[sl]] = sl;
+    bool built = false;
+    if (np > 0) {  // on the device (the host only when the keys are too shallow)
+        hipStream_t s = g->stream;
+        DevBuf dp, ds, dr;
+        to_dev(c, dp, v->ppos, s);
+        to_dev(c, ds, v->pstr, s);
+        to_dev(c, dr, v->prad, s);
+        TimedScope ts("bh_build", s);
+        built = build_tree_device(c, np, dp.as<double>(), ds.as<double>(), dr.as<double>(), maxm, s, t.get());
+        CP_HIP(hipStreamSynchronize(s));
+    }
+    if (!built) build_tree_host(c, np, v->ppos.data(), v->pstr.data(), v->prad.data(), maxm, t.get());
     v->tree = std::move(t);
 }
 

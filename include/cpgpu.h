@@ -199,13 +199,17 @@ typedef struct {
     double core;            /* CP_VP_ALGEBRAIC: core radius (> 0) */
     int32_t fp32;           /* CP_VP_ALGEBRAIC: 1 = pair arithmetic in FP32 (pair
                                offsets formed in FP64), 0 = FP64 */
-    int32_t pad;
+    int32_t host_build;     /* 1: build the tree on the host (the same tree; for
+                               checking the device build); 0: on the device */
 } cp_bh_opts;
 
 /* Builds the particle octree and its pseudo-particles (octree::addData,
- * octree.h:151-186; particleFMM::computeMultExp, particleFMM.cpp:29-58) and
- * uploads it.  xyz, alpha: 3 per particle (interleaved), radius: 1 per particle.
- * On a host-only context the tree is built for cp_bh_info only. */
+ * octree.h:151-186; particleFMM::computeMultExp, particleFMM.cpp:29-58) on the
+ * device: octant keys, a radix sort, the nodes level by level.  xyz, alpha: 3
+ * per particle (interleaved), radius: 1 per particle.  Particles closer than
+ * 2^-21 of the root width in a cell that must still split exceed the keys'
+ * depth: that tree is built on the host (the same tree).  On a host-only
+ * context the tree is built on the host for cp_bh_info / cp_bh_read only. */
 cp_status cp_bh_build(cp_ctx* ctx, int32_t n, const double* xyz, const double* alpha, const double* radius,
                       const cp_bh_opts* opts, cp_tree** out);
 /* node count, deepest level (root = 1), root origin[3] and half width */
@@ -215,6 +219,12 @@ cp_status cp_bh_info(const cp_tree* t, int32_t* n_nodes, int32_t* levels, double
  * the other modes: the tree's own n particles -> out[3n] (pts ignored).
  * A CP_VP_ALGEBRAIC tree has no CP_BH_DIFF (CP_BAD_ARG). */
 cp_status cp_bh_eval(cp_tree* t, cp_bh_mode mode, int32_t m, const double* pts, double* out);
+/* Diagnostics (no counterpart in the reference): the flattened tree in
+ * pre-order, nodes: 8 per node (pseudo-particle x, y, z, cell width, strength
+ * x, y, z, radius), links: 3 per node (next node after the subtree, first
+ * member slot, member count or -1 for an internal node), order: member slot
+ * -> particle index (n); any output may be NULL. */
+cp_status cp_bh_read(const cp_tree* t, double* nodes, int32_t* links, int32_t* order);
 void cp_bh_release(cp_tree* t);
 
 /* ---- panel -> point velocity (bodyPanel::panelV, bodyPanel.cpp:122-130;

@@ -35,3 +35,13 @@ void near_tile_lists(int, int64_t, const std::vector<int>&, const std::vector<in
 void launch_bj_invert(const float*, int64_t, const int*, const int*, const int64_t*, int, int, double*, int*, int*, int*,
                       float*, hipStream_t) { nope("bj_invert"); }
 }  // namespace cpgpu
+
+// the device octree build needs hipCUB's radix sort and scan, which the
+// replay does not provide: report "not built", so the host build runs
+namespace cpgpu {
+namespace capi {
+bool build_tree_device(cp_ctx*, int, const double*, const double*, const double*, int, hipStream_t, cp_tree*) {
+    return false;
+}
+}  // namespace capi
+}  // namespace cpgpu

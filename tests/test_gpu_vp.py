@@ -49,6 +49,46 @@ def close(got, ref, tol=TOL):
     return err <= tol * scale, err / max(scale, 1e-300)
 
 
+def tree_cases():
+    rng = np.random.default_rng(11)
+    X, A, R = meshgen.c4_particles(meshgen.config_c4())
+    grid = np.stack(np.meshgrid(np.arange(12.), np.arange(12.), np.arange(12.)), -1).reshape(-1, 3)
+    Y = rng.normal(size=(3000, 3))
+    Y[:100] = Y[0] + 1e-9 * rng.normal(size=(100, 3))
+    return [("c4 80k, 10 per leaf", X, A, R, 10), ("c4 80k, 32 per leaf", X, A, R, 32),
+            ("random 5000", rng.normal(size=(5000, 3)), rng.normal(size=(5000, 3)), np.full(5000, 0.1), 10),
+            ("grid 1728 (particles on the split planes)", grid, rng.normal(size=grid.shape), np.full(len(grid), 0.2),
+             10),
+            ("7 particles (the root is a leaf)", rng.normal(size=(7, 3)), rng.normal(size=(7, 3)), np.full(7, 0.1), 10),
+            ("clustered within 1e-9 (beyond the keys: host build)", Y, rng.normal(size=Y.shape), np.full(3000, 0.1),
+             10)]
+
+
+def test_device_tree_equals_host_tree(ctx):
+    """The device build (octant keys by the host's own centre recurrence,
+    radix sort, nodes level by level, pseudo-particles bottom-up) gives the
+    host build's tree: the same nodes in the same pre-order, skip links,
+    member ranges and member order, and the same pseudo-particles (FP64 with
+    contraction off on both sides, correctly rounded sqrt and division)."""
+    for name, X, A, R, maxm in tree_cases():
+        say("[gpu-vp] device tree: %s" % name)
+        dev = ctx.bh_tree(X, A, R, max_members=maxm)
+        host = ctx.bh_tree(X, A, R, max_members=maxm, host_build=True)
+        dn, dl, do = dev.read()
+        hn, hl, ho = host.read()
+        assert dev.info()["n_nodes"] == host.info()["n_nodes"] and dev.info()["levels"] == host.info()["levels"]
+        assert np.array_equal(dl, hl) and np.array_equal(do, ho), name
+        scale = np.maximum(np.abs(hn), 1e-300)
+        rel = float(np.nanmax(np.abs(dn - hn) / scale))
+        bitwise = np.array_equal(np.isnan(dn), np.isnan(hn)) and np.array_equal(np.nan_to_num(dn), np.nan_to_num(hn))
+        say("[gpu-vp]   %d nodes, bitwise %s, max rel diff %.1e" % (len(dn), bitwise, rel))
+        assert rel <= 1e-15, (name, rel)
+        for mode in (abi.CP_BH_VEL, abi.CP_BH_STRETCH):
+            assert np.array_equal(dev.eval(mode), host.eval(mode)) or rel > 0
+        dev.close()
+        host.close()
+
+
 @pytest.mark.parametrize("theta", [0.5, 0.0])
 @pytest.mark.parametrize("mode", [abi.CP_BH_VEL_POINTS, abi.CP_BH_VEL, abi.CP_BH_STRETCH, abi.CP_BH_DIFF])
 def test_bh_matches_oracle(ctx, mode, theta):
