@@ -30,6 +30,7 @@
 // rounded once.
 #include "device.h"
 #include "timer.h"
+#include "wavequeue.h"
 
 namespace cpgpu {
 
@@ -155,37 +156,6 @@ __global__ __launch_bounds__(256) void far_f32_kernel(
             __builtin_nontemporal_store(live ? dubf : 0.0f, &A[o]);
             __builtin_nontemporal_store(live ? srcf : 0.0f, &B[o]);
         }
-    }
-}
-
-// A queue of collocation rows of one column, one row per lane in lanes
-// [0, qn): rows marked in a tile join it (compacted with lane permutes) and
-// every time 64 are queued the whole wavefront evaluates them.
-struct RowQueue {
-    int p = 0;
-    int qn = 0;  // wave-uniform
-};
-
-template <class F>
-__device__ inline void queue_push(RowQueue& q, bool take, int row, int lane, F&& eval) {
-    const uint64_t m = __ballot(take);
-    if (m == 0) return;
-    const int c = __popcll(m);
-    const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-    const int rn = __popcll(m & below);
-    // a permutation: the marked rows to lanes [0, c), the others above
-    const int dst = take ? rn : c + (lane - rn);
-    const int b = __builtin_amdgcn_ds_permute(dst << 2, row);
-    // the new rows after the queued ones (lanes [qn, qn + c), wrapping)
-    const int sh = __builtin_amdgcn_ds_bpermute(((lane - q.qn) & 63) << 2, b);
-    const int merged = lane < q.qn ? q.p : sh;
-    if (q.qn + c >= 64) {
-        eval(merged, true);
-        q.p = sh;  // lanes [0, qn + c - 64): the rows that did not fit
-        q.qn = q.qn + c - 64;
-    } else {
-        q.p = merged;
-        q.qn += c;
     }
 }
 

@@ -1211,17 +1211,16 @@ cp_status cp_panel_to_points(cp_geom* g, const double* sigma, const double* mu, 
         std::memcpy(str.data(), sigma, nb * sizeof(double));
         std::memcpy(str.data() + nb, mu, nb * sizeof(double));
         if (mu_wake && nw > 0) std::memcpy(str.data() + 2 * nb, mu_wake, nw * sizeof(double));
-        const int KS = p2p_slices(m, nb + nw);
         DevBuf dstr, dpts, dpart, dout;
         upload(c, dstr, str);
         dalloc(c, dpts, (size_t)3 * m * sizeof(double));
-        dalloc(c, dpart, (size_t)3 * m * KS * sizeof(double));
+        dalloc(c, dpart, p2p_part_doubles(m, nb + nw) * sizeof(double));
         dalloc(c, dout, (size_t)3 * m * sizeof(double));
         CP_HIP(hipMemcpyAsync(dpts.p, pts, (size_t)3 * m * sizeof(double), hipMemcpyHostToDevice, s));
         {
             TimedScope ts("panel_to_points", s);
             launch_panel_to_points(g->dev, dstr.as<double>(), dstr.as<double>() + nb, dstr.as<double>() + 2 * nb, m,
-                                   dpts.as<double>(), KS, dpart.as<double>(), dout.as<double>(), s, fp32 != 0);
+                                   dpts.as<double>(), dpart.as<double>(), dout.as<double>(), s, fp32 != 0);
         }
         CP_HIP(hipMemcpyAsync(vel, dout.p, (size_t)3 * m * sizeof(double), hipMemcpyDeviceToHost, s));
         CP_HIP(hipStreamSynchronize(s));
@@ -1372,19 +1371,18 @@ cp_status cp_panel_stretching(cp_geom* g, const double* sigma, const double* mu,
         std::memcpy(str.data(), sigma, nb * sizeof(double));
         std::memcpy(str.data() + nb, mu, nb * sizeof(double));
         if (mu_wake && nw > 0) std::memcpy(str.data() + 2 * nb, mu_wake, nw * sizeof(double));
-        const int KS = p2p_slices(m, nb + nw);
         std::vector<double> pa(6 * (size_t)m);
         std::memcpy(pa.data(), pos, 3 * (size_t)m * sizeof(double));
         std::memcpy(pa.data() + 3 * (size_t)m, alpha, 3 * (size_t)m * sizeof(double));
         DevBuf dstr, dpa, dpart, dout;
         upload(c, dstr, str);
         upload(c, dpa, pa);
-        dalloc(c, dpart, (size_t)3 * m * KS * sizeof(double));
+        dalloc(c, dpart, p2p_part_doubles(m, nb + nw) * sizeof(double));
         dalloc(c, dout, (size_t)3 * m * sizeof(double));
         {
             TimedScope ts("panel_stretching", s);
             launch_panel_stretching(g->dev, dstr.as<double>(), dstr.as<double>() + nb, dstr.as<double>() + 2 * nb, m,
-                                    dpa.as<double>(), dpa.as<double>() + 3 * (size_t)m, KS, dpart.as<double>(),
+                                    dpa.as<double>(), dpa.as<double>() + 3 * (size_t)m, dpart.as<double>(),
                                     dout.as<double>(), s, fp32 != 0);
         }
         CP_HIP(hipMemcpyAsync(out, dout.p, (size_t)3 * m * sizeof(double), hipMemcpyDeviceToHost, s));

@@ -40,8 +40,10 @@ struct alignas(256) PanelRec {
 static_assert(sizeof(PanelRec) == 256, "PanelRec must be 256 bytes");
 
 // far-field test of the reference: |pjk| / longSide > 5, decided exactly
-// from r^2 except inside a relative band of 1e-13 around the threshold.
-CP_HD bool is_far(const PanelRec& P, double r2) {
+// from r^2 except inside a relative band of 1e-13 around the threshold (R: a
+// panel record, or any record with far2_lo, far2_hi and long_side)
+template <class R>
+CP_HD bool is_far(const R& P, double r2) {
 #pragma clang fp contract(off)
     if (r2 > P.far2_hi) return true;
     if (r2 < P.far2_lo) return false;
@@ -189,7 +191,8 @@ CP_HD void vortex_v(const double* a, const double* b, const double* s, double* o
 }
 
 // panel::pntDubV (panel.cpp:773-776): area (3 (pjk.n) pjk - r^2 n) / (4 pi r^5)
-CP_HD void pnt_dub_v(const PanelRec& P, const double* pjk, double r, double* out) {
+template <class R>
+CP_HD void pnt_dub_v(const R& P, const double* pjk, double r, double* out) {
 #pragma clang fp contract(off)
     double pn3 = 3 * (pjk[0] * P.n[0] + pjk[1] * P.n[1] + pjk[2] * P.n[2]);
     double r2 = r * r;

@@ -33,7 +33,9 @@ CP_HD void to_local(const PanelRec& P, const double* p, double* q) {
 }
 
 // bodyPanel::velocityGradientPointSource (bodyPanel.cpp:733-763), added to G
-CP_HD void vg_point_source_add(const PanelRec& P, double sigma, const double* poi, Mat3& G) {
+// (R: a panel record, or any record with c and area)
+template <class R>
+CP_HD void vg_point_source_add(const R& P, double sigma, const double* poi, Mat3& G) {
 #pragma clang fp contract(off)
     double x = P.c[0], y = P.c[1], z = P.c[2], x0 = poi[0], y0 = poi[1], z0 = poi[2];
     double dx = x - x0, dy = y - y0;
@@ -106,7 +108,8 @@ CP_HD void vg_panel_source_add(const PanelRec& P, double sigma, const double* po
 }
 
 // panel::velocityGradientPointDoublet (panel.cpp:813-840), added to G
-CP_HD void vg_point_doublet_add(const PanelRec& P, double mu, const double* poi, Mat3& G) {
+template <class R>
+CP_HD void vg_point_doublet_add(const R& P, double mu, const double* poi, Mat3& G) {
 #pragma clang fp contract(off)
     double x = P.c[0], y = P.c[1], z = P.c[2], x0 = poi[0], y0 = poi[1];
     double dx2 = (x - x0) * (x - x0), dy2 = (y - y0) * (y - y0);

@@ -43,19 +43,26 @@ void launch_gauss_points(int n, const double* pos, const double* str, const doub
 void launch_filament_points(int nf, const double* ends, const double* fstr, int m, const double* pts, double* out,
                             hipStream_t s);
 
-// K slices of the panel -> point evaluation for m points and np panels
+// K slices of the far pass of the panel sums for m points and np panels
 int p2p_slices(int m, int npanels);
+// doubles of the `part` scratch the two launchers below need for m points
+// and npanels panels (far and near slices, the point tiles, the spatial
+// order and its sort)
+size_t p2p_part_doubles(int m, int npanels);
 // out[3m] = velocity at pts of the body (sigma, mu) and wake (muw) panels;
-// part: KS * 3m scratch
 // f32far: far-field pairs in FP32 arithmetic (near field FP64)
 void launch_panel_to_points(const DevGeom& g, const double* sigma, const double* mu, const double* muw, int m,
-                            const double* pts, int KS, double* part, double* out, hipStream_t s, bool f32far = false);
-
+                            const double* pts, double* part, double* out, hipStream_t s, bool f32far = false);
 // out[3m] = sum over body and wake panels of partStretching at the particles
-// (pos, alpha: 3 per particle); part: KS * 3m scratch
+// (pos, alpha: 3 per particle)
 void launch_panel_stretching(const DevGeom& g, const double* sigma, const double* mu, const double* muw, int m,
-                             const double* pos, const double* alpha, int KS, double* part, double* out,
-                             hipStream_t s, bool f32far = false);
+                             const double* pos, const double* alpha, double* part, double* out, hipStream_t s,
+                             bool f32far = false);
+// Spatial order of m points (vp_build.hip): perm[k] = the point at position
+// k of the order of their 30-bit Morton keys over the points' bounding box
+// (stable); scratch: morton_sort_bytes(m) bytes.
+size_t morton_sort_bytes(int m);
+void sort_points_morton(int m, const double* pts, void* scratch, int* perm, hipStream_t s);
 
 // The algebraic particle kernel (velInflAlgSmooth with one core radius for
 // every particle): mode CP_BH_VEL (velocity; tstr unused) or CP_BH_STRETCH

@@ -9,14 +9,16 @@
 // reference's recursion (node.h:236-421) in the same visiting order.  FP64,
 // FMA contraction off.
 //
-// K5, panel -> point velocity: one lane per point, the panels of a K slice
-// read as wave-uniform 256-byte records, FP64 (bodyPanel::panelV,
-// wakePanel::panelV); slices are summed in a fixed order.
+// K5, panel -> point velocity and stretching (bodyPanel::panelV,
+// wakePanel::panelV, partStretching): a far pass over LDS-staged panel
+// chunks and a near pass over compacted (panel, point) pairs, summed in a
+// fixed order (see "panel -> point sums in two passes" below).
 #include <hip/hip_runtime.h>
 
 #include "device.h"
 #include "stretch.h"
 #include "vp.h"
+#include "wavequeue.h"
 
 namespace cpgpu {
 
@@ -341,69 +343,6 @@ __global__ __launch_bounds__(kT) void filament_points_kernel(int nf, const doubl
     out[3 * t + 2] = acc[2];
 }
 
-// panel -> point velocity over the panels [p0, p1) of the combined list
-// (body panels, then wake panels); partial sums per K slice
-__global__ __launch_bounds__(kT) void p2p_kernel(const PanelRec* __restrict__ body, int nb,
-                                                 const PanelRec* __restrict__ wake, int nw,
-                                                 const double* __restrict__ sigma, const double* __restrict__ mu,
-                                                 const double* __restrict__ muw, int m,
-                                                 const double* __restrict__ pts, int chunk,
-                                                 double* __restrict__ part) {
-    const int t = blockIdx.x * kT + threadIdx.x;
-    const int ks = blockIdx.y;
-    const int p0 = ks * chunk, p1 = min(nb + nw, p0 + chunk);
-    double acc[3] = {0, 0, 0};
-    if (t < m) {
-        double p[3] = {pts[3 * t], pts[3 * t + 1], pts[3 * t + 2]};
-        for (int j = p0; j < p1; j++) {
-            if (j < nb) {
-                double vs[3], vd[3];
-                panel_v_inf_f64(body[j], p, vs, vd);
-                const double sg = sigma[j], mj = mu[j];
-                for (int q = 0; q < 3; q++) acc[q] += sg * vs[q] + mj * vd[q];
-            } else {
-                double vd[3];
-                dub_v_inf_f64(wake[j - nb], p, vd);
-                const double mw = muw[j - nb];
-                for (int q = 0; q < 3; q++) acc[q] += mw * vd[q];
-            }
-        }
-    }
-    if (t < m) {
-        double* o = part + ((int64_t)ks * m + t) * 3;
-        o[0] = acc[0];
-        o[1] = acc[1];
-        o[2] = acc[2];
-    }
-}
-
-// panel -> particle stretching over the panels [p0, p1): per particle the
-// velocity-gradient matrices of the slice are summed, then applied to alpha
-__global__ __launch_bounds__(kT) void stretch_kernel(const PanelRec* __restrict__ body, int nb,
-                                                     const PanelRec* __restrict__ wake, int nw,
-                                                     const double* __restrict__ sigma, const double* __restrict__ mu,
-                                                     const double* __restrict__ muw, int m,
-                                                     const double* __restrict__ pos,
-                                                     const double* __restrict__ alpha, int chunk,
-                                                     double* __restrict__ part) {
-    const int t = blockIdx.x * kT + threadIdx.x;
-    const int ks = blockIdx.y;
-    const int p0 = ks * chunk, p1 = min(nb + nw, p0 + chunk);
-    if (t >= m) return;
-    const double p[3] = {pos[3 * t], pos[3 * t + 1], pos[3 * t + 2]};
-    Mat3 G;
-    mat3_zero(G);
-    for (int j = p0; j < p1; j++) {
-        if (j < nb)
-            part_stretch_add(body[j], true, sigma[j], mu[j], p, G);
-        else
-            part_stretch_add(wake[j - nb], false, 0.0, muw[j - nb], p, G);
-    }
-    const double a0 = alpha[3 * t], a1 = alpha[3 * t + 1], a2 = alpha[3 * t + 2];
-    double* o = part + ((int64_t)ks * m + t) * 3;
-    for (int r = 0; r < 3; r++) o[r] = G.a[r][0] * a0 + G.a[r][1] * a1 + G.a[r][2] * a2;
-}
-
 // ---------------------------------------------------------------------------
 // The configured unsteady case's particle kernel: velInflAlgSmooth's
 // algebraic smoothing (particle.cpp:31-38) with one core radius for every
@@ -529,56 +468,112 @@ __global__ __launch_bounds__(kT) void dc_kernel(int n, const double* __restrict_
     out[3 * t + 2] = acc[2];
 }
 
-// ---- FP32 far field of the panel -> particle sums (FP64 near field)
-// point source (negated, bodyPanel.cpp:314) and point doublet velocity of a
-// far panel times their strengths, added to acc
-__device__ inline void p2p_far_f32(const PanelRec& P, const double* p, float sg, float mj, bool body, float* acc) {
-    const float x = (float)(p[0] - P.c[0]), y = (float)(p[1] - P.c[1]), z = (float)(p[2] - P.c[2]);
-    const float ri = rsqrtf(x * x + y * y + z * z);
-    const float ri2 = ri * ri;
-    const float ari3 = (float)P.area_4pi * ri2 * ri;  // area / (4 pi r^3)
-    const float nx = (float)P.n[0], ny = (float)P.n[1], nz = (float)P.n[2];
-    const float t = 3.0f * (x * nx + y * ny + z * nz) * ri2;
-    const float fd = mj * ari3;
-    acc[0] += fd * (t * x - nx);
-    acc[1] += fd * (t * y - ny);
-    acc[2] += fd * (t * z - nz);
-    if (body) {
-        const float fs = -sg * ari3;
-        acc[0] += fs * x;
-        acc[1] += fs * y;
-        acc[2] += fs * z;
+// ---- panel -> point sums in two passes ----------------------------------
+// Far pass: every (point, panel) pair whose panel is far (is_far: the
+// reference's |pjk| / longSide > 5) with the point source and point doublet
+// forms, branch-free over panel chunks staged in LDS (a broadcast read per
+// pair instead of a 256-byte record load), one lane per point, panels split
+// over K slices.  Near pass: one wavefront per tile of 64 points; panels
+// whose 5 longSide reach meets the tile's bounding sphere are candidates,
+// the (panel, point) pairs of a candidate that are near join a queue
+// (lane permutes), and every 64 queued pairs are evaluated by the whole
+// wavefront, one pair per lane, with the reference's near-field formulas;
+// each point then adds its pairs' results in queue order.  Both passes sum
+// in a fixed order, so results do not depend on the launch.
+
+// far-field data of one panel, staged in LDS by the far pass
+struct FarRec {
+    double c[3];
+    double n[3];
+    double area;
+    double far2_lo, far2_hi, long_side;
+    double sg, mj;  // source and doublet strength (sg = 0 on wake panels)
+};
+
+__device__ inline void load_far(const PanelRec* __restrict__ body, int nb, const PanelRec* __restrict__ wake,
+                                const double* __restrict__ sigma, const double* __restrict__ mu,
+                                const double* __restrict__ muw, int j, FarRec& F) {
+    const bool isb = j < nb;
+    const PanelRec& P = isb ? body[j] : wake[j - nb];
+    for (int k = 0; k < 3; k++) {
+        F.c[k] = P.c[k];
+        F.n[k] = P.n[k];
     }
+    F.area = P.area;
+    F.far2_lo = P.far2_lo;
+    F.far2_hi = P.far2_hi;
+    F.long_side = P.long_side;
+    F.sg = isb ? sigma[j] : 0.0;
+    F.mj = isb ? mu[j] : muw[j - nb];
 }
 
-// the point source and point doublet gradients of vg_point_source_add /
-// vg_point_doublet_add (bodyPanel.cpp:733-763, panel.cpp:813-840) in FP32,
-// with the reference's frames (the panel centre's z), added to G
-__device__ inline void vg_far_f32(const PanelRec& P, bool body, float sigma, float mu, const double* poi,
-                                  float G[3][3]) {
-    const float z = (float)P.c[2], zz0 = (float)(P.c[2] - poi[2]);
-    const float dx = (float)(P.c[0] - poi[0]), dy = (float)(P.c[1] - poi[1]);
+// far velocity of one panel (panel_v_inf_f64's far branch: point source
+// with the reference's negated sign, bodyPanel.cpp:314, and point doublet),
+// times its strengths, added to acc
+__device__ inline void far_vel_f64(const FarRec& F, const double* p, double* acc) {
+    const double pjk[3] = {p[0] - F.c[0], p[1] - F.c[1], p[2] - F.c[2]};
+    const double r2 = sq_norm3(pjk[0], pjk[1], pjk[2]);
+    if (!is_far(F, r2)) return;
+    const double r = sqrt(r2);
+    const double den = 4 * kPI * (r * r * r);
+    const double f = F.area / den;
+    double vs[3], vd[3];
+    for (int k = 0; k < 3; k++) vs[k] = -1.0 * (pjk[k] * f);
+    pnt_dub_v(F, pjk, r, vd);
+    for (int q = 0; q < 3; q++) acc[q] += F.sg * vs[q] + F.mj * vd[q];
+}
+
+// the same in FP32 arithmetic (offset and far test in FP64)
+__device__ inline void far_vel_f32(const FarRec& F, const double* p, float* acc) {
+    const double d0 = p[0] - F.c[0], d1 = p[1] - F.c[1], d2 = p[2] - F.c[2];
+    if (!is_far(F, sq_norm3(d0, d1, d2))) return;
+    const float x = (float)d0, y = (float)d1, z = (float)d2;
+    const float ri = rsqrtf(x * x + y * y + z * z);
+    const float ri2 = ri * ri;
+    const float ari3 = (float)(F.area * (1.0 / (4 * kPI))) * ri2 * ri;  // area / (4 pi r^3)
+    const float nx = (float)F.n[0], ny = (float)F.n[1], nz = (float)F.n[2];
+    const float t = 3.0f * (x * nx + y * ny + z * nz) * ri2;
+    const float fd = (float)F.mj * ari3;
+    const float fs = -(float)F.sg * ari3;
+    acc[0] += fd * (t * x - nx) + fs * x;
+    acc[1] += fd * (t * y - ny) + fs * y;
+    acc[2] += fd * (t * z - nz) + fs * z;
+}
+
+// far velocity gradient (point source and point doublet gradients of
+// bodyPanel::partStretching, with the reference's frames), added to G
+__device__ inline void far_grad_f64(const FarRec& F, const double* p, Mat3& G) {
+    const double d0 = p[0] - F.c[0], d1 = p[1] - F.c[1], d2 = p[2] - F.c[2];
+    if (!is_far(F, sq_norm3(d0, d1, d2))) return;
+    vg_point_source_add(F, F.sg, p, G);
+    vg_point_doublet_add(F, F.mj, p, G);
+}
+
+// the same in FP32 arithmetic (far test in FP64; the forms use the panel
+// centre's z as the reference writes them)
+__device__ inline void far_grad_f32(const FarRec& F, const double* poi, float G[3][3]) {
+    const double e0 = poi[0] - F.c[0], e1 = poi[1] - F.c[1], e2 = poi[2] - F.c[2];
+    if (!is_far(F, sq_norm3(e0, e1, e2))) return;
+    const float z = (float)F.c[2], zz0 = (float)(F.c[2] - poi[2]);
+    const float dx = (float)(F.c[0] - poi[0]), dy = (float)(F.c[1] - poi[1]);
     const float dx2 = dx * dx, dy2 = dy * dy;
     const float q2 = dx2 + dy2 + z * z;
     const float sq = sqrtf(q2);
-    const float area = (float)P.area;
-    const float i4p = (float)(1.0 / (4 * kPI));
-    if (body) {
-        const float isdd = 1.0f / (q2 * q2 * sq);
-        const float c = sigma * area * i4p;
-        G[0][0] += c * ((-2 * dx2 + dy2 + z * z) * isdd);
-        G[1][0] += c * (-3 * dx * dy * isdd);
-        G[2][0] += c * (-3 * dx * z * isdd);
-        G[0][1] += c * (-3 * dx * dy * isdd);
-        G[1][1] += c * ((dx2 - dy2 + z * z) * isdd);
-        G[2][1] += c * (-3 * z * dy * isdd);
-        G[0][2] += c * (-3 * dx * zz0 * isdd);
-        G[1][2] += c * (-3 * dy * zz0 * isdd);
-        G[2][2] += c * ((-3 * z * zz0 + (dx2 + dy2 + z * z)) * isdd);
-    }
+    const float a4 = (float)(F.area * (1.0 / (4 * kPI)));
+    const float isdd = 1.0f / (q2 * q2 * sq);
+    const float c = (float)F.sg * a4;
+    G[0][0] += c * ((-2 * dx2 + dy2 + z * z) * isdd);
+    G[1][0] += c * (-3 * dx * dy * isdd);
+    G[2][0] += c * (-3 * dx * z * isdd);
+    G[0][1] += c * (-3 * dx * dy * isdd);
+    G[1][1] += c * ((dx2 - dy2 + z * z) * isdd);
+    G[2][1] += c * (-3 * z * dy * isdd);
+    G[0][2] += c * (-3 * dx * zz0 * isdd);
+    G[1][2] += c * (-3 * dy * zz0 * isdd);
+    G[2][2] += c * ((-3 * z * zz0 + (dx2 + dy2 + z * z)) * isdd);
     const float iddd = 1.0f / (q2 * q2 * q2 * sq);
-    const float uv = 3 * mu * area * i4p;
-    const float w = -mu * area * i4p;
+    const float uv = 3 * (float)F.mj * a4;
+    const float w = -(float)F.mj * a4;
     G[0][0] += uv * z * (-4 * dx2 + dy2 + z * z) * iddd;
     G[1][0] += uv * (-5) * z * dy * dx * iddd;
     G[2][0] += uv * dx * (dx2 + dy2 - 4 * z * z) * iddd;
@@ -590,76 +585,200 @@ __device__ inline void vg_far_f32(const PanelRec& P, bool body, float sigma, flo
     G[2][2] += w * (-3) * z * (3 * dx2 + 3 * dy2 - 2 * z * z) * iddd;
 }
 
-// panel -> point velocity, far field in FP32 and near field in FP64 (one
-// accumulator each, added at the end)
-__global__ __launch_bounds__(kT) void p2p_f32_kernel(const PanelRec* __restrict__ body, int nb,
-                                                     const PanelRec* __restrict__ wake, int nw,
-                                                     const double* __restrict__ sigma, const double* __restrict__ mu,
-                                                     const double* __restrict__ muw, int m,
-                                                     const double* __restrict__ pts, int chunk,
-                                                     double* __restrict__ part) {
+enum { PS_VEL = 0, PS_STRETCH = 1 };
+
+// far pass over the panels of K slice blockIdx.y: part slice ks (3 per
+// point: the velocity, or the gradient applied to the particle's alpha)
+template <int MODE, bool F32>
+__global__ __launch_bounds__(kT) void panels_far_kernel(const PanelRec* __restrict__ body, int nb,
+                                                        const PanelRec* __restrict__ wake, int nw,
+                                                        const double* __restrict__ sigma,
+                                                        const double* __restrict__ mu,
+                                                        const double* __restrict__ muw, int m,
+                                                        const double* __restrict__ pts,
+                                                        const double* __restrict__ alpha, int chunk,
+                                                        double* __restrict__ part) {
+    __shared__ FarRec sh[kT];
     const int t = blockIdx.x * kT + threadIdx.x;
+    const bool live = t < m;
     const int ks = blockIdx.y;
     const int p0 = ks * chunk, p1 = min(nb + nw, p0 + chunk);
-    if (t >= m) return;
-    const double p[3] = {pts[3 * t], pts[3 * t + 1], pts[3 * t + 2]};
+    double p[3] = {0, 0, 0};
+    if (live)
+        for (int k = 0; k < 3; k++) p[k] = pts[3 * t + k];
     double acc[3] = {0, 0, 0};
     float accf[3] = {0, 0, 0};
-    for (int j = p0; j < p1; j++) {
-        const bool isb = j < nb;
-        const PanelRec& P = isb ? body[j] : wake[j - nb];
-        const double d0 = p[0] - P.c[0], d1 = p[1] - P.c[1], d2 = p[2] - P.c[2];
-        if (is_far(P, sq_norm3(d0, d1, d2))) {
-            p2p_far_f32(P, p, isb ? (float)sigma[j] : 0.0f, isb ? (float)mu[j] : (float)muw[j - nb], isb, accf);
-        } else if (isb) {
-            double vs[3], vd[3];
-            panel_v_inf_f64(P, p, vs, vd);
-            const double sg = sigma[j], mj = mu[j];
-            for (int q = 0; q < 3; q++) acc[q] += sg * vs[q] + mj * vd[q];
-        } else {
-            double vd[3];
-            dub_v_inf_f64(P, p, vd);
-            const double mw = muw[j - nb];
-            for (int q = 0; q < 3; q++) acc[q] += mw * vd[q];
+    Mat3 G;
+    mat3_zero(G);
+    float Gf[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    for (int b0 = p0; b0 < p1; b0 += kT) {
+        const int cnt = min(kT, p1 - b0);
+        __syncthreads();
+        if ((int)threadIdx.x < cnt) load_far(body, nb, wake, sigma, mu, muw, b0 + (int)threadIdx.x, sh[threadIdx.x]);
+        __syncthreads();
+        if (!live) continue;
+        for (int k = 0; k < cnt; k++) {
+            if (MODE == PS_VEL) {
+                if (F32)
+                    far_vel_f32(sh[k], p, accf);
+                else
+                    far_vel_f64(sh[k], p, acc);
+            } else {
+                if (F32)
+                    far_grad_f32(sh[k], p, Gf);
+                else
+                    far_grad_f64(sh[k], p, G);
+            }
         }
     }
+    if (!live) return;
     double* o = part + ((int64_t)ks * m + t) * 3;
-    for (int q = 0; q < 3; q++) o[q] = acc[q] + (double)accf[q];
+    if (MODE == PS_VEL) {
+        for (int q = 0; q < 3; q++) o[q] = F32 ? (double)accf[q] : acc[q];
+    } else {
+        const double a0 = alpha[3 * t], a1 = alpha[3 * t + 1], a2 = alpha[3 * t + 2];
+        for (int r = 0; r < 3; r++) {
+            const double g0 = F32 ? (double)Gf[r][0] : G.a[r][0];
+            const double g1 = F32 ? (double)Gf[r][1] : G.a[r][1];
+            const double g2 = F32 ? (double)Gf[r][2] : G.a[r][2];
+            o[r] = g0 * a0 + g1 * a1 + g2 * a2;
+        }
+    }
 }
 
-// panel -> particle stretching, far field in FP32 and near field in FP64
-__global__ __launch_bounds__(kT) void stretch_f32_kernel(const PanelRec* __restrict__ body, int nb,
+// bounding sphere of each tile of 64 points in the spatial order perm
+// (centre, radius with a margin over its rounding): one wavefront per tile
+__global__ __launch_bounds__(64) void point_tiles_kernel(int m, const double* __restrict__ pts,
+                                                         const int* __restrict__ perm, double* __restrict__ tsph) {
+    const int tile = blockIdx.x, lane = threadIdx.x;
+    const int t = tile * 64 + lane;
+    const int src = perm[t < m ? t : tile * 64];  // the tile's first point stands in for missing ones
+    const double x = pts[3 * src], y = pts[3 * src + 1], z = pts[3 * src + 2];
+    double lo[3] = {x, y, z}, hi[3] = {x, y, z};
+    for (int o = 32; o >= 1; o >>= 1)
+        for (int d = 0; d < 3; d++) {
+            lo[d] = fmin(lo[d], __shfl_xor(lo[d], o, 64));
+            hi[d] = fmax(hi[d], __shfl_xor(hi[d], o, 64));
+        }
+    const double cx = 0.5 * (lo[0] + hi[0]), cy = 0.5 * (lo[1] + hi[1]), cz = 0.5 * (lo[2] + hi[2]);
+    double r = sqrt((x - cx) * (x - cx) + (y - cy) * (y - cy) + (z - cz) * (z - cz));
+    for (int o = 32; o >= 1; o >>= 1) r = fmax(r, __shfl_xor(r, o, 64));
+    if (lane == 0) {
+        tsph[4 * tile] = cx;
+        tsph[4 * tile + 1] = cy;
+        tsph[4 * tile + 2] = cz;
+        tsph[4 * tile + 3] = r * (1 + 1e-12) + 1e-300;
+    }
+}
+
+// near pass: one 64-lane workgroup per (tile of points in the spatial
+// order perm, slice blockIdx.y of the panels); out: one slice of 3 per point
+// per panel slice
+template <int MODE>
+__global__ __launch_bounds__(64) void panels_near_kernel(const PanelRec* __restrict__ body, int nb,
                                                          const PanelRec* __restrict__ wake, int nw,
                                                          const double* __restrict__ sigma,
                                                          const double* __restrict__ mu,
                                                          const double* __restrict__ muw, int m,
-                                                         const double* __restrict__ pos,
-                                                         const double* __restrict__ alpha, int chunk,
-                                                         double* __restrict__ part) {
-    const int t = blockIdx.x * kT + threadIdx.x;
-    const int ks = blockIdx.y;
-    const int p0 = ks * chunk, p1 = min(nb + nw, p0 + chunk);
-    if (t >= m) return;
-    const double p[3] = {pos[3 * t], pos[3 * t + 1], pos[3 * t + 2]};
-    Mat3 G;
-    mat3_zero(G);
-    float Gf[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-    for (int j = p0; j < p1; j++) {
-        const bool isb = j < nb;
-        const PanelRec& P = isb ? body[j] : wake[j - nb];
-        const double d0 = p[0] - P.c[0], d1 = p[1] - P.c[1], d2 = p[2] - P.c[2];
-        if (is_far(P, sq_norm3(d0, d1, d2)))
-            vg_far_f32(P, isb, isb ? (float)sigma[j] : 0.0f, isb ? (float)mu[j] : (float)muw[j - nb], p, Gf);
-        else if (isb)
-            part_stretch_add(P, true, sigma[j], mu[j], p, G);
-        else
-            part_stretch_add(P, false, 0.0, muw[j - nb], p, G);
+                                                         const double* __restrict__ pts,
+                                                         const double* __restrict__ alpha,
+                                                         const int* __restrict__ perm,
+                                                         const double* __restrict__ tsph, int chunk,
+                                                         double* __restrict__ out) {
+    constexpr int NE = MODE == PS_VEL ? 3 : 9;  // values per pair
+    __shared__ double sp[64][3];
+    __shared__ double res[64][NE];
+    __shared__ int rl[64];
+    const int lane = threadIdx.x, tile = blockIdx.x;
+    const int ts = tile * 64 + lane;
+    const bool live = ts < m;
+    const int t = live ? perm[ts] : 0;  // the point this lane holds
+    double p[3] = {0, 0, 0};
+    if (live)
+        for (int k = 0; k < 3; k++) p[k] = pts[3 * t + k];
+    for (int k = 0; k < 3; k++) sp[lane][k] = p[k];
+    const double tc0 = tsph[4 * tile], tc1 = tsph[4 * tile + 1], tc2 = tsph[4 * tile + 2], tr = tsph[4 * tile + 3];
+    double acc[NE];
+    for (int e = 0; e < NE; e++) acc[e] = 0;
+    __syncthreads();
+    // one queued pair per lane: item = panel * 64 + point lane
+    auto eval = [&](int item, bool ok) {
+        double r[NE];
+        for (int e = 0; e < NE; e++) r[e] = 0;
+        int L = -1;
+        if (ok) {
+            const int j = item >> 6;
+            L = item & 63;
+            const double q[3] = {sp[L][0], sp[L][1], sp[L][2]};
+            const bool isb = j < nb;
+            const PanelRec& P = isb ? body[j] : wake[j - nb];
+            if (MODE == PS_VEL) {
+                if (isb) {
+                    double vs[3], vd[3];
+                    panel_v_inf_f64(P, q, vs, vd);
+                    const double sg = sigma[j], mj = mu[j];
+                    for (int k = 0; k < 3; k++) r[k] = sg * vs[k] + mj * vd[k];
+                } else {
+                    double vd[3];
+                    dub_v_inf_f64(P, q, vd);
+                    const double mw = muw[j - nb];
+                    for (int k = 0; k < 3; k++) r[k] = mw * vd[k];
+                }
+            } else {
+                Mat3 Gq;
+                mat3_zero(Gq);
+                if (isb)
+                    part_stretch_add(P, true, sigma[j], mu[j], q, Gq);
+                else
+                    part_stretch_add(P, false, 0.0, muw[j - nb], q, Gq);
+                for (int a = 0; a < 3; a++)
+                    for (int b = 0; b < 3; b++) r[3 * a + b] = Gq.a[a][b];
+            }
+        }
+        rl[lane] = L;
+        for (int e = 0; e < NE; e++) res[lane][e] = r[e];
+        __syncthreads();
+        for (int k = 0; k < 64; k++)
+            if (rl[k] == lane)
+                for (int e = 0; e < NE; e++) acc[e] += res[k][e];
+        __syncthreads();
+    };
+    RowQueue q;
+    const int jb = blockIdx.y * chunk, je = min(nb + nw, jb + chunk);
+    for (int j0 = jb; j0 < je; j0 += 64) {
+        const int jl = j0 + lane;
+        bool cand = false;
+        if (jl < je) {
+            const PanelRec& P = jl < nb ? body[jl] : wake[jl - nb];
+            // 5 longSide plus a margin above the is_far band and the
+            // rounding of the sphere test: a panel it drops has no near point
+            const double reach = 5.0 * P.long_side * (1.0 + 1e-9);
+            const double dx = P.c[0] - tc0, dy = P.c[1] - tc1, dz = P.c[2] - tc2;
+            const double lim = reach + tr;
+            cand = dx * dx + dy * dy + dz * dz <= lim * lim;
+        }
+        uint64_t cm = __ballot(cand);
+        while (cm) {
+            const int b = __ffsll((unsigned long long)cm) - 1;
+            cm &= cm - 1;
+            const int j = j0 + b;
+            const PanelRec& P = j < nb ? body[j] : wake[j - nb];
+            bool near = false;
+            if (live) {
+                const double d0 = p[0] - P.c[0], d1 = p[1] - P.c[1], d2 = p[2] - P.c[2];
+                near = !is_far(P, sq_norm3(d0, d1, d2));
+            }
+            queue_push(q, near, j * 64 + lane, lane, eval);
+        }
     }
-    const double a0 = alpha[3 * t], a1 = alpha[3 * t + 1], a2 = alpha[3 * t + 2];
-    double* o = part + ((int64_t)ks * m + t) * 3;
-    for (int r = 0; r < 3; r++) {
-        const double g0 = G.a[r][0] + Gf[r][0], g1 = G.a[r][1] + Gf[r][1], g2 = G.a[r][2] + Gf[r][2];
-        o[r] = g0 * a0 + g1 * a1 + g2 * a2;
+    eval(q.p, lane < q.qn);
+    if (!live) return;
+    double* o = out + ((int64_t)blockIdx.y * m + t) * 3;
+    if (MODE == PS_VEL) {
+        for (int k = 0; k < 3; k++) o[k] = acc[k];
+    } else {
+        const double a0 = alpha[3 * t], a1 = alpha[3 * t + 1], a2 = alpha[3 * t + 2];
+        for (int r = 0; r < 3; r++) o[r] = acc[3 * r] * a0 + acc[3 * r + 1] * a1 + acc[3 * r + 2] * a2;
     }
 }
 
@@ -794,41 +913,89 @@ int p2p_slices(int m, int npanels) {
     return ks;
 }
 
-void launch_panel_to_points(const DevGeom& g, const double* sigma, const double* mu, const double* muw, int m,
-                            const double* pts, int KS, double* part, double* out, hipStream_t s, bool f32far) {
+namespace {
+
+constexpr int kSortMin = 4096;  // points below which the near pass keeps their order
+
+__global__ void iota_kernel(int m, int* __restrict__ a) {
+    const int i = blockIdx.x * kT + threadIdx.x;
+    if (i < m) a[i] = i;
+}
+
+// the scratch of one panel-sum launch, in doubles from `part`: KS far and
+// KN near slices of 3 per point, the point tiles' spheres, the spatial
+// order, the sort's scratch
+struct PanelSumPlan {
+    int KS, KN, ntiles;
+    size_t tsph, perm, sort, total;
+};
+
+PanelSumPlan plan_panel_sums(int m, int np) {
+    PanelSumPlan P;
+    P.KS = p2p_slices(m, np);
+    P.ntiles = (m + 63) / 64;
+    // the near work gathers in the few tiles close to the body: split the
+    // panels finely so those tiles spread over many wavefronts
+    P.KN = 1;
+    while (P.KN < 32 && np / (2 * P.KN) >= 96) P.KN *= 2;
+    P.tsph = ((size_t)P.KS + P.KN) * 3 * (size_t)m;
+    P.perm = P.tsph + 4 * (size_t)P.ntiles;
+    P.sort = (P.perm + ((size_t)m + 1) / 2 + 31) / 32 * 32;  // 256-byte aligned
+    P.total = P.sort + (m > kSortMin ? (morton_sort_bytes(m) + 7) / 8 : 0);
+    return P;
+}
+
+// far pass into slices [0, KS), near pass into [KS, KS + KN), their sum
+// into out
+template <int MODE>
+void launch_panel_sums(const DevGeom& g, const double* sigma, const double* mu, const double* muw, int m,
+                       const double* pts, const double* alpha, double* part, double* out, hipStream_t s,
+                       bool f32far) {
     if (m <= 0) return;
     const int np = g.n + g.nw;
-    const int chunk = (np + KS - 1) / KS;
-    dim3 grid((unsigned)((m + kT - 1) / kT), (unsigned)KS);
+    const PanelSumPlan P = plan_panel_sums(m, np);
+    const int chunk = (np + P.KS - 1) / P.KS;
+    int* perm = reinterpret_cast<int*>(part + P.perm);
+    double* tsph = part + P.tsph;
+    dim3 grid((unsigned)((m + kT - 1) / kT), (unsigned)P.KS);
     if (f32far)
-        hipLaunchKernelGGL(p2p_f32_kernel, grid, dim3(kT), 0, s, g.body, g.n, g.wake, g.nw, sigma, mu, muw, m, pts,
-                           chunk, part);
+        hipLaunchKernelGGL((panels_far_kernel<MODE, true>), grid, dim3(kT), 0, s, g.body, g.n, g.wake, g.nw, sigma, mu,
+                           muw, m, pts, alpha, chunk, part);
     else
-        hipLaunchKernelGGL(p2p_kernel, grid, dim3(kT), 0, s, g.body, g.n, g.wake, g.nw, sigma, mu, muw, m, pts, chunk,
-                           part);
+        hipLaunchKernelGGL((panels_far_kernel<MODE, false>), grid, dim3(kT), 0, s, g.body, g.n, g.wake, g.nw, sigma,
+                           mu, muw, m, pts, alpha, chunk, part);
+    CP_LAUNCH_CHECK();
+    // spatial order for the near pass's tiles; a few thousand points (the
+    // seeding and collocation calls of the VP step) keep their own order,
+    // which the sort's launches would cost more than they save
+    if (m > kSortMin)
+        sort_points_morton(m, pts, part + P.sort, perm, s);
+    else
+        hipLaunchKernelGGL(iota_kernel, dim3((unsigned)((m + kT - 1) / kT)), dim3(kT), 0, s, m, perm);
+    hipLaunchKernelGGL(point_tiles_kernel, dim3((unsigned)P.ntiles), dim3(64), 0, s, m, pts, (const int*)perm, tsph);
+    const int nchunk = (np + P.KN - 1) / P.KN;
+    hipLaunchKernelGGL((panels_near_kernel<MODE>), dim3((unsigned)P.ntiles, (unsigned)P.KN), dim3(64), 0, s, g.body,
+                       g.n, g.wake, g.nw, sigma, mu, muw, m, pts, alpha, (const int*)perm, (const double*)tsph, nchunk,
+                       part + (size_t)P.KS * 3 * m);
     CP_LAUNCH_CHECK();
     hipLaunchKernelGGL(p2p_reduce_kernel, dim3((unsigned)((3 * m + kT - 1) / kT)), dim3(kT), 0, s,
-                       (const double*)part, KS, m, out);
+                       (const double*)part, P.KS + P.KN, m, out);
     CP_LAUNCH_CHECK();
 }
 
+}  // namespace
+
+size_t p2p_part_doubles(int m, int npanels) { return m > 0 ? plan_panel_sums(m, npanels).total : 1; }
+
+void launch_panel_to_points(const DevGeom& g, const double* sigma, const double* mu, const double* muw, int m,
+                            const double* pts, double* part, double* out, hipStream_t s, bool f32far) {
+    launch_panel_sums<PS_VEL>(g, sigma, mu, muw, m, pts, nullptr, part, out, s, f32far);
+}
+
 void launch_panel_stretching(const DevGeom& g, const double* sigma, const double* mu, const double* muw, int m,
-                             const double* pos, const double* alpha, int KS, double* part, double* out,
-                             hipStream_t s, bool f32far) {
-    if (m <= 0) return;
-    const int np = g.n + g.nw;
-    const int chunk = (np + KS - 1) / KS;
-    dim3 grid((unsigned)((m + kT - 1) / kT), (unsigned)KS);
-    if (f32far)
-        hipLaunchKernelGGL(stretch_f32_kernel, grid, dim3(kT), 0, s, g.body, g.n, g.wake, g.nw, sigma, mu, muw, m,
-                           pos, alpha, chunk, part);
-    else
-        hipLaunchKernelGGL(stretch_kernel, grid, dim3(kT), 0, s, g.body, g.n, g.wake, g.nw, sigma, mu, muw, m, pos,
-                           alpha, chunk, part);
-    CP_LAUNCH_CHECK();
-    hipLaunchKernelGGL(p2p_reduce_kernel, dim3((unsigned)((3 * m + kT - 1) / kT)), dim3(kT), 0, s,
-                       (const double*)part, KS, m, out);
-    CP_LAUNCH_CHECK();
+                             const double* pos, const double* alpha, double* part, double* out, hipStream_t s,
+                             bool f32far) {
+    launch_panel_sums<PS_STRETCH>(g, sigma, mu, muw, m, pos, alpha, part, out, s, f32far);
 }
 
 void launch_core_bh(const DevBhTree& T, int mode, double theta, double core, bool f32, int m, const double* tpos,

@@ -26,6 +26,9 @@
 #include "capi_internal.h"
 
 namespace cpgpu {
+
+constexpr int kB = 256;
+
 namespace capi {
 namespace {
 
@@ -33,7 +36,6 @@ namespace {
 
 constexpr int kKeyLevels = 21;          // octant levels 2..22 in the key
 constexpr int kDeepest = 1 + kKeyLevels;  // deepest level a key resolves
-constexpr int kB = 256;
 
 inline unsigned nblk(int64_t n) { return (unsigned)((n + kB - 1) / kB); }
 
@@ -66,23 +68,33 @@ __global__ __launch_bounds__(kB) void bbox_part_kernel(int n, const double* __re
 }
 
 // box[0..2] = root centre, box[3] = half width (octree::setDimensions,
-// octree.h:70-80)
-__global__ void bbox_final_kernel(int nb, const double* __restrict__ part, double* __restrict__ box) {
-    if (threadIdx.x != 0) return;
-    double mn[3], mx[3];
-    for (int d = 0; d < 3; d++) {
-        mn[d] = part[d];
-        mx[d] = part[3 + d];
-    }
-    for (int b = 1; b < nb; b++)
+// octree.h:70-80), from the per-block extremes (one workgroup)
+__global__ __launch_bounds__(kB) void bbox_final_kernel(int nb, const double* __restrict__ part,
+                                                        double* __restrict__ box) {
+    __shared__ double sh[6][kB];
+    double v[6];
+    for (int d = 0; d < 6; d++) v[d] = part[d];
+    for (int b = threadIdx.x; b < nb; b += kB)
         for (int d = 0; d < 3; d++) {
-            mn[d] = fmin(mn[d], part[6 * b + d]);
-            mx[d] = fmax(mx[d], part[6 * b + 3 + d]);
+            v[d] = fmin(v[d], part[6 * b + d]);
+            v[3 + d] = fmax(v[3 + d], part[6 * b + 3 + d]);
         }
+    for (int d = 0; d < 6; d++) sh[d][threadIdx.x] = v[d];
+    __syncthreads();
+    for (int w = kB / 2; w >= 1; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int d = 0; d < 3; d++) {
+                sh[d][threadIdx.x] = fmin(sh[d][threadIdx.x], sh[d][threadIdx.x + w]);
+                sh[3 + d][threadIdx.x] = fmax(sh[3 + d][threadIdx.x], sh[3 + d][threadIdx.x + w]);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
     double maxD = 0;
     for (int d = 0; d < 3; d++) {
-        box[d] = 0.5 * (mn[d] + mx[d]);
-        const double e = mx[d] - mn[d];
+        const double mn = sh[d][0], mx = sh[3 + d][0];
+        box[d] = 0.5 * (mn + mx);
+        const double e = mx - mn;
         maxD = d == 0 ? e : (e > maxD ? e : maxD);
     }
     box[3] = 0.51 * maxD;
@@ -336,7 +348,7 @@ bool build_tree_device(cp_ctx* c, int n, const double* dxyz, const double* dalph
     dalloc(c, bpart, (size_t)6 * nb * sizeof(double));
     dalloc(c, box, 4 * sizeof(double));
     hipLaunchKernelGGL(bbox_part_kernel, dim3(nb), dim3(kB), 0, s, n, dxyz, bpart.as<double>());
-    hipLaunchKernelGGL(bbox_final_kernel, dim3(1), dim3(64), 0, s, (int)nb, bpart.as<double>(), box.as<double>());
+    hipLaunchKernelGGL(bbox_final_kernel, dim3(1), dim3(kB), 0, s, (int)nb, bpart.as<double>(), box.as<double>());
     // keys, stable sort
     DevBuf keys, keys2, idx, idx2;
     dalloc(c, keys, (size_t)n * sizeof(uint64_t));
@@ -448,4 +460,79 @@ bool build_tree_device(cp_ctx* c, int n, const double* dxyz, const double* dalph
 }
 
 }  // namespace capi
+
+// ---- spatial order of a point set (the panel sums' near pass) -------------
+namespace {
+
+constexpr size_t kAlign = 256;
+inline size_t align_up(size_t b) { return (b + kAlign - 1) / kAlign * kAlign; }
+
+__device__ inline uint32_t spread10(uint32_t v) {  // 10 bits -> every third bit
+    v &= 0x3ff;
+    v = (v | (v << 16)) & 0x030000ffu;
+    v = (v | (v << 8)) & 0x0300f00fu;
+    v = (v | (v << 4)) & 0x030c30c3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+// 30-bit Morton key of each point over the cube of the points' bounding box
+__global__ __launch_bounds__(kB) void morton_kernel(int m, const double* __restrict__ pts,
+                                                    const double* __restrict__ box, uint32_t* __restrict__ keys,
+                                                    int* __restrict__ idx) {
+    const int i = blockIdx.x * kB + threadIdx.x;
+    if (i >= m) return;
+    const double h = box[3];
+    uint32_t q[3] = {0, 0, 0};
+    if (h > 0)
+        for (int d = 0; d < 3; d++) {
+            const double u = (pts[3 * i + d] - (box[d] - h)) / (2 * h) * 1024.0;
+            q[d] = (uint32_t)fmin(fmax(u, 0.0), 1023.0);
+        }
+    keys[i] = (spread10(q[0]) << 2) | (spread10(q[1]) << 1) | spread10(q[2]);
+    idx[i] = i;
+}
+
+struct SortLayout {
+    size_t parts, box, keys, keys2, idx, tmp, sort_bytes, total;
+};
+
+SortLayout sort_layout(int m) {
+    SortLayout L;
+    L.sort_bytes = 0;
+    (void)hipcub::DeviceRadixSort::SortPairs(nullptr, L.sort_bytes, (const uint32_t*)nullptr, (uint32_t*)nullptr,
+                                             (const int*)nullptr, (int*)nullptr, m, 0, 30);
+    const size_t nb = capi::nblk(m);
+    L.parts = 0;
+    L.box = align_up(L.parts + 6 * nb * sizeof(double));
+    L.keys = align_up(L.box + 4 * sizeof(double));
+    L.keys2 = align_up(L.keys + (size_t)m * sizeof(uint32_t));
+    L.idx = align_up(L.keys2 + (size_t)m * sizeof(uint32_t));
+    L.tmp = align_up(L.idx + (size_t)m * sizeof(int));
+    L.total = align_up(L.tmp + L.sort_bytes);
+    return L;
+}
+
+}  // namespace
+
+size_t morton_sort_bytes(int m) { return m > 0 ? sort_layout(m).total : 0; }
+
+void sort_points_morton(int m, const double* pts, void* scratch, int* perm, hipStream_t s) {
+    if (m <= 0) return;
+    const SortLayout L = sort_layout(m);
+    char* base = static_cast<char*>(scratch);
+    double* parts = reinterpret_cast<double*>(base + L.parts);
+    double* box = reinterpret_cast<double*>(base + L.box);
+    uint32_t* keys = reinterpret_cast<uint32_t*>(base + L.keys);
+    uint32_t* keys2 = reinterpret_cast<uint32_t*>(base + L.keys2);
+    int* idx = reinterpret_cast<int*>(base + L.idx);
+    const unsigned nb = capi::nblk(m);
+    hipLaunchKernelGGL(capi::bbox_part_kernel, dim3(nb), dim3(kB), 0, s, m, pts, parts);
+    hipLaunchKernelGGL(capi::bbox_final_kernel, dim3(1), dim3(kB), 0, s, (int)nb, (const double*)parts, box);
+    hipLaunchKernelGGL(morton_kernel, dim3(nb), dim3(kB), 0, s, m, pts, (const double*)box, keys, idx);
+    CP_LAUNCH_CHECK();
+    size_t bytes = L.sort_bytes;
+    CP_HIP(hipcub::DeviceRadixSort::SortPairs(base + L.tmp, bytes, keys, keys2, idx, perm, m, 0, 30, s));
+}
+
 }  // namespace cpgpu

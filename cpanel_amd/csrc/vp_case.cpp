@@ -107,12 +107,11 @@ std::vector<double> vel_panels(cp_vp* v, const std::vector<double>& pts) {
     to_dev(c, dmu, v->mu, s);
     to_dev(c, dmw, v->mu1, s);
     to_dev(c, dpts, pts, s);
-    const int KS = p2p_slices(m, g->dev.n + g->dev.nw);
-    dalloc(c, dpart, (size_t)KS * 3 * m * sizeof(double));
+    dalloc(c, dpart, p2p_part_doubles(m, g->dev.n + g->dev.nw) * sizeof(double));
     dalloc(c, dout, (size_t)3 * m * sizeof(double));
     {
         TimedScope ts("panel_to_points", s);
-        launch_panel_to_points(g->dev, dsig.as<double>(), dmu.as<double>(), dmw.as<double>(), m, dpts.as<double>(), KS,
+        launch_panel_to_points(g->dev, dsig.as<double>(), dmu.as<double>(), dmw.as<double>(), m, dpts.as<double>(),
                                dpart.as<double>(), dout.as<double>(), s, v->f32);
     }
     from_dev(out, dout, (size_t)3 * m, s);
@@ -130,10 +129,9 @@ std::vector<double> vel_row2(cp_vp* v, const std::vector<double>& pts) {
     DevBuf dmw, dpts, dpart, dout;
     to_dev(c, dmw, v->mu2, s);
     to_dev(c, dpts, pts, s);
-    const int KS = p2p_slices(m, v->m);
-    dalloc(c, dpart, (size_t)KS * 3 * m * sizeof(double));
+    dalloc(c, dpart, p2p_part_doubles(m, v->m) * sizeof(double));
     dalloc(c, dout, (size_t)3 * m * sizeof(double));
-    launch_panel_to_points(v->row2, nullptr, nullptr, dmw.as<double>(), m, dpts.as<double>(), KS, dpart.as<double>(),
+    launch_panel_to_points(v->row2, nullptr, nullptr, dmw.as<double>(), m, dpts.as<double>(), dpart.as<double>(),
                            dout.as<double>(), s, v->f32);
     from_dev(out, dout, (size_t)3 * m, s);
     return out;
@@ -291,13 +289,12 @@ std::vector<double> panel_stretch(cp_vp* v) {
     to_dev(c, dmw, v->mu1, s);
     to_dev(c, dp, v->ppos, s);
     to_dev(c, ds, v->pstr, s);
-    const int KS = p2p_slices(np, g->dev.n + g->dev.nw);
-    dalloc(c, dpart, (size_t)KS * 3 * np * sizeof(double));
+    dalloc(c, dpart, p2p_part_doubles(np, g->dev.n + g->dev.nw) * sizeof(double));
     dalloc(c, dout, (size_t)3 * np * sizeof(double));
     {
         TimedScope ts("panel_stretching", s);
         launch_panel_stretching(g->dev, dsig.as<double>(), dmu.as<double>(), dmw.as<double>(), np, dp.as<double>(),
-                                ds.as<double>(), KS, dpart.as<double>(), dout.as<double>(), s, v->f32);
+                                ds.as<double>(), dpart.as<double>(), dout.as<double>(), s, v->f32);
     }
     from_dev(out, dout, (size_t)3 * np, s);
     return out;

@@ -4,6 +4,7 @@ the sanitizer runtime preloaded (tests/test_replay.py).
     python tests/replay/run_case.py LIB assembly
     python tests/replay/run_case.py LIB vp STEPS
     python tests/replay/run_case.py LIB vpcfg STEPS
+    python tests/replay/run_case.py LIB panels
     python tests/replay/run_case.py LIB teardown
 
 Each prints "ok ..." lines; any sanitizer report aborts the process.
@@ -124,6 +125,39 @@ def vp_config(steps):
     c.close()
 
 
+def panels():
+    """The panel -> point velocity and panel -> particle stretching sums
+    (far pass over LDS-staged panels, near pass over queued pairs) against
+    the oracle on a small wing, points near the surface and far away, FP64
+    and with the far field in FP32."""
+    from oracle import vp_oracle as VO
+    m = meshgen.wing("4412", 1.0, 4.0, 24, 8, "uniform", 2.0)
+    c = Context(0)
+    g = c.geometry(m)
+    tri, xyz = np.asarray(m["tri"]), np.asarray(m["xyz"])
+    bv, wv = xyz[tri[g.body()["tri_index"]]], xyz[tri[g.wake()["tri_index"]]]
+    rng = np.random.default_rng(4)
+    sigma, mu, muw = rng.normal(size=g.nb), rng.normal(size=g.nb), rng.normal(size=g.nw)
+    ctr, nrm = g.body()["center"], g.body()["normal"]
+    pts = np.vstack([ctr[::5] + 0.03 * nrm[::5], rng.normal(size=(150, 3)) * 3.0,
+                     np.column_stack([rng.uniform(1.0, 3.0, 70), rng.uniform(-2, 2, 70), rng.normal(0, 0.05, 70)])])
+    alpha = rng.normal(size=pts.shape)
+    ref = VO.panels_to_points(bv, sigma, mu, wv, muw, pts)
+    sref = VO.panels_stretching(bv, sigma, mu, wv, muw, pts, alpha)
+    ok = np.isfinite(sref).all(axis=1)
+    for f32, tol in ((False, 1e-10), (True, 1e-5)):
+        got = g.panel_to_points(sigma, mu, pts, mu_wake=muw, fp32=f32)
+        e = np.abs(got - ref).max() / np.abs(ref).max()
+        sg = g.panel_stretching(sigma, mu, pts, alpha, mu_wake=muw, fp32=f32)
+        assert np.array_equal(np.isnan(sg), np.isnan(sref))
+        es = np.abs(sg[ok] - sref[ok]).max() / np.abs(sref[ok]).max()
+        assert e <= tol and es <= (1e-8 if not f32 else 1e-5), (f32, e, es)
+        print("ok panels fp32=%d velocity %.1e stretching %.1e (m=%d, %d panels)" %
+              (f32, e, es, len(pts), g.nb + g.nw), flush=True)
+    g.release()
+    c.close()
+
+
 def teardown():
     """Parents closed before their children: the context before a geometry,
     a tree and a vortex-particle case, the geometry before its case."""
@@ -157,5 +191,7 @@ if __name__ == "__main__":
         vp(int(sys.argv[3]))
     elif what == "vpcfg":
         vp_config(int(sys.argv[3]))
+    elif what == "panels":
+        panels()
     elif what == "teardown":
         teardown()

@@ -2,6 +2,11 @@
 // builtins) is not replayed: its solver entry points throw, which the C ABI
 // reports as CP_HIP_ERROR.  launch_sigma (used by the FP64 solve) is restated
 // on the host; the bf16 packing is skipped.
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <vector>
+
 #include "capi_internal.h"
 namespace cpgpu {
 [[noreturn]] static void nope(const char* w) { throw HipError{1, std::string("replay: not emulated: ") + w}; }
@@ -44,4 +49,47 @@ bool build_tree_device(cp_ctx*, int, const double*, const double*, const double*
     return false;
 }
 }  // namespace capi
+}  // namespace cpgpu
+
+// the spatial order of the panel sums' near pass (hipCUB on the device):
+// the same 30-bit Morton keys over the bounding cube, stably sorted on the host
+namespace cpgpu {
+size_t morton_sort_bytes(int) { return 0; }
+void sort_points_morton(int m, const double* pts, void*, int* perm, hipStream_t) {
+    if (m <= 0) return;
+    double mn[3], mx[3];
+    for (int d = 0; d < 3; d++) mn[d] = mx[d] = pts[d];
+    for (int i = 0; i < m; i++)
+        for (int d = 0; d < 3; d++) {
+            mn[d] = std::min(mn[d], pts[3 * i + d]);
+            mx[d] = std::max(mx[d], pts[3 * i + d]);
+        }
+    double c[3], maxD = 0;
+    for (int d = 0; d < 3; d++) {
+        c[d] = 0.5 * (mn[d] + mx[d]);
+        const double e = mx[d] - mn[d];
+        maxD = d == 0 ? e : (e > maxD ? e : maxD);
+    }
+    const double h = 0.51 * maxD;
+    auto spread = [](uint32_t v) {
+        v &= 0x3ff;
+        v = (v | (v << 16)) & 0x030000ffu;
+        v = (v | (v << 8)) & 0x0300f00fu;
+        v = (v | (v << 4)) & 0x030c30c3u;
+        v = (v | (v << 2)) & 0x09249249u;
+        return v;
+    };
+    std::vector<uint32_t> key(m);
+    for (int i = 0; i < m; i++) {
+        uint32_t q[3] = {0, 0, 0};
+        if (h > 0)
+            for (int d = 0; d < 3; d++) {
+                const double u = (pts[3 * i + d] - (c[d] - h)) / (2 * h) * 1024.0;
+                q[d] = (uint32_t)std::fmin(std::fmax(u, 0.0), 1023.0);
+            }
+        key[i] = (spread(q[0]) << 2) | (spread(q[1]) << 1) | spread(q[2]);
+        perm[i] = i;
+    }
+    std::stable_sort(perm, perm + m, [&](int a, int b) { return key[a] < key[b]; });
+}
 }  // namespace cpgpu

@@ -148,8 +148,9 @@ def test_panel_to_points_matches_oracle(ctx):
     muw = rng.normal(size=g.nw)
     c = g.body()["center"]
     nrm = g.body()["normal"]
-    # points off the surface (near field of many panels) and far away
-    pts = np.vstack([c[::7] + 0.02 * nrm[::7], rng.normal(size=(200, 3)) * 4.0])
+    # points off the surface (near field of many panels) and far away; over
+    # 4,096 points, so the near pass tiles them in their Morton order
+    pts = np.vstack([c[::7] + 0.02 * nrm[::7], rng.normal(size=(4200, 3)) * 4.0])
     say("[gpu-vp] panel to points m=%d panels=%d" % (len(pts), g.nb + g.nw))
     got = g.panel_to_points(sigma, mu, pts, mu_wake=muw)
     ref = V.panels_to_points(bv, sigma, mu, wv, muw, pts)

@@ -14,6 +14,8 @@ ThreadSanitizer.
 * vpcfg: the configured form of the case (algebraic fixed-core kernel,
   particles split along each shedding panel, FP32 sums) on a small swept,
   tapered wing, and theta = 0 trees of that kernel against the oracle.
+* panels: the panel -> point velocity and stretching sums (far pass, queued
+  near pass) against the oracle.
 * assembly: the far and near passes (tile culling, lane-permute compaction)
   against the FP64 oracle, 1e-12 of max |A| and max |B|.
 
@@ -56,6 +58,13 @@ def test_assembly_passes_match_oracle_asan():
 def test_vp_steps_and_interpreter_teardown_asan():
     out = run("asan", "vp", "6")
     assert "ok vp step 6" in out and "ok vp finish" in out
+
+
+def test_panel_sums_match_oracle_asan():
+    """Panel -> point velocity and stretching: far pass and queued near pass
+    against the oracle, FP64 and with the far field in FP32."""
+    out = run("asan", "panels")
+    assert out.count("ok panels") == 2
 
 
 def test_vp_config_asan():
