@@ -107,6 +107,9 @@ SIGNATURES = {
     "cp_load_matrices": (C.c_int, [C.c_void_p, _d, _d, C.c_int64]),
     "cp_solve": (C.c_int, [C.c_void_p, C.c_int, C.c_int32, C.POINTER(Freestream), C.POINTER(SolveOpts), _d, _d,
                            C.POINTER(SolveInfo)]),
+    "cp_solve_group": (C.c_int, [C.c_int32, C.POINTER(C.c_void_p), C.c_int, C.POINTER(C.c_int32),
+                                 C.POINTER(C.POINTER(Freestream)), C.POINTER(SolveOpts), C.POINTER(_d),
+                                 C.POINTER(_d), C.POINTER(C.POINTER(SolveInfo))]),
     "cp_matvec": (C.c_int, [C.c_void_p, C.c_int, C.c_int32, _d, _d]),
     "cp_matvec_f64_noalloc": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64]),
     "cp_solver_report": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int32]),

@@ -61,6 +61,41 @@ class Context:
     def geometry(self, mesh, key=0):
         return Geometry(self, mesh, key)
 
+    def solve_group(self, jobs, prec=CP_FP32_REFINE, mach=0.1, Sref=1.0, bref=1.0, cref=1.0, cg=(0, 0, 0),
+                    tol=1e-10, max_iter=0, max_refine=0, inner_tol=0.0, want_mu=True, want_cp=False,
+                    allow_unconverged=False, precond=0):
+        """Several geometries solved in one pass (cp_solve_group): jobs is a
+        list of (geometry, conditions) with distinct geometries and at most 64
+        conditions each; returns one list of per-condition results per job,
+        as Geometry.solve does."""
+        G = len(jobs)
+        geoms = (C.c_void_p * G)(*[g.h for g, _ in jobs])
+        ks = (C.c_int32 * G)(*[len(c) for _, c in jobs])
+        fs_keep = [(abi.Freestream * len(c)).from_buffer_copy(np.ascontiguousarray(c, dtype=np.float64).reshape(-1, 3))
+                   for _, c in jobs]
+        fs = (C.POINTER(abi.Freestream) * G)(*[C.cast(f, C.POINTER(abi.Freestream)) for f in fs_keep])
+        o = abi.SolveOpts(mach, Sref, bref, cref, (C.c_double * 3)(*cg), tol, max_iter, max_refine, inner_tol,
+                          precond, 0)
+        mus = [np.zeros((len(c), g.nb)) if want_mu else None for g, c in jobs]
+        cps = [np.zeros((len(c), g.nb)) if want_cp else None for g, c in jobs]
+        infos = [(abi.SolveInfo * len(c))() for _, c in jobs]
+        dp = C.POINTER(C.c_double)
+        mu_p = (dp * G)(*[_p(m) if m is not None else dp() for m in mus])
+        cp_p = (dp * G)(*[_p(q) if q is not None else dp() for q in cps])
+        in_p = (C.POINTER(abi.SolveInfo) * G)(*[C.cast(i, C.POINTER(abi.SolveInfo)) for i in infos])
+        st = abi.lib().cp_solve_group(G, geoms, prec, ks, fs, C.byref(o), mu_p, cp_p, in_p)
+        if not (st == abi.CP_NOT_CONVERGED and allow_unconverged):
+            self.check(st, "solve_group")
+        out = []
+        for (g, c), m, q, info in zip(jobs, mus, cps, infos):
+            rec = np.frombuffer(info, dtype=_SOLVE_INFO)
+            out.append([dict(mu=None if m is None else m[i], cp=None if q is None else q[i], CL=float(rec["CL"][i]),
+                             CDi=float(rec["CDi"][i]), Fbody=rec["Fbody"][i].copy(), Fwind=rec["Fwind"][i].copy(),
+                             CM=rec["CM"][i].copy(), rel_residual=float(rec["rel_residual"][i]),
+                             iters=int(rec["iters"][i]), refinements=int(rec["refinements"][i]),
+                             converged=bool(rec["converged"][i])) for i in range(len(c))])
+        return out
+
     def geometry_vp(self, mesh, V=1.0, dt=0.0, key=0):
         """Vortex-particle geometry: no prescribed wake, two buffer rows of
         quads behind the trailing edge (time step dt; <= 0: computed)."""

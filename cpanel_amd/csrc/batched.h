@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <memory>
 #include <vector>
 
 #include "krylov.h"
@@ -21,6 +22,27 @@ constexpr double kInnerTolTarget = 2e-4;
 // block-Jacobi inverses in matrix-core operand order: FP32 (FP32 MFMA) or
 // bf16 hi/lo pairs (bf16 MFMA)
 enum { kBjPackFp32 = 0, kBjPackBf16 = 1 };
+
+// geometries per grouped solve (BatchedSolver::solve_group)
+constexpr int kMaxGroup = 16;
+
+struct GmProb;  // one geometry's arrays and sizes in a grouped launch (batched.hip)
+class BatchedSolver;
+
+// one geometry's part of a grouped solve: its solver (set up as for solve())
+// and k <= 64 steady conditions
+struct GroupItem {
+    BatchedSolver* sv;
+    const float* A32;
+    const float* B32;
+    const double* normals;
+    const double* vinf4;
+    int k;
+    double* x;
+    int64_t ldx;
+    SolveStats* stats;
+    const struct BatchedOpts* o;  // this geometry's options (as for its own solve())
+};
 
 struct BatchedOpts {
     double tol = 1e-10;       // outer: ||b - A x|| / ||b|| in FP64
@@ -75,11 +97,27 @@ public:
     // and x0 (initial guess, or null = zero)
     void solve_rhs(const float* A32, const float* B32, const double* sigma, const double* rhs_add, const double* x0,
                    int k, const BatchedOpts& o, double* x, int64_t ldx, SolveStats* stats, hipStream_t s);
+    // Several geometries' steady solves at once (k <= 64 conditions each, on
+    // one stream s): every GMRES iteration stage is one launch covering all
+    // geometries still sweeping (grouped kernels), the rest of the
+    // refinement loop runs per geometry.  A geometry's launches are exactly
+    // those of its own solve(), so its results are bit-identical to it.
+    // dev: group_bytes(items.size()) bytes of device memory.
+    static void solve_group(std::vector<GroupItem>& items, void* dev, hipStream_t s);
+    static size_t group_bytes(int G);
+    // the grouped iteration applies to this solver now: GMRES with the bf16
+    // block-Jacobi product and a bf16x2 product shape (valid after tuning)
+    bool group_capable() const;
     void set_shape(const GemmShape& g) {
         shape_ = g;
         tuned_ = true;
     }
     void set_autotune(bool on) { autotune_ = on; }
+    // off: never capture graphs (solvers used from several host threads at
+    // once: a capture there failed other threads' HIP calls)
+    void allow_graphs(bool on) {
+        if (!on) graphs_ = false;
+    }
     // times every candidate shape on A32 and keeps the fastest (once per
     // leading dimension and process)
     void tune(const float* A32, hipStream_t s);
@@ -147,13 +185,64 @@ private:
     void tile_solve(const float* A32, const float* B32, const double* normals, const double* vinf4, int c0,
                     int kc, const BatchedOpts& o, double* x, int64_t ldx, SolveStats* stats, hipStream_t s,
                     const double* sig_cm = nullptr, const double* add_cm = nullptr, const double* x0_cm = nullptr);
-    void launch_gemm32(const GemmShape& sh, const float* A32, const float* P, hipStream_t s);
+    // parts: 1 packs P into bf16 tiles (variants >= 5), 2 runs the product, 3 both
+    void launch_gemm32(const GemmShape& sh, const float* A32, const float* P, hipStream_t s, int parts = 3);
     void apply_block_jacobi(const float* P, float* Y, hipStream_t s);
     // Y = B (rows >= n zeroed) with the column partials of ||Y_c||^2 in dpart_
     void copy64(const double* B, double* Y, hipStream_t s);
     // inner solve of one refinement sweep by right-preconditioned GMRES
     void bj_apply_pk(const float* PP, float* Y, const double* cscale, int cstride, hipStream_t s);
-    void gmres_sweep(const float* A32, const BatchedOpts& o, int kc, SolveStats* stats, hipStream_t s);
+    // one 64-column tile of a solve: inputs and the refinement loop's host state
+    struct Tile {
+        const float* A32 = nullptr;
+        const float* B32 = nullptr;
+        const double* normals = nullptr;
+        const double* vinf4 = nullptr;
+        int kc = 0;
+        BatchedOpts o;
+        double* x = nullptr;
+        int64_t ldx = 0;
+        SolveStats* stats = nullptr;
+        const double* sig_cm = nullptr;
+        const double* add_cm = nullptr;
+        const double* x0_cm = nullptr;
+        std::vector<double> bb, rel, prev;
+        std::vector<int> done;
+        int sweep = 0;
+    };
+    void tile_begin(Tile& t, hipStream_t s);
+    void tile_begin_host(Tile& t);
+    void tile_residual(Tile& t, hipStream_t s);
+    bool tile_check(Tile& t);
+    void tile_sweep_open(Tile& t, hipStream_t s);
+    void tile_sweep_close(Tile& t, hipStream_t s);
+    void tile_end(Tile& t, hipStream_t s);
+    void bicgstab_sweep(Tile& t, hipStream_t s);
+    void colsum_q(const double* dp, int nq, int q0, int q1, int op, hipStream_t s);
+    bool gmres_ready() const;
+    int gm_bound(const BatchedOpts& o) const;
+    void gm_begin(hipStream_t s);
+    void gm_iterations(const float* A32, const BatchedOpts& o, hipStream_t s);
+    void gm_end(hipStream_t s);
+    bool gm_counted_ = false;  // a GMRES sweep left its iteration counts in sc_[Q_GMNJ]
+    void fill_prob(GmProb& P, const BatchedOpts& o, int* flag) const;
+    static void group_iterations(const std::vector<BatchedSolver*>& sv, const GmProb* probs,
+                                 const std::vector<GmProb>& hp, int* flags, hipStream_t s);
+    // GMRES iteration j's launches captured once as HIP graphs and replayed
+    // in every later sweep.  The iteration is four segments (block-Jacobi
+    // product, P packing, product, Gram-Schmidt + Givens); a segment whose
+    // kernel is being timed is launched on the stream between graphs (events
+    // recorded inside a graph carry no timestamps here), the others are
+    // captured, consecutive ones into one graph.  The cache is keyed by
+    // everything the captured launches depend on.
+    struct IterPlan {
+        std::vector<std::pair<hipGraphExec_t, int>> items;  // (graph, -1) or (null, segment launched directly)
+        ~IterPlan();
+    };
+    bool graphs_ = false;  // CPGPU_GRAPH=1: replay captured iteration graphs
+    std::vector<std::unique_ptr<IterPlan>> gm_plans_;
+    std::vector<uintptr_t> gm_plan_key_;
+    void gm_plan_clear();
     const double* a64_ = nullptr;
     const double* b64_ = nullptr;
     const void* apk_ = nullptr;

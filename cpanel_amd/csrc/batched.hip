@@ -41,7 +41,8 @@ constexpr int kRedRows = 64;   // rows per reduction block
 
 enum {
     Q_RHO = 0, Q_ALPHA, Q_OMEGA, Q_RR, Q_R0R, Q_R0V, Q_TT, Q_TS, Q_RR0, Q_BB, Q_RR64, Q_SCALE, Q_BETA, Q_IT, Q_CHK,
-    Q_CHKIT, Q_MIN, Q_ITOL, Q_NQ = 20  // Q_ITOL: inner residual reduction of the current sweep
+    Q_CHKIT, Q_MIN, Q_ITOL, Q_GMNJ, Q_NQ = 20  // Q_ITOL: inner residual reduction of the current sweep;
+                                              // Q_GMNJ: GMRES iterations of the last sweep
 };
 
 // stagnation window of the inner FP32 solve: a column stops when the best
@@ -328,10 +329,10 @@ __global__ __launch_bounds__(256) void pack_a_bf16x2_kernel(const float* __restr
 }
 
 // P (row-major lda x 64) -> tiles [K block][column tile 0/1]; rows >= n are 0
-__global__ __launch_bounds__(256) void pack_p_bf16x2_kernel(const float* __restrict__ P, int n, int nkb,
-                                                            uint4* __restrict__ out, const int* __restrict__ gate) {
+__device__ inline void pack_p_body(int bx, const float* __restrict__ P, int n, int nkb, uint4* __restrict__ out,
+                                   const int* __restrict__ gate) {
     if (gate && gate_idle(gate)) return;
-    const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t g = (int64_t)bx * 256 + threadIdx.x;
     const int64_t tile = g >> 6;
     if (tile >= 2 * (int64_t)nkb) return;
     const int lane = (int)(g & 63);
@@ -348,6 +349,10 @@ __global__ __launch_bounds__(256) void pack_p_bf16x2_kernel(const float* __restr
     split8(v, hi, lo);
     out[2 * g] = hi;
     out[2 * g + 1] = lo;
+}
+__global__ __launch_bounds__(256) void pack_p_bf16x2_kernel(const float* __restrict__ P, int n, int nkb,
+                                                            uint4* __restrict__ out, const int* __restrict__ gate) {
+    pack_p_body(blockIdx.x, P, n, nkb, out, gate);
 }
 
 // Y_part[ks] = A~ P~ over the K blocks of slice ks.  Wavefront: 4 row tiles
@@ -546,14 +551,12 @@ __device__ inline void lo_tile_row(const uint4* __restrict__ Alo, int nkb, const
 // restoration (lo_tile_row, 4 row tiles each) on the CUs and wave slots the
 // product leaves free, writing the extra partial slice in the same launch.
 template <int RT, int S>
-__global__ __launch_bounds__(256) void gemm_bf16x2_s_kernel(const uint4* __restrict__ Apk, int nkb,
-                                                            const uint4* __restrict__ Ppk, float* __restrict__ part,
-                                                            int64_t lda, int kbchunk, int KS, int ngemm,
-                                                            int64_t plane, const int* __restrict__ tptr,
-                                                            const int* __restrict__ tkb,
-                                                            const int* __restrict__ gate) {
+__device__ inline void gemm_bf16x2_s_body(int bid, const uint4* __restrict__ Apk, int nkb,
+                                          const uint4* __restrict__ Ppk, float* __restrict__ part, int64_t lda,
+                                          int kbchunk, int KS, int ngemm, int64_t plane,
+                                          const int* __restrict__ tptr, const int* __restrict__ tkb,
+                                          const int* __restrict__ gate) {
     if (gate && gate_idle(gate)) return;
-    const int bid = blockIdx.x;
     if (bid >= ngemm) {
         lo_tile_row(Apk + plane, nkb, tptr, tkb, Ppk, part + (int64_t)KS * lda * KP,
                     (int64_t)(bid - ngemm) * 4 + (threadIdx.x >> 6), lda / 32);
@@ -625,6 +628,15 @@ __global__ __launch_bounds__(256) void gemm_bf16x2_s_kernel(const uint4* __restr
             out[r * KP + m] = acc[t][0][j];
             out[r * KP + 32 + m] = acc[t][1][j];
         }
+}
+template <int RT, int S>
+__global__ __launch_bounds__(256) void gemm_bf16x2_s_kernel(const uint4* __restrict__ Apk, int nkb,
+                                                            const uint4* __restrict__ Ppk, float* __restrict__ part,
+                                                            int64_t lda, int kbchunk, int KS, int ngemm,
+                                                            int64_t plane, const int* __restrict__ tptr,
+                                                            const int* __restrict__ tkb,
+                                                            const int* __restrict__ gate) {
+    gemm_bf16x2_s_body<RT, S>(blockIdx.x, Apk, nkb, Ppk, part, lda, kbchunk, KS, ngemm, plane, tptr, tkb, gate);
 }
 
 __global__ __launch_bounds__(256) void gemm_f64acc_kernel(const float* __restrict__ A, int64_t lda, int n,
@@ -1159,10 +1171,9 @@ __global__ __launch_bounds__(1024) void gm_start_kernel(const double* __restrict
 // w = sum_ks part (the product A~ z_j, row order) stored as u_{j+1} in block
 // order; a thread sums four columns (skipped when none of them is active;
 // an inactive column's u_{j+1} is never read)
-__global__ __launch_bounds__(256) void gm_w_kernel(const float* __restrict__ part, int KS, int64_t lda, int n,
-                                                   const int* __restrict__ act, const int* __restrict__ iperm,
-                                                   float* __restrict__ W) {
-    const int64_t idx = (int64_t)blockIdx.x * kT + threadIdx.x;  // float4 index
+__device__ inline void gm_w_body(int bx, const float* __restrict__ part, int KS, int64_t lda, int n,
+                                 const int* __restrict__ act, const int* __restrict__ iperm, float* __restrict__ W) {
+    const int64_t idx = (int64_t)bx * kT + threadIdx.x;  // float4 index
     if (idx >= (int64_t)n * (KP / 4)) return;
     const int c = 4 * (int)(idx & 15);
     if (!(act[c] | act[c + 1] | act[c + 2] | act[c + 3])) return;
@@ -1172,17 +1183,22 @@ __global__ __launch_bounds__(256) void gm_w_kernel(const float* __restrict__ par
     for (int q = 0; q < KS; q++) v += P4[(int64_t)q * lda * (KP / 4) + idx];
     *reinterpret_cast<floatx4*>(W + (int64_t)iperm[idx >> 4] * KP + c) = v;
 }
+__global__ __launch_bounds__(256) void gm_w_kernel(const float* __restrict__ part, int KS, int64_t lda, int n,
+                                                   const int* __restrict__ act, const int* __restrict__ iperm,
+                                                   float* __restrict__ W) {
+    gm_w_body(blockIdx.x, part, KS, lda, n, act, iperm, W);
+}
 
 // dots u_t . u_{j+1}: one wavefront per (chunk of kGmDotRows block-order
 // positions, t <= j), lane = column; eight position pairs in flight, each
 // wavefront writes its own partial (no block reduction)
 constexpr int kGmDotRows = 256;
-__global__ __launch_bounds__(256) void gm_dots_kernel(int n, const int* __restrict__ act, const float* __restrict__ U,
-                                                      int64_t us, int j, double* __restrict__ dpart) {
+__device__ inline void gm_dots_body(int bx, int by, int nchunk, int n, const int* __restrict__ act,
+                                    const float* __restrict__ U, int64_t us, int j, double* __restrict__ dpart) {
     const int c = threadIdx.x & 63, g = threadIdx.x >> 6;
-    const int t = blockIdx.y * 4 + g;
+    const int t = by * 4 + g;
     if (t > j) return;  // whole wavefront
-    const int64_t chunk = blockIdx.x;
+    const int64_t chunk = bx;
     double acc = 0;
     if (act[c]) {
         const float* ut = U + (int64_t)t * us + c;
@@ -1201,21 +1217,29 @@ __global__ __launch_bounds__(256) void gm_dots_kernel(int n, const int* __restri
         }
         for (; p < p1; p++) acc += (double)ut[p * KP] * w[p * KP];
     }
-    dpart[((int64_t)t * gridDim.x + chunk) * KP + c] = acc;
+    dpart[((int64_t)t * nchunk + chunk) * KP + c] = acc;
+}
+__global__ __launch_bounds__(256) void gm_dots_kernel(int n, const int* __restrict__ act, const float* __restrict__ U,
+                                                      int64_t us, int j, double* __restrict__ dpart) {
+    gm_dots_body(blockIdx.x, blockIdx.y, gridDim.x, n, act, U, us, j, dpart);
 }
 
 // Hessenberg entries h_{q,j} = (u_q . w) / s_q, one quantity q per block
-__global__ __launch_bounds__(1024) void gm_hcol_kernel(const double* __restrict__ dpart, int nblk,
-                                                       const int* __restrict__ act, int j, double* __restrict__ st,
-                                                       float* __restrict__ coef) {
-    __shared__ double sh[1024];
-    const int q = blockIdx.x;
+__device__ inline void gm_hcol_body(int q, double* sh, const double* __restrict__ dpart, int nblk,
+                                    const int* __restrict__ act, int j, double* __restrict__ st,
+                                    float* __restrict__ coef) {
     const double t = colsum1024(dpart + (int64_t)q * nblk * KP, nblk, sh);
     if (threadIdx.x >= KP || !act[threadIdx.x]) return;
     double* S = st + threadIdx.x * kGmStride;
     const double h = t / S[GM_S + q];
     S[GM_H + j * (kGmMax + 1) + q] = h;
     coef[q * KP + threadIdx.x] = (float)(h / S[GM_S + q]);
+}
+__global__ __launch_bounds__(1024) void gm_hcol_kernel(const double* __restrict__ dpart, int nblk,
+                                                       const int* __restrict__ act, int j, double* __restrict__ st,
+                                                       float* __restrict__ coef) {
+    __shared__ double sh[1024];
+    gm_hcol_body(blockIdx.x, sh, dpart, nblk, act, j, st, coef);
 }
 
 // u_{j+1} = w - sum_{q <= j} h_{q,j} v_q on the active columns (coef[t][c] =
@@ -1225,14 +1249,12 @@ __global__ __launch_bounds__(1024) void gm_hcol_kernel(const double* __restrict_
 // workgroup
 constexpr int kGmUpPos = 8;
 constexpr int kGmUpRounds = 2;
-__global__ __launch_bounds__(256) void gm_update_kernel(int n, const int* __restrict__ act, float* __restrict__ U,
-                                                        int64_t us, int j, const float* __restrict__ coef,
-                                                        double* __restrict__ dpart) {
-    __shared__ double sh[kT];
+__device__ inline void gm_update_body(int bx, double* sh, int n, const int* __restrict__ act, float* __restrict__ U,
+                                      int64_t us, int j, const float* __restrict__ coef, double* __restrict__ dpart) {
     const int c = threadIdx.x & 63, g = threadIdx.x >> 6;
     double acc = 0;
     for (int round = 0; round < kGmUpRounds && act[c]; round++) {
-        const int64_t p0 = (((int64_t)blockIdx.x * kGmUpRounds + round) * 4 + g) * kGmUpPos;
+        const int64_t p0 = (((int64_t)bx * kGmUpRounds + round) * 4 + g) * kGmUpPos;
         float* w = U + (int64_t)(j + 1) * us + c;
         float v[kGmUpPos];
 #pragma unroll
@@ -1266,19 +1288,21 @@ __global__ __launch_bounds__(256) void gm_update_kernel(int n, const int* __rest
                 acc += (double)v[k] * v[k];
             }
     }
-    col_partial(acc, sh, dpart + (int64_t)blockIdx.x * KP);
+    col_partial(acc, sh, dpart + (int64_t)bx * KP);
+}
+__global__ __launch_bounds__(256) void gm_update_kernel(int n, const int* __restrict__ act, float* __restrict__ U,
+                                                        int64_t us, int j, const float* __restrict__ coef,
+                                                        double* __restrict__ dpart) {
+    __shared__ double sh[kT];
+    gm_update_body(blockIdx.x, sh, n, act, U, us, j, coef, dpart);
 }
 
 // h_{j+1,j} = ||u_{j+1}||; the earlier rotations and a new one applied to
 // column j; g rotated; a column stops when |g_{j+1}| <= tol s_0, at the
 // Krylov bound m, or on a breakdown (h_{j+1,j} = 0: the solution is exact);
 // tol is the column's sc[Q_ITOL]
-__global__ __launch_bounds__(1024) void gm_givens_kernel(const double* __restrict__ dpart, int nblk,
-                                                         int* __restrict__ act, int j, int m,
-                                                         const double* __restrict__ sc, double* __restrict__ st) {
-    __shared__ double sh[1024];
-    const double t = colsum1024(dpart, nblk, sh);
-    if (threadIdx.x >= KP || !act[threadIdx.x]) return;
+__device__ inline void gm_givens_col(double t, int* __restrict__ act, int j, int m, const double* __restrict__ sc,
+                                     double* __restrict__ st) {
     double* S = st + threadIdx.x * kGmStride;
     double* h = S + GM_H + j * (kGmMax + 1);
     const double hn = sqrt(t);
@@ -1302,13 +1326,33 @@ __global__ __launch_bounds__(1024) void gm_givens_kernel(const double* __restric
     const double tol = sc[threadIdx.x * Q_NQ + Q_ITOL];
     if (!(fabs(S[GM_G + j + 1]) > tol * S[GM_S]) || j + 1 >= m || !(hn > 0)) act[threadIdx.x] = 0;
 }
+// gflag (or null): set to whether any column is still active (wavefront 0
+// holds the 64 columns)
+__device__ inline void gm_givens_body(double* sh, const double* __restrict__ dpart, int nblk, int* __restrict__ act,
+                                      int j, int m, const double* __restrict__ sc, double* __restrict__ st,
+                                      int* __restrict__ gflag) {
+    const double t = colsum1024(dpart, nblk, sh);
+    if (threadIdx.x < KP && act[threadIdx.x]) gm_givens_col(t, act, j, m, sc, st);
+    if (gflag && threadIdx.x < KP) {
+        const bool any = __ballot(act[threadIdx.x] != 0) != 0;
+        if (threadIdx.x == 0) *gflag = any ? 1 : 0;
+    }
+}
+__global__ __launch_bounds__(1024) void gm_givens_kernel(const double* __restrict__ dpart, int nblk,
+                                                         int* __restrict__ act, int j, int m,
+                                                         const double* __restrict__ sc, double* __restrict__ st) {
+    __shared__ double sh[1024];
+    gm_givens_body(sh, dpart, nblk, act, j, m, sc, st, nullptr);
+}
 
-// y = H^-1 g over the iterations each column took (upper triangular)
-__global__ void gm_backsub_kernel(double* __restrict__ st) {
+// y = H^-1 g over the iterations each column took (upper triangular); the
+// count of those iterations goes to sc[Q_GMNJ] for the host
+__global__ void gm_backsub_kernel(double* __restrict__ st, double* __restrict__ sc) {
     const int c = threadIdx.x;
     if (c >= KP) return;
     double* S = st + c * kGmStride;
     const int nj = (int)S[GM_NJ];
+    sc[c * Q_NQ + Q_GMNJ] = nj;
     for (int i = nj - 1; i >= 0; i--) {
         double v = S[GM_G + i];
         for (int k = i + 1; k < nj; k++) v -= S[GM_H + k * (kGmMax + 1) + i] * S[GM_Y + k];
@@ -1604,17 +1648,14 @@ __global__ __launch_bounds__(256) void bj_pack_bf_kernel(const float* __restrict
 // wavefronts take every fourth K chunk of 16 (each multivector load feeds the
 // RT tiles) and add their accumulators through LDS.
 template <bool LO, int RT>
-__global__ __launch_bounds__(256) void bj_apply_bf_kernel(const int* __restrict__ perm, const int* __restrict__ ptr,
-                                                          const int64_t* __restrict__ poff,
-                                                          const float* __restrict__ pk,
-                                                          const float* __restrict__ PP, float* __restrict__ Y,
-                                                          const double* __restrict__ cscale, int cstride,
-                                                          const int* __restrict__ gate) {
+__device__ inline void bj_apply_bf_body(int bx, int q, float (&red)[4][RT * 32][64], const int* __restrict__ perm,
+                                        const int* __restrict__ ptr, const int64_t* __restrict__ poff,
+                                        const float* __restrict__ pk, const float* __restrict__ PP,
+                                        float* __restrict__ Y, const double* __restrict__ cscale, int cstride,
+                                        const int* __restrict__ gate) {
     if (gate && gate_idle(gate)) return;
-    __shared__ float red[4][RT * 32][64];
-    const int q = blockIdx.y;
     const int b0 = ptr[q], b = ptr[q + 1] - b0;
-    const int rt0 = blockIdx.x * RT;
+    const int rt0 = bx * RT;
     if (rt0 * 32 >= b) return;  // whole workgroup, before any barrier
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int cl = lane & 31, kh = lane >> 5;
@@ -1707,6 +1748,128 @@ __global__ __launch_bounds__(256) void bj_apply_bf_kernel(const int* __restrict_
                 Y[(int64_t)pq[rr] * KP + 32 + cl] = v * f1;
         }
     }
+}
+template <bool LO, int RT>
+__global__ __launch_bounds__(256) void bj_apply_bf_kernel(const int* __restrict__ perm, const int* __restrict__ ptr,
+                                                          const int64_t* __restrict__ poff,
+                                                          const float* __restrict__ pk,
+                                                          const float* __restrict__ PP, float* __restrict__ Y,
+                                                          const double* __restrict__ cscale, int cstride,
+                                                          const int* __restrict__ gate) {
+    __shared__ float red[4][RT * 32][64];
+    bj_apply_bf_body<LO, RT>(blockIdx.x, blockIdx.y, red, perm, ptr, poff, pk, PP, Y, cscale, cstride, gate);
+}
+
+// ---------------------------------------------------------------- groups
+// One GMRES iteration stage for several geometries in one launch: workgroup
+// id -> (geometry p, its own workgroup id) through the prefix sums of the
+// stage's per-geometry grids, then the same body as the single-geometry
+// kernel on that geometry's arrays.  Each geometry's workgroups do exactly
+// what its own launch would, so its results do not depend on the group.
+}  // namespace
+
+struct GmProb {
+    int n, nkb, kbchunk, kslices, nslices, ngemm, variant, m, ndc, nuc, bj_gx, bj_nblocks, bj_lo;
+    int64_t lda, plane, us;
+    const uint4* apk;
+    uint4* ppk;
+    float* part32;
+    const int* tptr;
+    const int* tkb;
+    float* gm_u;
+    double* gm_st;
+    double* gm_dpart;
+    float* gm_coef;
+    double* dpart;
+    int* act;
+    const double* sc;
+    const int* iperm;
+    float* y;
+    const int* bj_perm;
+    const int* bj_ptr;
+    const int64_t* bj_poff;
+    const float* bj_pk;
+    int* gflag;  // any column of this geometry still active after the iteration
+};
+
+namespace {
+
+struct GroupLaunch {
+    const GmProb* probs;
+    int G;  // geometries in this launch
+    int j;  // Krylov iteration
+    int idx[kMaxGroup];      // launch slot -> index into probs
+    int pre[kMaxGroup + 1];  // workgroup prefix of the slots
+};
+
+__device__ inline const GmProb& group_of(const GroupLaunch& L, int& local) {
+    const int bid = blockIdx.x;
+    int p = 0;
+    while (p + 1 < L.G && bid >= L.pre[p + 1]) p++;
+    local = bid - L.pre[p];
+    return L.probs[L.idx[p]];
+}
+
+__global__ __launch_bounds__(256) void grp_bj_kernel(GroupLaunch L) {
+    __shared__ float red[4][64][64];
+    int l;
+    const GmProb& P = group_of(L, l);
+    const float* PP = P.gm_u + (int64_t)L.j * P.us;
+    const double* cs = P.gm_st + GM_S + L.j;
+    if (P.bj_lo)
+        bj_apply_bf_body<true, 2>(l % P.bj_gx, l / P.bj_gx, red, P.bj_perm, P.bj_ptr, P.bj_poff, P.bj_pk, PP, P.y, cs,
+                                  kGmStride, P.act);
+    else
+        bj_apply_bf_body<false, 2>(l % P.bj_gx, l / P.bj_gx, red, P.bj_perm, P.bj_ptr, P.bj_poff, P.bj_pk, PP, P.y,
+                                   cs, kGmStride, P.act);
+}
+
+__global__ __launch_bounds__(256) void grp_pack_kernel(GroupLaunch L) {
+    int l;
+    const GmProb& P = group_of(L, l);
+    pack_p_body(l, P.y, P.n, P.nkb, P.ppk, P.act);
+}
+
+// every slot of one launch has the product variant of (RT, S)
+template <int RT, int S>
+__global__ __launch_bounds__(256) void grp_gemm_kernel(GroupLaunch L) {
+    int l;
+    const GmProb& P = group_of(L, l);
+    gemm_bf16x2_s_body<RT, S>(l, P.apk, P.nkb, P.ppk, P.part32, P.lda, P.kbchunk, P.kslices, P.ngemm, P.plane, P.tptr,
+                              P.tkb, P.act);
+}
+
+__global__ __launch_bounds__(256) void grp_w_kernel(GroupLaunch L) {
+    int l;
+    const GmProb& P = group_of(L, l);
+    gm_w_body(l, P.part32, P.nslices, P.lda, P.n, P.act, P.iperm, P.gm_u + (int64_t)(L.j + 1) * P.us);
+}
+
+__global__ __launch_bounds__(256) void grp_dots_kernel(GroupLaunch L) {
+    int l;
+    const GmProb& P = group_of(L, l);
+    gm_dots_body(l % P.ndc, l / P.ndc, P.ndc, P.n, P.act, P.gm_u, P.us, L.j, P.gm_dpart);
+}
+
+__global__ __launch_bounds__(1024) void grp_hcol_kernel(GroupLaunch L) {
+    __shared__ double sh[1024];
+    int l;
+    const GmProb& P = group_of(L, l);
+    gm_hcol_body(l, sh, P.gm_dpart, P.ndc, P.act, L.j, P.gm_st, P.gm_coef);
+}
+
+__global__ __launch_bounds__(256) void grp_update_kernel(GroupLaunch L) {
+    __shared__ double sh[kT];
+    int l;
+    const GmProb& P = group_of(L, l);
+    gm_update_body(l, sh, P.n, P.act, P.gm_u, P.us, L.j, P.gm_coef, P.dpart);
+}
+
+__global__ __launch_bounds__(1024) void grp_givens_kernel(GroupLaunch L) {
+    __shared__ double sh[1024];
+    int l;
+    const GmProb& P = group_of(L, l);
+    gm_givens_body(sh, P.dpart, P.nuc, P.act, L.j, P.m, P.sc, P.gm_st, P.gflag);
 }
 
 inline unsigned nblocks(int64_t work) { return (unsigned)((work + kT - 1) / kT); }
@@ -1844,6 +2007,9 @@ BatchedSolver::BatchedSolver(int n, int64_t lda, int capacity, void* ws, void* h
         const int v = std::atoi(e);
         if (v >= 4 && v <= kGmMax) gm_m_ = v;
     }
+    // CPGPU_GRAPH=1 replays the GMRES iterations from captured graphs
+    // (default: launched kernel by kernel, which measured faster)
+    if (const char* e = std::getenv("CPGPU_GRAPH")) graphs_ = e[0] == '1';
     // CPGPU_NO_BF16X2=1 keeps the bf16x2 product out of the tuner's candidates
     if (const char* e = std::getenv("CPGPU_NO_BF16X2")) bf16x2_ = e[0] != '1';
     // CPGPU_BJ_VALU=1 selects the vector-ALU block-Jacobi product (tests)
@@ -1880,23 +2046,39 @@ BatchedSolver::BatchedSolver(int n, int64_t lda, int capacity, void* ws, void* h
     }
 }
 
+BatchedSolver::IterPlan::~IterPlan() {
+    for (auto& it : items)
+        if (it.first) (void)hipGraphExecDestroy(it.first);
+}
+
+// the owner synchronises the stream before a solver is destroyed, and the
+// cache is cleared only between sweeps (after the residual's synchronisation),
+// so no replay of these graphs is in flight
+void BatchedSolver::gm_plan_clear() {
+    gm_plans_.clear();
+    gm_plan_key_.clear();
+}
+
 BatchedSolver::~BatchedSolver() {
+    gm_plan_clear();
     for (hipEvent_t& e : act_ev_)
         if (e) (void)hipEventDestroy(e);
 }
 
-void BatchedSolver::launch_gemm32(const GemmShape& sh, const float* A32, const float* P, hipStream_t s) {
+void BatchedSolver::launch_gemm32(const GemmShape& sh, const float* A32, const float* P, hipStream_t s, int parts) {
     const int KS = sh.kslices;
     int kchunk = (int)round_up((n_ + KS - 1) / KS, 16);
+    if (sh.variant >= 5 && (parts & 1)) {  // the bf16 variants read P as hi/lo tiles
+        if (!apk_) throw HipError{hipErrorInvalidValue, "bf16 GEMM without packed A"};
+        TimedScope tp("pack_p", s);
+        hipLaunchKernelGGL(pack_p_bf16x2_kernel, dim3((unsigned)((2 * (int64_t)nkb_ * 64 + 255) / 256)), dim3(256), 0,
+                           s, P, n_, nkb_, (uint4*)ppk_, gate_);
+        CP_LAUNCH_CHECK();
+    }
+    if (!(parts & 2)) return;
     if (sh.variant == 5) {  // bf16x3 on packed tiles
         if (!apk_) throw HipError{hipErrorInvalidValue, "bf16x3 GEMM without packed A"};
         const int nkb = nkb_;
-        {
-            TimedScope tp("pack_p", s);
-            hipLaunchKernelGGL(pack_p_bf16x2_kernel, dim3((unsigned)((2 * (int64_t)nkb * 64 + 255) / 256)),
-                               dim3(256), 0, s, P, n_, nkb, (uint4*)ppk_, gate_);
-        }
-        CP_LAUNCH_CHECK();
         const int kbchunk = (nkb + KS - 1) / KS;
         dim3 grid((unsigned)((lda_ / 512) * KS));
         TimedScope ts("gemm_f32", s);
@@ -1908,12 +2090,6 @@ void BatchedSolver::launch_gemm32(const GemmShape& sh, const float* A32, const f
     if (sh.variant >= 6 && sh.variant <= 9) {  // bf16x3, RT row tiles, S stages
         if (!apk_) throw HipError{hipErrorInvalidValue, "bf16x3 GEMM without packed A"};
         const int nkb = nkb_;
-        {
-            TimedScope tp("pack_p", s);
-            hipLaunchKernelGGL(pack_p_bf16x2_kernel, dim3((unsigned)((2 * (int64_t)nkb * 64 + 255) / 256)),
-                               dim3(256), 0, s, P, n_, nkb, (uint4*)ppk_, gate_);
-        }
-        CP_LAUNCH_CHECK();
         const int kbchunk = (nkb + KS - 1) / KS;
         const int rows_wg = (sh.variant == 6 || sh.variant == 7) ? 256 : 512;
         dim3 grid((unsigned)((lda_ / rows_wg) * KS));
@@ -1945,12 +2121,6 @@ void BatchedSolver::launch_gemm32(const GemmShape& sh, const float* A32, const f
         if (!apk_ || !tptr_)
             throw HipError{hipErrorInvalidValue, "bf16x2 GEMM without packed A or near-field tile lists"};
         const int nkb = nkb_;
-        {
-            TimedScope tp("pack_p", s);
-            hipLaunchKernelGGL(pack_p_bf16x2_kernel, dim3((unsigned)((2 * (int64_t)nkb * 64 + 255) / 256)),
-                               dim3(256), 0, s, P, n_, nkb, (uint4*)ppk_, gate_);
-        }
-        CP_LAUNCH_CHECK();
         const int kbchunk = (nkb + KS - 1) / KS;
         const int rows_wg = (sh.variant == 13 || sh.variant == 14) ? 256 : 512;
         const int ngemm = (int)((lda_ / rows_wg) * KS);
@@ -2190,29 +2360,62 @@ void BatchedSolver::apply_block_jacobi(const float* P, float* Y, hipStream_t s) 
     CP_LAUNCH_CHECK();
 }
 
-// One refinement sweep's inner solve by right-preconditioned GMRES (see the
-// gm_* kernels): R / |R| on the sweeping columns (act_), at most gm_m_
-// iterations, d_ = M^-1 V y on return.  The host reads iteration it's active
-// flags while iteration it is queued behind them, as in the BiCGSTAB loop.
-void BatchedSolver::gmres_sweep(const float* A32, const BatchedOpts& o, int kc, SolveStats* stats, hipStream_t s) {
-    const int64_t us = lda_ * KP;
+int BatchedSolver::gm_bound(const BatchedOpts& o) const {
     int m = gm_m_;
     if (o.max_iter > 0) m = std::min(m, o.max_iter);
+    return m;
+}
+
+bool BatchedSolver::gmres_ready() const { return gmres_ && bj_ && bj_mfma_ && bj_iperm_; }
+
+bool BatchedSolver::group_capable() const {
+    return gmres_ready() && bj_kind_ == kBjPackBf16 && bj_rt_ == 2 && shape_.variant >= 10 && shape_.variant <= 14 &&
+           apk_ && tptr_;
+}
+
+// One refinement sweep's inner solve by right-preconditioned GMRES (see the
+// gm_* kernels): gm_begin forms u_0 = R / |R| on the sweeping columns (act_),
+// gm_iterations runs at most gm_bound() iterations, gm_end leaves
+// d_ = M^-1 V y and each column's iteration count in sc_[Q_GMNJ].
+void BatchedSolver::gm_begin(hipStream_t s) {
     hipLaunchKernelGGL(gm_init_kernel, dim3((unsigned)nrb_), dim3(kT), 0, s, n_, (const double*)R_,
                        (const double*)sc_, (const int*)act_, bj_iperm_, gm_u_, dpart_);
     hipLaunchKernelGGL(gm_start_kernel, dim3(1), dim3(1024), 0, s, (const double*)dpart_, nrb_, gm_st_);
     CP_LAUNCH_CHECK();
+}
+
+void BatchedSolver::gm_end(hipStream_t s) {
+    // d = M^-1 sum_q y_q v_q
+    hipLaunchKernelGGL(gm_backsub_kernel, dim3(1), dim3(KP), 0, s, gm_st_, sc_);
+    hipLaunchKernelGGL(gm_combine_kernel, dim3(nblocks((int64_t)n_ * KP)), dim3(kT), 0, s, n_, (const float*)gm_u_,
+                       lda_ * KP, (const double*)gm_st_, pp_);
+    CP_LAUNCH_CHECK();
+    apply_block_jacobi(nullptr, d_, s);
+    gm_counted_ = true;
+}
+
+// The host reads iteration it's active flags while iteration it is queued
+// behind them, as in the BiCGSTAB loop.
+void BatchedSolver::gm_iterations(const float* A32, const BatchedOpts& o, hipStream_t s) {
+    const int64_t us = lda_ * KP;
+    const int m = gm_bound(o);
     int* hact[2] = {host_act_, host_act_ + 2 * KP};  // host_act_ + KP holds the sweep flags
     auto top = [&](int k) {
         CP_HIP(hipMemcpyAsync(hact[k & 1], act_, KP * sizeof(int), hipMemcpyDeviceToHost, s));
         CP_HIP(hipEventRecord(act_ev_[k & 1], s));
     };
-    auto body = [&](int j) {
-        {  // z = M^-1 v_j (row order)
+    // iteration j in four segments: 0 z = M^-1 v_j (row order), 1 packing of
+    // z, 2 the product w = A~ z, 3 Gram-Schmidt, Hessenberg column, Givens
+    auto segment = [&](int j, int k) {
+        if (k == 0) {
             TimedScope tb("bj_apply", s);
             bj_apply_pk((const float*)(gm_u_ + j * us), y_, (const double*)(gm_st_ + GM_S + j), kGmStride, s);
+            return;
         }
-        matvec32(A32, y_, nullptr, s);  // w = A~ z
+        if (k == 1 || k == 2) {
+            launch_gemm32(shape_, A32, y_, s, k);
+            return;
+        }
         const int KS = slices(shape_);
         const int ndc = (int)((n_ + kGmDotRows - 1) / kGmDotRows);     // position chunks of the dots
         const int nuc = (int)((n_ + 4 * kGmUpRounds * kGmUpPos - 1) / (4 * kGmUpRounds * kGmUpPos));  // update WGs
@@ -2228,13 +2431,94 @@ void BatchedSolver::gmres_sweep(const float* A32, const BatchedOpts& o, int kc, 
                            (const double*)sc_, gm_st_);
         CP_LAUNCH_CHECK();
     };
+    auto body = [&](int j) {
+        for (int k = 0; k < 4; k++) segment(j, k);
+    };
+    // iteration j's launches depend only on j and on state fixed for the
+    // solver's life (or named in the key), so they are captured once as
+    // graphs and replayed: at most three launch calls per iteration instead
+    // of eight (one when nothing is timed)
+    const bool use_graph = graphs_;
+    unsigned timed = 0;  // segments launched directly so that their timer records them
+    if (KernelTimer* tm = current_timer())
+        timed = (unsigned)tm->wants("bj_apply") | (unsigned)tm->wants("pack_p") << 1 |
+                (unsigned)tm->wants("gemm_f32") << 2;
+    if (use_graph) {
+        const std::vector<uintptr_t> key = {(uintptr_t)A32, (uintptr_t)shape_.variant, (uintptr_t)shape_.rt,
+                                            (uintptr_t)shape_.kslices, (uintptr_t)bj_level_, (uintptr_t)bj_lo_,
+                                            (uintptr_t)bj_rt_, (uintptr_t)bj_kind_, (uintptr_t)bj_u_,
+                                            (uintptr_t)bj_maxb_, (uintptr_t)bj_nblocks_, (uintptr_t)bj_perm_,
+                                            (uintptr_t)bj_ptr_, (uintptr_t)bj_poff_, (uintptr_t)bj_pk_,
+                                            (uintptr_t)bj_iperm_, (uintptr_t)apk_, (uintptr_t)tptr_,
+                                            (uintptr_t)tkb_, (uintptr_t)m, (uintptr_t)timed};
+        if (key != gm_plan_key_) {
+            gm_plan_clear();
+            gm_plan_key_ = key;
+        }
+    }
+    // captures segments k0..k1-1 of iteration j; null when the runtime
+    // refuses (the solver then launches on the stream from here on)
+    auto capture = [&](int j, int k0, int k1) -> hipGraphExec_t {
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        CP_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+        bool captured = false;
+        std::string what;
+        try {
+            for (int k = k0; k < k1; k++) segment(j, k);
+            captured = true;
+        } catch (const HipError& err) {
+            what = err.where;
+        }
+        hipError_t e = hipStreamEndCapture(s, &graph);
+        if (captured && e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (captured && e == hipSuccess) return exec;
+        (void)hipGetLastError();
+        std::fprintf(stderr, "cpgpu: GMRES iteration graph not captured (%s%s%s); launching on the stream\n",
+                     hipGetErrorString(e), what.empty() ? "" : ", ", what.c_str());
+        graphs_ = false;
+        return nullptr;
+    };
+    auto run = [&](int j) {
+        if (!use_graph || !graphs_) {
+            body(j);
+            return;
+        }
+        if ((int)gm_plans_.size() <= j) gm_plans_.resize(j + 1);
+        std::unique_ptr<IterPlan>& P = gm_plans_[j];
+        if (!P) {
+            std::unique_ptr<IterPlan> np(new IterPlan);
+            int k0 = 0;
+            for (int k = 0; k <= 4; k++) {
+                if (k < 4 && !(k < 3 && (timed >> k & 1u))) continue;  // segment k joins the open graph
+                if (k > k0) {
+                    hipGraphExec_t ex = capture(j, k0, k);
+                    if (!ex) {  // refused: nothing of iteration j has run yet
+                        body(j);
+                        return;
+                    }
+                    np->items.emplace_back(ex, -1);
+                }
+                if (k < 4) np->items.emplace_back(nullptr, k);
+                k0 = k + 1;
+            }
+            P = std::move(np);
+        }
+        for (const auto& it : P->items) {
+            if (it.first)
+                CP_HIP(hipGraphLaunch(it.first, s));
+            else
+                segment(j, it.second);
+        }
+    };
     gate_ = act_;
     top(0);
     int it = 0;
     for (;;) {
         const bool more = it < m;
         if (more) {
-            body(it);
+            run(it);
             top(it + 1);
         }
         CP_HIP(hipEventSynchronize(act_ev_[it & 1]));
@@ -2242,22 +2526,17 @@ void BatchedSolver::gmres_sweep(const float* A32, const BatchedOpts& o, int kc, 
         int nact = 0;
         for (int c = 0; c < KP; c++) nact += fl[c] != 0;
         if (!more || nact == 0) {
+            // the iteration queued behind this check found no active column:
+            // its launches did no work, so they are not timed
             if (more)
                 if (KernelTimer* t = current_timer())
-                    for (const char* nm : {"gemm_f32", "pack_p", "bj_apply"}) t->discard(nm, 1);
+                    for (const char* nm : {"gemm_f32", "pack_p", "bj_apply"})
+                        if (t->wants(nm)) t->discard(nm, 1);
             break;
         }
-        for (int c = 0; c < kc; c++)
-            if (fl[c]) stats[c].iters++;
         ++it;
     }
     gate_ = nullptr;
-    // d = M^-1 sum_q y_q v_q
-    hipLaunchKernelGGL(gm_backsub_kernel, dim3(1), dim3(KP), 0, s, gm_st_);
-    hipLaunchKernelGGL(gm_combine_kernel, dim3(nblocks((int64_t)n_ * KP)), dim3(kT), 0, s, n_, (const float*)gm_u_,
-                       us, (const double*)gm_st_, pp_);
-    CP_LAUNCH_CHECK();
-    apply_block_jacobi(nullptr, d_, s);
 }
 
 void BatchedSolver::solve(const float* A32, const float* B32, const double* normals, const double* vinf4, int k,
@@ -2279,196 +2558,473 @@ void BatchedSolver::solve_rhs(const float* A32, const float* B32, const double* 
     }
 }
 
-void BatchedSolver::tile_solve(const float* A32, const float* B32, const double* normals, const double* vinf4,
-                               int c0, int kc, const BatchedOpts& o, double* x, int64_t ldx, SolveStats* stats,
-                               hipStream_t s, const double* sig_cm, const double* add_cm, const double* x0_cm) {
-    (void)c0;
-    const int64_t rows = lda_;
-    const unsigned nel = nblocks(rows * KP);
-    int* host_sweep = host_act_ + KP;
+// ------------------------------------------------ refinement loop phases
+// One 64-column tile's solve is: tile_begin (right-hand side, initial guess,
+// |b|^2 read-back), tile_begin_host after the stream synchronised, then per
+// sweep tile_residual + synchronisation + tile_check (which columns sweep),
+// tile_sweep_open, the inner solve, tile_sweep_close; finally tile_end.
+// tile_solve runs them for one geometry, solve_group for several.
 
+void BatchedSolver::colsum_q(const double* dp, int nq, int q0, int q1, int op, hipStream_t s) {
+    hipLaunchKernelGGL(colsum_kernel, dim3(1), dim3(1024), 0, s, dp, nrb_, nq, q0, q1, sc_, op, act_);
+    CP_LAUNCH_CHECK();
+}
+
+void BatchedSolver::tile_begin(Tile& t, hipStream_t s) {
+    const unsigned nel = nblocks(lda_ * KP);
     // RHS b = add - B sigma with FP64 accumulation (B: the FP64 matrix, or
     // B32 + the near-field correction)
-    if (!sig_cm && !add_cm) {
+    if (!t.sig_cm && !t.add_cm) {
         // steady sources sigma_c = -Vinf_c . n: b_c is a combination of the
         // three columns -B n_x, -B n_y, -B n_z, computed once per matrix
         const int mode = b64_ ? 1 : 0;
         if (bn_mode_ != mode) {
-            hipLaunchKernelGGL(normal_cols_kernel, dim3(nel), dim3(kT), 0, s, n_, lda_, normals, S_);
-            matvec64acc(B32, S_, R_, nullptr, -1.0, s, near_dB_, b64_);
+            hipLaunchKernelGGL(normal_cols_kernel, dim3(nel), dim3(kT), 0, s, n_, lda_, t.normals, S_);
+            matvec64acc(t.B32, S_, R_, nullptr, -1.0, s, near_dB_, b64_);
             hipLaunchKernelGGL(keep3_kernel, dim3(nblocks(lda_)), dim3(kT), 0, s, lda_, (const double*)R_, bn_);
             CP_LAUNCH_CHECK();
             bn_mode_ = mode;
         }
-        hipLaunchKernelGGL(rhs_bn_kernel, dim3(nel), dim3(kT), 0, s, n_, lda_, kc, (const double*)bn_, vinf4, R_);
+        hipLaunchKernelGGL(rhs_bn_kernel, dim3(nel), dim3(kT), 0, s, n_, lda_, t.kc, (const double*)bn_, t.vinf4, R_);
         copy64(R_, Bv_, s);
     } else {
-        if (sig_cm)
-            hipLaunchKernelGGL(gather_cols_kernel, dim3(nel), dim3(kT), 0, s, n_, lda_, kc, sig_cm, ldx, S_);
+        if (t.sig_cm)
+            hipLaunchKernelGGL(gather_cols_kernel, dim3(nel), dim3(kT), 0, s, n_, lda_, t.kc, t.sig_cm, t.ldx, S_);
         else
-            hipLaunchKernelGGL(sigma_rows_kernel, dim3(nel), dim3(kT), 0, s, n_, lda_, kc, normals, vinf4, S_);
-        if (add_cm)
-            hipLaunchKernelGGL(gather_cols_kernel, dim3(nel), dim3(kT), 0, s, n_, lda_, kc, add_cm, ldx, R_);
-        matvec64acc(B32, S_, Bv_, add_cm ? R_ : nullptr, -1.0, s, near_dB_, b64_);
+            hipLaunchKernelGGL(sigma_rows_kernel, dim3(nel), dim3(kT), 0, s, n_, lda_, t.kc, t.normals, t.vinf4, S_);
+        if (t.add_cm)
+            hipLaunchKernelGGL(gather_cols_kernel, dim3(nel), dim3(kT), 0, s, n_, lda_, t.kc, t.add_cm, t.ldx, R_);
+        matvec64acc(t.B32, S_, Bv_, t.add_cm ? R_ : nullptr, -1.0, s, near_dB_, b64_);
     }
-    auto colsum = [&](const double* dp, int nq, int q0, int q1, int op = COLSUM_NONE) {
-        hipLaunchKernelGGL(colsum_kernel, dim3(1), dim3(1024), 0, s, dp, nrb_, nq, q0, q1, sc_, op, act_);
-        CP_LAUNCH_CHECK();
-    };
-    colsum(dpart_, 1, Q_BB, 0);
-    hipLaunchKernelGGL(invd32_kernel, dim3(nblocks(lda_)), dim3(kT), 0, s, n_, A32, lda_, invd_);
-    if (autotune_) tune(A32, s);
-    const int KS = slices(shape_);  // partial slices the reductions sum
-    if (x0_cm)
-        hipLaunchKernelGGL(gather_cols_kernel, dim3(nel), dim3(kT), 0, s, n_, lda_, kc, x0_cm, ldx, X_);
+    colsum_q(dpart_, 1, Q_BB, 0, COLSUM_NONE, s);
+    hipLaunchKernelGGL(invd32_kernel, dim3(nblocks(lda_)), dim3(kT), 0, s, n_, t.A32, lda_, invd_);
+    if (autotune_) tune(t.A32, s);
+    if (t.x0_cm)
+        hipLaunchKernelGGL(gather_cols_kernel, dim3(nel), dim3(kT), 0, s, n_, lda_, t.kc, t.x0_cm, t.ldx, X_);
     else
         CP_HIP(hipMemsetAsync(X_, 0, (size_t)lda_ * KP * sizeof(double), s));
-    std::vector<double> bb(KP, 0.0);
     CP_HIP(hipMemcpyAsync(host_sc_, sc_, KP * Q_NQ * sizeof(double), hipMemcpyDeviceToHost, s));
-    CP_HIP(hipStreamSynchronize(s));
-    for (int c = 0; c < kc; c++) bb[c] = host_sc_[c * Q_NQ + Q_BB];
-    std::vector<int> done(KP, 1);
-    for (int c = 0; c < kc; c++) {
-        done[c] = bb[c] == 0;
-        stats[c] = SolveStats();
+}
+
+void BatchedSolver::tile_begin_host(Tile& t) {
+    t.bb.assign(KP, 0.0);
+    t.done.assign(KP, 1);
+    t.rel.assign(KP, 0.0);
+    t.prev.assign(KP, 0.0);
+    t.sweep = 0;
+    for (int c = 0; c < t.kc; c++) {
+        t.bb[c] = host_sc_[c * Q_NQ + Q_BB];
+        t.done[c] = t.bb[c] == 0;
+        t.stats[c] = SolveStats();
     }
-    const double tol2 = o.tol * o.tol;
-    int sweep = 0;
-    std::vector<double> rel(KP, 0.0), prev(KP, 0.0);
-    while (true) {
-        // residual R = b - A X in FP64 (R = b while X is still zero)
-        if (sweep == 0 && !x0_cm)
-            copy64(Bv_, R_, s);
-        else
-            matvec64acc(A32, X_, R_, Bv_, -1.0, s, near_dA_, a64_);
-        colsum(dpart_, 1, Q_RR64, 0);
-        CP_HIP(hipMemcpyAsync(host_sc_, sc_, KP * Q_NQ * sizeof(double), hipMemcpyDeviceToHost, s));
-        CP_HIP(hipStreamSynchronize(s));
-        bool any = false;
-        for (int c = 0; c < KP; c++) {
-            host_sweep[c] = 0;
-            if (c >= kc || bb[c] == 0) continue;
-            double rr = host_sc_[c * Q_NQ + Q_RR64];
-            rel[c] = std::sqrt(rr / bb[c]);
-            stats[c].rel_residual = rel[c];
-            if (rr <= tol2 * bb[c]) done[c] = 1;
-            if (!done[c]) {
-                host_sweep[c] = 1;
-                any = true;
-                host_sc_[c * Q_NQ + Q_SCALE] = std::sqrt(rr);
-                // the caller's inner reduction, or kInnerTolTarget relaxed to
-                // what this column still needs (half of tol / rel)
-                host_sc_[c * Q_NQ + Q_ITOL] = o.inner_tol > 0
-                                                  ? o.inner_tol
-                                                  : std::min(0.5, std::max(kInnerTolTarget, 0.5 * o.tol / rel[c]));
-            } else {
-                host_sc_[c * Q_NQ + Q_SCALE] = 0;
-            }
-        }
-        // a sweep whose inner solve did not halve the residual of every column
-        // it ran on: the preconditioner is too weak for this geometry (bf16
-        // rounding of an ill-conditioned block's inverse can leave the
-        // preconditioned operator with eigenvalues near zero, and GMRES then
-        // stagnates); strengthen it for the remaining sweeps and later solves
-        if (sweep > 0 && bj_ && any) {
-            bool stalled = false;
-            for (int c = 0; c < kc; c++)
-                if (host_sweep[c] && !(rel[c] <= 0.5 * prev[c])) stalled = true;
-            if (stalled) escalate_block_jacobi();
-        }
-        for (int c = 0; c < kc; c++) prev[c] = rel[c];
-        if (!any || sweep >= o.max_refine) break;
-        ++sweep;
-        // inner FP32 solve of A32 d = R / |R| on the sweeping columns
-        CP_HIP(hipMemcpyAsync(sc_, host_sc_, KP * Q_NQ * sizeof(double), hipMemcpyHostToDevice, s));
-        CP_HIP(hipMemcpyAsync(act_, host_sweep, KP * sizeof(int), hipMemcpyHostToDevice, s));
-        if (gmres_ && bj_ && bj_mfma_ && bj_iperm_) {
-            gmres_sweep(A32, o, kc, stats, s);
+    gm_counted_ = false;
+}
+
+// residual R = b - A X in FP64 (R = b while X is still zero) and its column
+// norms to the host
+void BatchedSolver::tile_residual(Tile& t, hipStream_t s) {
+    if (t.sweep == 0 && !t.x0_cm)
+        copy64(Bv_, R_, s);
+    else
+        matvec64acc(t.A32, X_, R_, Bv_, -1.0, s, near_dA_, a64_);
+    colsum_q(dpart_, 1, Q_RR64, 0, COLSUM_NONE, s);
+    CP_HIP(hipMemcpyAsync(host_sc_, sc_, KP * Q_NQ * sizeof(double), hipMemcpyDeviceToHost, s));
+}
+
+bool BatchedSolver::tile_check(Tile& t) {
+    int* host_sweep = host_act_ + KP;
+    if (gm_counted_) {  // the GMRES sweep just finished left its iteration counts
+        for (int c = 0; c < t.kc; c++) t.stats[c].iters += (int)host_sc_[c * Q_NQ + Q_GMNJ];
+        gm_counted_ = false;
+    }
+    const double tol2 = t.o.tol * t.o.tol;
+    bool any = false;
+    for (int c = 0; c < KP; c++) {
+        host_sweep[c] = 0;
+        if (c >= t.kc || t.bb[c] == 0) continue;
+        double rr = host_sc_[c * Q_NQ + Q_RR64];
+        t.rel[c] = std::sqrt(rr / t.bb[c]);
+        t.stats[c].rel_residual = t.rel[c];
+        if (rr <= tol2 * t.bb[c]) t.done[c] = 1;
+        if (!t.done[c]) {
+            host_sweep[c] = 1;
+            any = true;
+            host_sc_[c * Q_NQ + Q_SCALE] = std::sqrt(rr);
+            // the caller's inner reduction, or kInnerTolTarget relaxed to
+            // what this column still needs (half of tol / rel)
+            host_sc_[c * Q_NQ + Q_ITOL] = t.o.inner_tol > 0
+                                              ? t.o.inner_tol
+                                              : std::min(0.5, std::max(kInnerTolTarget, 0.5 * t.o.tol / t.rel[c]));
         } else {
-            hipLaunchKernelGGL(inner_init_kernel, dim3((unsigned)nrb_), dim3(kT), 0, s, n_, (const double*)R_,
-                               (const double*)sc_, (const int*)act_, r_, r0_, p_, v_, d_, dpart_);
-            colsum(dpart_, 1, Q_RR, 0);
-            hipLaunchKernelGGL(scal_init_kernel, dim3(1), dim3(KP), 0, s, sc_);
-            int it = 0;
-            const int maxit = o.max_iter > 0 ? o.max_iter : 2 * n_;
-            // The host reads iteration it's active flags while iteration it is
-            // already queued behind them, so the device never waits for the
-            // host's check.  An iteration queued past the end of its sweep finds
-            // no active column: its heavy kernels return at once (gate_) and the
-            // per-column updates are masked as for any finished column.
-            int* hact[2] = {host_act_, host_act_ + 2 * KP};  // host_act_ + KP holds the sweep flags
-            auto top = [&](int k) {  // flag read-back of iteration k (its rho/beta step ends iteration k - 1)
-                if (k == 0) hipLaunchKernelGGL(scal_beta_kernel, dim3(1), dim3(KP), 0, s, sc_, act_);
-                CP_HIP(hipMemcpyAsync(hact[k & 1], act_, KP * sizeof(int), hipMemcpyDeviceToHost, s));
-                CP_HIP(hipEventRecord(act_ev_[k & 1], s));
-            };
-            auto body = [&]() {  // one BiCGSTAB iteration on the active columns
-                const int* ip = (bj_ && bj_mfma_) ? bj_iperm_ : nullptr;
-                hipLaunchKernelGGL(inner_p_kernel, dim3(nel), dim3(kT), 0, s, rows, n_, (const double*)sc_,
-                                   (const int*)act_, (const float*)r_, p_, (const float*)v_,
-                                   bj_ ? (const float*)nullptr : (const float*)invd_, y_, ip, pp_);
-                if (bj_) apply_block_jacobi(p_, y_, s);
-                matvec32(A32, y_, nullptr, s);
-                {
-                    TimedScope tr("reduce_f32", s);
-                    hipLaunchKernelGGL(inner_v_kernel, dim3((unsigned)nrb_), dim3(kT), 0, s, (const float*)part32_, KS,
-                                       lda_, n_, (const int*)act_, v_, (const float*)r0_, dpart_);
-                }
-                colsum(dpart_, 1, Q_R0V, 0, COLSUM_ALPHA);
-                hipLaunchKernelGGL(inner_s_kernel, dim3(nel), dim3(kT), 0, s, rows, n_, (const double*)sc_,
-                                   (const int*)act_, (const float*)r_, (const float*)v_,
-                                   bj_ ? (const float*)nullptr : (const float*)invd_, sv_, z_, ip, pp_);
-                if (bj_) apply_block_jacobi(sv_, z_, s);
-                matvec32(A32, z_, nullptr, s);
-                {
-                    TimedScope tr("reduce_f32", s);
-                    hipLaunchKernelGGL(inner_t_kernel, dim3((unsigned)nrb_), dim3(kT), 0, s, (const float*)part32_, KS,
-                                       lda_, n_, (const int*)act_, t_, (const float*)sv_, dpart_,
-                                       dpart_ + (size_t)nrb_ * KP);
-                }
-                colsum(dpart_, 2, Q_TT, Q_TS, COLSUM_OMEGA);
-                hipLaunchKernelGGL(inner_x_kernel, dim3((unsigned)nrb_), dim3(kT), 0, s, n_, (const double*)sc_,
-                                   (const int*)act_, d_, (const float*)y_, (const float*)z_, r_, (const float*)sv_,
-                                   (const float*)t_, (const float*)r0_, dpart_, dpart_ + (size_t)nrb_ * KP);
-                colsum(dpart_, 2, Q_RR, Q_R0R, COLSUM_BETA);  // and the next iteration's rho/beta step
-                CP_LAUNCH_CHECK();
-            };
-            gate_ = act_;
-            top(0);
-            for (;;) {
-                const bool more = it < maxit;
-                if (more) {
-                    body();
-                    top(it + 1);
-                }
-                CP_HIP(hipEventSynchronize(act_ev_[it & 1]));
-                const int* fl = hact[it & 1];
-                int nact = 0;
-                for (int c = 0; c < KP; c++) nact += fl[c] != 0;
-                if (!more || nact == 0) {
-                    // the iteration queued behind this check found no active
-                    // column: its launches did no work, so they are not timed
-                    if (more)
-                        if (KernelTimer* t = current_timer())
-                            for (const char* nm : {"gemm_f32", "pack_p", "reduce_f32", "bj_apply"}) t->discard(nm, 2);
-                    break;
-                }
-                for (int c = 0; c < kc; c++)
-                    if (fl[c]) stats[c].iters++;
-                ++it;
-            }
-            gate_ = nullptr;
+            host_sc_[c * Q_NQ + Q_SCALE] = 0;
         }
-        // X += |R| d on the columns of this sweep
-        CP_HIP(hipMemcpyAsync(act_, host_sweep, KP * sizeof(int), hipMemcpyHostToDevice, s));
-        hipLaunchKernelGGL(correct_kernel, dim3(nel), dim3(kT), 0, s, rows, (const double*)sc_, (const int*)act_, X_,
-                           (const float*)d_);
-        CP_LAUNCH_CHECK();
-        for (int c = 0; c < kc; c++)
-            if (host_sweep[c]) stats[c].refinements++;
     }
-    hipLaunchKernelGGL(scatter_x_kernel, dim3(nel), dim3(kT), 0, s, n_, kc, (const double*)X_, x, ldx);
+    // a sweep whose inner solve did not halve the residual of every column
+    // it ran on: the preconditioner is too weak for this geometry (bf16
+    // rounding of an ill-conditioned block's inverse can leave the
+    // preconditioned operator with eigenvalues near zero, and GMRES then
+    // stagnates); strengthen it for the remaining sweeps and later solves
+    if (t.sweep > 0 && bj_ && any) {
+        bool stalled = false;
+        for (int c = 0; c < t.kc; c++)
+            if (host_sweep[c] && !(t.rel[c] <= 0.5 * t.prev[c])) stalled = true;
+        if (stalled) escalate_block_jacobi();
+    }
+    for (int c = 0; c < t.kc; c++) t.prev[c] = t.rel[c];
+    if (!any || t.sweep >= t.o.max_refine) return false;
+    ++t.sweep;
+    return true;
+}
+
+// the sweep's per-column scale and inner tolerance, and its columns
+void BatchedSolver::tile_sweep_open(Tile& t, hipStream_t s) {
+    (void)t;
+    CP_HIP(hipMemcpyAsync(sc_, host_sc_, KP * Q_NQ * sizeof(double), hipMemcpyHostToDevice, s));
+    CP_HIP(hipMemcpyAsync(act_, host_act_ + KP, KP * sizeof(int), hipMemcpyHostToDevice, s));
+}
+
+// X += |R| d on the columns of this sweep
+void BatchedSolver::tile_sweep_close(Tile& t, hipStream_t s) {
+    const int* host_sweep = host_act_ + KP;
+    CP_HIP(hipMemcpyAsync(act_, host_sweep, KP * sizeof(int), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(correct_kernel, dim3(nblocks(lda_ * KP)), dim3(kT), 0, s, lda_, (const double*)sc_,
+                       (const int*)act_, X_, (const float*)d_);
     CP_LAUNCH_CHECK();
+    for (int c = 0; c < t.kc; c++)
+        if (host_sweep[c]) t.stats[c].refinements++;
+}
+
+void BatchedSolver::tile_end(Tile& t, hipStream_t s) {
+    hipLaunchKernelGGL(scatter_x_kernel, dim3(nblocks(lda_ * KP)), dim3(kT), 0, s, n_, t.kc, (const double*)X_, t.x,
+                       t.ldx);
+    CP_LAUNCH_CHECK();
+}
+
+// inner FP32 solve of A32 d = R / |R| on the sweeping columns by BiCGSTAB
+// (point Jacobi, or the block-Jacobi product without the GMRES form)
+void BatchedSolver::bicgstab_sweep(Tile& t, hipStream_t s) {
+    const int64_t rows = lda_;
+    const unsigned nel = nblocks(rows * KP);
+    const int KS = slices(shape_);  // partial slices the reductions sum
+        hipLaunchKernelGGL(inner_init_kernel, dim3((unsigned)nrb_), dim3(kT), 0, s, n_, (const double*)R_,
+                           (const double*)sc_, (const int*)act_, r_, r0_, p_, v_, d_, dpart_);
+        colsum_q(dpart_, 1, Q_RR, 0, COLSUM_NONE, s);
+        hipLaunchKernelGGL(scal_init_kernel, dim3(1), dim3(KP), 0, s, sc_);
+        int it = 0;
+        const int maxit = t.o.max_iter > 0 ? t.o.max_iter : 2 * n_;
+        // The host reads iteration it's active flags while iteration it is
+        // already queued behind them, so the device never waits for the
+        // host's check.  An iteration queued past the end of its sweep finds
+        // no active column: its heavy kernels return at once (gate_) and the
+        // per-column updates are masked as for any finished column.
+        int* hact[2] = {host_act_, host_act_ + 2 * KP};  // host_act_ + KP holds the sweep flags
+        auto top = [&](int k) {  // flag read-back of iteration k (its rho/beta step ends iteration k - 1)
+            if (k == 0) hipLaunchKernelGGL(scal_beta_kernel, dim3(1), dim3(KP), 0, s, sc_, act_);
+            CP_HIP(hipMemcpyAsync(hact[k & 1], act_, KP * sizeof(int), hipMemcpyDeviceToHost, s));
+            CP_HIP(hipEventRecord(act_ev_[k & 1], s));
+        };
+        auto body = [&]() {  // one BiCGSTAB iteration on the active columns
+            const int* ip = (bj_ && bj_mfma_) ? bj_iperm_ : nullptr;
+            hipLaunchKernelGGL(inner_p_kernel, dim3(nel), dim3(kT), 0, s, rows, n_, (const double*)sc_,
+                               (const int*)act_, (const float*)r_, p_, (const float*)v_,
+                               bj_ ? (const float*)nullptr : (const float*)invd_, y_, ip, pp_);
+            if (bj_) apply_block_jacobi(p_, y_, s);
+            matvec32(t.A32, y_, nullptr, s);
+            {
+                TimedScope tr("reduce_f32", s);
+                hipLaunchKernelGGL(inner_v_kernel, dim3((unsigned)nrb_), dim3(kT), 0, s, (const float*)part32_, KS,
+                                   lda_, n_, (const int*)act_, v_, (const float*)r0_, dpart_);
+            }
+            colsum_q(dpart_, 1, Q_R0V, 0, COLSUM_ALPHA, s);
+            hipLaunchKernelGGL(inner_s_kernel, dim3(nel), dim3(kT), 0, s, rows, n_, (const double*)sc_,
+                               (const int*)act_, (const float*)r_, (const float*)v_,
+                               bj_ ? (const float*)nullptr : (const float*)invd_, sv_, z_, ip, pp_);
+            if (bj_) apply_block_jacobi(sv_, z_, s);
+            matvec32(t.A32, z_, nullptr, s);
+            {
+                TimedScope tr("reduce_f32", s);
+                hipLaunchKernelGGL(inner_t_kernel, dim3((unsigned)nrb_), dim3(kT), 0, s, (const float*)part32_, KS,
+                                   lda_, n_, (const int*)act_, t_, (const float*)sv_, dpart_,
+                                   dpart_ + (size_t)nrb_ * KP);
+            }
+            colsum_q(dpart_, 2, Q_TT, Q_TS, COLSUM_OMEGA, s);
+            hipLaunchKernelGGL(inner_x_kernel, dim3((unsigned)nrb_), dim3(kT), 0, s, n_, (const double*)sc_,
+                               (const int*)act_, d_, (const float*)y_, (const float*)z_, r_, (const float*)sv_,
+                               (const float*)t_, (const float*)r0_, dpart_, dpart_ + (size_t)nrb_ * KP);
+            colsum_q(dpart_, 2, Q_RR, Q_R0R, COLSUM_BETA, s);  // and the next iteration's rho/beta step
+            CP_LAUNCH_CHECK();
+        };
+        gate_ = act_;
+        top(0);
+        for (;;) {
+            const bool more = it < maxit;
+            if (more) {
+                body();
+                top(it + 1);
+            }
+            CP_HIP(hipEventSynchronize(act_ev_[it & 1]));
+            const int* fl = hact[it & 1];
+            int nact = 0;
+            for (int c = 0; c < KP; c++) nact += fl[c] != 0;
+            if (!more || nact == 0) {
+                // the iteration queued behind this check found no active
+                // column: its launches did no work, so they are not timed
+                if (more)
+                    if (KernelTimer* t = current_timer())
+                        for (const char* nm : {"gemm_f32", "pack_p", "reduce_f32", "bj_apply"}) t->discard(nm, 2);
+                break;
+            }
+            for (int c = 0; c < t.kc; c++)
+                if (fl[c]) t.stats[c].iters++;
+            ++it;
+        }
+        gate_ = nullptr;
+}
+
+void BatchedSolver::tile_solve(const float* A32, const float* B32, const double* normals, const double* vinf4,
+                               int c0, int kc, const BatchedOpts& o, double* x, int64_t ldx, SolveStats* stats,
+                               hipStream_t s, const double* sig_cm, const double* add_cm, const double* x0_cm) {
+    (void)c0;
+    Tile t;
+    t.A32 = A32;
+    t.B32 = B32;
+    t.normals = normals;
+    t.vinf4 = vinf4;
+    t.kc = kc;
+    t.o = o;
+    t.x = x;
+    t.ldx = ldx;
+    t.stats = stats;
+    t.sig_cm = sig_cm;
+    t.add_cm = add_cm;
+    t.x0_cm = x0_cm;
+    tile_begin(t, s);
+    CP_HIP(hipStreamSynchronize(s));
+    tile_begin_host(t);
+    for (;;) {
+        tile_residual(t, s);
+        CP_HIP(hipStreamSynchronize(s));
+        if (!tile_check(t)) break;
+        tile_sweep_open(t, s);
+        if (gmres_ready()) {
+            gm_begin(s);
+            gm_iterations(A32, o, s);
+            gm_end(s);
+        } else {
+            bicgstab_sweep(t, s);
+        }
+        tile_sweep_close(t, s);
+    }
+    tile_end(t, s);
+}
+
+size_t BatchedSolver::group_bytes(int G) { return (size_t)round_up((int64_t)G * sizeof(GmProb), 256) + 256; }
+
+void BatchedSolver::fill_prob(GmProb& P, const BatchedOpts& o, int* flag) const {
+    P = GmProb{};
+    const int KS = shape_.kslices;
+    P.n = n_;
+    P.nkb = nkb_;
+    P.kslices = KS;
+    P.nslices = slices(shape_);
+    P.kbchunk = (nkb_ + KS - 1) / KS;
+    P.ngemm = (int)((lda_ / ((shape_.variant == 13 || shape_.variant == 14) ? 256 : 512)) * KS);
+    P.variant = shape_.variant;
+    P.m = gm_bound(o);
+    P.ndc = (int)((n_ + kGmDotRows - 1) / kGmDotRows);
+    P.nuc = (int)((n_ + 4 * kGmUpRounds * kGmUpPos - 1) / (4 * kGmUpRounds * kGmUpPos));
+    P.bj_gx = (bj_maxb_ + 63) / 64;
+    P.bj_nblocks = bj_nblocks_;
+    P.bj_lo = (bj_lo_ || bj_level_ >= 1) ? 1 : 0;
+    P.lda = lda_;
+    P.plane = plane_;
+    P.us = lda_ * KP;
+    P.apk = (const uint4*)apk_;
+    P.ppk = (uint4*)ppk_;
+    P.part32 = part32_;
+    P.tptr = tptr_;
+    P.tkb = tkb_;
+    P.gm_u = gm_u_;
+    P.gm_st = gm_st_;
+    P.gm_dpart = gm_dpart_;
+    P.gm_coef = gm_coef_;
+    P.dpart = dpart_;
+    P.act = act_;
+    P.sc = sc_;
+    P.iperm = bj_iperm_;
+    P.y = y_;
+    P.bj_perm = bj_perm_;
+    P.bj_ptr = bj_ptr_;
+    P.bj_poff = bj_poff_;
+    P.bj_pk = bj_pk_;
+    P.gflag = flag;
+}
+
+// GMRES iterations of several geometries' sweeps in lockstep: iteration j of
+// every geometry still active is one launch per stage.  probs: the group's
+// descriptors on the device (already uploaded), hp: the same on the host;
+// flags: one int per slot.  As in gm_iterations the host reads the flags of
+// iteration it - 1 while iteration it is queued, and drops a geometry from
+// the launches once its flag reads 0 (a launch it is still in returns at
+// once for it).
+void BatchedSolver::group_iterations(const std::vector<BatchedSolver*>& sv, const GmProb* probs,
+                                     const std::vector<GmProb>& hp, int* flags, hipStream_t s) {
+    const int G = (int)sv.size();
+    int mmax = 0;
+    for (const GmProb& P : hp) mmax = std::max(mmax, P.m);
+    BatchedSolver* s0 = sv[0];
+    int* hfl[2] = {s0->host_act_, s0->host_act_ + 2 * KP};  // s0->host_act_ + KP holds its sweep flags
+    std::vector<int> cur(G);
+    for (int q = 0; q < G; q++) cur[q] = q;
+    GroupLaunch L{};
+    L.probs = probs;
+    auto launch = [&](const std::vector<int>& slots, int j, auto count, auto go) {
+        int acc = 0, g = 0;
+        for (int q : slots) {
+            const int c = count(hp[q]);
+            if (c <= 0) continue;
+            L.idx[g] = q;
+            L.pre[g] = acc;
+            acc += c;
+            g++;
+        }
+        if (g == 0) return;
+        L.G = g;
+        L.j = j;
+        L.pre[g] = acc;
+        go((unsigned)acc);
+        CP_LAUNCH_CHECK();
+    };
+    auto run = [&](int j) {
+        launch(cur, j, [](const GmProb& P) { return P.bj_gx * P.bj_nblocks; },
+               [&](unsigned g) { hipLaunchKernelGGL(grp_bj_kernel, dim3(g), dim3(kT), 0, s, L); });
+        launch(cur, j, [](const GmProb& P) { return (int)((2 * (int64_t)P.nkb * 64 + 255) / 256); },
+               [&](unsigned g) { hipLaunchKernelGGL(grp_pack_kernel, dim3(g), dim3(kT), 0, s, L); });
+        for (int v = 10; v <= 14; v++) {  // one product launch per variant present
+            std::vector<int> sl;
+            for (int q : cur)
+                if (hp[q].variant == v) sl.push_back(q);
+            if (sl.empty()) continue;
+            auto cnt = [](const GmProb& P) { return P.ngemm + (int)(P.lda / 128); };
+            switch (v) {
+                case 10: launch(sl, j, cnt, [&](unsigned g) { hipLaunchKernelGGL((grp_gemm_kernel<4, 2>), dim3(g), dim3(kT), 0, s, L); }); break;
+                case 11: launch(sl, j, cnt, [&](unsigned g) { hipLaunchKernelGGL((grp_gemm_kernel<4, 3>), dim3(g), dim3(kT), 0, s, L); }); break;
+                case 12: launch(sl, j, cnt, [&](unsigned g) { hipLaunchKernelGGL((grp_gemm_kernel<4, 4>), dim3(g), dim3(kT), 0, s, L); }); break;
+                case 13: launch(sl, j, cnt, [&](unsigned g) { hipLaunchKernelGGL((grp_gemm_kernel<2, 2>), dim3(g), dim3(kT), 0, s, L); }); break;
+                default: launch(sl, j, cnt, [&](unsigned g) { hipLaunchKernelGGL((grp_gemm_kernel<2, 3>), dim3(g), dim3(kT), 0, s, L); }); break;
+            }
+        }
+        launch(cur, j, [](const GmProb& P) { return (int)(((int64_t)P.n * (KP / 4) + kT - 1) / kT); },
+               [&](unsigned g) { hipLaunchKernelGGL(grp_w_kernel, dim3(g), dim3(kT), 0, s, L); });
+        launch(cur, j, [j](const GmProb& P) { return P.ndc * ((j + 4) / 4); },
+               [&](unsigned g) { hipLaunchKernelGGL(grp_dots_kernel, dim3(g), dim3(kT), 0, s, L); });
+        launch(cur, j, [j](const GmProb&) { return j + 1; },
+               [&](unsigned g) { hipLaunchKernelGGL(grp_hcol_kernel, dim3(g), dim3(1024), 0, s, L); });
+        launch(cur, j, [](const GmProb& P) { return P.nuc; },
+               [&](unsigned g) { hipLaunchKernelGGL(grp_update_kernel, dim3(g), dim3(kT), 0, s, L); });
+        launch(cur, j, [](const GmProb&) { return 1; },
+               [&](unsigned g) { hipLaunchKernelGGL(grp_givens_kernel, dim3(g), dim3(1024), 0, s, L); });
+    };
+    auto top = [&](int k) {
+        CP_HIP(hipMemcpyAsync(hfl[k & 1], flags, (size_t)G * sizeof(int), hipMemcpyDeviceToHost, s));
+        CP_HIP(hipEventRecord(s0->act_ev_[k & 1], s));
+    };
+    int it = 0;
+    for (;;) {
+        const bool more = it < mmax;
+        if (more) {
+            run(it);
+            top(it + 1);
+        }
+        if (it > 0) {  // every slot sweeps some column entering iteration 0
+            CP_HIP(hipEventSynchronize(s0->act_ev_[it & 1]));
+            const int* fl = hfl[it & 1];
+            std::vector<int> nx;
+            for (int q : cur)
+                if (fl[q]) nx.push_back(q);
+            cur.swap(nx);
+            if (cur.empty()) break;  // iteration it (if queued) found nothing active
+        }
+        if (!more) break;
+        ++it;
+    }
+}
+
+void BatchedSolver::solve_group(std::vector<GroupItem>& items, void* dev, hipStream_t s) {
+    const int G = (int)items.size();
+    if (G <= 0) return;
+    if (G > kMaxGroup) throw HipError{hipErrorInvalidValue, "solve_group: more than kMaxGroup geometries"};
+    std::vector<Tile> T(G);
+    for (int p = 0; p < G; p++) {
+        const GroupItem& it = items[p];
+        if (it.k <= 0 || it.k > KP) throw HipError{hipErrorInvalidValue, "solve_group: 1..64 conditions per geometry"};
+        Tile& t = T[p];
+        t.A32 = it.A32;
+        t.B32 = it.B32;
+        t.normals = it.normals;
+        t.vinf4 = it.vinf4;
+        t.kc = it.k;
+        t.o = *it.o;
+        t.x = it.x;
+        t.ldx = it.ldx;
+        t.stats = it.stats;
+    }
+    for (int p = 0; p < G; p++) items[p].sv->tile_begin(T[p], s);
+    CP_HIP(hipStreamSynchronize(s));
+    for (int p = 0; p < G; p++) items[p].sv->tile_begin_host(T[p]);
+    GmProb* dprobs = static_cast<GmProb*>(dev);
+    int* dflags = reinterpret_cast<int*>(static_cast<char*>(dev) + round_up((int64_t)G * sizeof(GmProb), 256));
+    std::vector<int> live(G, 1);
+    std::vector<GmProb> hp;  // uploaded descriptors: kept until the next synchronisation
+    for (;;) {
+        for (int p = 0; p < G; p++)
+            if (live[p]) items[p].sv->tile_residual(T[p], s);
+        CP_HIP(hipStreamSynchronize(s));
+        std::vector<int> grp, solo;
+        for (int p = 0; p < G; p++) {
+            if (!live[p]) continue;
+            if (!items[p].sv->tile_check(T[p])) {
+                live[p] = 0;
+                continue;
+            }
+            (items[p].sv->group_capable() ? grp : solo).push_back(p);
+        }
+        if (grp.size() == 1) {  // a group of one runs its own launches
+            solo.push_back(grp[0]);
+            grp.clear();
+        }
+        if (grp.empty() && solo.empty()) break;
+        for (int p : solo) {
+            BatchedSolver* sv = items[p].sv;
+            sv->tile_sweep_open(T[p], s);
+            if (sv->gmres_ready()) {
+                sv->gm_begin(s);
+                sv->gm_iterations(T[p].A32, T[p].o, s);
+                sv->gm_end(s);
+            } else {
+                sv->bicgstab_sweep(T[p], s);
+            }
+            sv->tile_sweep_close(T[p], s);
+        }
+        if (!grp.empty()) {
+            std::vector<BatchedSolver*> sv;
+            hp.assign(grp.size(), GmProb{});
+            for (size_t q = 0; q < grp.size(); q++) {
+                BatchedSolver* b = items[grp[q]].sv;
+                b->tile_sweep_open(T[grp[q]], s);
+                b->gm_begin(s);
+                b->fill_prob(hp[q], T[grp[q]].o, dflags + q);
+                sv.push_back(b);
+            }
+            CP_HIP(hipMemcpyAsync(dprobs, hp.data(), hp.size() * sizeof(GmProb), hipMemcpyHostToDevice, s));
+            group_iterations(sv, dprobs, hp, dflags, s);
+            for (size_t q = 0; q < grp.size(); q++) {
+                sv[q]->gm_end(s);
+                sv[q]->tile_sweep_close(T[grp[q]], s);
+            }
+        }
+    }
+    for (int p = 0; p < G; p++) items[p].sv->tile_end(T[p], s);
 }
 
 }  // namespace cpgpu

@@ -108,6 +108,8 @@ void ctx_unref(cp_ctx* c) {
     delete c;
 }
 
+void mark_served(cp_ctx* c) { c->served.store(true); }
+
 void geom_ref(cp_geom* g) { g->refs.fetch_add(1, std::memory_order_relaxed); }
 
 // the last reference frees the geometry; its buffers, pinned block and
@@ -399,6 +401,63 @@ void fill_info(const double* r, const double T[9], cp_solve_info& inf) {
     for (int a = 0; a < 3; a++) inf.CM[a] = r[8 + a];
     // bodyToWind: T^T F (cpCase.cpp:168-172)
     for (int a = 0; a < 3; a++) inf.Fwind[a] = T[0 * 3 + a] * r[2] + T[1 * 3 + a] * r[3] + T[2 * 3 + a] * r[4];
+}
+
+// The geometry's mixed-precision solver with its matrices, near-field
+// correction, packed tiles and preconditioner attached for `prec`.
+void prepare_mixed(cp_geom* g, cp_prec prec, const cp_solve_opts* opts) {
+    const int n = g->dev.n;
+    ensure_batched(g);
+    g->batched->set_near(g->near_rowptr.as<int>(), g->near_cols.as<int>(), g->near_dA.as<double>(),
+                         g->near_dB.as<double>());
+    g->batched->set_packed(g->A16.p, packed_kblocks(n));
+    g->batched->set_near_tiles(g->near_tptr.as<int>(), g->near_tkb.as<int>());
+    if (prec == CP_FP32_REFINE64)
+        g->batched->set_f64(g->A64.as<double>(), g->B64.as<double>());
+    else
+        g->batched->set_f64(nullptr, nullptr);
+    if (opts->precond != 1) ensure_block_jacobi(g);
+    if (opts->precond != 1 && !g->bj_plan.perm.empty())
+        g->batched->set_block_jacobi(g->bj_perm.as<int>(), g->bj_ptr.as<int>(), g->bj_off.as<int64_t>(),
+                                     g->bj_inv.as<float>(), g->bj_plan.nblocks(), g->bj_plan.max_b,
+                                     g->bj_iperm.as<int>(), g->bj_pk.as<float>(), g->bj_poff.as<int64_t>(),
+                                     g->bj_pk_kind);
+    else
+        g->batched->set_block_jacobi(nullptr, nullptr, nullptr, nullptr, 0, 0);
+    if (g->ctx->served.load(std::memory_order_relaxed)) g->batched->allow_graphs(false);
+}
+
+BatchedOpts mixed_opts(const cp_solve_opts* opts, int n) {
+    BatchedOpts bo;
+    bo.tol = opts->tol > 0 ? opts->tol : 1e-10;
+    bo.max_iter = opts->max_iter > 0 ? opts->max_iter : 2 * n;
+    bo.max_refine = opts->max_refine > 0 ? opts->max_refine : 20;
+    bo.inner_tol = opts->inner_tol > 0 ? opts->inner_tol : 0;  // 0: adaptive
+    return bo;
+}
+
+// Cp, forces and Trefftz CL/CDi of k solutions mu (device, leading dimension
+// lda) on stream s; the per-condition results land in res (host, 16 doubles
+// per condition) after the caller synchronises s.  dpart, dres, dcp: the
+// launch's device buffers (dcp only when Cp is wanted).
+void launch_post_k(cp_geom* g, const cp_solve_opts* opts, int k, const double* dmu, const double* dv4, DevBuf& dpart,
+                   DevBuf& dres, DevBuf* dcp, double* res, hipStream_t s) {
+    cp_ctx* c = g->ctx;
+    const int n = g->dev.n;
+    const int64_t lda = g->dev.lda;
+    int nblk = post_partial_blocks(n);
+    dalloc(c, dpart, (size_t)nblk * k * 6 * sizeof(double));
+    dalloc(c, dres, (size_t)16 * k * sizeof(double));
+    if (dcp) dalloc(c, *dcp, (size_t)lda * k * sizeof(double));
+    PostParams pp;
+    pp.PG = std::sqrt(1 - opts->mach * opts->mach);
+    pp.Sref = opts->Sref;
+    pp.bref = opts->bref;
+    pp.cref = opts->cref;
+    for (int a = 0; a < 3; a++) pp.cg[a] = opts->cg[a];
+    launch_post(g->post, pp, k, dmu, lda, dv4, dcp ? dcp->as<double>() : nullptr, lda, dpart.as<double>(), nblk,
+                dres.as<double>(), s);
+    CP_HIP(hipMemcpyAsync(res, dres.p, (size_t)16 * k * sizeof(double), hipMemcpyDeviceToHost, s));
 }
 
 }  // namespace capi
@@ -713,30 +772,8 @@ cp_status cp_solve(cp_geom* g, cp_prec prec, int32_t k, const cp_freestream* fs,
                                           opts->max_iter, s);
             }
         } else {
-            ensure_batched(g);
-            g->batched->set_near(g->near_rowptr.as<int>(), g->near_cols.as<int>(), g->near_dA.as<double>(),
-                                 g->near_dB.as<double>());
-            g->batched->set_packed(g->A16.p, packed_kblocks(n));
-            g->batched->set_near_tiles(g->near_tptr.as<int>(), g->near_tkb.as<int>());
-            if (prec == CP_FP32_REFINE64)
-                g->batched->set_f64(g->A64.as<double>(), g->B64.as<double>());
-            else
-                g->batched->set_f64(nullptr, nullptr);
-            if (opts->precond != 1) {
-                ensure_block_jacobi(g);
-            }
-            if (opts->precond != 1 && !g->bj_plan.perm.empty())
-                g->batched->set_block_jacobi(g->bj_perm.as<int>(), g->bj_ptr.as<int>(), g->bj_off.as<int64_t>(),
-                                             g->bj_inv.as<float>(), g->bj_plan.nblocks(), g->bj_plan.max_b,
-                                             g->bj_iperm.as<int>(), g->bj_pk.as<float>(),
-                                             g->bj_poff.as<int64_t>(), g->bj_pk_kind);
-            else
-                g->batched->set_block_jacobi(nullptr, nullptr, nullptr, nullptr, 0, 0);
-            BatchedOpts bo;
-            bo.tol = tol;
-            bo.max_iter = opts->max_iter > 0 ? opts->max_iter : 2 * n;
-            bo.max_refine = opts->max_refine > 0 ? opts->max_refine : 20;
-            bo.inner_tol = opts->inner_tol > 0 ? opts->inner_tol : 0;  // 0: adaptive
+            prepare_mixed(g, prec, opts);
+            const BatchedOpts bo = mixed_opts(opts, n);
             g->batched->solve(g->A32.as<float>(), g->B32.as<float>(), g->normals.as<double>(), dv4.as<double>(), k,
                               bo, dmu.as<double>(), lda, stats.data(), s);
         }
@@ -773,6 +810,100 @@ cp_status cp_solve(cp_geom* g, cp_prec prec, int32_t k, const cp_freestream* fs,
             }
             all_conv = all_conv && stats[q].rel_residual <= tol;
         }
+        return all_conv ? CP_OK : CP_NOT_CONVERGED;
+    });
+}
+
+cp_status cp_solve_group(int32_t G, cp_geom* const* geoms, cp_prec prec, const int32_t* k,
+                         const cp_freestream* const* fs, const cp_solve_opts* opts, double* const* mu,
+                         double* const* cpout, cp_solve_info* const* info) {
+    if (G <= 0 || G > kMaxGroup || !geoms || !k || !fs || !opts) return CP_BAD_ARG;
+    if (prec != CP_FP32_REFINE && prec != CP_FP32_REFINE64) return CP_BAD_ARG;
+    for (int i = 0; i < G; i++) {
+        if (!geoms[i] || !fs[i] || k[i] <= 0 || k[i] > kRhsTile) return CP_BAD_ARG;
+        if (geoms[i]->ctx != geoms[0]->ctx) return CP_BAD_ARG;
+        for (int j = 0; j < i; j++)
+            if (geoms[j] == geoms[i]) return CP_BAD_ARG;
+    }
+    cp_ctx* c = geoms[0]->ctx;
+    if (c->device < 0) return fail(c, CP_NO_DEVICE, "host-only context");
+    // every geometry's lock, taken in address order (no deadlock with other
+    // groups or single calls)
+    std::vector<cp_geom*> order(geoms, geoms + G);
+    std::sort(order.begin(), order.end());
+    std::vector<std::unique_lock<std::mutex>> locks;
+    for (cp_geom* g : order) locks.emplace_back(g->mu);
+    return guarded_impl(c, nullptr, [&]() -> cp_status {
+        for (int i = 0; i < G; i++) {
+            cp_geom* g = geoms[i];
+            if (!g->have32 || (prec == CP_FP32_REFINE64 && !g->have64))
+                return fail(c, CP_NOT_ASSEMBLED, "cp_solve_group: a geometry is not assembled for this precision");
+        }
+        // the group runs on the first geometry's stream; the others' set-up
+        // work (ensure_*) is on their own streams, finished before the group
+        hipStream_t s = geoms[0]->stream;
+        const double tol = opts->tol > 0 ? opts->tol : 1e-10;
+        std::vector<std::vector<double>> T(G), res(G);
+        std::vector<DevBuf> dv4(G), dmu(G), dpart(G), dres(G), dcp(G);
+        std::vector<std::vector<SolveStats>> stats(G);
+        std::vector<GroupItem> items(G);
+        std::vector<BatchedOpts> bo(G);
+        for (int i = 0; i < G; i++) {
+            cp_geom* g = geoms[i];
+            ensure_post(g);
+            prepare_mixed(g, prec, opts);
+            CP_HIP(hipStreamSynchronize(g->stream));
+            bo[i] = mixed_opts(opts, g->dev.n);
+            std::vector<double> vinf4(4 * (size_t)k[i]);
+            T[i].resize(9 * (size_t)k[i]);
+            for (int q = 0; q < k[i]; q++) {
+                double Vb[3];
+                wind_to_body(fs[i][q], &T[i][9 * (size_t)q], Vb);
+                vinf4[4 * q] = Vb[0];
+                vinf4[4 * q + 1] = Vb[1];
+                vinf4[4 * q + 2] = Vb[2];
+                vinf4[4 * q + 3] = fs[i][q].V;
+            }
+            upload(c, dv4[i], vinf4);
+            dalloc(c, dmu[i], (size_t)g->dev.lda * k[i] * sizeof(double));
+            stats[i].resize(k[i]);
+            items[i] = GroupItem{g->batched.get(), g->A32.as<float>(), g->B32.as<float>(), g->normals.as<double>(),
+                                 dv4[i].as<double>(), k[i], dmu[i].as<double>(), g->dev.lda, stats[i].data(),
+                                 &bo[i]};
+        }
+        DevBuf gbuf;
+        dalloc(c, gbuf, BatchedSolver::group_bytes(G));
+        BatchedSolver::solve_group(items, gbuf.p, s);
+        for (int i = 0; i < G; i++) {
+            cp_geom* g = geoms[i];
+            const int n = g->dev.n;
+            const bool want_cp = cpout && cpout[i];
+            res[i].resize(16 * (size_t)k[i]);
+            launch_post_k(g, opts, k[i], dmu[i].as<double>(), dv4[i].as<double>(), dpart[i], dres[i],
+                          want_cp ? &dcp[i] : nullptr, res[i].data(), s);
+            if (mu && mu[i])
+                CP_HIP(hipMemcpy2DAsync(mu[i], n * sizeof(double), dmu[i].p, g->dev.lda * sizeof(double),
+                                        n * sizeof(double), k[i], hipMemcpyDeviceToHost, s));
+            if (want_cp)
+                CP_HIP(hipMemcpy2DAsync(cpout[i], n * sizeof(double), dcp[i].p, g->dev.lda * sizeof(double),
+                                        n * sizeof(double), k[i], hipMemcpyDeviceToHost, s));
+        }
+        CP_HIP(hipStreamSynchronize(s));
+        bool all_conv = true;
+        for (int i = 0; i < G; i++)
+            for (int q = 0; q < k[i]; q++) {
+                const SolveStats& st = stats[i][q];
+                if (info && info[i]) {
+                    cp_solve_info& inf = info[i][q];
+                    std::memset(&inf, 0, sizeof(cp_solve_info));
+                    fill_info(&res[i][16 * (size_t)q], &T[i][9 * (size_t)q], inf);
+                    inf.rel_residual = st.rel_residual;
+                    inf.iters = st.iters;
+                    inf.refinements = st.refinements;
+                    inf.converged = st.rel_residual <= tol;
+                }
+                all_conv = all_conv && st.rel_residual <= tol;
+            }
         return all_conv ? CP_OK : CP_NOT_CONVERGED;
     });
 }

@@ -64,6 +64,9 @@ struct cp_ctx {
     // reason (hipHostFree and stream teardown synchronise)
     std::vector<void*> pinned;
     std::vector<hipStream_t> streams;
+    // a server runs on this context: its solvers are used from several host
+    // threads at once and never capture graphs
+    std::atomic<bool> served{false};
 };
 
 namespace cpgpu {
@@ -244,6 +247,8 @@ void ensure_post(cp_geom* g);
 void ensure_batched(cp_geom* g);
 void ensure_block_jacobi(cp_geom* g);
 void fill_info(const double* r, const double T[9], cp_solve_info& inf);
+void prepare_mixed(cp_geom* g, cp_prec prec, const cp_solve_opts* opts);
+BatchedOpts mixed_opts(const cp_solve_opts* opts, int n);
 
 }  // namespace capi
 }  // namespace cpgpu

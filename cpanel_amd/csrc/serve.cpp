@@ -42,6 +42,8 @@ namespace capi {
 // context lifetime (capi.cpp): the server holds one reference
 void ctx_ref(cp_ctx* c);
 void ctx_unref(cp_ctx* c);
+// its solvers are used from several host threads from now on
+void mark_served(cp_ctx* c);
 }  // namespace capi
 }  // namespace cpgpu
 
@@ -269,6 +271,7 @@ cp_status cp_server_create(cp_ctx* ctx, int32_t slots, uint64_t hbm_budget, cp_s
     cp_server* S = new cp_server();
     S->ctx = ctx;
     capi::ctx_ref(ctx);  // released by cp_server_destroy
+    capi::mark_served(ctx);
     S->slots = slots;
     S->budget = hbm_budget;
     for (int i = 0; i < slots; i++) S->workers.emplace_back([S] { S->worker(); });

@@ -145,6 +145,21 @@ typedef struct {
 cp_status cp_solve(cp_geom* g, cp_prec prec, int32_t k, const cp_freestream* fs, const cp_solve_opts* opts,
                    double* mu, double* cp, cp_solve_info* info);
 
+/* Several geometries' mixed-precision steady solves in one pass, replacing
+ * the reference's sequential loop of cases (CPanelMgr.cpp:49-88, one
+ * cpCase::run at a time; a caller of the reference runs one geometry per
+ * process).  geoms: G distinct geometries of one context (G <= 16), each
+ * assembled for prec (CP_FP32_REFINE or CP_FP32_REFINE64); k[i] <= 64
+ * conditions fs[i] for geometry i; mu[i], cp[i] (host, n_body_i * k[i], or
+ * NULL; the arrays themselves may be NULL) and info[i] (k[i] entries) as in
+ * cp_solve.  Every Krylov iteration stage runs as one launch over all
+ * geometries still iterating, geometries of uneven panel count included;
+ * each geometry's results are bit-identical to its own cp_solve with the
+ * same options.  Returns CP_NOT_CONVERGED if any condition missed tol. */
+cp_status cp_solve_group(int32_t G, cp_geom* const* geoms, cp_prec prec, const int32_t* k,
+                         const cp_freestream* const* fs, const cp_solve_opts* opts, double* const* mu,
+                         double* const* cp, cp_solve_info* const* info);
+
 /* Y = A X through exactly one device kernel (tests, kernel timing); the
  * product A*x that cpCase::solveMatrixEq hands to Eigen (cpCase.cpp:191-198).
  *   kernel 0: FP64 GEMV on the FP64 A (k launches)

@@ -87,6 +87,7 @@ typedef int hipError_t;
 enum { hipSuccess = 0, hipErrorInvalidValue = 1, hipErrorOutOfMemory = 2 };
 typedef struct rp_stream* hipStream_t;
 typedef struct rp_event* hipEvent_t;
+typedef struct rp_graph_exec* hipGraphExec_t;  // the solver that captures graphs is not replayed
 enum hipMemcpyKind { hipMemcpyHostToHost, hipMemcpyHostToDevice, hipMemcpyDeviceToHost, hipMemcpyDeviceToDevice, hipMemcpyDefault };
 enum { hipStreamNonBlocking = 1, hipHostMallocDefault = 0, hipEventDisableTiming = 2 };
 

@@ -14,6 +14,7 @@ BatchedSolver::BatchedSolver(int, int64_t, int, void*, void*, hipStream_t) { nop
 size_t BatchedSolver::workspace_bytes(int64_t) { return 256; }
 size_t BatchedSolver::host_bytes() { return 256; }
 BatchedSolver::~BatchedSolver() {}
+BatchedSolver::IterPlan::~IterPlan() {}
 void BatchedSolver::solve(const float*, const float*, const double*, const double*, int, const BatchedOpts&, double*,
                           int64_t, SolveStats*, hipStream_t) { nope("solve"); }
 void BatchedSolver::solve_rhs(const float*, const float*, const double*, const double*, const double*, int,
