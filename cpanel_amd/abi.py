@@ -55,14 +55,15 @@ class SolveInfo(C.Structure):
 class RequestStats(C.Structure):
     _fields_ = [("queue_s", C.c_double), ("prepare_s", C.c_double), ("solve_s", C.c_double),
                 ("latency_s", C.c_double), ("cache_hit", C.c_int32), ("batch_conditions", C.c_int32),
-                ("n_body", C.c_int32), ("pad", C.c_int32)]
+                ("n_body", C.c_int32), ("group_geometries", C.c_int32)]
 
 
 class ServerStats(C.Structure):
     _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("conditions", C.c_int64),
                 ("batches", C.c_int64), ("cache_hits", C.c_int64), ("cache_misses", C.c_int64),
                 ("evictions", C.c_int64), ("resident_geometries", C.c_int64),
-                ("resident_bytes_est", C.c_uint64), ("device_bytes", C.c_uint64)]
+                ("resident_bytes_est", C.c_uint64), ("device_bytes", C.c_uint64),
+                ("grouped_solves", C.c_int64)]
 
 
 class VpOpts(C.Structure):

@@ -65,17 +65,26 @@ class Context:
                     tol=1e-10, max_iter=0, max_refine=0, inner_tol=0.0, want_mu=True, want_cp=False,
                     allow_unconverged=False, precond=0):
         """Several geometries solved in one pass (cp_solve_group): jobs is a
-        list of (geometry, conditions) with distinct geometries and at most 64
-        conditions each; returns one list of per-condition results per job,
-        as Geometry.solve does."""
+        list of (geometry, conditions) or (geometry, conditions, options)
+        with distinct geometries and at most 64 conditions each; options (a
+        dict of this method's keywords) override the shared ones for that
+        geometry.  Returns one list of per-condition results per job, as
+        Geometry.solve does."""
+        shared = dict(mach=mach, Sref=Sref, bref=bref, cref=cref, cg=cg, tol=tol, max_iter=max_iter,
+                      max_refine=max_refine, inner_tol=inner_tol, precond=precond)
+        opts = []
+        for j in jobs:
+            o = dict(shared, **(j[2] if len(j) > 2 else {}))
+            opts.append(abi.SolveOpts(o["mach"], o["Sref"], o["bref"], o["cref"], (C.c_double * 3)(*o["cg"]),
+                                      o["tol"], o["max_iter"], o["max_refine"], o["inner_tol"], o["precond"], 0))
+        jobs = [(j[0], j[1]) for j in jobs]
         G = len(jobs)
         geoms = (C.c_void_p * G)(*[g.h for g, _ in jobs])
         ks = (C.c_int32 * G)(*[len(c) for _, c in jobs])
         fs_keep = [(abi.Freestream * len(c)).from_buffer_copy(np.ascontiguousarray(c, dtype=np.float64).reshape(-1, 3))
                    for _, c in jobs]
         fs = (C.POINTER(abi.Freestream) * G)(*[C.cast(f, C.POINTER(abi.Freestream)) for f in fs_keep])
-        o = abi.SolveOpts(mach, Sref, bref, cref, (C.c_double * 3)(*cg), tol, max_iter, max_refine, inner_tol,
-                          precond, 0)
+        o = (abi.SolveOpts * G)(*opts)
         mus = [np.zeros((len(c), g.nb)) if want_mu else None for g, c in jobs]
         cps = [np.zeros((len(c), g.nb)) if want_cp else None for g, c in jobs]
         infos = [(abi.SolveInfo * len(c))() for _, c in jobs]
@@ -83,7 +92,7 @@ class Context:
         mu_p = (dp * G)(*[_p(m) if m is not None else dp() for m in mus])
         cp_p = (dp * G)(*[_p(q) if q is not None else dp() for q in cps])
         in_p = (C.POINTER(abi.SolveInfo) * G)(*[C.cast(i, C.POINTER(abi.SolveInfo)) for i in infos])
-        st = abi.lib().cp_solve_group(G, geoms, prec, ks, fs, C.byref(o), mu_p, cp_p, in_p)
+        st = abi.lib().cp_solve_group(G, geoms, prec, ks, fs, o, mu_p, cp_p, in_p)
         if not (st == abi.CP_NOT_CONVERGED and allow_unconverged):
             self.check(st, "solve_group")
         out = []

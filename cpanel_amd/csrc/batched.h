@@ -218,7 +218,8 @@ private:
     void tile_sweep_close(Tile& t, hipStream_t s);
     void tile_end(Tile& t, hipStream_t s);
     void bicgstab_sweep(Tile& t, hipStream_t s);
-    void colsum_q(const double* dp, int nq, int q0, int q1, int op, hipStream_t s);
+    // column sums of nq quantities' block partials (nblk blocks; 0: nrb_)
+    void colsum_q(const double* dp, int nq, int q0, int q1, int op, hipStream_t s, int nblk = 0);
     bool gmres_ready() const;
     int gm_bound(const BatchedOpts& o) const;
     void gm_begin(hipStream_t s);

@@ -38,6 +38,7 @@ typedef double doublex4 __attribute__((ext_vector_type(4)));
 constexpr int KP = kRhsTile;   // 64 columns
 constexpr int kT = 256;
 constexpr int kRedRows = 64;   // rows per reduction block
+constexpr int kRed64Rows = 16; // rows per block of the FP64 residual's reduction (its partials: lda / 16)
 
 enum {
     Q_RHO = 0, Q_ALPHA, Q_OMEGA, Q_RR, Q_R0R, Q_R0V, Q_TT, Q_TS, Q_RR0, Q_BB, Q_RR64, Q_SCALE, Q_BETA, Q_IT, Q_CHK,
@@ -826,6 +827,10 @@ __global__ void rhs_bn_kernel(int n, int64_t lda, int kc, const double* __restri
 
 // Y = Badd + sign * (sum_ks part + D X) ; partial ||Y_c||^2.  D is the
 // optional sparse FP64 near-field correction (CSR by row).
+// Y = Badd + sign * (sum_ks part + near-field correction), rows >= n zeroed,
+// and the per-block partials of ||Y_c||^2.  A workgroup owns kRed64Rows rows
+// (four per 64-lane group, lane = column); a row's near-field entries (~120)
+// run as four independent chains so the gathers of X overlap.
 __global__ __launch_bounds__(256) void reduce64_kernel(const double* __restrict__ part, int KS, int64_t lda, int n,
                                                        const double* __restrict__ Badd, double sign,
                                                        double* __restrict__ Y, double* __restrict__ dpart,
@@ -835,14 +840,25 @@ __global__ __launch_bounds__(256) void reduce64_kernel(const double* __restrict_
     __shared__ double sh[kT];
     const int c = threadIdx.x & 63, g = threadIdx.x >> 6;
     double acc = 0;
-    for (int rr = g; rr < kRedRows; rr += 4) {
-        int64_t i = (int64_t)blockIdx.x * kRedRows + rr;
+    for (int rr = g; rr < kRed64Rows; rr += 4) {
+        int64_t i = (int64_t)blockIdx.x * kRed64Rows + rr;
         int64_t o = i * KP + c;
         double s = 0;
 #pragma unroll 8
         for (int q = 0; q < KS; q++) s += part[(int64_t)q * lda * KP + o];
-        if (vals && i < n)
-            for (int e = rowptr[i]; e < rowptr[i + 1]; e++) s += vals[e] * X[(int64_t)cols[e] * KP + c];
+        if (vals && i < n) {
+            const int e1 = rowptr[i + 1];
+            int e = rowptr[i];
+            double t0 = 0, t1 = 0, t2 = 0, t3 = 0;
+            for (; e + 4 <= e1; e += 4) {
+                t0 += vals[e] * X[(int64_t)cols[e] * KP + c];
+                t1 += vals[e + 1] * X[(int64_t)cols[e + 1] * KP + c];
+                t2 += vals[e + 2] * X[(int64_t)cols[e + 2] * KP + c];
+                t3 += vals[e + 3] * X[(int64_t)cols[e + 3] * KP + c];
+            }
+            for (; e < e1; e++) t0 += vals[e] * X[(int64_t)cols[e] * KP + c];
+            s += (t0 + t1) + (t2 + t3);
+        }
         double y = (Badd ? Badd[o] : 0.0) + sign * s;
         if (i >= n) y = 0;
         Y[o] = y;
@@ -2277,7 +2293,8 @@ void BatchedSolver::matvec32(const float* A32, const float* P, float* Y, hipStre
 void BatchedSolver::copy64(const double* B, double* Y, hipStream_t s) {
     // the reduction with no K slices: Y = B (rows >= n zeroed) and the
     // per-block column partials of ||Y_c||^2
-    hipLaunchKernelGGL(reduce64_kernel, dim3((unsigned)nrb_), dim3(kT), 0, s, (const double*)nullptr, 0, lda_, n_, B,
+    hipLaunchKernelGGL(reduce64_kernel, dim3((unsigned)(lda_ / kRed64Rows)), dim3(kT), 0, s, (const double*)nullptr,
+                       0, lda_, n_, B,
                        1.0, Y, dpart_, (const int*)nullptr, (const int*)nullptr, (const double*)nullptr,
                        (const double*)nullptr);
     CP_LAUNCH_CHECK();
@@ -2316,7 +2333,8 @@ void BatchedSolver::matvec64acc(const float* A32, const double* X, double* Y, co
     }
     CP_LAUNCH_CHECK();
     const double* vals = (corr && near_rowptr_ && !A64) ? corr : nullptr;
-    hipLaunchKernelGGL(reduce64_kernel, dim3((unsigned)nrb_), dim3(kT), 0, s, (const double*)part64_, KS, lda_, n_,
+    hipLaunchKernelGGL(reduce64_kernel, dim3((unsigned)(lda_ / kRed64Rows)), dim3(kT), 0, s, (const double*)part64_,
+                       KS, lda_, n_,
                        Badd, sign, Y, dpart_, near_rowptr_, near_cols_, vals, X);
     CP_LAUNCH_CHECK();
 }
@@ -2565,8 +2583,9 @@ void BatchedSolver::solve_rhs(const float* A32, const float* B32, const double* 
 // tile_sweep_open, the inner solve, tile_sweep_close; finally tile_end.
 // tile_solve runs them for one geometry, solve_group for several.
 
-void BatchedSolver::colsum_q(const double* dp, int nq, int q0, int q1, int op, hipStream_t s) {
-    hipLaunchKernelGGL(colsum_kernel, dim3(1), dim3(1024), 0, s, dp, nrb_, nq, q0, q1, sc_, op, act_);
+void BatchedSolver::colsum_q(const double* dp, int nq, int q0, int q1, int op, hipStream_t s, int nblk) {
+    hipLaunchKernelGGL(colsum_kernel, dim3(1), dim3(1024), 0, s, dp, nblk > 0 ? nblk : nrb_, nq, q0, q1, sc_, op,
+                       act_);
     CP_LAUNCH_CHECK();
 }
 
@@ -2596,7 +2615,7 @@ void BatchedSolver::tile_begin(Tile& t, hipStream_t s) {
             hipLaunchKernelGGL(gather_cols_kernel, dim3(nel), dim3(kT), 0, s, n_, lda_, t.kc, t.add_cm, t.ldx, R_);
         matvec64acc(t.B32, S_, Bv_, t.add_cm ? R_ : nullptr, -1.0, s, near_dB_, b64_);
     }
-    colsum_q(dpart_, 1, Q_BB, 0, COLSUM_NONE, s);
+    colsum_q(dpart_, 1, Q_BB, 0, COLSUM_NONE, s, (int)(lda_ / kRed64Rows));
     hipLaunchKernelGGL(invd32_kernel, dim3(nblocks(lda_)), dim3(kT), 0, s, n_, t.A32, lda_, invd_);
     if (autotune_) tune(t.A32, s);
     if (t.x0_cm)
@@ -2627,7 +2646,7 @@ void BatchedSolver::tile_residual(Tile& t, hipStream_t s) {
         copy64(Bv_, R_, s);
     else
         matvec64acc(t.A32, X_, R_, Bv_, -1.0, s, near_dA_, a64_);
-    colsum_q(dpart_, 1, Q_RR64, 0, COLSUM_NONE, s);
+    colsum_q(dpart_, 1, Q_RR64, 0, COLSUM_NONE, s, (int)(lda_ / kRed64Rows));
     CP_HIP(hipMemcpyAsync(host_sc_, sc_, KP * Q_NQ * sizeof(double), hipMemcpyDeviceToHost, s));
 }
 

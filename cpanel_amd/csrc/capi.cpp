@@ -842,7 +842,6 @@ cp_status cp_solve_group(int32_t G, cp_geom* const* geoms, cp_prec prec, const i
         // the group runs on the first geometry's stream; the others' set-up
         // work (ensure_*) is on their own streams, finished before the group
         hipStream_t s = geoms[0]->stream;
-        const double tol = opts->tol > 0 ? opts->tol : 1e-10;
         std::vector<std::vector<double>> T(G), res(G);
         std::vector<DevBuf> dv4(G), dmu(G), dpart(G), dres(G), dcp(G);
         std::vector<std::vector<SolveStats>> stats(G);
@@ -851,9 +850,9 @@ cp_status cp_solve_group(int32_t G, cp_geom* const* geoms, cp_prec prec, const i
         for (int i = 0; i < G; i++) {
             cp_geom* g = geoms[i];
             ensure_post(g);
-            prepare_mixed(g, prec, opts);
+            prepare_mixed(g, prec, &opts[i]);
             CP_HIP(hipStreamSynchronize(g->stream));
-            bo[i] = mixed_opts(opts, g->dev.n);
+            bo[i] = mixed_opts(&opts[i], g->dev.n);
             std::vector<double> vinf4(4 * (size_t)k[i]);
             T[i].resize(9 * (size_t)k[i]);
             for (int q = 0; q < k[i]; q++) {
@@ -879,7 +878,7 @@ cp_status cp_solve_group(int32_t G, cp_geom* const* geoms, cp_prec prec, const i
             const int n = g->dev.n;
             const bool want_cp = cpout && cpout[i];
             res[i].resize(16 * (size_t)k[i]);
-            launch_post_k(g, opts, k[i], dmu[i].as<double>(), dv4[i].as<double>(), dpart[i], dres[i],
+            launch_post_k(g, &opts[i], k[i], dmu[i].as<double>(), dv4[i].as<double>(), dpart[i], dres[i],
                           want_cp ? &dcp[i] : nullptr, res[i].data(), s);
             if (mu && mu[i])
                 CP_HIP(hipMemcpy2DAsync(mu[i], n * sizeof(double), dmu[i].p, g->dev.lda * sizeof(double),
@@ -890,7 +889,8 @@ cp_status cp_solve_group(int32_t G, cp_geom* const* geoms, cp_prec prec, const i
         }
         CP_HIP(hipStreamSynchronize(s));
         bool all_conv = true;
-        for (int i = 0; i < G; i++)
+        for (int i = 0; i < G; i++) {
+            const double tol = opts[i].tol > 0 ? opts[i].tol : 1e-10;
             for (int q = 0; q < k[i]; q++) {
                 const SolveStats& st = stats[i][q];
                 if (info && info[i]) {
@@ -904,6 +904,7 @@ cp_status cp_solve_group(int32_t G, cp_geom* const* geoms, cp_prec prec, const i
                 }
                 all_conv = all_conv && st.rel_residual <= tol;
             }
+        }
         return all_conv ? CP_OK : CP_NOT_CONVERGED;
     });
 }

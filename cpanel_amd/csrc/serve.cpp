@@ -21,6 +21,7 @@
 // conditions arriving over time.
 #include <algorithm>
 #include <chrono>
+#include <cstdlib>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
@@ -66,6 +67,7 @@ struct Req {
     bool done = false;
     bool cache_hit = false;
     int batch_cols = 0;
+    int group = 1;  // geometries of the grouped solve it ran in
     int n_body = 0;
     std::vector<cp_solve_info> info;
     std::vector<double> mu;
@@ -76,6 +78,7 @@ struct Entry {
     cp_geom* g = nullptr;
     bool busy = false;
     bool ready = false;
+    int n_body = 0;
     uint64_t est = 0;  // estimated device bytes
     double last_used = 0;
 };
@@ -116,6 +119,10 @@ struct cp_server {
     uint64_t next_id = 1;
     cp_server_stats st{};
     uint64_t budget = 0;
+    // grouping of small resident geometries (see cp_server_create's header
+    // comment): geometries per group and their panel limit
+    int group_max = 8;
+    int group_n = 4096;
 
     // device bytes of resident geometries (estimates) + entries being built
     uint64_t resident() const {
@@ -146,6 +153,25 @@ struct cp_server {
         return true;
     }
 
+    // r and every later queued request for its geometry with its options,
+    // oldest first, up to 64 columns, taken off the queue (mu held)
+    std::vector<Req*> take_batch(Req* r) {
+        std::vector<Req*> batch;
+        int cols = 0;
+        for (auto it = queue.begin(); it != queue.end();) {
+            Req* q = *it;
+            if (q->key == r->key && same_opts(q->opts, r->opts) &&
+                (batch.empty() || cols + (int)q->fs.size() <= kRhsTile)) {
+                batch.push_back(q);
+                cols += (int)q->fs.size();
+                it = queue.erase(it);
+            } else {
+                ++it;
+            }
+        }
+        return batch;
+    }
+
     // next runnable request (mu held): the oldest whose geometry is idle and
     // resident, or absent and admissible
     Req* pick() {
@@ -169,34 +195,56 @@ struct cp_server {
             if (stop) return;
             // batch: this request and every queued one for the same geometry
             // and options, oldest first, up to 64 columns
-            std::vector<Req*> batch;
-            int cols = 0;
-            for (auto it = queue.begin(); it != queue.end();) {
-                Req* q = *it;
-                if (q->key == r->key && same_opts(q->opts, r->opts) &&
-                    (batch.empty() || cols + (int)q->fs.size() <= kRhsTile)) {
-                    batch.push_back(q);
-                    cols += (int)q->fs.size();
-                    it = queue.erase(it);
-                } else {
-                    ++it;
-                }
-            }
+            std::vector<Req*> batch = take_batch(r);
             Entry& e = cache[r->key];
             const bool hit = e.ready;
             if (!hit) e.est = estimate_bytes(*r);
             e.busy = true;
-            const double t0 = now_s();
-            for (Req* q : batch) {
-                q->t_start = t0;
-                q->cache_hit = hit;
-                q->batch_cols = cols;
+            // other idle resident small geometries with queued requests of the
+            // same options join a small resident geometry's solve as a group
+            std::vector<std::vector<Req*>> others;
+            if (hit && group_max > 1 && e.n_body <= group_n) {
+                std::vector<uint64_t> keys;
+                for (Req* q : queue) {
+                    if ((int)keys.size() + 1 >= group_max) break;
+                    if (q->key == r->key) continue;
+                    if (std::find(keys.begin(), keys.end(), q->key) != keys.end()) continue;
+                    auto ce = cache.find(q->key);
+                    if (ce == cache.end() || !ce->second.ready || ce->second.busy || ce->second.n_body > group_n)
+                        continue;
+                    keys.push_back(q->key);
+                }
+                for (uint64_t key : keys) {
+                    Req* first = nullptr;  // the oldest queued request of that geometry
+                    for (Req* q : queue)
+                        if (q->key == key) {
+                            first = q;
+                            break;
+                        }
+                    others.push_back(take_batch(first));
+                    cache[key].busy = true;
+                }
             }
-            st.batches++;
-            if (hit)
-                st.cache_hits += batch.size();
-            else
-                st.cache_misses++;
+            const double t0 = now_s();
+            const int ngroup = 1 + (int)others.size();
+            auto start = [&](std::vector<Req*>& b, bool h) {
+                int cols = 0;
+                for (Req* q : b) cols += (int)q->fs.size();
+                for (Req* q : b) {
+                    q->t_start = t0;
+                    q->cache_hit = h;
+                    q->batch_cols = cols;
+                    q->group = ngroup;
+                }
+                st.batches++;
+                if (h)
+                    st.cache_hits += b.size();
+                else
+                    st.cache_misses++;
+            };
+            start(batch, hit);
+            for (auto& b : others) start(b, true);
+            if (ngroup > 1) st.grouped_solves++;
             lk.unlock();
 
             cp_status s = CP_OK;
@@ -210,18 +258,61 @@ struct cp_server {
                 if (s != CP_OK) err = cp_last_error(ctx);
             }
             const double t1 = now_s();
-            std::vector<cp_freestream> fs;
-            for (Req* q : batch) fs.insert(fs.end(), q->fs.begin(), q->fs.end());
-            std::vector<cp_solve_info> info(fs.size());
-            std::vector<double> mu_all;
-            bool want_mu = false;
-            for (Req* q : batch) want_mu = want_mu || q->want_mu;
-            cp_geom_info gi{};
+            // every batch of the solve: its geometry, conditions and results
+            struct Part {
+                std::vector<Req*>* b;
+                cp_geom* g;
+                cp_geom_info gi;
+                std::vector<cp_freestream> fs;
+                std::vector<cp_solve_info> info;
+                std::vector<double> mu;
+                bool want_mu = false;
+            };
+            std::vector<Part> parts(ngroup);
+            {
+                std::lock_guard<std::mutex> lk2(mu);  // the cache map is read under the lock
+                parts[0].b = &batch;
+                parts[0].g = g;
+                for (int i = 1; i < ngroup; i++) {
+                    parts[i].b = &others[i - 1];
+                    parts[i].g = cache[others[i - 1][0]->key].g;
+                }
+            }
+            for (Part& P : parts) {
+                for (Req* q : *P.b) {
+                    P.fs.insert(P.fs.end(), q->fs.begin(), q->fs.end());
+                    P.want_mu = P.want_mu || q->want_mu;
+                }
+                P.info.resize(P.fs.size());
+                P.gi = cp_geom_info{};
+            }
             if (s == CP_OK) {
-                cp_geometry_info(g, &gi);
-                if (want_mu) mu_all.resize((size_t)gi.n_body * fs.size());
-                s = cp_solve(g, CP_FP32_REFINE, (int32_t)fs.size(), fs.data(), &r->opts,
-                             want_mu ? mu_all.data() : nullptr, nullptr, info.data());
+                for (Part& P : parts) {
+                    cp_geometry_info(P.g, &P.gi);
+                    if (P.want_mu) P.mu.resize((size_t)P.gi.n_body * P.fs.size());
+                }
+                if (ngroup == 1) {
+                    Part& P = parts[0];
+                    s = cp_solve(P.g, CP_FP32_REFINE, (int32_t)P.fs.size(), P.fs.data(), &r->opts,
+                                 P.want_mu ? P.mu.data() : nullptr, nullptr, P.info.data());
+                } else {
+                    std::vector<cp_geom*> gs;
+                    std::vector<int32_t> ks;
+                    std::vector<const cp_freestream*> fsp;
+                    std::vector<double*> mup;
+                    std::vector<cp_solve_info*> infp;
+                    std::vector<cp_solve_opts> op;
+                    for (Part& P : parts) {
+                        op.push_back((*P.b)[0]->opts);
+                        gs.push_back(P.g);
+                        ks.push_back((int32_t)P.fs.size());
+                        fsp.push_back(P.fs.data());
+                        mup.push_back(P.want_mu ? P.mu.data() : nullptr);
+                        infp.push_back(P.info.data());
+                    }
+                    s = cp_solve_group(ngroup, gs.data(), CP_FP32_REFINE, ks.data(), fsp.data(), op.data(),
+                                       mup.data(), nullptr, infp.data());
+                }
                 if (s != CP_OK && s != CP_NOT_CONVERGED) err = cp_last_error(ctx);
             }
             const double t2 = now_s();
@@ -231,31 +322,34 @@ struct cp_server {
                 if (g && (s == CP_OK || s == CP_NOT_CONVERGED)) {
                     e.g = g;
                     e.ready = true;
+                    e.n_body = parts[0].gi.n_body;
                 } else {
                     if (g) cp_geometry_release(g);
                     cache.erase(r->key);
                 }
             }
-            auto it = cache.find(r->key);
-            if (it != cache.end()) {
-                it->second.busy = false;
-                it->second.last_used = t2;
-            }
-            size_t off = 0;
-            for (Req* q : batch) {
-                const size_t k = q->fs.size();
-                q->info.assign(info.begin() + off, info.begin() + off + k);
-                if (q->want_mu && !mu_all.empty())
-                    q->mu.assign(mu_all.begin() + off * gi.n_body, mu_all.begin() + (off + k) * gi.n_body);
-                q->n_body = gi.n_body;
-                off += k;
-                q->status = s;
-                q->err = err;
-                q->t_ready = t1;
-                q->t_end = t2;
-                q->done = true;
-                st.completed++;
-                st.conditions += k;
+            for (Part& P : parts) {
+                auto it = cache.find((*P.b)[0]->key);
+                if (it != cache.end()) {
+                    it->second.busy = false;
+                    it->second.last_used = t2;
+                }
+                size_t off = 0;
+                for (Req* q : *P.b) {
+                    const size_t k = q->fs.size();
+                    q->info.assign(P.info.begin() + off, P.info.begin() + off + k);
+                    if (q->want_mu && !P.mu.empty())
+                        q->mu.assign(P.mu.begin() + off * P.gi.n_body, P.mu.begin() + (off + k) * P.gi.n_body);
+                    q->n_body = P.gi.n_body;
+                    off += k;
+                    q->status = s;
+                    q->err = err;
+                    q->t_ready = t1;
+                    q->t_end = t2;
+                    q->done = true;
+                    st.completed++;
+                    st.conditions += k;
+                }
             }
             cv_done.notify_all();
             cv_work.notify_all();
@@ -274,6 +368,8 @@ cp_status cp_server_create(cp_ctx* ctx, int32_t slots, uint64_t hbm_budget, cp_s
     capi::mark_served(ctx);
     S->slots = slots;
     S->budget = hbm_budget;
+    if (const char* e = std::getenv("CPGPU_SERVE_GROUP")) S->group_max = std::max(1, std::min(kMaxGroup, std::atoi(e)));
+    if (const char* e = std::getenv("CPGPU_SERVE_GROUP_N")) S->group_n = std::max(0, std::atoi(e));
     for (int i = 0; i < slots; i++) S->workers.emplace_back([S] { S->worker(); });
     *out = S;
     return CP_OK;
@@ -319,7 +415,7 @@ cp_status cp_server_wait(cp_server* S, uint64_t id, cp_solve_info* info, double*
         rs->cache_hit = r->cache_hit;
         rs->batch_conditions = r->batch_cols;
         rs->n_body = r->n_body;
-        rs->pad = 0;
+        rs->group_geometries = r->group;
     }
     cp_status s = r->status;
     S->reqs.erase(it);

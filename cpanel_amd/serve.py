@@ -218,7 +218,9 @@ def main(argv=None):
     S.close()
     out = summarize(res, wall)
     out.update(dict(qps=a.qps, slots=a.slots, seed=a.seed, geometries=len(geoms), evictions=st["evictions"],
-                    batches=st["batches"], budget_gb=a.budget_gb,
+                    batches=st["batches"], grouped_solves=st["grouped_solves"],
+                    requests_in_groups=sum(1 for r in res if r["stats"]["group_geometries"] > 1),
+                    budget_gb=a.budget_gb,
                     p90_ms=1e3 * float(np.percentile([r["stats"]["latency_s"] for r in res], 90)),
                     mean_prepare_ms_new_geometry=1e3 * float(np.mean(
                         [r["stats"]["prepare_s"] for r in res if not r["stats"]["cache_hit"]] or [0])),

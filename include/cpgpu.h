@@ -150,9 +150,9 @@ cp_status cp_solve(cp_geom* g, cp_prec prec, int32_t k, const cp_freestream* fs,
  * cpCase::run at a time; a caller of the reference runs one geometry per
  * process).  geoms: G distinct geometries of one context (G <= 16), each
  * assembled for prec (CP_FP32_REFINE or CP_FP32_REFINE64); k[i] <= 64
- * conditions fs[i] for geometry i; mu[i], cp[i] (host, n_body_i * k[i], or
- * NULL; the arrays themselves may be NULL) and info[i] (k[i] entries) as in
- * cp_solve.  Every Krylov iteration stage runs as one launch over all
+ * conditions fs[i] and options opts[i] (an array of G) for geometry i;
+ * mu[i], cp[i] (host, n_body_i * k[i], or NULL; the arrays themselves may be
+ * NULL) and info[i] (k[i] entries) as in cp_solve.  Every Krylov iteration stage runs as one launch over all
  * geometries still iterating, geometries of uneven panel count included;
  * each geometry's results are bit-identical to its own cp_solve with the
  * same options.  Returns CP_NOT_CONVERGED if any condition missed tol. */
@@ -355,7 +355,12 @@ void cp_vp_release(cp_vp* v);
  * geometries overlap on the GPU; queued requests for the same geometry and
  * options are merged into one batched solve of up to 64 conditions.  New
  * geometries are admitted against hbm_budget (0: no limit), evicting idle
- * geometries least recently used first. */
+ * geometries least recently used first.  When a worker takes a request for
+ * a resident geometry of at most 4,096 body panels, queued requests for other
+ * idle resident geometries of that size join it (up to 8 geometries) and the
+ * worker solves them together with cp_solve_group; each result is the same
+ * as in a solve of its own (environment: CPGPU_SERVE_GROUP = geometries per
+ * group, 1 turns grouping off; CPGPU_SERVE_GROUP_N = panel limit). */
 typedef struct cp_server cp_server;
 
 typedef struct {
@@ -366,7 +371,7 @@ typedef struct {
     int32_t cache_hit;      /* geometry was resident */
     int32_t batch_conditions;  /* columns of the solve it ran in */
     int32_t n_body;
-    int32_t pad;
+    int32_t group_geometries;  /* geometries of the grouped solve it ran in (1: alone) */
 } cp_request_stats;
 
 typedef struct {
@@ -374,6 +379,7 @@ typedef struct {
     int64_t cache_hits, cache_misses, evictions;
     int64_t resident_geometries;
     uint64_t resident_bytes_est, device_bytes;
+    int64_t grouped_solves;  /* cp_solve_group calls of two or more geometries */
 } cp_server_stats;
 
 cp_status cp_server_create(cp_ctx* ctx, int32_t slots, uint64_t hbm_budget, cp_server** out);

@@ -51,14 +51,13 @@ def test_group_equals_alone(setup):
     ctx, geoms, rng = setup
     ks = [7, 1, 16, 3, 32, 12]
     conds = [conds_for(rng, k) for k in ks]
-    # one reference-length set for the whole group (the options are shared)
-    kw = kw_of(geoms[0][1])
+    # each geometry with its own reference lengths
     alone = []
-    for (g, _, _), c in zip(geoms, conds):
-        alone.append(g.solve(c, CP_FP32_REFINE, tol=1e-10, want_cp=True, **kw))
+    for (g, prm, _), c in zip(geoms, conds):
+        alone.append(g.solve(c, CP_FP32_REFINE, tol=1e-10, want_cp=True, **kw_of(prm)))
     say("[gpu-group] alone done; iterations %s" % [max(r["iters"] for r in a) for a in alone])
-    grp = ctx.solve_group([(g, c) for (g, _, _), c in zip(geoms, conds)], CP_FP32_REFINE, tol=1e-10,
-                          want_cp=True, **kw)
+    grp = ctx.solve_group([(g, c, kw_of(prm)) for (g, prm, _), c in zip(geoms, conds)], CP_FP32_REFINE, tol=1e-10,
+                          want_cp=True)
     say("[gpu-group] group done")
     for i, (a, b) in enumerate(zip(alone, grp)):
         for q, (x, y) in enumerate(zip(a, b)):

@@ -69,6 +69,36 @@ def test_mix_equals_alone_and_fp64(ctx):
             assert abs(a["CL"] - b["CL"]) <= 1e-5
 
 
+def test_burst_groups_equal_alone(ctx, monkeypatch):
+    """A burst of requests over resident small geometries: one worker solves
+    several geometries' requests together (cp_solve_group), and every result
+    equals the same request solved alone with grouping off."""
+    stream, geoms = serve.make_stream(40, 1e6, seed=21, n_lo=400, n_hi=2500, reuse=0.8, max_cond=6)
+    S = serve.Server(ctx, slots=1)
+    serve.run_stream(S, stream, geoms)  # builds every geometry
+    burst, wall = serve.run_stream(S, stream, geoms, want_mu=True)  # all resident, all queued at once
+    st = S.stats()
+    S.close()
+    grouped = sum(1 for r in burst if r["stats"]["group_geometries"] > 1)
+    say("[gpu-serve] burst: %d grouped solves, %d of %d requests in a group" % (st["grouped_solves"], grouped,
+                                                                                 len(burst)))
+    assert all(r["status"] == abi.CP_OK for r in burst)
+    assert st["grouped_solves"] > 0 and grouped > 0
+    monkeypatch.setenv("CPGPU_SERVE_GROUP", "1")
+    A = serve.Server(ctx, slots=1)
+    for i, r in enumerate(stream):
+        mesh, prm = geoms[r["key"]]
+        nb = int(np.sum(np.asarray(mesh["ids"]) < 1000))
+        opts = serve.solve_opts(Sref=prm["Sref"], bref=prm["bref"], cref=prm["cref"])
+        rid = A.submit(r["key"], mesh, r["conds"], opts, want_mu=True)
+        s, info, mu, rs = A.wait(rid, nb)
+        assert s == abi.CP_OK and rs["group_geometries"] == 1
+        for a, b in zip(info, burst[i]["info"]):
+            assert a["CL"] == b["CL"] and a["iters"] == b["iters"], (i, a["CL"], b["CL"])
+        assert np.array_equal(mu, burst[i]["mu"]), i
+    A.close()
+
+
 def test_budget_evicts_and_results_hold(ctx):
     stream, geoms = serve.make_stream(12, 1000.0, seed=5, n_lo=800, n_hi=1500, reuse=0.3, max_cond=6)
     say("[gpu-serve] budget stream")
