@@ -144,9 +144,12 @@ def vp_bench(ctx, log, case_steps=0, faithful_steps=0):
     mesh = meshgen.config_c4()
     X, A, R = meshgen.c4_particles(mesh)
     out = {"particles": int(len(R))}
+    ctx.bh_tree(X, A, R).close()  # first build: the pool's allocations
+    ctx.timing_reset()
     t0 = time.perf_counter()
     tree = ctx.bh_tree(X, A, R)
-    out["bh_build_ms"] = 1e3 * (time.perf_counter() - t0)
+    out["bh_build_wall_ms"] = 1e3 * (time.perf_counter() - t0)  # with the upload of the particles
+    out["bh_build_ms"] = ctx.timing("bh_build")[0]  # the device build (events on its stream)
     out["bh_nodes"] = tree.info()["n_nodes"]
     names = {abi.CP_BH_VEL: "vel", abi.CP_BH_STRETCH: "stretch", abi.CP_BH_DIFF: "diff"}
     for mode, nm in names.items():
@@ -235,6 +238,87 @@ def serve_bench(ctx, n_req, qps, slots, seed, log):
                      mean_solve_ms=1e3 * float(np.mean([r["stats"]["solve_s"] for r in res]))))
     log("serving: %s" % json.dumps(summ))
     return summ
+
+
+def group_bench(ctx, log, seed=3, reps=3):
+    """Windows of small geometries (uneven panel counts, the serving stream's
+    generator) solved one after another with cp_solve and together with
+    cp_solve_group: wall ms per window, geometries cached, CP_FP32_REFINE
+    at tol 1e-10 as in serving; the grouped results equal the single ones
+    bit for bit (checked)."""
+    from cpanel_amd import CP_FP32_REFINE, serve
+    rng = np.random.default_rng(seed)
+    out = []
+    for G, lo, hi, k in ((8, 500, 3000, 16), (16, 500, 1500, 8), (4, 4000, 8000, 32)):
+        geoms = []
+        for _ in range(G):
+            mesh, prm = serve.request_geometry(rng, lo, hi)
+            g = ctx.geometry(mesh)
+            g.assemble(CP_FP32_REFINE)
+            geoms.append((g, dict(mach=0.1, Sref=prm["Sref"], bref=prm["bref"], cref=prm["cref"], cg=(0.25, 0, 0))))
+        conds = [[(1.0, float(rng.uniform(-4, 10)), float(rng.uniform(-5, 5))) for _ in range(k)] for _ in range(G)]
+        jobs = [(g, c, kw) for (g, kw), c in zip(geoms, conds)]
+        ref = [g.solve(c, CP_FP32_REFINE, want_cp=False, **kw) for (g, kw), c in zip(geoms, conds)]  # warm
+        grp = ctx.solve_group(jobs, CP_FP32_REFINE, want_cp=False)
+        same = all(np.array_equal(a["mu"], b["mu"]) for ra, rb in zip(ref, grp) for a, b in zip(ra, rb))
+        ts, tg = [], []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            for (g, kw), c in zip(geoms, conds):
+                g.solve(c, CP_FP32_REFINE, want_mu=False, want_cp=False, **kw)
+            t1 = time.perf_counter()
+            ctx.solve_group(jobs, CP_FP32_REFINE, want_mu=False, want_cp=False)
+            t2 = time.perf_counter()
+            ts.append(t1 - t0)
+            tg.append(t2 - t1)
+        r = {"geometries": G, "n_body": [g.nb for g, _ in geoms], "conditions_each": k,
+             "sequential_ms": 1e3 * min(ts), "grouped_ms": 1e3 * min(tg), "speedup": min(ts) / min(tg),
+             "solves_per_s_grouped": G * k / min(tg), "bit_identical": bool(same)}
+        log("grouped solves: %s" % json.dumps(r))
+        out.append(r)
+        for g, _ in geoms:
+            g.release()
+    return out
+
+
+def serve_hot_bench(ctx, log, n_req=2000, n_geoms=100, qps=1000.0, seed=4):
+    """A hot-cache burst through the server: 2,000 requests at 1,000/s over
+    100 resident geometries of 500..4,000 body panels (log-uniform), 1..16
+    conditions each, four slots; with grouping of small geometries (the
+    default) and with it turned off (CPGPU_SERVE_GROUP=1)."""
+    from cpanel_amd import serve
+    rng = np.random.default_rng(seed)
+    geoms = [serve.request_geometry(rng, 500, 4000) for _ in range(n_geoms)]
+    t = 0.0
+    stream = []
+    for _ in range(n_req):
+        t += rng.exponential(1.0 / qps)
+        key = int(rng.integers(0, n_geoms))
+        k = int(rng.integers(1, 17))
+        stream.append(dict(t=t, key=key, conds=[(1.0, float(rng.uniform(-4, 10)), float(rng.uniform(-5, 5)))
+                                                for _ in range(k)]))
+    warm = [dict(t=0.0, key=i, conds=[(1.0, 2.0, 0.0)]) for i in range(n_geoms)]
+    out = {}
+    for label, grp in (("grouped", None), ("alone", "1")):
+        if grp:
+            os.environ["CPGPU_SERVE_GROUP"] = grp
+        try:
+            S = serve.Server(ctx, slots=4)
+            serve.run_stream(S, warm, geoms)  # every geometry resident
+            st0 = S.stats()
+            res, wall = serve.run_stream(S, stream, geoms)
+            st = S.stats()
+            S.close()
+        finally:
+            os.environ.pop("CPGPU_SERVE_GROUP", None)
+        summ = serve.summarize(res, wall)
+        summ.update(dict(grouped_solves=st["grouped_solves"] - st0["grouped_solves"],
+                         requests_in_groups=sum(1 for r in res if r["stats"]["group_geometries"] > 1)))
+        out[label] = summ
+        log("serving, hot cache (%s): %s" % (label, json.dumps(summ)))
+    out["workload"] = ("%d requests at %.0f/s over %d resident geometries of 500..4,000 panels, 1..16 conditions, "
+                       "4 slots, seed %d" % (n_req, qps, n_geoms, seed))
+    return out
 
 
 def log(*a):
@@ -537,6 +621,8 @@ def main():
             del os.environ["CPGPU_NO_BF16X2"]
             del os.environ["CPGPU_NO_BF16X3"]
 
+    groups = None if args.no_variants else group_bench(ctx, log)
+    hot = serve_hot_bench(ctx, log) if args.serve_requests > 0 else None
     vp = None if args.no_vp else vp_bench(ctx, log, args.vp_steps, args.vp_reference_steps)
     srv = serve_bench(ctx, args.serve_requests, args.serve_qps, args.serve_slots, args.serve_seed, log) \
         if args.serve_requests > 0 else None
@@ -615,6 +701,8 @@ def main():
             "gemm_f32_avg_ms": gemm_avg,
             "vortex_particles": vp,
             "serving": srv,
+            "grouped_solves": groups,
+            "serving_hot_cache": hot,
             "gemm_shape": rep.get("gemm_shape"),
             "block_jacobi_blocks": rep.get("block_jacobi_blocks"),
         },

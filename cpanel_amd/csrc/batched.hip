@@ -330,7 +330,7 @@ __global__ __launch_bounds__(256) void pack_a_bf16x2_kernel(const float* __restr
 }
 
 // P (row-major lda x 64) -> tiles [K block][column tile 0/1]; rows >= n are 0
-__device__ inline void pack_p_body(int bx, const float* __restrict__ P, int n, int nkb, uint4* __restrict__ out,
+__device__ __forceinline__ void pack_p_body(int bx, const float* __restrict__ P, int n, int nkb, uint4* __restrict__ out,
                                    const int* __restrict__ gate) {
     if (gate && gate_idle(gate)) return;
     const int64_t g = (int64_t)bx * 256 + threadIdx.x;
@@ -552,7 +552,7 @@ __device__ inline void lo_tile_row(const uint4* __restrict__ Alo, int nkb, const
 // restoration (lo_tile_row, 4 row tiles each) on the CUs and wave slots the
 // product leaves free, writing the extra partial slice in the same launch.
 template <int RT, int S>
-__device__ inline void gemm_bf16x2_s_body(int bid, const uint4* __restrict__ Apk, int nkb,
+__device__ __forceinline__ void gemm_bf16x2_s_body(int bid, const uint4* __restrict__ Apk, int nkb,
                                           const uint4* __restrict__ Ppk, float* __restrict__ part, int64_t lda,
                                           int kbchunk, int KS, int ngemm, int64_t plane,
                                           const int* __restrict__ tptr, const int* __restrict__ tkb,
@@ -1187,7 +1187,7 @@ __global__ __launch_bounds__(1024) void gm_start_kernel(const double* __restrict
 // w = sum_ks part (the product A~ z_j, row order) stored as u_{j+1} in block
 // order; a thread sums four columns (skipped when none of them is active;
 // an inactive column's u_{j+1} is never read)
-__device__ inline void gm_w_body(int bx, const float* __restrict__ part, int KS, int64_t lda, int n,
+__device__ __forceinline__ void gm_w_body(int bx, const float* __restrict__ part, int KS, int64_t lda, int n,
                                  const int* __restrict__ act, const int* __restrict__ iperm, float* __restrict__ W) {
     const int64_t idx = (int64_t)bx * kT + threadIdx.x;  // float4 index
     if (idx >= (int64_t)n * (KP / 4)) return;
@@ -1209,7 +1209,7 @@ __global__ __launch_bounds__(256) void gm_w_kernel(const float* __restrict__ par
 // positions, t <= j), lane = column; eight position pairs in flight, each
 // wavefront writes its own partial (no block reduction)
 constexpr int kGmDotRows = 256;
-__device__ inline void gm_dots_body(int bx, int by, int nchunk, int n, const int* __restrict__ act,
+__device__ __forceinline__ void gm_dots_body(int bx, int by, int nchunk, int n, const int* __restrict__ act,
                                     const float* __restrict__ U, int64_t us, int j, double* __restrict__ dpart) {
     const int c = threadIdx.x & 63, g = threadIdx.x >> 6;
     const int t = by * 4 + g;
@@ -1241,7 +1241,7 @@ __global__ __launch_bounds__(256) void gm_dots_kernel(int n, const int* __restri
 }
 
 // Hessenberg entries h_{q,j} = (u_q . w) / s_q, one quantity q per block
-__device__ inline void gm_hcol_body(int q, double* sh, const double* __restrict__ dpart, int nblk,
+__device__ __forceinline__ void gm_hcol_body(int q, double* sh, const double* __restrict__ dpart, int nblk,
                                     const int* __restrict__ act, int j, double* __restrict__ st,
                                     float* __restrict__ coef) {
     const double t = colsum1024(dpart + (int64_t)q * nblk * KP, nblk, sh);
@@ -1265,7 +1265,7 @@ __global__ __launch_bounds__(1024) void gm_hcol_kernel(const double* __restrict_
 // workgroup
 constexpr int kGmUpPos = 8;
 constexpr int kGmUpRounds = 2;
-__device__ inline void gm_update_body(int bx, double* sh, int n, const int* __restrict__ act, float* __restrict__ U,
+__device__ __forceinline__ void gm_update_body(int bx, double* sh, int n, const int* __restrict__ act, float* __restrict__ U,
                                       int64_t us, int j, const float* __restrict__ coef, double* __restrict__ dpart) {
     const int c = threadIdx.x & 63, g = threadIdx.x >> 6;
     double acc = 0;
@@ -1317,7 +1317,7 @@ __global__ __launch_bounds__(256) void gm_update_kernel(int n, const int* __rest
 // column j; g rotated; a column stops when |g_{j+1}| <= tol s_0, at the
 // Krylov bound m, or on a breakdown (h_{j+1,j} = 0: the solution is exact);
 // tol is the column's sc[Q_ITOL]
-__device__ inline void gm_givens_col(double t, int* __restrict__ act, int j, int m, const double* __restrict__ sc,
+__device__ __forceinline__ void gm_givens_col(double t, int* __restrict__ act, int j, int m, const double* __restrict__ sc,
                                      double* __restrict__ st) {
     double* S = st + threadIdx.x * kGmStride;
     double* h = S + GM_H + j * (kGmMax + 1);
@@ -1344,7 +1344,7 @@ __device__ inline void gm_givens_col(double t, int* __restrict__ act, int j, int
 }
 // gflag (or null): set to whether any column is still active (wavefront 0
 // holds the 64 columns)
-__device__ inline void gm_givens_body(double* sh, const double* __restrict__ dpart, int nblk, int* __restrict__ act,
+__device__ __forceinline__ void gm_givens_body(double* sh, const double* __restrict__ dpart, int nblk, int* __restrict__ act,
                                       int j, int m, const double* __restrict__ sc, double* __restrict__ st,
                                       int* __restrict__ gflag) {
     const double t = colsum1024(dpart, nblk, sh);
@@ -1664,12 +1664,13 @@ __global__ __launch_bounds__(256) void bj_pack_bf_kernel(const float* __restrict
 // wavefronts take every fourth K chunk of 16 (each multivector load feeds the
 // RT tiles) and add their accumulators through LDS.
 template <bool LO, int RT>
-__device__ inline void bj_apply_bf_body(int bx, int q, float (&red)[4][RT * 32][64], const int* __restrict__ perm,
-                                        const int* __restrict__ ptr, const int64_t* __restrict__ poff,
-                                        const float* __restrict__ pk, const float* __restrict__ PP,
-                                        float* __restrict__ Y, const double* __restrict__ cscale, int cstride,
-                                        const int* __restrict__ gate) {
+__device__ __forceinline__ void bj_apply_bf_body(int bx, int q, const int* __restrict__ perm,
+                                                 const int* __restrict__ ptr, const int64_t* __restrict__ poff,
+                                                 const float* __restrict__ pk, const float* __restrict__ PP,
+                                                 float* __restrict__ Y, const double* __restrict__ cscale,
+                                                 int cstride, const int* __restrict__ gate) {
     if (gate && gate_idle(gate)) return;
+    __shared__ float red[4][RT * 32][64];
     const int b0 = ptr[q], b = ptr[q + 1] - b0;
     const int rt0 = bx * RT;
     if (rt0 * 32 >= b) return;  // whole workgroup, before any barrier
@@ -1765,6 +1766,10 @@ __device__ inline void bj_apply_bf_body(int bx, int q, float (&red)[4][RT * 32][
         }
     }
 }
+// The single-geometry launch keeps its own copy of the body: calling the
+// shared body (bj_apply_bf_body, above) from it measured 56 -> 66 us per
+// product at 25,916 panels, with the same memory instructions but different
+// scheduling.
 template <bool LO, int RT>
 __global__ __launch_bounds__(256) void bj_apply_bf_kernel(const int* __restrict__ perm, const int* __restrict__ ptr,
                                                           const int64_t* __restrict__ poff,
@@ -1772,8 +1777,103 @@ __global__ __launch_bounds__(256) void bj_apply_bf_kernel(const int* __restrict_
                                                           const float* __restrict__ PP, float* __restrict__ Y,
                                                           const double* __restrict__ cscale, int cstride,
                                                           const int* __restrict__ gate) {
+    if (gate && gate_idle(gate)) return;
     __shared__ float red[4][RT * 32][64];
-    bj_apply_bf_body<LO, RT>(blockIdx.x, blockIdx.y, red, perm, ptr, poff, pk, PP, Y, cscale, cstride, gate);
+    const int q = blockIdx.y;
+    const int b0 = ptr[q], b = ptr[q + 1] - b0;
+    const int rt0 = blockIdx.x * RT;
+    if (rt0 * 32 >= b) return;  // whole workgroup, before any barrier
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int cl = lane & 31, kh = lane >> 5;
+    const int nkc = (b + 15) / 16, ntile = (b + 31) / 32;
+    const u32x4* T = reinterpret_cast<const u32x4*>(pk + poff[q]) + (int64_t)rt0 * nkc * 128 + lane;
+    const float* Pq = PP + (int64_t)b0 * KP;
+    floatx16 acc[RT][2];
+#pragma unroll
+    for (int r = 0; r < RT; r++) acc[r][0] = acc[r][1] = floatx16{};
+    u32x4 ah[RT], al[RT], nah[RT], nal[RT];
+    float x0[8], x1[8], nx0[8], nx1[8];
+    auto load = [&](int c, u32x4 (&h)[RT], u32x4 (&l)[RT], float (&y0)[8], float (&y1)[8]) {
+        const bool cok = c < nkc;
+#pragma unroll
+        for (int r = 0; r < RT; r++) {
+            const bool ok = cok && rt0 + r < ntile;  // tiles past the block's last are not stored
+            h[r] = ok ? __builtin_nontemporal_load(T + ((int64_t)r * nkc + c) * 128) : u32x4{};
+            if (LO) l[r] = ok ? __builtin_nontemporal_load(T + ((int64_t)r * nkc + c) * 128 + 64) : u32x4{};
+        }
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const int k = 16 * c + 8 * kh + i;
+            const bool ok = cok && k < b;
+            y0[i] = ok ? Pq[(int64_t)k * KP + cl] : 0.f;
+            y1[i] = ok ? Pq[(int64_t)k * KP + 32 + cl] : 0.f;
+        }
+    };
+    load(wave, ah, al, x0, x1);
+    for (int c = wave; c < nkc; c += 4) {
+        const bool more = c + 4 < nkc;
+        if (more) load(c + 4, nah, nal, nx0, nx1);
+        uint4 h0, l0, h1, l1;
+        split8(x0, h0, l0);
+        split8(x1, h1, l1);
+#pragma unroll
+        for (int r = 0; r < RT; r++) {
+            const bf16x8 a = __builtin_bit_cast(bf16x8, ah[r]);
+            acc[r][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, __builtin_bit_cast(bf16x8, l0), acc[r][0], 0, 0, 0);
+            acc[r][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, __builtin_bit_cast(bf16x8, l1), acc[r][1], 0, 0, 0);
+            if (LO) {
+                const bf16x8 alo = __builtin_bit_cast(bf16x8, al[r]);
+                acc[r][0] =
+                    __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo, __builtin_bit_cast(bf16x8, h0), acc[r][0], 0, 0, 0);
+                acc[r][1] =
+                    __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo, __builtin_bit_cast(bf16x8, h1), acc[r][1], 0, 0, 0);
+            }
+            acc[r][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, __builtin_bit_cast(bf16x8, h0), acc[r][0], 0, 0, 0);
+            acc[r][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, __builtin_bit_cast(bf16x8, h1), acc[r][1], 0, 0, 0);
+        }
+        if (more) {
+#pragma unroll
+            for (int r = 0; r < RT; r++) {
+                ah[r] = nah[r];
+                if (LO) al[r] = nal[r];
+            }
+#pragma unroll
+            for (int i = 0; i < 8; i++) {
+                x0[i] = nx0[i];
+                x1[i] = nx1[i];
+            }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < RT; r++)
+#pragma unroll
+        for (int j = 0; j < 16; j++) {
+            red[wave][r * 32 + j][lane] = acc[r][0][j];
+            red[wave][r * 32 + 16 + j][lane] = acc[r][1][j];
+        }
+    __syncthreads();
+    float f0 = 1.f, f1 = 1.f;  // optional per-column factor 1 / cscale[c * cstride] (0 where that is 0)
+    if (cscale) {
+        const double s0 = cscale[(int64_t)cl * cstride], s1 = cscale[(int64_t)(32 + cl) * cstride];
+        f0 = s0 != 0 ? (float)(1.0 / s0) : 0.f;
+        f1 = s1 != 0 ? (float)(1.0 / s1) : 0.f;
+    }
+    const int* pq = perm + b0;
+    // wavefront w finishes entries 8 RT w .. 8 RT w + 8 RT - 1 of the 32 RT
+    // (tile r: entries 32 r + 0..15 of acc[r][0], 32 r + 16..31 of acc[r][1])
+#pragma unroll
+    for (int i = 0; i < 8 * RT; i++) {
+        const int jj = 8 * RT * wave + i;
+        const float v = red[0][jj][lane] + red[1][jj][lane] + red[2][jj][lane] + red[3][jj][lane];
+        const int r = jj >> 5, e = jj & 31, j = e & 15;
+        const int rr = (rt0 + r) * 32 + 8 * (j >> 2) + 4 * kh + (j & 3);
+        if (rr < b) {
+            if (e < 16)
+                Y[(int64_t)pq[rr] * KP + cl] = v * f0;
+            else
+                Y[(int64_t)pq[rr] * KP + 32 + cl] = v * f1;
+        }
+    }
 }
 
 // ---------------------------------------------------------------- groups
@@ -1784,28 +1884,37 @@ __global__ __launch_bounds__(256) void bj_apply_bf_kernel(const int* __restrict_
 // what its own launch would, so its results do not depend on the group.
 }  // namespace
 
+// Device pass: the descriptors' pointers are declared global-memory
+// pointers (every one of them is a device allocation), so the grouped
+// kernels access through them with global instructions; loaded from memory
+// as plain pointers they would be flat ones.  Same layout in both passes.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define CP_GLOBAL __attribute__((address_space(1)))
+#else
+#define CP_GLOBAL
+#endif
 struct GmProb {
     int n, nkb, kbchunk, kslices, nslices, ngemm, variant, m, ndc, nuc, bj_gx, bj_nblocks, bj_lo;
     int64_t lda, plane, us;
-    const uint4* apk;
-    uint4* ppk;
-    float* part32;
-    const int* tptr;
-    const int* tkb;
-    float* gm_u;
-    double* gm_st;
-    double* gm_dpart;
-    float* gm_coef;
-    double* dpart;
-    int* act;
-    const double* sc;
-    const int* iperm;
-    float* y;
-    const int* bj_perm;
-    const int* bj_ptr;
-    const int64_t* bj_poff;
-    const float* bj_pk;
-    int* gflag;  // any column of this geometry still active after the iteration
+    CP_GLOBAL const uint4* apk;
+    CP_GLOBAL uint4* ppk;
+    CP_GLOBAL float* part32;
+    CP_GLOBAL const int* tptr;
+    CP_GLOBAL const int* tkb;
+    CP_GLOBAL float* gm_u;
+    CP_GLOBAL double* gm_st;
+    CP_GLOBAL double* gm_dpart;
+    CP_GLOBAL float* gm_coef;
+    CP_GLOBAL double* dpart;
+    CP_GLOBAL int* act;
+    CP_GLOBAL const double* sc;
+    CP_GLOBAL const int* iperm;
+    CP_GLOBAL float* y;
+    CP_GLOBAL const int* bj_perm;
+    CP_GLOBAL const int* bj_ptr;
+    CP_GLOBAL const int64_t* bj_poff;
+    CP_GLOBAL const float* bj_pk;
+    CP_GLOBAL int* gflag;  // any column of this geometry still active after the iteration
 };
 
 namespace {
@@ -1818,26 +1927,26 @@ struct GroupLaunch {
     int pre[kMaxGroup + 1];  // workgroup prefix of the slots
 };
 
-__device__ inline const GmProb& group_of(const GroupLaunch& L, int& local) {
+// Slot of this workgroup: the number of slot starts (after the first) at or
+// below its id.  Unrolled and branch-free, so the descriptor's pointers count
+// as loaded from unclobbered kernel-argument memory and the bodies access
+// global memory with global (not flat) instructions.
+__device__ __forceinline__ const GmProb& group_of(const GroupLaunch& L, int& local) {
     const int bid = blockIdx.x;
     int p = 0;
-    while (p + 1 < L.G && bid >= L.pre[p + 1]) p++;
+#pragma unroll
+    for (int i = 1; i < kMaxGroup; i++) p += (i < L.G && bid >= L.pre[i]) ? 1 : 0;
     local = bid - L.pre[p];
     return L.probs[L.idx[p]];
 }
 
+// every slot of one launch has the preconditioner form LO
+template <bool LO>
 __global__ __launch_bounds__(256) void grp_bj_kernel(GroupLaunch L) {
-    __shared__ float red[4][64][64];
     int l;
     const GmProb& P = group_of(L, l);
-    const float* PP = P.gm_u + (int64_t)L.j * P.us;
-    const double* cs = P.gm_st + GM_S + L.j;
-    if (P.bj_lo)
-        bj_apply_bf_body<true, 2>(l % P.bj_gx, l / P.bj_gx, red, P.bj_perm, P.bj_ptr, P.bj_poff, P.bj_pk, PP, P.y, cs,
-                                  kGmStride, P.act);
-    else
-        bj_apply_bf_body<false, 2>(l % P.bj_gx, l / P.bj_gx, red, P.bj_perm, P.bj_ptr, P.bj_poff, P.bj_pk, PP, P.y,
-                                   cs, kGmStride, P.act);
+    bj_apply_bf_body<LO, 2>(l % P.bj_gx, l / P.bj_gx, P.bj_perm, P.bj_ptr, P.bj_poff, P.bj_pk,
+                            P.gm_u + (int64_t)L.j * P.us, P.y, P.gm_st + GM_S + L.j, kGmStride, P.act);
 }
 
 __global__ __launch_bounds__(256) void grp_pack_kernel(GroupLaunch L) {
@@ -1851,8 +1960,8 @@ template <int RT, int S>
 __global__ __launch_bounds__(256) void grp_gemm_kernel(GroupLaunch L) {
     int l;
     const GmProb& P = group_of(L, l);
-    gemm_bf16x2_s_body<RT, S>(l, P.apk, P.nkb, P.ppk, P.part32, P.lda, P.kbchunk, P.kslices, P.ngemm, P.plane, P.tptr,
-                              P.tkb, P.act);
+    gemm_bf16x2_s_body<RT, S>(l, P.apk, P.nkb, P.ppk, P.part32, P.lda, P.kbchunk, P.kslices, P.ngemm,
+                              P.plane, P.tptr, P.tkb, P.act);
 }
 
 __global__ __launch_bounds__(256) void grp_w_kernel(GroupLaunch L) {
@@ -2858,25 +2967,25 @@ void BatchedSolver::fill_prob(GmProb& P, const BatchedOpts& o, int* flag) const 
     P.lda = lda_;
     P.plane = plane_;
     P.us = lda_ * KP;
-    P.apk = (const uint4*)apk_;
-    P.ppk = (uint4*)ppk_;
-    P.part32 = part32_;
-    P.tptr = tptr_;
-    P.tkb = tkb_;
-    P.gm_u = gm_u_;
-    P.gm_st = gm_st_;
-    P.gm_dpart = gm_dpart_;
-    P.gm_coef = gm_coef_;
-    P.dpart = dpart_;
-    P.act = act_;
-    P.sc = sc_;
-    P.iperm = bj_iperm_;
-    P.y = y_;
-    P.bj_perm = bj_perm_;
-    P.bj_ptr = bj_ptr_;
-    P.bj_poff = bj_poff_;
-    P.bj_pk = bj_pk_;
-    P.gflag = flag;
+    P.apk = (decltype(P.apk))apk_;
+    P.ppk = (decltype(P.ppk))ppk_;
+    P.part32 = (decltype(P.part32))part32_;
+    P.tptr = (decltype(P.tptr))tptr_;
+    P.tkb = (decltype(P.tkb))tkb_;
+    P.gm_u = (decltype(P.gm_u))gm_u_;
+    P.gm_st = (decltype(P.gm_st))gm_st_;
+    P.gm_dpart = (decltype(P.gm_dpart))gm_dpart_;
+    P.gm_coef = (decltype(P.gm_coef))gm_coef_;
+    P.dpart = (decltype(P.dpart))dpart_;
+    P.act = (decltype(P.act))act_;
+    P.sc = (decltype(P.sc))sc_;
+    P.iperm = (decltype(P.iperm))bj_iperm_;
+    P.y = (decltype(P.y))y_;
+    P.bj_perm = (decltype(P.bj_perm))bj_perm_;
+    P.bj_ptr = (decltype(P.bj_ptr))bj_ptr_;
+    P.bj_poff = (decltype(P.bj_poff))bj_poff_;
+    P.bj_pk = (decltype(P.bj_pk))bj_pk_;
+    P.gflag = (decltype(P.gflag))flag;
 }
 
 // GMRES iterations of several geometries' sweeps in lockstep: iteration j of
@@ -2915,8 +3024,16 @@ void BatchedSolver::group_iterations(const std::vector<BatchedSolver*>& sv, cons
         CP_LAUNCH_CHECK();
     };
     auto run = [&](int j) {
-        launch(cur, j, [](const GmProb& P) { return P.bj_gx * P.bj_nblocks; },
-               [&](unsigned g) { hipLaunchKernelGGL(grp_bj_kernel, dim3(g), dim3(kT), 0, s, L); });
+        for (int lo = 0; lo < 2; lo++) {  // one block-Jacobi launch per preconditioner form present
+            std::vector<int> sl;
+            for (int q : cur)
+                if (hp[q].bj_lo == lo) sl.push_back(q);
+            auto cnt = [](const GmProb& P) { return P.bj_gx * P.bj_nblocks; };
+            if (lo)
+                launch(sl, j, cnt, [&](unsigned g) { hipLaunchKernelGGL(grp_bj_kernel<true>, dim3(g), dim3(kT), 0, s, L); });
+            else
+                launch(sl, j, cnt, [&](unsigned g) { hipLaunchKernelGGL(grp_bj_kernel<false>, dim3(g), dim3(kT), 0, s, L); });
+        }
         launch(cur, j, [](const GmProb& P) { return (int)((2 * (int64_t)P.nkb * 64 + 255) / 256); },
                [&](unsigned g) { hipLaunchKernelGGL(grp_pack_kernel, dim3(g), dim3(kT), 0, s, L); });
         for (int v = 10; v <= 14; v++) {  // one product launch per variant present
