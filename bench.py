@@ -437,6 +437,8 @@ def main():
                     help="steps of the vortex-particle case in the reference's form (0: skip)")
     ap.add_argument("--no-variants", action="store_true",
                     help="skip the amortised-setup and FP32-operand measurements")
+    ap.add_argument("--group-only", action="store_true",
+                    help="run only the grouped-solve windows (profiling of the grouped kernels)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="run one elementwise copy of 256 MiB first (HBM counter calibration under a profiler)")
     args = ap.parse_args()
@@ -464,6 +466,9 @@ def main():
         del x, y
 
     ctx = Context(0)
+    if args.group_only:
+        print(json.dumps({"grouped_solves": group_bench(ctx, log)}), flush=True)
+        return
     t0 = time.perf_counter()
     g = ctx.geometry(mesh)
     t_geom = time.perf_counter() - t0
