@@ -15,6 +15,8 @@ size_t BatchedSolver::workspace_bytes(int64_t) { return 256; }
 size_t BatchedSolver::host_bytes() { return 256; }
 BatchedSolver::~BatchedSolver() {}
 BatchedSolver::IterPlan::~IterPlan() {}
+size_t BatchedSolver::group_bytes(int) { return 256; }
+void BatchedSolver::solve_group(std::vector<GroupItem>&, void*, hipStream_t) { nope("solve_group"); }
 void BatchedSolver::solve(const float*, const float*, const double*, const double*, int, const BatchedOpts&, double*,
                           int64_t, SolveStats*, hipStream_t) { nope("solve"); }
 void BatchedSolver::solve_rhs(const float*, const float*, const double*, const double*, const double*, int,
