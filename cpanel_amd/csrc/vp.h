@@ -36,9 +36,15 @@ void launch_bh_refresh(const DevBhTree& T, const double* pos, const double* str,
 // particles, each particle the called object of the pair functions
 void launch_pairs_direct(int mode, int n, const double* pos, const double* str, const double* rad, double* out,
                          hipStream_t s);
+// Direct sums over n particles at m points split the particles into
+// direct_slices(m, n) slices when m alone leaves the device idle (a few
+// hundred Trefftz points against 80,000 particles), adding the slices'
+// partial sums in order; part: direct_part_doubles(m, n) doubles of scratch.
+int direct_slices(int m, int n);
+size_t direct_part_doubles(int m, int n);
 // direct Gaussian point velocity of n particles at m points
 void launch_gauss_points(int n, const double* pos, const double* str, const double* rad, int m, const double* pts,
-                         double* out, hipStream_t s);
+                         double* part, double* out, hipStream_t s);
 // vortex-filament velocity (ends: 6 per filament, fstr) at m points
 void launch_filament_points(int nf, const double* ends, const double* fstr, int m, const double* pts, double* out,
                             hipStream_t s);
@@ -76,7 +82,7 @@ void launch_core_bh(const DevBhTree& T, int mode, double theta, double core, boo
 // direct sums over n particles at m targets; own: the targets are the
 // particles themselves (m = n), each skipping itself
 void launch_core_direct(int mode, double core, bool f32, int n, const double* pos, const double* str, int m,
-                        const double* tpos, const double* tstr, bool own, double* out, hipStream_t s);
+                        const double* tpos, const double* tstr, bool own, double* part, double* out, hipStream_t s);
 // y[0..n) += a[0..n)
 void launch_add_vec(int n, const double* a, double* y, hipStream_t s);
 // out (column-major, ld) = -C mu for k columns: C is lda x m (FP64), mu m x k

@@ -292,15 +292,19 @@ __global__ __launch_bounds__(kT) void bh_refresh_kernel(int n, const int* __rest
 // particle::velInfl(POI) (particle.cpp:60-77) summed over all particles in
 // their order: Gaussian kernel, sigma = coreOverlap * radius, a particle at
 // the point itself contributes nothing
+// (the particles j0..j1 of slice blockIdx.y: slice q of out at out + 3 m q)
 __global__ __launch_bounds__(kT) void gauss_points_kernel(int n, const double* __restrict__ ppos,
                                                           const double* __restrict__ pstr,
                                                           const double* __restrict__ prad, int m,
-                                                          const double* __restrict__ pts, double* __restrict__ out) {
+                                                          const double* __restrict__ pts, int chunk,
+                                                          double* __restrict__ out) {
     const int t = blockIdx.x * kT + threadIdx.x;
     if (t >= m) return;
+    const int j0 = blockIdx.y * chunk, j1 = min(n, j0 + chunk);
+    out += (int64_t)blockIdx.y * 3 * m;
     const double px = pts[3 * t], py = pts[3 * t + 1], pz = pts[3 * t + 2];
     double acc[3] = {0, 0, 0};
-    for (int j = 0; j < n; j++) {
+    for (int j = j0; j < j1; j++) {
         const double dx = px - ppos[3 * j], dy = py - ppos[3 * j + 1], dz = pz - ppos[3 * j + 2];
         const double d = nrm3(dx, dy, dz);
         if (d == 0) continue;
@@ -442,9 +446,12 @@ __global__ __launch_bounds__(kT) void bhc_kernel(const BhNode* __restrict__ node
 template <int MODE, class T>
 __global__ __launch_bounds__(kT) void dc_kernel(int n, const double* __restrict__ pos, const double* __restrict__ str,
                                                 double s2d, int m, const double* __restrict__ tpos,
-                                                const double* __restrict__ tstr, int own, double* __restrict__ out) {
+                                                const double* __restrict__ tstr, int own, int chunk,
+                                                double* __restrict__ out) {
     const int t = blockIdx.x * kT + threadIdx.x;
     if (t >= m) return;
+    const int j0 = blockIdx.y * chunk, j1 = min(n, j0 + chunk);
+    out += (int64_t)blockIdx.y * 3 * m;  // this slice's partial sums
     const double tp[3] = {tpos[3 * t], tpos[3 * t + 1], tpos[3 * t + 2]};
     T ta[3] = {0, 0, 0};
     if (MODE == KC_STRETCH) {
@@ -455,7 +462,7 @@ __global__ __launch_bounds__(kT) void dc_kernel(int n, const double* __restrict_
     const int self = own ? t : -1;
     const T s2 = (T)s2d;
     T acc[3] = {0, 0, 0};
-    for (int j = 0; j < n; j++) {
+    for (int j = j0; j < j1; j++) {
         if (j == self) continue;
         const T sa[3] = {(T)str[3 * j], (T)str[3 * j + 1], (T)str[3 * j + 2]};
         if (MODE == KC_VEL)
@@ -890,12 +897,31 @@ void launch_pairs_direct(int mode, int n, const double* pos, const double* str, 
     CP_LAUNCH_CHECK();
 }
 
+int direct_slices(int m, int n) {
+    const int tiles = (m + kT - 1) / kT;
+    int KS = 1;
+    while (KS < 64 && (int64_t)tiles * KS < 2048 && n / (2 * KS) >= 512) KS *= 2;
+    return KS;
+}
+
+size_t direct_part_doubles(int m, int n) {
+    const int KS = m > 0 ? direct_slices(m, n) : 1;
+    return KS > 1 ? (size_t)KS * 3 * m : 1;
+}
+
 void launch_gauss_points(int n, const double* pos, const double* str, const double* rad, int m, const double* pts,
-                         double* out, hipStream_t s) {
+                         double* part, double* out, hipStream_t s) {
     if (m <= 0) return;
-    hipLaunchKernelGGL(gauss_points_kernel, dim3((unsigned)((m + kT - 1) / kT)), dim3(kT), 0, s, n, pos, str, rad, m,
-                       pts, out);
+    const int KS = direct_slices(m, n);
+    const int chunk = (n + KS - 1) / KS;
+    hipLaunchKernelGGL(gauss_points_kernel, dim3((unsigned)((m + kT - 1) / kT), (unsigned)KS), dim3(kT), 0, s, n, pos,
+                       str, rad, m, pts, chunk, KS > 1 ? part : out);
     CP_LAUNCH_CHECK();
+    if (KS > 1) {
+        hipLaunchKernelGGL(p2p_reduce_kernel, dim3((unsigned)((3 * m + kT - 1) / kT)), dim3(kT), 0, s,
+                           (const double*)part, KS, m, out);
+        CP_LAUNCH_CHECK();
+    }
 }
 
 void launch_filament_points(int nf, const double* ends, const double* fstr, int m, const double* pts, double* out,
@@ -1022,13 +1048,16 @@ void launch_core_bh(const DevBhTree& T, int mode, double theta, double core, boo
 }
 
 void launch_core_direct(int mode, double core, bool f32, int n, const double* pos, const double* str, int m,
-                        const double* tpos, const double* tstr, bool own, double* out, hipStream_t s) {
+                        const double* tpos, const double* tstr, bool own, double* part, double* out, hipStream_t s) {
     if (m <= 0) return;
-    dim3 grid((unsigned)((m + kT - 1) / kT));
+    const int KS = direct_slices(m, n);
+    const int chunk = (n + KS - 1) / KS;
+    dim3 grid((unsigned)((m + kT - 1) / kT), (unsigned)KS);
     const double s2 = core * core;
-#define CP_DC(MODE, TY)                                                                                    \
+    double* dst = KS > 1 ? part : out;
+#define CP_DC(MODE, TY)                                                                                        \
     hipLaunchKernelGGL((dc_kernel<MODE, TY>), grid, dim3(kT), 0, s, n, pos, str, s2, m, tpos, tstr, (int)own, \
-                       out)
+                       chunk, dst)
     if (mode == CP_BH_STRETCH) {
         if (f32)
             CP_DC(KC_STRETCH, float);
@@ -1042,6 +1071,11 @@ void launch_core_direct(int mode, double core, bool f32, int n, const double* po
     }
 #undef CP_DC
     CP_LAUNCH_CHECK();
+    if (KS > 1) {  // the slices' partial sums in slice order
+        hipLaunchKernelGGL(p2p_reduce_kernel, dim3((unsigned)((3 * m + kT - 1) / kT)), dim3(kT), 0, s,
+                           (const double*)part, KS, m, out);
+        CP_LAUNCH_CHECK();
+    }
 }
 
 }  // namespace cpgpu

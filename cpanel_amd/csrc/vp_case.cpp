@@ -163,18 +163,19 @@ std::vector<double> vel_direct(cp_vp* v, const std::vector<double>& pts) {
     cp_geom* g = v->g;
     cp_ctx* c = g->ctx;
     hipStream_t s = g->stream;
-    DevBuf dp, ds, dr, dpts, dout;
+    DevBuf dp, ds, dr, dpts, dpart, dout;
     to_dev(c, dp, v->ppos, s);
     to_dev(c, ds, v->pstr, s);
     to_dev(c, dpts, pts, s);
+    dalloc(c, dpart, direct_part_doubles(m, np) * sizeof(double));
     dalloc(c, dout, (size_t)3 * m * sizeof(double));
     if (v->alg) {
         launch_core_direct(CP_BH_VEL, v->core, v->f32, np, dp.as<double>(), ds.as<double>(), m, dpts.as<double>(),
-                           nullptr, false, dout.as<double>(), s);
+                           nullptr, false, dpart.as<double>(), dout.as<double>(), s);
     } else {
         to_dev(c, dr, v->prad, s);
         launch_gauss_points(np, dp.as<double>(), ds.as<double>(), dr.as<double>(), m, dpts.as<double>(),
-                            dout.as<double>(), s);
+                            dpart.as<double>(), dout.as<double>(), s);
     }
     from_dev(out, dout, (size_t)3 * m, s);
     return out;
@@ -261,16 +262,18 @@ std::vector<double> pairs_direct(cp_vp* v, int mode) {
     cp_geom* g = v->g;
     cp_ctx* c = g->ctx;
     hipStream_t s = g->stream;
-    DevBuf dp, ds, dr, dout;
+    DevBuf dp, ds, dr, dpart, dout;
     to_dev(c, dp, v->ppos, s);
     to_dev(c, ds, v->pstr, s);
     to_dev(c, dr, v->prad, s);
     dalloc(c, dout, (size_t)3 * np * sizeof(double));
-    if (v->alg)
+    if (v->alg) {
+        dalloc(c, dpart, direct_part_doubles(np, np) * sizeof(double));
         launch_core_direct(mode, v->core, v->f32, np, dp.as<double>(), ds.as<double>(), np, dp.as<double>(),
-                           ds.as<double>(), true, dout.as<double>(), s);
-    else
+                           ds.as<double>(), true, dpart.as<double>(), dout.as<double>(), s);
+    } else {
         launch_pairs_direct(mode, np, dp.as<double>(), ds.as<double>(), dr.as<double>(), dout.as<double>(), s);
+    }
     from_dev(out, dout, (size_t)3 * np, s);
     return out;
 }
