@@ -441,6 +441,69 @@ __global__ __launch_bounds__(kT) void bhc_kernel(const BhNode* __restrict__ node
     out[3 * o + 2] = acc[2];
 }
 
+// The same walk with one wavefront per target, for target sets too small to
+// fill the device (a step's few hundred RK4 seeds, the collocation points):
+// the node decisions are the wavefront's (uniform), a leaf's members are
+// split over the lanes, a far cell's pseudo-particle is lane 0's; the lanes'
+// sums are added in a fixed butterfly order.  Targets are caller-given
+// points (tslot as in bhc_kernel).
+template <int MODE, class T>
+__global__ __launch_bounds__(kT) void bhc_wave_kernel(const BhNode* __restrict__ nodes, int n_nodes,
+                                                      const double* __restrict__ mpos,
+                                                      const double* __restrict__ mstr, double theta, double s2d,
+                                                      int m, const double* __restrict__ tpos,
+                                                      const double* __restrict__ tstr,
+                                                      const int* __restrict__ tslot, double* __restrict__ out) {
+    const int t = blockIdx.x * (kT / 64) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (t >= m) return;  // whole wavefronts
+    const double tp[3] = {tpos[3 * t], tpos[3 * t + 1], tpos[3 * t + 2]};
+    T ta[3] = {0, 0, 0};
+    if (MODE == KC_STRETCH) {
+        ta[0] = (T)tstr[3 * t];
+        ta[1] = (T)tstr[3 * t + 1];
+        ta[2] = (T)tstr[3 * t + 2];
+    }
+    const int self = tslot ? tslot[t] : -1;
+    const T s2 = (T)s2d;
+    T acc[3] = {0, 0, 0};
+    int i = 1;
+    while (i < n_nodes) {
+        const BhNode nd = nodes[i];
+        if (nd.count >= 0) {
+            for (int k = lane; k < nd.count; k += 64) {
+                const int ms = nd.first + k;
+                if (ms == self) continue;
+                const T sa[3] = {(T)mstr[3 * ms], (T)mstr[3 * ms + 1], (T)mstr[3 * ms + 2]};
+                if (MODE == KC_VEL)
+                    kc_vel<T>(mpos + 3 * ms, sa, tp, s2, acc);
+                else
+                    kc_stretch<T>(mpos + 3 * ms, sa, tp, ta, s2, acc);
+            }
+            i = nd.skip;
+        } else if (far_cell(nd, tp[0], tp[1], tp[2], theta)) {
+            if (lane == 0) {
+                const double sp[3] = {nd.ex, nd.ey, nd.ez};
+                const T sa[3] = {(T)nd.sx, (T)nd.sy, (T)nd.sz};
+                if (MODE == KC_VEL)
+                    kc_vel<T>(sp, sa, tp, s2, acc);
+                else
+                    kc_stretch<T>(sp, sa, tp, ta, s2, acc);
+            }
+            i = nd.skip;
+        } else {
+            i = i + 1;
+        }
+    }
+    for (int off = 32; off >= 1; off >>= 1)
+        for (int q = 0; q < 3; q++) acc[q] += __shfl_xor(acc[q], off, 64);
+    if (lane == 0) {
+        out[3 * t] = acc[0];
+        out[3 * t + 1] = acc[1];
+        out[3 * t + 2] = acc[2];
+    }
+}
+
 // direct sums of the configured kernel over n particles at m targets, in
 // particle order; with own, target t is particle t and skips itself
 template <int MODE, class T>
@@ -897,6 +960,10 @@ void launch_pairs_direct(int mode, int n, const double* pos, const double* str, 
     CP_LAUNCH_CHECK();
 }
 
+// caller-given targets up to which the algebraic-core walk runs a wavefront
+// per target
+constexpr int kBhcWaveMax = 16384;
+
 int direct_slices(int m, int n) {
     const int tiles = (m + kT - 1) / kT;
     int KS = 1;
@@ -961,9 +1028,14 @@ PanelSumPlan plan_panel_sums(int m, int np) {
     P.KS = p2p_slices(m, np);
     P.ntiles = (m + 63) / 64;
     // the near work gathers in the few tiles close to the body: split the
-    // panels finely so those tiles spread over many wavefronts
+    // panels finely so those tiles spread over many wavefronts (up to 32
+    // slices of at least 48 panels; with few tiles, such as a step's few
+    // hundred seed points, up to 256 slices of at least 12 panels until
+    // about 1,024 workgroups)
     P.KN = 1;
-    while (P.KN < 32 && np / (2 * P.KN) >= 96) P.KN *= 2;
+    while ((P.KN < 32 && np / (2 * P.KN) >= 96) ||
+           (P.KN < 256 && (int64_t)P.ntiles * P.KN < 1024 && np / (2 * P.KN) >= 24))
+        P.KN *= 2;
     P.tsph = ((size_t)P.KS + P.KN) * 3 * (size_t)m;
     P.perm = P.tsph + 4 * (size_t)P.ntiles;
     P.sort = (P.perm + ((size_t)m + 1) / 2 + 31) / 32 * 32;  // 256-byte aligned
@@ -1027,8 +1099,30 @@ void launch_panel_stretching(const DevGeom& g, const double* sigma, const double
 void launch_core_bh(const DevBhTree& T, int mode, double theta, double core, bool f32, int m, const double* tpos,
                     const double* tstr, const int* tslot, const int* own_order, double* out, hipStream_t s) {
     if (m <= 0) return;
-    dim3 grid((unsigned)((m + kT - 1) / kT));
     const double s2 = core * core;
+    // a wavefront per target below kBhcWaveMax caller-given targets (a lane
+    // per target leaves the device idle there)
+    if (!own_order && m <= kBhcWaveMax) {
+        dim3 grid((unsigned)((m + kT / 64 - 1) / (kT / 64)));
+#define CP_BHW(MODE, TY)                                                                                      \
+    hipLaunchKernelGGL((bhc_wave_kernel<MODE, TY>), grid, dim3(kT), 0, s, T.nodes, T.n_nodes, T.mpos, T.mstr, \
+                       theta, s2, m, tpos, tstr, tslot, out)
+        if (mode == CP_BH_STRETCH) {
+            if (f32)
+                CP_BHW(KC_STRETCH, float);
+            else
+                CP_BHW(KC_STRETCH, double);
+        } else {
+            if (f32)
+                CP_BHW(KC_VEL, float);
+            else
+                CP_BHW(KC_VEL, double);
+        }
+#undef CP_BHW
+        CP_LAUNCH_CHECK();
+        return;
+    }
+    dim3 grid((unsigned)((m + kT - 1) / kT));
 #define CP_BHC(MODE, TY)                                                                                      \
     hipLaunchKernelGGL((bhc_kernel<MODE, TY>), grid, dim3(kT), 0, s, T.nodes, T.n_nodes, T.mpos, T.mstr, theta, \
                        s2, m, tpos, tstr, tslot, own_order, out)
