@@ -245,6 +245,91 @@ __global__ __launch_bounds__(kT) void bh_targets_kernel(const BhNode* __restrict
     out[3 * t + 2] = acc[2];
 }
 
+// caller-given targets up to which the tree walks run a wavefront per target
+constexpr int kBhWaveMax = 16384;
+constexpr int kBhcWaveMax = kBhWaveMax;
+
+// The walk with one wavefront per target, for target sets too small to fill
+// the device (the reference form's few thousand particles, a step's seed
+// points): the node decisions are the wavefront's, a leaf's members are
+// split over the lanes (member(ms) adds member ms to the lane's sums), a far
+// cell is lane 0's (cell(nd)); the caller adds the lanes' sums.
+template <class FM, class FC>
+__device__ __forceinline__ void walk_wave(const BhNode* __restrict__ nodes, int n_nodes, double px, double py,
+                                          double pz, double theta, int lane, FM&& member_fn, FC&& cell_fn) {
+    int i = 1;
+    while (i < n_nodes) {
+        const BhNode nd = nodes[i];
+        if (nd.count >= 0) {
+            for (int k = lane; k < nd.count; k += 64) member_fn(nd.first + k);
+            i = nd.skip;
+        } else if (far_cell(nd, px, py, pz, theta)) {
+            if (lane == 0) cell_fn(nd);
+            i = nd.skip;
+        } else {
+            i = i + 1;
+        }
+    }
+}
+
+// sum of acc[3] over the wavefront's lanes in a fixed butterfly order (every
+// lane ends with the total)
+__device__ __forceinline__ void wave_sum3(double* acc) {
+    for (int off = 32; off >= 1; off >>= 1)
+        for (int q = 0; q < 3; q++) acc[q] += __shfl_xor(acc[q], off, 64);
+}
+
+__global__ __launch_bounds__(kT) void bh_points_wave_kernel(const BhNode* __restrict__ nodes, int n_nodes,
+                                                            const double* __restrict__ mpos,
+                                                            const double* __restrict__ mstr,
+                                                            const double* __restrict__ mrad, double theta, int m,
+                                                            const double* __restrict__ pts,
+                                                            double* __restrict__ out) {
+    const int t = blockIdx.x * (kT / 64) + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (t >= m) return;  // whole wavefronts
+    double p[3] = {pts[3 * t], pts[3 * t + 1], pts[3 * t + 2]};
+    double acc[3] = {0, 0, 0};
+    walk_wave(nodes, n_nodes, p[0], p[1], p[2], theta, lane,
+              [&](int ms) { k_alg(member(mpos, mstr, mrad, ms), p, acc); },
+              [&](const BhNode& nd) { k_alg(pseudo(nd), p, acc); });
+    wave_sum3(acc);
+    if (lane == 0)
+        for (int q = 0; q < 3; q++) out[3 * t + q] = acc[q];
+}
+
+template <int MODE>
+__global__ __launch_bounds__(kT) void bh_targets_wave_kernel(const BhNode* __restrict__ nodes, int n_nodes,
+                                                             const double* __restrict__ mpos,
+                                                             const double* __restrict__ mstr,
+                                                             const double* __restrict__ mrad, double theta, int m,
+                                                             const double* __restrict__ tpos,
+                                                             const double* __restrict__ tstr,
+                                                             const double* __restrict__ trad,
+                                                             const int* __restrict__ tslot,
+                                                             double* __restrict__ out) {
+    const int t = blockIdx.x * (kT / 64) + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (t >= m) return;  // whole wavefronts
+    const Src tg{tpos[3 * t], tpos[3 * t + 1], tpos[3 * t + 2], tstr[3 * t], tstr[3 * t + 1], tstr[3 * t + 2], trad[t]};
+    const int self = tslot[t];
+    double acc[3] = {0, 0, 0};
+    auto add = [&](const Src& sp) {
+        if (MODE == CP_BH_VEL)
+            k_gauss(sp, tg, acc);
+        else if (MODE == CP_BH_STRETCH)
+            k_stretch(sp, tg, acc);
+        else
+            k_diff(sp, tg, acc);
+    };
+    walk_wave(nodes, n_nodes, tg.x, tg.y, tg.z, theta, lane,
+              [&](int ms) {
+                  if (ms != self) add(member(mpos, mstr, mrad, ms));
+              },
+              [&](const BhNode& nd) { add(pseudo(nd)); });
+    wave_sum3(acc);
+    if (lane == 0)
+        for (int q = 0; q < 3; q++) out[3 * t + q] = acc[q];
+}
+
 // direct pairwise stretching or diffusion on every particle i from every
 // other particle j, in j order, as cpCaseVP::particleStrengthUpdate's
 // direct branch calls them (cpCaseVP.cpp:489-496, 507-514): particle i is
@@ -896,6 +981,12 @@ void launch_buffer_rhs(int n, int64_t lda, int m, const double* Cm, const double
 
 void launch_bh_points(const DevBhTree& T, double theta, int m, const double* pts, double* out, hipStream_t s) {
     if (m <= 0) return;
+    if (m <= kBhWaveMax) {
+        hipLaunchKernelGGL(bh_points_wave_kernel, dim3((unsigned)((m + kT / 64 - 1) / (kT / 64))), dim3(kT), 0, s,
+                           T.nodes, T.n_nodes, T.mpos, T.mstr, T.mrad, theta, m, pts, out);
+        CP_LAUNCH_CHECK();
+        return;
+    }
     hipLaunchKernelGGL(bh_points_kernel, dim3((unsigned)((m + kT - 1) / kT)), dim3(kT), 0, s, T.nodes, T.n_nodes,
                        T.mpos, T.mstr, T.mrad, theta, m, pts, out);
     CP_LAUNCH_CHECK();
@@ -924,6 +1015,25 @@ void launch_bh_particles(const DevBhTree& T, int mode, double theta, double* out
 void launch_bh_targets(const DevBhTree& T, int mode, double theta, int m, const double* tpos, const double* tstr,
                        const double* trad, const int* tslot, double* out, hipStream_t s) {
     if (m <= 0) return;
+    if (m <= kBhWaveMax) {
+        const dim3 gw((unsigned)((m + kT / 64 - 1) / (kT / 64)));
+        switch (mode) {
+            case CP_BH_VEL:
+                hipLaunchKernelGGL(bh_targets_wave_kernel<CP_BH_VEL>, gw, dim3(kT), 0, s, T.nodes, T.n_nodes, T.mpos,
+                                   T.mstr, T.mrad, theta, m, tpos, tstr, trad, tslot, out);
+                break;
+            case CP_BH_STRETCH:
+                hipLaunchKernelGGL(bh_targets_wave_kernel<CP_BH_STRETCH>, gw, dim3(kT), 0, s, T.nodes, T.n_nodes,
+                                   T.mpos, T.mstr, T.mrad, theta, m, tpos, tstr, trad, tslot, out);
+                break;
+            default:
+                hipLaunchKernelGGL(bh_targets_wave_kernel<CP_BH_DIFF>, gw, dim3(kT), 0, s, T.nodes, T.n_nodes,
+                                   T.mpos, T.mstr, T.mrad, theta, m, tpos, tstr, trad, tslot, out);
+                break;
+        }
+        CP_LAUNCH_CHECK();
+        return;
+    }
     dim3 grid((unsigned)((m + kT - 1) / kT));
     switch (mode) {
         case CP_BH_VEL:
@@ -960,9 +1070,6 @@ void launch_pairs_direct(int mode, int n, const double* pos, const double* str, 
     CP_LAUNCH_CHECK();
 }
 
-// caller-given targets up to which the algebraic-core walk runs a wavefront
-// per target
-constexpr int kBhcWaveMax = 16384;
 
 int direct_slices(int m, int n) {
     const int tiles = (m + kT - 1) / kT;
