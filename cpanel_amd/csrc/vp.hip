@@ -843,7 +843,7 @@ __global__ __launch_bounds__(64) void panels_near_kernel(const PanelRec* __restr
     constexpr int NE = MODE == PS_VEL ? 3 : 9;  // values per pair
     __shared__ double sp[64][3];
     __shared__ double res[64][NE];
-    __shared__ int rl[64];
+    __shared__ unsigned long long tmask[64];
     const int lane = threadIdx.x, tile = blockIdx.x;
     const int ts = tile * 64 + lane;
     const bool live = ts < m;
@@ -890,12 +890,20 @@ __global__ __launch_bounds__(64) void panels_near_kernel(const PanelRec* __restr
                     for (int b = 0; b < 3; b++) r[3 * a + b] = Gq.a[a][b];
             }
         }
-        rl[lane] = L;
+        // each point lane adds the entries of its point in entry order: the
+        // entries' bits are ORed into their point's mask, and a lane then
+        // walks only its own bits (not all 64 entries)
         for (int e = 0; e < NE; e++) res[lane][e] = r[e];
+        tmask[lane] = 0;
         __syncthreads();
-        for (int k = 0; k < 64; k++)
-            if (rl[k] == lane)
-                for (int e = 0; e < NE; e++) acc[e] += res[k][e];
+        if (L >= 0) atomicOr(&tmask[L], 1ull << lane);
+        __syncthreads();
+        unsigned long long mine = tmask[lane];
+        while (mine) {
+            const int k = __ffsll(mine) - 1;
+            mine &= mine - 1;
+            for (int e = 0; e < NE; e++) acc[e] += res[k][e];
+        }
         __syncthreads();
     };
     RowQueue q;

@@ -66,6 +66,12 @@ inline uint64_t __ballot(int pred) {
     return m;
 }
 inline int __popcll(unsigned long long x) { return __builtin_popcountll(x); }
+// the lanes run one after another between switches, so a plain update is atomic
+inline unsigned long long atomicOr(unsigned long long* p, unsigned long long v) {
+    const unsigned long long o = *p;
+    *p = o | v;
+    return o;
+}
 inline int __ffsll(unsigned long long x) { return __builtin_ffsll((long long)x); }
 inline int __builtin_amdgcn_ds_permute(int addr, int v) {
     return (int)(uint32_t)rp_wave_exchange((uint32_t)v, (addr >> 2) & 63)[rp_lane()];
