@@ -334,11 +334,13 @@ void launch_pack_a(const float* A32, int64_t lda, int n, void* out, hipStream_t 
 void near_tile_lists(int n, int64_t lda, const std::vector<int>& rowptr, const std::vector<int>& cols,
                      std::vector<int>& tptr, std::vector<int>& tkb);
 // Inverts the diagonal blocks of A32 named by the block plan (perm, ptr,
-// off; rows per block <= 1024) in FP64 on the device and stores the inverses
+// off; rows per block <= 3072) in FP64 on the device and stores the inverses
 // in FP32, packed column-major, in out.  Work buffers: X (off[nblocks]
 // doubles), used and piv (n ints each, n = ptr[nblocks]), status (nblocks
-// ints; status[q] != 0 marks a singular block, whose inverse is not written).
+// ints; status[q] != 0 marks a singular block, whose inverse is not written),
+// Y (bj_invert_rows_doubles(nblocks, max_b) doubles).
+int64_t bj_invert_rows_doubles(int nblocks, int max_b);
 void launch_bj_invert(const float* A32, int64_t lda, const int* perm, const int* ptr, const int64_t* off, int nblocks,
-                      int max_b, double* X, int* used, int* piv, int* status, float* out, hipStream_t s);
+                      int max_b, double* X, int* used, int* piv, int* status, double* Y, float* out, hipStream_t s);
 
 }  // namespace cpgpu

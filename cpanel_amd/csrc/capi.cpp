@@ -370,15 +370,16 @@ void ensure_block_jacobi(cp_geom* g) {
     }
     const int nb = P.nblocks();
     const int np = (int)P.perm.size();
-    DevBuf X, flags, st;
+    DevBuf X, flags, st, Y;
     dalloc(c, X, (size_t)std::max<int64_t>(P.off.back(), 1) * sizeof(double));
+    dalloc(c, Y, (size_t)bj_invert_rows_doubles(nb, P.max_b) * sizeof(double));
     dalloc(c, flags, 2 * (size_t)std::max(np, 1) * sizeof(int));
     dalloc(c, st, (size_t)std::max(nb, 1) * sizeof(int));
     {
         TimedScope ts("bj_invert", g->stream);
         launch_bj_invert(g->A32.as<float>(), g->dev.lda, g->bj_perm.as<int>(), g->bj_ptr.as<int>(),
                          g->bj_off.as<int64_t>(), nb, P.max_b, X.as<double>(), flags.as<int>(), flags.as<int>() + np,
-                         st.as<int>(), g->bj_inv.as<float>(), g->stream);
+                         st.as<int>(), Y.as<double>(), g->bj_inv.as<float>(), g->stream);
         // CPGPU_BJ_PREC=fp32: FP32 matrix-core product (default: bf16 operands)
         const char* e = std::getenv("CPGPU_BJ_PREC");
         g->bj_pk_kind = (e && std::string(e) == "fp32") ? kBjPackFp32 : kBjPackBf16;

@@ -40,8 +40,9 @@ void near_tile_lists(int, int64_t, const std::vector<int>&, const std::vector<in
     tptr.clear();
     tkb.clear();
 }
+int64_t bj_invert_rows_doubles(int, int) { return 1; }
 void launch_bj_invert(const float*, int64_t, const int*, const int*, const int64_t*, int, int, double*, int*, int*, int*,
-                      float*, hipStream_t) { nope("bj_invert"); }
+                      double*, float*, hipStream_t) { nope("bj_invert"); }
 }  // namespace cpgpu
 
 // the device octree build needs hipCUB's radix sort and scan, which the
