@@ -329,10 +329,6 @@ int64_t bj_packed_floats(int b);
 void launch_bj_pack(const float* inv, const int* ptr, const int64_t* off, const int64_t* poff, int nblocks, int max_b,
                     float* pk, int kind, hipStream_t s);
 void launch_pack_a(const float* A32, int64_t lda, int n, void* out, hipStream_t s);
-// near-field tile lists of the packed A: for each 32-row tile, the ascending
-// 16-wide K blocks holding an entry of the near-field CSR (rowptr, cols; n rows)
-void near_tile_lists(int n, int64_t lda, const std::vector<int>& rowptr, const std::vector<int>& cols,
-                     std::vector<int>& tptr, std::vector<int>& tkb);
 // Inverts the diagonal blocks of A32 named by the block plan (perm, ptr,
 // off; rows per block <= 3072) in FP64 on the device and stores the inverses
 // in FP32, packed column-major, in out.  Work buffers: X (off[nblocks]

@@ -2029,23 +2029,6 @@ void launch_pack_a(const float* A32, int64_t lda, int n, void* out, hipStream_t 
     CP_LAUNCH_CHECK();
 }
 
-void near_tile_lists(int n, int64_t lda, const std::vector<int>& rowptr, const std::vector<int>& cols,
-                     std::vector<int>& tptr, std::vector<int>& tkb) {
-    const int64_t nrt = lda / 32;
-    const int nkb = packed_kblocks(n);
-    std::vector<uint8_t> mark((size_t)nrt * nkb, 0);
-    for (int i = 0; i < n; i++)
-        for (int e = rowptr[i]; e < rowptr[i + 1]; e++) mark[(size_t)(i / 32) * nkb + cols[e] / 16] = 1;
-    tptr.assign((size_t)nrt + 1, 0);
-    tkb.clear();
-    for (int64_t rt = 0; rt < nrt; rt++) {
-        tptr[rt] = (int)tkb.size();
-        for (int kb = 0; kb < nkb; kb++)
-            if (mark[(size_t)rt * nkb + kb]) tkb.push_back(kb);
-    }
-    tptr[nrt] = (int)tkb.size();
-}
-
 void launch_sigma(int n, int64_t lda, const double* normals, const double* vinf4, double* sigma, hipStream_t s) {
     hipLaunchKernelGGL(sigma_kernel, dim3(nblocks(lda)), dim3(kT), 0, s, n, normals, lda, vinf4, sigma);
     CP_LAUNCH_CHECK();

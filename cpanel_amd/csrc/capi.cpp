@@ -603,6 +603,43 @@ cp_status cp_geometry_clusters(const cp_geom* g, int32_t* cl16, int32_t* ts_orde
     return CP_OK;
 }
 
+// The near-field pairs |c_i - c_j| <= longSide_j (CSR by row, columns
+// ascending, with each entry's row) and the packed 32 x 16 tiles of A that
+// hold one, built on the geometry's stream; two small reads of the counts
+// size the lists.
+static void build_near_lists(cp_geom* g) {
+    cp_ctx* c = g->ctx;
+    hipStream_t s = g->stream;
+    const int n = g->dev.n;
+    const int nrt = (int)(g->dev.lda / 32), nkb = packed_kblocks(n);
+    DevBuf cnt, mark;
+    dalloc(c, cnt, (size_t)(std::max(n, nrt) + 1) * sizeof(int));
+    dalloc(c, g->near_rowptr, (size_t)(n + 1) * sizeof(int));
+    launch_near_count(g->dev, 1.0, cnt.as<int>(), s);
+    launch_exclusive_scan(cnt.as<int>(), n, g->near_rowptr.as<int>(), s);
+    int nnz = 0;
+    CP_HIP(hipMemcpyAsync(&nnz, g->near_rowptr.as<int>() + n, sizeof(int), hipMemcpyDeviceToHost, s));
+    CP_HIP(hipStreamSynchronize(s));
+    g->near_nnz = nnz;
+    dalloc(c, g->near_cols, (size_t)std::max(nnz, 1) * sizeof(int));
+    dalloc(c, g->near_rows, (size_t)std::max(nnz, 1) * sizeof(int));
+    launch_near_fill(g->dev, 1.0, g->near_rowptr.as<int>(), g->near_cols.as<int>(), g->near_rows.as<int>(), s);
+    dalloc(c, mark, std::max<size_t>((size_t)nrt * nkb, 1));
+    CP_HIP(hipMemsetAsync(mark.p, 0, std::max<size_t>((size_t)nrt * nkb, 1), s));
+    launch_near_tile_count(nnz, g->near_rows.as<int>(), g->near_cols.as<int>(), nrt, nkb, mark.as<unsigned char>(),
+                           cnt.as<int>(), s);
+    dalloc(c, g->near_tptr, (size_t)(nrt + 1) * sizeof(int));
+    launch_exclusive_scan(cnt.as<int>(), nrt, g->near_tptr.as<int>(), s);
+    int ntk = 0;
+    CP_HIP(hipMemcpyAsync(&ntk, g->near_tptr.as<int>() + nrt, sizeof(int), hipMemcpyDeviceToHost, s));
+    CP_HIP(hipStreamSynchronize(s));
+    dalloc(c, g->near_tkb, (size_t)std::max(ntk, 1) * sizeof(int));
+    launch_near_tile_fill(mark.as<unsigned char>(), nrt, nkb, g->near_tptr.as<int>(), g->near_tkb.as<int>(), s);
+    dalloc(c, g->near_dA, (size_t)std::max(nnz, 1) * sizeof(double));
+    dalloc(c, g->near_dB, (size_t)std::max(nnz, 1) * sizeof(double));
+    CP_HIP(hipStreamSynchronize(s));  // mark and cnt go back to the pool
+}
+
 cp_status cp_assemble(cp_geom* g, cp_prec prec) {
     if (!g || (prec != CP_FP64 && prec != CP_FP32_REFINE && prec != CP_FP32_REFINE64)) return CP_BAD_ARG;
     if (prec == CP_FP32_REFINE64) {  // both sets: FP64 for refinement, FP32 for the Krylov iterations
@@ -635,23 +672,7 @@ cp_status cp_assemble(cp_geom* g, cp_prec prec) {
                 TimedScope ts("assemble_f32", g->stream);
                 launch_assemble_f32(g->dev, W, lda, g->A32.as<float>(), g->B32.as<float>(), g->stream);
             }
-            if (!g->near_rowptr.p) {
-                std::vector<int> rp, cl;
-                near_pairs(g->host, 1.0, rp, cl);
-                std::vector<int> rows(cl.size());
-                for (int i = 0; i < g->dev.n; i++)
-                    for (int e = rp[i]; e < rp[i + 1]; e++) rows[e] = i;
-                g->near_nnz = (int)cl.size();
-                upload(c, g->near_rowptr, rp);
-                upload(c, g->near_cols, cl);
-                upload(c, g->near_rows, rows);
-                dalloc(c, g->near_dA, std::max<size_t>(cl.size(), 1) * sizeof(double));
-                dalloc(c, g->near_dB, std::max<size_t>(cl.size(), 1) * sizeof(double));
-                std::vector<int> tptr, tkb;
-                near_tile_lists(g->dev.n, lda, rp, cl, tptr, tkb);
-                upload(c, g->near_tptr, tptr);
-                upload(c, g->near_tkb, tkb);
-            }
+            if (!g->near_rowptr.p) build_near_lists(g);
             {
                 TimedScope ts("near_f64", g->stream);
                 launch_near_correction(g->dev, W, lda, g->A32.as<float>(), g->B32.as<float>(), g->near_nnz,

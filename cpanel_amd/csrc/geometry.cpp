@@ -193,50 +193,6 @@ void parallel_for(int n, int nthreads, const std::function<void(int, int)>& body
     for (auto& x : th) x.join();
 }
 
-void near_pairs(const Geometry& G, double factor, std::vector<int>& rowptr, std::vector<int>& cols) {
-    const int n = (int)G.body.size();
-    double h = 0;
-    for (const PanelGeom& p : G.body) h = std::max(h, factor * p.long_side);
-    if (h <= 0) h = 1;
-    auto cell = [&](double v) { return (int64_t)std::floor(v / h); };
-    auto key = [](int64_t a, int64_t b, int64_t c) {
-        return ((uint64_t)(a & 0x1FFFFF) << 42) | ((uint64_t)(b & 0x1FFFFF) << 21) | (uint64_t)(c & 0x1FFFFF);
-    };
-    // bin the panels (columns) by centre; a collocation point i can only be
-    // within factor*L_j <= h of centres in the 27 surrounding cells
-    std::unordered_map<uint64_t, std::vector<int>> grid;
-    for (int j = 0; j < n; j++) {
-        const Vec3& c = G.body[j].center;
-        grid[key(cell(c.x), cell(c.y), cell(c.z))].push_back(j);
-    }
-    std::vector<std::vector<int>> rows(n);
-    parallel_for(n, 0, [&](int a, int b) {
-        for (int i = a; i < b; i++) {
-            const Vec3& q = G.body[i].center;
-            int64_t cx = cell(q.x), cy = cell(q.y), cz = cell(q.z);
-            std::vector<int>& r = rows[i];
-            for (int dx = -1; dx <= 1; dx++)
-                for (int dy = -1; dy <= 1; dy++)
-                    for (int dz = -1; dz <= 1; dz++) {
-                        auto it = grid.find(key(cx + dx, cy + dy, cz + dz));
-                        if (it == grid.end()) continue;
-                        for (int j : it->second) {
-                            Vec3 d = sub(q, G.body[j].center);
-                            if (len(d) <= factor * G.body[j].long_side) r.push_back(j);
-                        }
-                    }
-            std::sort(r.begin(), r.end());
-        }
-    });
-    rowptr.assign(n + 1, 0);
-    cols.clear();
-    for (int i = 0; i < n; i++) {
-        rowptr[i] = (int)cols.size();
-        cols.insert(cols.end(), rows[i].begin(), rows[i].end());
-    }
-    rowptr[n] = (int)cols.size();
-}
-
 bool read_tri_file(const std::string& path, std::vector<double>& xyz, std::vector<int32_t>& tri,
                    std::vector<int32_t>& ids, std::string& err) {
     std::ifstream f(path);

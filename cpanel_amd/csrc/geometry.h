@@ -100,12 +100,6 @@ bool build_geometry(int n_nodes, const double* xyz, int n_tris, const int32_t* t
 bool read_tri_file(const std::string& path, std::vector<double>& xyz, std::vector<int32_t>& tri,
                    std::vector<int32_t>& ids, std::string& err);
 
-// Collocation point / panel pairs closer than factor * longSide of the panel
-// (|c_i - c_j| <= factor * L_j), CSR by row i (collocation point), columns j
-// ascending.  These are the entries whose FP64 value is kept beside the FP32
-// matrices of the mixed-precision path.
-void near_pairs(const Geometry& G, double factor, std::vector<int>& rowptr, std::vector<int>& cols);
-
 // Simple parallel loop over [0, n) on the host using std::thread.
 void parallel_for(int n, int nthreads, const std::function<void(int, int)>& body);
 

@@ -23,7 +23,7 @@ OUT = os.path.join(HERE, "_build")
 
 HOST_TU = ["capi.cpp", "vp_case.cpp", "serve.cpp", "geometry.cpp", "post_host.cpp", "precond.cpp", "vp_tree.cpp",
            "timer.cpp"]
-KERNEL_TU = ["krylov.hip", "post.hip", "vp.hip", "assemble.hip", "noalloc.hip"]
+KERNEL_TU = ["krylov.hip", "post.hip", "vp.hip", "assemble.hip", "noalloc.hip", "nearcsr.hip"]
 
 
 def available():

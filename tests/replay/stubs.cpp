@@ -32,14 +32,9 @@ int packed_kblocks(int n) { return (n + 15) / 16; }
 size_t packed_a_bytes(int64_t, int) { return 256; }
 int64_t bj_packed_floats(int b) { return (int64_t)b * b; }
 void launch_bj_pack(const float*, const int*, const int64_t*, const int64_t*, int, int, float*, int, hipStream_t) { nope("bj_pack"); }
-// the bf16 tile planes and their near-field tile lists feed only the batched
-// products, which are not replayed
+// the bf16 tile planes feed only the batched products, which are not
+// replayed
 void launch_pack_a(const float*, int64_t, int, void*, hipStream_t) {}
-void near_tile_lists(int, int64_t, const std::vector<int>&, const std::vector<int>&, std::vector<int>& tptr,
-                     std::vector<int>& tkb) {
-    tptr.clear();
-    tkb.clear();
-}
 int64_t bj_invert_rows_doubles(int, int) { return 1; }
 void launch_bj_invert(const float*, int64_t, const int*, const int*, const int64_t*, int, int, double*, int*, int*, int*,
                       double*, float*, hipStream_t) { nope("bj_invert"); }

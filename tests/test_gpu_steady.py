@@ -340,6 +340,37 @@ def near_field_mask(g, m):
     return d2 <= ls[None, :] ** 2
 
 
+def test_near_pairs_match_brute_force(ctx):
+    """The near-field pairs the device lists (|c_i - c_j| <= longSide_j, the
+    entries whose FP64 value the mixed path keeps) against all n^2 pairs
+    tested in numpy with the same arithmetic: the same count, and the
+    corrected product equals the product of A32 with A64's values on exactly
+    those entries."""
+    rng = np.random.default_rng(13)
+    for m in (meshgen.wing("4412", 1.0, 6.0, 40, 12, "uniform", 20.0), meshgen.config_c1()):
+        g = ctx.geometry(m)
+        g.assemble(CP_FP64)
+        g.assemble(CP_FP32_REFINE)
+        b = g.body()
+        V = m["xyz"][m["tri"][b["tri_index"]]]
+        e = V - np.roll(V, 1, axis=1)
+        ls = np.sqrt((e[..., 0] * e[..., 0] + e[..., 1] * e[..., 1]) + e[..., 2] * e[..., 2]).max(axis=1)
+        c = b["center"]
+        d = c[:, None, :] - c[None, :, :]
+        near = np.sqrt((d[..., 0] * d[..., 0] + d[..., 1] * d[..., 1]) + d[..., 2] * d[..., 2]) <= ls[None, :]
+        del d
+        rep = g.solver_report()
+        say("[gpu] near pairs n=%d: device %d, numpy %d" % (g.nb, rep["near_nnz"], near.sum()))
+        assert rep["near_nnz"] == int(near.sum())
+        A64, _ = g.matrices(CP_FP64)
+        A32, _ = g.matrices(CP_FP32_REFINE)
+        X = rng.normal(size=(g.nb, 8))
+        Y3 = g.matvec(X, 3)
+        ref = np.where(near, A64, A32) @ X
+        assert np.abs(Y3 - ref).max() <= 1e-12 * np.abs(ref).max()
+        g.release()
+
+
 @pytest.mark.parametrize("shape", ["0,1,8", "0,2,16", "0,4,4", "1,4,8", "2,4,32", "3,4,16", "4,4,32", "3,4,4",
                                    "5,4,4", "5,4,32", "6,4,8", "7,4,4", "8,4,16", "9,4,32",
                                    "10,4,8", "11,4,16", "12,4,4", "10,4,32", "13,2,8", "14,2,4"])
