@@ -75,6 +75,11 @@ struct cp_vp {
     bool converged = false;
     DevBuf w2rec, fil_dev;
     DevGeom row2;
+    // the particles' positions, strengths and radii on the device: uploaded
+    // when the host state (version ver, bumped by every update) is newer than
+    // the copy (dev_ver); the tree's leaves hold version leaves_ver
+    DevBuf dpos, dstr, drad;
+    uint64_t ver = 1, dev_ver = 0, leaves_ver = 0;
 };
 
 namespace {
@@ -93,6 +98,35 @@ void from_dev(std::vector<double>& v, const DevBuf& b, size_t count, hipStream_t
     CP_HIP(hipStreamSynchronize(s));
 }
 
+// The particle state on the device, uploaded once per change.  Every sum
+// ends with a synchronisation of the stream before the host updates the
+// state, so no copy is in flight when the host arrays change.
+void sync_particles(cp_vp* v) {
+    if (v->dev_ver == v->ver) return;
+    cp_ctx* c = v->g->ctx;
+    hipStream_t s = v->g->stream;
+    auto put = [&](DevBuf& b, const std::vector<double>& h) {
+        const size_t bytes = std::max<size_t>(h.size(), 1) * sizeof(double);
+        if (b.bytes < bytes) dalloc(c, b, bytes + bytes / 8);  // headroom: the particles grow every step
+        if (!h.empty()) CP_HIP(hipMemcpyAsync(b.p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    };
+    put(v->dpos, v->ppos);
+    put(v->dstr, v->pstr);
+    put(v->drad, v->prad);
+    v->dev_ver = v->ver;
+}
+
+// points on the device: the particles' own positions when pts is the
+// particle array, else a copy in tmp
+const double* dev_points(cp_vp* v, const std::vector<double>& pts, DevBuf& tmp) {
+    if (&pts == &v->ppos) {
+        sync_particles(v);
+        return v->dpos.as<double>();
+    }
+    to_dev(v->g->ctx, tmp, pts, v->g->stream);
+    return tmp.as<double>();
+}
+
 // body panels (sigma, mu) and the first buffer row (mu1): bodyPanel::panelV,
 // wakePanel::panelV summed over the panels
 std::vector<double> vel_panels(cp_vp* v, const std::vector<double>& pts) {
@@ -106,12 +140,12 @@ std::vector<double> vel_panels(cp_vp* v, const std::vector<double>& pts) {
     to_dev(c, dsig, v->sigma, s);
     to_dev(c, dmu, v->mu, s);
     to_dev(c, dmw, v->mu1, s);
-    to_dev(c, dpts, pts, s);
+    const double* P = dev_points(v, pts, dpts);
     dalloc(c, dpart, p2p_part_doubles(m, g->dev.n + g->dev.nw) * sizeof(double));
     dalloc(c, dout, (size_t)3 * m * sizeof(double));
     {
         TimedScope ts("panel_to_points", s);
-        launch_panel_to_points(g->dev, dsig.as<double>(), dmu.as<double>(), dmw.as<double>(), m, dpts.as<double>(),
+        launch_panel_to_points(g->dev, dsig.as<double>(), dmu.as<double>(), dmw.as<double>(), m, P,
                                dpart.as<double>(), dout.as<double>(), s, v->f32);
     }
     from_dev(out, dout, (size_t)3 * m, s);
@@ -128,11 +162,11 @@ std::vector<double> vel_row2(cp_vp* v, const std::vector<double>& pts) {
     if (m == 0) return out;
     DevBuf dmw, dpts, dpart, dout;
     to_dev(c, dmw, v->mu2, s);
-    to_dev(c, dpts, pts, s);
+    const double* P = dev_points(v, pts, dpts);
     dalloc(c, dpart, p2p_part_doubles(m, v->m) * sizeof(double));
     dalloc(c, dout, (size_t)3 * m * sizeof(double));
-    launch_panel_to_points(v->row2, nullptr, nullptr, dmw.as<double>(), m, dpts.as<double>(), dpart.as<double>(),
-                           dout.as<double>(), s, v->f32);
+    launch_panel_to_points(v->row2, nullptr, nullptr, dmw.as<double>(), m, P, dpart.as<double>(), dout.as<double>(),
+                           s, v->f32);
     from_dev(out, dout, (size_t)3 * m, s);
     return out;
 }
@@ -146,10 +180,9 @@ std::vector<double> vel_filaments(cp_vp* v, const std::vector<double>& pts) {
     hipStream_t s = g->stream;
     DevBuf dstr, dpts, dout;
     to_dev(c, dstr, v->fil_str, s);
-    to_dev(c, dpts, pts, s);
+    const double* P = dev_points(v, pts, dpts);
     dalloc(c, dout, (size_t)3 * m * sizeof(double));
-    launch_filament_points(v->m, v->fil_dev.as<double>(), dstr.as<double>(), m, dpts.as<double>(), dout.as<double>(),
-                           s);
+    launch_filament_points(v->m, v->fil_dev.as<double>(), dstr.as<double>(), m, P, dout.as<double>(), s);
     from_dev(out, dout, (size_t)3 * m, s);
     return out;
 }
@@ -163,20 +196,17 @@ std::vector<double> vel_direct(cp_vp* v, const std::vector<double>& pts) {
     cp_geom* g = v->g;
     cp_ctx* c = g->ctx;
     hipStream_t s = g->stream;
-    DevBuf dp, ds, dr, dpts, dpart, dout;
-    to_dev(c, dp, v->ppos, s);
-    to_dev(c, ds, v->pstr, s);
-    to_dev(c, dpts, pts, s);
+    DevBuf dpts, dpart, dout;
+    const double* P = dev_points(v, pts, dpts);
+    sync_particles(v);
     dalloc(c, dpart, direct_part_doubles(m, np) * sizeof(double));
     dalloc(c, dout, (size_t)3 * m * sizeof(double));
-    if (v->alg) {
-        launch_core_direct(CP_BH_VEL, v->core, v->f32, np, dp.as<double>(), ds.as<double>(), m, dpts.as<double>(),
-                           nullptr, false, dpart.as<double>(), dout.as<double>(), s);
-    } else {
-        to_dev(c, dr, v->prad, s);
-        launch_gauss_points(np, dp.as<double>(), ds.as<double>(), dr.as<double>(), m, dpts.as<double>(),
+    if (v->alg)
+        launch_core_direct(CP_BH_VEL, v->core, v->f32, np, v->dpos.as<double>(), v->dstr.as<double>(), m, P, nullptr,
+                           false, dpart.as<double>(), dout.as<double>(), s);
+    else
+        launch_gauss_points(np, v->dpos.as<double>(), v->dstr.as<double>(), v->drad.as<double>(), m, P,
                             dpart.as<double>(), dout.as<double>(), s);
-    }
     from_dev(out, dout, (size_t)3 * m, s);
     return out;
 }
@@ -185,15 +215,12 @@ std::vector<double> vel_direct(cp_vp* v, const std::vector<double>& pts) {
 void refresh_tree(cp_vp* v) {
     cp_tree* t = v->tree.get();
     if (!t || t->dev.n == 0) return;
-    cp_geom* g = v->g;
-    cp_ctx* c = g->ctx;
-    hipStream_t s = g->stream;
-    DevBuf dp, ds, dr;
-    to_dev(c, dp, v->ppos, s);
-    to_dev(c, ds, v->pstr, s);
-    to_dev(c, dr, v->prad, s);
-    launch_bh_refresh(t->dev, dp.as<double>(), ds.as<double>(), dr.as<double>(), s);
+    if (v->leaves_ver == v->ver) return;
+    hipStream_t s = v->g->stream;
+    sync_particles(v);
+    launch_bh_refresh(t->dev, v->dpos.as<double>(), v->dstr.as<double>(), v->drad.as<double>(), s);
     CP_HIP(hipStreamSynchronize(s));
+    v->leaves_ver = v->ver;
 }
 
 bool tree_usable(const cp_vp* v) { return v->tree && v->tree->dev.n_nodes > 1; }
@@ -231,10 +258,11 @@ std::vector<double> bh_on_particles(cp_vp* v, int mode) {
     cp_geom* g = v->g;
     cp_ctx* c = g->ctx;
     hipStream_t s = g->stream;
-    DevBuf dp, ds, dr, dsl, dout;
-    to_dev(c, dp, v->ppos, s);
-    to_dev(c, ds, v->pstr, s);
-    to_dev(c, dr, v->prad, s);
+    DevBuf dsl, dout;
+    sync_particles(v);
+    const double* dp = v->dpos.as<double>();
+    const double* ds = v->dstr.as<double>();
+    const double* dr = v->drad.as<double>();
     // each particle's slot in the tree; particles shed since the build have
     // none (-1)
     dalloc(c, dsl, (size_t)np * sizeof(int));
@@ -245,11 +273,11 @@ std::vector<double> bh_on_particles(cp_vp* v, int mode) {
     {
         TimedScope ts("bh_eval", s);
         if (v->alg)
-            launch_core_bh(v->tree->dev, mode, v->tree->theta, v->core, v->f32, np, dp.as<double>(), ds.as<double>(),
-                           dsl.as<int>(), nullptr, dout.as<double>(), s);
+            launch_core_bh(v->tree->dev, mode, v->tree->theta, v->core, v->f32, np, dp, ds, dsl.as<int>(), nullptr,
+                           dout.as<double>(), s);
         else
-            launch_bh_targets(v->tree->dev, mode, v->tree->theta, np, dp.as<double>(), ds.as<double>(),
-                              dr.as<double>(), dsl.as<int>(), dout.as<double>(), s);
+            launch_bh_targets(v->tree->dev, mode, v->tree->theta, np, dp, ds, dr, dsl.as<int>(), dout.as<double>(),
+                              s);
     }
     from_dev(out, dout, (size_t)3 * np, s);
     return out;
@@ -262,17 +290,17 @@ std::vector<double> pairs_direct(cp_vp* v, int mode) {
     cp_geom* g = v->g;
     cp_ctx* c = g->ctx;
     hipStream_t s = g->stream;
-    DevBuf dp, ds, dr, dpart, dout;
-    to_dev(c, dp, v->ppos, s);
-    to_dev(c, ds, v->pstr, s);
-    to_dev(c, dr, v->prad, s);
+    DevBuf dpart, dout;
+    sync_particles(v);
+    const double* dp = v->dpos.as<double>();
+    const double* ds = v->dstr.as<double>();
     dalloc(c, dout, (size_t)3 * np * sizeof(double));
     if (v->alg) {
         dalloc(c, dpart, direct_part_doubles(np, np) * sizeof(double));
-        launch_core_direct(mode, v->core, v->f32, np, dp.as<double>(), ds.as<double>(), np, dp.as<double>(),
-                           ds.as<double>(), true, dpart.as<double>(), dout.as<double>(), s);
+        launch_core_direct(mode, v->core, v->f32, np, dp, ds, np, dp, ds, true, dpart.as<double>(), dout.as<double>(),
+                           s);
     } else {
-        launch_pairs_direct(mode, np, dp.as<double>(), ds.as<double>(), dr.as<double>(), dout.as<double>(), s);
+        launch_pairs_direct(mode, np, dp, ds, v->drad.as<double>(), dout.as<double>(), s);
     }
     from_dev(out, dout, (size_t)3 * np, s);
     return out;
@@ -286,18 +314,18 @@ std::vector<double> panel_stretch(cp_vp* v) {
     cp_geom* g = v->g;
     cp_ctx* c = g->ctx;
     hipStream_t s = g->stream;
-    DevBuf dsig, dmu, dmw, dp, ds, dpart, dout;
+    DevBuf dsig, dmu, dmw, dpart, dout;
     to_dev(c, dsig, v->sigma, s);
     to_dev(c, dmu, v->mu, s);
     to_dev(c, dmw, v->mu1, s);
-    to_dev(c, dp, v->ppos, s);
-    to_dev(c, ds, v->pstr, s);
+    sync_particles(v);
     dalloc(c, dpart, p2p_part_doubles(np, g->dev.n + g->dev.nw) * sizeof(double));
     dalloc(c, dout, (size_t)3 * np * sizeof(double));
     {
         TimedScope ts("panel_stretching", s);
-        launch_panel_stretching(g->dev, dsig.as<double>(), dmu.as<double>(), dmw.as<double>(), np, dp.as<double>(),
-                                ds.as<double>(), dpart.as<double>(), dout.as<double>(), s, v->f32);
+        launch_panel_stretching(g->dev, dsig.as<double>(), dmu.as<double>(), dmw.as<double>(), np,
+                                v->dpos.as<double>(), v->dstr.as<double>(), dpart.as<double>(), dout.as<double>(), s,
+                                v->f32);
     }
     from_dev(out, dout, (size_t)3 * np, s);
     return out;
@@ -319,16 +347,15 @@ void build_tree(cp_vp* v) {
     bool built = false;
     if (np > 0) {  // on the device (the host only when the keys are too shallow)
         hipStream_t s = g->stream;
-        DevBuf dp, ds, dr;
-        to_dev(c, dp, v->ppos, s);
-        to_dev(c, ds, v->pstr, s);
-        to_dev(c, dr, v->prad, s);
+        sync_particles(v);
         TimedScope ts("bh_build", s);
-        built = build_tree_device(c, np, dp.as<double>(), ds.as<double>(), dr.as<double>(), maxm, s, t.get());
+        built = build_tree_device(c, np, v->dpos.as<double>(), v->dstr.as<double>(), v->drad.as<double>(), maxm, s,
+                                  t.get());
         CP_HIP(hipStreamSynchronize(s));
     }
     if (!built) build_tree_host(c, np, v->ppos.data(), v->pstr.data(), v->prad.data(), maxm, t.get());
     v->tree = std::move(t);
+    v->leaves_ver = 0;  // refreshed before its first use, as every tree is
 }
 
 // ---------------------------------------------------------------- helpers
@@ -450,6 +477,7 @@ void convect_particles(cp_vp* v) {
         }
         for (int q = 0; q < 3; q++) p0[q] = vel[3 * i + q];
     }
+    v->ver++;
 }
 
 // cpCaseVP::collapseBufferWake (cpCaseVP.cpp:319-367): one particle per
@@ -545,6 +573,7 @@ void collapse_buffer(cp_vp* v) {
             v->pshed.push_back(v->ts);
             v->pparent.push_back(i);
             v->pseed.insert(v->pseed.end(), {sd[0], sd[1], sd[2]});
+            v->ver++;
         }
     }
     if (!v->have_fil) {  // vortexFil from pointsInOrder 2 -> 3
@@ -617,6 +646,7 @@ void strength_update(cp_vp* v) {
         }
         for (int q = 0; q < 3; q++) d0[q] = upd[3 * i + q];
     }
+    v->ver++;
 }
 
 // sigma_i = -(V_i . n_i) with V_i = Vinf + particle and filament velocity at
