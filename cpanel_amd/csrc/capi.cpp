@@ -612,10 +612,13 @@ static void build_near_lists(cp_geom* g) {
     hipStream_t s = g->stream;
     const int n = g->dev.n;
     const int nrt = (int)(g->dev.lda / 32), nkb = packed_kblocks(n);
-    DevBuf cnt, mark;
+    DevBuf cnt, qcnt, mark, reach;
     dalloc(c, cnt, (size_t)(std::max(n, nrt) + 1) * sizeof(int));
+    dalloc(c, qcnt, (size_t)std::max(near_chunks(g->dev), 1) * std::max(n, 1) * sizeof(int));
+    dalloc(c, reach, (size_t)std::max(g->dev.ntiles, 1) * sizeof(double));
     dalloc(c, g->near_rowptr, (size_t)(n + 1) * sizeof(int));
-    launch_near_count(g->dev, 1.0, cnt.as<int>(), s);
+    launch_near_reach(g->dev, 1.0, reach.as<double>(), s);
+    launch_near_count(g->dev, 1.0, reach.as<double>(), qcnt.as<int>(), cnt.as<int>(), s);
     launch_exclusive_scan(cnt.as<int>(), n, g->near_rowptr.as<int>(), s);
     int nnz = 0;
     CP_HIP(hipMemcpyAsync(&nnz, g->near_rowptr.as<int>() + n, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -623,7 +626,8 @@ static void build_near_lists(cp_geom* g) {
     g->near_nnz = nnz;
     dalloc(c, g->near_cols, (size_t)std::max(nnz, 1) * sizeof(int));
     dalloc(c, g->near_rows, (size_t)std::max(nnz, 1) * sizeof(int));
-    launch_near_fill(g->dev, 1.0, g->near_rowptr.as<int>(), g->near_cols.as<int>(), g->near_rows.as<int>(), s);
+    launch_near_fill(g->dev, 1.0, reach.as<double>(), qcnt.as<int>(), g->near_rowptr.as<int>(),
+                     g->near_cols.as<int>(), g->near_rows.as<int>(), s);
     dalloc(c, mark, std::max<size_t>((size_t)nrt * nkb, 1));
     CP_HIP(hipMemsetAsync(mark.p, 0, std::max<size_t>((size_t)nrt * nkb, 1), s));
     launch_near_tile_count(nnz, g->near_rows.as<int>(), g->near_cols.as<int>(), nrt, nkb, mark.as<unsigned char>(),
@@ -637,7 +641,7 @@ static void build_near_lists(cp_geom* g) {
     launch_near_tile_fill(mark.as<unsigned char>(), nrt, nkb, g->near_tptr.as<int>(), g->near_tkb.as<int>(), s);
     dalloc(c, g->near_dA, (size_t)std::max(nnz, 1) * sizeof(double));
     dalloc(c, g->near_dB, (size_t)std::max(nnz, 1) * sizeof(double));
-    CP_HIP(hipStreamSynchronize(s));  // mark and cnt go back to the pool
+    CP_HIP(hipStreamSynchronize(s));  // the work buffers go back to the pool
 }
 
 cp_status cp_assemble(cp_geom* g, cp_prec prec) {

@@ -57,14 +57,20 @@ void launch_near_correction(const DevGeom& g, const double* W, int64_t ldw, cons
                             int nnz, const int* rows, const int* cols, double* dA, double* dB, hipStream_t s);
 
 // ---- near-field pairs and tiles on the device (nearcsr.hip) ----
-// Pairs |c_i - c_j| <= factor * longSide_j of the body panels: cnt[i] = row
-// i's count (n ints), then, from rowptr = exclusive scan of cnt, each row's
-// columns ascending in cols and its row index in rows (rowptr[n] entries).
-void launch_near_count(const DevGeom& g, double factor, int* cnt, hipStream_t s);
-void launch_near_fill(const DevGeom& g, double factor, const int* rowptr, int* cols, int* rows, hipStream_t s);
+// Pairs |c_i - c_j| <= factor * longSide_j of the body panels: reach[t] =
+// factor * the largest longSide of 64-panel tile t (g.ntiles doubles); then
+// the counts per column chunk (cnt: near_chunks(g) * n ints, left holding
+// each chunk's offset within its row) and per row (total: n ints); then,
+// from rowptr = exclusive scan of total, each row's columns ascending in
+// cols and its row index in rows (rowptr[n] entries).
+void launch_near_reach(const DevGeom& g, double factor, double* reach, hipStream_t s);
+int near_chunks(const DevGeom& g);
+void launch_near_count(const DevGeom& g, double factor, const double* reach, int* cnt, int* total, hipStream_t s);
+void launch_near_fill(const DevGeom& g, double factor, const double* reach, const int* cnt, const int* rowptr,
+                      int* cols, int* rows, hipStream_t s);
 // out[0] = 0, out[t + 1] = in[0] + ... + in[t] (n + 1 ints; one workgroup)
 void launch_exclusive_scan(const int* in, int n, int* out, hipStream_t s);
-// The 32-row x 16-column tiles holding a pair: mark (nkb x nrt bytes,
+// The 32-row x 16-column tiles holding a pair: mark (nrt x nkb bytes,
 // zeroed by the caller) and cnt[rt] = row tile rt's count (nrt ints), then
 // from tptr = exclusive scan of cnt, each row tile's column blocks ascending.
 void launch_near_tile_count(int nnz, const int* rows, const int* cols, int nrt, int nkb, unsigned char* mark,
