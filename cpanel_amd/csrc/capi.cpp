@@ -765,7 +765,6 @@ cp_status cp_solve(cp_geom* g, cp_prec prec, int32_t k, const cp_freestream* fs,
             return fail(c, CP_NOT_ASSEMBLED, "call cp_assemble(CP_FP32_REFINE) first");
         if (prec == CP_FP32_REFINE64 && !(g->have32 && g->have64))
             return fail(c, CP_NOT_ASSEMBLED, "call cp_assemble(CP_FP32_REFINE64) first");
-        ensure_post(g);
         const int n = g->dev.n;
         const int64_t lda = g->dev.lda;
         hipStream_t s = g->stream;
@@ -803,6 +802,10 @@ cp_status cp_solve(cp_geom* g, cp_prec prec, int32_t k, const cp_freestream* fs,
             g->batched->solve(g->A32.as<float>(), g->B32.as<float>(), g->normals.as<double>(), dv4.as<double>(), k,
                               bo, dmu.as<double>(), lda, stats.data(), s);
         }
+        // the post-processing tables (built on a host thread since the
+        // geometry's upload) are joined only now, so that a new geometry's
+        // table build overlaps its block inversion and first solve
+        ensure_post(g);
         int nblk = post_partial_blocks(n);
         dalloc(c, dpart, (size_t)nblk * k * 6 * sizeof(double));
         dalloc(c, dres, (size_t)16 * k * sizeof(double));
@@ -875,7 +878,6 @@ cp_status cp_solve_group(int32_t G, cp_geom* const* geoms, cp_prec prec, const i
         std::vector<BatchedOpts> bo(G);
         for (int i = 0; i < G; i++) {
             cp_geom* g = geoms[i];
-            ensure_post(g);
             prepare_mixed(g, prec, &opts[i]);
             CP_HIP(hipStreamSynchronize(g->stream));
             bo[i] = mixed_opts(&opts[i], g->dev.n);
@@ -899,6 +901,7 @@ cp_status cp_solve_group(int32_t G, cp_geom* const* geoms, cp_prec prec, const i
         DevBuf gbuf;
         dalloc(c, gbuf, BatchedSolver::group_bytes(G));
         BatchedSolver::solve_group(items, gbuf.p, s);
+        for (int i = 0; i < G; i++) ensure_post(geoms[i]);  // after the solve, as in cp_solve
         for (int i = 0; i < G; i++) {
             cp_geom* g = geoms[i];
             const int n = g->dev.n;

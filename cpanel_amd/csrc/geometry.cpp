@@ -12,6 +12,7 @@
 // Implementation is array based: edges through a hash map, wake welding
 // through a uniform grid instead of the all-pairs scan.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <fstream>
 #include <functional>
@@ -88,7 +89,22 @@ struct Builder {
     std::vector<int> group;         // surface / wake group index
     std::vector<std::array<int, 3>> pedge;
     std::vector<EdgeRec> edges;
-    std::unordered_map<uint64_t, int> emap;
+    // edge index by node pair: open addressing over a power-of-two table
+    // (key = lo << 32 | hi + 1, 0 = empty), edges numbered by first use
+    std::vector<uint64_t> ekey;
+    std::vector<int> eval;
+    int eshift = 60;  // 64 - log2(table size): Fibonacci hashing takes the top bits
+    void edges_reserve(size_t n) {
+        size_t cap = 16;
+        eshift = 60;
+        while (cap < 2 * n) {
+            cap *= 2;
+            eshift--;
+        }
+        ekey.assign(cap, 0);
+        eval.assign(cap, -1);
+    }
+    size_t eslot(uint64_t key) const { return (size_t)((key * 0x9E3779B97F4A7C15ull) >> eshift); }
     // per panel state
     std::vector<char> upper, lower, te_panel;
     std::vector<int> up_pan, lo_pan;
@@ -106,14 +122,30 @@ struct Builder {
 
     int edge_of(int a, int b) {
         uint64_t lo = (uint64_t)std::min(a, b), hi = (uint64_t)std::max(a, b);
-        uint64_t key = (lo << 32) | hi;
-        auto it = emap.find(key);
-        if (it != emap.end()) return it->second;
+        const uint64_t key = ((lo << 32) | hi) + 1;
+        if (2 * (edges.size() + 1) > ekey.size()) {  // keep the load under one half
+            std::vector<uint64_t> ok = std::move(ekey);
+            std::vector<int> ov = std::move(eval);
+            edges_reserve(ok.size());
+            for (size_t q = 0; q < ok.size(); q++)
+                if (ok[q]) {
+                    size_t h = eslot(ok[q]);
+                    while (ekey[h]) h = (h + 1) & (ekey.size() - 1);
+                    ekey[h] = ok[q];
+                    eval[h] = ov[q];
+                }
+        }
+        size_t h = eslot(key);
+        while (ekey[h]) {
+            if (ekey[h] == key) return eval[h];
+            h = (h + 1) & (ekey.size() - 1);
+        }
         EdgeRec e;
         e.n1 = a;
         e.n2 = b;
         edges.push_back(e);
-        emap.emplace(key, (int)edges.size() - 1);
+        ekey[h] = key;
+        eval[h] = (int)edges.size() - 1;
         return (int)edges.size() - 1;
     }
 
@@ -507,15 +539,23 @@ bool build_geometry(int nn, const double* xyz, int nt, const int32_t* tri_in, co
     B.lo_pan.assign(nt, -1);
     B.wte.assign(nt, 0);
     B.nbr.resize(nt);
+    // each panel's own geometry (no order dependence) on host threads; the
+    // edges, groups and trailing-edge checks below follow the file order
+    parallel_for(nt, 0, [&](int a, int b) {
+        for (int i = a; i < b; i++) {
+            PanelGeom& p = B.pan[i];
+            p.nv = 3;
+            for (int k = 0; k < 3; k++) {
+                p.node[k] = B.tri[3 * i + k];
+                p.v[k] = B.nodes[p.node[k]];
+            }
+            setup_panel(p);
+        }
+    });
+    B.edges_reserve(3 * (size_t)nt);
     for (int i = 0; i < nt; i++) {
         PanelGeom& p = B.pan[i];
-        p.nv = 3;
-        for (int k = 0; k < 3; k++) {
-            p.node[k] = B.tri[3 * i + k];
-            p.v[k] = B.nodes[p.node[k]];
-        }
         for (int k = 0; k < 3; k++) B.pedge[i][k] = B.edge_of(p.node[k], p.node[(k + 1) % 3]);
-        setup_panel(p);
         bool newGroup = (i == 0 || ids[i] != ids[i - 1]);
         if (ids[i] <= 1000) {
             if (newGroup) B.surf_lifting.push_back(0);
@@ -652,55 +692,64 @@ bool build_geometry(int nn, const double* xyz, int nt, const int32_t* tri_in, co
     // --- clusters (bodyPanel::setCluster, bodyPanel.cpp:402-487)
     G.cluster.assign(16 * (size_t)nb, -1);
     G.ts_order.assign(nb, 3);
-    for (int bi = 0; bi < nb; bi++) {
-        int fi = border[bi];
-        int dim = tip[fi] ? 2 : 3;
-        int order = 3;
-        size_t buffer = 10;
-        size_t nObs = (size_t)(fact(order + dim) / (fact(dim) * fact(order)) - 1) + buffer;
-        size_t nPan = (nObs + 1) / 2;
-        std::vector<int> cl{fi};
-        size_t first = 0;
-        bool upF = B.upper[fi], loF = B.lower[fi];
-        Vec3 n = B.pan[fi].normal;
-        auto accept = [&](int o) {
-            if ((upF && B.lower[o]) || (loF && B.upper[o])) return false;
-            if (tip[fi] != tip[o]) return false;
-            double d = dot(B.pan[o].normal, n);
-            d = std::min(1.0, std::max(-1.0, d));
-            return std::acos(d) < 5 * kPi / 6 && std::find(cl.begin(), cl.end(), o) == cl.end() && o != fi;
-        };
-        while (cl.size() < nPan) {
-            std::vector<int> add;
-            for (size_t i = first; i < cl.size(); i++)
-                for (int o : B.nbr[cl[i]])
-                    if (accept(o)) {
-                        if (B.upper[o]) upF = true;
-                        else if (B.lower[o]) loF = true;
-                        if (std::find(add.begin(), add.end(), o) == add.end()) add.push_back(o);
+    // each panel's cluster depends only on the finished topology and fills
+    // only its own slots: built on host threads
+    std::atomic<bool> too_big{false};
+    parallel_for(nb, 0, [&](int bi0, int bi1) {
+        for (int bi = bi0; bi < bi1; bi++) {
+            int fi = border[bi];
+            int dim = tip[fi] ? 2 : 3;
+            int order = 3;
+            size_t buffer = 10;
+            size_t nObs = (size_t)(fact(order + dim) / (fact(dim) * fact(order)) - 1) + buffer;
+            size_t nPan = (nObs + 1) / 2;
+            std::vector<int> cl{fi};
+            size_t first = 0;
+            bool upF = B.upper[fi], loF = B.lower[fi];
+            Vec3 n = B.pan[fi].normal;
+            auto accept = [&](int o) {
+                if ((upF && B.lower[o]) || (loF && B.upper[o])) return false;
+                if (tip[fi] != tip[o]) return false;
+                double d = dot(B.pan[o].normal, n);
+                d = std::min(1.0, std::max(-1.0, d));
+                return std::acos(d) < 5 * kPi / 6 && std::find(cl.begin(), cl.end(), o) == cl.end() && o != fi;
+            };
+            while (cl.size() < nPan) {
+                std::vector<int> add;
+                for (size_t i = first; i < cl.size(); i++)
+                    for (int o : B.nbr[cl[i]])
+                        if (accept(o)) {
+                            if (B.upper[o]) upF = true;
+                            else if (B.lower[o]) loF = true;
+                            if (std::find(add.begin(), add.end(), o) == add.end()) add.push_back(o);
+                        }
+                if (add.empty()) {
+                    while (cl.size() < nPan + 1 && order >= 1) {
+                        if (buffer > 0) buffer--;
+                        else order--;
+                        nObs = (size_t)(fact(order + dim) / (fact(dim) * fact(order))) + buffer;
+                        nPan = (nObs + 1) / 2;
                     }
-            if (add.empty()) {
-                while (cl.size() < nPan + 1 && order >= 1) {
-                    if (buffer > 0) buffer--;
-                    else order--;
-                    nObs = (size_t)(fact(order + dim) / (fact(dim) * fact(order))) + buffer;
-                    nPan = (nObs + 1) / 2;
+                    if (nPan + 1 < cl.size()) cl.resize(nPan + 1);
+                    break;
                 }
-                if (nPan + 1 < cl.size()) cl.resize(nPan + 1);
-                break;
+                first = cl.size();
+                for (int o : add) {
+                    cl.push_back(o);
+                    if (cl.size() == nPan + 1) break;
+                }
             }
-            first = cl.size();
-            for (int o : add) {
-                cl.push_back(o);
-                if (cl.size() == nPan + 1) break;
+            if (cl.size() > 16) {
+                too_big = true;
+                continue;
             }
+            for (size_t k = 0; k < cl.size(); k++) G.cluster[16 * (size_t)bi + k] = bidx[cl[k]];
+            G.ts_order[bi] = order;
         }
-        if (cl.size() > 16) {
-            err = "cluster larger than 16 panels";
-            return false;
-        }
-        for (size_t k = 0; k < cl.size(); k++) G.cluster[16 * (size_t)bi + k] = bidx[cl[k]];
-        G.ts_order[bi] = order;
+    });
+    if (too_big) {
+        err = "cluster larger than 16 panels";
+        return false;
     }
 
     // --- outputs
