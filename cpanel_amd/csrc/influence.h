@@ -204,20 +204,15 @@ CP_HD void pnt_dub_v(const R& P, const double* pjk, double r, double* out) {
 // bodyPanel::panelVInf (bodyPanel.cpp:305-348) in FP64: source and doublet
 // velocity of a unit-strength panel at p.  Far field: point doublet and the
 // point source with the reference's sign (negated, bodyPanel.cpp:314).
-CP_HD void panel_v_inf_f64(const PanelRec& P, const double* p, double* vs, double* vd) {
+// the edge sums of panel_v_inf_f64 (a point the far test has excluded); NV:
+// the vertex count when known at compile time (3: a body triangle), 0 to read
+// it from the record
+template <int NV>
+CP_HD void panel_v_near_nv(const PanelRec& P, const double* p, double* vs, double* vd) {
 #pragma clang fp contract(off)
     double pjk[3] = {p[0] - P.c[0], p[1] - P.c[1], p[2] - P.c[2]};
-    double r2 = sq_norm3(pjk[0], pjk[1], pjk[2]);
-    if (is_far(P, r2)) {
-        double r = sqrt(r2);
-        double den = 4 * kPI * (r * r * r);  // r^3 (the reference's pow(r, 3), to a few ulp)
-        const double f = P.area / den;
-        for (int k = 0; k < 3; k++) vs[k] = -1.0 * (pjk[k] * f);
-        pnt_dub_v(P, pjk, r, vd);
-        return;
-    }
     double PN = pjk[0] * P.n[0] + pjk[1] * P.n[1] + pjk[2] * P.n[2];
-    const int nv = P.nv;
+    const int nv = NV > 0 ? NV : P.nv;
     double S[3] = {0, 0, 0}, D[3] = {0, 0, 0};
     for (int k = 0; k < nv; k++) {
         int k2 = (k + 1 == nv) ? 0 : k + 1;
@@ -249,17 +244,27 @@ CP_HD void panel_v_inf_f64(const PanelRec& P, const double* p, double* vs, doubl
     }
 }
 
-// panel::dubVInf (panel.cpp:666-705) in FP64 (wake panels)
-CP_HD void dub_v_inf_f64(const PanelRec& P, const double* p, double* vd) {
+CP_HD void panel_v_inf_f64(const PanelRec& P, const double* p, double* vs, double* vd) {
 #pragma clang fp contract(off)
     double pjk[3] = {p[0] - P.c[0], p[1] - P.c[1], p[2] - P.c[2]};
     double r2 = sq_norm3(pjk[0], pjk[1], pjk[2]);
     if (is_far(P, r2)) {
-        pnt_dub_v(P, pjk, sqrt(r2), vd);
+        double r = sqrt(r2);
+        double den = 4 * kPI * (r * r * r);  // r^3 (the reference's pow(r, 3), to a few ulp)
+        const double f = P.area / den;
+        for (int k = 0; k < 3; k++) vs[k] = -1.0 * (pjk[k] * f);
+        pnt_dub_v(P, pjk, r, vd);
         return;
     }
+    panel_v_near_nv<0>(P, p, vs, vd);
+}
+
+// the edge sums of dub_v_inf_f64 (a point the far test has excluded)
+template <int NV>
+CP_HD void dub_v_near_nv(const PanelRec& P, const double* p, double* vd) {
+#pragma clang fp contract(off)
     double D[3] = {0, 0, 0};
-    const int nv = P.nv;
+    const int nv = NV > 0 ? NV : P.nv;
     for (int k = 0; k < nv; k++) {
         int k2 = (k + 1 == nv) ? 0 : k + 1;
         double a[3] = {p[0] - P.v[3 * k], p[1] - P.v[3 * k + 1], p[2] - P.v[3 * k + 2]};
@@ -272,6 +277,18 @@ CP_HD void dub_v_inf_f64(const PanelRec& P, const double* p, double* vd) {
         D[2] += w[2];
     }
     for (int q = 0; q < 3; q++) vd[q] = D[q] / (4 * kPI);
+}
+
+// panel::dubVInf (panel.cpp:666-705) in FP64 (wake panels)
+CP_HD void dub_v_inf_f64(const PanelRec& P, const double* p, double* vd) {
+#pragma clang fp contract(off)
+    double pjk[3] = {p[0] - P.c[0], p[1] - P.c[1], p[2] - P.c[2]};
+    double r2 = sq_norm3(pjk[0], pjk[1], pjk[2]);
+    if (is_far(P, r2)) {
+        pnt_dub_v(P, pjk, sqrt(r2), vd);
+        return;
+    }
+    dub_v_near_nv<0>(P, p, vd);
 }
 
 }  // namespace cpgpu

@@ -55,10 +55,13 @@ CP_HD void vg_point_source_add(const R& P, double sigma, const double* poi, Mat3
 }
 
 // bodyPanel::velocityGradientTriSource / QuadSource (bodyPanel.cpp:766-1063):
-// per edge i -> i+1 in local coordinates, added to G
+// per edge i -> i+1 in local coordinates, added to G.  NV: the vertex count
+// when known at compile time (3: a body triangle), 0 to read it from the
+// record (here and below)
+template <int NV = 0>
 CP_HD void vg_panel_source_add(const PanelRec& P, double sigma, const double* poi, Mat3& G) {
 #pragma clang fp contract(off)
-    const int nv = P.nv;
+    const int nv = NV > 0 ? NV : P.nv;
     double X[4], Y[4];
     for (int k = 0; k < nv; k++) {
         double q[3];
@@ -170,11 +173,12 @@ CP_HD bool near_filament(const double* p1, const double* p2, const double* poi) 
 
 // panel::onPanelCheck (panel.cpp:140-153): projection strictly inside the
 // panel and local z below 5 % of the longest side
+template <int NV = 0>
 CP_HD bool on_panel(const PanelRec& P, const double* poi) {
 #pragma clang fp contract(off)
     double q[3];
     to_local(P, poi, q);
-    const int nv = P.nv;
+    const int nv = NV > 0 ? NV : P.nv;
     for (int k = 0; k < nv; k++) {
         const int k2 = (k + 1 == nv) ? 0 : k + 1;
         double a[3], b[3];
@@ -187,10 +191,11 @@ CP_HD bool on_panel(const PanelRec& P, const double* poi) {
 }
 
 // panel::velocityGradientDoublet (panel.cpp:842-874), added to G
+template <int NV = 0>
 CP_HD void vg_panel_doublet_add(const PanelRec& P, double mu, const double* poi, Mat3& G) {
 #pragma clang fp contract(off)
-    const bool onp = on_panel(P, poi);
-    const int nv = P.nv;
+    const bool onp = on_panel<NV>(P, poi);
+    const int nv = NV > 0 ? NV : P.nv;
     const double scale = mu / (4 * kPI);
     for (int i = 0; i < nv; i++) {
         const int j = (i + 1 == nv) ? 0 : i + 1;
@@ -202,6 +207,14 @@ CP_HD void vg_panel_doublet_add(const PanelRec& P, double mu, const double* poi,
         double s[3] = {p2[0] - p1[0], p2[1] - p1[1], p2[2] - p1[2]};
         grad_doub_add(a, b, s, scale, onp, G);
     }
+}
+
+// part_stretch_add for a point the far test has excluded (the near pass of
+// the panel sums queues only such pairs)
+template <int NV>
+CP_HD void part_stretch_near_add(const PanelRec& P, bool body, double sigma, double mu, const double* pos, Mat3& G) {
+    if (body) vg_panel_source_add<NV>(P, sigma, pos, G);
+    vg_panel_doublet_add<NV>(P, mu, pos, G);
 }
 
 // partStretching gradient of one panel at pos, added to G

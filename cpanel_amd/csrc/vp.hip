@@ -867,25 +867,26 @@ __global__ __launch_bounds__(64) void panels_near_kernel(const PanelRec* __restr
             const double q[3] = {sp[L][0], sp[L][1], sp[L][2]};
             const bool isb = j < nb;
             const PanelRec& P = isb ? body[j] : wake[j - nb];
+            // a queued pair is near (the far test excluded it): the edge
+            // sums alone (for the velocity, body panels as triangles)
             if (MODE == PS_VEL) {
                 if (isb) {
                     double vs[3], vd[3];
-                    panel_v_inf_f64(P, q, vs, vd);
+                    panel_v_near_nv<3>(P, q, vs, vd);
                     const double sg = sigma[j], mj = mu[j];
                     for (int k = 0; k < 3; k++) r[k] = sg * vs[k] + mj * vd[k];
                 } else {
                     double vd[3];
-                    dub_v_inf_f64(P, q, vd);
+                    dub_v_near_nv<0>(P, q, vd);
                     const double mw = muw[j - nb];
                     for (int k = 0; k < 3; k++) r[k] = mw * vd[k];
                 }
             } else {
+                // one call site for body and wake panels (the vertex count
+                // from the record): one inlined copy of the gradient sums
                 Mat3 Gq;
                 mat3_zero(Gq);
-                if (isb)
-                    part_stretch_add(P, true, sigma[j], mu[j], q, Gq);
-                else
-                    part_stretch_add(P, false, 0.0, muw[j - nb], q, Gq);
+                part_stretch_near_add<0>(P, isb, isb ? sigma[j] : 0.0, isb ? mu[j] : muw[j - nb], q, Gq);
                 for (int a = 0; a < 3; a++)
                     for (int b = 0; b < 3; b++) r[3 * a + b] = Gq.a[a][b];
             }
