@@ -76,10 +76,11 @@ CP_HD void vg_panel_source_add(const PanelRec& P, double sigma, const double* po
     for (int k = 0; k < nv; k++) {
         double q = (x - X[k]) * (x - X[k]) + (y - Y[k]) * (y - Y[k]) + z * z;
         double sq = sqrt(q);
+        const double isq = 1 / sq;  // one division for the three derivatives
         r[k] = sq;
-        drx[k] = (x - X[k]) / sq;
-        dry[k] = (y - Y[k]) / sq;
-        drz[k] = z / sq;
+        drx[k] = (x - X[k]) * isq;
+        dry[k] = (y - Y[k]) * isq;
+        drz[k] = z * isq;
         e[k] = (x - X[k]) * (x - X[k]) + z * z;
         h[k] = (x - X[k]) * (y - Y[k]);
     }
@@ -89,9 +90,13 @@ CP_HD void vg_panel_source_add(const PanelRec& P, double sigma, const double* po
         double d = sqrt((X[j] - X[i]) * (X[j] - X[i]) + (Y[j] - Y[i]) * (Y[j] - Y[i]));
         double S = r[i] + r[j];
         double Gk[3] = {drx[i] + drx[j], dry[i] + dry[j], drz[i] + drz[j]};
+        // (S + d) 2d / d / (S - d) / (S + d)^2 of the reference, as
+        // (2d / d) / ((S - d)(S + d)): two divisions for the six entries, the
+        // same value to rounding and the same inf / NaN where d or S - d is 0
+        const double cc = ((2 * d) / d) / ((S - d) * (S + d));
         for (int k = 0; k < 3; k++) {
-            M[k][0] += (Y[j] - Y[i]) * (S + d) * Gk[k] * (2 * d) / d / (S - d) / ((S + d) * (S + d));
-            M[k][1] += (X[i] - X[j]) * (S + d) * Gk[k] * (2 * d) / d / (S - d) / ((S + d) * (S + d));
+            M[k][0] += (Y[j] - Y[i]) * Gk[k] * cc;
+            M[k][1] += (X[i] - X[j]) * Gk[k] * cc;
         }
         const double mm = (Y[j] - Y[i]) / (X[j] - X[i]);
         for (int t = 0; t < 2; t++) {
@@ -99,10 +104,12 @@ CP_HD void vg_panel_source_add(const PanelRec& P, double sigma, const double* po
             const double sg = t == 0 ? 1.0 : -1.0;
             const double me = mm * e[q] - h[q];
             const double zr = z * r[q];
-            const double f = 1 / (1 + (me / zr) * (me / zr));
-            M[0][2] += sg * f * ((zr * (mm * 2 * (x - X[q]) - (y - Y[q])) - me * (z * drx[q])) / (zr * zr));
-            M[1][2] += sg * f * ((-z * r[q] * (x - X[q]) - me * (z * dry[q])) / (zr * zr));
-            M[2][2] += sg * f * ((zr * mm * (2 * z) - me * (r[q] + z * drz[q])) / (zr * zr));
+            const double mz = me / zr;
+            // f / (zr zr) once for the three entries
+            const double fz = (1 / (1 + mz * mz)) / (zr * zr);
+            M[0][2] += sg * ((zr * (mm * 2 * (x - X[q]) - (y - Y[q])) - me * (z * drx[q])) * fz);
+            M[1][2] += sg * ((-z * r[q] * (x - X[q]) - me * (z * dry[q])) * fz);
+            M[2][2] += sg * ((zr * mm * (2 * z) - me * (r[q] + z * drz[q])) * fz);
         }
     }
     const double c = sigma / (4 * kPI);
@@ -142,16 +149,20 @@ CP_HD void grad_doub_add(const double* a, const double* b, const double* sv, dou
     double cs = kCore * sqrt(sq_norm3(sv[0], sv[1], sv[2]));
     double top[3] = {(na + nb) * axb[0], (na + nb) * axb[1], (na + nb) * axb[2]};
     double bot = na * nb * (na * nb + ab) + cs * cs;
+    // one reciprocal each for |a|, |b| and bot^2 in place of the reference's
+    // fifteen divisions (the same values to rounding, the same inf / NaN
+    // where a norm or bot is 0)
+    const double ina = 1 / na, inb = 1 / nb, ibot2 = 1 / (bot * bot);
     for (int k = 0; k < 3; k++) {
         // e_k x b + a x e_k
         double ek[3] = {k == 0 ? 1.0 : 0.0, k == 1 ? 1.0 : 0.0, k == 2 ? 1.0 : 0.0};
         double c1[3] = {ek[1] * b[2] - ek[2] * b[1], ek[2] * b[0] - ek[0] * b[2], ek[0] * b[1] - ek[1] * b[0]};
         double c2[3] = {a[1] * ek[2] - a[2] * ek[1], a[2] * ek[0] - a[0] * ek[2], a[0] * ek[1] - a[1] * ek[0]};
-        double dan = a[k] / na, dbn = b[k] / nb;
+        double dan = a[k] * ina, dbn = b[k] * inb;
         double dbot = dan * nb * (na * nb + ab) + na * dbn * (na * nb + ab) + na * nb * (dan * nb + na * dbn + b[k] + a[k]);
         for (int j = wcol_only ? 2 : 0; j < 3; j++) {
             double dtop = (na + nb) * (c1[j] + c2[j]) + (dan + dbn) * axb[j];
-            G.a[k][j] += scale * ((bot * dtop - dbot * top[j]) / (bot * bot));
+            G.a[k][j] += scale * ((bot * dtop - dbot * top[j]) * ibot2);
         }
     }
 }
