@@ -201,9 +201,6 @@ CP_HD void pnt_dub_v(const R& P, const double* pjk, double r, double* out) {
     for (int k = 0; k < 3; k++) out[k] = (pn3 * pjk[k] - r2 * P.n[k]) * f;
 }
 
-// bodyPanel::panelVInf (bodyPanel.cpp:305-348) in FP64: source and doublet
-// velocity of a unit-strength panel at p.  Far field: point doublet and the
-// point source with the reference's sign (negated, bodyPanel.cpp:314).
 // the edge sums of panel_v_inf_f64 (a point the far test has excluded); NV:
 // the vertex count when known at compile time (3: a body triangle), 0 to read
 // it from the record
@@ -244,6 +241,9 @@ CP_HD void panel_v_near_nv(const PanelRec& P, const double* p, double* vs, doubl
     }
 }
 
+// bodyPanel::panelVInf (bodyPanel.cpp:305-348) in FP64: source and doublet
+// velocity of a unit-strength panel at p.  Far field: point doublet and the
+// point source with the reference's sign (negated, bodyPanel.cpp:314).
 CP_HD void panel_v_inf_f64(const PanelRec& P, const double* p, double* vs, double* vd) {
 #pragma clang fp contract(off)
     double pjk[3] = {p[0] - P.c[0], p[1] - P.c[1], p[2] - P.c[2]};
