@@ -495,30 +495,44 @@ __global__ __launch_bounds__(kT) void bhc_kernel(const BhNode* __restrict__ node
     const int o = own_order ? own_order[t] : t;
     const T s2 = (T)s2d;
     T acc[3] = {0, 0, 0};
-    int i = 1;
+    // The wavefront walks the nodes together (uniform i: one load of each
+    // node, one of each member, for all lanes).  A lane that takes a cell's
+    // pseudo-particle sits out the cell's subtree (nodes below its skip
+    // index); the wavefront descends when any active lane opens the cell and
+    // jumps to the skip index when none does.  Each lane so meets exactly the
+    // nodes of its own walk, in the same order: the same sums, bit for bit.
+    int i = 1, skip_to = 0;
     while (i < n_nodes) {
         const BhNode nd = nodes[i];
+        const bool act = skip_to <= i;
         if (nd.count >= 0) {
-            for (int k = 0; k < nd.count; k++) {
-                const int ms = nd.first + k;
-                if (ms == self) continue;
-                const T sa[3] = {(T)mstr[3 * ms], (T)mstr[3 * ms + 1], (T)mstr[3 * ms + 2]};
-                if (MODE == KC_VEL)
-                    kc_vel<T>(mpos + 3 * ms, sa, tp, s2, acc);
-                else
-                    kc_stretch<T>(mpos + 3 * ms, sa, tp, ta, s2, acc);
-            }
-            i = nd.skip;
-        } else if (far_cell(nd, tp[0], tp[1], tp[2], theta)) {
-            const double sp[3] = {nd.ex, nd.ey, nd.ez};
-            const T sa[3] = {(T)nd.sx, (T)nd.sy, (T)nd.sz};
-            if (MODE == KC_VEL)
-                kc_vel<T>(sp, sa, tp, s2, acc);
-            else
-                kc_stretch<T>(sp, sa, tp, ta, s2, acc);
+            if (act)
+                for (int k = 0; k < nd.count; k++) {
+                    const int ms = nd.first + k;
+                    if (ms == self) continue;
+                    const T sa[3] = {(T)mstr[3 * ms], (T)mstr[3 * ms + 1], (T)mstr[3 * ms + 2]};
+                    if (MODE == KC_VEL)
+                        kc_vel<T>(mpos + 3 * ms, sa, tp, s2, acc);
+                    else
+                        kc_stretch<T>(mpos + 3 * ms, sa, tp, ta, s2, acc);
+                }
             i = nd.skip;
         } else {
-            i = i + 1;
+            bool open = false;
+            if (act) {
+                if (far_cell(nd, tp[0], tp[1], tp[2], theta)) {
+                    const double sp[3] = {nd.ex, nd.ey, nd.ez};
+                    const T sa[3] = {(T)nd.sx, (T)nd.sy, (T)nd.sz};
+                    if (MODE == KC_VEL)
+                        kc_vel<T>(sp, sa, tp, s2, acc);
+                    else
+                        kc_stretch<T>(sp, sa, tp, ta, s2, acc);
+                    skip_to = nd.skip;
+                } else {
+                    open = true;
+                }
+            }
+            i = __ballot(open) ? i + 1 : nd.skip;
         }
     }
     out[3 * o] = acc[0];
