@@ -157,33 +157,44 @@ __global__ __launch_bounds__(kT) void bh_particles_kernel(const BhNode* __restri
     if (st >= n) return;
     const Src tg = member(mpos, mstr, mrad, st);
     double acc[3] = {0, 0, 0};
-    int i = 1;
+    // the wavefront's walk of bhc_kernel: uniform node index, a lane that
+    // takes a cell's pseudo-particle sits out its subtree (same sums per
+    // target, in the same order)
+    int i = 1, skip_to = 0;
     while (i < n_nodes) {
         const BhNode nd = nodes[i];
+        const bool act = skip_to <= i;
         if (nd.count >= 0) {
-            for (int k = 0; k < nd.count; k++) {
-                int ms = nd.first + k;
-                if (ms == st) continue;  // pointer self-exclusion of node.h:272
-                const Src s = member(mpos, mstr, mrad, ms);
-                if (MODE == CP_BH_VEL)
-                    k_gauss(s, tg, acc);
-                else if (MODE == CP_BH_STRETCH)
-                    k_stretch(s, tg, acc);
-                else
-                    k_diff(s, tg, acc);
-            }
-            i = nd.skip;
-        } else if (far_cell(nd, tg.x, tg.y, tg.z, theta)) {
-            const Src s = pseudo(nd);
-            if (MODE == CP_BH_VEL)
-                k_gauss(s, tg, acc);
-            else if (MODE == CP_BH_STRETCH)
-                k_stretch(s, tg, acc);
-            else
-                k_diff(s, tg, acc);
+            if (act)
+                for (int k = 0; k < nd.count; k++) {
+                    int ms = nd.first + k;
+                    if (ms == st) continue;  // pointer self-exclusion of node.h:272
+                    const Src s = member(mpos, mstr, mrad, ms);
+                    if (MODE == CP_BH_VEL)
+                        k_gauss(s, tg, acc);
+                    else if (MODE == CP_BH_STRETCH)
+                        k_stretch(s, tg, acc);
+                    else
+                        k_diff(s, tg, acc);
+                }
             i = nd.skip;
         } else {
-            i = i + 1;
+            bool open = false;
+            if (act) {
+                if (far_cell(nd, tg.x, tg.y, tg.z, theta)) {
+                    const Src s = pseudo(nd);
+                    if (MODE == CP_BH_VEL)
+                        k_gauss(s, tg, acc);
+                    else if (MODE == CP_BH_STRETCH)
+                        k_stretch(s, tg, acc);
+                    else
+                        k_diff(s, tg, acc);
+                    skip_to = nd.skip;
+                } else {
+                    open = true;
+                }
+            }
+            i = __ballot(open) ? i + 1 : nd.skip;
         }
     }
     const int o = order[st];
@@ -211,33 +222,42 @@ __global__ __launch_bounds__(kT) void bh_targets_kernel(const BhNode* __restrict
     const Src tg{tpos[3 * t], tpos[3 * t + 1], tpos[3 * t + 2], tstr[3 * t], tstr[3 * t + 1], tstr[3 * t + 2], trad[t]};
     const int self = tslot[t];
     double acc[3] = {0, 0, 0};
-    int i = 1;
+    // the wavefront's walk of bhc_kernel (same sums per target)
+    int i = 1, skip_to = 0;
     while (i < n_nodes) {
         const BhNode nd = nodes[i];
+        const bool act = skip_to <= i;
         if (nd.count >= 0) {
-            for (int k = 0; k < nd.count; k++) {
-                const int ms = nd.first + k;
-                if (ms == self) continue;
-                const Src sp = member(mpos, mstr, mrad, ms);
-                if (MODE == CP_BH_VEL)
-                    k_gauss(sp, tg, acc);
-                else if (MODE == CP_BH_STRETCH)
-                    k_stretch(sp, tg, acc);
-                else
-                    k_diff(sp, tg, acc);
-            }
-            i = nd.skip;
-        } else if (far_cell(nd, tg.x, tg.y, tg.z, theta)) {
-            const Src sp = pseudo(nd);
-            if (MODE == CP_BH_VEL)
-                k_gauss(sp, tg, acc);
-            else if (MODE == CP_BH_STRETCH)
-                k_stretch(sp, tg, acc);
-            else
-                k_diff(sp, tg, acc);
+            if (act)
+                for (int k = 0; k < nd.count; k++) {
+                    const int ms = nd.first + k;
+                    if (ms == self) continue;
+                    const Src sp = member(mpos, mstr, mrad, ms);
+                    if (MODE == CP_BH_VEL)
+                        k_gauss(sp, tg, acc);
+                    else if (MODE == CP_BH_STRETCH)
+                        k_stretch(sp, tg, acc);
+                    else
+                        k_diff(sp, tg, acc);
+                }
             i = nd.skip;
         } else {
-            i = i + 1;
+            bool open = false;
+            if (act) {
+                if (far_cell(nd, tg.x, tg.y, tg.z, theta)) {
+                    const Src sp = pseudo(nd);
+                    if (MODE == CP_BH_VEL)
+                        k_gauss(sp, tg, acc);
+                    else if (MODE == CP_BH_STRETCH)
+                        k_stretch(sp, tg, acc);
+                    else
+                        k_diff(sp, tg, acc);
+                    skip_to = nd.skip;
+                } else {
+                    open = true;
+                }
+            }
+            i = __ballot(open) ? i + 1 : nd.skip;
         }
     }
     out[3 * t] = acc[0];
