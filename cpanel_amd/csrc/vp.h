@@ -23,6 +23,9 @@ struct DevBhTree {
 // mode CP_BH_VEL_POINTS: m points pts[3m] -> out[3m]
 void launch_bh_points(const DevBhTree& T, double theta, int m, const double* pts, double* out, hipStream_t s);
 // modes CP_BH_VEL / CP_BH_STRETCH / CP_BH_DIFF on the tree's particles -> out[3n]
+// caller-given targets up to which the tree walks run a wavefront per
+// target (above: a lane per target)
+constexpr int kBhWaveTargets = 16384;
 void launch_bh_particles(const DevBhTree& T, int mode, double theta, double* out, hipStream_t s);
 
 // modes CP_BH_VEL / CP_BH_STRETCH / CP_BH_DIFF on m caller-given targets

@@ -266,7 +266,8 @@ __global__ __launch_bounds__(kT) void bh_targets_kernel(const BhNode* __restrict
 }
 
 // caller-given targets up to which the tree walks run a wavefront per target
-constexpr int kBhWaveMax = 16384;
+// (kBhWaveTargets in vp.h)
+constexpr int kBhWaveMax = kBhWaveTargets;
 constexpr int kBhcWaveMax = kBhWaveMax;
 
 // The walk with one wavefront per target, for target sets too small to fill

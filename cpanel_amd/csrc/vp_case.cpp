@@ -263,21 +263,43 @@ std::vector<double> bh_on_particles(cp_vp* v, int mode) {
     const double* dp = v->dpos.as<double>();
     const double* ds = v->dstr.as<double>();
     const double* dr = v->drad.as<double>();
-    // each particle's slot in the tree; particles shed since the build have
-    // none (-1)
-    dalloc(c, dsl, (size_t)np * sizeof(int));
-    const int nb = std::min(np, v->tree->dev.n);
-    if (nb > 0) CP_HIP(hipMemcpyAsync(dsl.p, v->tree->slot.p, (size_t)nb * sizeof(int), hipMemcpyDeviceToDevice, s));
-    if (np > nb) CP_HIP(hipMemsetAsync(dsl.as<int>() + nb, 0xFF, (size_t)(np - nb) * sizeof(int), s));
+    // Above kBhWaveTargets particles (a lane per target), the tree's members
+    // (the particles of its build, refreshed to their current state) walk as
+    // the tree's own targets, in its spatial order (coherent wavefronts),
+    // their sums written back through its order array, and only the particles
+    // shed since the build (no slot: -1) as caller-given targets.  Each
+    // target's walk and self-exclusion are those of a caller-given target
+    // with its slot: the same sums.  Below it every particle is a
+    // caller-given target (a wavefront each).
+    const DevBhTree& T = v->tree->dev;
+    const int nb = np > kBhWaveTargets ? std::min(np, T.n) : 0;
     dalloc(c, dout, (size_t)3 * np * sizeof(double));
+    if (np > nb) {  // the caller-given targets' slots
+        const int nm = std::min(np, T.n);
+        dalloc(c, dsl, (size_t)(np - nb) * sizeof(int));
+        if (nm > nb)
+            CP_HIP(hipMemcpyAsync(dsl.p, v->tree->slot.as<int>() + nb, (size_t)(nm - nb) * sizeof(int),
+                                  hipMemcpyDeviceToDevice, s));
+        if (np > nm) CP_HIP(hipMemsetAsync(dsl.as<int>() + (nm - nb), 0xFF, (size_t)(np - nm) * sizeof(int), s));
+    }
     {
         TimedScope ts("bh_eval", s);
-        if (v->alg)
-            launch_core_bh(v->tree->dev, mode, v->tree->theta, v->core, v->f32, np, dp, ds, dsl.as<int>(), nullptr,
-                           dout.as<double>(), s);
-        else
-            launch_bh_targets(v->tree->dev, mode, v->tree->theta, np, dp, ds, dr, dsl.as<int>(), dout.as<double>(),
-                              s);
+        if (nb > 0) {
+            if (v->alg)
+                launch_core_bh(T, mode, v->tree->theta, v->core, v->f32, nb, T.mpos, T.mstr, nullptr, T.order,
+                               dout.as<double>(), s);
+            else
+                launch_bh_particles(T, mode, v->tree->theta, dout.as<double>(), s);
+        }
+        if (np > nb) {
+            double* o = dout.as<double>() + 3 * (size_t)nb;
+            if (v->alg)
+                launch_core_bh(T, mode, v->tree->theta, v->core, v->f32, np - nb, dp + 3 * (size_t)nb,
+                               ds + 3 * (size_t)nb, dsl.as<int>(), nullptr, o, s);
+            else
+                launch_bh_targets(T, mode, v->tree->theta, np - nb, dp + 3 * (size_t)nb, ds + 3 * (size_t)nb,
+                                  dr + nb, dsl.as<int>(), o, s);
+        }
     }
     from_dev(out, dout, (size_t)3 * np, s);
     return out;
