@@ -929,7 +929,15 @@ __global__ __launch_bounds__(1024) void colsum_kernel(const double* __restrict__
     for (int q = 0; q < nq; q++) {
         double s = 0;
         const double* d = dpart + (int64_t)q * nblk * KP;
-        for (int b = g; b < nblk; b += 16) s += d[(int64_t)b * KP + c];
+        // blocks g, g + 16, ... added in that order; eight loads in flight
+        // (a masked slot adds +0.0, which leaves a sum started at +0 as it is)
+        for (int b = g; b < nblk; b += 128) {
+            double v[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) v[k] = b + 16 * k < nblk ? d[(int64_t)(b + 16 * k) * KP + c] : 0.0;
+#pragma unroll
+            for (int k = 0; k < 8; k++) s += v[k];
+        }
         sh[threadIdx.x] = s;
         __syncthreads();
         if (g == 0) {
@@ -1157,7 +1165,15 @@ __device__ inline double colsum1024(const double* __restrict__ d, int nblk, doub
     for (; b + 240 < nblk; b += 256)
 #pragma unroll
         for (int k = 0; k < 16; k++) r[k] += d[(int64_t)(b + 16 * k) * KP + c];
-    for (; b < nblk; b += 16) r[0] += d[(int64_t)b * KP + c];
+    // the rest into r[0] in block order, eight loads in flight (a masked slot
+    // adds +0.0)
+    for (; b < nblk; b += 128) {
+        double v[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) v[k] = b + 16 * k < nblk ? d[(int64_t)(b + 16 * k) * KP + c] : 0.0;
+#pragma unroll
+        for (int k = 0; k < 8; k++) r[0] += v[k];
+    }
 #pragma unroll
     for (int w = 8; w >= 1; w >>= 1)
 #pragma unroll
@@ -1322,14 +1338,30 @@ __device__ __forceinline__ void gm_givens_col(double t, int* __restrict__ act, i
     double* S = st + threadIdx.x * kGmStride;
     double* h = S + GM_H + j * (kGmMax + 1);
     const double hn = sqrt(t);
-    for (int i = 0; i < j; i++) {
-        const double cs = S[GM_CS + i], sn = S[GM_SN + i];
-        const double a = cs * h[i] + sn * h[i + 1];
-        h[i + 1] = -sn * h[i] + cs * h[i + 1];
-        h[i] = a;
+    // rotation i maps (h_i, h_{i+1}) to (cs h_i + sn h_{i+1}, -sn h_i + cs h_{i+1});
+    // hc carries the rotated h_i, and the rotations' operands are loaded eight
+    // at a time (h_{i+1} is still the unrotated value when rotation i reads it)
+    double hc = h[0];
+    for (int i0 = 0; i0 < j; i0 += 8) {
+        double c8[8], s8[8], n8[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const bool ok = i0 + k < j;
+            c8[k] = ok ? S[GM_CS + i0 + k] : 0.0;
+            s8[k] = ok ? S[GM_SN + i0 + k] : 0.0;
+            n8[k] = ok ? h[i0 + k + 1] : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < 8; k++)
+            if (i0 + k < j) {
+                const double cs = c8[k], sn = s8[k];
+                const double a = cs * hc + sn * n8[k];
+                hc = -sn * hc + cs * n8[k];
+                h[i0 + k] = a;
+            }
     }
-    const double den = sqrt(h[j] * h[j] + hn * hn);
-    const double cs = den > 0 ? h[j] / den : 1.0, sn = den > 0 ? hn / den : 0.0;
+    const double den = sqrt(hc * hc + hn * hn);
+    const double cs = den > 0 ? hc / den : 1.0, sn = den > 0 ? hn / den : 0.0;
     S[GM_CS + j] = cs;
     S[GM_SN + j] = sn;
     h[j] = den;
@@ -1363,7 +1395,10 @@ __global__ __launch_bounds__(1024) void gm_givens_kernel(const double* __restric
 
 // y = H^-1 g over the iterations each column took (upper triangular); the
 // count of those iterations goes to sc[Q_GMNJ] for the host
-__global__ void gm_backsub_kernel(double* __restrict__ st, double* __restrict__ sc) {
+// (y kept in LDS, one column per thread; the H entries of a row are loaded
+// eight at a time)
+__global__ __launch_bounds__(KP) void gm_backsub_kernel(double* __restrict__ st, double* __restrict__ sc) {
+    __shared__ double ys[kGmMax][KP];
     const int c = threadIdx.x;
     if (c >= KP) return;
     double* S = st + c * kGmStride;
@@ -1371,12 +1406,19 @@ __global__ void gm_backsub_kernel(double* __restrict__ st, double* __restrict__ 
     sc[c * Q_NQ + Q_GMNJ] = nj;
     for (int i = nj - 1; i >= 0; i--) {
         double v = S[GM_G + i];
-        for (int k = i + 1; k < nj; k++) v -= S[GM_H + k * (kGmMax + 1) + i] * S[GM_Y + k];
+        for (int k0 = i + 1; k0 < nj; k0 += 8) {
+            double h8[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) h8[k] = k0 + k < nj ? S[GM_H + (k0 + k) * (kGmMax + 1) + i] : 0.0;
+#pragma unroll
+            for (int k = 0; k < 8; k++)
+                if (k0 + k < nj) v -= h8[k] * ys[k0 + k][c];
+        }
         const double d = S[GM_H + i * (kGmMax + 1) + i];
-        S[GM_Y + i] = d != 0 ? v / d : 0.0;
+        ys[i][c] = d != 0 ? v / d : 0.0;
     }
     // the combination reads y_q / s_q
-    for (int i = 0; i < nj; i++) S[GM_Y + i] /= S[GM_S + i];
+    for (int i = 0; i < nj; i++) S[GM_Y + i] = ys[i][c] / S[GM_S + i];
 }
 
 // T = sum_q y_q v_q in block order (zero on columns without iterations;
@@ -1389,7 +1431,19 @@ __global__ __launch_bounds__(256) void gm_combine_kernel(int n, const float* __r
     const double* S = st + c * kGmStride;
     const int nj = (int)S[GM_NJ];
     float v = 0.f;
-    for (int q = 0; q < nj; q++) v += (float)S[GM_Y + q] * U[(int64_t)q * us + idx];
+    // q = 0, 1, ... in order, eight loads in flight
+    for (int q0 = 0; q0 < nj; q0 += 8) {
+        float y8[8], u8[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const bool ok = q0 + k < nj;
+            y8[k] = ok ? (float)S[GM_Y + q0 + k] : 0.f;
+            u8[k] = ok ? U[(int64_t)(q0 + k) * us + idx] : 0.f;
+        }
+#pragma unroll
+        for (int k = 0; k < 8; k++)
+            if (q0 + k < nj) v += y8[k] * u8[k];
+    }
     T[idx] = v;
 }
 
