@@ -1222,8 +1222,8 @@ __global__ __launch_bounds__(256) void gm_w_kernel(const float* __restrict__ par
 }
 
 // dots u_t . u_{j+1}: one wavefront per (chunk of kGmDotRows block-order
-// positions, t <= j), lane = column; eight position pairs in flight, each
-// wavefront writes its own partial (no block reduction)
+// positions, t <= j), lane = column; 32 position pairs in flight, added in
+// position order; each wavefront writes its own partial (no block reduction)
 constexpr int kGmDotRows = 256;
 __device__ __forceinline__ void gm_dots_body(int bx, int by, int nchunk, int n, const int* __restrict__ act,
                                     const float* __restrict__ U, int64_t us, int j, double* __restrict__ dpart) {
@@ -1237,6 +1237,16 @@ __device__ __forceinline__ void gm_dots_body(int bx, int by, int nchunk, int n, 
         const float* w = U + (int64_t)(j + 1) * us + c;
         const int64_t p1 = std::min((int64_t)n, (chunk + 1) * kGmDotRows);
         int64_t p = chunk * kGmDotRows;
+        for (; p + 32 <= p1; p += 32) {
+            float a[32], b[32];
+#pragma unroll
+            for (int k = 0; k < 32; k++) {
+                a[k] = ut[(p + k) * KP];
+                b[k] = w[(p + k) * KP];
+            }
+#pragma unroll
+            for (int k = 0; k < 32; k++) acc += (double)a[k] * b[k];
+        }
         for (; p + 8 <= p1; p += 8) {
             float a[8], b[8];
 #pragma unroll
@@ -1276,8 +1286,8 @@ __global__ __launch_bounds__(1024) void gm_hcol_kernel(const double* __restrict_
 
 // u_{j+1} = w - sum_{q <= j} h_{q,j} v_q on the active columns (coef[t][c] =
 // h_{t,j} / s_t), and the partials of ||u_{j+1}||^2: kGmUpRounds groups of
-// kGmUpPos block-order positions per thread, four basis vectors at a time (32
-// loads in flight), 4 * kGmUpRounds * kGmUpPos positions and one partial per
+// kGmUpPos block-order positions per thread, eight basis vectors at a time (64
+// loads in flight), subtracted in t order, 4 * kGmUpRounds * kGmUpPos positions and one partial per
 // workgroup
 constexpr int kGmUpPos = 8;
 constexpr int kGmUpRounds = 2;
@@ -1292,6 +1302,20 @@ __device__ __forceinline__ void gm_update_body(int bx, double* sh, int n, const 
 #pragma unroll
         for (int k = 0; k < kGmUpPos; k++) v[k] = p0 + k < n ? w[(p0 + k) * KP] : 0.f;
         int t = 0;
+        for (; t + 8 <= j + 1; t += 8) {
+            float f[8], x[8][kGmUpPos];
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                f[q] = coef[(t + q) * KP + c];
+                const float* ut = U + (int64_t)(t + q) * us + c;
+#pragma unroll
+                for (int k = 0; k < kGmUpPos; k++) x[q][k] = p0 + k < n ? ut[(p0 + k) * KP] : 0.f;
+            }
+#pragma unroll
+            for (int q = 0; q < 8; q++)
+#pragma unroll
+                for (int k = 0; k < kGmUpPos; k++) v[k] -= f[q] * x[q][k];
+        }
         for (; t + 4 <= j + 1; t += 4) {
             float f[4], x[4][kGmUpPos];
 #pragma unroll
