@@ -303,7 +303,9 @@ private:
     float* gm_u_ = nullptr;
     double* gm_st_ = nullptr;
     double* gm_dpart_ = nullptr;
-    float* gm_coef_ = nullptr;  // [t][column] h_{t,j} / s_t of the current iteration
+    // [t][column] h_{t,j} / s_t of the current iteration; after a sweep, y_q / s_q
+    // (rows q) and the iteration count (row kGmMax)
+    float* gm_coef_ = nullptr;
     double* sc_;        // 64 x 16 per-column scalars
     int* act_;          // 64 active flags
     const int* near_rowptr_ = nullptr;

@@ -1420,8 +1420,10 @@ __global__ __launch_bounds__(1024) void gm_givens_kernel(const double* __restric
 // y = H^-1 g over the iterations each column took (upper triangular); the
 // count of those iterations goes to sc[Q_GMNJ] for the host
 // (y kept in LDS, one column per thread; the H entries of a row are loaded
-// eight at a time)
-__global__ __launch_bounds__(KP) void gm_backsub_kernel(double* __restrict__ st, double* __restrict__ sc) {
+// eight at a time).  yc[q][c] = (float)(y_q / s_q) and yc[kGmMax][c] = the
+// column's iteration count: the combination's coefficients, one row per q
+__global__ __launch_bounds__(KP) void gm_backsub_kernel(double* __restrict__ st, double* __restrict__ sc,
+                                                        float* __restrict__ yc) {
     __shared__ double ys[kGmMax][KP];
     const int c = threadIdx.x;
     if (c >= KP) return;
@@ -1442,18 +1444,22 @@ __global__ __launch_bounds__(KP) void gm_backsub_kernel(double* __restrict__ st,
         ys[i][c] = d != 0 ? v / d : 0.0;
     }
     // the combination reads y_q / s_q
-    for (int i = 0; i < nj; i++) S[GM_Y + i] = ys[i][c] / S[GM_S + i];
+    for (int i = 0; i < nj; i++) {
+        const double y = ys[i][c] / S[GM_S + i];
+        S[GM_Y + i] = y;
+        yc[i * KP + c] = (float)y;
+    }
+    yc[kGmMax * KP + c] = (float)nj;
 }
 
 // T = sum_q y_q v_q in block order (zero on columns without iterations;
-// GM_Y holds y_q / s_q after gm_backsub_kernel)
+// yc as gm_backsub_kernel leaves it)
 __global__ __launch_bounds__(256) void gm_combine_kernel(int n, const float* __restrict__ U, int64_t us,
-                                                         const double* __restrict__ st, float* __restrict__ T) {
+                                                         const float* __restrict__ yc, float* __restrict__ T) {
     const int64_t idx = (int64_t)blockIdx.x * kT + threadIdx.x;
     if (idx >= (int64_t)n * KP) return;
     const int c = (int)(idx & 63);
-    const double* S = st + c * kGmStride;
-    const int nj = (int)S[GM_NJ];
+    const int nj = (int)yc[kGmMax * KP + c];
     float v = 0.f;
     // q = 0, 1, ... in order, eight loads in flight
     for (int q0 = 0; q0 < nj; q0 += 8) {
@@ -1461,7 +1467,7 @@ __global__ __launch_bounds__(256) void gm_combine_kernel(int n, const float* __r
 #pragma unroll
         for (int k = 0; k < 8; k++) {
             const bool ok = q0 + k < nj;
-            y8[k] = ok ? (float)S[GM_Y + q0 + k] : 0.f;
+            y8[k] = ok ? yc[(q0 + k) * KP + c] : 0.f;
             u8[k] = ok ? U[(int64_t)(q0 + k) * us + idx] : 0.f;
         }
 #pragma unroll
@@ -2574,9 +2580,9 @@ void BatchedSolver::gm_begin(hipStream_t s) {
 
 void BatchedSolver::gm_end(hipStream_t s) {
     // d = M^-1 sum_q y_q v_q
-    hipLaunchKernelGGL(gm_backsub_kernel, dim3(1), dim3(KP), 0, s, gm_st_, sc_);
+    hipLaunchKernelGGL(gm_backsub_kernel, dim3(1), dim3(KP), 0, s, gm_st_, sc_, gm_coef_);
     hipLaunchKernelGGL(gm_combine_kernel, dim3(nblocks((int64_t)n_ * KP)), dim3(kT), 0, s, n_, (const float*)gm_u_,
-                       lda_ * KP, (const double*)gm_st_, pp_);
+                       lda_ * KP, (const float*)gm_coef_, pp_);
     CP_LAUNCH_CHECK();
     apply_block_jacobi(nullptr, d_, s);
     gm_counted_ = true;
