@@ -1286,61 +1286,50 @@ __global__ __launch_bounds__(1024) void gm_hcol_kernel(const double* __restrict_
 
 // u_{j+1} = w - sum_{q <= j} h_{q,j} v_q on the active columns (coef[t][c] =
 // h_{t,j} / s_t), and the partials of ||u_{j+1}||^2: kGmUpRounds groups of
-// kGmUpPos block-order positions per thread, eight basis vectors at a time (64
-// loads in flight), subtracted in t order, 4 * kGmUpRounds * kGmUpPos positions and one partial per
+// kGmUpPos block-order positions per thread (all rounds' positions loaded
+// together), four basis vectors at a time (64 loads in flight), subtracted in
+// t order, 4 * kGmUpRounds * kGmUpPos positions and one partial per
 // workgroup
 constexpr int kGmUpPos = 8;
 constexpr int kGmUpRounds = 2;
 __device__ __forceinline__ void gm_update_body(int bx, double* sh, int n, const int* __restrict__ act, float* __restrict__ U,
                                       int64_t us, int j, const float* __restrict__ coef, double* __restrict__ dpart) {
     const int c = threadIdx.x & 63, g = threadIdx.x >> 6;
+    constexpr int NP = kGmUpRounds * kGmUpPos;
     double acc = 0;
-    for (int round = 0; round < kGmUpRounds && act[c]; round++) {
-        const int64_t p0 = (((int64_t)bx * kGmUpRounds + round) * 4 + g) * kGmUpPos;
+    if (act[c]) {
+        // round r's positions p0(r) .. p0(r) + kGmUpPos - 1, all rounds at once
+        int64_t pp[NP];
+#pragma unroll
+        for (int r = 0; r < kGmUpRounds; r++)
+#pragma unroll
+            for (int k = 0; k < kGmUpPos; k++)
+                pp[r * kGmUpPos + k] = (((int64_t)bx * kGmUpRounds + r) * 4 + g) * kGmUpPos + k;
         float* w = U + (int64_t)(j + 1) * us + c;
-        float v[kGmUpPos];
+        float v[NP];
 #pragma unroll
-        for (int k = 0; k < kGmUpPos; k++) v[k] = p0 + k < n ? w[(p0 + k) * KP] : 0.f;
-        int t = 0;
-        for (; t + 8 <= j + 1; t += 8) {
-            float f[8], x[8][kGmUpPos];
-#pragma unroll
-            for (int q = 0; q < 8; q++) {
-                f[q] = coef[(t + q) * KP + c];
-                const float* ut = U + (int64_t)(t + q) * us + c;
-#pragma unroll
-                for (int k = 0; k < kGmUpPos; k++) x[q][k] = p0 + k < n ? ut[(p0 + k) * KP] : 0.f;
-            }
-#pragma unroll
-            for (int q = 0; q < 8; q++)
-#pragma unroll
-                for (int k = 0; k < kGmUpPos; k++) v[k] -= f[q] * x[q][k];
-        }
-        for (; t + 4 <= j + 1; t += 4) {
-            float f[4], x[4][kGmUpPos];
+        for (int k = 0; k < NP; k++) v[k] = pp[k] < n ? w[pp[k] * KP] : 0.f;
+        // four basis vectors per batch (the last batch masked), subtracted in t order
+        for (int t = 0; t <= j; t += 4) {
+            float f[4], x[4][NP];
 #pragma unroll
             for (int q = 0; q < 4; q++) {
-                f[q] = coef[(t + q) * KP + c];
+                const bool tq = t + q <= j;
+                f[q] = tq ? coef[(t + q) * KP + c] : 0.f;
                 const float* ut = U + (int64_t)(t + q) * us + c;
 #pragma unroll
-                for (int k = 0; k < kGmUpPos; k++) x[q][k] = p0 + k < n ? ut[(p0 + k) * KP] : 0.f;
+                for (int k = 0; k < NP; k++) x[q][k] = tq && pp[k] < n ? ut[pp[k] * KP] : 0.f;
             }
 #pragma unroll
             for (int q = 0; q < 4; q++)
+                if (t + q <= j)
 #pragma unroll
-                for (int k = 0; k < kGmUpPos; k++) v[k] -= f[q] * x[q][k];
-        }
-        for (; t <= j; t++) {
-            const float f = coef[t * KP + c];
-            const float* ut = U + (int64_t)t * us + c;
-#pragma unroll
-            for (int k = 0; k < kGmUpPos; k++)
-                if (p0 + k < n) v[k] -= f * ut[(p0 + k) * KP];
+                    for (int k = 0; k < NP; k++) v[k] -= f[q] * x[q][k];
         }
 #pragma unroll
-        for (int k = 0; k < kGmUpPos; k++)
-            if (p0 + k < n) {
-                w[(p0 + k) * KP] = v[k];
+        for (int k = 0; k < NP; k++)
+            if (pp[k] < n) {
+                w[pp[k] * KP] = v[k];
                 acc += (double)v[k] * v[k];
             }
     }
