@@ -1141,12 +1141,25 @@ __global__ __launch_bounds__(256) void gm_init_kernel(int n, const double* __res
     const double scale = sc[c * Q_NQ + Q_SCALE];
     const double inv = scale > 0 ? 1.0 / scale : 0.0;
     double acc = 0;
-    for (int rr = g; rr < kRedRows; rr += 4) {
-        const int64_t i = (int64_t)blockIdx.x * kRedRows + rr;
-        if (i >= n) break;
-        const float u = act[c] ? (float)(R[i * KP + c] * inv) : 0.f;
-        U0[(int64_t)iperm[i] * KP + c] = u;
-        acc += (double)u * u;
+    const bool on = act[c] != 0;
+    // rows g, g + 4, ... in order, eight at a time
+    for (int rr0 = g; rr0 < kRedRows; rr0 += 32) {
+        double r8[8];
+        int p8[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const int64_t i = (int64_t)blockIdx.x * kRedRows + rr0 + 4 * k;
+            const bool ok = rr0 + 4 * k < kRedRows && i < n;
+            r8[k] = ok && on ? R[i * KP + c] : 0.0;
+            p8[k] = ok ? iperm[i] : -1;
+        }
+#pragma unroll
+        for (int k = 0; k < 8; k++)
+            if (p8[k] >= 0) {
+                const float u = on ? (float)(r8[k] * inv) : 0.f;
+                U0[(int64_t)p8[k] * KP + c] = u;
+                acc += (double)u * u;
+            }
     }
     col_partial(acc, sh, dpart + (int64_t)blockIdx.x * KP);
 }
